@@ -1,0 +1,279 @@
+#!/usr/bin/env python3
+"""Warm GB/s and % of HBM peak of the scan/filter -> group-by/aggregate ->
+join path on one MI355X, over a 1B-row 4 x u32 table (plus a u32 foreign
+key) and two dimension tables, all resident in HBM.
+
+A step runs the query suite once (algorithmic bytes = rows x 4 B x u32
+columns the query reads in full):
+  q1 SUM/MIN/MAX(amount)                                    amount              4 B/row
+  q2 COUNT(*) WHERE amount < 200                            amount              4 B/row
+  q3 region, COUNT(*), SUM(amount) GROUP BY region          region, amount      8 B/row
+  q4 amount, COUNT/SUM/MIN/MAX(amount) GROUP BY amount      amount              4 B/row
+  q5 COUNT(*) WHERE (region = r AND amount < a) OR ts > t   region, amount, ts 12 B/row
+     (literals drawn per step from a seeded RNG)
+  q6 JOIN dim A (1M unique keys spread over u32) GROUP BY segment, SUM(amount)
+                                                            fk, amount          8 B/row
+  q7 JOIN dim B (50 rows) WHERE attr < 3, COUNT(*)          region              4 B/row
+  q8 point lookup id = k (zone maps leave one 8192-row zone): latency only
+value = algorithmic bytes of q1..q7 / wall time of the timed steps (GB/s),
+inputs resident in HBM before the timed region starts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_GBS = 8000.0
+METRIC = "warm GB/s & % HBM peak at 1B-row u32 agg/filter/join; served QPS at p99 latency"
+SEED = 0x1234567890ABCDEF  # bench_common.h:59
+DIM_A_ROWS = 1_000_000
+QUERY_BYTES_PER_ROW = {"q1": 4, "q2": 4, "q3": 8, "q4": 4, "q5": 12, "q6": 8, "q7": 4}
+# the kernel that reads the fact table in each query
+QUERY_KERNEL = {"q1": "aggregate_global", "q2": "scan_simple", "q3": "aggregate_dense", "q4": "aggregate_dense",
+                "q5": "qk_scan*", "q6": "join_probe", "q7": "aggregate_dense", "q8": "scan_simple"}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def dim_b_attr():
+    import numpy as np
+
+    return (((np.arange(50, dtype=np.uint64) * 0x9E3779B1) >> 7) % 10).astype(np.uint32)
+
+
+def build_tables(cq, rows: int, seed: int):
+    import numpy as np
+
+    t = cq.Table(rows)
+    for name in ("id", "region", "amount", "ts", "fk"):
+        t.add_column(name, cq.U32)
+    t.gen_events(seed, id=0, region=1, amount=2, ts=3)
+    dima = cq.Table(DIM_A_ROWS)
+    dima.add_column("key", cq.U32)
+    dima.add_column("segment", cq.U32)
+    dima.gen_dim(seed ^ 0xA, 100, spread_keys=True, key=0, attr=1)
+    t.gen_fact_fk(4, seed, dima, 0)
+    dimb = cq.Table(50)
+    dimb.add_column("key", cq.U32, np.arange(50, dtype=np.uint32))
+    dimb.add_column("attr", cq.U32, dim_b_attr())
+    return t, dima, dimb
+
+
+def q5_literals(step: int, seed: int):
+    import numpy as np
+
+    r = np.random.default_rng([seed & 0xFFFFFFFF, step])
+    return int(r.integers(0, 50)), int(r.integers(100, 10000)), int(r.integers(2**31, 2**32 - 2**28))
+
+
+def point_key(step: int, rows: int) -> int:
+    return (step * 2654435761) % rows
+
+
+def run_suite(cq, lib, t, dima, dimb, step: int, seed: int, rows: int):
+    from computeql_amd.ir import col
+
+    res = {}
+
+    def tag(q):
+        lib.gpu_hip_profile_tag(q.encode())
+
+    tag("q1"); res["q1"] = t.aggregate([cq.SUM, cq.MIN, cq.MAX], [2, 2, 2])
+    tag("q2"); res["q2"] = t.scan(col(2) < 200, count_only=True)
+    tag("q3"); res["q3"] = t.aggregate([cq.COUNT, cq.SUM], [None, 2], group=1)
+    tag("q4"); res["q4"] = t.aggregate([cq.COUNT, cq.SUM, cq.MIN, cq.MAX], [None, 2, 2, 2], group=2)
+    r, a, ts = q5_literals(step, seed)
+    tag("q5"); res["q5"] = t.scan(((col(1) == r) & (col(2) < a)) | (col(3) > ts), count_only=True)
+    tag("q6"); res["q6"] = cq.aggregate_join(t, 4, dima, 0, [cq.SUM], [2], dim_group=1)
+    tag("q7"); res["q7"] = cq.aggregate_join(t, 1, dimb, 0, [cq.COUNT], [None], dim_where=col(1) < 3)
+    tag("q8"); res["q8"] = t.scan(col(0) == point_key(step, rows))
+    tag("")
+    return res
+
+
+def trace_of(r):
+    return r.trace if hasattr(r, "trace") else r[2]
+
+
+def check_invariants(res, rows: int, step: int) -> list:
+    """Size-independent checks at full size: group counts sum to N, group
+    sums to the global sum, join A covers every fact row once, q2 equals the
+    first 200 groups of q4, the point lookup finds its one row."""
+    errs = []
+    total = res["q1"].column(0)[0]
+    if int(res["q3"].column(0).sum()) != rows or float(res["q3"].column(1).sum()) != total:
+        errs.append("q3 totals")
+    if int(res["q4"].column(0).sum()) != rows or float(res["q4"].column(1).sum()) != total:
+        errs.append("q4 totals")
+    if res["q4"].groups != 10000 or res["q4"].column(2)[123] != 123.0 or res["q1"].column(2)[0] != 9999.0:
+        errs.append("q4 groups")
+    if float(res["q6"].column(0).sum()) != total or res["q6"].groups != 100:
+        errs.append("q6 sum")
+    if res["q2"][0] != int(res["q4"].column(0)[:200].sum()):
+        errs.append("q2 vs q4")
+    if res["q7"].groups != 1 or int(res["q7"].cells[0, 0]) != int(res["q3"].column(0)[dim_b_attr() < 3].sum()):
+        errs.append("q7 vs q3")
+    n8, ids8, _ = res["q8"]
+    if n8 != 1 or int(ids8[0]) != point_key(step, rows):
+        errs.append("q8")
+    return errs
+
+
+def timed_cpu_baseline(sample_rows: int, seed: int, workers: int):
+    """The oracle (C restatement of the reference CPU operator path) over a
+    bounded sample of the same tables on all host cores.  Returns the GB/s
+    of algorithmic bytes and per-query seconds."""
+    import numpy as np
+
+    from computeql_amd.ir import col
+    from oracle import oracle as orc
+
+    id_, region, amount, ts = orc.gen_events(sample_rows, seed=seed, nthreads=workers)
+    dk, seg = orc.gen_dim(DIM_A_ROWS, 100, seed=seed ^ 0xA)
+    fk = orc.gen_fact_fk(sample_rows, dk, seed=seed, nthreads=workers)
+    bk = np.arange(50, dtype=np.uint32)
+    battr = dim_b_attr()
+    cols = [id_, region, amount, ts]
+    r, a, tsl = q5_literals(0, seed)
+    secs = {}
+
+    def timeit(name, fn):
+        t0 = time.perf_counter()
+        fn()
+        secs[name] = time.perf_counter() - t0
+
+    timeit("q1", lambda: orc.aggregate_global([amount], workers=workers))
+    timeit("q2", lambda: orc.scan_chunked_count(cols, [(2, 1, 200.0)], workers=workers))
+    timeit("q3", lambda: orc.dense_group_u32(region, amount, 0, 50, workers=workers))
+    timeit("q4", lambda: orc.dense_group_u32(amount, amount, 0, 10000, workers=workers))
+    timeit("q5", lambda: orc.scan(cols, ((col(1) == r) & (col(2) < a)) | (col(3) > tsl), nthreads=workers,
+                                  count_only=True))
+    timeit("q6", lambda: orc.join_aggregate(fk, [amount], [1], dk, dim_group=seg, nthreads=workers))
+    timeit("q7", lambda: orc.join_aggregate(region, [None], [0], bk, dim_pass=(battr < 3), nthreads=workers))
+    total_bytes = sum(QUERY_BYTES_PER_ROW.values()) * sample_rows
+    return total_bytes / sum(secs.values()) / 1e9, secs
+
+
+def profile_stats(lib, name: str):
+    d = ctypes.c_uint64(0)
+    ns = lib.gpu_hip_profile_query(name.encode(), ctypes.byref(d))
+    return int(ns), int(d.value)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rows", type=int, default=1_000_000_000)
+    ap.add_argument("--cpu-sample-rows", type=int, default=16_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {a.gpus}); this build has no multi-GPU path, "
+              "run it with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import computeql_amd as cq
+
+    lib = cq.load()
+    seed, rows = SEED, a.rows
+    log(f"building tables: {rows} fact rows x 5 u32 columns in HBM")
+    t0 = time.perf_counter()
+    t, dima, dimb = build_tables(cq, rows, seed)
+    lib.gpu_hip_wait()
+    log(f"tables ready in {time.perf_counter() - t0:.1f}s")
+
+    cq.profile(True)
+    errs = []
+    for w in range(a.warmup):
+        res = run_suite(cq, lib, t, dima, dimb, w, seed, rows)
+        errs += check_invariants(res, rows, w)
+        log(f"warmup {w} done")
+    lib.gpu_hip_wait()
+    lib.gpu_hip_profile_reset()
+    wall = {q: 0.0 for q in QUERY_KERNEL}
+    t_start = time.perf_counter()
+    for s in range(a.steps):
+        ts0 = time.perf_counter()
+        res = run_suite(cq, lib, t, dima, dimb, a.warmup + s, seed, rows)
+        for q in wall:
+            wall[q] += trace_of(res[q])["wall_ns"] / 1e9
+        log(f"step {s}: {1e3 * (time.perf_counter() - ts0):.2f} ms")
+    lib.gpu_hip_wait()
+    elapsed = time.perf_counter() - t_start
+    errs += check_invariants(res, rows, a.warmup + a.steps - 1)
+
+    total_bytes = sum(QUERY_BYTES_PER_ROW.values()) * rows * a.steps
+    value = total_bytes / elapsed / 1e9
+    kernels = {}
+    for q, k in QUERY_KERNEL.items():
+        ns, d = profile_stats(lib, f"{q}:{k}")
+        if d:
+            avg_us = ns / d / 1e3
+            entry = {"kernel": k, "dispatches": d, "avg_us": round(avg_us, 2)}
+            if q in QUERY_BYTES_PER_ROW:
+                entry["gbs"] = round(QUERY_BYTES_PER_ROW[q] * rows / (avg_us * 1e-6) / 1e9, 1)
+            kernels[q] = entry
+    # dominant kernel: the query kernel with the most device time
+    dom_q = max((q for q in kernels if q in QUERY_BYTES_PER_ROW),
+                key=lambda q: kernels[q]["avg_us"] * kernels[q]["dispatches"])
+    dom = kernels[dom_q]
+    achieved = QUERY_BYTES_PER_ROW[dom_q] * rows / (dom["avg_us"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_GBS, 4), "traffic": None,
+                "kernel": f"{dom['kernel']} ({dom_q})", "bytes_per_launch": QUERY_BYTES_PER_ROW[dom_q] * rows}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        try:
+            workers = len(os.sched_getaffinity(0))
+        except Exception:
+            workers = os.cpu_count() or 1
+        log(f"cpu baseline: {a.cpu_sample_rows} rows, {workers} threads")
+        gbs, secs = timed_cpu_baseline(a.cpu_sample_rows, seed, workers)
+        cpu = {"value": round(gbs, 3), "unit": "GB/s", "cores": workers, "kind": "port",
+               "sample": f"q1-q7 once each over a {a.cpu_sample_rows}-row sample of the same seeded tables "
+                         f"(oracle C restatement of the CPU operator path, {workers} threads)",
+               "per_query_s": {k: round(v, 4) for k, v in secs.items()}}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * elapsed / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic: splitmix64-seeded tables generated in HBM",
+        "config": {"workload": f"{rows}-row 4xu32 fact table + u32 fk, dim A 1M rows, dim B 50 rows; q1-q7 per step",
+                   "rows": rows, "pct_hbm_peak": round(100 * value / PEAK_GBS, 2),
+                   "per_query_wall_ms": {q: round(1e3 * v / a.steps, 3) for q, v in wall.items()},
+                   "kernels": kernels, "invariants": "ok" if not errs else sorted(set(errs))},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    t.close()
+    dima.close()
+    dimb.close()
+    return 0 if not errs else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,956 @@
+/*
+ * gpu_hip.cpp - the GPU_Backend function table (gpu_backend.h:17-71) over
+ * HIP for one MI355X (gfx950).  Replaces src/gpu/vulkan/gpu_vulkan.c for
+ * ComputeQL's operator path.
+ *
+ * Design (MI355X-first, not a translation of the Vulkan backend):
+ *  - buffers are plain hipMalloc allocations in HBM, rounded up to 256 B so
+ *    every kernel may issue 16-byte loads over a column's padded tail;
+ *  - a kernel object holds its bound args (the reference's descriptor set
+ *    per kernel, gpu_vulkan.h:34-48) but every dispatch snapshots them into
+ *    the launch's kernarg block, so rebinding after a dispatch never
+ *    retargets it (the hazard prefix_sum2.comp:3-5 works around);
+ *  - kernel objects are cached per name and per host thread, so concurrent
+ *    operators on different threads never share bound args;
+ *  - a batch (gpu_vulkan.c:3062-3116) is a HIP stream taken from a pool;
+ *    batches on different threads run concurrently;
+ *  - stream slots (gpu_vulkan.h:231-345) are one HIP stream each.
+ */
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cqlhip.h"
+#include "qk_library.h"
+
+struct GPU_Buffer {
+  void* ptr;
+  U64 size;       /* bytes asked for */
+  U64 alloc_size; /* bytes allocated (padded) */
+  U32 flags;
+  bool pooled;
+  bool host_import;
+  std::string pool_name;
+};
+
+struct GPU_Kernel {
+  std::string name;
+  const QK_LibraryKernel* lib;
+  hipModule_t module;
+  hipFunction_t fn;
+  QK_Args args;
+  U32 lds_bytes;
+  U32 missing_mask; /* bindings set to a null buffer (gpu_dispatch.c:260-306) */
+  bool generated;
+};
+
+struct PendingProfile {
+  std::string name;
+  hipEvent_t start, stop;
+};
+
+struct GPU_Batch {
+  hipStream_t stream;
+  bool failed;
+  std::vector<PendingProfile> prof;
+};
+
+struct GPU_StreamSlot {
+  hipStream_t stream;
+  hipEvent_t done;
+  bool submitted;
+  std::vector<PendingProfile> prof;
+};
+
+struct GPU_Stream {
+  std::vector<GPU_StreamSlot> slots;
+  GPU_StreamStats stats;
+};
+
+namespace {
+
+struct HipState {
+  std::mutex lock;
+  bool initialised = false;
+  int device = 0;
+  hipDeviceProp_t props;
+  hipStream_t default_stream = nullptr;
+  std::vector<hipStream_t> free_streams;
+  GPU_MemoryStats mem = {};
+  std::atomic<bool> lost{false};
+  std::unordered_map<std::string, GPU_Buffer*> pool;
+  /* generated kernels: name -> (module, function), LRU by use tick */
+  struct Generated {
+    hipModule_t module;
+    hipFunction_t fn;
+    U64 tick;
+  };
+  std::unordered_map<std::string, Generated> generated;
+  U64 generated_tick = 0;
+  U64 generated_compiles = 0;
+  U64 generated_evictions = 0;
+  U64 kernel_time_us = 0;
+  bool profile = false;
+  std::map<std::string, std::pair<U64, U64>> profile_table; /* ns, dispatches */
+  U64 fault_alloc_at = 0;
+  U64 fault_alloc_counter = 0;
+};
+
+HipState& st()
+{
+  static HipState* s = new HipState();
+  return *s;
+}
+
+thread_local char t_last_error[512];
+thread_local char t_profile_tag[64];
+
+void set_error(const char* fmt, const char* a = "", long long b = 0)
+{
+  snprintf(t_last_error, sizeof(t_last_error), fmt, a, b);
+}
+
+bool check(hipError_t e, const char* what)
+{
+  if (e == hipSuccess) return true;
+  snprintf(t_last_error, sizeof(t_last_error), "%s: %s", what, hipGetErrorString(e));
+  if (e == hipErrorIllegalAddress || e == hipErrorLaunchFailure || e == hipErrorECCNotCorrectable ||
+      e == hipErrorAssert)
+    st().lost = true; /* sticky, like gpu_vulkan.c:1523-1538 */
+  return false;
+}
+
+U64 max_generated()
+{
+  const char* env = getenv("GPU_HIP_MAX_GENERATED_KERNELS");
+  U64 v = env ? strtoull(env, nullptr, 10) : 0;
+  return v ? v : 64; /* GPU_VULKAN_MAX_GENERATED_KERNELS default, gpu_vulkan.c:2696-2761 */
+}
+
+/* per-thread kernel objects keyed by name */
+thread_local std::unordered_map<std::string, GPU_Kernel*>* t_kernels;
+
+std::unordered_map<std::string, GPU_Kernel*>& kernels()
+{
+  if (!t_kernels) t_kernels = new std::unordered_map<std::string, GPU_Kernel*>();
+  return *t_kernels;
+}
+
+hipStream_t take_stream()
+{
+  HipState& s = st();
+  std::lock_guard<std::mutex> g(s.lock);
+  if (!s.free_streams.empty()) {
+    hipStream_t h = s.free_streams.back();
+    s.free_streams.pop_back();
+    return h;
+  }
+  hipStream_t h = nullptr;
+  if (!check(hipStreamCreateWithFlags(&h, hipStreamNonBlocking), "hipStreamCreate")) return nullptr;
+  return h;
+}
+
+void give_stream(hipStream_t h)
+{
+  if (!h) return;
+  HipState& s = st();
+  std::lock_guard<std::mutex> g(s.lock);
+  s.free_streams.push_back(h);
+}
+
+void flush_profile(std::vector<PendingProfile>& prof)
+{
+  HipState& s = st();
+  for (auto& p : prof) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+      U64 ns = (U64)((double)ms * 1.0e6);
+      std::lock_guard<std::mutex> g(s.lock);
+      auto& e = s.profile_table[p.name];
+      e.first += ns;
+      e.second += 1;
+      s.kernel_time_us += ns / 1000;
+    }
+    hipEventDestroy(p.start);
+    hipEventDestroy(p.stop);
+  }
+  prof.clear();
+}
+
+bool launch(GPU_Kernel* k, U32 global_work_size, U32 local_work_size, hipStream_t stream,
+            std::vector<PendingProfile>* prof)
+{
+  if (!k) {
+    set_error("dispatch of a null kernel");
+    return false;
+  }
+  if (k->missing_mask) {
+    /* gpu_dispatch.c:260-306/:415-426: a dispatch whose args failed to
+     * allocate is skipped and counted */
+    set_error("dispatch of %s skipped: an argument buffer is missing", k->name.c_str());
+    return false;
+  }
+  U32 block = k->lib ? k->lib->block : (local_work_size ? local_work_size : 256);
+  if (k->lib && local_work_size && local_work_size != block) {
+    set_error("%s: workgroup size mismatch", k->name.c_str());
+    return false;
+  }
+  U64 blocks = ((U64)global_work_size + block - 1) / block;
+  if (blocks == 0) return true;
+  if (blocks > 0x7fffffffULL) {
+    set_error("%s: grid too large", k->name.c_str());
+    return false;
+  }
+  QK_Args args = k->args; /* snapshot */
+  args.global_work_size = global_work_size;
+  args.grid_blocks = (U32)blocks;
+  PendingProfile pp;
+  bool timed = st().profile && prof;
+  if (timed) {
+    pp.name = t_profile_tag[0] ? std::string(t_profile_tag) + ":" + k->name : k->name;
+    hipEventCreate(&pp.start);
+    hipEventCreate(&pp.stop);
+    hipEventRecord(pp.start, stream);
+  }
+  hipError_t e;
+  if (k->lib) {
+    void* params[1] = {&args};
+    e = hipLaunchKernel(k->lib->host_fn, dim3((U32)blocks), dim3(block), params, k->lds_bytes, stream);
+  } else {
+    void* params[1] = {&args};
+    e = hipModuleLaunchKernel(k->fn, (U32)blocks, 1, 1, block, 1, 1, k->lds_bytes, stream, params, nullptr);
+  }
+  if (timed) {
+    hipEventRecord(pp.stop, stream);
+    prof->push_back(pp);
+  }
+  return check(e, k->name.c_str());
+}
+
+GPU_Buffer* alloc_impl(U64 size, U32 flags, void* data)
+{
+  HipState& s = st();
+  if (!s.initialised) gpu_hip_init();
+  {
+    std::lock_guard<std::mutex> g(s.lock);
+    s.mem.alloc_calls++;
+    if (s.fault_alloc_at) {
+      s.fault_alloc_counter++;
+      if (s.fault_alloc_counter == s.fault_alloc_at) {
+        s.mem.alloc_failures++;
+        s.mem.unrecovered_alloc_failures++;
+        s.fault_alloc_at = 0;
+        set_error("injected allocation failure");
+        return nullptr;
+      }
+    }
+  }
+  U64 padded = ((size ? size : 1) + 255) & ~(U64)255;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, padded);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    /* evict pooled buffers and retry once (gpu_vulkan.c:1578) */
+    std::vector<GPU_Buffer*> victims;
+    {
+      std::lock_guard<std::mutex> g(s.lock);
+      s.mem.alloc_failures++;
+      for (auto& kv : s.pool) victims.push_back(kv.second);
+      s.pool.clear();
+    }
+    for (GPU_Buffer* v : victims) {
+      std::lock_guard<std::mutex> g(s.lock);
+      s.mem.pool_evictions++;
+      s.mem.pool_evicted_bytes += v->alloc_size;
+      s.mem.bytes_in_use -= v->alloc_size;
+      hipFree(v->ptr);
+      delete v;
+    }
+    e = hipMalloc(&p, padded);
+    std::lock_guard<std::mutex> g(s.lock);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      s.mem.unrecovered_alloc_failures++;
+      set_error("hipMalloc of %s bytes failed", "", (long long)padded);
+      return nullptr;
+    }
+    s.mem.alloc_retry_successes++;
+  }
+  GPU_Buffer* b = new GPU_Buffer();
+  b->ptr = p;
+  b->size = size;
+  b->alloc_size = padded;
+  b->flags = flags;
+  b->pooled = false;
+  b->host_import = false;
+  {
+    std::lock_guard<std::mutex> g(s.lock);
+    s.mem.bytes_in_use += padded;
+    if (s.mem.bytes_in_use > s.mem.bytes_peak) s.mem.bytes_peak = s.mem.bytes_in_use;
+  }
+  if (data && size) {
+    if (!check(hipMemcpy(p, data, size, hipMemcpyHostToDevice), "hipMemcpy H2D")) {
+      gpu_hip_buffer_release(b);
+      return nullptr;
+    }
+  }
+  if (padded > size) hipMemset((char*)p + size, 0, padded - size);
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+void gpu_hip_init(void)
+{
+  HipState& s = st();
+  std::lock_guard<std::mutex> g(s.lock);
+  if (s.initialised) return;
+  int count = 0;
+  if (!check(hipGetDeviceCount(&count), "hipGetDeviceCount") || count < 1) {
+    if (count < 1) set_error("no HIP device");
+    return;
+  }
+  const char* dev = getenv("CQLHIP_DEVICE");
+  s.device = dev ? atoi(dev) : 0;
+  if (!check(hipSetDevice(s.device), "hipSetDevice")) return;
+  if (!check(hipGetDeviceProperties(&s.props, s.device), "hipGetDeviceProperties")) return;
+  if (!check(hipStreamCreateWithFlags(&s.default_stream, hipStreamNonBlocking), "hipStreamCreate")) return;
+  s.initialised = true;
+}
+
+void gpu_hip_release(void)
+{
+  HipState& s = st();
+  if (!s.initialised) return;
+  hipDeviceSynchronize();
+  std::lock_guard<std::mutex> g(s.lock);
+  for (auto& kv : s.pool) {
+    hipFree(kv.second->ptr);
+    delete kv.second;
+  }
+  s.pool.clear();
+  for (auto& kv : s.generated) hipModuleUnload(kv.second.module);
+  s.generated.clear();
+  for (hipStream_t h : s.free_streams) hipStreamDestroy(h);
+  s.free_streams.clear();
+  /* kernel objects of other threads keep dangling module pointers; the
+   * reference re-inits the whole backend on switch (gpu_dispatch.c:130-151) */
+  if (t_kernels) {
+    for (auto& kv : *t_kernels) delete kv.second;
+    t_kernels->clear();
+  }
+}
+
+void gpu_hip_wait(void)
+{
+  if (!st().initialised) return;
+  check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+U64 gpu_hip_get_executed_kernel_time_microseconds(void) { return st().kernel_time_us; }
+
+U64 gpu_hip_device_total_memory(void)
+{
+  size_t fr = 0, tot = 0;
+  if (!st().initialised) gpu_hip_init();
+  hipMemGetInfo(&fr, &tot);
+  return tot;
+}
+
+U64 gpu_hip_device_free_memory(void)
+{
+  size_t fr = 0, tot = 0;
+  if (!st().initialised) gpu_hip_init();
+  hipMemGetInfo(&fr, &tot);
+  return fr;
+}
+
+/* no per-binding range limit on HIP; report total memory */
+U64 gpu_hip_device_max_storage_buffer_range(void) { return gpu_hip_device_total_memory(); }
+
+B32 gpu_hip_device_lost(void) { return st().lost ? 1 : 0; }
+
+void gpu_hip_pool_begin_epoch(void) {}
+
+GPU_MemoryStats gpu_hip_memory_stats(void)
+{
+  std::lock_guard<std::mutex> g(st().lock);
+  return st().mem;
+}
+
+GPU_Buffer* gpu_hip_buffer_alloc(U64 size, GPU_BufferFlags flags, void* data)
+{
+  return alloc_impl(size, (U32)flags, data);
+}
+
+GPU_Buffer* gpu_hip_buffer_alloc_pooled(String8 name, U64 size, GPU_BufferFlags flags, void* data)
+{
+  HipState& s = st();
+  std::string key((const char*)name.str, (size_t)name.size);
+  GPU_Buffer* hit = nullptr;
+  {
+    std::lock_guard<std::mutex> g(s.lock);
+    auto it = s.pool.find(key);
+    if (it != s.pool.end()) {
+      if (it->second->alloc_size >= size) {
+        hit = it->second;
+        s.pool.erase(it);
+      }
+    }
+  }
+  if (hit) {
+    hit->size = size;
+    if (data && size) hipMemcpy(hit->ptr, data, size, hipMemcpyHostToDevice);
+    hit->pooled = true;
+    hit->pool_name = key;
+    return hit;
+  }
+  GPU_Buffer* b = alloc_impl(size, (U32)flags, data);
+  if (b) {
+    b->pooled = true;
+    b->pool_name = key;
+  }
+  return b;
+}
+
+/* Host-pointer import (gpu_vulkan.c:2213-2358).  Registering existing host
+ * memory is not available on this platform, so the import is a device copy
+ * made once; callers treat it as read-only like the reference does. */
+GPU_Buffer* gpu_hip_buffer_import_host_readonly(void* host_ptr, U64 size)
+{
+  GPU_Buffer* b = alloc_impl(size, GPU_BufferFlag_Read, host_ptr);
+  if (b) b->host_import = true;
+  return b;
+}
+
+GPU_Buffer* gpu_hip_buffer_import_host_readonly_pooled(String8 name, void* host_ptr, U64 size)
+{
+  return gpu_hip_buffer_alloc_pooled(name, size, GPU_BufferFlag_Read, host_ptr);
+}
+
+void gpu_hip_buffer_release(GPU_Buffer* buffer)
+{
+  if (!buffer) return;
+  HipState& s = st();
+  if (buffer->pooled) {
+    /* back to the pool under its name; a previous holder is freed */
+    GPU_Buffer* old = nullptr;
+    {
+      std::lock_guard<std::mutex> g(s.lock);
+      auto it = s.pool.find(buffer->pool_name);
+      if (it != s.pool.end()) old = it->second;
+      s.pool[buffer->pool_name] = buffer;
+    }
+    if (old && old != buffer) {
+      hipFree(old->ptr);
+      std::lock_guard<std::mutex> g(s.lock);
+      s.mem.bytes_in_use -= old->alloc_size;
+      delete old;
+    }
+    return;
+  }
+  hipFree(buffer->ptr);
+  std::lock_guard<std::mutex> g(s.lock);
+  s.mem.bytes_in_use -= buffer->alloc_size;
+  delete buffer;
+}
+
+void gpu_hip_buffer_write(GPU_Buffer* buffer, void* data, U64 size)
+{
+  if (!buffer || !data) return;
+  if (size > buffer->alloc_size) size = buffer->alloc_size;
+  check(hipMemcpy(buffer->ptr, data, size, hipMemcpyHostToDevice), "buffer_write");
+}
+
+void gpu_hip_buffer_read(GPU_Buffer* buffer, void* data, U64 size)
+{
+  if (!buffer || !data) return;
+  if (size > buffer->alloc_size) size = buffer->alloc_size;
+  if (!check(hipMemcpy(data, buffer->ptr, size, hipMemcpyDeviceToHost), "buffer_read")) memset(data, 0, size);
+}
+
+GPU_Kernel* gpu_hip_kernel_alloc(String8 name)
+{
+  HipState& s = st();
+  if (!s.initialised) gpu_hip_init();
+  std::string key((const char*)name.str, (size_t)name.size);
+  auto& ks = kernels();
+  auto it = ks.find(key);
+  if (it != ks.end()) return it->second;
+  const QK_LibraryKernel* lib = qk_library_find(key.c_str());
+  if (!lib) {
+    /* a generated kernel already loaded under this name */
+    std::lock_guard<std::mutex> g(s.lock);
+    auto gi = s.generated.find(key);
+    if (gi == s.generated.end()) {
+      set_error("no kernel named %s", key.c_str());
+      return nullptr;
+    }
+    GPU_Kernel* k = new GPU_Kernel();
+    memset(&k->args, 0, sizeof(k->args));
+    k->name = key;
+    k->lib = nullptr;
+    k->module = gi->second.module;
+    k->fn = gi->second.fn;
+    k->generated = true;
+    k->missing_mask = 0;
+    k->lds_bytes = 0;
+    gi->second.tick = ++s.generated_tick;
+    ks[key] = k;
+    return k;
+  }
+  GPU_Kernel* k = new GPU_Kernel();
+  memset(&k->args, 0, sizeof(k->args));
+  k->name = key;
+  k->lib = lib;
+  k->module = nullptr;
+  k->fn = nullptr;
+  k->generated = false;
+  k->missing_mask = 0;
+  k->lds_bytes = 0;
+  ks[key] = k;
+  return k;
+}
+
+/* A kernel from a code object made at run time by hiprtc (the reference
+ * passes SPIR-V from shaderc here; the ABI name is kept).  An empty blob only
+ * looks the name up.  Loaded modules live in an LRU of
+ * GPU_HIP_MAX_GENERATED_KERNELS (default 64) entries; the kernel function in
+ * the code object must be named `qk_main`. */
+GPU_Kernel* gpu_hip_kernel_alloc_from_spirv(String8 name, String8 code_object)
+{
+  HipState& s = st();
+  if (!s.initialised) gpu_hip_init();
+  std::string key((const char*)name.str, (size_t)name.size);
+  {
+    std::lock_guard<std::mutex> g(s.lock);
+    auto gi = s.generated.find(key);
+    if (gi != s.generated.end()) {
+      gi->second.tick = ++s.generated_tick;
+    }
+  }
+  auto& ks = kernels();
+  auto it = ks.find(key);
+  if (it != ks.end() && it->second->generated) {
+    std::lock_guard<std::mutex> g(s.lock);
+    auto gi = s.generated.find(key);
+    if (gi != s.generated.end() && gi->second.module == it->second->module) return it->second;
+    /* evicted meanwhile: drop the stale object */
+    delete it->second;
+    ks.erase(it);
+  }
+  bool have = false;
+  {
+    std::lock_guard<std::mutex> g(s.lock);
+    have = s.generated.count(key) != 0;
+  }
+  if (!have) {
+    if (code_object.size == 0) return nullptr;
+    hipModule_t mod = nullptr;
+    if (!check(hipModuleLoadData(&mod, code_object.str), "hipModuleLoadData")) return nullptr;
+    hipFunction_t fn = nullptr;
+    if (!check(hipModuleGetFunction(&fn, mod, "qk_main"), "hipModuleGetFunction")) {
+      hipModuleUnload(mod);
+      return nullptr;
+    }
+    std::lock_guard<std::mutex> g(s.lock);
+    if (s.generated.size() >= max_generated()) {
+      /* evict the least recently used module; callers on other threads hold
+       * kernel objects only for the duration of one operator call */
+      auto victim = s.generated.begin();
+      for (auto gi = s.generated.begin(); gi != s.generated.end(); ++gi)
+        if (gi->second.tick < victim->second.tick) victim = gi;
+      hipModuleUnload(victim->second.module);
+      auto vk = ks.find(victim->first);
+      if (vk != ks.end()) {
+        delete vk->second;
+        ks.erase(vk);
+      }
+      s.generated.erase(victim);
+      s.generated_evictions++;
+    }
+    s.generated[key] = HipState::Generated{mod, fn, ++s.generated_tick};
+    s.generated_compiles++;
+  }
+  return gpu_hip_kernel_alloc(name);
+}
+
+/* specialization constants have no HIP counterpart: run-time compiled
+ * source bakes them in instead */
+GPU_Kernel* gpu_hip_kernel_alloc_from_spirv_spec(String8 name, String8 code_object, U32* spec_values, U32 spec_count)
+{
+  (void)spec_values;
+  (void)spec_count;
+  return gpu_hip_kernel_alloc_from_spirv(name, code_object);
+}
+
+void gpu_hip_generated_kernel_stats(U64* out_compiles, U64* out_evictions)
+{
+  std::lock_guard<std::mutex> g(st().lock);
+  if (out_compiles) *out_compiles = st().generated_compiles;
+  if (out_evictions) *out_evictions = st().generated_evictions;
+}
+
+/* kernels are cached by name, release is a no-op (gpu_vulkan.c:2771-2774) */
+void gpu_hip_kernel_release(GPU_Kernel* kernel) { (void)kernel; }
+
+void gpu_hip_kernel_execute(GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size)
+{
+  HipState& s = st();
+  std::vector<PendingProfile> prof;
+  if (launch(kernel, global_work_size, local_work_size, s.default_stream, &prof))
+    check(hipStreamSynchronize(s.default_stream), "kernel_execute");
+  flush_profile(prof);
+}
+
+void gpu_hip_kernel_set_arg_buffer(GPU_Kernel* kernel, U32 index, GPU_Buffer* buffer)
+{
+  if (!kernel || index >= QK_MAX_BINDINGS) return;
+  if (!buffer) {
+    kernel->missing_mask |= 1u << index;
+    kernel->args.ptr[index] = 0;
+    kernel->args.bytes[index] = 0;
+    return;
+  }
+  kernel->missing_mask &= ~(1u << index);
+  kernel->args.ptr[index] = (U64)(uintptr_t)buffer->ptr;
+  kernel->args.bytes[index] = buffer->size;
+}
+
+void gpu_hip_kernel_clear_arg(GPU_Kernel* kernel, U32 index)
+{
+  if (!kernel || index >= QK_MAX_BINDINGS) return;
+  kernel->missing_mask &= ~(1u << index);
+  kernel->args.ptr[index] = 0;
+  kernel->args.bytes[index] = 0;
+}
+
+void gpu_hip_kernel_set_arg_u32(GPU_Kernel* kernel, U32 word, U32 value)
+{
+  if (!kernel || word >= QK_PUSH_WORDS) return;
+  kernel->args.words[word] = value;
+}
+
+void gpu_hip_kernel_set_arg_u64(GPU_Kernel* kernel, U32 index, U64 value)
+{
+  if (!kernel || index >= QK_PUSH_WORDS / 2) return;
+  kernel->args.words[2 * index] = (U32)(value & 0xffffffffu);
+  kernel->args.words[2 * index + 1] = (U32)(value >> 32);
+}
+
+GPU_Batch* gpu_hip_batch_begin(U64 upload_bytes_needed, U64 download_bytes_needed)
+{
+  (void)upload_bytes_needed;
+  (void)download_bytes_needed;
+  if (!st().initialised) gpu_hip_init();
+  hipStream_t h = take_stream();
+  if (!h) return nullptr;
+  GPU_Batch* b = new GPU_Batch();
+  b->stream = h;
+  b->failed = false;
+  return b;
+}
+
+void gpu_hip_batch_buffer_write(GPU_Batch* batch, GPU_Buffer* buffer, void* data, U64 size)
+{
+  if (!batch || !buffer || !data) return;
+  if (size > buffer->alloc_size) size = buffer->alloc_size;
+  if (!check(hipMemcpyAsync(buffer->ptr, data, size, hipMemcpyHostToDevice, batch->stream), "batch_buffer_write"))
+    batch->failed = true;
+}
+
+void gpu_hip_batch_buffer_fill(GPU_Batch* batch, GPU_Buffer* buffer, U64 size, U32 value)
+{
+  if (!batch || !buffer) return;
+  if (size > buffer->alloc_size) size = buffer->alloc_size;
+  /* a fill kernel rather than hipMemsetAsync, so a captured batch replays
+   * correctly */
+  GPU_Kernel* k = gpu_hip_kernel_alloc(String8{(U8*)"fill_u32", 8});
+  if (!k) {
+    batch->failed = true;
+    return;
+  }
+  QK_Args saved = k->args;
+  U32 saved_missing = k->missing_mask;
+  gpu_hip_kernel_set_arg_buffer(k, 0, buffer);
+  gpu_hip_kernel_set_arg_u64(k, 0, size / 4);
+  gpu_hip_kernel_set_arg_u64(k, 1, value);
+  U64 words = size / 4;
+  U64 threads = (words + 3) / 4;
+  U64 cap = (U64)st().props.multiProcessorCount * 8 * 256;
+  if (threads > cap) threads = cap;
+  if (threads == 0) threads = 1;
+  if (!launch(k, (U32)threads, 256, batch->stream, &batch->prof)) batch->failed = true;
+  k->args = saved;
+  k->missing_mask = saved_missing;
+}
+
+void gpu_hip_batch_buffer_zero(GPU_Batch* batch, GPU_Buffer* buffer, U64 size)
+{
+  gpu_hip_batch_buffer_fill(batch, buffer, size, 0);
+}
+
+void gpu_hip_batch_kernel_execute(GPU_Batch* batch, GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size)
+{
+  if (!batch) return;
+  if (!launch(kernel, global_work_size, local_work_size, batch->stream, &batch->prof)) batch->failed = true;
+}
+
+void gpu_hip_batch_buffer_read(GPU_Batch* batch, GPU_Buffer* buffer, void* out_data, U64 size)
+{
+  if (!batch || !buffer || !out_data) return;
+  if (size > buffer->alloc_size) size = buffer->alloc_size;
+  if (!check(hipMemcpyAsync(out_data, buffer->ptr, size, hipMemcpyDeviceToHost, batch->stream), "batch_buffer_read"))
+    batch->failed = true;
+}
+
+B32 gpu_hip_batch_end(GPU_Batch* batch)
+{
+  if (!batch) return 0;
+  bool ok = check(hipStreamSynchronize(batch->stream), "batch_end") && !batch->failed;
+  flush_profile(batch->prof);
+  give_stream(batch->stream);
+  delete batch;
+  return ok ? 1 : 0;
+}
+
+GPU_Stream* gpu_hip_stream_alloc(U32 slot_count)
+{
+  if (!st().initialised) gpu_hip_init();
+  if (slot_count == 0) slot_count = 1;
+  GPU_Stream* s = new GPU_Stream();
+  memset(&s->stats, 0, sizeof(s->stats));
+  s->stats.separate_copy_queue = 1;
+  s->slots.resize(slot_count);
+  for (auto& sl : s->slots) {
+    sl.stream = take_stream();
+    hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    sl.submitted = false;
+  }
+  return s;
+}
+
+void gpu_hip_stream_release(GPU_Stream* stream)
+{
+  if (!stream) return;
+  for (auto& sl : stream->slots) {
+    hipStreamSynchronize(sl.stream);
+    flush_profile(sl.prof);
+    hipEventDestroy(sl.done);
+    give_stream(sl.stream);
+  }
+  delete stream;
+}
+
+B32 gpu_hip_stream_slot_begin(GPU_Stream* stream, U32 slot, U64 upload_bytes, U64 readback_bytes)
+{
+  (void)upload_bytes;
+  (void)readback_bytes;
+  if (!stream || slot >= stream->slots.size()) return 0;
+  GPU_StreamSlot& sl = stream->slots[slot];
+  if (sl.submitted) {
+    if (!check(hipEventSynchronize(sl.done), "stream_slot_begin")) return 0;
+    sl.submitted = false;
+  }
+  return 1;
+}
+
+B32 gpu_hip_stream_copy(GPU_Stream* stream, U32 slot, GPU_Buffer* destination, void* host_ptr, U64 size)
+{
+  if (!stream || slot >= stream->slots.size() || !destination || !host_ptr) return 0;
+  if (size > destination->alloc_size) size = destination->alloc_size;
+  stream->stats.bytes_copied += size;
+  return check(hipMemcpyAsync(destination->ptr, host_ptr, size, hipMemcpyHostToDevice, stream->slots[slot].stream),
+               "stream_copy")
+             ? 1
+             : 0;
+}
+
+void gpu_hip_stream_fill(GPU_Stream* stream, U32 slot, GPU_Buffer* buffer, U64 size, U32 value)
+{
+  if (!stream || slot >= stream->slots.size()) return;
+  GPU_Batch tmp;
+  tmp.stream = stream->slots[slot].stream;
+  tmp.failed = false;
+  gpu_hip_batch_buffer_fill(&tmp, buffer, size, value);
+  for (auto& p : tmp.prof) stream->slots[slot].prof.push_back(p);
+}
+
+void gpu_hip_stream_zero(GPU_Stream* stream, U32 slot, GPU_Buffer* buffer, U64 size)
+{
+  gpu_hip_stream_fill(stream, slot, buffer, size, 0);
+}
+
+void gpu_hip_stream_dispatch(GPU_Stream* stream, U32 slot, GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size)
+{
+  if (!stream || slot >= stream->slots.size()) return;
+  launch(kernel, global_work_size, local_work_size, stream->slots[slot].stream, &stream->slots[slot].prof);
+}
+
+void gpu_hip_stream_read(GPU_Stream* stream, U32 slot, GPU_Buffer* buffer, void* out_data, U64 size)
+{
+  if (!stream || slot >= stream->slots.size() || !buffer || !out_data) return;
+  if (size > buffer->alloc_size) size = buffer->alloc_size;
+  check(hipMemcpyAsync(out_data, buffer->ptr, size, hipMemcpyDeviceToHost, stream->slots[slot].stream), "stream_read");
+}
+
+B32 gpu_hip_stream_submit(GPU_Stream* stream, U32 slot)
+{
+  if (!stream || slot >= stream->slots.size()) return 0;
+  GPU_StreamSlot& sl = stream->slots[slot];
+  if (!check(hipEventRecord(sl.done, sl.stream), "stream_submit")) return 0;
+  sl.submitted = true;
+  return 1;
+}
+
+B32 gpu_hip_stream_wait(GPU_Stream* stream, U32 slot)
+{
+  if (!stream || slot >= stream->slots.size()) return 0;
+  GPU_StreamSlot& sl = stream->slots[slot];
+  bool ok = check(hipStreamSynchronize(sl.stream), "stream_wait");
+  sl.submitted = false;
+  flush_profile(sl.prof);
+  return ok ? 1 : 0;
+}
+
+GPU_StreamStats gpu_hip_stream_get_stats(GPU_Stream* stream)
+{
+  GPU_StreamStats z;
+  memset(&z, 0, sizeof(z));
+  return stream ? stream->stats : z;
+}
+
+void* gpu_hip_scratch_arena(void) { return nullptr; }
+
+String8 gpu_hip_name(void) { return String8{(U8*)"hip", 3}; }
+
+U32 gpu_hip_compute_units(void)
+{
+  if (!st().initialised) gpu_hip_init();
+  return st().initialised ? (U32)st().props.multiProcessorCount : 0;
+}
+
+U64 gpu_hip_buffer_size(GPU_Buffer* b) { return b ? b->size : 0; }
+
+void gpu_hip_kernel_set_lds_bytes(GPU_Kernel* kernel, U32 bytes)
+{
+  if (!kernel) return;
+  if (bytes > 65536 && kernel->lib && bytes != kernel->lds_bytes)
+    (void)hipFuncSetAttribute(kernel->lib->host_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  kernel->lds_bytes = bytes;
+}
+
+U32 gpu_hip_max_lds_bytes(void)
+{
+  if (!st().initialised) gpu_hip_init();
+  if (!st().initialised) return 0;
+  size_t optin = st().props.sharedMemPerBlockOptin;
+  size_t base = st().props.sharedMemPerBlock;
+  return (U32)(optin > base ? optin : base);
+}
+
+U64 gpu_hip_buffer_device_address(GPU_Buffer* b) { return b ? (U64)(uintptr_t)b->ptr : 0; }
+
+void gpu_hip_profile_enable(B32 enabled) { st().profile = enabled != 0; }
+
+void gpu_hip_profile_tag(const char* tag)
+{
+  snprintf(t_profile_tag, sizeof(t_profile_tag), "%s", tag ? tag : "");
+}
+
+void gpu_hip_profile_reset(void)
+{
+  std::lock_guard<std::mutex> g(st().lock);
+  st().profile_table.clear();
+  st().kernel_time_us = 0;
+}
+
+U64 gpu_hip_profile_query(const char* name, U64* out_dispatches)
+{
+  std::lock_guard<std::mutex> g(st().lock);
+  U64 ns = 0, n = 0;
+  for (auto& kv : st().profile_table) {
+    size_t len = name ? strlen(name) : 0;
+    bool prefix = len > 0 && name[len - 1] == '*';
+    if (!name || !name[0] || kv.first == name || (prefix && kv.first.compare(0, len - 1, name, len - 1) == 0)) {
+      ns += kv.second.first;
+      n += kv.second.second;
+    }
+  }
+  if (out_dispatches) *out_dispatches = n;
+  return ns;
+}
+
+void gpu_hip_fault_alloc_at(U64 k)
+{
+  std::lock_guard<std::mutex> g(st().lock);
+  st().fault_alloc_at = k;
+  st().fault_alloc_counter = 0;
+}
+
+const char* gpu_hip_last_error(void) { return t_last_error; }
+
+GPU_Backend* gpu_hip_backend(void)
+{
+  static GPU_Backend b = {
+      (U32)sizeof(GPU_Backend),
+      gpu_hip_init,
+      gpu_hip_release,
+      gpu_hip_wait,
+      gpu_hip_get_executed_kernel_time_microseconds,
+      gpu_hip_device_total_memory,
+      gpu_hip_device_free_memory,
+      gpu_hip_device_max_storage_buffer_range,
+      gpu_hip_device_lost,
+      gpu_hip_pool_begin_epoch,
+      gpu_hip_memory_stats,
+      gpu_hip_buffer_alloc,
+      gpu_hip_buffer_alloc_pooled,
+      gpu_hip_buffer_import_host_readonly,
+      gpu_hip_buffer_import_host_readonly_pooled,
+      gpu_hip_buffer_release,
+      gpu_hip_buffer_write,
+      gpu_hip_buffer_read,
+      gpu_hip_kernel_alloc,
+      gpu_hip_kernel_alloc_from_spirv,
+      gpu_hip_kernel_alloc_from_spirv_spec,
+      gpu_hip_generated_kernel_stats,
+      gpu_hip_kernel_release,
+      gpu_hip_kernel_execute,
+      gpu_hip_kernel_set_arg_buffer,
+      gpu_hip_kernel_set_arg_u64,
+      gpu_hip_batch_begin,
+      gpu_hip_batch_buffer_write,
+      gpu_hip_batch_buffer_zero,
+      gpu_hip_batch_buffer_fill,
+      gpu_hip_batch_kernel_execute,
+      gpu_hip_batch_buffer_read,
+      gpu_hip_batch_end,
+      gpu_hip_stream_alloc,
+      gpu_hip_stream_release,
+      gpu_hip_stream_slot_begin,
+      gpu_hip_stream_copy,
+      gpu_hip_stream_zero,
+      gpu_hip_stream_fill,
+      gpu_hip_stream_dispatch,
+      gpu_hip_stream_read,
+      gpu_hip_stream_submit,
+      gpu_hip_stream_wait,
+      gpu_hip_stream_get_stats,
+      gpu_hip_scratch_arena,
+      gpu_hip_name,
+  };
+  return &b;
+}
+
+}  // extern "C"

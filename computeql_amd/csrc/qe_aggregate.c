@@ -1,0 +1,474 @@
+/*
+ * qe_aggregate.c - global and grouped aggregates on the GPU.
+ *
+ *   global (no GROUP BY)   qe_aggregate_global_streamed :8842-9110 and
+ *                          qe_aggregate_global_gpu :9309-9437 (whole columns);
+ *                          qe_fused_filter_aggregate :9888-10253 (WHERE fused)
+ *   dense GROUP BY         qe_dense_stream_core :7632-7980,
+ *                          qe_aggregate_dense_streamed :7984-8049;
+ *                          qe_fused_filter_group_aggregate :9442-9886 (WHERE fused)
+ *   output typing          qe_aggregate_build_output :7053, qe_agg_func_policy
+ *                          (query_exec.h:917-926)
+ *
+ * Arithmetic: U32/I32 arguments are summed as exact 64-bit integers and
+ * converted to F64 once per group; MIN/MAX are exact and converted to F64.
+ * The CPU path sums in F64 in row order, which is exact (and therefore
+ * identical) while every partial sum stays below 2^53.  F32/F64/U64/I64
+ * arguments are summed in F64: per thread over its rows in order, then a
+ * fixed wave/block tree, then blocks in block order.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdlib.h>
+
+#include "qe_internal.h"
+
+#define QE_AGG_MAX_ARGS 4
+
+typedef struct QE_AggPlan {
+  U32 expr_count;
+  U32 funcs[QE_AGG_MAX_EXPRS];
+  S32 expr_arg[QE_AGG_MAX_EXPRS]; /* index into args, -1 for COUNT(*) */
+  U32 arg_count;
+  U32 arg_col[QE_AGG_MAX_ARGS];
+  B32 need_sum, need_minmax;
+} QE_AggPlan;
+
+static B32 qe_agg_plan(QE_Table* t, const U32* funcs, const S32* arg_cols, U32 expr_count, QE_AggPlan* p)
+{
+  memset(p, 0, sizeof(*p));
+  if (expr_count == 0 || expr_count > QE_AGG_MAX_EXPRS) {
+    qe_set_error("aggregate: 1..%d expressions", QE_AGG_MAX_EXPRS);
+    return 0;
+  }
+  p->expr_count = expr_count;
+  for (U32 e = 0; e < expr_count; e++) {
+    U32 f = funcs[e];
+    if (f > QE_AGG_FUNC_MAX) {
+      qe_set_error("aggregate: unsupported function %u", f);
+      return 0;
+    }
+    p->funcs[e] = f;
+    p->expr_arg[e] = -1;
+    if (arg_cols[e] < 0) {
+      if (f != QE_AGG_FUNC_COUNT) {
+        qe_set_error("aggregate: function %u needs an argument", f);
+        return 0;
+      }
+      continue;
+    }
+    if ((U32)arg_cols[e] >= t->column_count) {
+      qe_set_error("aggregate: unknown column %d", arg_cols[e]);
+      return 0;
+    }
+    if (f == QE_AGG_FUNC_COUNT) continue; /* COUNT(col): no NULLs on this path */
+    U32 a = 0;
+    while (a < p->arg_count && p->arg_col[a] != (U32)arg_cols[e]) a++;
+    if (a == p->arg_count) {
+      if (p->arg_count == QE_AGG_MAX_ARGS) {
+        qe_set_error("aggregate: more than %d distinct argument columns", QE_AGG_MAX_ARGS);
+        return 0;
+      }
+      p->arg_col[p->arg_count++] = (U32)arg_cols[e];
+    }
+    p->expr_arg[e] = (S32)a;
+    if (f == QE_AGG_FUNC_SUM || f == QE_AGG_FUNC_AVG) p->need_sum = 1;
+    if (f == QE_AGG_FUNC_MIN || f == QE_AGG_FUNC_MAX) p->need_minmax = 1;
+  }
+  return 1;
+}
+
+static F64 qe_key_to_f64(U32 type, U64 key)
+{
+  switch (type) {
+    case CQL_ColumnType_U32: return (F64)(U32)key;
+    case CQL_ColumnType_I32: return (F64)(S32)((U32)key ^ 0x80000000u);
+    case CQL_ColumnType_U64: return (F64)key;
+    case CQL_ColumnType_I64: return (F64)(S64)(key ^ 0x8000000000000000ull);
+    default: {
+      U64 b = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+      F64 d;
+      memcpy(&d, &b, 8);
+      return d;
+    }
+  }
+}
+
+static F64 qe_sum_to_f64(U32 type, U64 sum)
+{
+  if (type == CQL_ColumnType_U32) return (F64)sum;
+  if (type == CQL_ColumnType_I32) return (F64)(S64)sum;
+  F64 d;
+  memcpy(&d, &sum, 8);
+  return d;
+}
+
+/* ---------------- global aggregate (one pass, optional fused WHERE) ---------------- */
+static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune,
+                               const QE_AggPlan* p, U64* out_cells, QE_Trace* trace)
+{
+  GPU_Kernel* k = qe_kernel("aggregate_global");
+  if (!k) return 0;
+  QE_ColumnSlots slots;
+  memset(&slots, 0, sizeof(slots));
+  U32 arg_slot[QE_AGG_MAX_ARGS];
+  U32 slot_bits = 0, type_bits = 0;
+  for (U32 a = 0; a < p->arg_count; a++) {
+    arg_slot[a] = qe_slot_of(&slots, p->arg_col[a]);
+    slot_bits |= arg_slot[a] << (4 * a);
+    type_bits |= t->columns[p->arg_col[a]].type << (8 * a);
+  }
+  U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
+  for (U32 l = 0; l < leaf_count; l++) {
+    U32 slot = qe_slot_of(&slots, leaves[l].column);
+    if (slot == ~0u) {
+      qe_set_error("aggregate: too many columns");
+      return 0;
+    }
+    qe_simple_leaf_words(&leaf_words[6 * l], slot, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+  }
+  U64 r0 = prune->row_begin, r1 = prune->row_end;
+  U64 quads = (r1 + 3) / 4 - r0 / 4;
+  U32 blocks = qe_stream_blocks(quads);
+  GPU_Buffer* result = qe_scratch("agg_global_result", 16 * 8);
+  GPU_Buffer* partials = qe_scratch("agg_global_partials", (U64)blocks * 13 * 8);
+  GPU_Buffer* leaf_buf = leaf_count ? qe_scratch("agg_global_leaves", sizeof(leaf_words)) : 0;
+  GPU_Buffer* skip_buf = prune->skip_bits ? qe_scratch("agg_global_skip", prune->skip_words * 4) : 0;
+  B32 ok = result && partials && (!leaf_count || leaf_buf) && (!prune->skip_bits || skip_buf);
+  U64 res[16];
+  memset(res, 0, sizeof(res));
+  if (ok) {
+    if (leaf_buf) gpu_buffer_write(leaf_buf, leaf_words, 24 * (U64)leaf_count);
+    if (skip_buf) gpu_buffer_write(skip_buf, prune->skip_bits, prune->skip_words * 4);
+    if (leaf_buf) gpu_kernel_set_arg_buffer(k, 0, leaf_buf);
+    else gpu_hip_kernel_clear_arg(k, 0);
+    gpu_kernel_set_arg_buffer(k, 1, result);
+    gpu_kernel_set_arg_buffer(k, 2, partials);
+    if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
+    else gpu_hip_kernel_clear_arg(k, 5);
+    for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
+      if (s < slots.count) gpu_kernel_set_arg_buffer(k, 6 + s, t->columns[slots.column[s]].buffer);
+      else gpu_hip_kernel_clear_arg(k, 6 + s);
+    }
+    gpu_kernel_set_arg_u64(k, 0, r1 - r0);
+    gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
+    gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
+    gpu_kernel_set_arg_u64(k, 5, r0);
+    gpu_kernel_set_arg_u64(k, 7, leaf_count);
+    qe_kernel_set_word(k, 12, p->arg_count | (slot_bits << 8));
+    qe_kernel_set_word(k, 13, type_bits);
+    GPU_Batch* b = gpu_batch_begin(0, sizeof(res));
+    /* the ticket word of the result must start at zero; the kernel leaves
+     * it at zero when it finishes */
+    gpu_batch_buffer_zero(b, result, 16 * 8);
+    gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+    gpu_batch_buffer_read(b, result, res, sizeof(res));
+    ok = gpu_batch_end(b);
+  }
+  qe_scratch_release(result);
+  qe_scratch_release(partials);
+  qe_scratch_release(leaf_buf);
+  qe_scratch_release(skip_buf);
+  if (!ok) {
+    qe_set_error("aggregate_global: %s", gpu_hip_last_error());
+    return 0;
+  }
+  U64 count = res[0];
+  for (U32 e = 0; e < p->expr_count; e++) {
+    S32 a = p->expr_arg[e];
+    F64 sum = 0.0, mn = 0.0, mx = 0.0;
+    if (a >= 0 && count > 0) {
+      U32 type = t->columns[p->arg_col[a]].type;
+      sum = qe_sum_to_f64(type, res[1 + 3 * a]);
+      mn = qe_key_to_f64(type, res[2 + 3 * a]);
+      mx = qe_key_to_f64(type, res[3 + 3 * a]);
+    }
+    out_cells[e] = qe_agg_cell(p->funcs[e], count, sum, mn, mx);
+  }
+  if (trace) {
+    trace->path = leaf_count ? QE_AggPath_FusedGlobal : QE_AggPath_Global;
+    trace->kernel_count = 1;
+    trace->rows_scanned = r1 - r0;
+    trace->rows_after = count;
+    U64 bytes = 0;
+    for (U32 s = 0; s < slots.count; s++) bytes += (r1 - r0) * t->columns[slots.column[s]].width;
+    trace->bytes_read = bytes;
+    snprintf(trace->kernel, sizeof(trace->kernel), "aggregate_global");
+  }
+  return 1;
+}
+
+/* ---------------- dense GROUP BY ---------------- */
+/* per-group device totals of aggregate_dense / join_probe: count u64[R];
+ * sum u64[E][R]; min u32[E][R]; max u32[E][R] */
+
+static B32 qe_dense_totals_alloc(QE_DenseTotals* d, U32 range, U32 args)
+{
+  d->range = range;
+  d->args = args;
+  U64 bytes = (U64)range * 8 * (1 + args) + (U64)range * 4 * 2 * args;
+  d->block = malloc(bytes ? bytes : 8);
+  if (!d->block) return 0;
+  d->count = (U64*)d->block;
+  d->sum = d->count + range;
+  d->mn = (U32*)(d->sum + (U64)range * args);
+  d->mx = d->mn + (U64)range * args;
+  return 1;
+}
+
+U64 qe_dense_totals_bytes(U32 range, U32 args) { return (U64)range * 8 * (1 + args) + (U64)range * 4 * 2 * args; }
+
+/* runs aggregate_dense (key_col of `t`, args) or join_probe; fills totals */
+B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B32 need_sum, B32 need_minmax,
+                 const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune, U32 key_min_code, U32 range,
+                 QE_DenseTotals* out, QE_Trace* trace)
+{
+  GPU_Kernel* k = qe_kernel("aggregate_dense");
+  if (!k) return 0;
+  U32 flags = (need_sum ? 1u : 0u) | (need_minmax ? 2u : 0u);
+  U64 lds = (U64)range * 4 + ((flags & 1u) ? (U64)arg_count * range * 8 : 0) + ((flags & 2u) ? (U64)arg_count * range * 8 : 0);
+  U32 max_lds = gpu_hip_max_lds_bytes();
+  if (lds > max_lds) {
+    qe_set_error("dense group by: %llu bytes of LDS needed, %u available", (unsigned long long)lds, max_lds);
+    return 0;
+  }
+  QE_ColumnSlots slots;
+  memset(&slots, 0, sizeof(slots));
+  U32 key_slot = qe_slot_of(&slots, key_col);
+  U32 slot_bits = 0, type_bits = 0;
+  for (U32 a = 0; a < arg_count; a++) {
+    U32 s = qe_slot_of(&slots, arg_cols[a]);
+    U32 ty = t->columns[arg_cols[a]].type;
+    if (ty != CQL_ColumnType_U32 && ty != CQL_ColumnType_I32) {
+      qe_set_error("dense group by: arguments must be U32/I32");
+      return 0;
+    }
+    slot_bits |= s << (4 * a);
+    type_bits |= ty << (8 * a);
+  }
+  U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
+  for (U32 l = 0; l < leaf_count; l++) {
+    U32 s = qe_slot_of(&slots, leaves[l].column);
+    if (s == ~0u) {
+      qe_set_error("dense group by: too many columns");
+      return 0;
+    }
+    qe_simple_leaf_words(&leaf_words[6 * l], s, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+  }
+  if (!qe_dense_totals_alloc(out, range, arg_count)) return 0;
+  U64 r0 = prune->row_begin, r1 = prune->row_end;
+  U64 quads = (r1 + 3) / 4 - r0 / 4;
+  /* one LDS table per block: fewer, fuller blocks when the table is big */
+  U32 blocks = qe_stream_blocks(quads);
+  U32 per_cu_lds = blocks;
+  U64 cus = gpu_hip_compute_units();
+  U64 lds_blocks = lds ? (160ull * 1024 / (lds + 1024)) : 8;
+  if (lds_blocks < 1) lds_blocks = 1;
+  if (lds_blocks > qe_setting_blocks_per_cu()) lds_blocks = qe_setting_blocks_per_cu();
+  if ((U64)blocks > cus * lds_blocks) blocks = (U32)(cus * lds_blocks);
+  (void)per_cu_lds;
+  U64 gbytes = qe_dense_totals_bytes(range, arg_count);
+  GPU_Buffer* acc = qe_scratch("dense_totals", gbytes);
+  GPU_Buffer* leaf_buf = leaf_count ? qe_scratch("dense_leaves", sizeof(leaf_words)) : 0;
+  GPU_Buffer* skip_buf = prune->skip_bits ? qe_scratch("dense_skip", prune->skip_words * 4) : 0;
+  B32 ok = acc && (!leaf_count || leaf_buf) && (!prune->skip_bits || skip_buf);
+  if (ok) {
+    if (leaf_buf) gpu_buffer_write(leaf_buf, leaf_words, 24 * (U64)leaf_count);
+    if (skip_buf) gpu_buffer_write(skip_buf, prune->skip_bits, prune->skip_words * 4);
+    if (leaf_buf) gpu_kernel_set_arg_buffer(k, 0, leaf_buf);
+    else gpu_hip_kernel_clear_arg(k, 0);
+    gpu_kernel_set_arg_buffer(k, 1, acc);
+    if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
+    else gpu_hip_kernel_clear_arg(k, 5);
+    for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
+      if (s < slots.count) gpu_kernel_set_arg_buffer(k, 6 + s, t->columns[slots.column[s]].buffer);
+      else gpu_hip_kernel_clear_arg(k, 6 + s);
+    }
+    gpu_kernel_set_arg_u64(k, 0, r1 - r0);
+    gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
+    gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
+    gpu_kernel_set_arg_u64(k, 5, r0);
+    gpu_kernel_set_arg_u64(k, 7, leaf_count);
+    qe_kernel_set_word(k, 10, key_min_code);
+    qe_kernel_set_word(k, 11, range);
+    qe_kernel_set_word(k, 12, arg_count | (slot_bits << 8));
+    qe_kernel_set_word(k, 13, type_bits);
+    qe_kernel_set_word(k, 14, key_slot | (t->columns[key_col].type << 8) | (flags << 16));
+    gpu_hip_kernel_set_lds_bytes(k, (U32)lds);
+    GPU_Batch* b = gpu_batch_begin(0, gbytes);
+    /* count/sum/max start at 0, min at all ones: fills run from offset 0,
+     * so zero all, fill through the min array, zero count+sum again */
+    U64 zero_bytes = (U64)range * 8 * (1 + arg_count);
+    gpu_batch_buffer_zero(b, acc, gbytes);
+    if (arg_count && need_minmax) {
+      gpu_batch_buffer_fill(b, acc, zero_bytes + (U64)range * 4 * arg_count, 0xffffffffu);
+      gpu_batch_buffer_zero(b, acc, zero_bytes);
+    }
+    gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+    gpu_batch_buffer_read(b, acc, out->block, gbytes);
+    ok = gpu_batch_end(b);
+    gpu_hip_kernel_set_lds_bytes(k, 0);
+  }
+  qe_scratch_release(acc);
+  qe_scratch_release(leaf_buf);
+  qe_scratch_release(skip_buf);
+  if (!ok) {
+    free(out->block);
+    out->block = 0;
+    qe_set_error("aggregate_dense: %s", gpu_hip_last_error());
+    return 0;
+  }
+  if (trace) {
+    trace->kernel_count = 1;
+    trace->rows_scanned = r1 - r0;
+    U64 bytes = 0;
+    for (U32 s = 0; s < slots.count; s++) bytes += (r1 - r0) * t->columns[slots.column[s]].width;
+    trace->bytes_read = bytes;
+    snprintf(trace->kernel, sizeof(trace->kernel), "aggregate_dense");
+  }
+  return 1;
+}
+
+void qe_dense_totals_free(QE_DenseTotals* d)
+{
+  free(d->block);
+  d->block = 0;
+}
+
+/* ---------------- entry point ---------------- */
+B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 group_col, const U32* funcs,
+                 const S32* arg_cols, U32 expr_count, U64 cap, U64* out_group_count, F64* out_keys, U64* out_cells,
+                 QE_Trace* trace)
+{
+  U64 t0, p0;
+  qe_trace_begin(trace, &t0, &p0);
+  if (!table || !funcs || !arg_cols || !out_group_count || !out_cells) {
+    qe_set_error("qe_aggregate: null argument");
+    return 0;
+  }
+  QE_AggPlan plan;
+  if (!qe_agg_plan(table, funcs, arg_cols, expr_count, &plan)) return 0;
+  if (trace) trace->rows_before = table->row_count;
+
+  /* WHERE: numeric AND leaves fuse into the aggregate kernels */
+  QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
+  U32 leaf_count = 0;
+  if (where && where_count) {
+    static __thread QE_Bytecode prog;
+    if (!qe_bytecode_program_build(table, where, where_count, &prog)) {
+      qe_set_error("qe_aggregate: %s", prog.error);
+      return 0;
+    }
+    if (!qe_collect_and_leaves(where, where_count, leaves, &leaf_count, QE_SCAN_MAX_LEAVES)) {
+      qe_set_error("qe_aggregate: WHERE must be an AND of column-literal comparisons to fuse");
+      return 0;
+    }
+  }
+  QE_Leaf prune_leaves[32];
+  U32 prune_count = qe_collect_prune_leaves(where, where_count, prune_leaves, 32);
+  QE_Prune prune;
+  qe_zonemap_prune(table, prune_leaves, prune_count, &prune);
+
+  B32 ok = 1;
+  if (group_col < 0) {
+    if (cap < 1) {
+      *out_group_count = 1;
+      qe_prune_free(&prune);
+      return 1;
+    }
+    if (prune.zones_left == 0 || table->row_count == 0) {
+      for (U32 e = 0; e < expr_count; e++) out_cells[e] = qe_agg_cell(plan.funcs[e], 0, 0.0, 0.0, 0.0);
+      if (trace) trace->path = prune.zones_left == 0 ? QE_AggPath_FusedGlobal : QE_AggPath_Global;
+    } else {
+      ok = qe_aggregate_global(table, leaves, leaf_count, &prune, &plan, out_cells, trace);
+    }
+    if (ok) {
+      *out_group_count = 1;
+      if (out_keys) out_keys[0] = 0.0;
+    }
+    qe_prune_free(&prune);
+    qe_trace_end(trace, t0, p0);
+    return ok;
+  }
+
+  if ((U32)group_col >= table->column_count) {
+    qe_prune_free(&prune);
+    qe_set_error("qe_aggregate: unknown group column");
+    return 0;
+  }
+  QE_Column* key = &table->columns[group_col];
+  if (key->type != CQL_ColumnType_U32 && key->type != CQL_ColumnType_I32) {
+    qe_prune_free(&prune);
+    qe_set_error("qe_aggregate: GROUP BY key must be U32/I32 on this path");
+    return 0;
+  }
+  if (prune.zones_left == 0 || table->row_count == 0) {
+    qe_prune_free(&prune);
+    *out_group_count = 0;
+    qe_trace_end(trace, t0, p0);
+    return 1;
+  }
+  /* key range from the zone map (query_exec.c:7636-7647) */
+  F64 kmin = key->min_value, kmax = key->max_value;
+  F64 span = kmax - kmin + 1.0;
+  U64 max_range = qe_settings_u64("QE_AGG_DENSE_MAX_RANGE", 65536);
+  if (qe_settings_u64("QE_AGG_DENSE", 1) == 0 || span > (F64)max_range) {
+    qe_prune_free(&prune);
+    qe_set_error("qe_aggregate: key range %.0f above the dense limit (hash GROUP BY not built)", span);
+    return 0;
+  }
+  U32 range = (U32)span;
+  U32 key_min_code = key->type == CQL_ColumnType_I32 ? ((U32)(S32)kmin ^ 0x80000000u) : (U32)kmin;
+
+  /* args equal to the key fold into the count: SUM = key*count,
+   * MIN = MAX = key (exact for whole-number keys) */
+  B32 fold = qe_settings_u64("QE_AGG_KEY_FOLD", 1) != 0;
+  for (U32 a = 0; a < plan.arg_count && fold; a++)
+    if (plan.arg_col[a] != (U32)group_col) fold = 0;
+  QE_DenseTotals tot;
+  memset(&tot, 0, sizeof(tot));
+  if (fold) ok = qe_dense_run(table, (U32)group_col, 0, 0, 0, 0, leaves, leaf_count, &prune, key_min_code, range, &tot, trace);
+  else
+    ok = qe_dense_run(table, (U32)group_col, plan.arg_col, plan.arg_count, plan.need_sum, plan.need_minmax, leaves,
+                      leaf_count, &prune, key_min_code, range, &tot, trace);
+  qe_prune_free(&prune);
+  if (!ok) return 0;
+  /* groups in key order (query_exec.c:8006-8039) */
+  U64 g = 0;
+  for (U32 i = 0; i < range; i++) {
+    U64 cnt = tot.count[i];
+    if (!cnt) continue;
+    if (g < cap) {
+      U32 code = key_min_code + i;
+      F64 kv = key->type == CQL_ColumnType_I32 ? (F64)(S32)(code ^ 0x80000000u) : (F64)code;
+      if (out_keys) out_keys[g] = kv;
+      for (U32 e = 0; e < expr_count; e++) {
+        S32 a = plan.expr_arg[e];
+        F64 sum = 0.0, mn = 0.0, mx = 0.0;
+        if (a >= 0) {
+          if (fold) {
+            sum = kv * (F64)cnt;
+            mn = mx = kv;
+          } else {
+            U32 ty = table->columns[plan.arg_col[a]].type;
+            U64 s = tot.sum[(U64)a * range + i];
+            sum = ty == CQL_ColumnType_I32 ? (F64)(S64)s : (F64)s;
+            U32 lo = tot.mn[(U64)a * range + i], hi = tot.mx[(U64)a * range + i];
+            mn = ty == CQL_ColumnType_I32 ? (F64)(S32)(lo ^ 0x80000000u) : (F64)lo;
+            mx = ty == CQL_ColumnType_I32 ? (F64)(S32)(hi ^ 0x80000000u) : (F64)hi;
+          }
+        }
+        out_cells[g * expr_count + e] = qe_agg_cell(plan.funcs[e], cnt, sum, mn, mx);
+      }
+    }
+    g++;
+  }
+  qe_dense_totals_free(&tot);
+  *out_group_count = g;
+  if (trace) {
+    trace->path = leaf_count ? QE_AggPath_FusedDense : (fold ? QE_AggPath_DenseKeyFold : QE_AggPath_Dense);
+    trace->rows_after = g;
+  }
+  qe_trace_end(trace, t0, p0);
+  return 1;
+}

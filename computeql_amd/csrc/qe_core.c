@@ -1,0 +1,503 @@
+/*
+ * qe_core.c - settings, errors, backend access, device-resident tables,
+ * zone maps and the synthetic-table generators of the host operator layer.
+ *
+ * Reference: settings.c:158-242 (settings_u64), cql.h:206-322 (CQL_Column),
+ * cql.c:3818 (zone maps), gpu_dispatch.c:70-151 (backend selection).
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdlib.h>
+#include <time.h>
+
+#include "qe_internal.h"
+
+/* ------------------------------------------------------------------ */
+/* settings                                                             */
+/* ------------------------------------------------------------------ */
+#define QE_SETTINGS_CAP 128
+typedef struct {
+  char key[64];
+  U64 value;
+} QE_Setting;
+static QE_Setting g_settings[QE_SETTINGS_CAP];
+static U32 g_setting_count;
+static pthread_mutex_t g_settings_lock = PTHREAD_MUTEX_INITIALIZER;
+
+void qe_settings_set_u64(const char* key, U64 value)
+{
+  pthread_mutex_lock(&g_settings_lock);
+  for (U32 i = 0; i < g_setting_count; i++) {
+    if (strcmp(g_settings[i].key, key) == 0) {
+      g_settings[i].value = value;
+      pthread_mutex_unlock(&g_settings_lock);
+      return;
+    }
+  }
+  if (g_setting_count < QE_SETTINGS_CAP) {
+    snprintf(g_settings[g_setting_count].key, sizeof(g_settings[0].key), "%s", key);
+    g_settings[g_setting_count].value = value;
+    g_setting_count++;
+  }
+  pthread_mutex_unlock(&g_settings_lock);
+}
+
+U64 qe_settings_u64(const char* key, U64 default_value)
+{
+  U64 v = default_value;
+  pthread_mutex_lock(&g_settings_lock);
+  for (U32 i = 0; i < g_setting_count; i++) {
+    if (strcmp(g_settings[i].key, key) == 0) {
+      v = g_settings[i].value;
+      break;
+    }
+  }
+  pthread_mutex_unlock(&g_settings_lock);
+  return v;
+}
+
+void qe_settings_reset(void)
+{
+  pthread_mutex_lock(&g_settings_lock);
+  g_setting_count = 0;
+  pthread_mutex_unlock(&g_settings_lock);
+}
+
+/* ------------------------------------------------------------------ */
+/* errors and time                                                      */
+/* ------------------------------------------------------------------ */
+static __thread char t_error[512];
+
+void qe_set_error(const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(t_error, sizeof(t_error), fmt, ap);
+  va_end(ap);
+}
+
+const char* qe_last_error(void) { return t_error; }
+
+U64 qe_now_ns(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (U64)ts.tv_sec * 1000000000ull + (U64)ts.tv_nsec;
+}
+
+void qe_trace_begin(QE_Trace* t, U64* t0, U64* prof0)
+{
+  *t0 = qe_now_ns();
+  *prof0 = gpu_hip_profile_query(0, 0);
+  if (t) memset(t, 0, sizeof(*t));
+}
+
+void qe_trace_end(QE_Trace* t, U64 t0, U64 prof0)
+{
+  if (!t) return;
+  t->wall_ns = qe_now_ns() - t0;
+  U64 p = gpu_hip_profile_query(0, 0);
+  t->gpu_kernel_ns = p >= prof0 ? p - prof0 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* backend access                                                       */
+/* ------------------------------------------------------------------ */
+static GPU_Backend* g_active_backend;
+
+GPU_Backend* qe_gpu(void)
+{
+  if (!g_active_backend) {
+    g_active_backend = gpu_hip_backend();
+    g_active_backend->init();
+  }
+  return g_active_backend;
+}
+
+String8 qe_str8(const char* s)
+{
+  String8 r;
+  r.str = (U8*)s;
+  r.size = strlen(s);
+  return r;
+}
+
+GPU_Kernel* qe_kernel(const char* name)
+{
+  GPU_Kernel* k = gpu_kernel_alloc(qe_str8(name));
+  if (!k) qe_set_error("kernel %s unavailable: %s", name, gpu_hip_last_error());
+  return k;
+}
+
+void qe_kernel_set_word(GPU_Kernel* k, U32 word, U32 value) { gpu_hip_kernel_set_arg_u32(k, word, value); }
+
+GPU_Buffer* qe_scratch(const char* name, U64 bytes)
+{
+  char key[96];
+  snprintf(key, sizeof(key), "%s@%lu", name, (unsigned long)pthread_self());
+  return qe_gpu()->buffer_alloc_pooled(qe_str8(key), bytes, GPU_BufferFlag_ReadWrite, 0);
+}
+
+void qe_scratch_release(GPU_Buffer* b)
+{
+  if (b) gpu_buffer_release(b);
+}
+
+U32 qe_setting_blocks_per_cu(void) { return (U32)qe_settings_u64("QE_HIP_BLOCKS_PER_CU", 4); }
+
+U32 qe_stream_blocks(U64 quads)
+{
+  U64 cus = gpu_hip_compute_units();
+  if (cus == 0) cus = 1;
+  U64 cap = cus * qe_setting_blocks_per_cu();
+  U64 step = (U64)QE_WG * 4; /* quads per block step */
+  U64 need = (quads + step - 1) / step;
+  if (need < 1) need = 1;
+  return (U32)(need < cap ? need : cap);
+}
+
+/* ------------------------------------------------------------------ */
+/* tables                                                               */
+/* ------------------------------------------------------------------ */
+static U32 qe_type_width(U32 type)
+{
+  switch (type) {
+    case CQL_ColumnType_U32:
+    case CQL_ColumnType_I32:
+    case CQL_ColumnType_F32: return 4;
+    case CQL_ColumnType_U64:
+    case CQL_ColumnType_I64:
+    case CQL_ColumnType_F64: return 8;
+    default: return 0;
+  }
+}
+
+QE_Table* qe_table_alloc(U64 row_count)
+{
+  QE_Table* t = (QE_Table*)calloc(1, sizeof(QE_Table));
+  if (t) t->row_count = row_count;
+  return t;
+}
+
+void qe_table_release(QE_Table* table)
+{
+  if (!table) return;
+  for (U32 c = 0; c < table->column_count; c++) {
+    QE_Column* col = &table->columns[c];
+    if (col->buffer) gpu_buffer_release(col->buffer);
+    free(col->zone_min);
+    free(col->zone_max);
+  }
+  free(table);
+}
+
+U64 qe_table_row_count(QE_Table* table) { return table ? table->row_count : 0; }
+
+S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void* host_data, const U8* null_flags)
+{
+  if (!table || table->column_count >= QE_MAX_COLUMNS) {
+    qe_set_error("qe_table_add_column: table full");
+    return -1;
+  }
+  U32 width = qe_type_width(type);
+  if (!width) {
+    qe_set_error("qe_table_add_column: unsupported type %u", type);
+    return -1;
+  }
+  if (null_flags) {
+    for (U64 i = 0; i < table->row_count; i++) {
+      if (null_flags[i]) {
+        qe_set_error("qe_table_add_column: NULL values are not supported on this path");
+        return -1;
+      }
+    }
+  }
+  QE_Column* col = &table->columns[table->column_count];
+  memset(col, 0, sizeof(*col));
+  snprintf(col->name, sizeof(col->name), "%s", name ? name : "");
+  col->type = type;
+  col->width = width;
+  /* 64 bytes of slack past the last row keep every 16-byte quad load of
+   * the kernels inside the allocation */
+  U64 bytes = table->row_count * width + 64;
+  col->buffer = gpu_buffer_alloc(bytes, GPU_BufferFlag_ReadWrite | GPU_BufferFlag_DeviceLocal, 0);
+  if (!col->buffer) {
+    qe_set_error("qe_table_add_column: allocation of %llu bytes failed: %s", (unsigned long long)bytes,
+                 gpu_hip_last_error());
+    return -1;
+  }
+  if (host_data && table->row_count) gpu_buffer_write(col->buffer, (void*)host_data, table->row_count * width);
+  S32 index = (S32)table->column_count++;
+  if (host_data && !qe_table_build_zonemap(table, (U32)index)) return -1;
+  return index;
+}
+
+GPU_Buffer* qe_table_column_buffer(QE_Table* table, U32 column)
+{
+  if (!table || column >= table->column_count) return 0;
+  return table->columns[column].buffer;
+}
+
+B32 qe_table_column_read(QE_Table* table, U32 column, U64 row0, U64 rows, void* out)
+{
+  if (!table || column >= table->column_count || row0 + rows > table->row_count) {
+    qe_set_error("qe_table_column_read: bad range");
+    return 0;
+  }
+  QE_Column* col = &table->columns[column];
+  if (row0 == 0) {
+    gpu_buffer_read(col->buffer, out, rows * col->width);
+    return 1;
+  }
+  /* read the prefix then keep the tail (buffers are read from offset 0) */
+  U64 bytes = (row0 + rows) * col->width;
+  U8* tmp = (U8*)malloc(bytes ? bytes : 1);
+  if (!tmp) return 0;
+  gpu_buffer_read(col->buffer, tmp, bytes);
+  memcpy(out, tmp + row0 * col->width, rows * col->width);
+  free(tmp);
+  return 1;
+}
+
+B32 qe_table_build_zonemap(QE_Table* table, U32 column)
+{
+  if (!table || column >= table->column_count) return 0;
+  QE_Column* col = &table->columns[column];
+  U64 zones = (table->row_count + (1ull << QE_ZONE_SHIFT) - 1) >> QE_ZONE_SHIFT;
+  free(col->zone_min);
+  free(col->zone_max);
+  col->zone_min = col->zone_max = 0;
+  col->zone_count = zones;
+  col->zonemap_valid = 0;
+  col->min_value = 0.0;
+  col->max_value = 0.0;
+  if (zones == 0) {
+    col->zonemap_valid = 1;
+    return 1;
+  }
+  col->zone_min = (F64*)malloc(sizeof(F64) * zones);
+  col->zone_max = (F64*)malloc(sizeof(F64) * zones);
+  GPU_Buffer* bmin = qe_scratch("zonemap_min", zones * 8);
+  GPU_Buffer* bmax = qe_scratch("zonemap_max", zones * 8);
+  GPU_Kernel* k = qe_kernel("zonemap_build");
+  if (!bmin || !bmax || !k || !col->zone_min || !col->zone_max) {
+    qe_scratch_release(bmin);
+    qe_scratch_release(bmax);
+    qe_set_error("qe_table_build_zonemap: setup failed");
+    return 0;
+  }
+  gpu_kernel_set_arg_buffer(k, 6, col->buffer);
+  gpu_kernel_set_arg_buffer(k, 1, bmin);
+  gpu_kernel_set_arg_buffer(k, 2, bmax);
+  gpu_kernel_set_arg_u64(k, 0, table->row_count);
+  qe_kernel_set_word(k, 2, col->type);
+  qe_kernel_set_word(k, 3, QE_ZONE_SHIFT);
+  U64 blocks = zones < 65536 ? zones : 65536;
+  GPU_Batch* b = gpu_batch_begin(0, zones * 16);
+  gpu_batch_kernel_execute(b, k, (U32)(blocks * QE_WG), QE_WG);
+  gpu_batch_buffer_read(b, bmin, col->zone_min, zones * 8);
+  gpu_batch_buffer_read(b, bmax, col->zone_max, zones * 8);
+  B32 ok = gpu_batch_end(b);
+  qe_scratch_release(bmin);
+  qe_scratch_release(bmax);
+  if (!ok) {
+    qe_set_error("qe_table_build_zonemap: %s", gpu_hip_last_error());
+    return 0;
+  }
+  F64 mn = col->zone_min[0], mx = col->zone_max[0];
+  for (U64 z = 1; z < zones; z++) {
+    if (col->zone_min[z] < mn) mn = col->zone_min[z];
+    if (col->zone_max[z] > mx) mx = col->zone_max[z];
+  }
+  col->min_value = mn;
+  col->max_value = mx;
+  col->zonemap_valid = 1;
+  return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* generators                                                           */
+/* ------------------------------------------------------------------ */
+static B32 qe_run_gen(QE_Table* table, GPU_Kernel* k, U64 rows)
+{
+  U64 blocks = (rows + QE_WG - 1) / QE_WG;
+  U64 cap = (U64)gpu_hip_compute_units() * 16;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  GPU_Batch* b = gpu_batch_begin(0, 0);
+  gpu_batch_kernel_execute(b, k, (U32)(blocks * QE_WG), QE_WG);
+  if (!gpu_batch_end(b)) {
+    qe_set_error("generator failed: %s", gpu_hip_last_error());
+    return 0;
+  }
+  (void)table;
+  return 1;
+}
+
+B32 qe_gen_events(QE_Table* table, U64 seed, S32 id_col, S32 region_col, S32 amount_col, S32 ts_col)
+{
+  if (!table) return 0;
+  GPU_Kernel* k = qe_kernel("gen_events");
+  if (!k) return 0;
+  S32 cols[4] = {id_col, region_col, amount_col, ts_col};
+  for (U32 i = 0; i < 4; i++) {
+    if (cols[i] >= 0 && ((U32)cols[i] >= table->column_count || table->columns[cols[i]].type != CQL_ColumnType_U32)) {
+      qe_set_error("qe_gen_events: column %d must be a U32 column", cols[i]);
+      return 0;
+    }
+  }
+  for (U32 i = 0; i < 4; i++) {
+    if (cols[i] >= 0) gpu_kernel_set_arg_buffer(k, 6 + i, table->columns[cols[i]].buffer);
+    else gpu_hip_kernel_clear_arg(k, 6 + i); /* the kernel skips absent outputs */
+  }
+  gpu_kernel_set_arg_u64(k, 0, table->row_count);
+  gpu_kernel_set_arg_u64(k, 1, seed);
+  gpu_kernel_set_arg_u64(k, 2, 0);
+  if (!qe_run_gen(table, k, table->row_count)) return 0;
+  for (U32 i = 0; i < 4; i++)
+    if (cols[i] >= 0 && !qe_table_build_zonemap(table, (U32)cols[i])) return 0;
+  return 1;
+}
+
+B32 qe_gen_scale_events(QE_Table* table, S32 id_col, S32 region_col, S32 amount_col, S32 key_col)
+{
+  if (!table) return 0;
+  GPU_Kernel* k = qe_kernel("gen_scale_events");
+  if (!k) return 0;
+  S32 cols[4] = {id_col, region_col, amount_col, key_col};
+  for (U32 i = 0; i < 4; i++) {
+    if (cols[i] >= 0 && ((U32)cols[i] >= table->column_count || table->columns[cols[i]].type != CQL_ColumnType_U32)) {
+      qe_set_error("qe_gen_scale_events: column %d must be a U32 column", cols[i]);
+      return 0;
+    }
+  }
+  for (U32 i = 0; i < 4; i++) {
+    if (cols[i] >= 0) gpu_kernel_set_arg_buffer(k, 6 + i, table->columns[cols[i]].buffer);
+    else gpu_hip_kernel_clear_arg(k, 6 + i);
+  }
+  gpu_kernel_set_arg_u64(k, 0, table->row_count);
+  gpu_kernel_set_arg_u64(k, 2, 0);
+  if (!qe_run_gen(table, k, table->row_count)) return 0;
+  for (U32 i = 0; i < 4; i++)
+    if (cols[i] >= 0 && !qe_table_build_zonemap(table, (U32)cols[i])) return 0;
+  return 1;
+}
+
+/* dimension rows: keys are a Fisher-Yates permutation of 0..n-1 (draws of
+ * the splitmix64 stream seeded with seed ^ 0x2545F4914F6CDD1D), optionally
+ * spread over the u32 range by the bijection k -> k * 0x85EBCA77 +
+ * 0x165667B1; attr = d(j) % attr_range with seed ^ 0x14057B7EF767814F.
+ * Built on the host (n is small) and uploaded. */
+static U64 qe_splitmix64_at(U64 seed, U64 k)
+{
+  U64 z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+B32 qe_gen_dim(QE_Table* table, U64 seed, U32 attr_range, B32 spread_keys, S32 key_col, S32 attr_col)
+{
+  if (!table || key_col < 0 || (U32)key_col >= table->column_count || table->columns[key_col].type != CQL_ColumnType_U32 ||
+      (attr_col >= 0 && ((U32)attr_col >= table->column_count || table->columns[attr_col].type != CQL_ColumnType_U32)) ||
+      (attr_col >= 0 && attr_range == 0)) {
+    qe_set_error("qe_gen_dim: bad arguments");
+    return 0;
+  }
+  U64 n = table->row_count;
+  U32* keys = (U32*)malloc(sizeof(U32) * (n ? n : 1));
+  if (!keys) return 0;
+  for (U64 i = 0; i < n; i++) keys[i] = (U32)i;
+  U64 ps = seed ^ 0x2545F4914F6CDD1DULL;
+  for (U64 i = n; i > 1; i--) {
+    U64 j = qe_splitmix64_at(ps, i - 1) % i;
+    U32 tmp = keys[i - 1];
+    keys[i - 1] = keys[j];
+    keys[j] = tmp;
+  }
+  if (spread_keys)
+    for (U64 i = 0; i < n; i++) keys[i] = keys[i] * 0x85EBCA77u + 0x165667B1u;
+  gpu_buffer_write(table->columns[key_col].buffer, keys, n * 4);
+  if (attr_col >= 0) {
+    U64 as = seed ^ 0x14057B7EF767814FULL;
+    for (U64 i = 0; i < n; i++) keys[i] = (U32)(qe_splitmix64_at(as, i) % attr_range);
+    gpu_buffer_write(table->columns[attr_col].buffer, keys, n * 4);
+  }
+  free(keys);
+  if (!qe_table_build_zonemap(table, (U32)key_col)) return 0;
+  return attr_col < 0 || qe_table_build_zonemap(table, (U32)attr_col);
+}
+
+B32 qe_gen_fact_fk(QE_Table* table, U32 fk_col, U64 seed, QE_Table* dim, U32 dim_key_col)
+{
+  if (!table || !dim || fk_col >= table->column_count || dim_key_col >= dim->column_count ||
+      table->columns[fk_col].type != CQL_ColumnType_U32 || dim->columns[dim_key_col].type != CQL_ColumnType_U32 ||
+      dim->row_count == 0) {
+    qe_set_error("qe_gen_fact_fk: bad arguments");
+    return 0;
+  }
+  GPU_Kernel* k = qe_kernel("gen_fact_fk");
+  if (!k) return 0;
+  gpu_kernel_set_arg_buffer(k, 6, table->columns[fk_col].buffer);
+  gpu_kernel_set_arg_buffer(k, 0, dim->columns[dim_key_col].buffer);
+  gpu_kernel_set_arg_u64(k, 0, table->row_count);
+  gpu_kernel_set_arg_u64(k, 1, seed);
+  gpu_kernel_set_arg_u64(k, 2, 0);
+  gpu_kernel_set_arg_u64(k, 3, dim->row_count);
+  if (!qe_run_gen(table, k, table->row_count)) return 0;
+  return qe_table_build_zonemap(table, fk_col);
+}
+
+/* ------------------------------------------------------------------ */
+/* result typing: COUNT -> U64, SUM/AVG/MIN/MAX -> F64                  */
+/* (qe_agg_func_policy, query_exec.h:917-926; qe_aggregate_finish,      */
+/* query_exec.c:7269-7293)                                              */
+/* ------------------------------------------------------------------ */
+U64 qe_agg_cell(U32 func, U64 count, F64 sum, F64 mn, F64 mx)
+{
+  F64 v;
+  switch (func) {
+    case QE_AGG_FUNC_COUNT: return count;
+    case QE_AGG_FUNC_SUM: v = sum; break;
+    case QE_AGG_FUNC_AVG: v = count ? sum / (F64)count : 0.0; break;
+    case QE_AGG_FUNC_MIN: v = mn; break;
+    default: v = mx; break;
+  }
+  U64 bits;
+  memcpy(&bits, &v, 8);
+  return bits;
+}
+
+/* ------------------------------------------------------------------ */
+/* bandwidth probe                                                      */
+/* ------------------------------------------------------------------ */
+B32 qe_bandwidth_read(GPU_Buffer* buffer, U64 bytes, U32 blocks, U64* out_sink, QE_Trace* trace)
+{
+  U64 t0, p0;
+  qe_trace_begin(trace, &t0, &p0);
+  GPU_Kernel* k = qe_kernel("bandwidth_read");
+  GPU_Buffer* sink = qe_scratch("bw_sink", 64);
+  if (!k || !sink) return 0;
+  U64 units = bytes / 16;
+  if (!blocks) blocks = qe_stream_blocks(units);
+  gpu_kernel_set_arg_buffer(k, 0, buffer);
+  gpu_kernel_set_arg_buffer(k, 1, sink);
+  gpu_kernel_set_arg_u64(k, 0, units);
+  U64 s = 0;
+  GPU_Batch* b = gpu_batch_begin(0, 8);
+  gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+  gpu_batch_buffer_read(b, sink, &s, 8);
+  B32 ok = gpu_batch_end(b);
+  qe_scratch_release(sink);
+  if (out_sink) *out_sink = s;
+  if (trace) {
+    trace->kernel_count = 1;
+    trace->bytes_read = units * 16;
+    snprintf(trace->kernel, sizeof(trace->kernel), "bandwidth_read");
+  }
+  qe_trace_end(trace, t0, p0);
+  if (!ok) qe_set_error("bandwidth_read: %s", gpu_hip_last_error());
+  return ok;
+}

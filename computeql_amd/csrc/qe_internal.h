@@ -1,0 +1,160 @@
+/*
+ * qe_internal.h - shared declarations of the host operator layer (C99).
+ * The operators drive the GPU only through the GPU_Backend table, the way
+ * query_exec.c drives it through the gpu_* wrappers (gpu.h:95-159).
+ */
+#ifndef QE_INTERNAL_H
+#define QE_INTERNAL_H
+
+#include <stddef.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/cqlhip.h"
+
+#define QE_MAX_COLUMNS 64
+#define QE_ZONE_SHIFT 13 /* CQL_ZONEMAP_CHUNK_ROWS 8192, cql.h:70-71 */
+#define QE_SCAN_MAX_LEAVES 8
+#define QE_MAX_COLUMN_BINDINGS 10 /* query_exec.h:30 */
+#define QE_WG 256
+
+typedef struct QE_Column {
+  char name[64];
+  U32 type;
+  U32 width; /* bytes per row */
+  GPU_Buffer* buffer;
+  B32 has_nulls;
+  /* zone maps: min/max per 8192 rows, and of the whole column */
+  U64 zone_count;
+  F64* zone_min;
+  F64* zone_max;
+  B32 zonemap_valid;
+  F64 min_value;
+  F64 max_value;
+} QE_Column;
+
+struct QE_Table {
+  U64 row_count;
+  U32 column_count;
+  QE_Column columns[QE_MAX_COLUMNS];
+};
+
+/* ---- gpu_* wrappers over the active backend (gpu_dispatch.c:153-603) ---- */
+GPU_Backend* qe_gpu(void);
+#define gpu_buffer_alloc(s, f, d) (qe_gpu()->buffer_alloc((s), (f), (d)))
+#define gpu_buffer_release(b) (qe_gpu()->buffer_release((b)))
+#define gpu_buffer_read(b, d, s) (qe_gpu()->buffer_read((b), (d), (s)))
+#define gpu_buffer_write(b, d, s) (qe_gpu()->buffer_write((b), (d), (s)))
+#define gpu_kernel_alloc(n) (qe_gpu()->kernel_alloc((n)))
+#define gpu_kernel_alloc_from_spirv(n, c) (qe_gpu()->kernel_alloc_from_spirv((n), (c)))
+#define gpu_kernel_set_arg_buffer(k, i, b) (qe_gpu()->kernel_set_arg_buffer((k), (i), (b)))
+#define gpu_kernel_set_arg_u64(k, i, v) (qe_gpu()->kernel_set_arg_u64((k), (i), (v)))
+#define gpu_batch_begin(u, d) (qe_gpu()->batch_begin((u), (d)))
+#define gpu_batch_buffer_write(b, buf, d, s) (qe_gpu()->batch_buffer_write((b), (buf), (d), (s)))
+#define gpu_batch_buffer_zero(b, buf, s) (qe_gpu()->batch_buffer_zero((b), (buf), (s)))
+#define gpu_batch_buffer_fill(b, buf, s, v) (qe_gpu()->batch_buffer_fill((b), (buf), (s), (v)))
+#define gpu_batch_kernel_execute(b, k, g, l) (qe_gpu()->batch_kernel_execute((b), (k), (g), (l)))
+#define gpu_batch_buffer_read(b, buf, d, s) (qe_gpu()->batch_buffer_read((b), (buf), (d), (s)))
+#define gpu_batch_end(b) (qe_gpu()->batch_end((b)))
+
+String8 qe_str8(const char* s);
+GPU_Kernel* qe_kernel(const char* name);
+/* set push words directly (words[i]) */
+void qe_kernel_set_word(GPU_Kernel* k, U32 word, U32 value);
+/* pooled scratch buffer of at least `bytes`, by name (per thread) */
+GPU_Buffer* qe_scratch(const char* name, U64 bytes);
+void qe_scratch_release(GPU_Buffer* b);
+
+/* ---- errors / timing ---- */
+void qe_set_error(const char* fmt, ...);
+U64 qe_now_ns(void);
+void qe_trace_begin(QE_Trace* t, U64* t0, U64* prof0);
+void qe_trace_end(QE_Trace* t, U64 t0, U64 prof0);
+
+/* ---- grid sizing ---- */
+U32 qe_stream_blocks(U64 quads);      /* blocks for a streaming pass over `quads` quads */
+U32 qe_setting_blocks_per_cu(void);
+
+/* ---- WHERE compilation (query_exec.c:203-545, :1189-1456) ---- */
+typedef struct QE_ColumnSlots {
+  U32 count;
+  U32 column[QE_MAX_COLUMN_BINDINGS]; /* table column of slot i */
+} QE_ColumnSlots;
+
+U32 qe_slot_of(QE_ColumnSlots* slots, U32 column); /* ~0u when out of slots */
+
+typedef struct QE_Bytecode {
+  U32 words[4096];
+  U32 word_count;
+  F64 consts[256];
+  U32 const_count;
+  QE_ColumnSlots slots;
+  B32 overflow;
+  char error[128];
+} QE_Bytecode;
+
+B32 qe_bytecode_program_build(QE_Table* table, const QE_IRNode* nodes, U32 count, QE_Bytecode* out);
+U32 qe_ir_subtree_end(const QE_IRNode* nodes, U32 i);
+
+typedef struct QE_Leaf {
+  U32 column;
+  U32 op; /* 0 EQ, 1 LT, 2 LE, 3 GT, 4 GE (query_exec.c QE_CPU_OP_*) */
+  F64 target;
+} QE_Leaf;
+
+/* top-level AND of `column op literal` comparisons; 0 if the tree is not */
+B32 qe_collect_and_leaves(const QE_IRNode* nodes, U32 count, QE_Leaf* leaves, U32* leaf_count, U32 max_leaves);
+/* conjuncts usable for zone-map pruning (any top-level AND leaf) */
+U32 qe_collect_prune_leaves(const QE_IRNode* nodes, U32 count, QE_Leaf* leaves, U32 max_leaves);
+void qe_simple_leaf_words(U32* words, U32 slot, U32 type, U32 op, F64 target);
+B32 qe_scan_int_interval(U32 op, F64 target, F64 code_offset, F64 max_code, U32* out_lo, U32* out_span);
+
+/* zone-map pruning: dispatch range [row_begin,row_end) and a skip bitmap
+ * (one bit per zone over the whole table, set = skipped); returns the
+ * surviving zone count */
+typedef struct QE_Prune {
+  U64 row_begin, row_end;
+  U64 zones_total, zones_left;
+  U32* skip_bits; /* malloc'd, zones_total bits, or 0 when nothing skipped */
+  U64 skip_words;
+} QE_Prune;
+void qe_zonemap_prune(QE_Table* table, const QE_Leaf* leaves, U32 leaf_count, QE_Prune* out);
+void qe_prune_free(QE_Prune* p);
+
+/* ---- generated kernels (query_kernel.c) ---- */
+/* the WHERE tree as one HIP predicate kernel; returns 0 (and sets the error)
+ * when hiprtc is unavailable or compilation fails */
+GPU_Kernel* qk_scan_kernel(QE_Table* table, const QE_IRNode* nodes, U32 count, QE_ColumnSlots* slots,
+                           F64* consts, U32* const_count, U64* compile_ns);
+GPU_Kernel* qk_simple_kernel(QE_Table* table, const QE_Leaf* leaves, U32 leaf_count, QE_ColumnSlots* slots,
+                             U64* compile_ns);
+
+/* ---- aggregate result typing (query_exec.h:898-926, query_exec.c:7053) ---- */
+typedef struct QE_GroupTotals {
+  U64 count;
+  S64 isum[QE_AGG_MAX_EXPRS]; /* exact integer sums per distinct arg */
+  F64 fsum[QE_AGG_MAX_EXPRS];
+  F64 mn[QE_AGG_MAX_EXPRS];
+  F64 mx[QE_AGG_MAX_EXPRS];
+} QE_GroupTotals;
+
+U64 qe_agg_cell(U32 func, U64 count, F64 sum, F64 mn, F64 mx);
+
+/* per-group device totals of aggregate_dense / join_probe: count u64[R];
+ * sum u64[E][R]; min u32[E][R]; max u32[E][R] */
+typedef struct QE_DenseTotals {
+  U32 range, args;
+  U64* count;
+  U64* sum;
+  U32* mn;
+  U32* mx;
+  void* block;
+} QE_DenseTotals;
+
+B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B32 need_sum, B32 need_minmax,
+                 const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune, U32 key_min_code, U32 range,
+                 QE_DenseTotals* out, QE_Trace* trace);
+void qe_dense_totals_free(QE_DenseTotals* d);
+U64 qe_dense_totals_bytes(U32 range, U32 args);
+
+#endif

@@ -1,0 +1,408 @@
+/*
+ * qe_join.c - aggregate over fact JOIN dim, pushed down to the fact scan.
+ *
+ *   qe_aggregate_join          qe_aggregate_join_pushdown, query_exec.c:8442-8775
+ *   dense per-key reduction    qe_dense_stream_core :7632-7980 (small key range)
+ *   hash probe                 qe_join_probe_stream_core :8080-8422 with
+ *                              join_probe_native.comp
+ *
+ * The fact table is reduced on the GPU.  Three forms, chosen like the
+ * reference chooses (dense first, else probe):
+ *  - JoinDense: fact key range <= QE_AGG_DENSE_MAX_RANGE: totals per key
+ *    value, then one host pass over the (filtered) dim rows adds each dim
+ *    row's key totals into its group (query_exec.c:8621-8719);
+ *  - JoinProbeGroup: the hash table's payload is the dim row's group code,
+ *    so the probe aggregates straight into LDS-private group tables and no
+ *    per-dim-row totals exist (MI355X form: one HBM pass, no scattered
+ *    global atomics);
+ *  - JoinProbeRow: payload = dim row, totals per dim row in HBM with global
+ *    atomics, then the host combine (the reference's form).
+ * Sums are exact 64-bit integers per key/group, converted to F64 and added
+ * in F64 per dim row as the reference's combine does.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdlib.h>
+
+#include "qe_internal.h"
+
+
+
+typedef struct QE_GroupAcc {
+  U32 value;
+  U64 count;
+  F64 sum[QE_AGG_MAX_EXPRS];
+  F64 mn[QE_AGG_MAX_EXPRS];
+  F64 mx[QE_AGG_MAX_EXPRS];
+  B32 seen;
+} QE_GroupAcc;
+
+static int qe_cmp_group(const void* a, const void* b)
+{
+  U32 x = ((const QE_GroupAcc*)a)->value, y = ((const QE_GroupAcc*)b)->value;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* find-or-add group by value (open addressing over a pow2 table) */
+typedef struct QE_GroupMap {
+  U32* slots; /* group index + 1 */
+  U64 mask;
+  QE_GroupAcc* groups;
+  U64 count;
+} QE_GroupMap;
+
+static B32 qe_group_map_init(QE_GroupMap* m, U64 max_groups)
+{
+  U64 size = 64;
+  while (size < max_groups * 2 + 2) size <<= 1;
+  m->slots = (U32*)calloc(size, sizeof(U32));
+  m->groups = (QE_GroupAcc*)calloc(max_groups ? max_groups : 1, sizeof(QE_GroupAcc));
+  m->mask = size - 1;
+  m->count = 0;
+  return m->slots && m->groups;
+}
+
+static QE_GroupAcc* qe_group_get(QE_GroupMap* m, U32 value)
+{
+  U64 h = ((U64)value * 0x9E3779B97F4A7C15ull) >> 29;
+  for (U64 s = h & m->mask;; s = (s + 1) & m->mask) {
+    U32 g = m->slots[s];
+    if (!g) {
+      QE_GroupAcc* a = &m->groups[m->count++];
+      a->value = value;
+      m->slots[s] = (U32)m->count;
+      return a;
+    }
+    if (m->groups[g - 1].value == value) return &m->groups[g - 1];
+  }
+}
+
+static void qe_group_map_free(QE_GroupMap* m)
+{
+  free(m->slots);
+  free(m->groups);
+}
+
+/* add `matches` fact rows with the given key/row totals into a group,
+ * query_exec.c:8693-8718 */
+static void qe_group_add(QE_GroupAcc* g, const U32* funcs, const S32* expr_arg, U32 expr_count, const U32* arg_types,
+                         U64 matches, const U64* sums, const U32* mins, const U32* maxs, U64 stride, U64 index)
+{
+  g->count += matches;
+  for (U32 e = 0; e < expr_count; e++) {
+    S32 a = expr_arg[e];
+    if (funcs[e] == QE_AGG_FUNC_COUNT || a < 0) continue;
+    U32 ty = arg_types[a];
+    U64 s = sums ? sums[(U64)a * stride + index] : 0;
+    F64 sv = ty == CQL_ColumnType_I32 ? (F64)(S64)s : (F64)s;
+    U32 lo = mins ? mins[(U64)a * stride + index] : 0, hi = maxs ? maxs[(U64)a * stride + index] : 0;
+    F64 mn = ty == CQL_ColumnType_I32 ? (F64)(S32)(lo ^ 0x80000000u) : (F64)lo;
+    F64 mx = ty == CQL_ColumnType_I32 ? (F64)(S32)(hi ^ 0x80000000u) : (F64)hi;
+    g->sum[e] += sv;
+    g->mn[e] = g->seen ? fmin(g->mn[e], mn) : mn;
+    g->mx[e] = g->seen ? fmax(g->mx[e], mx) : mx;
+  }
+  g->seen = 1;
+}
+
+static B32 qe_dim_u32(QE_Table* dim, U32 col, U32* out)
+{
+  if (dim->columns[col].type != CQL_ColumnType_U32 && dim->columns[col].type != CQL_ColumnType_I32) {
+    qe_set_error("join: dim key/group columns must be U32/I32");
+    return 0;
+  }
+  return qe_table_column_read(dim, col, 0, dim->row_count, out);
+}
+
+B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_key_col, const QE_IRNode* dim_where,
+                      U32 dim_where_count, S32 dim_group_col, const U32* funcs, const S32* fact_arg_cols, U32 expr_count,
+                      U64 cap, U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace)
+{
+  U64 t0, p0;
+  qe_trace_begin(trace, &t0, &p0);
+  if (!fact || !dim || !funcs || !fact_arg_cols || !out_group_count || !out_cells || fact_key_col >= fact->column_count ||
+      dim_key_col >= dim->column_count || (dim_group_col >= 0 && (U32)dim_group_col >= dim->column_count) ||
+      expr_count == 0 || expr_count > QE_AGG_MAX_EXPRS) {
+    qe_set_error("qe_aggregate_join: bad arguments");
+    return 0;
+  }
+  if (qe_settings_u64("QE_JOIN_PUSHDOWN", 1) == 0) {
+    qe_set_error("qe_aggregate_join: QE_JOIN_PUSHDOWN is off");
+    return 0;
+  }
+  QE_Column* fkey = &fact->columns[fact_key_col];
+  if (fkey->type != CQL_ColumnType_U32 && fkey->type != CQL_ColumnType_I32) {
+    qe_set_error("qe_aggregate_join: fact key must be U32/I32");
+    return 0;
+  }
+  /* fact-side aggregate arguments (distinct columns, <= 4) */
+  U32 arg_col[4], arg_type[4], arg_count = 0;
+  S32 expr_arg[QE_AGG_MAX_EXPRS];
+  B32 need_sum = 0, need_minmax = 0;
+  for (U32 e = 0; e < expr_count; e++) {
+    expr_arg[e] = -1;
+    if (funcs[e] > QE_AGG_FUNC_MAX) {
+      qe_set_error("qe_aggregate_join: bad function");
+      return 0;
+    }
+    if (fact_arg_cols[e] < 0 || funcs[e] == QE_AGG_FUNC_COUNT) {
+      if (fact_arg_cols[e] < 0 && funcs[e] != QE_AGG_FUNC_COUNT) {
+        qe_set_error("qe_aggregate_join: function needs an argument");
+        return 0;
+      }
+      continue;
+    }
+    U32 c = (U32)fact_arg_cols[e];
+    if (c >= fact->column_count ||
+        (fact->columns[c].type != CQL_ColumnType_U32 && fact->columns[c].type != CQL_ColumnType_I32)) {
+      qe_set_error("qe_aggregate_join: fact arguments must be U32/I32 columns");
+      return 0;
+    }
+    U32 a = 0;
+    while (a < arg_count && arg_col[a] != c) a++;
+    if (a == arg_count) {
+      if (arg_count == 4) {
+        qe_set_error("qe_aggregate_join: more than 4 argument columns");
+        return 0;
+      }
+      arg_col[arg_count] = c;
+      arg_type[arg_count] = fact->columns[c].type;
+      arg_count++;
+    }
+    expr_arg[e] = (S32)a;
+    if (funcs[e] == QE_AGG_FUNC_SUM || funcs[e] == QE_AGG_FUNC_AVG) need_sum = 1;
+    if (funcs[e] == QE_AGG_FUNC_MIN || funcs[e] == QE_AGG_FUNC_MAX) need_minmax = 1;
+  }
+  if (trace) trace->rows_before = fact->row_count;
+
+  /* dim side: rows passing the dim WHERE (on the GPU), keys and groups */
+  U64 dn = dim->row_count;
+  U32* dkeys = (U32*)malloc(sizeof(U32) * (dn ? dn : 1));
+  U32* dgroups = dim_group_col >= 0 ? (U32*)malloc(sizeof(U32) * (dn ? dn : 1)) : 0;
+  U64* pass = (U64*)malloc(sizeof(U64) * (dn ? dn : 1));
+  U64 pass_count = 0;
+  B32 ok = dkeys && pass && (dim_group_col < 0 || dgroups);
+  if (ok) ok = qe_dim_u32(dim, dim_key_col, dkeys);
+  if (ok && dgroups) ok = qe_dim_u32(dim, (U32)dim_group_col, dgroups);
+  if (ok) {
+    if (dim_where && dim_where_count) ok = qe_scan_filter(dim, dim_where, dim_where_count, 0, pass, dn, &pass_count, 0);
+    else {
+      for (U64 i = 0; i < dn; i++) pass[i] = i;
+      pass_count = dn;
+    }
+  }
+  if (!ok) {
+    free(dkeys);
+    free(dgroups);
+    free(pass);
+    return 0;
+  }
+  B32 key_i32 = fkey->type == CQL_ColumnType_I32;
+  B32 group_i32 = dim_group_col >= 0 && dim->columns[dim_group_col].type == CQL_ColumnType_I32;
+
+  QE_GroupMap groups;
+  memset(&groups, 0, sizeof(groups));
+  ok = qe_group_map_init(&groups, pass_count + 1);
+  U32 path = 0;
+  QE_Prune all;
+  memset(&all, 0, sizeof(all));
+  all.row_begin = 0;
+  all.row_end = fact->row_count;
+  all.zones_total = all.zones_left = 1;
+
+  F64 kspan = fkey->max_value - fkey->min_value + 1.0;
+  B32 dense = ok && fact->row_count > 0 && qe_settings_u64("QE_JOIN_DENSE", 1) != 0 &&
+              kspan <= (F64)qe_settings_u64("QE_AGG_DENSE_MAX_RANGE", 65536);
+  if (ok && fact->row_count == 0) {
+    /* no fact rows: no groups */
+  } else if (dense) {
+    path = QE_AggPath_JoinDense;
+    U32 range = (U32)kspan;
+    U32 kmin_code = key_i32 ? ((U32)(S32)fkey->min_value ^ 0x80000000u) : (U32)fkey->min_value;
+    QE_DenseTotals tot;
+    memset(&tot, 0, sizeof(tot));
+    ok = qe_dense_run(fact, fact_key_col, arg_col, arg_count, need_sum, need_minmax, 0, 0, &all, kmin_code, range, &tot,
+                      trace);
+    if (ok) {
+      for (U64 v = 0; v < pass_count; v++) {
+        U64 d = pass[v];
+        U32 code = key_i32 ? (dkeys[d] ^ 0x80000000u) : dkeys[d];
+        U32 kid = code - kmin_code;
+        if (code < kmin_code || kid >= range || tot.count[kid] == 0) continue;
+        QE_GroupAcc* g = qe_group_get(&groups, dgroups ? dgroups[d] : 0u);
+        qe_group_add(g, funcs, expr_arg, expr_count, arg_type, tot.count[kid], tot.sum, tot.mn, tot.mx, range, kid);
+      }
+      qe_dense_totals_free(&tot);
+    }
+  } else if (ok) {
+    if (qe_settings_u64("QE_JOIN_PROBE", 1) == 0) {
+      ok = 0;
+      qe_set_error("qe_aggregate_join: key range too wide for the dense form and QE_JOIN_PROBE is off");
+    }
+    /* group codes of the passing dim rows */
+    U32 gmin = 0, gmax = 0;
+    if (ok && dgroups && pass_count) {
+      gmin = gmax = group_i32 ? dgroups[pass[0]] ^ 0x80000000u : dgroups[pass[0]];
+      for (U64 v = 1; v < pass_count; v++) {
+        U32 c = group_i32 ? dgroups[pass[v]] ^ 0x80000000u : dgroups[pass[v]];
+        if (c < gmin) gmin = c;
+        if (c > gmax) gmax = c;
+      }
+    }
+    U64 grange = dgroups ? (U64)gmax - gmin + 1 : 1;
+    U64 lds_need = grange * 4 + (need_sum ? grange * 8 * arg_count : 0) + (need_minmax ? grange * 8 * arg_count : 0);
+    B32 group_mode = qe_settings_u64("QE_JOIN_GROUP_PAYLOAD", 1) != 0 && grange <= qe_settings_u64("QE_JOIN_GROUP_MAX", 8192) &&
+                     lds_need <= gpu_hip_max_lds_bytes();
+    path = group_mode ? QE_AggPath_JoinProbeGroup : QE_AggPath_JoinProbeRow;
+    U64 R = group_mode ? grange : dn;
+    U64 slots_n = 64;
+    while (slots_n < pass_count * 2) slots_n <<= 1;
+    U32 lg = 0;
+    while ((1ull << lg) < slots_n) lg++;
+    U32* payload = (U32*)malloc(sizeof(U32) * (dn ? dn : 1));
+    U32* list = (U32*)malloc(sizeof(U32) * (pass_count ? pass_count : 1));
+    if (!payload || !list) ok = 0;
+    for (U64 v = 0; ok && v < pass_count; v++) {
+      U64 d = pass[v];
+      list[v] = (U32)d;
+      if (group_mode) payload[d] = dgroups ? ((group_i32 ? dgroups[d] ^ 0x80000000u : dgroups[d]) - gmin) : 0u;
+      else payload[d] = (U32)d;
+    }
+    GPU_Kernel* kb = ok ? qe_kernel("join_build") : 0;
+    GPU_Kernel* kp = ok ? qe_kernel("join_probe") : 0;
+    U64 acc_bytes = qe_dense_totals_bytes((U32)R, arg_count);
+    GPU_Buffer* table = qe_scratch("join_table", slots_n * 8);
+    GPU_Buffer* flags = qe_scratch("join_flags", 64);
+    GPU_Buffer* pay_buf = qe_scratch("join_payload", (dn ? dn : 1) * 4);
+    GPU_Buffer* list_buf = qe_scratch("join_list", (pass_count ? pass_count : 1) * 4);
+    GPU_Buffer* acc = qe_scratch("join_totals", acc_bytes);
+    void* host_acc = malloc(acc_bytes ? acc_bytes : 8);
+    ok = ok && kb && kp && table && flags && pay_buf && list_buf && acc && host_acc;
+    U32 fl[2] = {0, 0};
+    if (ok) {
+      gpu_buffer_write(pay_buf, payload, dn * 4);
+      if (pass_count) gpu_buffer_write(list_buf, list, pass_count * 4);
+      gpu_kernel_set_arg_buffer(kb, 0, dim->columns[dim_key_col].buffer);
+      gpu_kernel_set_arg_buffer(kb, 1, pay_buf);
+      gpu_kernel_set_arg_buffer(kb, 2, table);
+      gpu_kernel_set_arg_buffer(kb, 3, flags);
+      gpu_kernel_set_arg_buffer(kb, 4, list_buf);
+      gpu_kernel_set_arg_u64(kb, 0, pass_count);
+      qe_kernel_set_word(kb, 2, lg);
+      QE_ColumnSlots slots;
+      memset(&slots, 0, sizeof(slots));
+      qe_slot_of(&slots, fact_key_col); /* slot 0: the fact key */
+      U32 slot_bits = 0, type_bits = 0;
+      for (U32 a = 0; a < arg_count; a++) {
+        slot_bits |= qe_slot_of(&slots, arg_col[a]) << (4 * a);
+        type_bits |= arg_type[a] << (8 * a);
+      }
+      for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
+        if (s < slots.count) gpu_kernel_set_arg_buffer(kp, 6 + s, fact->columns[slots.column[s]].buffer);
+        else gpu_hip_kernel_clear_arg(kp, 6 + s);
+      }
+      U32 kflags = (need_sum ? 1u : 0u) | (need_minmax ? 2u : 0u) | (group_mode ? 0u : 4u);
+      gpu_kernel_set_arg_buffer(kp, 1, acc);
+      gpu_kernel_set_arg_buffer(kp, 2, table);
+      gpu_hip_kernel_clear_arg(kp, 5);
+      gpu_kernel_set_arg_u64(kp, 0, fact->row_count);
+      gpu_kernel_set_arg_u64(kp, 4, 0);
+      gpu_kernel_set_arg_u64(kp, 5, 0);
+      qe_kernel_set_word(kp, 2, lg);
+      qe_kernel_set_word(kp, 11, (U32)R);
+      qe_kernel_set_word(kp, 12, arg_count | (slot_bits << 8));
+      qe_kernel_set_word(kp, 13, type_bits);
+      qe_kernel_set_word(kp, 14, kflags << 16);
+      U32 lds = group_mode ? (U32)lds_need : 0u;
+      gpu_hip_kernel_set_lds_bytes(kp, lds);
+      U64 quads = (fact->row_count + 3) / 4;
+      U32 blocks = qe_stream_blocks(quads);
+      U64 build_threads = pass_count < 256 ? 256 : pass_count;
+      if (build_threads > (U64)gpu_hip_compute_units() * 4 * 256) build_threads = (U64)gpu_hip_compute_units() * 4 * 256;
+      U64 zero_bytes = R * 8 * (1 + arg_count);
+      GPU_Batch* b = gpu_batch_begin(0, acc_bytes + 8);
+      gpu_batch_buffer_zero(b, table, slots_n * 8);
+      gpu_batch_buffer_zero(b, flags, 8);
+      gpu_batch_buffer_zero(b, acc, acc_bytes);
+      if (arg_count && need_minmax) {
+        gpu_batch_buffer_fill(b, acc, zero_bytes + R * 4 * arg_count, 0xffffffffu);
+        gpu_batch_buffer_zero(b, acc, zero_bytes);
+      }
+      gpu_batch_kernel_execute(b, kb, (U32)build_threads, QE_WG);
+      gpu_batch_kernel_execute(b, kp, blocks * QE_WG, QE_WG);
+      gpu_batch_buffer_read(b, flags, fl, 8);
+      gpu_batch_buffer_read(b, acc, host_acc, acc_bytes);
+      ok = gpu_batch_end(b);
+      gpu_hip_kernel_set_lds_bytes(kp, 0);
+      if (!ok) qe_set_error("join probe: %s", gpu_hip_last_error());
+      if (ok && fl[0]) {
+        ok = 0;
+        qe_set_error("qe_aggregate_join: duplicate dim keys (the probe form needs unique keys)");
+      }
+      if (ok && fl[1]) {
+        ok = 0;
+        qe_set_error("qe_aggregate_join: hash table full");
+      }
+      if (trace) {
+        trace->kernel_count = 2;
+        trace->rows_scanned = fact->row_count;
+        trace->bytes_read = fact->row_count * 4 * (U64)slots.count;
+        snprintf(trace->kernel, sizeof(trace->kernel), "join_probe");
+      }
+    }
+    if (ok) {
+      U64* cnt = (U64*)host_acc;
+      U64* sums = cnt + R;
+      U32* mins = (U32*)(sums + R * arg_count);
+      U32* maxs = mins + R * arg_count;
+      if (group_mode) {
+        for (U64 gi = 0; gi < R; gi++) {
+          if (!cnt[gi]) continue;
+          U32 code = gmin + (U32)gi;
+          U32 value = group_i32 ? code ^ 0x80000000u : code;
+          QE_GroupAcc* g = qe_group_get(&groups, dgroups ? value : 0u);
+          qe_group_add(g, funcs, expr_arg, expr_count, arg_type, cnt[gi], sums, mins, maxs, R, gi);
+        }
+      } else {
+        for (U64 v = 0; v < pass_count; v++) {
+          U64 d = pass[v];
+          if (!cnt[d]) continue;
+          QE_GroupAcc* g = qe_group_get(&groups, dgroups ? dgroups[d] : 0u);
+          qe_group_add(g, funcs, expr_arg, expr_count, arg_type, cnt[d], sums, mins, maxs, R, d);
+        }
+      }
+    }
+    free(payload);
+    free(list);
+    free(host_acc);
+    qe_scratch_release(table);
+    qe_scratch_release(flags);
+    qe_scratch_release(pay_buf);
+    qe_scratch_release(list_buf);
+    qe_scratch_release(acc);
+  }
+  if (ok) {
+    /* ascending group value (signed order for I32 groups) */
+    if (group_i32)
+      for (U64 g = 0; g < groups.count; g++) groups.groups[g].value ^= 0x80000000u;
+    qsort(groups.groups, groups.count, sizeof(QE_GroupAcc), qe_cmp_group);
+    if (group_i32)
+      for (U64 g = 0; g < groups.count; g++) groups.groups[g].value ^= 0x80000000u;
+    for (U64 g = 0; g < groups.count && g < cap; g++) {
+      QE_GroupAcc* a = &groups.groups[g];
+      if (out_keys) out_keys[g] = group_i32 ? (F64)(S32)a->value : (F64)a->value;
+      for (U32 e = 0; e < expr_count; e++) out_cells[g * expr_count + e] = qe_agg_cell(funcs[e], a->count, a->sum[e], a->mn[e], a->mx[e]);
+    }
+    *out_group_count = groups.count;
+    if (trace) {
+      trace->path = path;
+      trace->rows_after = groups.count;
+    }
+  }
+  qe_group_map_free(&groups);
+  free(dkeys);
+  free(dgroups);
+  free(pass);
+  qe_trace_end(trace, t0, p0);
+  return ok;
+}

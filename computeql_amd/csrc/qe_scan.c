@@ -1,0 +1,589 @@
+/*
+ * qe_scan.c - WHERE compilation and the GPU scan driver.
+ *
+ *   qe_bytecode_program_build   query_exec.c:524-545 (+ qe_compile_condition :203-522)
+ *   qe_collect_and_leaves       qe_cpu_scan_collect_leaves :1189-1235
+ *   qe_scan_int_interval        :1378-1404
+ *   qe_simple_leaf_words        :1442-1456 (+ qe_scan_int_leaf_words :1408-1438)
+ *   qe_zonemap_prune            qe_scan_build_dispatch_ranges :13307-13545,
+ *                               qe_zonemap_chunk_is_prunable :13294-13304
+ *   qe_scan_filter              qe_scan_filter_selected :2495-3273
+ *
+ * The columns are resident in HBM whole, so the 16M-row chunk grid and the
+ * slot pipeline of qe_scan_filter_streamed (:2242-2491) reduce to one
+ * dispatch over the zone-map dispatch range.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdlib.h>
+
+#include "qe_internal.h"
+
+/* QE_Opcode, query_exec.h:58-82 */
+enum {
+  QE_Opcode_PushTrue = 0,
+  QE_Opcode_LoadNumCol = 1,
+  QE_Opcode_PushConst = 2,
+  QE_Opcode_CmpEq = 3,
+  QE_Opcode_CmpNe = 4,
+  QE_Opcode_CmpLt = 5,
+  QE_Opcode_CmpGt = 6,
+  QE_Opcode_CmpLe = 7,
+  QE_Opcode_CmpGe = 8,
+  QE_Opcode_And = 9,
+  QE_Opcode_Or = 10,
+  QE_Opcode_Halt = 13,
+  QE_Opcode_PushFalse = 14,
+};
+
+U32 qe_ir_subtree_end(const QE_IRNode* nodes, U32 i)
+{
+  U32 end = i + 1;
+  for (U32 c = 0; c < nodes[i].child_count; c++) end = qe_ir_subtree_end(nodes, end);
+  return end;
+}
+
+U32 qe_slot_of(QE_ColumnSlots* slots, U32 column)
+{
+  for (U32 i = 0; i < slots->count; i++)
+    if (slots->column[i] == column) return i;
+  if (slots->count >= QE_MAX_COLUMN_BINDINGS) return ~0u;
+  slots->column[slots->count] = column;
+  return slots->count++;
+}
+
+static void qe_emit(QE_Bytecode* p, U32 w)
+{
+  if (p->word_count >= sizeof(p->words) / sizeof(p->words[0])) {
+    p->overflow = 1;
+    return;
+  }
+  p->words[p->word_count++] = w;
+}
+
+static U32 qe_add_const(QE_Bytecode* p, F64 v)
+{
+  if (p->const_count >= sizeof(p->consts) / sizeof(p->consts[0])) {
+    p->overflow = 1;
+    return 0;
+  }
+  p->consts[p->const_count] = v;
+  return p->const_count++;
+}
+
+static U32 qe_cmp_opcode(U32 irop)
+{
+  switch (irop) {
+    case QE_IROP_EQ: return QE_Opcode_CmpEq;
+    case QE_IROP_NE: return QE_Opcode_CmpNe;
+    case QE_IROP_LT: return QE_Opcode_CmpLt;
+    case QE_IROP_GT: return QE_Opcode_CmpGt;
+    case QE_IROP_LE: return QE_Opcode_CmpLe;
+    case QE_IROP_GE: return QE_Opcode_CmpGe;
+    default: return ~0u;
+  }
+}
+
+/* qe_compile_load_value, query_exec.c:180-200 */
+static B32 qe_compile_load_value(QE_Table* t, const QE_IRNode* n, QE_Bytecode* p)
+{
+  if (n->type == QE_IR_COLUMN) {
+    if (n->column >= t->column_count) {
+      snprintf(p->error, sizeof(p->error), "unknown column %u", n->column);
+      return 0;
+    }
+    U32 slot = qe_slot_of(&p->slots, n->column);
+    if (slot == ~0u) {
+      snprintf(p->error, sizeof(p->error), "more than %d columns in WHERE", QE_MAX_COLUMN_BINDINGS);
+      return 0;
+    }
+    qe_emit(p, QE_Opcode_LoadNumCol);
+    qe_emit(p, (t->columns[n->column].type << 8) | (slot & 0xffu));
+    return 1;
+  }
+  if (n->type == QE_IR_NUMERIC) {
+    U32 ci = qe_add_const(p, n->value);
+    qe_emit(p, QE_Opcode_PushConst);
+    qe_emit(p, ci);
+    return 1;
+  }
+  snprintf(p->error, sizeof(p->error), "unsupported operand node type %u", n->type);
+  return 0;
+}
+
+/* qe_compile_condition, query_exec.c:203-522 (numeric subset) */
+static B32 qe_compile_condition(QE_Table* t, const QE_IRNode* nodes, U32 i, QE_Bytecode* p)
+{
+  const QE_IRNode* n = &nodes[i];
+  if (n->type != QE_IR_OPERATOR) return qe_compile_load_value(t, n, p); /* bare value: nonzero = true */
+  U32 left = i + 1;
+  U32 right = n->child_count > 1 ? qe_ir_subtree_end(nodes, left) : 0;
+  switch (n->op) {
+    case QE_IROP_AND:
+    case QE_IROP_OR:
+      if (n->child_count != 2) break;
+      if (!qe_compile_condition(t, nodes, left, p) || !qe_compile_condition(t, nodes, right, p)) return 0;
+      qe_emit(p, n->op == QE_IROP_AND ? QE_Opcode_And : QE_Opcode_Or);
+      return 1;
+    case QE_IROP_IS_NULL:
+    case QE_IROP_IS_NOT_NULL:
+      /* :233-249: columns on this path hold no NULLs */
+      if (n->op == QE_IROP_IS_NOT_NULL) qe_emit(p, QE_Opcode_PushTrue);
+      else {
+        qe_emit(p, QE_Opcode_PushConst);
+        qe_emit(p, qe_add_const(p, 0.0));
+      }
+      return 1;
+    case QE_IROP_IN:
+    case QE_IROP_NOT_IN: {
+      /* :251-305: an OR chain of '=' (AND of '!=' for NOT IN), at most
+       * QE_IN_LIST_GPU_MAX_ITEMS items */
+      B32 is_not = n->op == QE_IROP_NOT_IN;
+      if (n->child_count != 2 || nodes[left].type != QE_IR_COLUMN || nodes[right].type != QE_IR_INLIST) {
+        snprintf(p->error, sizeof(p->error), "IN needs a column and a value list");
+        return 0;
+      }
+      U32 items = nodes[right].child_count;
+      if (items == 0) {
+        qe_emit(p, is_not ? QE_Opcode_PushTrue : QE_Opcode_PushFalse);
+        return 1;
+      }
+      if (items > qe_settings_u64("QE_IN_LIST_GPU_MAX_ITEMS", 32)) {
+        snprintf(p->error, sizeof(p->error), "IN list too long for GPU bytecode");
+        return 0;
+      }
+      for (U32 k = 0; k < items; k++) {
+        if (!qe_compile_load_value(t, &nodes[left], p)) return 0;
+        if (!qe_compile_load_value(t, &nodes[right + 1 + k], p)) return 0;
+        qe_emit(p, is_not ? QE_Opcode_CmpNe : QE_Opcode_CmpEq);
+        if (k > 0) qe_emit(p, is_not ? QE_Opcode_And : QE_Opcode_Or);
+      }
+      return 1;
+    }
+    default: {
+      U32 opc = qe_cmp_opcode(n->op);
+      if (opc == ~0u || n->child_count != 2) break;
+      if (!qe_compile_load_value(t, &nodes[left], p) || !qe_compile_load_value(t, &nodes[right], p)) return 0;
+      qe_emit(p, opc);
+      return 1;
+    }
+  }
+  snprintf(p->error, sizeof(p->error), "unsupported operator %u", n->op);
+  return 0;
+}
+
+B32 qe_bytecode_program_build(QE_Table* table, const QE_IRNode* nodes, U32 count, QE_Bytecode* out)
+{
+  memset(out, 0, sizeof(*out));
+  if (!nodes || count == 0) {
+    qe_emit(out, QE_Opcode_PushTrue);
+  } else {
+    if (qe_ir_subtree_end(nodes, 0) != count) {
+      snprintf(out->error, sizeof(out->error), "malformed WHERE tree");
+      return 0;
+    }
+    if (!qe_compile_condition(table, nodes, 0, out)) return 0;
+  }
+  qe_emit(out, QE_Opcode_Halt);
+  if (out->overflow) {
+    snprintf(out->error, sizeof(out->error), "WHERE too large for the bytecode buffers");
+    return 0;
+  }
+  return 1;
+}
+
+/* ---------------- numeric AND leaves ---------------- */
+static U32 qe_collect_and_nodes(const QE_IRNode* nodes, U32 i, U32* out, U32 n, U32 cap)
+{
+  if (nodes[i].type == QE_IR_OPERATOR && nodes[i].op == QE_IROP_AND && nodes[i].child_count == 2) {
+    U32 left = i + 1, right = qe_ir_subtree_end(nodes, left);
+    n = qe_collect_and_nodes(nodes, left, out, n, cap);
+    return qe_collect_and_nodes(nodes, right, out, n, cap);
+  }
+  if (n < cap) out[n] = i;
+  return n + 1;
+}
+
+/* qe_resolve_leaf_comparison: `column op literal` or `literal op column` */
+static B32 qe_resolve_leaf(const QE_IRNode* nodes, U32 i, QE_Leaf* leaf)
+{
+  const QE_IRNode* n = &nodes[i];
+  if (n->type != QE_IR_OPERATOR || n->child_count != 2) return 0;
+  U32 left = i + 1, right = qe_ir_subtree_end(nodes, left);
+  const QE_IRNode *l = &nodes[left], *r = &nodes[right];
+  U32 op;
+  switch (n->op) {
+    case QE_IROP_EQ: op = 0; break;
+    case QE_IROP_LT: op = 1; break;
+    case QE_IROP_LE: op = 2; break;
+    case QE_IROP_GT: op = 3; break;
+    case QE_IROP_GE: op = 4; break;
+    default: return 0;
+  }
+  if (l->type == QE_IR_COLUMN && r->type == QE_IR_NUMERIC) {
+    leaf->column = l->column;
+    leaf->op = op;
+    leaf->target = r->value;
+    return 1;
+  }
+  if (l->type == QE_IR_NUMERIC && r->type == QE_IR_COLUMN) {
+    static const U32 flip[5] = {0, 3, 4, 1, 2};
+    leaf->column = r->column;
+    leaf->op = flip[op];
+    leaf->target = l->value;
+    return 1;
+  }
+  return 0;
+}
+
+B32 qe_collect_and_leaves(const QE_IRNode* nodes, U32 count, QE_Leaf* leaves, U32* leaf_count, U32 max_leaves)
+{
+  if (!nodes || count == 0) return 0;
+  U32 idx[QE_SCAN_MAX_LEAVES + 1];
+  U32 n = qe_collect_and_nodes(nodes, 0, idx, 0, QE_SCAN_MAX_LEAVES + 1);
+  if (n == 0 || n > max_leaves || n > QE_SCAN_MAX_LEAVES) return 0;
+  for (U32 k = 0; k < n; k++)
+    if (!qe_resolve_leaf(nodes, idx[k], &leaves[k])) return 0;
+  *leaf_count = n;
+  return 1;
+}
+
+U32 qe_collect_prune_leaves(const QE_IRNode* nodes, U32 count, QE_Leaf* leaves, U32 max_leaves)
+{
+  if (!nodes || count == 0) return 0;
+  U32 idx[64];
+  U32 n = qe_collect_and_nodes(nodes, 0, idx, 0, 64);
+  if (n > 64) n = 64;
+  U32 out = 0;
+  for (U32 k = 0; k < n && out < max_leaves; k++)
+    if (qe_resolve_leaf(nodes, idx[k], &leaves[out])) out++;
+  return out;
+}
+
+/* query_exec.c:1378-1404 */
+B32 qe_scan_int_interval(U32 op, F64 target, F64 code_offset, F64 max_code, U32* out_lo, U32* out_span)
+{
+  if (target != target) return 0;
+  F64 low = -1e18, high = 1e18;
+  switch (op) {
+    case 0:
+      if (floor(target) != target) return 0;
+      low = target;
+      high = target;
+      break;
+    case 1: high = ceil(target) - 1.0; break;
+    case 2: high = floor(target); break;
+    case 3: low = floor(target) + 1.0; break;
+    case 4: low = ceil(target); break;
+    default: return 0;
+  }
+  low = fmax(low - code_offset, 0.0);
+  high = fmin(high - code_offset, max_code);
+  if (low > high) return 0;
+  *out_lo = (U32)low;
+  *out_span = (U32)(high - low);
+  return 1;
+}
+
+#define QE_SCAN_INT_FLAG 0x10000u
+#define QE_SCAN_INT_NEVER 0x20000u
+#define QE_SCAN_INT_BIAS 0x40000u
+
+/* query_exec.c:1408-1456, full-width columns (width 0) */
+void qe_simple_leaf_words(U32* words, U32 slot, U32 type, U32 op, F64 target)
+{
+  U64 bits;
+  memcpy(&bits, &target, 8);
+  words[0] = slot;
+  words[1] = type;
+  words[2] = op;
+  words[3] = (U32)(bits & 0xffffffffu);
+  words[4] = (U32)(bits >> 32);
+  words[5] = 0;
+  if ((type == CQL_ColumnType_U32 || type == CQL_ColumnType_I32) && qe_settings_u64("QE_SCAN_INT_COMPARE", 1) != 0) {
+    F64 code_offset = 0.0, max_code = 4294967295.0;
+    U32 flags = QE_SCAN_INT_FLAG;
+    if (type == CQL_ColumnType_I32) {
+      code_offset = -2147483648.0;
+      flags |= QE_SCAN_INT_BIAS;
+    }
+    U32 lo = 0, span = 0;
+    if (!qe_scan_int_interval(op, target, code_offset, max_code, &lo, &span)) flags |= QE_SCAN_INT_NEVER;
+    words[1] = type | flags;
+    words[3] = lo;
+    words[4] = span;
+    words[5] = 0;
+  }
+}
+
+/* ---------------- zone maps ---------------- */
+static B32 qe_zone_prunable(const QE_Leaf* leaf, F64 zmin, F64 zmax)
+{
+  switch (leaf->op) {
+    case 0: return leaf->target < zmin || leaf->target > zmax;
+    case 1: return zmin >= leaf->target;
+    case 2: return zmin > leaf->target;
+    case 3: return zmax <= leaf->target;
+    default: return zmax < leaf->target;
+  }
+}
+
+void qe_zonemap_prune(QE_Table* table, const QE_Leaf* leaves, U32 leaf_count, QE_Prune* out)
+{
+  memset(out, 0, sizeof(*out));
+  out->row_begin = 0;
+  out->row_end = table->row_count;
+  U64 zones = (table->row_count + (1ull << QE_ZONE_SHIFT) - 1) >> QE_ZONE_SHIFT;
+  out->zones_total = zones;
+  out->zones_left = zones;
+  if (leaf_count == 0 || zones == 0 || qe_settings_u64("QE_ZONEMAP", 1) == 0) return;
+  for (U32 l = 0; l < leaf_count; l++)
+    if (leaves[l].column >= table->column_count || !table->columns[leaves[l].column].zonemap_valid) return;
+  U64 words = (zones + 31) / 32;
+  U32* bits = (U32*)calloc(words, sizeof(U32));
+  if (!bits) return;
+  U64 left = 0, first = zones, last = 0;
+  for (U64 z = 0; z < zones; z++) {
+    B32 skip = 0;
+    for (U32 l = 0; l < leaf_count && !skip; l++) {
+      QE_Column* c = &table->columns[leaves[l].column];
+      skip = qe_zone_prunable(&leaves[l], c->zone_min[z], c->zone_max[z]);
+    }
+    if (skip) bits[z >> 5] |= 1u << (z & 31);
+    else {
+      left++;
+      if (z < first) first = z;
+      last = z;
+    }
+  }
+  out->zones_left = left;
+  if (left == zones) {
+    free(bits);
+    return;
+  }
+  out->skip_bits = bits;
+  out->skip_words = words;
+  if (left == 0) {
+    out->row_begin = out->row_end = 0;
+    return;
+  }
+  out->row_begin = first << QE_ZONE_SHIFT;
+  U64 end = (last + 1) << QE_ZONE_SHIFT;
+  out->row_end = end < table->row_count ? end : table->row_count;
+}
+
+void qe_prune_free(QE_Prune* p)
+{
+  free(p->skip_bits);
+  p->skip_bits = 0;
+}
+
+/* ---------------- the scan driver ---------------- */
+static void qe_bind_columns(GPU_Kernel* k, QE_Table* t, const QE_ColumnSlots* slots)
+{
+  for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
+    if (s < slots->count) gpu_kernel_set_arg_buffer(k, 6 + s, t->columns[slots->column[s]].buffer);
+    else gpu_hip_kernel_clear_arg(k, 6 + s);
+  }
+}
+
+B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only, U64* out_ids, U64 cap,
+                   U64* out_count, QE_Trace* trace)
+{
+  U64 t0, p0;
+  qe_trace_begin(trace, &t0, &p0);
+  if (!table || !out_count) {
+    qe_set_error("qe_scan_filter: null argument");
+    return 0;
+  }
+  U64 rows = table->row_count;
+  if (trace) trace->rows_before = rows;
+  static __thread QE_Bytecode prog;
+  if (!qe_bytecode_program_build(table, where, node_count, &prog)) {
+    qe_set_error("qe_scan_filter: %s", prog.error);
+    return 0;
+  }
+  /* constant predicates (no column read) */
+  if (prog.word_count == 2 && (prog.words[0] == QE_Opcode_PushTrue || prog.words[0] == QE_Opcode_PushFalse)) {
+    U64 n = prog.words[0] == QE_Opcode_PushTrue ? rows : 0;
+    *out_count = n;
+    if (!count_only && out_ids)
+      for (U64 i = 0; i < n && i < cap; i++) out_ids[i] = i;
+    if (trace) {
+      trace->path = QE_ScanPath_Constant;
+      trace->rows_after = n;
+    }
+    qe_trace_end(trace, t0, p0);
+    return 1;
+  }
+
+  QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
+  U32 leaf_count = 0;
+  B32 is_and = qe_collect_and_leaves(where, node_count, leaves, &leaf_count, QE_SCAN_MAX_LEAVES);
+  QE_Leaf prune_leaves[32];
+  U32 prune_count = qe_collect_prune_leaves(where, node_count, prune_leaves, 32);
+  QE_Prune prune;
+  qe_zonemap_prune(table, prune_leaves, prune_count, &prune);
+  if (trace) trace->rows_scanned = prune.row_end - prune.row_begin;
+  if (prune.zones_left == 0) {
+    qe_prune_free(&prune);
+    *out_count = 0;
+    if (trace) trace->path = QE_ScanPath_ZoneEmpty;
+    qe_trace_end(trace, t0, p0);
+    return 1;
+  }
+
+  /* kernel choice, in the order of query_exec.c:2566-2656 */
+  U32 path;
+  B32 force_interp = qe_settings_u64("QE_SCAN_FORCE_INTERPRETER", 0) != 0;
+  B32 perquery = qe_settings_u64("QE_PERQUERY_KERNELS", 1) != 0 && rows >= qe_settings_u64("QE_PERQUERY_MIN_ROWS", 0);
+  B32 simple_ok = is_and && qe_settings_u64("QE_SCAN_SIMPLE_KERNEL", 1) != 0;
+  if (force_interp) path = QE_ScanPath_Interpreter;
+  else if (simple_ok) path = perquery && qe_settings_u64("QE_PERQUERY_SIMPLE", 0) ? QE_ScanPath_GeneratedSimple : QE_ScanPath_Simple;
+  else if (perquery) path = QE_ScanPath_Generated;
+  else path = QE_ScanPath_Interpreter;
+  if (path == QE_ScanPath_Interpreter && qe_settings_u64("QE_SCAN_INTERPRETER", 1) == 0) {
+    qe_prune_free(&prune);
+    qe_set_error("qe_scan_filter: no GPU kernel for this WHERE (interpreter disabled)");
+    return 0;
+  }
+
+  GPU_Kernel* k = 0;
+  QE_ColumnSlots slots;
+  memset(&slots, 0, sizeof(slots));
+  U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
+  F64 consts[256];
+  U32 const_count = 0;
+  U64 compile_ns = 0;
+  if (path == QE_ScanPath_GeneratedSimple) {
+    k = qk_simple_kernel(table, leaves, leaf_count, &slots, &compile_ns);
+    if (!k) path = QE_ScanPath_Simple; /* the library kernel until a compile lands */
+  }
+  if (path == QE_ScanPath_Generated) {
+    k = qk_scan_kernel(table, where, node_count, &slots, consts, &const_count, &compile_ns);
+    if (!k) path = QE_ScanPath_Interpreter;
+  }
+  if (path == QE_ScanPath_Simple || path == QE_ScanPath_GeneratedSimple) {
+    if (path == QE_ScanPath_Simple) {
+      memset(&slots, 0, sizeof(slots));
+      k = qe_kernel("scan_simple");
+    }
+    for (U32 l = 0; l < leaf_count; l++) {
+      U32 slot = qe_slot_of(&slots, leaves[l].column);
+      qe_simple_leaf_words(&leaf_words[6 * l], slot, table->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+    }
+  }
+  if (path == QE_ScanPath_Interpreter) {
+    k = qe_kernel("scan_filter");
+    slots = prog.slots;
+  }
+  if (!k) {
+    qe_prune_free(&prune);
+    return 0;
+  }
+
+  /* device buffers: program, constants, skip bits, counter, block counts */
+  GPU_Buffer* prog_buf = 0;
+  GPU_Buffer* const_buf = 0;
+  if (path == QE_ScanPath_Simple || path == QE_ScanPath_GeneratedSimple) {
+    prog_buf = qe_scratch("scan_leaves", sizeof(leaf_words));
+    if (prog_buf) gpu_buffer_write(prog_buf, leaf_words, 6 * 4 * (U64)leaf_count);
+  } else if (path == QE_ScanPath_Interpreter) {
+    prog_buf = qe_scratch("scan_bytecode", prog.word_count * 4);
+    const_buf = qe_scratch("scan_consts", (prog.const_count ? prog.const_count : 1) * 8);
+    if (prog_buf) gpu_buffer_write(prog_buf, prog.words, prog.word_count * 4);
+    if (const_buf && prog.const_count) gpu_buffer_write(const_buf, prog.consts, prog.const_count * 8);
+  } else {
+    const_buf = qe_scratch("scan_consts", (const_count ? const_count : 1) * 8);
+    if (const_buf && const_count) gpu_buffer_write(const_buf, consts, const_count * 8);
+  }
+  GPU_Buffer* skip_buf = 0;
+  if (prune.skip_bits) {
+    skip_buf = qe_scratch("scan_skip", prune.skip_words * 4);
+    if (skip_buf) gpu_buffer_write(skip_buf, prune.skip_bits, prune.skip_words * 4);
+  }
+  U64 scan_rows = prune.row_end - prune.row_begin;
+  U64 quads = (prune.row_end + 3) / 4 - prune.row_begin / 4;
+  U32 blocks = qe_stream_blocks(quads);
+  GPU_Buffer* counter = qe_scratch("scan_counter", 64);
+  GPU_Buffer* block_counts = qe_scratch("scan_block_counts", (U64)blocks * 8);
+  B32 ok = counter && block_counts && (prog_buf || path == QE_ScanPath_Generated) && (!prune.skip_bits || skip_buf);
+  if (ok) {
+    if (prog_buf) gpu_kernel_set_arg_buffer(k, 0, prog_buf);
+    else gpu_hip_kernel_clear_arg(k, 0);
+    if (const_buf) gpu_kernel_set_arg_buffer(k, 1, const_buf);
+    else gpu_hip_kernel_clear_arg(k, 1);
+    gpu_kernel_set_arg_buffer(k, 2, block_counts);
+    gpu_kernel_set_arg_buffer(k, 4, counter);
+    if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
+    else gpu_hip_kernel_clear_arg(k, 5);
+    qe_bind_columns(k, table, &slots);
+    gpu_kernel_set_arg_u64(k, 0, scan_rows);
+    gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
+    gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
+    gpu_kernel_set_arg_u64(k, 5, prune.row_begin);
+    gpu_kernel_set_arg_u64(k, 7, leaf_count);
+    gpu_hip_kernel_clear_arg(k, 3);
+  }
+  U64 total = 0;
+  U32 kernels = 0;
+  if (ok && count_only) {
+    gpu_kernel_set_arg_u64(k, 6, 0); /* QK_MODE_COUNT */
+    GPU_Batch* b = gpu_batch_begin(0, 8);
+    gpu_batch_buffer_zero(b, counter, 8);
+    gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+    gpu_batch_buffer_read(b, counter, &total, 8);
+    ok = gpu_batch_end(b);
+    kernels = 1;
+  } else if (ok) {
+    U64* counts = (U64*)malloc(sizeof(U64) * blocks);
+    gpu_kernel_set_arg_u64(k, 6, 1); /* QK_MODE_BLOCK_COUNTS */
+    GPU_Batch* b = gpu_batch_begin(0, (U64)blocks * 8);
+    gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+    gpu_batch_buffer_read(b, block_counts, counts, (U64)blocks * 8);
+    ok = counts && gpu_batch_end(b);
+    kernels = 1;
+    if (ok) {
+      for (U32 i = 0; i < blocks; i++) total += counts[i];
+      U64 want = total < cap ? total : cap;
+      if (want > 0 && out_ids) {
+        GPU_Buffer* ids = qe_scratch("scan_ids", want * 8);
+        ok = ids != 0;
+        if (ok) {
+          gpu_kernel_set_arg_buffer(k, 3, ids);
+          gpu_kernel_set_arg_u64(k, 6, 2); /* QK_MODE_WRITE_IDS */
+          b = gpu_batch_begin(0, want * 8);
+          gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+          gpu_batch_buffer_read(b, ids, out_ids, want * 8);
+          ok = gpu_batch_end(b);
+          kernels = 2;
+          gpu_hip_kernel_clear_arg(k, 3);
+        }
+        qe_scratch_release(ids);
+      }
+    }
+    free(counts);
+  }
+  qe_scratch_release(prog_buf);
+  qe_scratch_release(const_buf);
+  qe_scratch_release(skip_buf);
+  qe_scratch_release(counter);
+  qe_scratch_release(block_counts);
+  qe_prune_free(&prune);
+  if (!ok) {
+    qe_set_error("qe_scan_filter: GPU dispatch failed: %s", gpu_hip_last_error());
+    return 0;
+  }
+  *out_count = total;
+  if (trace) {
+    trace->path = path;
+    trace->rows_after = total;
+    trace->kernel_count = kernels;
+    trace->compile_ns = compile_ns;
+    trace->bytes_read = scan_rows * 4 * (U64)slots.count; /* every slot column is read once */
+    snprintf(trace->kernel, sizeof(trace->kernel), "%s",
+             path == QE_ScanPath_Interpreter ? "scan_filter" : (path == QE_ScanPath_Simple ? "scan_simple" : "qk_main"));
+  }
+  qe_trace_end(trace, t0, p0);
+  return 1;
+}

@@ -1,0 +1,324 @@
+/*
+ * qk_device_common.h - device code shared by the library kernels
+ * (qk_library.hip) and the per-query kernels compiled at run time with
+ * hiprtc (query_kernel.c embeds this file verbatim as the prelude of every
+ * generated source).  It must therefore include no headers.
+ *
+ * Scan model (gfx950, wave64): a thread owns a quad of 4 consecutive rows
+ * read with one 16-byte load per 4-byte column; each block owns one
+ * contiguous range of quads and walks it in steps of 256 threads x QK_UNROLL
+ * quads, so the loads of a wave cover 1 KiB contiguous per column.  Row
+ * matches are 4-bit masks per quad; stream compaction is wave64 ballot-free
+ * prefix sums over the masks' popcounts, which keeps row ids in ascending
+ * order (the reference's scan_simple emits them in arbitrary order,
+ * scan_simple.comp:330-383).
+ */
+#ifndef QK_DEVICE_COMMON_H
+#define QK_DEVICE_COMMON_H
+
+typedef unsigned int qk_u32;
+typedef unsigned long long qk_u64;
+typedef int qk_s32;
+typedef long long qk_s64;
+typedef qk_u32 qk_u32x4 __attribute__((ext_vector_type(4)));
+typedef qk_u64 qk_u64x2 __attribute__((ext_vector_type(2)));
+
+#ifndef CQLHIP_H
+/* same layout as QK_Args of cqlhip.h */
+typedef struct QK_Args {
+  qk_u64 ptr[16];
+  qk_u64 bytes[16];
+  qk_u32 words[16];
+  qk_u32 global_work_size;
+  qk_u32 grid_blocks;
+} QK_Args;
+#endif
+
+#define QK_WG 256
+#define QK_UNROLL 4
+
+/* column type codes (CQL_ColumnType, cql.h:78-95) */
+#define QK_T_U32 1
+#define QK_T_U64 2
+#define QK_T_F32 3
+#define QK_T_F64 4
+#define QK_T_I32 7
+#define QK_T_I64 8
+
+/* scan leaf flags (query_exec.c:1372-1374) */
+#define QK_LEAF_INT 0x10000u
+#define QK_LEAF_NEVER 0x20000u
+#define QK_LEAF_BIAS 0x40000u
+
+/* scan bindings (query_exec.h:4-40) */
+#define QK_B_LEAVES 0
+#define QK_B_CONSTS 1
+#define QK_B_BLOCK_COUNTS 2
+#define QK_B_OUT_IDS 3
+#define QK_B_OUT_COUNT 4
+#define QK_B_SKIP 5
+#define QK_B_COL0 6
+
+/* scan push slots (U64): 0 row_count, 3 skip shift, 4 skip enabled,
+ * 5 first row (chunk start), 6 mode, 7 leaf count */
+#define QK_P_ROWS 0
+#define QK_P_SKIP_SHIFT 3
+#define QK_P_SKIP_ON 4
+#define QK_P_ROW_BEGIN 5
+#define QK_P_MODE 6
+#define QK_P_LEAVES 7
+
+#define QK_MODE_COUNT 0
+#define QK_MODE_BLOCK_COUNTS 1
+#define QK_MODE_WRITE_IDS 2
+
+__device__ __forceinline__ qk_u64 qk_push(const QK_Args& a, int slot)
+{
+  return (qk_u64)a.words[2 * slot] | ((qk_u64)a.words[2 * slot + 1] << 32);
+}
+
+template <class T>
+__device__ __forceinline__ const T* qk_ptr(const QK_Args& a, int binding)
+{
+  return (const T*)a.ptr[binding];
+}
+
+template <class T>
+__device__ __forceinline__ T* qk_wptr(const QK_Args& a, int binding)
+{
+  return (T*)a.ptr[binding];
+}
+
+/* the contiguous quad range [b0,b1) of this block over [q0,q1), a multiple
+ * of one step so no two blocks share a step */
+__device__ __forceinline__ void qk_block_range(qk_u64 q0, qk_u64 q1, qk_u64& b0, qk_u64& b1)
+{
+  qk_u64 n = q1 - q0;
+  qk_u64 step = (qk_u64)QK_WG * QK_UNROLL;
+  qk_u64 per = (n + gridDim.x - 1) / gridDim.x;
+  per = (per + step - 1) / step * step;
+  b0 = q0 + per * blockIdx.x;
+  if (b0 > q1) b0 = q1;
+  b1 = b0 + per;
+  if (b1 > q1) b1 = q1;
+}
+
+/* valid-row mask of quad q for rows [r0,r1) */
+__device__ __forceinline__ qk_u32 qk_quad_valid(qk_u64 q, qk_u64 r0, qk_u64 r1)
+{
+  qk_u64 first = q * 4;
+  qk_u32 m = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) m |= ((first + k >= r0) && (first + k < r1)) ? (1u << k) : 0u;
+  return m;
+}
+
+/* zone-map skip bitmap: set bit = every row of the zone is skipped */
+__device__ __forceinline__ bool qk_quad_skipped(const QK_Args& a, qk_u64 q)
+{
+  if (!qk_push(a, QK_P_SKIP_ON)) return false;
+  qk_u64 zone = (q * 4) >> qk_push(a, QK_P_SKIP_SHIFT);
+  const qk_u32* bits = qk_ptr<qk_u32>(a, QK_B_SKIP);
+  return (bits[zone >> 5] >> (zone & 31)) & 1u;
+}
+
+/* four values of a column as f64 (exact for every 4-byte type) */
+__device__ __forceinline__ void qk_load4_f64(const QK_Args& a, qk_u32 slot, qk_u32 type, qk_u64 q, double v[4])
+{
+  qk_u64 base = a.ptr[QK_B_COL0 + slot];
+  if (type == QK_T_U32 || type == QK_T_I32 || type == QK_T_F32) {
+    qk_u32x4 x = ((const qk_u32x4*)base)[q];
+    qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (type == QK_T_U32) v[k] = (double)w[k];
+      else if (type == QK_T_I32) v[k] = (double)(qk_s32)w[k];
+      else v[k] = (double)__uint_as_float(w[k]);
+    }
+  } else {
+    const qk_u64x2* p = (const qk_u64x2*)base + 2 * q;
+    qk_u64x2 x0 = p[0], x1 = p[1];
+    qk_u64 w[4] = {x0.x, x0.y, x1.x, x1.y};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (type == QK_T_F64) v[k] = __longlong_as_double((qk_s64)w[k]);
+      else if (type == QK_T_I64) v[k] = (double)(qk_s64)w[k];
+      else v[k] = (double)w[k];
+    }
+  }
+}
+
+/* one numeric leaf over a quad (qe_simple_leaf_words layout,
+ * query_exec.c:1442-1456; integer form :1408-1438) */
+__device__ __forceinline__ qk_u32 qk_leaf_quad(const QK_Args& a, const qk_u32* w, qk_u64 q)
+{
+  qk_u32 slot = w[0], tw = w[1], op = w[2];
+  qk_u32 type = tw & 0xffu;
+  if (tw & QK_LEAF_NEVER) return 0u;
+  qk_u32 m = 0;
+  if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
+    qk_u32 lo = w[3], span = w[4];
+    qk_u32 bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
+    qk_u32x4 x = ((const qk_u32x4*)a.ptr[QK_B_COL0 + slot])[q];
+    m |= (((x.x ^ bias) - lo) <= span) ? 1u : 0u;
+    m |= (((x.y ^ bias) - lo) <= span) ? 2u : 0u;
+    m |= (((x.z ^ bias) - lo) <= span) ? 4u : 0u;
+    m |= (((x.w ^ bias) - lo) <= span) ? 8u : 0u;
+    return m;
+  }
+  double t = __longlong_as_double((qk_s64)((qk_u64)w[3] | ((qk_u64)w[4] << 32)));
+  double v[4];
+  qk_load4_f64(a, slot, type, q, v);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    bool p;
+    switch (op) {
+      case 0: p = v[k] == t; break;
+      case 1: p = v[k] < t; break;
+      case 2: p = v[k] <= t; break;
+      case 3: p = v[k] > t; break;
+      default: p = v[k] >= t; break;
+    }
+    m |= p ? (1u << k) : 0u;
+  }
+  return m;
+}
+
+/* AND of leaf_count leaves (<= 8) */
+__device__ __forceinline__ qk_u32 qk_leaves_quad(const QK_Args& a, const qk_u32* leaves, qk_u32 leaf_count, qk_u64 q,
+                                                 qk_u32 mask)
+{
+  for (qk_u32 l = 0; l < leaf_count; l++) {
+    if (mask) mask &= qk_leaf_quad(a, leaves + 6 * l, q);
+  }
+  return mask;
+}
+
+/* ---- wave64 / block reductions ---- */
+__device__ __forceinline__ qk_u64 qk_wave_sum_u64(qk_u64 v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ qk_u32 qk_wave_incl_scan_u32(qk_u32 v)
+{
+  qk_u32 lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    qk_u32 y = __shfl_up(v, o, 64);
+    if (lane >= (qk_u32)o) v += y;
+  }
+  return v;
+}
+
+/* block-wide sum of a u64, result valid in every thread */
+__device__ __forceinline__ qk_u64 qk_block_sum_u64(qk_u64 v, qk_u64* lds4)
+{
+  v = qk_wave_sum_u64(v);
+  qk_u32 wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (__lane_id() == 0) lds4[wave] = v;
+  __syncthreads();
+  qk_u64 t = 0;
+#pragma unroll
+  for (int w = 0; w < QK_WG / 64; w++) t += lds4[w];
+  return t;
+}
+
+/* block exclusive scan of a u32 per thread; returns the exclusive prefix and
+ * the block total */
+__device__ __forceinline__ qk_u32 qk_block_excl_scan_u32(qk_u32 v, qk_u32* lds4, qk_u32& total)
+{
+  qk_u32 incl = qk_wave_incl_scan_u32(v);
+  qk_u32 wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (__lane_id() == 63) lds4[wave] = incl;
+  __syncthreads();
+  qk_u32 before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < QK_WG / 64; w++) {
+    qk_u32 x = lds4[w];
+    before += ((qk_u32)w < wave) ? x : 0u;
+    all += x;
+  }
+  total = all;
+  return before + incl - v;
+}
+
+/* ---- the scan driver: count / per-block counts / ordered row ids ----
+ * pred(q, mask) returns the match mask of quad q given the valid mask. */
+template <class Pred>
+__device__ __forceinline__ void qk_scan_driver(const QK_Args& a, Pred pred)
+{
+  __shared__ qk_u64 s_red[QK_WG / 64];
+  __shared__ qk_u32 s_scan[QK_WG / 64];
+  const qk_u64 rows = qk_push(a, QK_P_ROWS);
+  const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
+  const qk_u64 r1 = r0 + rows;
+  const qk_u32 mode = (qk_u32)qk_push(a, QK_P_MODE);
+  qk_u64 b0, b1;
+  qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
+
+  if (mode != QK_MODE_WRITE_IDS) {
+    qk_u64 count = 0;
+    for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+      qk_u32 m[QK_UNROLL];
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) {
+        qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+        qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
+        if (valid && qk_quad_skipped(a, q)) valid = 0u;
+        m[j] = valid ? pred(q, valid) : 0u;
+      }
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) count += (qk_u64)__popc(m[j]);
+    }
+    qk_u64 total = qk_block_sum_u64(count, s_red);
+    if (threadIdx.x == 0) {
+      if (mode == QK_MODE_COUNT) {
+        if (total) atomicAdd(qk_wptr<qk_u64>(a, QK_B_OUT_COUNT), total);
+      } else {
+        qk_wptr<qk_u64>(a, QK_B_BLOCK_COUNTS)[blockIdx.x] = total;
+      }
+    }
+    return;
+  }
+
+  /* ordered write: this block's first output index is the sum of the
+   * counts of the blocks before it */
+  const qk_u64* bc = qk_ptr<qk_u64>(a, QK_B_BLOCK_COUNTS);
+  qk_u64 part = 0;
+  for (qk_u32 b = threadIdx.x; b < blockIdx.x; b += QK_WG) part += bc[b];
+  qk_u64 base = qk_block_sum_u64(part, s_red);
+  qk_u64* out = qk_wptr<qk_u64>(a, QK_B_OUT_IDS);
+  const qk_u64 cap = a.bytes[QK_B_OUT_IDS] / 8;
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u32 m[QK_UNROLL];
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+      qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
+      if (valid && qk_quad_skipped(a, q)) valid = 0u;
+      m[j] = valid ? pred(q, valid) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u32 total = 0;
+      qk_u32 pre = qk_block_excl_scan_u32((qk_u32)__popc(m[j]), s_scan, total);
+      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+      qk_u64 o = base + pre;
+      qk_u32 mm = m[j];
+      while (mm) {
+        int k = __ffs(mm) - 1;
+        if (o < cap) out[o] = q * 4 + (qk_u64)k;
+        o++;
+        mm &= mm - 1;
+      }
+      base += total;
+    }
+  }
+}
+
+#endif

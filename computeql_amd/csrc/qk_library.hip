@@ -1,0 +1,831 @@
+/*
+ * qk_library.hip - hand-written gfx950 kernels behind gpu_hip_kernel_alloc.
+ *
+ * Every kernel takes one QK_Args by value (16 buffer bindings with byte
+ * sizes + 16 push words) and is launched with 256-thread workgroups (four
+ * wave64s).  The kernels are grid-size agnostic: the host picks the grid
+ * (normally a few blocks per CU), and each block walks one contiguous range.
+ *
+ * Counterparts in the reference (GLSL, src/gpu/vulkan/shaders/):
+ *   fill_u32            vkCmdFillBuffer of batch_buffer_fill
+ *   bandwidth_read      bandwidth_read.comp
+ *   scan_simple         scan_simple.comp
+ *   scan_filter         scan_filter.comp (bytecode interpreter)
+ *   aggregate_global    aggregate_global_native.comp / _f32.comp, fused with
+ *                       an optional numeric-AND WHERE (fused global aggregate)
+ *   aggregate_dense     aggregate_dense_native.comp, with an optional WHERE
+ *   join_build          host table build of qe_join_probe_stream_core
+ *                       (query_exec.c:8149-8191), done on the device
+ *   join_probe          join_probe_native.comp
+ *   gen_* / zonemap_build  data generators and zone maps (cql.c:3818)
+ */
+#include <hip/hip_runtime.h>
+
+#include "../../include/cqlhip.h"
+#include "qk_device_common.h"
+#include "qk_library.h"
+
+#define LB __launch_bounds__(QK_WG)
+
+/* ------------------------------------------------------------------ */
+/* fill_u32: binding 0 = buffer; push 0 = words, push 1 = value         */
+/* ------------------------------------------------------------------ */
+extern "C" __global__ void LB k_fill_u32(QK_Args a)
+{
+  qk_u32* p = qk_wptr<qk_u32>(a, 0);
+  qk_u64 n = qk_push(a, 0);
+  qk_u32 v = (qk_u32)qk_push(a, 1);
+  qk_u64 stride = (qk_u64)gridDim.x * QK_WG;
+  for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < (n + 3) / 4; i += stride) {
+    if (i * 4 + 3 < n) {
+      ((qk_u32x4*)p)[i] = qk_u32x4{v, v, v, v};
+    } else {
+      for (qk_u64 k = i * 4; k < n; k++) p[k] = v;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* bandwidth_read: binding 0 = data, binding 1 = sink (u64);            */
+/* push 0 = 16-byte units to read                                       */
+/* ------------------------------------------------------------------ */
+extern "C" __global__ void LB k_bandwidth_read(QK_Args a)
+{
+  const qk_u32x4* p = qk_ptr<qk_u32x4>(a, 0);
+  qk_u64 n = qk_push(a, 0);
+  qk_u64 b0, b1;
+  /* same contiguous-range walk as the scans, one 16-byte unit per quad */
+  {
+    qk_u64 step = (qk_u64)QK_WG * QK_UNROLL;
+    qk_u64 per = (n + gridDim.x - 1) / gridDim.x;
+    per = (per + step - 1) / step * step;
+    b0 = per * blockIdx.x;
+    if (b0 > n) b0 = n;
+    b1 = b0 + per < n ? b0 + per : n;
+  }
+  qk_u32x4 acc = {0, 0, 0, 0};
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u32x4 x[QK_UNROLL];
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u64 i = s + (qk_u64)j * QK_WG + threadIdx.x;
+      x[j] = i < b1 ? p[i] : qk_u32x4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) acc ^= x[j];
+  }
+  qk_u32 r = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  if (r == 0x9e3779b9u) qk_wptr<qk_u64>(a, 1)[0] = r; /* keeps the loads live */
+}
+
+/* ------------------------------------------------------------------ */
+/* aggregate_global: COUNT and SUM/MIN/MAX of up to 4 columns with an   */
+/* optional numeric-AND WHERE, one pass.                                */
+/*  binding 0 leaves (6 words each), 1 result (16 x u64), 2 partials    */
+/*  (grid x 13 u64), 5 skip bitmap, 6.. column slots                    */
+/*  push 0 rows, 3 skip shift, 4 skip on, 5 first row, 7 leaf count     */
+/*  words[12] = arg count | arg slots << 8 (4 bits each)                */
+/*  words[13] = arg types (8 bits each)                                 */
+/* result: [0] count, [1+3c] sum, [2+3c] min key, [3+3c] max key,       */
+/* [15] ticket.  Sums: exact 64-bit integers for U32/I32 (I32 as two's  */
+/* complement), f64 bits otherwise.  Min/max keys are order-preserving  */
+/* u64 encodings (u32: value; i32/i64: value ^ sign; f64: IEEE flip).   */
+/* The last block to finish folds the partials in a fixed order, so the */
+/* f64 result depends only on the grid size.                            */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ qk_u64 qk_f64_key(double d)
+{
+  qk_u64 b = (qk_u64)__double_as_longlong(d);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+
+__device__ __forceinline__ bool qk_type_is_int32(qk_u32 t) { return t == QK_T_U32 || t == QK_T_I32; }
+
+struct AggAcc {
+  qk_u64 sum, mn, mx;
+};
+
+__device__ __forceinline__ void agg_fold_quad(const QK_Args& a, qk_u32 slot, qk_u32 type, qk_u64 q, qk_u32 m,
+                                              AggAcc& acc)
+{
+  qk_u64 base = a.ptr[QK_B_COL0 + slot];
+  if (type == QK_T_U32) {
+    qk_u32x4 x = ((const qk_u32x4*)base)[q];
+    if (m == 0xfu) {
+      acc.sum += (qk_u64)x.x + (qk_u64)x.y + (qk_u64)x.z + (qk_u64)x.w;
+      qk_u32 lo = min(min(x.x, x.y), min(x.z, x.w));
+      qk_u32 hi = max(max(x.x, x.y), max(x.z, x.w));
+      acc.mn = min(acc.mn, (qk_u64)lo);
+      acc.mx = max(acc.mx, (qk_u64)hi);
+    } else {
+      qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (m & (1u << k)) {
+          acc.sum += (qk_u64)w[k];
+          acc.mn = min(acc.mn, (qk_u64)w[k]);
+          acc.mx = max(acc.mx, (qk_u64)w[k]);
+        }
+      }
+    }
+  } else if (type == QK_T_I32) {
+    qk_u32x4 x = ((const qk_u32x4*)base)[q];
+    qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (m & (1u << k)) {
+        acc.sum += (qk_u64)(qk_s64)(qk_s32)w[k];
+        qk_u64 key = (qk_u64)(w[k] ^ 0x80000000u);
+        acc.mn = min(acc.mn, key);
+        acc.mx = max(acc.mx, key);
+      }
+    }
+  } else {
+    double v[4];
+    qk_load4_f64(a, slot, type, q, v);
+    double s = __longlong_as_double((qk_s64)acc.sum);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (m & (1u << k)) {
+        s += v[k];
+        qk_u64 key;
+        if (type == QK_T_U64) {
+          qk_u64x2 x = ((const qk_u64x2*)base)[2 * q + (k >> 1)];
+          key = (k & 1) ? x.y : x.x;
+        } else if (type == QK_T_I64) {
+          qk_u64x2 x = ((const qk_u64x2*)base)[2 * q + (k >> 1)];
+          key = ((k & 1) ? x.y : x.x) ^ 0x8000000000000000ULL;
+        } else {
+          key = qk_f64_key(v[k]);
+        }
+        acc.mn = min(acc.mn, key);
+        acc.mx = max(acc.mx, key);
+      }
+    }
+    acc.sum = (qk_u64)__double_as_longlong(s);
+  }
+}
+
+__device__ __forceinline__ qk_u64 agg_combine_sum(qk_u32 type, qk_u64 x, qk_u64 y)
+{
+  if (qk_type_is_int32(type)) return x + y;
+  return (qk_u64)__double_as_longlong(__longlong_as_double((qk_s64)x) + __longlong_as_double((qk_s64)y));
+}
+
+extern "C" __global__ void LB k_aggregate_global(QK_Args a)
+{
+  __shared__ qk_u64 s_part[QK_WG / 64][13];
+  __shared__ qk_u32 s_last;
+  const qk_u64 rows = qk_push(a, QK_P_ROWS);
+  const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
+  const qk_u64 r1 = r0 + rows;
+  const qk_u32 leaf_count = (qk_u32)qk_push(a, QK_P_LEAVES);
+  const qk_u32 nargs = a.words[12] & 0xffu;
+  const qk_u32 slots = a.words[12] >> 8;
+  const qk_u32 types = a.words[13];
+  const qk_u32* leaves = qk_ptr<qk_u32>(a, QK_B_LEAVES);
+
+  qk_u64 count = 0;
+  AggAcc acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    acc[c].sum = 0;
+    acc[c].mn = ~0ULL;
+    acc[c].mx = 0;
+  }
+  qk_u64 b0, b1;
+  qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u32 m[QK_UNROLL];
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+      qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
+      if (valid && qk_quad_skipped(a, q)) valid = 0u;
+      m[j] = leaf_count ? qk_leaves_quad(a, leaves, leaf_count, q, valid) : valid;
+    }
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      count += (qk_u64)__popc(m[j]);
+      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        if ((qk_u32)c < nargs && m[j]) agg_fold_quad(a, (slots >> (4 * c)) & 0xfu, (types >> (8 * c)) & 0xffu, q, m[j], acc[c]);
+      }
+    }
+  }
+
+  /* wave reduce, then across the 4 waves in wave order */
+  qk_u32 lane = __lane_id(), wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    count += __shfl_xor(count, o, 64);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      qk_u32 t = (types >> (8 * c)) & 0xffu;
+      qk_u64 os = __shfl_xor(acc[c].sum, o, 64);
+      acc[c].sum = agg_combine_sum(t, acc[c].sum, os);
+      acc[c].mn = min(acc[c].mn, (qk_u64)__shfl_xor(acc[c].mn, o, 64));
+      acc[c].mx = max(acc[c].mx, (qk_u64)__shfl_xor(acc[c].mx, o, 64));
+    }
+  }
+  if (lane == 0) {
+    s_part[wave][0] = count;
+    for (int c = 0; c < 4; c++) {
+      s_part[wave][1 + 3 * c] = acc[c].sum;
+      s_part[wave][2 + 3 * c] = acc[c].mn;
+      s_part[wave][3 + 3 * c] = acc[c].mx;
+    }
+  }
+  __syncthreads();
+  qk_u64* partials = qk_wptr<qk_u64>(a, 2);
+  qk_u64* result = qk_wptr<qk_u64>(a, 1);
+  if (threadIdx.x < 13) {
+    int f = threadIdx.x;
+    qk_u64 v = s_part[0][f];
+    for (int w = 1; w < QK_WG / 64; w++) {
+      qk_u64 x = s_part[w][f];
+      if (f == 0) v += x;
+      else {
+        int c = (f - 1) / 3, kind = (f - 1) % 3;
+        qk_u32 t = (types >> (8 * c)) & 0xffu;
+        v = kind == 0 ? agg_combine_sum(t, v, x) : (kind == 1 ? min(v, x) : max(v, x));
+      }
+    }
+    partials[(qk_u64)blockIdx.x * 13 + f] = v;
+  }
+  /* last block folds (release the partials, then take a ticket) */
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    qk_u32 ticket = __hip_atomic_fetch_add((qk_u32*)&result[15], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = ticket == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  /* fixed-order fold: thread f < 13 walks the blocks in order */
+  if (threadIdx.x < 13) {
+    int f = threadIdx.x;
+    qk_u64 v = __hip_atomic_load(&partials[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (qk_u32 b = 1; b < gridDim.x; b++) {
+      qk_u64 x = __hip_atomic_load(&partials[(qk_u64)b * 13 + f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (f == 0) v += x;
+      else {
+        int c = (f - 1) / 3, kind = (f - 1) % 3;
+        qk_u32 t = (types >> (8 * c)) & 0xffu;
+        v = kind == 0 ? agg_combine_sum(t, v, x) : (kind == 1 ? min(v, x) : max(v, x));
+      }
+    }
+    result[f] = v;
+  }
+  if (threadIdx.x == 0) __hip_atomic_store((qk_u32*)&result[15], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* ------------------------------------------------------------------ */
+/* scan_simple: numeric-AND leaves, count / block counts / ordered ids  */
+/* (scan bindings and push slots of query_exec.h:4-40)                  */
+/* ------------------------------------------------------------------ */
+struct SimplePred {
+  const QK_Args* a;
+  const qk_u32* leaves;
+  qk_u32 leaf_count;
+  __device__ __forceinline__ qk_u32 operator()(qk_u64 q, qk_u32 valid) const
+  {
+    return qk_leaves_quad(*a, leaves, leaf_count, q, valid);
+  }
+};
+
+extern "C" __global__ void LB k_scan_simple(QK_Args a)
+{
+  SimplePred p{&a, qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES)};
+  qk_scan_driver(a, p);
+}
+
+/* ------------------------------------------------------------------ */
+/* scan_filter: the stack-bytecode interpreter (scan_filter.comp), kept  */
+/* as the fallback for shapes the other kernels do not take.  Opcodes   */
+/* of QE_Opcode (query_exec.h:58-82), numeric subset; stack of 8 f64.   */
+/* binding 0 bytecode (u32 words), 1 numeric constants (f64).           */
+/* ------------------------------------------------------------------ */
+#define OP_PUSH_TRUE 0
+#define OP_LOAD_NUM_COL 1
+#define OP_PUSH_CONST 2
+#define OP_CMP_EQ 3
+#define OP_CMP_NE 4
+#define OP_CMP_LT 5
+#define OP_CMP_GT 6
+#define OP_CMP_LE 7
+#define OP_CMP_GE 8
+#define OP_AND 9
+#define OP_OR 10
+#define OP_HALT 13
+#define OP_PUSH_FALSE 14
+#define QK_MAX_STACK 8
+
+__device__ __forceinline__ double qk_load1_f64(const QK_Args& a, qk_u32 slot, qk_u32 type, qk_u64 row)
+{
+  qk_u64 base = a.ptr[QK_B_COL0 + slot];
+  switch (type) {
+    case QK_T_U32: return (double)((const qk_u32*)base)[row];
+    case QK_T_I32: return (double)((const qk_s32*)base)[row];
+    case QK_T_F32: return (double)((const float*)base)[row];
+    case QK_T_F64: return ((const double*)base)[row];
+    case QK_T_I64: return (double)((const qk_s64*)base)[row];
+    case QK_T_U64: return (double)((const qk_u64*)base)[row];
+    default: return 0.0;
+  }
+}
+
+struct InterpPred {
+  const QK_Args* a;
+  const qk_u32* code;
+  const double* consts;
+  qk_u32 code_words;
+  __device__ __forceinline__ qk_u32 operator()(qk_u64 q, qk_u32 valid) const
+  {
+    qk_u32 m = 0;
+    for (int k = 0; k < 4; k++) {
+      if (!(valid & (1u << k))) continue;
+      qk_u64 row = q * 4 + k;
+      double st[QK_MAX_STACK];
+      int sp = 0;
+      qk_u32 ip = 0;
+      bool ok = true;
+      while (ip < code_words && ok) {
+        qk_u32 op = code[ip++];
+        if (op == OP_HALT) break;
+        switch (op) {
+          case OP_PUSH_TRUE: ok = sp < QK_MAX_STACK; if (ok) st[sp++] = 1.0; break;
+          case OP_PUSH_FALSE: ok = sp < QK_MAX_STACK; if (ok) st[sp++] = 0.0; break;
+          case OP_LOAD_NUM_COL: {
+            qk_u32 operand = code[ip++];
+            ok = sp < QK_MAX_STACK;
+            if (ok) st[sp++] = qk_load1_f64(*a, operand & 0xffu, operand >> 8, row);
+          } break;
+          case OP_PUSH_CONST: {
+            qk_u32 ci = code[ip++];
+            ok = sp < QK_MAX_STACK;
+            if (ok) st[sp++] = consts[ci];
+          } break;
+          default: {
+            ok = sp >= 2;
+            if (!ok) break;
+            double r = st[--sp], l = st[--sp];
+            bool v;
+            switch (op) {
+              case OP_CMP_EQ: v = l == r; break;
+              case OP_CMP_NE: v = l != r; break;
+              case OP_CMP_LT: v = l < r; break;
+              case OP_CMP_GT: v = l > r; break;
+              case OP_CMP_LE: v = l <= r; break;
+              case OP_CMP_GE: v = l >= r; break;
+              case OP_AND: v = (l != 0.0) && (r != 0.0); break;
+              case OP_OR: v = (l != 0.0) || (r != 0.0); break;
+              default: v = false; ok = false; break;
+            }
+            st[sp++] = v ? 1.0 : 0.0;
+          } break;
+        }
+      }
+      if (ok && sp >= 1 && st[sp - 1] != 0.0) m |= 1u << k;
+    }
+    return m;
+  }
+};
+
+extern "C" __global__ void LB k_scan_filter(QK_Args a)
+{
+  InterpPred p{&a, qk_ptr<qk_u32>(a, QK_B_LEAVES), qk_ptr<double>(a, QK_B_CONSTS), (qk_u32)(a.bytes[QK_B_LEAVES] / 4)};
+  qk_scan_driver(a, p);
+}
+
+/* ------------------------------------------------------------------ */
+/* aggregate_dense: GROUP BY one U32/I32 key of range <= LDS capacity    */
+/* (aggregate_dense_native.comp), optional numeric-AND WHERE.           */
+/*  binding 0 leaves, 1 global accumulators, 5 skip, 6.. columns        */
+/*  push 0 rows, 3 skip shift, 4 skip on, 5 first row, 7 leaf count     */
+/*  words[12] = arg count | arg slots << 8; words[13] = arg types       */
+/*  words[14] = key slot | key type << 8 | flags << 16                  */
+/*            flags: 1 sums, 2 min/max                                  */
+/*  words[10] = key code minimum, words[11] = range                     */
+/* Global layout (range R, args E): count u64[R]; sum u64[E][R];        */
+/* min u32[E][R]; max u32[E][R].  u32 keys of args are value for U32,   */
+/* value ^ 0x80000000 for I32 (order preserving).  Block-private LDS    */
+/* tables; counts u32, sums u64, then one global atomic per touched     */
+/* group and field at the end.                                          */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ qk_u32 qk_code32(qk_u32 v, qk_u32 type) { return type == QK_T_I32 ? (v ^ 0x80000000u) : v; }
+
+extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
+{
+  extern __shared__ qk_u64 lds_u64[];
+  const qk_u64 rows = qk_push(a, QK_P_ROWS);
+  const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
+  const qk_u64 r1 = r0 + rows;
+  const qk_u32 leaf_count = (qk_u32)qk_push(a, QK_P_LEAVES);
+  const qk_u32 nargs = a.words[12] & 0xffu;
+  const qk_u32 slots = a.words[12] >> 8;
+  const qk_u32 types = a.words[13];
+  const qk_u32 key_slot = a.words[14] & 0xffu;
+  const qk_u32 key_type = (a.words[14] >> 8) & 0xffu;
+  const qk_u32 flags = a.words[14] >> 16;
+  const qk_u32 key_min = a.words[10];
+  const qk_u32 range = a.words[11];
+  const qk_u32* leaves = qk_ptr<qk_u32>(a, QK_B_LEAVES);
+
+  /* LDS: sums u64[E][R] | counts u32[R] | min u32[E][R] | max u32[E][R] */
+  qk_u64* l_sum = lds_u64;
+  qk_u32* l_cnt = (qk_u32*)(lds_u64 + ((flags & 1u) ? (qk_u64)nargs * range : 0));
+  qk_u32* l_min = l_cnt + range;
+  qk_u32* l_max = l_min + ((flags & 2u) ? (qk_u64)nargs * range : 0);
+  const qk_u32 sum_words = (flags & 1u) ? nargs * range : 0u;
+  const qk_u32 mm_words = (flags & 2u) ? nargs * range : 0u;
+  for (qk_u32 i = threadIdx.x; i < sum_words; i += QK_WG) l_sum[i] = 0;
+  for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) l_cnt[i] = 0;
+  for (qk_u32 i = threadIdx.x; i < mm_words; i += QK_WG) {
+    l_min[i] = 0xffffffffu;
+    l_max[i] = 0u;
+  }
+  __syncthreads();
+
+  const qk_u32x4* keys = (const qk_u32x4*)a.ptr[QK_B_COL0 + key_slot];
+  qk_u64 b0, b1;
+  qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u32 m[QK_UNROLL];
+    qk_u32x4 kv[QK_UNROLL];
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+      qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
+      if (valid && qk_quad_skipped(a, q)) valid = 0u;
+      m[j] = leaf_count ? qk_leaves_quad(a, leaves, leaf_count, q, valid) : valid;
+      kv[j] = m[j] ? keys[q] : qk_u32x4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      if (!m[j]) continue;
+      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+      qk_u32 g[4] = {qk_code32(kv[j].x, key_type) - key_min, qk_code32(kv[j].y, key_type) - key_min,
+                     qk_code32(kv[j].z, key_type) - key_min, qk_code32(kv[j].w, key_type) - key_min};
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (m[j] & (1u << k)) atomicAdd(&l_cnt[g[k]], 1u);
+      for (qk_u32 c = 0; c < nargs; c++) {
+        qk_u32 at = (types >> (8 * c)) & 0xffu;
+        qk_u32x4 x = ((const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)])[q];
+        qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (!(m[j] & (1u << k))) continue;
+          if (flags & 1u) {
+            qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)w[k] : (qk_u64)w[k];
+            atomicAdd(&l_sum[c * range + g[k]], add);
+          }
+          if (flags & 2u) {
+            qk_u32 code = qk_code32(w[k], at);
+            atomicMin(&l_min[c * range + g[k]], code);
+            atomicMax(&l_max[c * range + g[k]], code);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  qk_u64* g_cnt = qk_wptr<qk_u64>(a, 1);
+  qk_u64* g_sum = g_cnt + range;
+  qk_u32* g_min = (qk_u32*)(g_sum + (qk_u64)nargs * range);
+  qk_u32* g_max = g_min + (qk_u64)nargs * range;
+  for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) {
+    qk_u32 cnt = l_cnt[i];
+    if (!cnt) continue;
+    atomicAdd(&g_cnt[i], (qk_u64)cnt);
+    for (qk_u32 c = 0; c < nargs; c++) {
+      if (flags & 1u) atomicAdd(&g_sum[(qk_u64)c * range + i], l_sum[c * range + i]);
+      if (flags & 2u) {
+        atomicMin(&g_min[(qk_u64)c * range + i], l_min[c * range + i]);
+        atomicMax(&g_max[(qk_u64)c * range + i], l_max[c * range + i]);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* join_build: open-addressing table of the dim keys on the device.     */
+/*  binding 0 keys (u32), 1 payload (u32, or none: payload = dim row),  */
+/*  2 table (u64 slots, 0 = empty, entry = (payload+1) << 32 | key),    */
+/*  3 flags (u32[2]: [0] duplicate key seen, [1] table full),           */
+/*  4 row list (u32 dim rows to insert, or none: all rows)              */
+/*  push 0 rows to insert, words[2] log2 slots                          */
+/* Slot of a key: (key * 0x9E3779B1) >> (32 - log2 slots), linear probe  */
+/* (the hashing of join_probe_native.comp / query_exec.c:8149-8191).    */
+/* ------------------------------------------------------------------ */
+extern "C" __global__ void LB k_join_build(QK_Args a)
+{
+  const qk_u64 n = qk_push(a, 0);
+  const qk_u32 lg = a.words[2];
+  const qk_u64 mask = (1ULL << lg) - 1;
+  const qk_u32* keys = qk_ptr<qk_u32>(a, 0);
+  const qk_u32* payload = a.ptr[1] ? qk_ptr<qk_u32>(a, 1) : nullptr;
+  const qk_u32* list = a.ptr[4] ? qk_ptr<qk_u32>(a, 4) : nullptr;
+  qk_u64* table = qk_wptr<qk_u64>(a, 2);
+  qk_u32* flags = qk_wptr<qk_u32>(a, 3);
+  for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
+    qk_u32 row = list ? list[i] : (qk_u32)i;
+    qk_u32 key = keys[row];
+    qk_u32 pay = payload ? payload[row] : row;
+    qk_u64 entry = ((qk_u64)(pay + 1u) << 32) | key;
+    qk_u64 slot = lg ? ((qk_u64)(key * 0x9E3779B1u) >> (32 - lg)) : 0;
+    qk_u64 probes = 0;
+    for (;;) {
+      qk_u64 prev = atomicCAS(&table[slot], 0ULL, entry);
+      if (prev == 0ULL) break;
+      if ((qk_u32)prev == key) {
+        atomicOr(&flags[0], 1u);
+        break;
+      }
+      slot = (slot + 1) & mask;
+      if (++probes > mask) {
+        atomicOr(&flags[1], 1u);
+        break;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* join_probe: probe the fact key of every row and aggregate.           */
+/*  binding 1 accumulators (layout of aggregate_dense, R = groups),     */
+/*  2 table, 5 skip, 6 fact key column, 7.. arg column slots            */
+/*  push 0 rows, 5 first row; words[2] log2 slots; words[12]/[13] args  */
+/*  as aggregate_dense; words[11] = R; words[14] = flags << 16          */
+/*  (1 sums, 2 min/max, 4 payload is a dim row: global atomics into     */
+/*  per-dim-row totals instead of LDS tables)                           */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ qk_u32 qk_probe(const qk_u64* table, qk_u32 lg, qk_u64 mask, qk_u32 key)
+{
+  qk_u64 slot = lg ? ((qk_u64)(key * 0x9E3779B1u) >> (32 - lg)) : 0;
+  for (qk_u64 p = 0; p <= mask; p++) {
+    qk_u64 e = table[slot];
+    if (e == 0ULL) return 0u;
+    if ((qk_u32)e == key) return (qk_u32)(e >> 32);
+    slot = (slot + 1) & mask;
+  }
+  return 0u;
+}
+
+extern "C" __global__ void LB k_join_probe(QK_Args a)
+{
+  extern __shared__ qk_u64 lds_u64[];
+  const qk_u64 rows = qk_push(a, QK_P_ROWS);
+  const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
+  const qk_u64 r1 = r0 + rows;
+  const qk_u32 lg = a.words[2];
+  const qk_u64 mask = (1ULL << lg) - 1;
+  const qk_u32 nargs = a.words[12] & 0xffu;
+  const qk_u32 slots = a.words[12] >> 8;
+  const qk_u32 types = a.words[13];
+  const qk_u32 range = a.words[11];
+  const qk_u32 flags = a.words[14] >> 16;
+  const bool per_row = (flags & 4u) != 0;
+  const qk_u64* table = qk_ptr<qk_u64>(a, 2);
+  qk_u64* g_cnt = qk_wptr<qk_u64>(a, 1);
+  qk_u64* g_sum = g_cnt + range;
+  qk_u32* g_min = (qk_u32*)(g_sum + (qk_u64)nargs * range);
+  qk_u32* g_max = g_min + (qk_u64)nargs * range;
+
+  qk_u64* l_sum = lds_u64;
+  qk_u32* l_cnt = (qk_u32*)(lds_u64 + ((flags & 1u) ? (qk_u64)nargs * range : 0));
+  qk_u32* l_min = l_cnt + range;
+  qk_u32* l_max = l_min + ((flags & 2u) ? (qk_u64)nargs * range : 0);
+  if (!per_row) {
+    const qk_u32 sum_words = (flags & 1u) ? nargs * range : 0u;
+    const qk_u32 mm_words = (flags & 2u) ? nargs * range : 0u;
+    for (qk_u32 i = threadIdx.x; i < sum_words; i += QK_WG) l_sum[i] = 0;
+    for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) l_cnt[i] = 0;
+    for (qk_u32 i = threadIdx.x; i < mm_words; i += QK_WG) {
+      l_min[i] = 0xffffffffu;
+      l_max[i] = 0u;
+    }
+    __syncthreads();
+  }
+  const qk_u32x4* keys = (const qk_u32x4*)a.ptr[QK_B_COL0];
+  qk_u64 b0, b1;
+  qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u32 m[QK_UNROLL];
+    qk_u32x4 kv[QK_UNROLL];
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+      qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
+      if (valid && qk_quad_skipped(a, q)) valid = 0u;
+      m[j] = valid;
+      kv[j] = valid ? keys[q] : qk_u32x4{0, 0, 0, 0};
+    }
+    qk_u32 pay[QK_UNROLL][4];
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) pay[j][k] = (m[j] & (1u << k)) ? qk_probe(table, lg, mask, w[k]) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u32 hit = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) hit |= pay[j][k] ? (1u << k) : 0u;
+      if (!hit) continue;
+      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!(hit & (1u << k))) continue;
+        qk_u32 g = pay[j][k] - 1u;
+        if (per_row) atomicAdd(&g_cnt[g], 1ULL);
+        else atomicAdd(&l_cnt[g], 1u);
+      }
+      for (qk_u32 c = 0; c < nargs; c++) {
+        qk_u32 at = (types >> (8 * c)) & 0xffu;
+        qk_u32x4 x = ((const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)])[q];
+        qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (!(hit & (1u << k))) continue;
+          qk_u32 g = pay[j][k] - 1u;
+          qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)w[k] : (qk_u64)w[k];
+          qk_u32 code = qk_code32(w[k], at);
+          if (per_row) {
+            if (flags & 1u) atomicAdd(&g_sum[(qk_u64)c * range + g], add);
+            if (flags & 2u) {
+              atomicMin(&g_min[(qk_u64)c * range + g], code);
+              atomicMax(&g_max[(qk_u64)c * range + g], code);
+            }
+          } else {
+            if (flags & 1u) atomicAdd(&l_sum[c * range + g], add);
+            if (flags & 2u) {
+              atomicMin(&l_min[c * range + g], code);
+              atomicMax(&l_max[c * range + g], code);
+            }
+          }
+        }
+      }
+    }
+  }
+  if (per_row) return;
+  __syncthreads();
+  for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) {
+    qk_u32 cnt = l_cnt[i];
+    if (!cnt) continue;
+    atomicAdd(&g_cnt[i], (qk_u64)cnt);
+    for (qk_u32 c = 0; c < nargs; c++) {
+      if (flags & 1u) atomicAdd(&g_sum[(qk_u64)c * range + i], l_sum[c * range + i]);
+      if (flags & 2u) {
+        atomicMin(&g_min[(qk_u64)c * range + i], l_min[c * range + i]);
+        atomicMax(&g_max[(qk_u64)c * range + i], l_max[c * range + i]);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* generators: splitmix64 addressed by output index                     */
+/* (bench_common.h:30-37): d(k) = mix(seed + (k+1) * 0x9E3779B97F4A7C15) */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ qk_u64 qk_splitmix64_at(qk_u64 seed, qk_u64 k)
+{
+  qk_u64 z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+/* gen_events: bindings 6 id, 7 region, 8 amount, 9 ts (absent = skip);
+ * push 0 rows, 1 seed, 2 first row.  region = d(3r)%50,
+ * amount = d(3r+1)%10000, ts = d(3r+2)>>32, id = r */
+extern "C" __global__ void LB k_gen_events(QK_Args a)
+{
+  qk_u64 n = qk_push(a, 0), seed = qk_push(a, 1), row0 = qk_push(a, 2);
+  qk_u32* id = (qk_u32*)a.ptr[6];
+  qk_u32* region = (qk_u32*)a.ptr[7];
+  qk_u32* amount = (qk_u32*)a.ptr[8];
+  qk_u32* ts = (qk_u32*)a.ptr[9];
+  for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
+    qk_u64 r = row0 + i;
+    if (id) id[i] = (qk_u32)r;
+    if (region) region[i] = (qk_u32)(qk_splitmix64_at(seed, 3 * r) % 50ULL);
+    if (amount) amount[i] = (qk_u32)(qk_splitmix64_at(seed, 3 * r + 1) % 10000ULL);
+    if (ts) ts[i] = (qk_u32)(qk_splitmix64_at(seed, 3 * r + 2) >> 32);
+  }
+}
+
+/* gen_scale_events: the closed-form table of bench_select_scale.c:39-45;
+ * bindings 6 id, 7 region, 8 amount, 9 needle key; push 0 rows, 2 first row */
+extern "C" __global__ void LB k_gen_scale_events(QK_Args a)
+{
+  qk_u64 n = qk_push(a, 0), row0 = qk_push(a, 2);
+  qk_u32* id = (qk_u32*)a.ptr[6];
+  qk_u32* region = (qk_u32*)a.ptr[7];
+  qk_u32* amount = (qk_u32*)a.ptr[8];
+  qk_u32* key = (qk_u32*)a.ptr[9];
+  for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
+    qk_u64 r = row0 + i;
+    if (id) id[i] = (qk_u32)(r + 1);
+    if (region) region[i] = (qk_u32)(r % 50);
+    if (amount) amount[i] = (qk_u32)(r % 10000);
+    if (key) key[i] = (qk_u32)(r * 2654435761ULL);
+  }
+}
+
+/* gen_fact_fk: fk[r] = dim_keys[d'(r) % dim_rows], d' seeded with
+ * seed ^ 0x5851F42D4C957F2D; binding 6 fk out, 0 dim keys;
+ * push 0 rows, 1 seed, 2 first row, 3 dim rows */
+extern "C" __global__ void LB k_gen_fact_fk(QK_Args a)
+{
+  qk_u64 n = qk_push(a, 0), seed = qk_push(a, 1) ^ 0x5851F42D4C957F2DULL, row0 = qk_push(a, 2), dn = qk_push(a, 3);
+  qk_u32* fk = (qk_u32*)a.ptr[6];
+  const qk_u32* dk = qk_ptr<qk_u32>(a, 0);
+  for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
+    fk[i] = dk[qk_splitmix64_at(seed, row0 + i) % dn];
+  }
+}
+
+/* zonemap_build: per zone of 2^shift rows, min and max as f64
+ * (CQL_ZoneMapChunk, cql.h:156-162).  binding 6 column, 1 mins, 2 maxs;
+ * push 0 rows, words[2] type, words[3] shift.  One block per zone. */
+extern "C" __global__ void LB k_zonemap_build(QK_Args a)
+{
+  __shared__ double s_mn[QK_WG / 64], s_mx[QK_WG / 64];
+  qk_u64 n = qk_push(a, 0);
+  qk_u32 type = a.words[2], shift = a.words[3];
+  qk_u64 zones = (n + (1ULL << shift) - 1) >> shift;
+  for (qk_u64 z = blockIdx.x; z < zones; z += gridDim.x) {
+    qk_u64 z0 = z << shift, z1 = min(n, (z + 1) << shift);
+    double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
+    for (qk_u64 r = z0 + threadIdx.x; r < z1; r += QK_WG) {
+      double v = qk_load1_f64(a, 0, type, r);
+      mn = min(mn, v);
+      mx = max(mx, v);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = min(mn, __shfl_xor(mn, o, 64));
+      mx = max(mx, __shfl_xor(mx, o, 64));
+    }
+    __syncthreads();
+    if (__lane_id() == 0) {
+      s_mn[threadIdx.x >> 6] = mn;
+      s_mx[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < QK_WG / 64; w++) {
+        mn = min(mn, s_mn[w]);
+        mx = max(mx, s_mx[w]);
+      }
+      mn = min(mn, s_mn[0]);
+      mx = max(mx, s_mx[0]);
+      ((double*)a.ptr[1])[z] = mn;
+      ((double*)a.ptr[2])[z] = mx;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* registry                                                             */
+/* ------------------------------------------------------------------ */
+static const QK_LibraryKernel g_library[] = {
+    {"fill_u32", (const void*)k_fill_u32, QK_WG},
+    {"bandwidth_read", (const void*)k_bandwidth_read, QK_WG},
+    {"aggregate_global", (const void*)k_aggregate_global, QK_WG},
+    {"scan_simple", (const void*)k_scan_simple, QK_WG},
+    {"scan_filter", (const void*)k_scan_filter, QK_WG},
+    {"aggregate_dense", (const void*)k_aggregate_dense, QK_WG},
+    {"join_build", (const void*)k_join_build, QK_WG},
+    {"join_probe", (const void*)k_join_probe, QK_WG},
+    {"gen_events", (const void*)k_gen_events, QK_WG},
+    {"gen_scale_events", (const void*)k_gen_scale_events, QK_WG},
+    {"gen_fact_fk", (const void*)k_gen_fact_fk, QK_WG},
+    {"zonemap_build", (const void*)k_zonemap_build, QK_WG},
+};
+
+extern "C" const QK_LibraryKernel* qk_library_find(const char* name)
+{
+  for (unsigned i = 0; i < sizeof(g_library) / sizeof(g_library[0]); i++) {
+    const char* a = g_library[i].name;
+    const char* b = name;
+    while (*a && *a == *b) {
+      a++;
+      b++;
+    }
+    if (*a == 0 && *b == 0) return &g_library[i];
+  }
+  return nullptr;
+}
+
+extern "C" U32 qk_library_count(void) { return (U32)(sizeof(g_library) / sizeof(g_library[0])); }
+
+extern "C" const QK_LibraryKernel* qk_library_at(U32 index)
+{
+  return index < qk_library_count() ? &g_library[index] : nullptr;
+}

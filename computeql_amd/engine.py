@@ -1,0 +1,238 @@
+"""Host-side mirror of ComputeQL's operator interface for the hot path.
+
+Thin Python over the C ABI (include/cqlhip.h): device-resident tables, the
+scan/filter operator, global and grouped aggregates and the join+aggregate
+pushdown.  Every call goes to the HIP library; an error in the library
+raises `QueryError` with its message (the reference's GPU operators return
+0 to their caller in the same situations).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import ir
+from .native import QE_Trace, load
+
+U32, U64, F32, F64, I32, I64 = 1, 2, 3, 4, 7, 8
+COUNT, SUM, AVG, MIN, MAX = 0, 1, 2, 3, 4
+FUNC_NAMES = {COUNT: "COUNT", SUM: "SUM", AVG: "AVG", MIN: "MIN", MAX: "MAX"}
+_NP = {U32: np.uint32, U64: np.uint64, F32: np.float32, F64: np.float64, I32: np.int32, I64: np.int64}
+SCAN_PATHS = {0: "none", 1: "generated_simple", 2: "generated", 3: "simple", 4: "interpreter", 5: "zone_empty",
+              6: "constant"}
+AGG_PATHS = {10: "global", 11: "fused_global", 12: "dense", 13: "dense_key_fold", 14: "fused_dense", 20: "join_dense",
+             21: "join_probe_group", 22: "join_probe_row", 30: "hash"}
+
+
+class QueryError(RuntimeError):
+    pass
+
+
+def _err(what: str) -> QueryError:
+    lib = load()
+    return QueryError(f"{what}: {lib.qe_last_error().decode(errors='replace')}")
+
+
+def set_setting(key: str, value: int) -> None:
+    load().qe_settings_set_u64(key.encode(), int(value))
+
+
+def get_setting(key: str, default: int = 0) -> int:
+    return int(load().qe_settings_u64(key.encode(), int(default)))
+
+
+def reset_settings() -> None:
+    load().qe_settings_reset()
+
+
+def profile(enabled: bool = True) -> None:
+    load().gpu_hip_profile_enable(1 if enabled else 0)
+
+
+def compute_units() -> int:
+    return int(load().gpu_hip_compute_units())
+
+
+@dataclass
+class AggResult:
+    funcs: list
+    keys: np.ndarray
+    cells: np.ndarray  # uint64 [groups, exprs]: COUNT as U64, others F64 bits
+    trace: dict = field(default_factory=dict)
+
+    @property
+    def groups(self) -> int:
+        return len(self.keys)
+
+    def column(self, e: int) -> np.ndarray:
+        c = self.cells[:, e]
+        return c.copy() if self.funcs[e] == COUNT else c.view(np.float64).copy()
+
+    def rows(self) -> list:
+        out = []
+        for g in range(self.groups):
+            row = [float(self.keys[g])]
+            for e, f in enumerate(self.funcs):
+                v = self.cells[g, e]
+                row.append(int(v) if f == COUNT else float(np.array([v], dtype=np.uint64).view(np.float64)[0]))
+            out.append(tuple(row))
+        return out
+
+
+class Table:
+    """A table whose columns live whole in HBM (CQL_Table, cql.h:206-322)."""
+
+    def __init__(self, rows: int):
+        self._lib = load()
+        self.handle = self._lib.qe_table_alloc(int(rows))
+        if not self.handle:
+            raise QueryError("qe_table_alloc failed")
+        self.rows = int(rows)
+        self.names: list[str] = []
+        self.types: list[int] = []
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.qe_table_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_column(self, name: str, ctype: int, data: np.ndarray | None = None) -> int:
+        ptr = None
+        if data is not None:
+            data = np.ascontiguousarray(data, dtype=_NP[ctype])
+            if data.shape != (self.rows,):
+                raise ValueError(f"column {name}: expected {self.rows} rows, got {data.shape}")
+            ptr = data.ctypes.data
+        idx = self._lib.qe_table_add_column(self.handle, name.encode(), ctype, ptr, None)
+        if idx < 0:
+            raise _err("add_column")
+        self.names.append(name)
+        self.types.append(ctype)
+        return int(idx)
+
+    def col(self, name: str) -> ir.Column:
+        return ir.col(self.names.index(name))
+
+    def index(self, name: str) -> int:
+        return self.names.index(name)
+
+    def read(self, column: int | str, row0: int = 0, rows: int | None = None) -> np.ndarray:
+        c = self.index(column) if isinstance(column, str) else column
+        n = self.rows - row0 if rows is None else rows
+        out = np.empty(n, dtype=_NP[self.types[c]])
+        if not self._lib.qe_table_column_read(self.handle, c, row0, n, out.ctypes.data):
+            raise _err("column_read")
+        return out
+
+    def build_zonemap(self, column: int) -> None:
+        if not self._lib.qe_table_build_zonemap(self.handle, column):
+            raise _err("build_zonemap")
+
+    # ---- generators (data produced in HBM) ----
+    def gen_events(self, seed: int, id=-1, region=-1, amount=-1, ts=-1) -> None:
+        if not self._lib.qe_gen_events(self.handle, seed & (2**64 - 1), id, region, amount, ts):
+            raise _err("gen_events")
+
+    def gen_scale_events(self, id=-1, region=-1, amount=-1, key=-1) -> None:
+        if not self._lib.qe_gen_scale_events(self.handle, id, region, amount, key):
+            raise _err("gen_scale_events")
+
+    def gen_dim(self, seed: int, attr_range: int, spread_keys: bool = True, key: int = 0, attr: int = -1) -> None:
+        if not self._lib.qe_gen_dim(self.handle, seed & (2**64 - 1), attr_range, 1 if spread_keys else 0, key, attr):
+            raise _err("gen_dim")
+
+    def gen_fact_fk(self, fk_col: int, seed: int, dim: "Table", dim_key_col: int) -> None:
+        if not self._lib.qe_gen_fact_fk(self.handle, fk_col, seed & (2**64 - 1), dim.handle, dim_key_col):
+            raise _err("gen_fact_fk")
+
+    # ---- operators ----
+    def scan(self, where, count_only: bool = False, cap: int | None = None):
+        """qe_scan_filter_selected: returns (count, ids or None, trace)."""
+        nodes = ir.to_nodes(where)
+        nptr = nodes.ctypes.data if nodes is not None else None
+        ncount = len(nodes) if nodes is not None else 0
+        cnt = C.c_uint64(0)
+        tr = QE_Trace()
+        if count_only:
+            ok = self._lib.qe_scan_filter(self.handle, nptr, ncount, 1, None, 0, C.byref(cnt), C.byref(tr))
+            if not ok:
+                raise _err("scan")
+            return int(cnt.value), None, tr.as_dict()
+        cap = self.rows if cap is None else cap
+        ids = np.empty(max(cap, 1), dtype=np.uint64)
+        ok = self._lib.qe_scan_filter(self.handle, nptr, ncount, 0, ids.ctypes.data, cap, C.byref(cnt), C.byref(tr))
+        if not ok:
+            raise _err("scan")
+        n = int(cnt.value)
+        return n, ids[: min(n, cap)], tr.as_dict()
+
+    def count(self, where) -> int:
+        return self.scan(where, count_only=True)[0]
+
+    def aggregate(self, funcs, args, where=None, group: int | None = None, cap: int | None = None) -> AggResult:
+        """qe_aggregate_impl for the hot-path shapes (args[e] < 0 or None = COUNT(*))."""
+        funcs = list(funcs)
+        e = len(funcs)
+        fa = (C.c_uint32 * e)(*funcs)
+        aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in args])
+        nodes = ir.to_nodes(where)
+        nptr = nodes.ctypes.data if nodes is not None else None
+        ncount = len(nodes) if nodes is not None else 0
+        if cap is None:
+            cap = 1 if group is None else 1 << 17
+        keys = np.zeros(max(cap, 1), dtype=np.float64)
+        cells = np.zeros((max(cap, 1), e), dtype=np.uint64)
+        g = C.c_uint64(0)
+        tr = QE_Trace()
+        ok = self._lib.qe_aggregate(self.handle, nptr, ncount, -1 if group is None else int(group), fa, aa, e, cap,
+                                    C.byref(g), keys.ctypes.data, cells.ctypes.data, C.byref(tr))
+        if not ok:
+            raise _err("aggregate")
+        n = int(g.value)
+        if n > cap:
+            return self.aggregate(funcs, args, where, group, cap=n)
+        return AggResult(funcs, keys[:n].copy(), cells[:n].copy(), tr.as_dict())
+
+
+def aggregate_join(fact: Table, fact_key: int, dim: Table, dim_key: int, funcs, fact_args, dim_where=None,
+                   dim_group: int | None = None, cap: int = 1 << 16) -> AggResult:
+    """qe_aggregate_join_pushdown: aggregates of fact rows grouped by a dim column."""
+    lib = load()
+    funcs = list(funcs)
+    e = len(funcs)
+    fa = (C.c_uint32 * e)(*funcs)
+    aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in fact_args])
+    nodes = ir.to_nodes(dim_where)
+    nptr = nodes.ctypes.data if nodes is not None else None
+    ncount = len(nodes) if nodes is not None else 0
+    keys = np.zeros(max(cap, 1), dtype=np.float64)
+    cells = np.zeros((max(cap, 1), e), dtype=np.uint64)
+    g = C.c_uint64(0)
+    tr = QE_Trace()
+    ok = lib.qe_aggregate_join(fact.handle, fact_key, dim.handle, dim_key, nptr, ncount,
+                               -1 if dim_group is None else int(dim_group), fa, aa, e, cap, C.byref(g),
+                               keys.ctypes.data, cells.ctypes.data, C.byref(tr))
+    if not ok:
+        raise _err("aggregate_join")
+    n = int(g.value)
+    if n > cap:
+        return aggregate_join(fact, fact_key, dim, dim_key, funcs, fact_args, dim_where, dim_group, cap=n)
+    return AggResult(funcs, keys[:n].copy(), cells[:n].copy(), tr.as_dict())
+
+
+def bandwidth_read(table: Table, column: int, blocks: int = 0) -> dict:
+    lib = load()
+    buf = lib.qe_table_column_buffer(table.handle, column)
+    sink = C.c_uint64(0)
+    tr = QE_Trace()
+    if not lib.qe_bandwidth_read(buf, table.rows * 4, blocks, C.byref(sink), C.byref(tr)):
+        raise _err("bandwidth_read")
+    return tr.as_dict()

@@ -1,0 +1,362 @@
+/*
+ * cqlhip.h - C ABI of the MI355X (gfx950) HIP execution backend for
+ * ComputeQL's scan+filter -> group-by/aggregate -> hash join path.
+ *
+ * Two layers, both plain C (no HIP or torch types in any signature):
+ *
+ *  1. The GPU_Backend function table (gpu_backend.h:17-71 of the reference)
+ *     implemented over HIP: buffers, library kernels, run-time compiled
+ *     kernels, batches and stream slots.  A ComputeQL build registers it as
+ *     GPU_BackendKind_HIP (gpu_backend.h:8, gpu_inc.h:6-19) with
+ *       gpu_backend_register(GPU_BackendKind_HIP, str8_lit("hip"), gpu_hip_backend());
+ *     and selects it with --gpu=hip (main.c:135-139).
+ *
+ *  2. The operator entry points of query_exec.c for this path (scan/filter,
+ *     global and grouped aggregates, join+aggregate), driving (1) the same
+ *     way the reference's host code drives its Vulkan backend.  Each cites
+ *     the reference function it replaces.
+ *
+ * All reference citations are file:line in the ComputeQL source tree.
+ */
+#ifndef CQLHIP_H
+#define CQLHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* base types of base_core.h */
+typedef uint8_t  U8;
+typedef uint16_t U16;
+typedef uint32_t U32;
+typedef uint64_t U64;
+typedef int32_t  S32;
+typedef int64_t  S64;
+typedef uint32_t B32;
+typedef float    F32;
+typedef double   F64;
+
+typedef struct String8 { U8* str; U64 size; } String8;
+
+/* ------------------------------------------------------------------ */
+/* 1. GPU_Backend (gpu.h:10-159, gpu_backend.h:17-71)                  */
+/* ------------------------------------------------------------------ */
+typedef enum GPU_BufferFlags {
+  GPU_BufferFlag_Read = (1 << 0),
+  GPU_BufferFlag_Write = (1 << 1),
+  GPU_BufferFlag_ReadWrite = (1 << 2),
+  GPU_BufferFlag_HostVisible = (1 << 3),
+  GPU_BufferFlag_DeviceLocal = (1 << 4),
+  GPU_BufferFlag_HostCached = (1 << 5),
+  GPU_BufferFlag_ZeroCopy = (1 << 6),
+  GPU_BufferFlag_CopyHostPointer = (1 << 7),
+  GPU_BufferFlag_Dynamic = (1 << 8),
+  GPU_BufferFlag_Shared = (1 << 9),
+} GPU_BufferFlags;
+
+typedef struct GPU_MemoryStats { /* gpu.h:25-36 */
+  U64 bytes_in_use;
+  U64 bytes_peak;
+  U64 alloc_failures;
+  U64 alloc_retry_successes;
+  U64 pool_evictions;
+  U64 pool_evicted_bytes;
+  U64 alloc_calls;
+  U64 unrecovered_alloc_failures;
+} GPU_MemoryStats;
+
+typedef struct GPU_StreamStats { /* gpu.h:104-111 */
+  U64 bytes_copied;
+  U64 imports_created;
+  U64 import_microseconds;
+  U64 staged_bytes;
+  B32 separate_copy_queue;
+} GPU_StreamStats;
+
+typedef struct GPU_Buffer GPU_Buffer;
+typedef struct GPU_Kernel GPU_Kernel;
+typedef struct GPU_Batch GPU_Batch;
+typedef struct GPU_Stream GPU_Stream;
+
+/* Same member order as struct GPU_Backend, gpu_backend.h:17-71.
+ * scratch_arena returns the backend's host scratch arena (opaque here). */
+typedef struct GPU_Backend {
+  U32 struct_size;
+  void (*init)(void);
+  void (*release)(void);
+  void (*wait)(void);
+  U64 (*get_executed_kernel_time_microseconds)(void);
+  U64 (*device_total_memory)(void);
+  U64 (*device_free_memory)(void);
+  U64 (*device_max_storage_buffer_range)(void);
+  B32 (*device_lost)(void);
+  void (*pool_begin_epoch)(void);
+  GPU_MemoryStats (*memory_stats)(void);
+  GPU_Buffer* (*buffer_alloc)(U64 size, GPU_BufferFlags flags, void* data);
+  GPU_Buffer* (*buffer_alloc_pooled)(String8 name, U64 size, GPU_BufferFlags flags, void* data);
+  GPU_Buffer* (*buffer_import_host_readonly)(void* host_ptr, U64 size);
+  GPU_Buffer* (*buffer_import_host_readonly_pooled)(String8 name, void* host_ptr, U64 size);
+  void (*buffer_release)(GPU_Buffer* buffer);
+  void (*buffer_write)(GPU_Buffer* buffer, void* data, U64 size);
+  void (*buffer_read)(GPU_Buffer* buffer, void* data, U64 size);
+  GPU_Kernel* (*kernel_alloc)(String8 name);
+  GPU_Kernel* (*kernel_alloc_from_spirv)(String8 name, String8 spirv);
+  GPU_Kernel* (*kernel_alloc_from_spirv_spec)(String8 name, String8 spirv, U32* spec_values, U32 spec_count);
+  void (*generated_kernel_stats)(U64* out_compiles, U64* out_evictions);
+  void (*kernel_release)(GPU_Kernel* kernel);
+  void (*kernel_execute)(GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size);
+  void (*kernel_set_arg_buffer)(GPU_Kernel* kernel, U32 index, GPU_Buffer* buffer);
+  void (*kernel_set_arg_u64)(GPU_Kernel* kernel, U32 index, U64 value);
+  GPU_Batch* (*batch_begin)(U64 upload_bytes_needed, U64 download_bytes_needed);
+  void (*batch_buffer_write)(GPU_Batch* batch, GPU_Buffer* buffer, void* data, U64 size);
+  void (*batch_buffer_zero)(GPU_Batch* batch, GPU_Buffer* buffer, U64 size);
+  void (*batch_buffer_fill)(GPU_Batch* batch, GPU_Buffer* buffer, U64 size, U32 value);
+  void (*batch_kernel_execute)(GPU_Batch* batch, GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size);
+  void (*batch_buffer_read)(GPU_Batch* batch, GPU_Buffer* buffer, void* out_data, U64 size);
+  B32 (*batch_end)(GPU_Batch* batch);
+  GPU_Stream* (*stream_alloc)(U32 slot_count);
+  void (*stream_release)(GPU_Stream* stream);
+  B32 (*stream_slot_begin)(GPU_Stream* stream, U32 slot, U64 upload_bytes, U64 readback_bytes);
+  B32 (*stream_copy)(GPU_Stream* stream, U32 slot, GPU_Buffer* destination, void* host_ptr, U64 size);
+  void (*stream_zero)(GPU_Stream* stream, U32 slot, GPU_Buffer* buffer, U64 size);
+  void (*stream_fill)(GPU_Stream* stream, U32 slot, GPU_Buffer* buffer, U64 size, U32 value);
+  void (*stream_dispatch)(GPU_Stream* stream, U32 slot, GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size);
+  void (*stream_read)(GPU_Stream* stream, U32 slot, GPU_Buffer* buffer, void* out_data, U64 size);
+  B32 (*stream_submit)(GPU_Stream* stream, U32 slot);
+  B32 (*stream_wait)(GPU_Stream* stream, U32 slot);
+  GPU_StreamStats (*stream_get_stats)(GPU_Stream* stream);
+  void* (*scratch_arena)(void);
+  String8 (*name)(void);
+} GPU_Backend;
+
+/* The filled table; replaces the Vulkan registration of gpu_dispatch.c:1-28. */
+GPU_Backend* gpu_hip_backend(void);
+
+/* The individual entries (gpu_vulkan.c counterparts named in the .cpp). */
+void gpu_hip_init(void);
+void gpu_hip_release(void);
+void gpu_hip_wait(void);
+U64 gpu_hip_get_executed_kernel_time_microseconds(void);
+U64 gpu_hip_device_total_memory(void);
+U64 gpu_hip_device_free_memory(void);
+U64 gpu_hip_device_max_storage_buffer_range(void);
+B32 gpu_hip_device_lost(void);
+void gpu_hip_pool_begin_epoch(void);
+GPU_MemoryStats gpu_hip_memory_stats(void);
+GPU_Buffer* gpu_hip_buffer_alloc(U64 size, GPU_BufferFlags flags, void* data);
+GPU_Buffer* gpu_hip_buffer_alloc_pooled(String8 name, U64 size, GPU_BufferFlags flags, void* data);
+GPU_Buffer* gpu_hip_buffer_import_host_readonly(void* host_ptr, U64 size);
+GPU_Buffer* gpu_hip_buffer_import_host_readonly_pooled(String8 name, void* host_ptr, U64 size);
+void gpu_hip_buffer_release(GPU_Buffer* buffer);
+void gpu_hip_buffer_write(GPU_Buffer* buffer, void* data, U64 size);
+void gpu_hip_buffer_read(GPU_Buffer* buffer, void* data, U64 size);
+GPU_Kernel* gpu_hip_kernel_alloc(String8 name);
+GPU_Kernel* gpu_hip_kernel_alloc_from_spirv(String8 name, String8 code_object);
+GPU_Kernel* gpu_hip_kernel_alloc_from_spirv_spec(String8 name, String8 code_object, U32* spec_values, U32 spec_count);
+void gpu_hip_generated_kernel_stats(U64* out_compiles, U64* out_evictions);
+void gpu_hip_kernel_release(GPU_Kernel* kernel);
+void gpu_hip_kernel_execute(GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size);
+void gpu_hip_kernel_set_arg_buffer(GPU_Kernel* kernel, U32 index, GPU_Buffer* buffer);
+void gpu_hip_kernel_set_arg_u64(GPU_Kernel* kernel, U32 index, U64 value);
+GPU_Batch* gpu_hip_batch_begin(U64 upload_bytes_needed, U64 download_bytes_needed);
+void gpu_hip_batch_buffer_write(GPU_Batch* batch, GPU_Buffer* buffer, void* data, U64 size);
+void gpu_hip_batch_buffer_zero(GPU_Batch* batch, GPU_Buffer* buffer, U64 size);
+void gpu_hip_batch_buffer_fill(GPU_Batch* batch, GPU_Buffer* buffer, U64 size, U32 value);
+void gpu_hip_batch_kernel_execute(GPU_Batch* batch, GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size);
+void gpu_hip_batch_buffer_read(GPU_Batch* batch, GPU_Buffer* buffer, void* out_data, U64 size);
+B32 gpu_hip_batch_end(GPU_Batch* batch);
+GPU_Stream* gpu_hip_stream_alloc(U32 slot_count);
+void gpu_hip_stream_release(GPU_Stream* stream);
+B32 gpu_hip_stream_slot_begin(GPU_Stream* stream, U32 slot, U64 upload_bytes, U64 readback_bytes);
+B32 gpu_hip_stream_copy(GPU_Stream* stream, U32 slot, GPU_Buffer* destination, void* host_ptr, U64 size);
+void gpu_hip_stream_zero(GPU_Stream* stream, U32 slot, GPU_Buffer* buffer, U64 size);
+void gpu_hip_stream_fill(GPU_Stream* stream, U32 slot, GPU_Buffer* buffer, U64 size, U32 value);
+void gpu_hip_stream_dispatch(GPU_Stream* stream, U32 slot, GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size);
+void gpu_hip_stream_read(GPU_Stream* stream, U32 slot, GPU_Buffer* buffer, void* out_data, U64 size);
+B32 gpu_hip_stream_submit(GPU_Stream* stream, U32 slot);
+B32 gpu_hip_stream_wait(GPU_Stream* stream, U32 slot);
+GPU_StreamStats gpu_hip_stream_get_stats(GPU_Stream* stream);
+void* gpu_hip_scratch_arena(void);
+String8 gpu_hip_name(void);
+
+/* HIP-specific extensions (no Vulkan counterpart). */
+U32 gpu_hip_compute_units(void);          /* CUs of the device (256 on MI355X) */
+U64 gpu_hip_buffer_size(GPU_Buffer* b);   /* bytes of a buffer */
+U64 gpu_hip_buffer_device_address(GPU_Buffer* b); /* device pointer as an integer */
+/* dynamic LDS bytes for the next dispatches of a kernel (the Vulkan side
+ * sizes shared memory with specialization constants instead) */
+void gpu_hip_kernel_set_lds_bytes(GPU_Kernel* kernel, U32 bytes);
+/* leave a binding intentionally empty (address 0) without the missing-arg
+ * mark that binding a null buffer sets */
+void gpu_hip_kernel_clear_arg(GPU_Kernel* kernel, U32 index);
+/* set one 32-bit push word (the half of U64 slot word/2) */
+void gpu_hip_kernel_set_arg_u32(GPU_Kernel* kernel, U32 word, U32 value);
+U32 gpu_hip_max_lds_bytes(void);          /* LDS a workgroup may use */
+/* kernel-time profiling: hipEvents around every dispatch of `name` (a
+ * trailing * matches a prefix)
+ * (gpu.h:38-66 g_gpu_kernel_profile).  Returns dispatches and total ns. */
+void gpu_hip_profile_enable(B32 enabled);
+void gpu_hip_profile_reset(void);
+U64 gpu_hip_profile_query(const char* name, U64* out_dispatches);
+/* prefix "tag:" to the profile names of this thread's next dispatches
+ * ("" clears), so one kernel used by several operators is told apart */
+void gpu_hip_profile_tag(const char* tag);
+/* fault injection: fail the k-th device allocation from now (gpu.h:68) */
+void gpu_hip_fault_alloc_at(U64 k);
+const char* gpu_hip_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Kernel argument block (the launch ABI of every library and run-time */
+/* compiled kernel).  The reference binds up to 16 storage buffers by  */
+/* index and 8 U64 push constants read as 16 u32 words               */
+/* (gpu_vulkan.h:6-9); a HIP kernel takes them by value, with each     */
+/* buffer's byte size, so shaders' .length() reads carry over.         */
+/* ------------------------------------------------------------------ */
+#define QK_MAX_BINDINGS 16
+#define QK_PUSH_WORDS 16
+typedef struct QK_Args {
+  U64 ptr[QK_MAX_BINDINGS];    /* device addresses by binding index */
+  U64 bytes[QK_MAX_BINDINGS];  /* byte sizes by binding index */
+  U32 words[QK_PUSH_WORDS];    /* push constants: U64 slot i = words[2i] | words[2i+1] << 32 */
+  U32 global_work_size;        /* threads asked for by the dispatch */
+  U32 grid_blocks;             /* workgroups launched */
+} QK_Args;
+
+/* ------------------------------------------------------------------ */
+/* 2. Operators (query_exec.c hot path)                                 */
+/* ------------------------------------------------------------------ */
+
+/* column types: CQL_ColumnType numbering, cql.h:78-95 */
+enum {
+  CQL_ColumnType_U32 = 1, CQL_ColumnType_U64 = 2, CQL_ColumnType_F32 = 3, CQL_ColumnType_F64 = 4,
+  CQL_ColumnType_I32 = 7, CQL_ColumnType_I64 = 8,
+};
+
+/* aggregate function codes, query_exec.h:904-910; result types follow
+ * qe_agg_func_policy (query_exec.h:917-926): COUNT -> U64, others F64 */
+enum { QE_AGG_FUNC_COUNT = 0, QE_AGG_FUNC_SUM = 1, QE_AGG_FUNC_AVG = 2, QE_AGG_FUNC_MIN = 3, QE_AGG_FUNC_MAX = 4 };
+#define QE_AGG_MAX_EXPRS 8
+
+/* WHERE tree as the planner hands it over (IR_Node, flattened in prefix
+ * order with child counts).  OPERATOR nodes carry an op; COLUMN nodes a
+ * column index of the scanned table; NUMERIC nodes a literal; INLIST holds
+ * NUMERIC children (the right side of IN / NOT IN). */
+enum { QE_IR_OPERATOR = 1, QE_IR_COLUMN = 2, QE_IR_NUMERIC = 3, QE_IR_INLIST = 4 };
+enum {
+  QE_IROP_AND = 1, QE_IROP_OR = 2, QE_IROP_EQ = 3, QE_IROP_NE = 4, QE_IROP_LT = 5, QE_IROP_GT = 6,
+  QE_IROP_LE = 7, QE_IROP_GE = 8, QE_IROP_IS_NULL = 9, QE_IROP_IS_NOT_NULL = 10, QE_IROP_IN = 11,
+  QE_IROP_NOT_IN = 12,
+};
+typedef struct QE_IRNode {
+  U32 type;
+  U32 op;
+  U32 column;
+  U32 child_count;
+  F64 value;
+} QE_IRNode;
+
+/* scan kernel choice, in the order of qe_scan_filter_selected's chain
+ * (query_exec.c:2566-2656) */
+enum {
+  QE_ScanPath_None = 0,
+  QE_ScanPath_GeneratedSimple = 1, /* qk_simple_kernel via hiprtc */
+  QE_ScanPath_Generated = 2,       /* qk_scan_kernel via hiprtc */
+  QE_ScanPath_Simple = 3,          /* library scan_simple */
+  QE_ScanPath_Interpreter = 4,     /* library scan_filter bytecode interpreter */
+  QE_ScanPath_ZoneEmpty = 5,       /* every zone pruned, nothing dispatched */
+  QE_ScanPath_Constant = 6,        /* predicate folded to a constant */
+};
+
+/* per-operator trace (QE_ScanTrace/QE_AggregateTrace, query_exec.h:133-311) */
+typedef struct QE_Trace {
+  U32 path;              /* QE_ScanPath_* for scans, QE_AggPath_* for aggregates */
+  U32 kernel_count;      /* dispatches made */
+  U64 rows_before;
+  U64 rows_after;
+  U64 rows_scanned;      /* rows left after zone-map pruning */
+  U64 bytes_read;        /* algorithmic column bytes the kernels read */
+  U64 gpu_kernel_ns;     /* summed hipEvent time of the dispatches (profiling on) */
+  U64 wall_ns;           /* prepare + execute + fetch */
+  U64 compile_ns;        /* hiprtc time spent in this call */
+  char kernel[64];       /* name of the main kernel */
+} QE_Trace;
+
+enum {
+  QE_AggPath_None = 0,
+  QE_AggPath_Global = 10,        /* aggregate_global (whole columns) */
+  QE_AggPath_FusedGlobal = 11,   /* WHERE + global aggregate in one kernel */
+  QE_AggPath_Dense = 12,         /* dense GROUP BY, LDS-private tables */
+  QE_AggPath_DenseKeyFold = 13,  /* dense GROUP BY whose args are the key */
+  QE_AggPath_FusedDense = 14,    /* WHERE + dense GROUP BY */
+  QE_AggPath_JoinDense = 20,     /* join: fact reduced per key value, combined per dim row */
+  QE_AggPath_JoinProbeGroup = 21,/* join: hash probe with the dim group code as payload */
+  QE_AggPath_JoinProbeRow = 22,  /* join: hash probe, totals per dim row */
+  QE_AggPath_Hash = 30,          /* hash GROUP BY, global open addressing */
+};
+
+typedef struct QE_Table QE_Table;
+
+/* settings (settings.c:158-242): key:value switches read at use sites */
+void qe_settings_set_u64(const char* key, U64 value);
+U64 qe_settings_u64(const char* key, U64 default_value);
+void qe_settings_reset(void);
+
+const char* qe_last_error(void);
+
+/* device-resident tables (CQL_Table/CQL_Column, cql.h:206-322, with the
+ * columns held whole in HBM; zone maps of CQL_ZONEMAP_CHUNK_ROWS rows,
+ * cql.h:70-71, cql.c:3818) */
+QE_Table* qe_table_alloc(U64 row_count);
+void qe_table_release(QE_Table* table);
+U64 qe_table_row_count(QE_Table* table);
+/* returns the column index or -1; host_data may be 0 (column left on the
+ * device uninitialised, for the generators); null_flags: byte per row,
+ * nonzero = NULL (cql.h:283) */
+S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void* host_data,
+                        const U8* null_flags);
+GPU_Buffer* qe_table_column_buffer(QE_Table* table, U32 column);
+B32 qe_table_column_read(QE_Table* table, U32 column, U64 row0, U64 rows, void* out);
+/* recompute zone maps (8192-row min/max) of a column on the device */
+B32 qe_table_build_zonemap(QE_Table* table, U32 column);
+
+/* synthetic tables generated in HBM (same definitions as the oracle's) */
+B32 qe_gen_events(QE_Table* table, U64 seed, S32 id_col, S32 region_col, S32 amount_col, S32 ts_col);
+B32 qe_gen_scale_events(QE_Table* table, S32 id_col, S32 region_col, S32 amount_col, S32 key_col);
+B32 qe_gen_fact_fk(QE_Table* table, U32 fk_col, U64 seed, QE_Table* dim, U32 dim_key_col);
+B32 qe_gen_dim(QE_Table* table, U64 seed, U32 attr_range, B32 spread_keys, S32 key_col, S32 attr_col);
+
+/* qe_scan_filter_selected (query_exec.c:2495-3273): WHERE over the table;
+ * count_only returns just the count; otherwise row ids (U64) are written to
+ * out_ids (cap entries), sorted ascending. */
+B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only,
+                   U64* out_ids, U64 cap, U64* out_count, QE_Trace* trace);
+
+/* qe_aggregate_impl (query_exec.c:10258) for the hot-path shapes: optional
+ * WHERE, no key or one GROUP BY column, up to 8 aggregates (arg_cols[e] < 0
+ * = COUNT(*)).  Writes out_group_count groups: out_keys[g] (key value as
+ * F64), out_cells[g*E+e] (U64 bits for COUNT, F64 bits otherwise).  Groups
+ * are in ascending key order. */
+B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 group_col,
+                 const U32* funcs, const S32* arg_cols, U32 expr_count, U64 cap,
+                 U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace);
+
+/* qe_aggregate_join_pushdown (query_exec.c:8442-8775):
+ *   SELECT [dim.g,] aggs(fact.*) FROM fact JOIN dim ON fact.k = dim.k
+ *   [WHERE dim-predicate] [GROUP BY dim.g]
+ * dim_group_col < 0 means one group.  Output as qe_aggregate, groups in
+ * ascending group value order. */
+B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_key_col,
+                      const QE_IRNode* dim_where, U32 dim_where_count, S32 dim_group_col,
+                      const U32* funcs, const S32* fact_arg_cols, U32 expr_count, U64 cap,
+                      U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace);
+
+/* bandwidth probe (bandwidth_read.comp): reads `bytes` of a buffer with
+ * 16-byte loads; returns a sink value.  For measuring the HBM ceiling. */
+B32 qe_bandwidth_read(GPU_Buffer* buffer, U64 bytes, U32 blocks, U64* out_sink, QE_Trace* trace);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,330 @@
+"""GPU parity: every operator through the C ABI against the CPU oracle on
+the same seeded tables.  Integer results bit-exact; group order normalised
+by key (the CPU path emits groups in first-appearance order, the dense GPU
+path in key order)."""
+import numpy as np
+import pytest
+
+import computeql_amd as cq
+from computeql_amd import ir
+from computeql_amd.ir import col
+
+pytestmark = pytest.mark.gpu
+
+N = 1_000_003  # ragged: not a multiple of 4, 64 or the 8192-row zone
+
+
+def _f64(cells):
+    return cells.view(np.float64)
+
+
+@pytest.fixture(scope="module")
+def events(orc):
+    t = cq.Table(N)
+    for name in ("id", "region", "amount", "ts"):
+        t.add_column(name, cq.U32)
+    t.gen_events(orc.SEED, id=0, region=1, amount=2, ts=3)
+    host = orc.gen_events(N, seed=orc.SEED)
+    yield t, host
+    t.close()
+
+
+def test_generator_matches_oracle(events):
+    t, host = events
+    for c in range(4):
+        assert np.array_equal(t.read(c), host[c]), c
+
+
+def test_generator_offset_rows(orc):
+    t = cq.Table(10)
+    t.add_column("amount", cq.U32)
+    t.gen_events(orc.SEED, amount=0)
+    assert np.array_equal(t.read(0), orc.gen_events(10, seed=orc.SEED)[2])
+
+
+SCAN_CASES = [
+    col(2) < 200,
+    col(2) > 9990,
+    col(1) == 7,
+    (col(2) >= 100) & (col(2) < 300) & (col(1) <= 3),
+    col(2) == 5.5,
+    col(2) <= -0.5,
+    col(3) > 4294967295,
+    col(3) >= 4294967294.5,
+    ir.lit(7) == col(1),
+    col(0) == 123_457,
+    (col(0) >= 999_000) & (col(0) < 1_000_001),
+    (col(1) == 7) | ((col(2) >= 5000) & (col(3) < 2**31)),
+    col(1).isin([1, 3, 49]),
+    col(1).isin([1, 3], negate=True),
+    col(2) != 17,
+    ((col(1) < 10) | (col(1) > 40)) & (col(2) < 500),
+]
+
+
+@pytest.mark.parametrize("case", range(len(SCAN_CASES)))
+@pytest.mark.parametrize("path", ["default", "interpreter", "generated"])
+def test_scan_ids_and_count(events, orc, case, path):
+    t, host = events
+    where = SCAN_CASES[case]
+    cq.reset_settings()
+    if path == "interpreter":
+        cq.set_setting("QE_SCAN_FORCE_INTERPRETER", 1)
+    elif path == "generated":
+        cq.set_setting("QE_SCAN_SIMPLE_KERNEL", 0)
+    try:
+        expect = orc.scan(host, where)
+        n, ids, tr = t.scan(where)
+        assert n == len(expect), (where, tr)
+        assert np.array_equal(ids, expect), where
+        cnt, _, _ = t.scan(where, count_only=True)
+        assert cnt == len(expect)
+        if path == "interpreter" and tr["path"] not in (5, 6):
+            assert cq.SCAN_PATHS[tr["path"]] == "interpreter"
+    finally:
+        cq.reset_settings()
+
+
+def test_scan_generated_simple_kernel(events, orc):
+    t, host = events
+    cq.set_setting("QE_PERQUERY_SIMPLE", 1)
+    try:
+        where = (col(2) >= 100) & (col(2) < 300) & (col(1) <= 3)
+        n, ids, tr = t.scan(where)
+        assert cq.SCAN_PATHS[tr["path"]] == "generated_simple", tr
+        assert np.array_equal(ids, orc.scan(host, where))
+    finally:
+        cq.reset_settings()
+
+
+def test_point_lookup_prunes_zones(events, orc):
+    t, host = events
+    n, ids, tr = t.scan(col(0) == 777_777)
+    assert n == 1 and ids[0] == 777_777
+    assert tr["rows_scanned"] == 8192
+    cq.set_setting("QE_ZONEMAP", 0)
+    try:
+        n2, ids2, tr2 = t.scan(col(0) == 777_777)
+        assert n2 == 1 and tr2["rows_scanned"] == N
+    finally:
+        cq.reset_settings()
+    n3, _, tr3 = t.scan(col(0) > 5_000_000)
+    assert n3 == 0 and cq.SCAN_PATHS[tr3["path"]] == "zone_empty"
+
+
+def test_scan_output_capacity(events, orc):
+    t, host = events
+    expect = orc.scan(host, col(2) < 200)
+    n, ids, _ = t.scan(col(2) < 200, cap=1000)
+    assert n == len(expect) and np.array_equal(ids, expect[:1000])
+
+
+GLOBAL = [cq.COUNT, cq.SUM, cq.MIN, cq.MAX, cq.AVG]
+
+
+@pytest.mark.parametrize("where", [None, col(2) < 200, (col(1) == 7) & (col(3) < 2**31), col(0) < 4096,
+                                   col(2) > 20000])
+def test_global_aggregate(events, orc, where):
+    t, host = events
+    res = t.aggregate(GLOBAL + [cq.SUM, cq.MIN, cq.MAX], [None, 2, 2, 2, 2, 3, 3, 3], where=where)
+    keys, ref = orc.group_aggregate(host, None, GLOBAL + [cq.SUM, cq.MIN, cq.MAX], [None, 2, 2, 2, 2, 3, 3, 3],
+                                    where=where)
+    assert res.groups == 1
+    assert int(res.cells[0, 0]) == int(ref[0, 0])
+    got = _f64(res.cells[0, 1:].copy())
+    assert np.array_equal(got, ref[0, 1:]), (got, ref[0, 1:])
+
+
+def test_global_aggregate_signed_and_f64(orc):
+    rng = np.random.default_rng(1)
+    n = 300_001
+    i32 = rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)
+    f64 = rng.lognormal(3.0, 1.0, n)
+    t = cq.Table(n)
+    t.add_column("i", cq.I32, i32)
+    t.add_column("f", cq.F64, f64)
+    res = t.aggregate([cq.SUM, cq.MIN, cq.MAX, cq.SUM, cq.MIN, cq.MAX], [0, 0, 0, 1, 1, 1])
+    vals = _f64(res.cells[0].copy())
+    assert vals[0] == float(i32.astype(np.int64).sum()) and vals[1] == i32.min() and vals[2] == i32.max()
+    import math
+
+    exact = math.fsum(f64)
+    assert abs(vals[3] - exact) <= 1e-12 * exact  # f64: summation order differs from the CPU's
+    assert vals[4] == f64.min() and vals[5] == f64.max()
+    w = col(0) < 0
+    res = t.aggregate([cq.COUNT, cq.SUM], [None, 0], where=w)
+    assert int(res.cells[0, 0]) == int((i32 < 0).sum())
+    assert _f64(res.cells[0, 1:].copy())[0] == float(i32[i32 < 0].astype(np.int64).sum())
+    t.close()
+
+
+def _norm(keys, vals):
+    order = np.argsort(keys, kind="stable")
+    return keys[order], vals[order]
+
+
+@pytest.mark.parametrize("key,args,funcs,where", [
+    (1, [None, 2], [cq.COUNT, cq.SUM], None),
+    (1, [None, 2, 2, 2, 2], [cq.COUNT, cq.SUM, cq.MIN, cq.MAX, cq.AVG], None),
+    (2, [None, 2, 2, 2], [cq.COUNT, cq.SUM, cq.MIN, cq.MAX], None),
+    (2, [None, 1], [cq.COUNT, cq.SUM], None),
+    (1, [None, 2, 3], [cq.COUNT, cq.SUM, cq.MAX], col(2) < 200),
+    (2, [None, 1, 1], [cq.COUNT, cq.MIN, cq.MAX], (col(1) == 7) & (col(3) > 1e9)),
+])
+def test_group_by(events, orc, key, args, funcs, where):
+    t, host = events
+    res = t.aggregate(funcs, args, where=where, group=key)
+    rk, rv = orc.group_aggregate(host, key, funcs, args, where=where)
+    rk, rv = _norm(rk, rv)
+    assert np.array_equal(res.keys, rk)
+    for e, f in enumerate(funcs):
+        got = res.column(e)
+        assert np.array_equal(got, rv[:, e]), (e, f)
+
+
+def test_group_by_signed_key(orc):
+    rng = np.random.default_rng(2)
+    n = 100_003
+    k = rng.integers(-300, 300, n).astype(np.int32)
+    v = rng.integers(-1000, 1000, n).astype(np.int32)
+    t = cq.Table(n)
+    t.add_column("k", cq.I32, k)
+    t.add_column("v", cq.I32, v)
+    funcs = [cq.COUNT, cq.SUM, cq.MIN, cq.MAX]
+    res = t.aggregate(funcs, [None, 1, 1, 1], group=0)
+    rk, rv = orc.group_aggregate([k, v], 0, funcs, [None, 1, 1, 1])
+    rk, rv = _norm(rk, rv)
+    assert np.array_equal(res.keys, rk)
+    for e in range(4):
+        assert np.array_equal(res.column(e), rv[:, e])
+    t.close()
+
+
+@pytest.fixture(scope="module")
+def star(orc):
+    """fact(fk, amount, region) + dim A (2^16 rows, spread keys, segment) +
+    dim B (50 rows, key = region, attr)."""
+    n = 2_000_011
+    t = cq.Table(n)
+    for name in ("id", "region", "amount", "ts", "fk"):
+        t.add_column(name, cq.U32)
+    t.gen_events(orc.SEED, id=0, region=1, amount=2, ts=3)
+    dk, seg = orc.gen_dim(65_536, 100, seed=5)
+    dim = cq.Table(len(dk))
+    dim.add_column("key", cq.U32, dk)
+    dim.add_column("segment", cq.U32, seg)
+    t.gen_fact_fk(4, orc.SEED, dim, 0)
+    bk = np.arange(50, dtype=np.uint32)
+    battr = (np.arange(50, dtype=np.uint32) * 7) % 10
+    dimb = cq.Table(50)
+    dimb.add_column("key", cq.U32, bk)
+    dimb.add_column("attr", cq.U32, battr)
+    host = orc.gen_events(n, seed=orc.SEED)
+    fk = orc.gen_fact_fk(n, dk, seed=orc.SEED)
+    yield t, dim, dimb, host, fk, (dk, seg), (bk, battr)
+    t.close()
+    dim.close()
+    dimb.close()
+
+
+def test_fact_fk_generator(star):
+    t, dim, dimb, host, fk, _, _ = star
+    assert np.array_equal(t.read(4), fk)
+
+
+@pytest.mark.parametrize("mode", ["group_payload", "row_payload"])
+def test_join_a_group_by_segment(star, orc, mode):
+    t, dim, dimb, host, fk, (dk, seg), _ = star
+    funcs = [cq.COUNT, cq.SUM, cq.MIN, cq.MAX, cq.AVG]
+    args = [None, 2, 2, 2, 2]
+    if mode == "row_payload":
+        cq.set_setting("QE_JOIN_GROUP_PAYLOAD", 0)
+    try:
+        res = cq.aggregate_join(t, 4, dim, 0, funcs, args, dim_group=1)
+    finally:
+        cq.reset_settings()
+    expect_path = "join_probe_group" if mode == "group_payload" else "join_probe_row"
+    assert cq.AGG_PATHS[res.trace["path"]] == expect_path
+    rk, rv = orc.join_aggregate(fk, [None, host[2], host[2], host[2], host[2]], funcs, dk, dim_group=seg)
+    rk, rv = _norm(rk, rv)
+    assert np.array_equal(res.keys, rk)
+    for e in range(len(funcs)):
+        assert np.array_equal(res.column(e), rv[:, e]), e
+
+
+def test_join_a_filtered_dim(star, orc):
+    t, dim, dimb, host, fk, (dk, seg), _ = star
+    res = cq.aggregate_join(t, 4, dim, 0, [cq.COUNT, cq.SUM], [None, 2], dim_where=col(1) < 10, dim_group=1)
+    rk, rv = orc.join_aggregate(fk, [None, host[2]], [cq.COUNT, cq.SUM], dk, dim_pass=(seg < 10), dim_group=seg)
+    rk, rv = _norm(rk, rv)
+    assert np.array_equal(res.keys, rk) and len(rk) == 10
+    assert np.array_equal(res.column(0), rv[:, 0]) and np.array_equal(res.column(1), rv[:, 1])
+
+
+def test_join_b_filtered_count(star, orc):
+    t, dim, dimb, host, fk, _, (bk, battr) = star
+    res = cq.aggregate_join(t, 1, dimb, 0, [cq.COUNT], [None], dim_where=col(1) < 3)
+    assert cq.AGG_PATHS[res.trace["path"]] == "join_dense"
+    rk, rv = orc.join_aggregate(host[1], [None], [cq.COUNT], bk, dim_pass=(battr < 3))
+    assert res.groups == 1 and int(res.cells[0, 0]) == int(rv[0, 0])
+    # the same join through the hash probe
+    cq.set_setting("QE_JOIN_DENSE", 0)
+    try:
+        res2 = cq.aggregate_join(t, 1, dimb, 0, [cq.COUNT], [None], dim_where=col(1) < 3)
+    finally:
+        cq.reset_settings()
+    assert cq.AGG_PATHS[res2.trace["path"]] == "join_probe_group"
+    assert int(res2.cells[0, 0]) == int(rv[0, 0])
+
+
+def test_closed_form_table(orc):
+    """The reference's known answers on the GPU (bench_select_scale.c:1123-1169)."""
+    n = 4_000_037
+    t = cq.Table(n)
+    for name in ("id", "region_id", "amount", "needle_key"):
+        t.add_column(name, cq.U32)
+    t.gen_scale_events(id=0, region=1, amount=2, key=3)
+    res = t.aggregate([cq.SUM, cq.MIN, cq.MAX], [2, 2, 2])
+    v = res.column(0)[0], res.column(1)[0], res.column(2)[0]
+    assert v == (orc.ss_sum_amount(n), 0.0, 9999.0)
+    assert t.count(col(2) > 9990) == orc.ss_count_amount_above(n, 9990)
+    assert t.count(col(1) == 7) == orc.ss_count_region(n, 7)
+    assert t.count((col(0) > 1000) & (col(0) <= 900_000_000)) == n - 1000
+    key = orc.ss_key_of_row(1_234_567)
+    n1, ids, _ = t.scan(col(3) == key)
+    assert n1 == 1 and ids[0] == 1_234_567
+    g = t.aggregate([cq.COUNT, cq.SUM], [None, 2], group=1)
+    assert g.groups == 50
+    for r in range(50):
+        assert g.column(0)[r] == orc.ss_count_region(n, r) and g.column(1)[r] == orc.ss_region_amount_sum(n, r)
+    g = t.aggregate([cq.COUNT, cq.SUM], [None, 1], group=2)
+    assert g.groups == 10000
+    assert g.column(0)[9999] == orc.ss_amount_rows(n, 9999)
+    assert g.column(1)[9999] == orc.ss_amount_rows(n, 9999) * (9999 % 50)
+    regions = cq.Table(50)
+    regions.add_column("region_id", cq.U32, np.arange(50, dtype=np.uint32))
+    regions.add_column("zone", cq.U32, np.arange(50, dtype=np.uint32) // 10)
+    j = cq.aggregate_join(t, 1, regions, 0, [cq.COUNT, cq.SUM], [None, 2], dim_where=col(1) == 2)
+    assert int(j.cells[0, 0]) == sum(orc.ss_count_region(n, r) for r in range(20, 30))
+    assert j.column(1)[0] == sum(orc.ss_region_amount_sum(n, r) for r in range(20, 30))
+    t.close()
+    regions.close()
+
+
+def test_empty_and_tiny_tables(orc):
+    for n in (1, 3, 5, 8191, 8193):
+        a = (np.arange(n, dtype=np.uint32) * 7919) % 1000
+        t = cq.Table(n)
+        t.add_column("a", cq.U32, a)
+        cnt, ids, _ = t.scan(col(0) < 500)
+        assert np.array_equal(ids, np.flatnonzero(a < 500))
+        res = t.aggregate([cq.COUNT, cq.SUM, cq.MAX], [None, 0, 0])
+        assert int(res.cells[0, 0]) == n and res.column(1)[0] == float(a.sum()) and res.column(2)[0] == a.max()
+        g = t.aggregate([cq.COUNT], [None], group=0)
+        assert g.groups == len(np.unique(a))
+        t.close()
+    t = cq.Table(1000)
+    t.add_column("a", cq.U32, np.arange(1000, dtype=np.uint32))
+    res = t.aggregate([cq.COUNT, cq.SUM, cq.MIN], [None, 0, 0], where=col(0) > 5000)
+    assert int(res.cells[0, 0]) == 0 and res.column(1)[0] == 0.0
+    t.close()
