@@ -48,8 +48,11 @@ def _run(cmd: list[str]) -> None:
 
 
 def write_prelude() -> str:
-    """qk_device_common.h as a C string literal for the hiprtc generator."""
+    """qk_abi.h + qk_device_common.h as one C string literal for the hiprtc
+    generator (hiprtc sees no include path, so the include is inlined)."""
+    abi = open(os.path.join(CSRC, "qk_abi.h")).read()
     src = open(os.path.join(CSRC, "qk_device_common.h")).read()
+    src = abi + "\n" + src.replace('#include "qk_abi.h"\n', "")
     lines = []
     for line in src.splitlines():
         esc = line.replace("\\", "\\\\").replace('"', '\\"')

@@ -57,10 +57,21 @@ struct PendingProfile {
   hipEvent_t start, stop;
 };
 
+/* small transfers of a batch go through pinned host staging so they are
+ * asynchronous on the batch stream; readbacks are copied out at batch_end */
+struct PendingRead {
+  void* user;
+  const void* pinned;
+  U64 size;
+};
+
 struct GPU_Batch {
   hipStream_t stream;
   bool failed;
   std::vector<PendingProfile> prof;
+  std::vector<char*> blocks; /* pinned staging blocks owned by this batch */
+  U64 block_used;
+  std::vector<PendingRead> reads;
 };
 
 struct GPU_StreamSlot {
@@ -302,7 +313,10 @@ GPU_Buffer* alloc_impl(U64 size, U32 flags, void* data)
       return nullptr;
     }
   }
-  if (padded > size) hipMemset((char*)p + size, 0, padded - size);
+  /* new buffers start zeroed (kernels keep their completion tickets in
+   * scratch buffers and reset them themselves) */
+  if (!data || !size) hipMemset(p, 0, padded);
+  else if (padded > size) hipMemset((char*)p + size, 0, padded - size);
   return b;
 }
 
@@ -661,11 +675,57 @@ GPU_Batch* gpu_hip_batch_begin(U64 upload_bytes_needed, U64 download_bytes_neede
   return b;
 }
 
+namespace {
+const U64 kStagingBlock = 1 << 20;
+const U64 kStagingMax = 256 << 10; /* larger transfers go direct */
+std::mutex g_staging_lock;
+std::vector<char*> g_staging_free;
+
+char* staging_take(GPU_Batch* b, U64 size)
+{
+  size = (size + 63) & ~(U64)63;
+  if (b->blocks.empty() || b->block_used + size > kStagingBlock) {
+    char* blk = nullptr;
+    {
+      std::lock_guard<std::mutex> g(g_staging_lock);
+      if (!g_staging_free.empty()) {
+        blk = g_staging_free.back();
+        g_staging_free.pop_back();
+      }
+    }
+    if (!blk && hipHostMalloc((void**)&blk, kStagingBlock, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    b->blocks.push_back(blk);
+    b->block_used = 0;
+  }
+  char* p = b->blocks.back() + b->block_used;
+  b->block_used += size;
+  return p;
+}
+
+void staging_return(GPU_Batch* b)
+{
+  std::lock_guard<std::mutex> g(g_staging_lock);
+  for (char* blk : b->blocks) g_staging_free.push_back(blk);
+  b->blocks.clear();
+}
+}  // namespace
+
 void gpu_hip_batch_buffer_write(GPU_Batch* batch, GPU_Buffer* buffer, void* data, U64 size)
 {
   if (!batch || !buffer || !data) return;
   if (size > buffer->alloc_size) size = buffer->alloc_size;
-  if (!check(hipMemcpyAsync(buffer->ptr, data, size, hipMemcpyHostToDevice, batch->stream), "batch_buffer_write"))
+  const void* src = data;
+  if (size <= kStagingMax) {
+    char* p = staging_take(batch, size);
+    if (p) {
+      memcpy(p, data, size);
+      src = p;
+    }
+  }
+  if (!check(hipMemcpyAsync(buffer->ptr, src, size, hipMemcpyHostToDevice, batch->stream), "batch_buffer_write"))
     batch->failed = true;
 }
 
@@ -710,7 +770,15 @@ void gpu_hip_batch_buffer_read(GPU_Batch* batch, GPU_Buffer* buffer, void* out_d
 {
   if (!batch || !buffer || !out_data) return;
   if (size > buffer->alloc_size) size = buffer->alloc_size;
-  if (!check(hipMemcpyAsync(out_data, buffer->ptr, size, hipMemcpyDeviceToHost, batch->stream), "batch_buffer_read"))
+  void* dst = out_data;
+  if (size <= kStagingMax) {
+    char* p = staging_take(batch, size);
+    if (p) {
+      dst = p;
+      batch->reads.push_back(PendingRead{out_data, p, size});
+    }
+  }
+  if (!check(hipMemcpyAsync(dst, buffer->ptr, size, hipMemcpyDeviceToHost, batch->stream), "batch_buffer_read"))
     batch->failed = true;
 }
 
@@ -718,6 +786,11 @@ B32 gpu_hip_batch_end(GPU_Batch* batch)
 {
   if (!batch) return 0;
   bool ok = check(hipStreamSynchronize(batch->stream), "batch_end") && !batch->failed;
+  for (auto& r : batch->reads) {
+    if (ok) memcpy(r.user, r.pinned, r.size);
+    else memset(r.user, 0, r.size); /* readbacks of a failed batch are zero (gpu_vulkan.c:3107-3111) */
+  }
+  staging_return(batch);
   flush_profile(batch->prof);
   give_stream(batch->stream);
   delete batch;

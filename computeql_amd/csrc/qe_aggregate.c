@@ -138,8 +138,6 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
   U64 res[16];
   memset(res, 0, sizeof(res));
   if (ok) {
-    if (leaf_buf) gpu_buffer_write(leaf_buf, leaf_words, 24 * (U64)leaf_count);
-    if (skip_buf) gpu_buffer_write(skip_buf, prune->skip_bits, prune->skip_words * 4);
     if (leaf_buf) gpu_kernel_set_arg_buffer(k, 0, leaf_buf);
     else gpu_hip_kernel_clear_arg(k, 0);
     gpu_kernel_set_arg_buffer(k, 1, result);
@@ -155,12 +153,13 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
     gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
     gpu_kernel_set_arg_u64(k, 5, r0);
     gpu_kernel_set_arg_u64(k, 7, leaf_count);
-    qe_kernel_set_word(k, 12, p->arg_count | (slot_bits << 8));
-    qe_kernel_set_word(k, 13, type_bits);
+    qe_kernel_set_word(k, QK_W_ARGS, p->arg_count | (slot_bits << 8));
+    qe_kernel_set_word(k, QK_W_TYPES, type_bits);
     GPU_Batch* b = gpu_batch_begin(0, sizeof(res));
-    /* the ticket word of the result must start at zero; the kernel leaves
-     * it at zero when it finishes */
-    gpu_batch_buffer_zero(b, result, 16 * 8);
+    if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
+    if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
+    /* the ticket word of the result starts at zero (new scratch is zeroed)
+     * and the kernel's last block leaves it at zero */
     gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
     gpu_batch_buffer_read(b, result, res, sizeof(res));
     ok = gpu_batch_end(b);
@@ -273,8 +272,6 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
   GPU_Buffer* skip_buf = prune->skip_bits ? qe_scratch("dense_skip", prune->skip_words * 4) : 0;
   B32 ok = acc && (!leaf_count || leaf_buf) && (!prune->skip_bits || skip_buf);
   if (ok) {
-    if (leaf_buf) gpu_buffer_write(leaf_buf, leaf_words, 24 * (U64)leaf_count);
-    if (skip_buf) gpu_buffer_write(skip_buf, prune->skip_bits, prune->skip_words * 4);
     if (leaf_buf) gpu_kernel_set_arg_buffer(k, 0, leaf_buf);
     else gpu_hip_kernel_clear_arg(k, 0);
     gpu_kernel_set_arg_buffer(k, 1, acc);
@@ -289,13 +286,15 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
     gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
     gpu_kernel_set_arg_u64(k, 5, r0);
     gpu_kernel_set_arg_u64(k, 7, leaf_count);
-    qe_kernel_set_word(k, 10, key_min_code);
-    qe_kernel_set_word(k, 11, range);
-    qe_kernel_set_word(k, 12, arg_count | (slot_bits << 8));
-    qe_kernel_set_word(k, 13, type_bits);
-    qe_kernel_set_word(k, 14, key_slot | (t->columns[key_col].type << 8) | (flags << 16));
+    qe_kernel_set_word(k, QK_W_KEY_MIN, key_min_code);
+    qe_kernel_set_word(k, QK_W_RANGE, range);
+    qe_kernel_set_word(k, QK_W_ARGS, arg_count | (slot_bits << 8));
+    qe_kernel_set_word(k, QK_W_TYPES, type_bits);
+    qe_kernel_set_word(k, QK_W_KEY, key_slot | (t->columns[key_col].type << 8) | (flags << 16));
     gpu_hip_kernel_set_lds_bytes(k, (U32)lds);
     GPU_Batch* b = gpu_batch_begin(0, gbytes);
+    if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
+    if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
     /* count/sum/max start at 0, min at all ones: fills run from offset 0,
      * so zero all, fill through the min array, zero count+sum again */
     U64 zero_bytes = (U64)range * 8 * (1 + arg_count);

@@ -292,8 +292,8 @@ B32 qe_table_build_zonemap(QE_Table* table, U32 column)
   gpu_kernel_set_arg_buffer(k, 1, bmin);
   gpu_kernel_set_arg_buffer(k, 2, bmax);
   gpu_kernel_set_arg_u64(k, 0, table->row_count);
-  qe_kernel_set_word(k, 2, col->type);
-  qe_kernel_set_word(k, 3, QE_ZONE_SHIFT);
+  qe_kernel_set_word(k, QK_W_ZTYPE, col->type);
+  qe_kernel_set_word(k, QK_W_ZSHIFT, QE_ZONE_SHIFT);
   U64 blocks = zones < 65536 ? zones : 65536;
   GPU_Batch* b = gpu_batch_begin(0, zones * 16);
   gpu_batch_kernel_execute(b, k, (U32)(blocks * QE_WG), QE_WG);

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include "../../include/cqlhip.h"
+#include "qk_abi.h"
 
 #define QE_MAX_COLUMNS 64
 #define QE_ZONE_SHIFT 13 /* CQL_ZONEMAP_CHUNK_ROWS 8192, cql.h:70-71 */
@@ -119,6 +120,7 @@ typedef struct QE_Prune {
   U64 skip_words;
 } QE_Prune;
 void qe_zonemap_prune(QE_Table* table, const QE_Leaf* leaves, U32 leaf_count, QE_Prune* out);
+B32 qe_scan_filter_device(QE_Table* table, const QE_IRNode* where, U32 node_count, GPU_Buffer** out_ids, U64* out_count);
 void qe_prune_free(QE_Prune* p);
 
 /* ---- generated kernels (query_kernel.c) ---- */

@@ -175,34 +175,69 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
   }
   if (trace) trace->rows_before = fact->row_count;
 
-  /* dim side: rows passing the dim WHERE (on the GPU), keys and groups */
-  U64 dn = dim->row_count;
-  U32* dkeys = (U32*)malloc(sizeof(U32) * (dn ? dn : 1));
-  U32* dgroups = dim_group_col >= 0 ? (U32*)malloc(sizeof(U32) * (dn ? dn : 1)) : 0;
-  U64* pass = (U64*)malloc(sizeof(U64) * (dn ? dn : 1));
-  U64 pass_count = 0;
-  B32 ok = dkeys && pass && (dim_group_col < 0 || dgroups);
-  if (ok) ok = qe_dim_u32(dim, dim_key_col, dkeys);
-  if (ok && dgroups) ok = qe_dim_u32(dim, (U32)dim_group_col, dgroups);
-  if (ok) {
-    if (dim_where && dim_where_count) ok = qe_scan_filter(dim, dim_where, dim_where_count, 0, pass, dn, &pass_count, 0);
-    else {
-      for (U64 i = 0; i < dn; i++) pass[i] = i;
-      pass_count = dn;
-    }
-  }
-  if (!ok) {
-    free(dkeys);
-    free(dgroups);
-    free(pass);
-    return 0;
-  }
   B32 key_i32 = fkey->type == CQL_ColumnType_I32;
   B32 group_i32 = dim_group_col >= 0 && dim->columns[dim_group_col].type == CQL_ColumnType_I32;
+  if (dim_group_col >= 0 && dim->columns[dim_group_col].type != CQL_ColumnType_U32 && !group_i32) {
+    qe_set_error("qe_aggregate_join: the dim group column must be U32/I32");
+    return 0;
+  }
+  if (dim->columns[dim_key_col].type != CQL_ColumnType_U32 && dim->columns[dim_key_col].type != CQL_ColumnType_I32) {
+    qe_set_error("qe_aggregate_join: the dim key must be U32/I32");
+    return 0;
+  }
+  U64 dn = dim->row_count;
+  F64 kspan = fkey->max_value - fkey->min_value + 1.0;
+  B32 dense = fact->row_count > 0 && qe_settings_u64("QE_JOIN_DENSE", 1) != 0 &&
+              kspan <= (F64)qe_settings_u64("QE_AGG_DENSE_MAX_RANGE", 65536);
+  /* group codes from the dim group column's zone map (a superset of the
+   * passing rows' range) */
+  U32 gmin = 0;
+  U64 grange = 1;
+  if (dim_group_col >= 0 && dn) {
+    QE_Column* gc = &dim->columns[dim_group_col];
+    gmin = group_i32 ? ((U32)(S32)gc->min_value ^ 0x80000000u) : (U32)gc->min_value;
+    grange = (U64)(gc->max_value - gc->min_value) + 1;
+  }
+  U64 lds_need = grange * 4 + (need_sum ? grange * 8 * arg_count : 0) + (need_minmax ? grange * 8 * arg_count : 0);
+  B32 group_mode = !dense && qe_settings_u64("QE_JOIN_GROUP_PAYLOAD", 1) != 0 &&
+                   grange <= qe_settings_u64("QE_JOIN_GROUP_MAX", 8192) && lds_need <= gpu_hip_max_lds_bytes();
+  if (!dense && qe_settings_u64("QE_JOIN_PROBE", 1) == 0) {
+    qe_set_error("qe_aggregate_join: key range too wide for the dense form and QE_JOIN_PROBE is off");
+    return 0;
+  }
+
+  /* dim rows passing the dim WHERE: on the device for the probe forms, on
+   * the host for the forms that combine per dim row */
+  GPU_Buffer* pass_dev = 0;
+  U64 pass_count = dn;
+  U64* pass = 0;
+  U32* dkeys = 0;
+  U32* dgroups = 0;
+  B32 ok = 1;
+  B32 host_dim = dense || !group_mode;
+  if (dim_where && dim_where_count) {
+    if (host_dim) {
+      pass = (U64*)malloc(sizeof(U64) * (dn ? dn : 1));
+      ok = pass && qe_scan_filter(dim, dim_where, dim_where_count, 0, pass, dn, &pass_count, 0);
+    } else {
+      ok = qe_scan_filter_device(dim, dim_where, dim_where_count, &pass_dev, &pass_count);
+    }
+  } else if (host_dim) {
+    pass = (U64*)malloc(sizeof(U64) * (dn ? dn : 1));
+    ok = pass != 0;
+    for (U64 i = 0; ok && i < dn; i++) pass[i] = i;
+  }
+  if (ok && host_dim) {
+    dkeys = (U32*)malloc(sizeof(U32) * (dn ? dn : 1));
+    dgroups = dim_group_col >= 0 ? (U32*)malloc(sizeof(U32) * (dn ? dn : 1)) : 0;
+    ok = dkeys && (dim_group_col < 0 || dgroups);
+    if (ok) ok = qe_dim_u32(dim, dim_key_col, dkeys);
+    if (ok && dgroups) ok = qe_dim_u32(dim, (U32)dim_group_col, dgroups);
+  }
 
   QE_GroupMap groups;
   memset(&groups, 0, sizeof(groups));
-  ok = qe_group_map_init(&groups, pass_count + 1);
+  if (ok) ok = qe_group_map_init(&groups, (host_dim ? pass_count : grange) + 1);
   U32 path = 0;
   QE_Prune all;
   memset(&all, 0, sizeof(all));
@@ -210,11 +245,8 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
   all.row_end = fact->row_count;
   all.zones_total = all.zones_left = 1;
 
-  F64 kspan = fkey->max_value - fkey->min_value + 1.0;
-  B32 dense = ok && fact->row_count > 0 && qe_settings_u64("QE_JOIN_DENSE", 1) != 0 &&
-              kspan <= (F64)qe_settings_u64("QE_AGG_DENSE_MAX_RANGE", 65536);
-  if (ok && fact->row_count == 0) {
-    /* no fact rows: no groups */
+  if (!ok || fact->row_count == 0 || pass_count == 0) {
+    /* nothing to join */
   } else if (dense) {
     path = QE_AggPath_JoinDense;
     U32 range = (U32)kspan;
@@ -234,61 +266,41 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
       }
       qe_dense_totals_free(&tot);
     }
-  } else if (ok) {
-    if (qe_settings_u64("QE_JOIN_PROBE", 1) == 0) {
-      ok = 0;
-      qe_set_error("qe_aggregate_join: key range too wide for the dense form and QE_JOIN_PROBE is off");
-    }
-    /* group codes of the passing dim rows */
-    U32 gmin = 0, gmax = 0;
-    if (ok && dgroups && pass_count) {
-      gmin = gmax = group_i32 ? dgroups[pass[0]] ^ 0x80000000u : dgroups[pass[0]];
-      for (U64 v = 1; v < pass_count; v++) {
-        U32 c = group_i32 ? dgroups[pass[v]] ^ 0x80000000u : dgroups[pass[v]];
-        if (c < gmin) gmin = c;
-        if (c > gmax) gmax = c;
-      }
-    }
-    U64 grange = dgroups ? (U64)gmax - gmin + 1 : 1;
-    U64 lds_need = grange * 4 + (need_sum ? grange * 8 * arg_count : 0) + (need_minmax ? grange * 8 * arg_count : 0);
-    B32 group_mode = qe_settings_u64("QE_JOIN_GROUP_PAYLOAD", 1) != 0 && grange <= qe_settings_u64("QE_JOIN_GROUP_MAX", 8192) &&
-                     lds_need <= gpu_hip_max_lds_bytes();
+  } else {
     path = group_mode ? QE_AggPath_JoinProbeGroup : QE_AggPath_JoinProbeRow;
     U64 R = group_mode ? grange : dn;
     U64 slots_n = 64;
     while (slots_n < pass_count * 2) slots_n <<= 1;
     U32 lg = 0;
     while ((1ull << lg) < slots_n) lg++;
-    U32* payload = (U32*)malloc(sizeof(U32) * (dn ? dn : 1));
-    U32* list = (U32*)malloc(sizeof(U32) * (pass_count ? pass_count : 1));
-    if (!payload || !list) ok = 0;
-    for (U64 v = 0; ok && v < pass_count; v++) {
-      U64 d = pass[v];
-      list[v] = (U32)d;
-      if (group_mode) payload[d] = dgroups ? ((group_i32 ? dgroups[d] ^ 0x80000000u : dgroups[d]) - gmin) : 0u;
-      else payload[d] = (U32)d;
-    }
-    GPU_Kernel* kb = ok ? qe_kernel("join_build") : 0;
-    GPU_Kernel* kp = ok ? qe_kernel("join_probe") : 0;
+    GPU_Kernel* kb = qe_kernel("join_build");
+    GPU_Kernel* kp = qe_kernel("join_probe");
     U64 acc_bytes = qe_dense_totals_bytes((U32)R, arg_count);
     GPU_Buffer* table = qe_scratch("join_table", slots_n * 8);
     GPU_Buffer* flags = qe_scratch("join_flags", 64);
-    GPU_Buffer* pay_buf = qe_scratch("join_payload", (dn ? dn : 1) * 4);
-    GPU_Buffer* list_buf = qe_scratch("join_list", (pass_count ? pass_count : 1) * 4);
     GPU_Buffer* acc = qe_scratch("join_totals", acc_bytes);
+    GPU_Buffer* list_buf = 0;
+    if (!group_mode && pass_count != dn) {
+      list_buf = qe_scratch("join_list", pass_count * 8);
+      if (list_buf) gpu_buffer_write(list_buf, pass, pass_count * 8);
+    }
     void* host_acc = malloc(acc_bytes ? acc_bytes : 8);
-    ok = ok && kb && kp && table && flags && pay_buf && list_buf && acc && host_acc;
+    ok = kb && kp && table && flags && acc && host_acc && (group_mode || pass_count == dn || list_buf);
     U32 fl[2] = {0, 0};
     if (ok) {
-      gpu_buffer_write(pay_buf, payload, dn * 4);
-      if (pass_count) gpu_buffer_write(list_buf, list, pass_count * 4);
       gpu_kernel_set_arg_buffer(kb, 0, dim->columns[dim_key_col].buffer);
-      gpu_kernel_set_arg_buffer(kb, 1, pay_buf);
+      if (dim_group_col >= 0) gpu_kernel_set_arg_buffer(kb, 1, dim->columns[dim_group_col].buffer);
+      else gpu_hip_kernel_clear_arg(kb, 1);
       gpu_kernel_set_arg_buffer(kb, 2, table);
       gpu_kernel_set_arg_buffer(kb, 3, flags);
-      gpu_kernel_set_arg_buffer(kb, 4, list_buf);
+      if (pass_dev) gpu_kernel_set_arg_buffer(kb, 4, pass_dev);
+      else if (list_buf) gpu_kernel_set_arg_buffer(kb, 4, list_buf);
+      else gpu_hip_kernel_clear_arg(kb, 4);
       gpu_kernel_set_arg_u64(kb, 0, pass_count);
-      qe_kernel_set_word(kb, 2, lg);
+      qe_kernel_set_word(kb, QK_W_LOG2, lg);
+      qe_kernel_set_word(kb, QK_W_PMODE, group_mode ? (dim_group_col >= 0 ? 1u : 2u) : 0u);
+      qe_kernel_set_word(kb, QK_W_GMIN, gmin);
+      qe_kernel_set_word(kb, QK_W_GTYPE, group_i32 ? CQL_ColumnType_I32 : CQL_ColumnType_U32);
       QE_ColumnSlots slots;
       memset(&slots, 0, sizeof(slots));
       qe_slot_of(&slots, fact_key_col); /* slot 0: the fact key */
@@ -308,13 +320,12 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
       gpu_kernel_set_arg_u64(kp, 0, fact->row_count);
       gpu_kernel_set_arg_u64(kp, 4, 0);
       gpu_kernel_set_arg_u64(kp, 5, 0);
-      qe_kernel_set_word(kp, 2, lg);
-      qe_kernel_set_word(kp, 11, (U32)R);
-      qe_kernel_set_word(kp, 12, arg_count | (slot_bits << 8));
-      qe_kernel_set_word(kp, 13, type_bits);
-      qe_kernel_set_word(kp, 14, kflags << 16);
-      U32 lds = group_mode ? (U32)lds_need : 0u;
-      gpu_hip_kernel_set_lds_bytes(kp, lds);
+      qe_kernel_set_word(kp, QK_W_LOG2, lg);
+      qe_kernel_set_word(kp, QK_W_RANGE, (U32)R);
+      qe_kernel_set_word(kp, QK_W_ARGS, arg_count | (slot_bits << 8));
+      qe_kernel_set_word(kp, QK_W_TYPES, type_bits);
+      qe_kernel_set_word(kp, QK_W_KEY, kflags << 16);
+      gpu_hip_kernel_set_lds_bytes(kp, group_mode ? (U32)lds_need : 0u);
       U64 quads = (fact->row_count + 3) / 4;
       U32 blocks = qe_stream_blocks(quads);
       U64 build_threads = pass_count < 256 ? 256 : pass_count;
@@ -360,7 +371,7 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
           if (!cnt[gi]) continue;
           U32 code = gmin + (U32)gi;
           U32 value = group_i32 ? code ^ 0x80000000u : code;
-          QE_GroupAcc* g = qe_group_get(&groups, dgroups ? value : 0u);
+          QE_GroupAcc* g = qe_group_get(&groups, dim_group_col >= 0 ? value : 0u);
           qe_group_add(g, funcs, expr_arg, expr_count, arg_type, cnt[gi], sums, mins, maxs, R, gi);
         }
       } else {
@@ -372,14 +383,11 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
         }
       }
     }
-    free(payload);
-    free(list);
     free(host_acc);
     qe_scratch_release(table);
     qe_scratch_release(flags);
-    qe_scratch_release(pay_buf);
-    qe_scratch_release(list_buf);
     qe_scratch_release(acc);
+    qe_scratch_release(list_buf);
   }
   if (ok) {
     /* ascending group value (signed order for I32 groups) */
@@ -400,6 +408,7 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     }
   }
   qe_group_map_free(&groups);
+  qe_scratch_release(pass_dev);
   free(dkeys);
   free(dgroups);
   free(pass);

@@ -172,6 +172,27 @@ static B32 qe_compile_condition(QE_Table* t, const QE_IRNode* nodes, U32 i, QE_B
   return 0;
 }
 
+/* qe_bytecode_program_max_stack_depth, query_exec.c:547-600 */
+static U32 qe_bytecode_max_stack_depth(const QE_Bytecode* p)
+{
+  S32 depth = 0, max_depth = 0;
+  for (U32 ip = 0; ip < p->word_count;) {
+    U32 op = p->words[ip++];
+    if (op == QE_Opcode_LoadNumCol || op == QE_Opcode_PushConst) {
+      ip++;
+      depth++;
+    } else if (op == QE_Opcode_PushTrue || op == QE_Opcode_PushFalse) {
+      depth++;
+    } else if (op == QE_Opcode_Halt) {
+      break;
+    } else {
+      depth--;
+    }
+    if (depth > max_depth) max_depth = depth;
+  }
+  return (U32)max_depth;
+}
+
 B32 qe_bytecode_program_build(QE_Table* table, const QE_IRNode* nodes, U32 count, QE_Bytecode* out)
 {
   memset(out, 0, sizeof(*out));
@@ -387,8 +408,26 @@ static void qe_bind_columns(GPU_Kernel* k, QE_Table* t, const QE_ColumnSlots* sl
   }
 }
 
+static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only, U64* out_ids,
+                        U64 cap, U64* out_count, QE_Trace* trace, GPU_Buffer** device_ids);
+
 B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only, U64* out_ids, U64 cap,
                    U64* out_count, QE_Trace* trace)
+{
+  return qe_scan_impl(table, where, node_count, count_only, out_ids, cap, out_count, trace, 0);
+}
+
+/* the matching row ids left on the device (a QE_DeviceSelection,
+ * query_exec.h:387-400): *out_ids is a scratch buffer of *out_count U64 ids
+ * the caller releases with qe_scratch_release (0 when nothing matched) */
+B32 qe_scan_filter_device(QE_Table* table, const QE_IRNode* where, U32 node_count, GPU_Buffer** out_ids, U64* out_count)
+{
+  *out_ids = 0;
+  return qe_scan_impl(table, where, node_count, 0, 0, ~0ull, out_count, 0, out_ids);
+}
+
+static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only, U64* out_ids,
+                        U64 cap, U64* out_count, QE_Trace* trace, GPU_Buffer** device_ids)
 {
   U64 t0, p0;
   qe_trace_begin(trace, &t0, &p0);
@@ -409,6 +448,19 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
     *out_count = n;
     if (!count_only && out_ids)
       for (U64 i = 0; i < n && i < cap; i++) out_ids[i] = i;
+    if (device_ids && n) {
+      U64* tmp = (U64*)malloc(sizeof(U64) * n);
+      GPU_Buffer* ids = tmp ? qe_scratch("scan_ids_dev", n * 8) : 0;
+      if (!ids) {
+        free(tmp);
+        qe_set_error("qe_scan_filter: out of memory");
+        return 0;
+      }
+      for (U64 i = 0; i < n; i++) tmp[i] = i;
+      gpu_buffer_write(ids, tmp, n * 8);
+      free(tmp);
+      *device_ids = ids;
+    }
     if (trace) {
       trace->path = QE_ScanPath_Constant;
       trace->rows_after = n;
@@ -442,6 +494,16 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
   else if (simple_ok) path = perquery && qe_settings_u64("QE_PERQUERY_SIMPLE", 0) ? QE_ScanPath_GeneratedSimple : QE_ScanPath_Simple;
   else if (perquery) path = QE_ScanPath_Generated;
   else path = QE_ScanPath_Interpreter;
+  /* the interpreter's stack holds QE_SCAN_MAX_STACK (8) values
+   * (query_exec.h:41); deeper programs need a generated kernel */
+  if (path == QE_ScanPath_Interpreter && qe_bytecode_max_stack_depth(&prog) > 8) {
+    if (perquery && !force_interp) path = QE_ScanPath_Generated;
+    else {
+      qe_prune_free(&prune);
+      qe_set_error("qe_scan_filter: WHERE needs a stack deeper than 8 and no generated kernel is allowed");
+      return 0;
+    }
+  }
   if (path == QE_ScanPath_Interpreter && qe_settings_u64("QE_SCAN_INTERPRETER", 1) == 0) {
     qe_prune_free(&prune);
     qe_set_error("qe_scan_filter: no GPU kernel for this WHERE (interpreter disabled)");
@@ -461,7 +523,13 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
   }
   if (path == QE_ScanPath_Generated) {
     k = qk_scan_kernel(table, where, node_count, &slots, consts, &const_count, &compile_ns);
-    if (!k) path = QE_ScanPath_Interpreter;
+    if (!k) {
+      if (qe_bytecode_max_stack_depth(&prog) > 8 || qe_settings_u64("QE_SCAN_INTERPRETER", 1) == 0) {
+        qe_prune_free(&prune);
+        return 0; /* the generator's error stays set */
+      }
+      path = QE_ScanPath_Interpreter;
+    }
   }
   if (path == QE_ScanPath_Simple || path == QE_ScanPath_GeneratedSimple) {
     if (path == QE_ScanPath_Simple) {
@@ -482,32 +550,38 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
     return 0;
   }
 
-  /* device buffers: program, constants, skip bits, counter, block counts */
+  /* device buffers: program, constants, skip bits, counter, block counts;
+   * the uploads ride the first batch (pinned staging, no host sync) */
   GPU_Buffer* prog_buf = 0;
   GPU_Buffer* const_buf = 0;
+  const void* prog_src = 0;
+  U64 prog_bytes = 0;
+  const void* const_src = 0;
+  U64 const_bytes = 0;
   if (path == QE_ScanPath_Simple || path == QE_ScanPath_GeneratedSimple) {
     prog_buf = qe_scratch("scan_leaves", sizeof(leaf_words));
-    if (prog_buf) gpu_buffer_write(prog_buf, leaf_words, 6 * 4 * (U64)leaf_count);
+    prog_src = leaf_words;
+    prog_bytes = 6 * 4 * (U64)leaf_count;
   } else if (path == QE_ScanPath_Interpreter) {
     prog_buf = qe_scratch("scan_bytecode", prog.word_count * 4);
     const_buf = qe_scratch("scan_consts", (prog.const_count ? prog.const_count : 1) * 8);
-    if (prog_buf) gpu_buffer_write(prog_buf, prog.words, prog.word_count * 4);
-    if (const_buf && prog.const_count) gpu_buffer_write(const_buf, prog.consts, prog.const_count * 8);
+    prog_src = prog.words;
+    prog_bytes = prog.word_count * 4;
+    const_src = prog.consts;
+    const_bytes = prog.const_count * 8;
   } else {
     const_buf = qe_scratch("scan_consts", (const_count ? const_count : 1) * 8);
-    if (const_buf && const_count) gpu_buffer_write(const_buf, consts, const_count * 8);
+    const_src = consts;
+    const_bytes = const_count * 8;
   }
-  GPU_Buffer* skip_buf = 0;
-  if (prune.skip_bits) {
-    skip_buf = qe_scratch("scan_skip", prune.skip_words * 4);
-    if (skip_buf) gpu_buffer_write(skip_buf, prune.skip_bits, prune.skip_words * 4);
-  }
+  GPU_Buffer* skip_buf = prune.skip_bits ? qe_scratch("scan_skip", prune.skip_words * 4) : 0;
   U64 scan_rows = prune.row_end - prune.row_begin;
   U64 quads = (prune.row_end + 3) / 4 - prune.row_begin / 4;
   U32 blocks = qe_stream_blocks(quads);
   GPU_Buffer* counter = qe_scratch("scan_counter", 64);
   GPU_Buffer* block_counts = qe_scratch("scan_block_counts", (U64)blocks * 8);
-  B32 ok = counter && block_counts && (prog_buf || path == QE_ScanPath_Generated) && (!prune.skip_bits || skip_buf);
+  B32 ok = counter && block_counts && (prog_buf || path == QE_ScanPath_Generated) && (!prune.skip_bits || skip_buf) &&
+           (!const_src || const_buf);
   if (ok) {
     if (prog_buf) gpu_kernel_set_arg_buffer(k, 0, prog_buf);
     else gpu_hip_kernel_clear_arg(k, 0);
@@ -527,10 +601,15 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
   }
   U64 total = 0;
   U32 kernels = 0;
+  GPU_Batch* b = ok ? gpu_batch_begin(prog_bytes + const_bytes, (U64)blocks * 8) : 0;
+  if (ok && b) {
+    if (prog_bytes) gpu_batch_buffer_write(b, prog_buf, (void*)prog_src, prog_bytes);
+    if (const_bytes) gpu_batch_buffer_write(b, const_buf, (void*)const_src, const_bytes);
+    if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune.skip_bits, prune.skip_words * 4);
+  }
   if (ok && count_only) {
+    /* count mode: the kernel's last block leaves the total in counter[0] */
     gpu_kernel_set_arg_u64(k, 6, 0); /* QK_MODE_COUNT */
-    GPU_Batch* b = gpu_batch_begin(0, 8);
-    gpu_batch_buffer_zero(b, counter, 8);
     gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
     gpu_batch_buffer_read(b, counter, &total, 8);
     ok = gpu_batch_end(b);
@@ -538,31 +617,33 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
   } else if (ok) {
     U64* counts = (U64*)malloc(sizeof(U64) * blocks);
     gpu_kernel_set_arg_u64(k, 6, 1); /* QK_MODE_BLOCK_COUNTS */
-    GPU_Batch* b = gpu_batch_begin(0, (U64)blocks * 8);
     gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
-    gpu_batch_buffer_read(b, block_counts, counts, (U64)blocks * 8);
-    ok = counts && gpu_batch_end(b);
+    if (counts) gpu_batch_buffer_read(b, block_counts, counts, (U64)blocks * 8);
+    ok = gpu_batch_end(b) && counts;
     kernels = 1;
     if (ok) {
       for (U32 i = 0; i < blocks; i++) total += counts[i];
       U64 want = total < cap ? total : cap;
-      if (want > 0 && out_ids) {
-        GPU_Buffer* ids = qe_scratch("scan_ids", want * 8);
+      if (want > 0 && (out_ids || device_ids)) {
+        GPU_Buffer* ids = qe_scratch(device_ids ? "scan_ids_dev" : "scan_ids", want * 8);
         ok = ids != 0;
         if (ok) {
           gpu_kernel_set_arg_buffer(k, 3, ids);
           gpu_kernel_set_arg_u64(k, 6, 2); /* QK_MODE_WRITE_IDS */
           b = gpu_batch_begin(0, want * 8);
           gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
-          gpu_batch_buffer_read(b, ids, out_ids, want * 8);
+          if (!device_ids) gpu_batch_buffer_read(b, ids, out_ids, want * 8);
           ok = gpu_batch_end(b);
           kernels = 2;
           gpu_hip_kernel_clear_arg(k, 3);
         }
-        qe_scratch_release(ids);
+        if (device_ids && ok) *device_ids = ids;
+        else qe_scratch_release(ids);
       }
     }
     free(counts);
+  } else if (b) {
+    gpu_batch_end(b);
   }
   qe_scratch_release(prog_buf);
   qe_scratch_release(const_buf);

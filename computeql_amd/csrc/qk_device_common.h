@@ -16,6 +16,8 @@
 #ifndef QK_DEVICE_COMMON_H
 #define QK_DEVICE_COMMON_H
 
+#include "qk_abi.h"
+
 typedef unsigned int qk_u32;
 typedef unsigned long long qk_u64;
 typedef int qk_s32;
@@ -28,7 +30,7 @@ typedef qk_u64 qk_u64x2 __attribute__((ext_vector_type(2)));
 typedef struct QK_Args {
   qk_u64 ptr[16];
   qk_u64 bytes[16];
-  qk_u32 words[16];
+  qk_u32 words[QK_PUSH_WORD_COUNT];
   qk_u32 global_work_size;
   qk_u32 grid_blocks;
 } QK_Args;
@@ -188,7 +190,7 @@ __device__ __forceinline__ qk_u32 qk_leaf_quad(const QK_Args& a, const qk_u32* w
 __device__ __forceinline__ qk_u32 qk_leaves_quad(const QK_Args& a, const qk_u32* leaves, qk_u32 leaf_count, qk_u64 q,
                                                  qk_u32 mask)
 {
-  for (qk_u32 l = 0; l < leaf_count; l++) {
+  for (qk_u32 l = 0; l < leaf_count && l < 8u; l++) {
     if (mask) mask &= qk_leaf_quad(a, leaves + 6 * l, q);
   }
   return mask;
@@ -276,12 +278,29 @@ __device__ __forceinline__ void qk_scan_driver(const QK_Args& a, Pred pred)
       for (int j = 0; j < QK_UNROLL; j++) count += (qk_u64)__popc(m[j]);
     }
     qk_u64 total = qk_block_sum_u64(count, s_red);
+    qk_u64* bc = qk_wptr<qk_u64>(a, QK_B_BLOCK_COUNTS);
+    if (threadIdx.x == 0) bc[blockIdx.x] = total;
+    if (mode != QK_MODE_COUNT) return;
+    /* count mode: the last block to finish adds the block counts (in block
+     * order) into out_count[0]; out_count[1] is its ticket, left at zero */
+    __shared__ qk_u32 s_last;
+    __threadfence();
+    __syncthreads();
+    qk_u64* oc = qk_wptr<qk_u64>(a, QK_B_OUT_COUNT);
     if (threadIdx.x == 0) {
-      if (mode == QK_MODE_COUNT) {
-        if (total) atomicAdd(qk_wptr<qk_u64>(a, QK_B_OUT_COUNT), total);
-      } else {
-        qk_wptr<qk_u64>(a, QK_B_BLOCK_COUNTS)[blockIdx.x] = total;
-      }
+      qk_u32 t = __hip_atomic_fetch_add((qk_u32*)&oc[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      s_last = t == gridDim.x - 1 ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    qk_u64 part = 0;
+    for (qk_u32 b = threadIdx.x; b < gridDim.x; b += QK_WG)
+      part += __hip_atomic_load(&bc[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    qk_u64 all = qk_block_sum_u64(part, s_red);
+    if (threadIdx.x == 0) {
+      oc[0] = all;
+      __hip_atomic_store((qk_u32*)&oc[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return;
   }

@@ -180,9 +180,9 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
   const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
   const qk_u64 r1 = r0 + rows;
   const qk_u32 leaf_count = (qk_u32)qk_push(a, QK_P_LEAVES);
-  const qk_u32 nargs = a.words[12] & 0xffu;
-  const qk_u32 slots = a.words[12] >> 8;
-  const qk_u32 types = a.words[13];
+  const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
+  const qk_u32 types = a.words[QK_W_TYPES];
   const qk_u32* leaves = qk_ptr<qk_u32>(a, QK_B_LEAVES);
 
   qk_u64 count = 0;
@@ -342,6 +342,7 @@ struct InterpPred {
   const qk_u32* code;
   const double* consts;
   qk_u32 code_words;
+  qk_u32 const_count;
   __device__ __forceinline__ qk_u32 operator()(qk_u64 q, qk_u32 valid) const
   {
     qk_u32 m = 0;
@@ -359,13 +360,17 @@ struct InterpPred {
           case OP_PUSH_TRUE: ok = sp < QK_MAX_STACK; if (ok) st[sp++] = 1.0; break;
           case OP_PUSH_FALSE: ok = sp < QK_MAX_STACK; if (ok) st[sp++] = 0.0; break;
           case OP_LOAD_NUM_COL: {
+            ok = ip < code_words && sp < QK_MAX_STACK;
+            if (!ok) break;
             qk_u32 operand = code[ip++];
-            ok = sp < QK_MAX_STACK;
+            ok = (operand & 0xffu) < 10u;
             if (ok) st[sp++] = qk_load1_f64(*a, operand & 0xffu, operand >> 8, row);
           } break;
           case OP_PUSH_CONST: {
+            ok = ip < code_words && sp < QK_MAX_STACK;
+            if (!ok) break;
             qk_u32 ci = code[ip++];
-            ok = sp < QK_MAX_STACK;
+            ok = ci < const_count;
             if (ok) st[sp++] = consts[ci];
           } break;
           default: {
@@ -396,7 +401,8 @@ struct InterpPred {
 
 extern "C" __global__ void LB k_scan_filter(QK_Args a)
 {
-  InterpPred p{&a, qk_ptr<qk_u32>(a, QK_B_LEAVES), qk_ptr<double>(a, QK_B_CONSTS), (qk_u32)(a.bytes[QK_B_LEAVES] / 4)};
+  InterpPred p{&a, qk_ptr<qk_u32>(a, QK_B_LEAVES), qk_ptr<double>(a, QK_B_CONSTS), (qk_u32)(a.bytes[QK_B_LEAVES] / 4),
+               (qk_u32)(a.bytes[QK_B_CONSTS] / 8)};
   qk_scan_driver(a, p);
 }
 
@@ -424,14 +430,14 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
   const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
   const qk_u64 r1 = r0 + rows;
   const qk_u32 leaf_count = (qk_u32)qk_push(a, QK_P_LEAVES);
-  const qk_u32 nargs = a.words[12] & 0xffu;
-  const qk_u32 slots = a.words[12] >> 8;
-  const qk_u32 types = a.words[13];
-  const qk_u32 key_slot = a.words[14] & 0xffu;
-  const qk_u32 key_type = (a.words[14] >> 8) & 0xffu;
-  const qk_u32 flags = a.words[14] >> 16;
-  const qk_u32 key_min = a.words[10];
-  const qk_u32 range = a.words[11];
+  const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
+  const qk_u32 types = a.words[QK_W_TYPES];
+  const qk_u32 key_slot = a.words[QK_W_KEY] & 0xffu;
+  const qk_u32 key_type = (a.words[QK_W_KEY] >> 8) & 0xffu;
+  const qk_u32 flags = a.words[QK_W_KEY] >> 16;
+  const qk_u32 key_min = a.words[QK_W_KEY_MIN];
+  const qk_u32 range = a.words[QK_W_RANGE];
   const qk_u32* leaves = qk_ptr<qk_u32>(a, QK_B_LEAVES);
 
   /* LDS: sums u64[E][R] | counts u32[R] | min u32[E][R] | max u32[E][R] */
@@ -469,16 +475,22 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
       qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
       qk_u32 g[4] = {qk_code32(kv[j].x, key_type) - key_min, qk_code32(kv[j].y, key_type) - key_min,
                      qk_code32(kv[j].z, key_type) - key_min, qk_code32(kv[j].w, key_type) - key_min};
+      /* a key outside [key_min, key_min + range) (stale zone map) is
+       * dropped rather than written out of the table */
+      qk_u32 mm = m[j];
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        if (m[j] & (1u << k)) atomicAdd(&l_cnt[g[k]], 1u);
+        if (g[k] >= range) mm &= ~(1u << k);
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (mm & (1u << k)) atomicAdd(&l_cnt[g[k]], 1u);
       for (qk_u32 c = 0; c < nargs; c++) {
         qk_u32 at = (types >> (8 * c)) & 0xffu;
         qk_u32x4 x = ((const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)])[q];
         qk_u32 w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          if (!(m[j] & (1u << k))) continue;
+          if (!(mm & (1u << k))) continue;
           if (flags & 1u) {
             qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)w[k] : (qk_u64)w[k];
             atomicAdd(&l_sum[c * range + g[k]], add);
@@ -513,28 +525,33 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
 
 /* ------------------------------------------------------------------ */
 /* join_build: open-addressing table of the dim keys on the device.     */
-/*  binding 0 keys (u32), 1 payload (u32, or none: payload = dim row),  */
+/*  binding 0 keys (u32), 1 dim group column (u32/i32, payload mode 1), */
 /*  2 table (u64 slots, 0 = empty, entry = (payload+1) << 32 | key),    */
 /*  3 flags (u32[2]: [0] duplicate key seen, [1] table full),           */
-/*  4 row list (u32 dim rows to insert, or none: all rows)              */
-/*  push 0 rows to insert, words[2] log2 slots                          */
+/*  4 row list (u64 dim row ids from a scan, or none: all rows)         */
+/*  push 0 rows to insert, words[2] log2 slots, words[3] payload mode   */
+/*  (0 dim row, 1 group code = code(group) - words[4], 2 zero),          */
+/*  words[5] group column type                                          */
 /* Slot of a key: (key * 0x9E3779B1) >> (32 - log2 slots), linear probe  */
 /* (the hashing of join_probe_native.comp / query_exec.c:8149-8191).    */
 /* ------------------------------------------------------------------ */
 extern "C" __global__ void LB k_join_build(QK_Args a)
 {
   const qk_u64 n = qk_push(a, 0);
-  const qk_u32 lg = a.words[2];
+  const qk_u32 lg = a.words[QK_W_LOG2];
+  const qk_u32 pmode = a.words[QK_W_PMODE];
+  const qk_u32 gmin = a.words[QK_W_GMIN];
+  const qk_u32 gtype = a.words[QK_W_GTYPE];
   const qk_u64 mask = (1ULL << lg) - 1;
   const qk_u32* keys = qk_ptr<qk_u32>(a, 0);
-  const qk_u32* payload = a.ptr[1] ? qk_ptr<qk_u32>(a, 1) : nullptr;
-  const qk_u32* list = a.ptr[4] ? qk_ptr<qk_u32>(a, 4) : nullptr;
+  const qk_u32* groups = qk_ptr<qk_u32>(a, 1);
+  const qk_u64* list = a.ptr[4] ? qk_ptr<qk_u64>(a, 4) : nullptr;
   qk_u64* table = qk_wptr<qk_u64>(a, 2);
   qk_u32* flags = qk_wptr<qk_u32>(a, 3);
   for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
-    qk_u32 row = list ? list[i] : (qk_u32)i;
+    qk_u32 row = list ? (qk_u32)list[i] : (qk_u32)i;
     qk_u32 key = keys[row];
-    qk_u32 pay = payload ? payload[row] : row;
+    qk_u32 pay = pmode == 0 ? row : (pmode == 1 ? qk_code32(groups[row], gtype) - gmin : 0u);
     qk_u64 entry = ((qk_u64)(pay + 1u) << 32) | key;
     qk_u64 slot = lg ? ((qk_u64)(key * 0x9E3779B1u) >> (32 - lg)) : 0;
     qk_u64 probes = 0;
@@ -581,13 +598,13 @@ extern "C" __global__ void LB k_join_probe(QK_Args a)
   const qk_u64 rows = qk_push(a, QK_P_ROWS);
   const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
   const qk_u64 r1 = r0 + rows;
-  const qk_u32 lg = a.words[2];
+  const qk_u32 lg = a.words[QK_W_LOG2];
   const qk_u64 mask = (1ULL << lg) - 1;
-  const qk_u32 nargs = a.words[12] & 0xffu;
-  const qk_u32 slots = a.words[12] >> 8;
-  const qk_u32 types = a.words[13];
-  const qk_u32 range = a.words[11];
-  const qk_u32 flags = a.words[14] >> 16;
+  const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
+  const qk_u32 types = a.words[QK_W_TYPES];
+  const qk_u32 range = a.words[QK_W_RANGE];
+  const qk_u32 flags = a.words[QK_W_KEY] >> 16;
   const bool per_row = (flags & 4u) != 0;
   const qk_u64* table = qk_ptr<qk_u64>(a, 2);
   qk_u64* g_cnt = qk_wptr<qk_u64>(a, 1);
@@ -635,7 +652,7 @@ extern "C" __global__ void LB k_join_probe(QK_Args a)
     for (int j = 0; j < QK_UNROLL; j++) {
       qk_u32 hit = 0;
 #pragma unroll
-      for (int k = 0; k < 4; k++) hit |= pay[j][k] ? (1u << k) : 0u;
+      for (int k = 0; k < 4; k++) hit |= (pay[j][k] && pay[j][k] - 1u < range) ? (1u << k) : 0u;
       if (!hit) continue;
       qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
 #pragma unroll
@@ -757,7 +774,7 @@ extern "C" __global__ void LB k_zonemap_build(QK_Args a)
 {
   __shared__ double s_mn[QK_WG / 64], s_mx[QK_WG / 64];
   qk_u64 n = qk_push(a, 0);
-  qk_u32 type = a.words[2], shift = a.words[3];
+  qk_u32 type = a.words[QK_W_ZTYPE], shift = a.words[QK_W_ZSHIFT];
   qk_u64 zones = (n + (1ULL << shift) - 1) >> shift;
   for (qk_u64 z = blockIdx.x; z < zones; z += gridDim.x) {
     qk_u64 z0 = z << shift, z1 = min(n, (z + 1) << shift);

@@ -27,7 +27,9 @@
 extern "C" {
 #endif
 
-/* base types of base_core.h */
+/* base types of base_core.h; a ComputeQL translation unit that already has
+ * them (and gpu.h / gpu_backend.h) defines CQLHIP_IN_COMPUTEQL first */
+#ifndef CQLHIP_IN_COMPUTEQL
 typedef uint8_t  U8;
 typedef uint16_t U16;
 typedef uint32_t U32;
@@ -130,6 +132,7 @@ typedef struct GPU_Backend {
   void* (*scratch_arena)(void);
   String8 (*name)(void);
 } GPU_Backend;
+#endif /* CQLHIP_IN_COMPUTEQL */
 
 /* The filled table; replaces the Vulkan registration of gpu_dispatch.c:1-28. */
 GPU_Backend* gpu_hip_backend(void);
@@ -215,7 +218,10 @@ const char* gpu_hip_last_error(void);
 /* buffer's byte size, so shaders' .length() reads carry over.         */
 /* ------------------------------------------------------------------ */
 #define QK_MAX_BINDINGS 16
-#define QK_PUSH_WORDS 16
+/* 16 words = the reference's 8 x U64 push constants; words 16..31 carry the
+ * operator parameters of the aggregate/join kernels (no U64 slot overlaps
+ * them) */
+#define QK_PUSH_WORDS 32
 typedef struct QK_Args {
   U64 ptr[QK_MAX_BINDINGS];    /* device addresses by binding index */
   U64 bytes[QK_MAX_BINDINGS];  /* byte sizes by binding index */

@@ -76,7 +76,12 @@ def test_scan_ids_and_count(events, orc, case, path):
         expect = orc.scan(host, where)
         n, ids, tr = t.scan(where)
         assert n == len(expect), (where, tr)
-        assert np.array_equal(ids, expect), where
+        if not np.array_equal(ids, expect):
+            bad = np.flatnonzero(ids != expect)
+            diff = sorted(set(ids.tolist()) ^ set(expect.tolist()))
+            pytest.fail(f"{where}: {len(bad)} positions differ (first {bad[:4].tolist()}: got {ids[bad[:4]].tolist()} "
+                        f"want {expect[bad[:4]].tolist()}); sorted={bool(np.all(np.diff(ids.astype(np.int64)) > 0))}; "
+                        f"set difference {len(diff)} {diff[:6]}; trace {tr}")
         cnt, _, _ = t.scan(where, count_only=True)
         assert cnt == len(expect)
         if path == "interpreter" and tr["path"] not in (5, 6):
