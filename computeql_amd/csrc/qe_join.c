@@ -114,6 +114,149 @@ static B32 qe_dim_u32(QE_Table* dim, U32 col, U32* out)
   return qe_table_column_read(dim, col, 0, dim->row_count, out);
 }
 
+/* the radix-partitioned probe for large dimension tables: build per-
+ * partition tables, scatter the fact rows into partition order, probe each
+ * partition from LDS.  Fills host_acc (layout of qe_dense_totals_bytes) and
+ * fl[2] (duplicate key, table full). */
+static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_col, const U32* arg_type, U32 arg_count,
+                               B32 need_sum, B32 need_minmax, QE_Table* dim, U32 dim_key_col, S32 dim_group_col,
+                               B32 group_mode, U32 gmin, B32 group_i32, GPU_Buffer* list, U64 pass_count, U64 R,
+                               GPU_Buffer* acc, U64 acc_bytes, void* host_acc, U32* fl, QE_Trace* trace)
+{
+  const U32 sbits = 12, S = 1u << sbits;
+  U32 pbits = 0;
+  while (((U64)1 << pbits) * (S * 45 / 100) < pass_count && pbits < 12) pbits++;
+  const U64 P = 1ull << pbits;
+  if (P * (S * 9 / 10) < pass_count) {
+    qe_set_error("qe_aggregate_join: %llu dim keys exceed the partitioned tables", (unsigned long long)pass_count);
+    return 0;
+  }
+  const U64 rows = fact->row_count;
+  const U32 B = qe_stream_blocks((rows + 3) / 4);
+  const U32 W = 1 + arg_count;
+  GPU_Kernel* kb = qe_kernel("join_part_build");
+  GPU_Kernel* kh = qe_kernel("join_part_hist");
+  GPU_Kernel* ko = qe_kernel("join_part_offsets");
+  GPU_Kernel* ks = qe_kernel("join_part_scatter");
+  GPU_Kernel* kp = qe_kernel("join_part_probe");
+  GPU_Buffer* table = qe_scratch("jp_table", P * S * 8);
+  GPU_Buffer* flags = qe_scratch("join_flags", 64);
+  GPU_Buffer* hist = qe_scratch("jp_hist", (U64)B * P * 4);
+  GPU_Buffer* off = qe_scratch("jp_off", (U64)B * P * 4);
+  GPU_Buffer* totals = qe_scratch("jp_totals", P * 4);
+  GPU_Buffer* rowsbuf = qe_scratch("jp_rows", rows * W * 4);
+  B32 ok = kb && kh && ko && ks && kp && table && flags && hist && off && totals && rowsbuf;
+  U64 cus = gpu_hip_compute_units();
+  U32 split = (U32)((cus * 4 + P - 1) / P);
+  if (split < 1) split = 1;
+  U64 acc_lds = group_mode ? R * 4 + (need_sum ? R * 8 * arg_count : 0) + (need_minmax ? R * 8 * arg_count : 0) : 0;
+  U64 probe_lds = P * 8 + (U64)S * 8 + acc_lds;
+  if (ok && probe_lds > gpu_hip_max_lds_bytes()) {
+    qe_set_error("qe_aggregate_join: partition probe needs %llu bytes of LDS", (unsigned long long)probe_lds);
+    ok = 0;
+  }
+  if (ok) {
+    /* build */
+    gpu_kernel_set_arg_buffer(kb, 0, dim->columns[dim_key_col].buffer);
+    if (dim_group_col >= 0) gpu_kernel_set_arg_buffer(kb, 1, dim->columns[dim_group_col].buffer);
+    else gpu_hip_kernel_clear_arg(kb, 1);
+    gpu_kernel_set_arg_buffer(kb, 2, table);
+    gpu_kernel_set_arg_buffer(kb, 3, flags);
+    if (list) gpu_kernel_set_arg_buffer(kb, 4, list);
+    else gpu_hip_kernel_clear_arg(kb, 4);
+    gpu_kernel_set_arg_u64(kb, 0, pass_count);
+    qe_kernel_set_word(kb, QK_W_PMODE, group_mode ? (dim_group_col >= 0 ? 1u : 2u) : 0u);
+    qe_kernel_set_word(kb, QK_W_GMIN, gmin);
+    qe_kernel_set_word(kb, QK_W_GTYPE, group_i32 ? CQL_ColumnType_I32 : CQL_ColumnType_U32);
+    qe_kernel_set_word(kb, QK_W_PBITS, pbits);
+    qe_kernel_set_word(kb, QK_W_SBITS, sbits);
+    /* fact columns: slot 0 the key, then the args */
+    QE_ColumnSlots slots;
+    memset(&slots, 0, sizeof(slots));
+    qe_slot_of(&slots, fact_key_col);
+    U32 slot_bits = 0, type_bits = 0;
+    for (U32 a = 0; a < arg_count; a++) {
+      slot_bits |= qe_slot_of(&slots, arg_col[a]) << (4 * a);
+      type_bits |= arg_type[a] << (8 * a);
+    }
+    GPU_Kernel* fk_kernels[2] = {kh, ks};
+    for (U32 i = 0; i < 2; i++) {
+      GPU_Kernel* k = fk_kernels[i];
+      for (U32 sl = 0; sl < QE_MAX_COLUMN_BINDINGS; sl++) {
+        if (sl < slots.count) gpu_kernel_set_arg_buffer(k, 6 + sl, fact->columns[slots.column[sl]].buffer);
+        else gpu_hip_kernel_clear_arg(k, 6 + sl);
+      }
+      gpu_kernel_set_arg_u64(k, 0, rows);
+      gpu_kernel_set_arg_u64(k, 4, 0);
+      gpu_kernel_set_arg_u64(k, 5, 0);
+      qe_kernel_set_word(k, QK_W_PBITS, pbits);
+      qe_kernel_set_word(k, QK_W_ROWW, W);
+      qe_kernel_set_word(k, QK_W_ARGS, arg_count | (slot_bits << 8));
+    }
+    gpu_kernel_set_arg_buffer(kh, 2, hist);
+    gpu_hip_kernel_set_lds_bytes(kh, (U32)(P * 4));
+    gpu_kernel_set_arg_buffer(ko, 2, hist);
+    gpu_kernel_set_arg_buffer(ko, 3, off);
+    gpu_kernel_set_arg_buffer(ko, 4, totals);
+    gpu_kernel_set_arg_u64(ko, 0, B);
+    qe_kernel_set_word(ko, QK_W_PBITS, pbits);
+    gpu_kernel_set_arg_buffer(ks, 2, rowsbuf);
+    gpu_kernel_set_arg_buffer(ks, 3, off);
+    gpu_kernel_set_arg_buffer(ks, 4, totals);
+    gpu_hip_kernel_set_lds_bytes(ks, (U32)(P * 12));
+    U32 kflags = (need_sum ? 1u : 0u) | (need_minmax ? 2u : 0u) | (group_mode ? 0u : 4u);
+    gpu_kernel_set_arg_buffer(kp, 1, acc);
+    gpu_kernel_set_arg_buffer(kp, 2, table);
+    gpu_kernel_set_arg_buffer(kp, 3, rowsbuf);
+    gpu_kernel_set_arg_buffer(kp, 4, totals);
+    qe_kernel_set_word(kp, QK_W_PBITS, pbits);
+    qe_kernel_set_word(kp, QK_W_SBITS, sbits);
+    qe_kernel_set_word(kp, QK_W_ROWW, W);
+    qe_kernel_set_word(kp, QK_W_SPLIT, split);
+    qe_kernel_set_word(kp, QK_W_ARGS, arg_count);
+    qe_kernel_set_word(kp, QK_W_TYPES, type_bits);
+    qe_kernel_set_word(kp, QK_W_RANGE, (U32)R);
+    qe_kernel_set_word(kp, QK_W_KEY, kflags << 16);
+    gpu_hip_kernel_set_lds_bytes(kp, (U32)probe_lds);
+    U64 build_threads = pass_count < 256 ? 256 : pass_count;
+    if (build_threads > cus * 4 * 256) build_threads = cus * 4 * 256;
+    U64 zero_bytes = R * 8 * (1 + arg_count);
+    GPU_Batch* b = gpu_batch_begin(0, acc_bytes + 8);
+    gpu_batch_buffer_zero(b, table, P * S * 8);
+    gpu_batch_buffer_zero(b, flags, 8);
+    gpu_batch_buffer_zero(b, acc, acc_bytes);
+    if (arg_count && need_minmax) {
+      gpu_batch_buffer_fill(b, acc, zero_bytes + R * 4 * arg_count, 0xffffffffu);
+      gpu_batch_buffer_zero(b, acc, zero_bytes);
+    }
+    gpu_batch_kernel_execute(b, kb, (U32)build_threads, QE_WG);
+    gpu_batch_kernel_execute(b, kh, B * QE_WG, QE_WG);
+    gpu_batch_kernel_execute(b, ko, (U32)(P * QE_WG), QE_WG);
+    gpu_batch_kernel_execute(b, ks, B * QE_WG, QE_WG);
+    gpu_batch_kernel_execute(b, kp, (U32)(P * split * QE_WG), QE_WG);
+    gpu_batch_buffer_read(b, flags, fl, 8);
+    gpu_batch_buffer_read(b, acc, host_acc, acc_bytes);
+    ok = gpu_batch_end(b);
+    gpu_hip_kernel_set_lds_bytes(kh, 0);
+    gpu_hip_kernel_set_lds_bytes(ks, 0);
+    gpu_hip_kernel_set_lds_bytes(kp, 0);
+    if (!ok) qe_set_error("partitioned join: %s", gpu_hip_last_error());
+    if (trace) {
+      trace->kernel_count = 5;
+      trace->rows_scanned = rows;
+      trace->bytes_read = rows * 4 * (U64)slots.count;
+      snprintf(trace->kernel, sizeof(trace->kernel), "join_part_probe");
+    }
+  }
+  qe_scratch_release(table);
+  qe_scratch_release(flags);
+  qe_scratch_release(hist);
+  qe_scratch_release(off);
+  qe_scratch_release(totals);
+  qe_scratch_release(rowsbuf);
+  return ok;
+}
+
 B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_key_col, const QE_IRNode* dim_where,
                       U32 dim_where_count, S32 dim_group_col, const U32* funcs, const S32* fact_arg_cols, U32 expr_count,
                       U64 cap, U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace)
@@ -287,7 +430,22 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     void* host_acc = malloc(acc_bytes ? acc_bytes : 8);
     ok = kb && kp && table && flags && acc && host_acc && (group_mode || pass_count == dn || list_buf);
     U32 fl[2] = {0, 0};
-    if (ok) {
+    B32 partitioned = qe_settings_u64("QE_JOIN_PARTITION", 1) != 0 &&
+                      pass_count >= qe_settings_u64("QE_JOIN_PARTITION_MIN_KEYS", 32768) && fact->row_count >= (1u << 20);
+    if (ok && partitioned) {
+      path = group_mode ? QE_AggPath_JoinPartGroup : QE_AggPath_JoinPartRow;
+      ok = qe_join_partitioned(fact, fact_key_col, arg_col, arg_type, arg_count, need_sum, need_minmax, dim, dim_key_col,
+                               dim_group_col, group_mode, gmin, group_i32, pass_dev ? pass_dev : list_buf, pass_count, R,
+                               acc, acc_bytes, host_acc, fl, trace);
+      if (ok && fl[0]) {
+        ok = 0;
+        qe_set_error("qe_aggregate_join: duplicate dim keys (the probe form needs unique keys)");
+      }
+      if (ok && fl[1]) {
+        ok = 0;
+        qe_set_error("qe_aggregate_join: partition table full");
+      }
+    } else if (ok) {
       gpu_kernel_set_arg_buffer(kb, 0, dim->columns[dim_key_col].buffer);
       if (dim_group_col >= 0) gpu_kernel_set_arg_buffer(kb, 1, dim->columns[dim_group_col].buffer);
       else gpu_hip_kernel_clear_arg(kb, 1);

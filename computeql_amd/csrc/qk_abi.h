@@ -24,5 +24,9 @@
 #define QK_W_GTYPE 24    /* join_build: group column type */
 #define QK_W_ZTYPE 25    /* zonemap_build: column type */
 #define QK_W_ZSHIFT 26   /* zonemap_build: log2 rows per zone */
+#define QK_W_PBITS 27    /* partitioned join: log2 partitions */
+#define QK_W_SBITS 28    /* partitioned join: log2 slots per partition table */
+#define QK_W_ROWW 29     /* partitioned join: u32 words per partitioned row (1 + args) */
+#define QK_W_SPLIT 30    /* partitioned join: probe blocks per partition */
 
 #endif

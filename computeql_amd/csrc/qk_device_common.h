@@ -186,14 +186,87 @@ __device__ __forceinline__ qk_u32 qk_leaf_quad(const QK_Args& a, const qk_u32* w
   return m;
 }
 
-/* AND of leaf_count leaves (<= 8) */
+/* AND of leaf_count leaves (<= 8) over one quad */
 __device__ __forceinline__ qk_u32 qk_leaves_quad(const QK_Args& a, const qk_u32* leaves, qk_u32 leaf_count, qk_u64 q,
                                                  qk_u32 mask)
 {
-  for (qk_u32 l = 0; l < leaf_count && l < 8u; l++) {
-    if (mask) mask &= qk_leaf_quad(a, leaves + 6 * l, q);
-  }
+  for (qk_u32 l = 0; l < leaf_count && l < 8u; l++) mask &= qk_leaf_quad(a, leaves + 6 * l, q);
   return mask;
+}
+
+/* the leaves of a numeric-AND WHERE held in registers for a whole kernel:
+ * loaded once (uniform, scalar loads) instead of once per step */
+struct QkLeafSet {
+  qk_u32 n;
+  qk_u32 w[8][5];
+};
+
+__device__ __forceinline__ QkLeafSet qk_load_leaves(const qk_u32* leaves, qk_u32 n)
+{
+  QkLeafSet L;
+  L.n = n < 8u ? n : 8u;
+#pragma unroll
+  for (int l = 0; l < 8; l++)
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+      L.w[l][i] = __builtin_amdgcn_readfirstlane(((qk_u32)l < L.n) ? leaves[6 * l + i] : 0u);
+  return L;
+}
+
+/* the AND over the QK_UNROLL quads of a step, leaf-major: the loads of one
+ * leaf for every quad are issued together before any compare */
+__device__ __forceinline__ void qk_leaves_step(const QK_Args& a, const QkLeafSet& L, const qk_u64* q, qk_u32* m)
+{
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    if ((qk_u32)l >= L.n) break;
+    const qk_u32* w = L.w[l];
+    qk_u32 slot = w[0], tw = w[1];
+    qk_u32 type = tw & 0xffu;
+    if (tw & QK_LEAF_NEVER) {
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) m[j] = 0u;
+      continue;
+    }
+    if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
+      qk_u32 lo = w[3], span = w[4];
+      qk_u32 bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
+      const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + slot];
+      qk_u32x4 x[QK_UNROLL];
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) {
+        qk_u32 mm = 0;
+        mm |= (((x[j].x ^ bias) - lo) <= span) ? 1u : 0u;
+        mm |= (((x[j].y ^ bias) - lo) <= span) ? 2u : 0u;
+        mm |= (((x[j].z ^ bias) - lo) <= span) ? 4u : 0u;
+        mm |= (((x[j].w ^ bias) - lo) <= span) ? 8u : 0u;
+        m[j] &= mm;
+      }
+    } else {
+      qk_u32 ww[6] = {w[0], w[1], w[2], w[3], w[4], 0u};
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) m[j] &= qk_leaf_quad(a, ww, q[j]);
+    }
+  }
+}
+
+/* the QK_UNROLL quads a thread owns in the step starting at s: indices
+ * clamped into [.., b1) so every load is in bounds, and valid-row masks
+ * (0 for quads past the block range or in skipped zones) */
+__device__ __forceinline__ void qk_step_quads(const QK_Args& a, qk_u64 s, qk_u64 b1, qk_u64 r0, qk_u64 r1, qk_u64* q,
+                                              qk_u32* valid)
+{
+#pragma unroll
+  for (int j = 0; j < QK_UNROLL; j++) {
+    qk_u64 qq = s + (qk_u64)j * QK_WG + threadIdx.x;
+    bool in = qq < b1;
+    qk_u32 v = in ? qk_quad_valid(qq, r0, r1) : 0u;
+    if (v && qk_quad_skipped(a, qq)) v = 0u;
+    valid[j] = v;
+    q[j] = in ? qq : b1 - 1;
+  }
 }
 
 /* ---- wave64 / block reductions ---- */
@@ -251,8 +324,21 @@ __device__ __forceinline__ qk_u32 qk_block_excl_scan_u32(qk_u32 v, qk_u32* lds4,
 
 /* ---- the scan driver: count / per-block counts / ordered row ids ----
  * pred(q, mask) returns the match mask of quad q given the valid mask. */
+/* a per-quad predicate p(q, valid) run over the quads of a step; the
+ * quads' loads are unconditional (clamped indices) so they issue together */
+template <class P>
+struct QkPerQuad {
+  P p;
+  __device__ __forceinline__ void eval(const qk_u64* q, const qk_u32* valid, qk_u32* m) const
+  {
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) m[j] = p(q[j], valid[j]) & valid[j];
+  }
+};
+
+/* pred.eval(q[QK_UNROLL], valid[QK_UNROLL], m[QK_UNROLL]) */
 template <class Pred>
-__device__ __forceinline__ void qk_scan_driver(const QK_Args& a, Pred pred)
+__device__ __forceinline__ void qk_scan_driver_step(const QK_Args& a, Pred pred)
 {
   __shared__ qk_u64 s_red[QK_WG / 64];
   __shared__ qk_u32 s_scan[QK_WG / 64];
@@ -266,14 +352,10 @@ __device__ __forceinline__ void qk_scan_driver(const QK_Args& a, Pred pred)
   if (mode != QK_MODE_WRITE_IDS) {
     qk_u64 count = 0;
     for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
-      qk_u32 m[QK_UNROLL];
-#pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) {
-        qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
-        qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
-        if (valid && qk_quad_skipped(a, q)) valid = 0u;
-        m[j] = valid ? pred(q, valid) : 0u;
-      }
+      qk_u64 q[QK_UNROLL];
+      qk_u32 valid[QK_UNROLL], m[QK_UNROLL];
+      qk_step_quads(a, s, b1, r0, r1, q, valid);
+      pred.eval(q, valid, m);
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) count += (qk_u64)__popc(m[j]);
     }
@@ -314,19 +396,15 @@ __device__ __forceinline__ void qk_scan_driver(const QK_Args& a, Pred pred)
   qk_u64* out = qk_wptr<qk_u64>(a, QK_B_OUT_IDS);
   const qk_u64 cap = a.bytes[QK_B_OUT_IDS] / 8;
   for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
-    qk_u32 m[QK_UNROLL];
-#pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
-      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
-      qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
-      if (valid && qk_quad_skipped(a, q)) valid = 0u;
-      m[j] = valid ? pred(q, valid) : 0u;
-    }
+    qk_u64 qs[QK_UNROLL];
+    qk_u32 valid[QK_UNROLL], m[QK_UNROLL];
+    qk_step_quads(a, s, b1, r0, r1, qs, valid);
+    pred.eval(qs, valid, m);
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) {
       qk_u32 total = 0;
       qk_u32 pre = qk_block_excl_scan_u32((qk_u32)__popc(m[j]), s_scan, total);
-      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+      qk_u64 q = qs[j];
       qk_u64 o = base + pre;
       qk_u32 mm = m[j];
       while (mm) {
@@ -338,6 +416,13 @@ __device__ __forceinline__ void qk_scan_driver(const QK_Args& a, Pred pred)
       base += total;
     }
   }
+}
+
+/* per-quad predicates (generated kernels, the interpreter) */
+template <class P>
+__device__ __forceinline__ void qk_scan_driver(const QK_Args& a, P pred)
+{
+  qk_scan_driver_step(a, QkPerQuad<P>{pred});
 }
 
 #endif

@@ -183,7 +183,7 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
   const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
   const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
   const qk_u32 types = a.words[QK_W_TYPES];
-  const qk_u32* leaves = qk_ptr<qk_u32>(a, QK_B_LEAVES);
+  const QkLeafSet L = qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), leaf_count);
 
   qk_u64 count = 0;
   AggAcc acc[4];
@@ -196,21 +196,40 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
   for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u64 q[QK_UNROLL];
     qk_u32 m[QK_UNROLL];
+    qk_step_quads(a, s, b1, r0, r1, q, m);
+    if (L.n) qk_leaves_step(a, L, q, m);
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
-      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
-      qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
-      if (valid && qk_quad_skipped(a, q)) valid = 0u;
-      m[j] = leaf_count ? qk_leaves_quad(a, leaves, leaf_count, q, valid) : valid;
-    }
+    for (int j = 0; j < QK_UNROLL; j++) count += (qk_u64)__popc(m[j]);
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
-      count += (qk_u64)__popc(m[j]);
-      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+    for (int c = 0; c < 4; c++) {
+      if ((qk_u32)c >= nargs) continue;
+      qk_u32 slot = (slots >> (4 * c)) & 0xfu, type = (types >> (8 * c)) & 0xffu;
+      if (type == QK_T_U32) {
+        /* the hot case: the U loads of the step in flight together */
+        const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + slot];
+        qk_u32x4 x[QK_UNROLL];
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        if ((qk_u32)c < nargs && m[j]) agg_fold_quad(a, (slots >> (4 * c)) & 0xfu, (types >> (8 * c)) & 0xffu, q, m[j], acc[c]);
+        for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) {
+          qk_u32 mm = m[j];
+          qk_u32 w0 = (mm & 1u) ? x[j].x : 0u, w1 = (mm & 2u) ? x[j].y : 0u;
+          qk_u32 w2 = (mm & 4u) ? x[j].z : 0u, w3 = (mm & 8u) ? x[j].w : 0u;
+          acc[c].sum += (qk_u64)w0 + (qk_u64)w1 + (qk_u64)w2 + (qk_u64)w3;
+          qk_u32 lo = min(min((mm & 1u) ? x[j].x : 0xffffffffu, (mm & 2u) ? x[j].y : 0xffffffffu),
+                          min((mm & 4u) ? x[j].z : 0xffffffffu, (mm & 8u) ? x[j].w : 0xffffffffu));
+          qk_u32 hi = max(max(w0, w1), max(w2, w3));
+          if (mm) {
+            acc[c].mn = min(acc[c].mn, (qk_u64)lo);
+            acc[c].mx = max(acc[c].mx, (qk_u64)hi);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++)
+          if (m[j]) agg_fold_quad(a, slot, type, q[j], m[j], acc[c]);
       }
     }
   }
@@ -264,20 +283,56 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  /* fixed-order fold: thread f < 13 walks the blocks in order */
+  /* fixed-order fold, all threads: thread t walks blocks t, t+256, ... in
+   * order, then the 256 partials meet in a fixed butterfly + wave order */
+  qk_u64 v[13];
+  v[0] = 0;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    v[1 + 3 * c] = 0;
+    v[2 + 3 * c] = ~0ULL;
+    v[3 + 3 * c] = 0;
+  }
+  for (qk_u32 b = threadIdx.x; b < gridDim.x; b += QK_WG) {
+    const qk_u64* pb = &partials[(qk_u64)b * 13];
+    v[0] += __hip_atomic_load(&pb[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if ((qk_u32)c >= nargs) continue;
+      qk_u32 t = (types >> (8 * c)) & 0xffu;
+      v[1 + 3 * c] = agg_combine_sum(t, v[1 + 3 * c], __hip_atomic_load(&pb[1 + 3 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      v[2 + 3 * c] = min(v[2 + 3 * c], (qk_u64)__hip_atomic_load(&pb[2 + 3 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      v[3 + 3 * c] = max(v[3 + 3 * c], (qk_u64)__hip_atomic_load(&pb[3 + 3 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v[0] += __shfl_xor(v[0], o, 64);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      qk_u32 t = (types >> (8 * c)) & 0xffu;
+      v[1 + 3 * c] = agg_combine_sum(t, v[1 + 3 * c], __shfl_xor(v[1 + 3 * c], o, 64));
+      v[2 + 3 * c] = min(v[2 + 3 * c], (qk_u64)__shfl_xor(v[2 + 3 * c], o, 64));
+      v[3 + 3 * c] = max(v[3 + 3 * c], (qk_u64)__shfl_xor(v[3 + 3 * c], o, 64));
+    }
+  }
+  __syncthreads();
+  if (lane == 0)
+    for (int f = 0; f < 13; f++) s_part[wave][f] = v[f];
+  __syncthreads();
   if (threadIdx.x < 13) {
     int f = threadIdx.x;
-    qk_u64 v = __hip_atomic_load(&partials[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (qk_u32 b = 1; b < gridDim.x; b++) {
-      qk_u64 x = __hip_atomic_load(&partials[(qk_u64)b * 13 + f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (f == 0) v += x;
+    qk_u64 r = s_part[0][f];
+    for (int w = 1; w < QK_WG / 64; w++) {
+      qk_u64 x = s_part[w][f];
+      if (f == 0) r += x;
       else {
         int c = (f - 1) / 3, kind = (f - 1) % 3;
         qk_u32 t = (types >> (8 * c)) & 0xffu;
-        v = kind == 0 ? agg_combine_sum(t, v, x) : (kind == 1 ? min(v, x) : max(v, x));
+        r = kind == 0 ? agg_combine_sum(t, r, x) : (kind == 1 ? min(r, x) : max(r, x));
       }
     }
-    result[f] = v;
+    result[f] = r;
   }
   if (threadIdx.x == 0) __hip_atomic_store((qk_u32*)&result[15], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -286,20 +341,21 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
 /* scan_simple: numeric-AND leaves, count / block counts / ordered ids  */
 /* (scan bindings and push slots of query_exec.h:4-40)                  */
 /* ------------------------------------------------------------------ */
-struct SimplePred {
+struct SimpleStepPred {
   const QK_Args* a;
-  const qk_u32* leaves;
-  qk_u32 leaf_count;
-  __device__ __forceinline__ qk_u32 operator()(qk_u64 q, qk_u32 valid) const
+  QkLeafSet L;
+  __device__ __forceinline__ void eval(const qk_u64* q, const qk_u32* valid, qk_u32* m) const
   {
-    return qk_leaves_quad(*a, leaves, leaf_count, q, valid);
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) m[j] = valid[j];
+    qk_leaves_step(*a, L, q, m);
   }
 };
 
 extern "C" __global__ void LB k_scan_simple(QK_Args a)
 {
-  SimplePred p{&a, qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES)};
-  qk_scan_driver(a, p);
+  SimpleStepPred p{&a, qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES))};
+  qk_scan_driver_step(a, p);
 }
 
 /* ------------------------------------------------------------------ */
@@ -438,7 +494,7 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
   const qk_u32 flags = a.words[QK_W_KEY] >> 16;
   const qk_u32 key_min = a.words[QK_W_KEY_MIN];
   const qk_u32 range = a.words[QK_W_RANGE];
-  const qk_u32* leaves = qk_ptr<qk_u32>(a, QK_B_LEAVES);
+  const QkLeafSet L = qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), leaf_count);
 
   /* LDS: sums u64[E][R] | counts u32[R] | min u32[E][R] | max u32[E][R] */
   qk_u64* l_sum = lds_u64;
@@ -459,46 +515,49 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
   for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u64 q[QK_UNROLL];
     qk_u32 m[QK_UNROLL];
     qk_u32x4 kv[QK_UNROLL];
+    qk_step_quads(a, s, b1, r0, r1, q, m);
+    if (L.n) qk_leaves_step(a, L, q, m);
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
+    qk_u32 g[QK_UNROLL][4];
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) {
-      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
-      qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
-      if (valid && qk_quad_skipped(a, q)) valid = 0u;
-      m[j] = leaf_count ? qk_leaves_quad(a, leaves, leaf_count, q, valid) : valid;
-      kv[j] = m[j] ? keys[q] : qk_u32x4{0, 0, 0, 0};
-    }
-#pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
-      if (!m[j]) continue;
-      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
-      qk_u32 g[4] = {qk_code32(kv[j].x, key_type) - key_min, qk_code32(kv[j].y, key_type) - key_min,
-                     qk_code32(kv[j].z, key_type) - key_min, qk_code32(kv[j].w, key_type) - key_min};
+      g[j][0] = qk_code32(kv[j].x, key_type) - key_min;
+      g[j][1] = qk_code32(kv[j].y, key_type) - key_min;
+      g[j][2] = qk_code32(kv[j].z, key_type) - key_min;
+      g[j][3] = qk_code32(kv[j].w, key_type) - key_min;
       /* a key outside [key_min, key_min + range) (stale zone map) is
        * dropped rather than written out of the table */
-      qk_u32 mm = m[j];
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        if (g[k] >= range) mm &= ~(1u << k);
+        if (g[j][k] >= range) m[j] &= ~(1u << k);
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        if (mm & (1u << k)) atomicAdd(&l_cnt[g[k]], 1u);
-      for (qk_u32 c = 0; c < nargs; c++) {
-        qk_u32 at = (types >> (8 * c)) & 0xffu;
-        qk_u32x4 x = ((const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)])[q];
-        qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+        if (m[j] & (1u << k)) atomicAdd(&l_cnt[g[j][k]], 1u);
+    }
+    for (qk_u32 c = 0; c < nargs; c++) {
+      qk_u32 at = (types >> (8 * c)) & 0xffu;
+      const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)];
+      qk_u32x4 x[QK_UNROLL];
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) {
+        qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          if (!(mm & (1u << k))) continue;
+          if (!(m[j] & (1u << k))) continue;
           if (flags & 1u) {
             qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)w[k] : (qk_u64)w[k];
-            atomicAdd(&l_sum[c * range + g[k]], add);
+            atomicAdd(&l_sum[c * range + g[j][k]], add);
           }
           if (flags & 2u) {
             qk_u32 code = qk_code32(w[k], at);
-            atomicMin(&l_min[c * range + g[k]], code);
-            atomicMax(&l_max[c * range + g[k]], code);
+            atomicMin(&l_min[c * range + g[j][k]], code);
+            atomicMax(&l_max[c * range + g[j][k]], code);
           }
         }
       }
@@ -580,10 +639,10 @@ extern "C" __global__ void LB k_join_build(QK_Args a)
 /*  (1 sums, 2 min/max, 4 payload is a dim row: global atomics into     */
 /*  per-dim-row totals instead of LDS tables)                           */
 /* ------------------------------------------------------------------ */
-__device__ __forceinline__ qk_u32 qk_probe(const qk_u64* table, qk_u32 lg, qk_u64 mask, qk_u32 key)
+/* payload+1 of `key` walking from `slot`, 0 when absent */
+__device__ __forceinline__ qk_u32 qk_probe_from(const qk_u64* table, qk_u64 mask, qk_u64 slot, qk_u32 key)
 {
-  qk_u64 slot = lg ? ((qk_u64)(key * 0x9E3779B1u) >> (32 - lg)) : 0;
-  for (qk_u64 p = 0; p <= mask; p++) {
+  for (qk_u64 p = 0; p < mask; p++) {
     qk_u64 e = table[slot];
     if (e == 0ULL) return 0u;
     if ((qk_u32)e == key) return (qk_u32)(e >> 32);
@@ -631,44 +690,68 @@ extern "C" __global__ void LB k_join_probe(QK_Args a)
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
   for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u64 qs[QK_UNROLL];
     qk_u32 m[QK_UNROLL];
     qk_u32x4 kv[QK_UNROLL];
+    qk_step_quads(a, s, b1, r0, r1, qs, m);
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
-      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
-      qk_u32 valid = q < b1 ? qk_quad_valid(q, r0, r1) : 0u;
-      if (valid && qk_quad_skipped(a, q)) valid = 0u;
-      m[j] = valid;
-      kv[j] = valid ? keys[q] : qk_u32x4{0, 0, 0, 0};
-    }
+    for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[qs[j]];
+    /* the home slots of all 16 keys are loaded together; only keys whose
+     * home slot holds another key walk on (linear probing) */
     qk_u32 pay[QK_UNROLL][4];
+    qk_u64 home[QK_UNROLL][4];
+    qk_u64 e[QK_UNROLL][4];
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) {
       qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
 #pragma unroll
-      for (int k = 0; k < 4; k++) pay[j][k] = (m[j] & (1u << k)) ? qk_probe(table, lg, mask, w[k]) : 0u;
+      for (int k = 0; k < 4; k++) {
+        home[j][k] = lg ? ((qk_u64)(w[k] * 0x9E3779B1u) >> (32 - lg)) : 0;
+        e[j][k] = table[home[j][k]];
+      }
     }
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) {
-      qk_u32 hit = 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) hit |= (pay[j][k] && pay[j][k] - 1u < range) ? (1u << k) : 0u;
-      if (!hit) continue;
-      qk_u64 q = s + (qk_u64)j * QK_WG + threadIdx.x;
+      qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        if (!(hit & (1u << k))) continue;
+        qk_u32 p = 0u;
+        if (m[j] & (1u << k)) {
+          qk_u64 ek = e[j][k];
+          if (ek != 0ULL) {
+            if ((qk_u32)ek == w[k]) p = (qk_u32)(ek >> 32);
+            else p = qk_probe_from(table, mask, (home[j][k] + 1) & mask, w[k]);
+          }
+        }
+        pay[j][k] = p;
+      }
+    }
+    qk_u32 hit[QK_UNROLL];
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      hit[j] = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) hit[j] |= (pay[j][k] && pay[j][k] - 1u < range) ? (1u << k) : 0u;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!(hit[j] & (1u << k))) continue;
         qk_u32 g = pay[j][k] - 1u;
         if (per_row) atomicAdd(&g_cnt[g], 1ULL);
         else atomicAdd(&l_cnt[g], 1u);
       }
-      for (qk_u32 c = 0; c < nargs; c++) {
-        qk_u32 at = (types >> (8 * c)) & 0xffu;
-        qk_u32x4 x = ((const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)])[q];
-        qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+    }
+    for (qk_u32 c = 0; c < nargs; c++) {
+      qk_u32 at = (types >> (8 * c)) & 0xffu;
+      const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)];
+      qk_u32x4 x[QK_UNROLL];
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) x[j] = col[qs[j]];
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) {
+        qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          if (!(hit & (1u << k))) continue;
+          if (!(hit[j] & (1u << k))) continue;
           qk_u32 g = pay[j][k] - 1u;
           qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)w[k] : (qk_u64)w[k];
           qk_u32 code = qk_code32(w[k], at);
@@ -684,6 +767,342 @@ extern "C" __global__ void LB k_join_probe(QK_Args a)
               atomicMin(&l_min[c * range + g], code);
               atomicMax(&l_max[c * range + g], code);
             }
+          }
+        }
+      }
+    }
+  }
+  if (per_row) return;
+  __syncthreads();
+  for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) {
+    qk_u32 cnt = l_cnt[i];
+    if (!cnt) continue;
+    atomicAdd(&g_cnt[i], (qk_u64)cnt);
+    for (qk_u32 c = 0; c < nargs; c++) {
+      if (flags & 1u) atomicAdd(&g_sum[(qk_u64)c * range + i], l_sum[c * range + i]);
+      if (flags & 2u) {
+        atomicMin(&g_min[(qk_u64)c * range + i], l_min[c * range + i]);
+        atomicMax(&g_max[(qk_u64)c * range + i], l_max[c * range + i]);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* radix-partitioned join (large dimension tables).                     */
+/* A key's hash h = fmix32(key) (a bijection of u32) picks its partition */
+/* p = h >> (32 - pbits) and its slot inside the partition's table from */
+/* the next sbits bits.  Fact rows are scattered into partition order   */
+/* (hist -> offsets -> scatter), then one or more blocks per partition  */
+/* load that partition's table into LDS and probe it from there, so no  */
+/* probe leaves the CU.                                                 */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ qk_u32 qk_fmix32(qk_u32 h)
+{
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+
+/* join_part_build: binding 0 dim keys, 1 dim group column, 2 partition
+ * tables (P x S u64, entry (payload+1) << 32 | key, 0 empty), 3 flags
+ * (u32[2]: duplicate, full), 4 row list (u64 ids, optional);
+ * push 0 rows; words PMODE/GMIN/GTYPE as join_build, PBITS, SBITS */
+extern "C" __global__ void LB k_join_part_build(QK_Args a)
+{
+  const qk_u64 n = qk_push(a, 0);
+  const qk_u32 pmode = a.words[QK_W_PMODE], gmin = a.words[QK_W_GMIN], gtype = a.words[QK_W_GTYPE];
+  const qk_u32 pbits = a.words[QK_W_PBITS], sbits = a.words[QK_W_SBITS];
+  const qk_u32 smask = (1u << sbits) - 1u;
+  const qk_u32* keys = qk_ptr<qk_u32>(a, 0);
+  const qk_u32* groups = qk_ptr<qk_u32>(a, 1);
+  const qk_u64* list = a.ptr[4] ? qk_ptr<qk_u64>(a, 4) : nullptr;
+  qk_u64* table = qk_wptr<qk_u64>(a, 2);
+  qk_u32* flags = qk_wptr<qk_u32>(a, 3);
+  for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
+    qk_u32 row = list ? (qk_u32)list[i] : (qk_u32)i;
+    qk_u32 key = keys[row];
+    qk_u32 pay = pmode == 0 ? row : (pmode == 1 ? qk_code32(groups[row], gtype) - gmin : 0u);
+    qk_u64 entry = ((qk_u64)(pay + 1u) << 32) | key;
+    qk_u32 h = qk_fmix32(key);
+    qk_u32 part = pbits ? (h >> (32 - pbits)) : 0u;
+    qk_u32 slot = (h >> (32 - pbits - sbits)) & smask;
+    qk_u64* t = table + ((qk_u64)part << sbits);
+    qk_u32 probes = 0;
+    for (;;) {
+      qk_u64 prev = atomicCAS(&t[slot], 0ULL, entry);
+      if (prev == 0ULL) break;
+      if ((qk_u32)prev == key) {
+        atomicOr(&flags[0], 1u);
+        break;
+      }
+      slot = (slot + 1) & smask;
+      if (++probes > smask) {
+        atomicOr(&flags[1], 1u);
+        break;
+      }
+    }
+  }
+}
+
+/* join_part_hist: binding 6 fact key column, 2 histogram out (u32
+ * [blocks][P]); push 0 rows, 5 first row; words PBITS.  The block ranges
+ * are those of qk_block_range, shared with join_part_scatter. */
+extern "C" __global__ void LB k_join_part_hist(QK_Args a)
+{
+  extern __shared__ qk_u64 lds_u64[];
+  qk_u32* hist = (qk_u32*)lds_u64;
+  const qk_u64 rows = qk_push(a, QK_P_ROWS);
+  const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
+  const qk_u64 r1 = r0 + rows;
+  const qk_u32 pbits = a.words[QK_W_PBITS];
+  const qk_u32 P = 1u << pbits;
+  for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) hist[i] = 0;
+  __syncthreads();
+  const qk_u32x4* keys = (const qk_u32x4*)a.ptr[QK_B_COL0];
+  qk_u64 b0, b1;
+  qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u64 q[QK_UNROLL];
+    qk_u32 m[QK_UNROLL];
+    qk_u32x4 kv[QK_UNROLL];
+    qk_step_quads(a, s, b1, r0, r1, q, m);
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (m[j] & (1u << k)) atomicAdd(&hist[pbits ? (qk_fmix32(w[k]) >> (32 - pbits)) : 0u], 1u);
+    }
+  }
+  __syncthreads();
+  qk_u32* out = qk_wptr<qk_u32>(a, 2) + (qk_u64)blockIdx.x * P;
+  for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) out[i] = hist[i];
+}
+
+/* join_part_offsets: one block per partition p: exclusive scan over the
+ * blocks of hist[b][p] -> off[b][p]; total[p].  binding 2 hist, 3 off,
+ * 4 totals; push 0 = hist blocks; words PBITS */
+extern "C" __global__ void LB k_join_part_offsets(QK_Args a)
+{
+  __shared__ qk_u32 s_scan[QK_WG / 64];
+  const qk_u32 nb = (qk_u32)qk_push(a, 0);
+  const qk_u32 P = 1u << a.words[QK_W_PBITS];
+  const qk_u32 p = blockIdx.x;
+  if (p >= P) return;
+  const qk_u32* hist = qk_ptr<qk_u32>(a, 2);
+  qk_u32* off = qk_wptr<qk_u32>(a, 3);
+  qk_u32 running = 0;
+  for (qk_u32 c = 0; c < nb; c += QK_WG) {
+    qk_u32 b = c + threadIdx.x;
+    qk_u32 v = b < nb ? hist[(qk_u64)b * P + p] : 0u;
+    qk_u32 total = 0;
+    qk_u32 pre = qk_block_excl_scan_u32(v, s_scan, total);
+    if (b < nb) off[(qk_u64)b * P + p] = running + pre;
+    running += total;
+  }
+  if (threadIdx.x == 0) qk_wptr<qk_u32>(a, 4)[p] = running;
+}
+
+/* exclusive scan of the P partition totals into LDS (base[p]); returns the
+ * grand total.  Every block of scatter/probe recomputes it (P <= 4096). */
+__device__ __forceinline__ qk_u64 qk_part_bases(const qk_u32* totals, qk_u32 P, qk_u64* base, qk_u32* s_scan)
+{
+  qk_u64 running = 0;
+  for (qk_u32 c = 0; c < P; c += QK_WG) {
+    qk_u32 i = c + threadIdx.x;
+    qk_u32 v = i < P ? totals[i] : 0u;
+    qk_u32 total = 0;
+    qk_u32 pre = qk_block_excl_scan_u32(v, s_scan, total);
+    if (i < P) base[i] = running + pre;
+    running += total;
+  }
+  __syncthreads();
+  return running;
+}
+
+/* join_part_scatter: binding 6 fact key, 7.. arg columns (slots in
+ * QK_W_ARGS as join_probe), 3 off, 4 totals, 2 out rows (u32, ROWW words
+ * per row: key then args); push 0 rows, 5 first row; words PBITS, ROWW */
+extern "C" __global__ void LB k_join_part_scatter(QK_Args a)
+{
+  extern __shared__ qk_u64 lds_u64[];
+  __shared__ qk_u32 s_scan[QK_WG / 64];
+  const qk_u64 rows = qk_push(a, QK_P_ROWS);
+  const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
+  const qk_u64 r1 = r0 + rows;
+  const qk_u32 pbits = a.words[QK_W_PBITS];
+  const qk_u32 P = 1u << pbits;
+  const qk_u32 W = a.words[QK_W_ROWW];
+  const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
+  qk_u64* base = lds_u64;                  /* P u64 */
+  qk_u32* cursor = (qk_u32*)(lds_u64 + P); /* P u32 */
+  qk_part_bases(qk_ptr<qk_u32>(a, 4), P, base, s_scan);
+  const qk_u32* off = qk_ptr<qk_u32>(a, 3) + (qk_u64)blockIdx.x * P;
+  for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cursor[i] = off[i];
+  __syncthreads();
+  qk_u32* out = qk_wptr<qk_u32>(a, 2);
+  const qk_u64 cap = a.bytes[2] / (4ULL * W);
+  const qk_u32x4* keys = (const qk_u32x4*)a.ptr[QK_B_COL0];
+  qk_u64 b0, b1;
+  qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    qk_u64 q[QK_UNROLL];
+    qk_u32 m[QK_UNROLL];
+    qk_u32x4 kv[QK_UNROLL];
+    qk_step_quads(a, s, b1, r0, r1, q, m);
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
+    qk_u64 pos[QK_UNROLL][4];
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        pos[j][k] = ~0ULL;
+        if (m[j] & (1u << k)) {
+          qk_u32 part = pbits ? (qk_fmix32(w[k]) >> (32 - pbits)) : 0u;
+          qk_u64 p = base[part] + atomicAdd(&cursor[part], 1u);
+          if (p < cap) {
+            pos[j][k] = p;
+            out[p * W] = w[k];
+          }
+        }
+      }
+    }
+    for (qk_u32 c = 0; c < nargs; c++) {
+      const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)];
+      qk_u32x4 x[QK_UNROLL];
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) {
+        qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (pos[j][k] != ~0ULL) out[pos[j][k] * W + 1 + c] = w[k];
+      }
+    }
+  }
+}
+
+/* join_part_probe: blocks (partition p, split i) for i < SPLIT; loads
+ * table p into LDS and probes its share of the partitioned rows;
+ * aggregates like join_probe (flags 1 sums, 2 min/max; payload = group
+ * code -> LDS group tables; flag 4 payload = dim row -> global atomics).
+ * binding 1 accumulators, 2 partition tables, 3 rows, 4 totals; words
+ * PBITS, SBITS, ROWW, SPLIT, ARGS (arg count), TYPES, RANGE, KEY (flags
+ * << 16) */
+extern "C" __global__ void LB k_join_part_probe(QK_Args a)
+{
+  extern __shared__ qk_u64 lds_u64[];
+  __shared__ qk_u32 s_scan[QK_WG / 64];
+  const qk_u32 pbits = a.words[QK_W_PBITS], sbits = a.words[QK_W_SBITS];
+  const qk_u32 P = 1u << pbits, S = 1u << sbits, smask = S - 1u;
+  const qk_u32 W = a.words[QK_W_ROWW];
+  const qk_u32 split = a.words[QK_W_SPLIT];
+  const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 types = a.words[QK_W_TYPES];
+  const qk_u32 range = a.words[QK_W_RANGE];
+  const qk_u32 flags = a.words[QK_W_KEY] >> 16;
+  const bool per_row = (flags & 4u) != 0;
+  const qk_u32 part = blockIdx.x / split, piece = blockIdx.x % split;
+  if (part >= P) return;
+  /* LDS: bases u64[P] | table u64[S] | sums u64[E][R] | cnt u32[R] | min/max u32[E][R] */
+  qk_u64* base = lds_u64;
+  qk_u64* t = base + P;
+  qk_u64* l_sum = t + S;
+  qk_u32* l_cnt = (qk_u32*)(l_sum + ((!per_row && (flags & 1u)) ? (qk_u64)nargs * range : 0));
+  qk_u32* l_min = l_cnt + (per_row ? 0u : range);
+  qk_u32* l_max = l_min + ((!per_row && (flags & 2u)) ? (qk_u64)nargs * range : 0);
+  qk_part_bases(qk_ptr<qk_u32>(a, 4), P, base, s_scan);
+  const qk_u64* gt = qk_ptr<qk_u64>(a, 2) + ((qk_u64)part << sbits);
+  for (qk_u32 i = threadIdx.x; i < S; i += QK_WG) t[i] = gt[i];
+  if (!per_row) {
+    const qk_u32 sum_words = (flags & 1u) ? nargs * range : 0u;
+    const qk_u32 mm_words = (flags & 2u) ? nargs * range : 0u;
+    for (qk_u32 i = threadIdx.x; i < sum_words; i += QK_WG) l_sum[i] = 0;
+    for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) l_cnt[i] = 0;
+    for (qk_u32 i = threadIdx.x; i < mm_words; i += QK_WG) {
+      l_min[i] = 0xffffffffu;
+      l_max[i] = 0u;
+    }
+  }
+  __syncthreads();
+  const qk_u64 pstart = base[part];
+  const qk_u64 pend = pstart + qk_ptr<qk_u32>(a, 4)[part];
+  const qk_u64 plen = pend - pstart;
+  const qk_u64 chunk = (plen + split - 1) / split;
+  const qk_u64 rs = pstart + chunk * piece;
+  const qk_u64 re = rs + chunk < pend ? rs + chunk : pend;
+  const qk_u32* rowsbuf = qk_ptr<qk_u32>(a, 3);
+  qk_u64* g_cnt = qk_wptr<qk_u64>(a, 1);
+  qk_u64* g_sum = g_cnt + range;
+  qk_u32* g_min = (qk_u32*)(g_sum + (qk_u64)nargs * range);
+  qk_u32* g_max = g_min + (qk_u64)nargs * range;
+  /* 4 rows per thread per step; their keys load together */
+  for (qk_u64 r = rs; r < re; r += (qk_u64)QK_WG * 4) {
+    qk_u64 rr[4];
+    bool ok[4];
+    qk_u32 key[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      qk_u64 x = r + (qk_u64)j * QK_WG + threadIdx.x;
+      ok[j] = x < re;
+      rr[j] = ok[j] ? x : rs;
+      key[j] = rowsbuf[rr[j] * W];
+    }
+    qk_u32 pay[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      qk_u32 h = qk_fmix32(key[j]);
+      qk_u32 slot = (h >> (32 - pbits - sbits)) & smask;
+      qk_u32 pj = 0;
+      for (qk_u32 n = 0; n < S; n++) {
+        qk_u64 e = t[slot];
+        if (e == 0ULL) break;
+        if ((qk_u32)e == key[j]) {
+          pj = (qk_u32)(e >> 32);
+          break;
+        }
+        slot = (slot + 1) & smask;
+      }
+      pay[j] = (ok[j] && pj && pj - 1u < range) ? pj : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (!pay[j]) continue;
+      if (per_row) atomicAdd(&g_cnt[pay[j] - 1u], 1ULL);
+      else atomicAdd(&l_cnt[pay[j] - 1u], 1u);
+    }
+    for (qk_u32 c = 0; c < nargs; c++) {
+      qk_u32 at = (types >> (8 * c)) & 0xffu;
+      qk_u32 v[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) v[j] = rowsbuf[rr[j] * W + 1 + c];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (!pay[j]) continue;
+        qk_u32 g = pay[j] - 1u;
+        qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)v[j] : (qk_u64)v[j];
+        qk_u32 code = qk_code32(v[j], at);
+        if (per_row) {
+          if (flags & 1u) atomicAdd(&g_sum[(qk_u64)c * range + g], add);
+          if (flags & 2u) {
+            atomicMin(&g_min[(qk_u64)c * range + g], code);
+            atomicMax(&g_max[(qk_u64)c * range + g], code);
+          }
+        } else {
+          if (flags & 1u) atomicAdd(&l_sum[c * range + g], add);
+          if (flags & 2u) {
+            atomicMin(&l_min[c * range + g], code);
+            atomicMax(&l_max[c * range + g], code);
           }
         }
       }
@@ -820,6 +1239,11 @@ static const QK_LibraryKernel g_library[] = {
     {"aggregate_dense", (const void*)k_aggregate_dense, QK_WG},
     {"join_build", (const void*)k_join_build, QK_WG},
     {"join_probe", (const void*)k_join_probe, QK_WG},
+    {"join_part_build", (const void*)k_join_part_build, QK_WG},
+    {"join_part_hist", (const void*)k_join_part_hist, QK_WG},
+    {"join_part_offsets", (const void*)k_join_part_offsets, QK_WG},
+    {"join_part_scatter", (const void*)k_join_part_scatter, QK_WG},
+    {"join_part_probe", (const void*)k_join_part_probe, QK_WG},
     {"gen_events", (const void*)k_gen_events, QK_WG},
     {"gen_scale_events", (const void*)k_gen_scale_events, QK_WG},
     {"gen_fact_fk", (const void*)k_gen_fact_fk, QK_WG},

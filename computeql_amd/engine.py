@@ -23,7 +23,8 @@ _NP = {U32: np.uint32, U64: np.uint64, F32: np.float32, F64: np.float64, I32: np
 SCAN_PATHS = {0: "none", 1: "generated_simple", 2: "generated", 3: "simple", 4: "interpreter", 5: "zone_empty",
               6: "constant"}
 AGG_PATHS = {10: "global", 11: "fused_global", 12: "dense", 13: "dense_key_fold", 14: "fused_dense", 20: "join_dense",
-             21: "join_probe_group", 22: "join_probe_row", 30: "hash"}
+             21: "join_probe_group", 22: "join_probe_row", 23: "join_part_group", 24: "join_part_row",
+             30: "hash"}
 
 
 class QueryError(RuntimeError):

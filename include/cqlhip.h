@@ -299,6 +299,8 @@ enum {
   QE_AggPath_JoinDense = 20,     /* join: fact reduced per key value, combined per dim row */
   QE_AggPath_JoinProbeGroup = 21,/* join: hash probe with the dim group code as payload */
   QE_AggPath_JoinProbeRow = 22,  /* join: hash probe, totals per dim row */
+  QE_AggPath_JoinPartGroup = 23, /* join: radix-partitioned, LDS probe, group payload */
+  QE_AggPath_JoinPartRow = 24,   /* join: radix-partitioned, LDS probe, totals per dim row */
   QE_AggPath_Hash = 30,          /* hash GROUP BY, global open addressing */
 };
 

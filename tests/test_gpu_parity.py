@@ -237,19 +237,26 @@ def test_fact_fk_generator(star):
     assert np.array_equal(t.read(4), fk)
 
 
-@pytest.mark.parametrize("mode", ["group_payload", "row_payload"])
+JOIN_MODES = {
+    "join_part_group": {},
+    "join_part_row": {"QE_JOIN_GROUP_PAYLOAD": 0},
+    "join_probe_group": {"QE_JOIN_PARTITION": 0},
+    "join_probe_row": {"QE_JOIN_PARTITION": 0, "QE_JOIN_GROUP_PAYLOAD": 0},
+}
+
+
+@pytest.mark.parametrize("mode", list(JOIN_MODES))
 def test_join_a_group_by_segment(star, orc, mode):
     t, dim, dimb, host, fk, (dk, seg), _ = star
     funcs = [cq.COUNT, cq.SUM, cq.MIN, cq.MAX, cq.AVG]
     args = [None, 2, 2, 2, 2]
-    if mode == "row_payload":
-        cq.set_setting("QE_JOIN_GROUP_PAYLOAD", 0)
+    for k, v in JOIN_MODES[mode].items():
+        cq.set_setting(k, v)
     try:
         res = cq.aggregate_join(t, 4, dim, 0, funcs, args, dim_group=1)
     finally:
         cq.reset_settings()
-    expect_path = "join_probe_group" if mode == "group_payload" else "join_probe_row"
-    assert cq.AGG_PATHS[res.trace["path"]] == expect_path
+    assert cq.AGG_PATHS[res.trace["path"]] == mode
     rk, rv = orc.join_aggregate(fk, [None, host[2], host[2], host[2], host[2]], funcs, dk, dim_group=seg)
     rk, rv = _norm(rk, rv)
     assert np.array_equal(res.keys, rk)
@@ -257,12 +264,14 @@ def test_join_a_group_by_segment(star, orc, mode):
         assert np.array_equal(res.column(e), rv[:, e]), e
 
 
-def test_join_a_filtered_dim(star, orc):
+@pytest.mark.parametrize("limit,path", [(10, "join_probe_group"), (60, "join_part_group")])
+def test_join_a_filtered_dim(star, orc, limit, path):
     t, dim, dimb, host, fk, (dk, seg), _ = star
-    res = cq.aggregate_join(t, 4, dim, 0, [cq.COUNT, cq.SUM], [None, 2], dim_where=col(1) < 10, dim_group=1)
-    rk, rv = orc.join_aggregate(fk, [None, host[2]], [cq.COUNT, cq.SUM], dk, dim_pass=(seg < 10), dim_group=seg)
+    res = cq.aggregate_join(t, 4, dim, 0, [cq.COUNT, cq.SUM], [None, 2], dim_where=col(1) < limit, dim_group=1)
+    assert cq.AGG_PATHS[res.trace["path"]] == path
+    rk, rv = orc.join_aggregate(fk, [None, host[2]], [cq.COUNT, cq.SUM], dk, dim_pass=(seg < limit), dim_group=seg)
     rk, rv = _norm(rk, rv)
-    assert np.array_equal(res.keys, rk) and len(rk) == 10
+    assert np.array_equal(res.keys, rk) and len(rk) == limit
     assert np.array_equal(res.column(0), rv[:, 0]) and np.array_equal(res.column(1), rv[:, 1])
 
 
