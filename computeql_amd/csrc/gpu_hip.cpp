@@ -314,9 +314,17 @@ GPU_Buffer* alloc_impl(U64 size, U32 flags, void* data)
     }
   }
   /* new buffers start zeroed (kernels keep their completion tickets in
-   * scratch buffers and reset them themselves) */
-  if (!data || !size) hipMemset(p, 0, padded);
-  else if (padded > size) hipMemset((char*)p + size, 0, padded - size);
+   * scratch buffers and reset them themselves).  The zeroing runs on the
+   * null stream, which batch streams (hipStreamNonBlocking) are not ordered
+   * against: it must be complete before the buffer is handed out, or a
+   * kernel's writes into the fresh buffer can be overwritten by it. */
+  hipError_t ze = hipSuccess;
+  if (!data || !size) ze = hipMemsetAsync(p, 0, padded, nullptr);
+  else if (padded > size) ze = hipMemsetAsync((char*)p + size, 0, padded - size, nullptr);
+  if (!check(ze, "hipMemset") || !check(hipStreamSynchronize(nullptr), "alloc zero")) {
+    gpu_hip_buffer_release(b);
+    return nullptr;
+  }
   return b;
 }
 
@@ -424,7 +432,10 @@ GPU_Buffer* gpu_hip_buffer_alloc_pooled(String8 name, U64 size, GPU_BufferFlags 
   }
   if (hit) {
     hit->size = size;
-    if (data && size) hipMemcpy(hit->ptr, data, size, hipMemcpyHostToDevice);
+    if (data && size) {
+      hipMemcpy(hit->ptr, data, size, hipMemcpyHostToDevice);
+      hipStreamSynchronize(nullptr);
+    }
     hit->pooled = true;
     hit->pool_name = key;
     return hit;
@@ -483,7 +494,10 @@ void gpu_hip_buffer_write(GPU_Buffer* buffer, void* data, U64 size)
 {
   if (!buffer || !data) return;
   if (size > buffer->alloc_size) size = buffer->alloc_size;
-  check(hipMemcpy(buffer->ptr, data, size, hipMemcpyHostToDevice), "buffer_write");
+  /* complete before returning: batch streams are not ordered against the
+   * null stream this copy runs on */
+  if (check(hipMemcpy(buffer->ptr, data, size, hipMemcpyHostToDevice), "buffer_write"))
+    check(hipStreamSynchronize(nullptr), "buffer_write");
 }
 
 void gpu_hip_buffer_read(GPU_Buffer* buffer, void* data, U64 size)

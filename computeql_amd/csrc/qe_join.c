@@ -123,11 +123,12 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
                                B32 group_mode, U32 gmin, B32 group_i32, GPU_Buffer* list, U64 pass_count, U64 R,
                                GPU_Buffer* acc, U64 acc_bytes, void* host_acc, U32* fl, QE_Trace* trace)
 {
-  const U32 sbits = 12, S = 1u << sbits;
+  /* 8192-slot tables (64 KB of LDS), at most half full */
+  const U32 sbits = 13, S = 1u << sbits;
   U32 pbits = 0;
-  while (((U64)1 << pbits) * (S * 45 / 100) < pass_count && pbits < 12) pbits++;
+  while (((U64)1 << pbits) * (S / 2) < pass_count && pbits < 12) pbits++;
   const U64 P = 1ull << pbits;
-  if (P * (S * 9 / 10) < pass_count) {
+  if (P * (S * 7 / 10) < pass_count) {
     qe_set_error("qe_aggregate_join: %llu dim keys exceed the partitioned tables", (unsigned long long)pass_count);
     return 0;
   }
@@ -203,7 +204,8 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     gpu_kernel_set_arg_buffer(ks, 2, rowsbuf);
     gpu_kernel_set_arg_buffer(ks, 3, off);
     gpu_kernel_set_arg_buffer(ks, 4, totals);
-    gpu_hip_kernel_set_lds_bytes(ks, (U32)(P * 12));
+    /* base u64[P] | cursor, cnt, start u32[P] | part u16[4096] | rows u32[4096 * W] */
+    gpu_hip_kernel_set_lds_bytes(ks, (U32)(P * 20 + 4096 * 2 + 4096 * 4 * (U64)W));
     U32 kflags = (need_sum ? 1u : 0u) | (need_minmax ? 2u : 0u) | (group_mode ? 0u : 4u);
     gpu_kernel_set_arg_buffer(kp, 1, acc);
     gpu_kernel_set_arg_buffer(kp, 2, table);
@@ -364,6 +366,22 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
       ok = pass && qe_scan_filter(dim, dim_where, dim_where_count, 0, pass, dn, &pass_count, 0);
     } else {
       ok = qe_scan_filter_device(dim, dim_where, dim_where_count, &pass_dev, &pass_count);
+      if (ok && pass_dev && qe_settings_u64("QE_JOIN_VERIFY_LIST", 0)) {
+        /* debug: the device selection must be strictly ascending row ids */
+        U64* chk = (U64*)malloc(sizeof(U64) * pass_count);
+        if (chk) {
+          gpu_buffer_read(pass_dev, chk, pass_count * 8);
+          for (U64 i = 0; i < pass_count; i++) {
+            if (chk[i] >= dn || (i && chk[i] <= chk[i - 1])) {
+              qe_set_error("qe_aggregate_join: device selection broken at %llu: %llu after %llu", (unsigned long long)i,
+                           (unsigned long long)chk[i], (unsigned long long)(i ? chk[i - 1] : 0));
+              ok = 0;
+              break;
+            }
+          }
+          free(chk);
+        }
+      }
     }
   } else if (host_dim) {
     pass = (U64*)malloc(sizeof(U64) * (dn ? dn : 1));

@@ -631,6 +631,9 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
           gpu_kernel_set_arg_buffer(k, 3, ids);
           gpu_kernel_set_arg_u64(k, 6, 2); /* QK_MODE_WRITE_IDS */
           b = gpu_batch_begin(0, want * 8);
+          /* QE_SCAN_IDS_SENTINEL: pre-fill the output so an id the write
+           * pass misses shows up as all ones */
+          if (qe_settings_u64("QE_SCAN_IDS_SENTINEL", 0)) gpu_batch_buffer_fill(b, ids, want * 8, 0xffffffffu);
           gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
           if (!device_ids) gpu_batch_buffer_read(b, ids, out_ids, want * 8);
           ok = gpu_batch_end(b);

@@ -928,7 +928,13 @@ __device__ __forceinline__ qk_u64 qk_part_bases(const qk_u32* totals, qk_u32 P, 
 
 /* join_part_scatter: binding 6 fact key, 7.. arg columns (slots in
  * QK_W_ARGS as join_probe), 3 off, 4 totals, 2 out rows (u32, ROWW words
- * per row: key then args); push 0 rows, 5 first row; words PBITS, ROWW */
+ * per row: key then args); push 0 rows, 5 first row; words PBITS, ROWW.
+ * Each step's 4096 rows are first sorted by partition in LDS (counting
+ * sort), then written out in partition runs so consecutive threads store
+ * consecutive rows of one partition.
+ * LDS: base u64[P] | cursor u32[P] | cnt u32[P] | start u32[P] |
+ *      part u16[4096] | buf u32[4096 * ROWW] */
+#define QK_SCATTER_STEP (QK_WG * QK_UNROLL * 4)
 extern "C" __global__ void LB k_join_part_scatter(QK_Args a)
 {
   extern __shared__ qk_u64 lds_u64[];
@@ -941,39 +947,62 @@ extern "C" __global__ void LB k_join_part_scatter(QK_Args a)
   const qk_u32 W = a.words[QK_W_ROWW];
   const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
   const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
-  qk_u64* base = lds_u64;                  /* P u64 */
-  qk_u32* cursor = (qk_u32*)(lds_u64 + P); /* P u32 */
+  qk_u64* base = lds_u64;
+  qk_u32* cursor = (qk_u32*)(base + P);
+  qk_u32* cnt = cursor + P;
+  qk_u32* start = cnt + P;
+  unsigned short* rpart = (unsigned short*)(start + P);
+  qk_u32* buf = (qk_u32*)(rpart + QK_SCATTER_STEP);
   qk_part_bases(qk_ptr<qk_u32>(a, 4), P, base, s_scan);
   const qk_u32* off = qk_ptr<qk_u32>(a, 3) + (qk_u64)blockIdx.x * P;
   for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cursor[i] = off[i];
-  __syncthreads();
   qk_u32* out = qk_wptr<qk_u32>(a, 2);
   const qk_u64 cap = a.bytes[2] / (4ULL * W);
   const qk_u32x4* keys = (const qk_u32x4*)a.ptr[QK_B_COL0];
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
   for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+    for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cnt[i] = 0;
+    __syncthreads();
     qk_u64 q[QK_UNROLL];
     qk_u32 m[QK_UNROLL];
     qk_u32x4 kv[QK_UNROLL];
     qk_step_quads(a, s, b1, r0, r1, q, m);
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
-    qk_u64 pos[QK_UNROLL][4];
+    qk_u32 part[QK_UNROLL][4], rank[QK_UNROLL][4];
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) {
       qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        pos[j][k] = ~0ULL;
-        if (m[j] & (1u << k)) {
-          qk_u32 part = pbits ? (qk_fmix32(w[k]) >> (32 - pbits)) : 0u;
-          qk_u64 p = base[part] + atomicAdd(&cursor[part], 1u);
-          if (p < cap) {
-            pos[j][k] = p;
-            out[p * W] = w[k];
-          }
-        }
+        part[j][k] = pbits ? (qk_fmix32(w[k]) >> (32 - pbits)) : 0u;
+        rank[j][k] = (m[j] & (1u << k)) ? atomicAdd(&cnt[part[j][k]], 1u) : 0u;
+      }
+    }
+    __syncthreads();
+    /* start[p] = exclusive scan of cnt over the partitions */
+    qk_u32 running = 0;
+    for (qk_u32 c = 0; c < P; c += QK_WG) {
+      qk_u32 i = c + threadIdx.x;
+      qk_u32 v = i < P ? cnt[i] : 0u;
+      qk_u32 total = 0;
+      qk_u32 pre = qk_block_excl_scan_u32(v, s_scan, total);
+      if (i < P) start[i] = running + pre;
+      running += total;
+    }
+    const qk_u32 step_rows = running;
+    __syncthreads();
+    /* place keys, then each arg column, into partition order */
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!(m[j] & (1u << k))) continue;
+        qk_u32 idx = start[part[j][k]] + rank[j][k];
+        rpart[idx] = (unsigned short)part[j][k];
+        buf[idx * W] = w[k];
       }
     }
     for (qk_u32 c = 0; c < nargs; c++) {
@@ -986,9 +1015,21 @@ extern "C" __global__ void LB k_join_part_scatter(QK_Args a)
         qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
 #pragma unroll
         for (int k = 0; k < 4; k++)
-          if (pos[j][k] != ~0ULL) out[pos[j][k] * W + 1 + c] = w[k];
+          if (m[j] & (1u << k)) buf[(start[part[j][k]] + rank[j][k]) * W + 1 + c] = w[k];
       }
     }
+    __syncthreads();
+    /* write out: consecutive threads take consecutive rows of the sorted
+     * step, so a partition's run goes out as contiguous stores */
+    for (qk_u32 i = threadIdx.x; i < step_rows * W; i += QK_WG) {
+      qk_u32 r = i / W, wd = i - r * W;
+      qk_u32 p = rpart[r];
+      qk_u64 dst = base[p] + cursor[p] + (r - start[p]);
+      if (dst < cap) out[dst * W + wd] = buf[i];
+    }
+    __syncthreads();
+    for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cursor[i] += cnt[i];
+    __syncthreads();
   }
 }
 
@@ -1046,48 +1087,56 @@ extern "C" __global__ void LB k_join_part_probe(QK_Args a)
   qk_u64* g_sum = g_cnt + range;
   qk_u32* g_min = (qk_u32*)(g_sum + (qk_u64)nargs * range);
   qk_u32* g_max = g_min + (qk_u64)nargs * range;
-  /* 4 rows per thread per step; their keys load together */
-  for (qk_u64 r = rs; r < re; r += (qk_u64)QK_WG * 4) {
-    qk_u64 rr[4];
-    bool ok[4];
-    qk_u32 key[4];
+  /* PR rows per thread per step: their keys load together, then their
+   * home slots in the LDS table, and only keys whose home slot holds
+   * another key walk on */
+#define QK_PROBE_ROWS 8
+  for (qk_u64 r = rs; r < re; r += (qk_u64)QK_WG * QK_PROBE_ROWS) {
+    qk_u64 rr[QK_PROBE_ROWS];
+    bool ok[QK_PROBE_ROWS];
+    qk_u32 key[QK_PROBE_ROWS], slot[QK_PROBE_ROWS];
+    qk_u64 e[QK_PROBE_ROWS];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < QK_PROBE_ROWS; j++) {
       qk_u64 x = r + (qk_u64)j * QK_WG + threadIdx.x;
       ok[j] = x < re;
       rr[j] = ok[j] ? x : rs;
       key[j] = rowsbuf[rr[j] * W];
     }
-    qk_u32 pay[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      qk_u32 h = qk_fmix32(key[j]);
-      qk_u32 slot = (h >> (32 - pbits - sbits)) & smask;
+    for (int j = 0; j < QK_PROBE_ROWS; j++) {
+      slot[j] = (qk_fmix32(key[j]) >> (32 - pbits - sbits)) & smask;
+      e[j] = t[slot[j]];
+    }
+    qk_u32 pay[QK_PROBE_ROWS];
+#pragma unroll
+    for (int j = 0; j < QK_PROBE_ROWS; j++) {
       qk_u32 pj = 0;
-      for (qk_u32 n = 0; n < S; n++) {
-        qk_u64 e = t[slot];
-        if (e == 0ULL) break;
-        if ((qk_u32)e == key[j]) {
-          pj = (qk_u32)(e >> 32);
+      qk_u64 ej = e[j];
+      qk_u32 sl = slot[j];
+      for (qk_u32 n = 0; n < S && ej != 0ULL; n++) {
+        if ((qk_u32)ej == key[j]) {
+          pj = (qk_u32)(ej >> 32);
           break;
         }
-        slot = (slot + 1) & smask;
+        sl = (sl + 1) & smask;
+        ej = t[sl];
       }
       pay[j] = (ok[j] && pj && pj - 1u < range) ? pj : 0u;
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < QK_PROBE_ROWS; j++) {
       if (!pay[j]) continue;
       if (per_row) atomicAdd(&g_cnt[pay[j] - 1u], 1ULL);
       else atomicAdd(&l_cnt[pay[j] - 1u], 1u);
     }
     for (qk_u32 c = 0; c < nargs; c++) {
       qk_u32 at = (types >> (8 * c)) & 0xffu;
-      qk_u32 v[4];
+      qk_u32 v[QK_PROBE_ROWS];
 #pragma unroll
-      for (int j = 0; j < 4; j++) v[j] = rowsbuf[rr[j] * W + 1 + c];
+      for (int j = 0; j < QK_PROBE_ROWS; j++) v[j] = rowsbuf[rr[j] * W + 1 + c];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < QK_PROBE_ROWS; j++) {
         if (!pay[j]) continue;
         qk_u32 g = pay[j] - 1u;
         qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)v[j] : (qk_u64)v[j];
