@@ -123,9 +123,12 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
                                B32 group_mode, U32 gmin, B32 group_i32, GPU_Buffer* list, U64 pass_count, U64 R,
                                GPU_Buffer* acc, U64 acc_bytes, void* host_acc, U32* fl, QE_Trace* trace)
 {
-  /* 8192-slot tables (64 KB of LDS), at most half full */
+  /* 8192-slot tables (32 KB of key words + 32 KB of payloads in LDS), at
+   * most half full; at least two partitions, so a slot's key word (the
+   * hash bits below the partition bits) has its top bit free for the
+   * occupied mark */
   const U32 sbits = 13, S = 1u << sbits;
-  U32 pbits = 0;
+  U32 pbits = 1;
   while (((U64)1 << pbits) * (S / 2) < pass_count && pbits < 12) pbits++;
   const U64 P = 1ull << pbits;
   if (P * (S * 7 / 10) < pass_count) {
@@ -133,13 +136,34 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     return 0;
   }
   const U64 rows = fact->row_count;
-  const U32 B = qe_stream_blocks((rows + 3) / 4);
+  /* hist and scatter blocks: each resident scatter block keeps up to P
+   * partially written output lines open in L2, so their number is bounded
+   * separately from the scan kernels' */
+  U32 B = qe_stream_blocks((rows + 3) / 4);
+  {
+    U64 cap = gpu_hip_compute_units() * qe_settings_u64("QE_JOIN_SCATTER_BLOCKS_PER_CU", 4);
+    if (cap >= 1 && B > cap) B = (U32)cap;
+  }
   const U32 W = 1 + arg_count;
   GPU_Kernel* kb = qe_kernel("join_part_build");
   GPU_Kernel* kh = qe_kernel("join_part_hist");
   GPU_Kernel* ko = qe_kernel("join_part_offsets");
-  GPU_Kernel* ks = qe_kernel("join_part_scatter");
-  GPU_Kernel* kp = qe_kernel("join_part_probe");
+  char sname[40] = "join_part_scatter";
+  if (arg_count <= 2 && qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1))
+    snprintf(sname, sizeof(sname), "join_part_scatter_%u", arg_count);
+  GPU_Kernel* ks = qe_kernel(sname);
+  /* a probe instance specialised for this shape when one exists */
+  char pname[48] = "join_part_probe";
+  if (group_mode) {
+    if (arg_count == 0) snprintf(pname, sizeof(pname), "join_part_probe_0");
+    else if (arg_count <= 2 && need_sum) snprintf(pname, sizeof(pname), "join_part_probe_%u%s", arg_count, need_minmax ? "sm" : "s");
+    else if (arg_count == 1 && need_minmax) snprintf(pname, sizeof(pname), "join_part_probe_1m");
+  } else {
+    if (arg_count == 0) snprintf(pname, sizeof(pname), "join_part_probe_0_row");
+    else if (arg_count == 1 && need_sum) snprintf(pname, sizeof(pname), "join_part_probe_1%s_row", need_minmax ? "sm" : "s");
+  }
+  if (!qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1)) snprintf(pname, sizeof(pname), "join_part_probe");
+  GPU_Kernel* kp = qe_kernel(pname);
   GPU_Buffer* table = qe_scratch("jp_table", P * S * 8);
   GPU_Buffer* flags = qe_scratch("join_flags", 64);
   GPU_Buffer* hist = qe_scratch("jp_hist", (U64)B * P * 4);
@@ -148,10 +172,11 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   GPU_Buffer* rowsbuf = qe_scratch("jp_rows", rows * W * 4);
   B32 ok = kb && kh && ko && ks && kp && table && flags && hist && off && totals && rowsbuf;
   U64 cus = gpu_hip_compute_units();
-  U32 split = (U32)((cus * 4 + P - 1) / P);
+  U32 split = (U32)((cus * qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU", 6) + P - 1) / P);
   if (split < 1) split = 1;
   U64 acc_lds = group_mode ? R * 4 + (need_sum ? R * 8 * arg_count : 0) + (need_minmax ? R * 8 * arg_count : 0) : 0;
-  U64 probe_lds = P * 8 + (U64)S * 8 + acc_lds;
+  /* key words u32[S], then payloads: u16 group codes or u32 dim rows */
+  U64 probe_lds = (U64)S * 4 + (U64)S * (group_mode ? 2 : 4) + acc_lds;
   if (ok && probe_lds > gpu_hip_max_lds_bytes()) {
     qe_set_error("qe_aggregate_join: partition probe needs %llu bytes of LDS", (unsigned long long)probe_lds);
     ok = 0;
@@ -247,7 +272,7 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
       trace->kernel_count = 5;
       trace->rows_scanned = rows;
       trace->bytes_read = rows * 4 * (U64)slots.count;
-      snprintf(trace->kernel, sizeof(trace->kernel), "join_part_probe");
+      snprintf(trace->kernel, sizeof(trace->kernel), "%s", pname);
     }
   }
   qe_scratch_release(table);

@@ -79,16 +79,30 @@ __device__ __forceinline__ qk_u64 qk_push(const QK_Args& a, int slot)
   return (qk_u64)a.words[2 * slot] | ((qk_u64)a.words[2 * slot + 1] << 32);
 }
 
+/* Buffer addresses arrive as integers inside the by-value argument struct,
+ * so on their own the compiler cannot tell that they point to global memory
+ * and emits flat accesses, which also count against the LDS wait counter:
+ * every wait for an LDS result would then wait for all loads in flight.
+ * Making the pointer in address space 1 and casting it to a generic pointer
+ * lets address-space inference turn the accesses through it into global_*
+ * instructions, as it does for pointer kernel arguments. */
+#define QK_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T* qk_gptr(qk_u64 addr)
+{
+  return (T*)(QK_GLOBAL T*)addr;
+}
+
 template <class T>
 __device__ __forceinline__ const T* qk_ptr(const QK_Args& a, int binding)
 {
-  return (const T*)a.ptr[binding];
+  return qk_gptr<const T>(a.ptr[binding]);
 }
 
 template <class T>
 __device__ __forceinline__ T* qk_wptr(const QK_Args& a, int binding)
 {
-  return (T*)a.ptr[binding];
+  return qk_gptr<T>(a.ptr[binding]);
 }
 
 /* the contiguous quad range [b0,b1) of this block over [q0,q1), a multiple
@@ -129,7 +143,7 @@ __device__ __forceinline__ void qk_load4_f64(const QK_Args& a, qk_u32 slot, qk_u
 {
   qk_u64 base = a.ptr[QK_B_COL0 + slot];
   if (type == QK_T_U32 || type == QK_T_I32 || type == QK_T_F32) {
-    qk_u32x4 x = ((const qk_u32x4*)base)[q];
+    qk_u32x4 x = qk_gptr<const qk_u32x4>(base)[q];
     qk_u32 w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -138,7 +152,7 @@ __device__ __forceinline__ void qk_load4_f64(const QK_Args& a, qk_u32 slot, qk_u
       else v[k] = (double)__uint_as_float(w[k]);
     }
   } else {
-    const qk_u64x2* p = (const qk_u64x2*)base + 2 * q;
+    const qk_u64x2* p = qk_gptr<const qk_u64x2>(base) + 2 * q;
     qk_u64x2 x0 = p[0], x1 = p[1];
     qk_u64 w[4] = {x0.x, x0.y, x1.x, x1.y};
 #pragma unroll
@@ -161,7 +175,7 @@ __device__ __forceinline__ qk_u32 qk_leaf_quad(const QK_Args& a, const qk_u32* w
   if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
     qk_u32 lo = w[3], span = w[4];
     qk_u32 bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
-    qk_u32x4 x = ((const qk_u32x4*)a.ptr[QK_B_COL0 + slot])[q];
+    qk_u32x4 x = (qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot))[q];
     m |= (((x.x ^ bias) - lo) <= span) ? 1u : 0u;
     m |= (((x.y ^ bias) - lo) <= span) ? 2u : 0u;
     m |= (((x.z ^ bias) - lo) <= span) ? 4u : 0u;
@@ -231,7 +245,7 @@ __device__ __forceinline__ void qk_leaves_step(const QK_Args& a, const QkLeafSet
     if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
       qk_u32 lo = w[3], span = w[4];
       qk_u32 bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
-      const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + slot];
+      const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot);
       qk_u32x4 x[QK_UNROLL];
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
@@ -269,6 +283,17 @@ __device__ __forceinline__ void qk_step_quads(const QK_Args& a, qk_u64 s, qk_u64
   }
 }
 
+/* Block barrier that orders LDS accesses only.  __syncthreads() is also a
+ * workgroup release fence, so it first waits for every global store of the
+ * wave to complete; in a loop that stores results and then rebuilds LDS
+ * state, that wait puts the HBM write latency on every iteration.  This
+ * barrier waits for the wave's LDS operations (lgkmcnt) and nothing else;
+ * the memory clobber keeps the compiler from moving accesses across it. */
+__device__ __forceinline__ void qk_lds_barrier()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 /* ---- wave64 / block reductions ---- */
 __device__ __forceinline__ qk_u64 qk_wave_sum_u64(qk_u64 v)
 {
@@ -304,13 +329,18 @@ __device__ __forceinline__ qk_u64 qk_block_sum_u64(qk_u64 v, qk_u64* lds4)
 
 /* block exclusive scan of a u32 per thread; returns the exclusive prefix and
  * the block total */
+template <bool LDS_ONLY = false>
 __device__ __forceinline__ qk_u32 qk_block_excl_scan_u32(qk_u32 v, qk_u32* lds4, qk_u32& total)
 {
   qk_u32 incl = qk_wave_incl_scan_u32(v);
   qk_u32 wave = threadIdx.x >> 6;
-  __syncthreads();
+  /* LDS_ONLY: the barriers order LDS only (qk_lds_barrier), for loops
+   * whose global stores need no ordering against the scan */
+  if (LDS_ONLY) qk_lds_barrier();
+  else __syncthreads();
   if (__lane_id() == 63) lds4[wave] = incl;
-  __syncthreads();
+  if (LDS_ONLY) qk_lds_barrier();
+  else __syncthreads();
   qk_u32 before = 0, all = 0;
 #pragma unroll
   for (int w = 0; w < QK_WG / 64; w++) {

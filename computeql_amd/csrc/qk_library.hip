@@ -110,7 +110,7 @@ __device__ __forceinline__ void agg_fold_quad(const QK_Args& a, qk_u32 slot, qk_
 {
   qk_u64 base = a.ptr[QK_B_COL0 + slot];
   if (type == QK_T_U32) {
-    qk_u32x4 x = ((const qk_u32x4*)base)[q];
+    qk_u32x4 x = qk_gptr<const qk_u32x4>(base)[q];
     if (m == 0xfu) {
       acc.sum += (qk_u64)x.x + (qk_u64)x.y + (qk_u64)x.z + (qk_u64)x.w;
       qk_u32 lo = min(min(x.x, x.y), min(x.z, x.w));
@@ -129,7 +129,7 @@ __device__ __forceinline__ void agg_fold_quad(const QK_Args& a, qk_u32 slot, qk_
       }
     }
   } else if (type == QK_T_I32) {
-    qk_u32x4 x = ((const qk_u32x4*)base)[q];
+    qk_u32x4 x = qk_gptr<const qk_u32x4>(base)[q];
     qk_u32 w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -150,10 +150,10 @@ __device__ __forceinline__ void agg_fold_quad(const QK_Args& a, qk_u32 slot, qk_
         s += v[k];
         qk_u64 key;
         if (type == QK_T_U64) {
-          qk_u64x2 x = ((const qk_u64x2*)base)[2 * q + (k >> 1)];
+          qk_u64x2 x = qk_gptr<const qk_u64x2>(base)[2 * q + (k >> 1)];
           key = (k & 1) ? x.y : x.x;
         } else if (type == QK_T_I64) {
-          qk_u64x2 x = ((const qk_u64x2*)base)[2 * q + (k >> 1)];
+          qk_u64x2 x = qk_gptr<const qk_u64x2>(base)[2 * q + (k >> 1)];
           key = ((k & 1) ? x.y : x.x) ^ 0x8000000000000000ULL;
         } else {
           key = qk_f64_key(v[k]);
@@ -208,7 +208,7 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
       qk_u32 slot = (slots >> (4 * c)) & 0xfu, type = (types >> (8 * c)) & 0xffu;
       if (type == QK_T_U32) {
         /* the hot case: the U loads of the step in flight together */
-        const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + slot];
+        const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot);
         qk_u32x4 x[QK_UNROLL];
 #pragma unroll
         for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
@@ -383,12 +383,12 @@ __device__ __forceinline__ double qk_load1_f64(const QK_Args& a, qk_u32 slot, qk
 {
   qk_u64 base = a.ptr[QK_B_COL0 + slot];
   switch (type) {
-    case QK_T_U32: return (double)((const qk_u32*)base)[row];
-    case QK_T_I32: return (double)((const qk_s32*)base)[row];
-    case QK_T_F32: return (double)((const float*)base)[row];
-    case QK_T_F64: return ((const double*)base)[row];
-    case QK_T_I64: return (double)((const qk_s64*)base)[row];
-    case QK_T_U64: return (double)((const qk_u64*)base)[row];
+    case QK_T_U32: return (double)qk_gptr<const qk_u32>(base)[row];
+    case QK_T_I32: return (double)qk_gptr<const qk_s32>(base)[row];
+    case QK_T_F32: return (double)qk_gptr<const float>(base)[row];
+    case QK_T_F64: return qk_gptr<const double>(base)[row];
+    case QK_T_I64: return (double)qk_gptr<const qk_s64>(base)[row];
+    case QK_T_U64: return (double)qk_gptr<const qk_u64>(base)[row];
     default: return 0.0;
   }
 }
@@ -511,7 +511,7 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
   }
   __syncthreads();
 
-  const qk_u32x4* keys = (const qk_u32x4*)a.ptr[QK_B_COL0 + key_slot];
+  const qk_u32x4* keys = qk_ptr<qk_u32x4>(a, QK_B_COL0 + key_slot);
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
   for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
@@ -540,7 +540,7 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
     }
     for (qk_u32 c = 0; c < nargs; c++) {
       qk_u32 at = (types >> (8 * c)) & 0xffu;
-      const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)];
+      const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + ((slots >> (4 * c)) & 0xfu));
       qk_u32x4 x[QK_UNROLL];
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
@@ -686,7 +686,7 @@ extern "C" __global__ void LB k_join_probe(QK_Args a)
     }
     __syncthreads();
   }
-  const qk_u32x4* keys = (const qk_u32x4*)a.ptr[QK_B_COL0];
+  const qk_u32x4* keys = qk_ptr<qk_u32x4>(a, QK_B_COL0);
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
   for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
@@ -742,7 +742,7 @@ extern "C" __global__ void LB k_join_probe(QK_Args a)
     }
     for (qk_u32 c = 0; c < nargs; c++) {
       qk_u32 at = (types >> (8 * c)) & 0xffu;
-      const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)];
+      const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + ((slots >> (4 * c)) & 0xfu));
       qk_u32x4 x[QK_UNROLL];
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) x[j] = col[qs[j]];
@@ -808,34 +808,44 @@ __device__ __forceinline__ qk_u32 qk_fmix32(qk_u32 h)
 }
 
 /* join_part_build: binding 0 dim keys, 1 dim group column, 2 partition
- * tables (P x S u64, entry (payload+1) << 32 | key, 0 empty), 3 flags
- * (u32[2]: duplicate, full), 4 row list (u64 ids, optional);
- * push 0 rows; words PMODE/GMIN/GTYPE as join_build, PBITS, SBITS */
+ * tables, 3 flags (u32[2]: duplicate, full), 4 row list (u64 ids,
+ * optional); push 0 rows; words PMODE/GMIN/GTYPE as join_build, PBITS
+ * (>= 1), SBITS.
+ * With h = fmix32(key) (a bijection of u32), the partition is the top PBITS
+ * bits of h and the home slot the next SBITS bits.  Inside partition p the
+ * key is fully given by the low 32-PBITS bits of h, so a slot stores the key
+ * word kw = (h & low) | 0x80000000 (never 0: 0 marks an empty slot) in
+ * keys u32[P][S], and the payload (dim row or group code) in pays u32[P][S]
+ * right after.  Collisions walk on by linear probing over slots. */
 extern "C" __global__ void LB k_join_part_build(QK_Args a)
 {
   const qk_u64 n = qk_push(a, 0);
   const qk_u32 pmode = a.words[QK_W_PMODE], gmin = a.words[QK_W_GMIN], gtype = a.words[QK_W_GTYPE];
   const qk_u32 pbits = a.words[QK_W_PBITS], sbits = a.words[QK_W_SBITS];
   const qk_u32 smask = (1u << sbits) - 1u;
+  const qk_u32 low = 0xffffffffu >> pbits;
   const qk_u32* keys = qk_ptr<qk_u32>(a, 0);
   const qk_u32* groups = qk_ptr<qk_u32>(a, 1);
   const qk_u64* list = a.ptr[4] ? qk_ptr<qk_u64>(a, 4) : nullptr;
-  qk_u64* table = qk_wptr<qk_u64>(a, 2);
+  qk_u32* tkeys = qk_wptr<qk_u32>(a, 2);
+  qk_u32* tpays = tkeys + ((qk_u64)1 << (pbits + sbits));
   qk_u32* flags = qk_wptr<qk_u32>(a, 3);
   for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
     qk_u32 row = list ? (qk_u32)list[i] : (qk_u32)i;
     qk_u32 key = keys[row];
     qk_u32 pay = pmode == 0 ? row : (pmode == 1 ? qk_code32(groups[row], gtype) - gmin : 0u);
-    qk_u64 entry = ((qk_u64)(pay + 1u) << 32) | key;
     qk_u32 h = qk_fmix32(key);
-    qk_u32 part = pbits ? (h >> (32 - pbits)) : 0u;
+    qk_u32 kw = (h & low) | 0x80000000u;
+    qk_u64 tbase = (qk_u64)(h >> (32 - pbits)) << sbits;
     qk_u32 slot = (h >> (32 - pbits - sbits)) & smask;
-    qk_u64* t = table + ((qk_u64)part << sbits);
     qk_u32 probes = 0;
     for (;;) {
-      qk_u64 prev = atomicCAS(&t[slot], 0ULL, entry);
-      if (prev == 0ULL) break;
-      if ((qk_u32)prev == key) {
+      qk_u32 prev = atomicCAS(&tkeys[tbase + slot], 0u, kw);
+      if (prev == 0u) {
+        tpays[tbase + slot] = pay;
+        break;
+      }
+      if (prev == kw) {
         atomicOr(&flags[0], 1u);
         break;
       }
@@ -862,7 +872,7 @@ extern "C" __global__ void LB k_join_part_hist(QK_Args a)
   const qk_u32 P = 1u << pbits;
   for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) hist[i] = 0;
   __syncthreads();
-  const qk_u32x4* keys = (const qk_u32x4*)a.ptr[QK_B_COL0];
+  const qk_u32x4* keys = qk_ptr<qk_u32x4>(a, QK_B_COL0);
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
   for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
@@ -928,14 +938,15 @@ __device__ __forceinline__ qk_u64 qk_part_bases(const qk_u32* totals, qk_u32 P, 
 
 /* join_part_scatter: binding 6 fact key, 7.. arg columns (slots in
  * QK_W_ARGS as join_probe), 3 off, 4 totals, 2 out rows (u32, ROWW words
- * per row: key then args); push 0 rows, 5 first row; words PBITS, ROWW.
+ * per row: fmix32(key) then args); push 0 rows, 5 first row; words PBITS, ROWW.
  * Each step's 4096 rows are first sorted by partition in LDS (counting
  * sort), then written out in partition runs so consecutive threads store
  * consecutive rows of one partition.
  * LDS: base u64[P] | cursor u32[P] | cnt u32[P] | start u32[P] |
  *      part u16[4096] | buf u32[4096 * ROWW] */
 #define QK_SCATTER_STEP (QK_WG * QK_UNROLL * 4)
-extern "C" __global__ void LB k_join_part_scatter(QK_Args a)
+template <int NA>
+__device__ __forceinline__ void qk_part_scatter(const QK_Args& a)
 {
   extern __shared__ qk_u64 lds_u64[];
   __shared__ qk_u32 s_scan[QK_WG / 64];
@@ -944,8 +955,8 @@ extern "C" __global__ void LB k_join_part_scatter(QK_Args a)
   const qk_u64 r1 = r0 + rows;
   const qk_u32 pbits = a.words[QK_W_PBITS];
   const qk_u32 P = 1u << pbits;
-  const qk_u32 W = a.words[QK_W_ROWW];
-  const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 nargs = NA >= 0 ? (qk_u32)NA : (a.words[QK_W_ARGS] & 0xffu);
+  const qk_u32 W = 1u + nargs;
   const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
   qk_u64* base = lds_u64;
   qk_u32* cursor = (qk_u32*)(base + P);
@@ -958,55 +969,82 @@ extern "C" __global__ void LB k_join_part_scatter(QK_Args a)
   for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cursor[i] = off[i];
   qk_u32* out = qk_wptr<qk_u32>(a, 2);
   const qk_u64 cap = a.bytes[2] / (4ULL * W);
-  const qk_u32x4* keys = (const qk_u32x4*)a.ptr[QK_B_COL0];
+  const qk_u32x4* keys = qk_ptr<qk_u32x4>(a, QK_B_COL0);
+  const qk_u32x4* col0 = nargs ? qk_ptr<qk_u32x4>(a, QK_B_COL0 + (slots & 0xfu)) : keys;
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
-  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+  const qk_u64 qstep = (qk_u64)QK_WG * QK_UNROLL;
+  /* the quads of step s+1 (keys and the first arg column) load while step
+   * s sorts and writes; the loop's barriers order LDS only, so the stores
+   * of one step stay in flight through the next */
+  qk_u64 nq[QK_UNROLL];
+  qk_u32 nm[QK_UNROLL];
+  qk_u32x4 nkv[QK_UNROLL], nx0[QK_UNROLL];
+  auto load_step = [&](qk_u64 s) {
+    qk_step_quads(a, s, b1, r0, r1, nq, nm);
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) {
+      nkv[j] = keys[nq[j]];
+      if (nargs) nx0[j] = col0[nq[j]];
+    }
+  };
+  if (b0 < b1) load_step(b0);
+  for (qk_u64 s = b0; s < b1; s += qstep) {
     for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cnt[i] = 0;
-    __syncthreads();
     qk_u64 q[QK_UNROLL];
     qk_u32 m[QK_UNROLL];
-    qk_u32x4 kv[QK_UNROLL];
-    qk_step_quads(a, s, b1, r0, r1, q, m);
+    qk_u32x4 kv[QK_UNROLL], x0[QK_UNROLL];
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
-    qk_u32 part[QK_UNROLL][4], rank[QK_UNROLL][4];
+    for (int j = 0; j < QK_UNROLL; j++) {
+      q[j] = nq[j];
+      m[j] = nm[j];
+      kv[j] = nkv[j];
+      x0[j] = nx0[j];
+    }
+    if (s + qstep < b1) load_step(s + qstep);
+    qk_lds_barrier();
+    /* a row is stored with its key's hash h = fmix32(key) in place of the
+     * key (a bijection, so nothing is lost): the probe reads the partition,
+     * slot and key word straight from it */
+    qk_u32 hv[QK_UNROLL][4], rank[QK_UNROLL][4];
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) {
       qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        part[j][k] = pbits ? (qk_fmix32(w[k]) >> (32 - pbits)) : 0u;
-        rank[j][k] = (m[j] & (1u << k)) ? atomicAdd(&cnt[part[j][k]], 1u) : 0u;
+        hv[j][k] = qk_fmix32(w[k]);
+        rank[j][k] = (m[j] & (1u << k)) ? atomicAdd(&cnt[pbits ? hv[j][k] >> (32 - pbits) : 0u], 1u) : 0u;
       }
     }
-    __syncthreads();
+    qk_lds_barrier();
     /* start[p] = exclusive scan of cnt over the partitions */
     qk_u32 running = 0;
     for (qk_u32 c = 0; c < P; c += QK_WG) {
       qk_u32 i = c + threadIdx.x;
       qk_u32 v = i < P ? cnt[i] : 0u;
       qk_u32 total = 0;
-      qk_u32 pre = qk_block_excl_scan_u32(v, s_scan, total);
+      qk_u32 pre = qk_block_excl_scan_u32<true>(v, s_scan, total);
       if (i < P) start[i] = running + pre;
       running += total;
     }
     const qk_u32 step_rows = running;
-    __syncthreads();
+    qk_lds_barrier();
     /* place keys, then each arg column, into partition order */
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) {
-      qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
+      qk_u32 x[4] = {x0[j].x, x0[j].y, x0[j].z, x0[j].w};
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         if (!(m[j] & (1u << k))) continue;
-        qk_u32 idx = start[part[j][k]] + rank[j][k];
-        rpart[idx] = (unsigned short)part[j][k];
-        buf[idx * W] = w[k];
+        qk_u32 p = pbits ? hv[j][k] >> (32 - pbits) : 0u;
+        qk_u32 idx = start[p] + rank[j][k];
+        rpart[idx] = (unsigned short)p;
+        buf[idx * W] = hv[j][k];
+        if (nargs) buf[idx * W + 1] = x[k];
       }
     }
-    for (qk_u32 c = 0; c < nargs; c++) {
-      const qk_u32x4* col = (const qk_u32x4*)a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)];
+    for (qk_u32 c = 1; c < nargs; c++) {
+      const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + ((slots >> (4 * c)) & 0xfu));
       qk_u32x4 x[QK_UNROLL];
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
@@ -1015,59 +1053,123 @@ extern "C" __global__ void LB k_join_part_scatter(QK_Args a)
         qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
 #pragma unroll
         for (int k = 0; k < 4; k++)
-          if (m[j] & (1u << k)) buf[(start[part[j][k]] + rank[j][k]) * W + 1 + c] = w[k];
+          if (m[j] & (1u << k)) buf[(start[pbits ? hv[j][k] >> (32 - pbits) : 0u] + rank[j][k]) * W + 1 + c] = w[k];
       }
     }
-    __syncthreads();
+    qk_lds_barrier();
     /* write out: consecutive threads take consecutive rows of the sorted
      * step, so a partition's run goes out as contiguous stores */
-    for (qk_u32 i = threadIdx.x; i < step_rows * W; i += QK_WG) {
-      qk_u32 r = i / W, wd = i - r * W;
-      qk_u32 p = rpart[r];
-      qk_u64 dst = base[p] + cursor[p] + (r - start[p]);
-      if (dst < cap) out[dst * W + wd] = buf[i];
+    if (W == 2) {
+      for (qk_u32 r = threadIdx.x; r < step_rows; r += QK_WG) {
+        qk_u32 p = rpart[r];
+        qk_u64 dst = base[p] + cursor[p] + (r - start[p]);
+        if (dst < cap) ((uint2*)out)[dst] = make_uint2(buf[2 * r], buf[2 * r + 1]);
+      }
+    } else {
+      for (qk_u32 i = threadIdx.x; i < step_rows * W; i += QK_WG) {
+        qk_u32 r = i / W, wd = i - r * W;
+        qk_u32 p = rpart[r];
+        qk_u64 dst = base[p] + cursor[p] + (r - start[p]);
+        if (dst < cap) out[dst * W + wd] = buf[i];
+      }
     }
-    __syncthreads();
+    qk_lds_barrier();
     for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cursor[i] += cnt[i];
-    __syncthreads();
   }
 }
 
+/* instances over the arg count (rows of 1 + NA words); NA < 0 reads it */
+extern "C" __global__ void LB k_join_part_scatter(QK_Args a) { qk_part_scatter<-1>(a); }
+extern "C" __global__ void LB k_join_part_scatter_0(QK_Args a) { qk_part_scatter<0>(a); }
+extern "C" __global__ void LB k_join_part_scatter_1(QK_Args a) { qk_part_scatter<1>(a); }
+extern "C" __global__ void LB k_join_part_scatter_2(QK_Args a) { qk_part_scatter<2>(a); }
+
 /* join_part_probe: blocks (partition p, split i) for i < SPLIT; loads
- * table p into LDS and probes its share of the partitioned rows;
- * aggregates like join_probe (flags 1 sums, 2 min/max; payload = group
- * code -> LDS group tables; flag 4 payload = dim row -> global atomics).
- * binding 1 accumulators, 2 partition tables, 3 rows, 4 totals; words
- * PBITS, SBITS, ROWW, SPLIT, ARGS (arg count), TYPES, RANGE, KEY (flags
- * << 16) */
-extern "C" __global__ void LB k_join_part_probe(QK_Args a)
+ * table p (join_part_build's layout) into LDS and probes its share of the
+ * partitioned rows; aggregates like join_probe (flags 1 sums, 2 min/max;
+ * payload = group code -> LDS group tables; flag 4 payload = dim row ->
+ * global atomics).  binding 1 accumulators, 2 partition tables, 3 rows,
+ * 4 totals; words PBITS (>= 1), SBITS (>= 3), ROWW, SPLIT, ARGS (arg
+ * count), TYPES, RANGE, KEY (flags << 16).
+ * A lookup reads the aligned 4-slot bucket holding the home slot (one
+ * 16-byte LDS read) and compares its key words: a key sits between
+ * its home slot and the first empty slot after it, so the bucket settles
+ * the lookup unless every slot from the home slot to the bucket's end is
+ * taken, and only then does the next bucket follow.
+ * The body is a template over the arg count and the aggregate flags so the
+ * common shapes run without per-row branches on them; NA < 0 is the
+ * generic form that reads them from the push words.  Group codes
+ * (< 65536) sit in LDS as u16, dim rows as u32. */
+#define QK_PROBE_ROWS 16
+/* QK_BUCKET slots (QK_BUCKET * 4 bytes, one 16-byte LDS read) */
+#define QK_BUCKET 4
+struct QkBucket {
+  qk_u32 w[QK_BUCKET];
+};
+
+__device__ __forceinline__ QkBucket qk_bucket_read(const qk_u32* tk, qk_u32 b)
 {
-  extern __shared__ qk_u64 lds_u64[];
-  __shared__ qk_u32 s_scan[QK_WG / 64];
+  const qk_u32x4 k0 = *(const qk_u32x4*)(tk + b);
+  return QkBucket{{k0.x, k0.y, k0.z, k0.w}};
+}
+
+__device__ __forceinline__ qk_u32 qk_bucket_match(const QkBucket& k, qk_u32 v)
+{
+  qk_u32 m = 0u;
+#pragma unroll
+  for (int i = 0; i < QK_BUCKET; i++) m |= (k.w[i] == v) ? (1u << i) : 0u;
+  return m;
+}
+
+template <int NA, bool SUMS, bool MM, bool ROW>
+__device__ __forceinline__ void qk_part_probe(const QK_Args& a)
+{
+  /* 16-byte aligned: the bucket reads are 16-byte LDS loads */
+  extern __shared__ __attribute__((aligned(16))) qk_u64 lds_u64[];
+  __shared__ qk_u64 s_red[QK_WG / 64];
+  constexpr bool FIXED = NA >= 0;
   const qk_u32 pbits = a.words[QK_W_PBITS], sbits = a.words[QK_W_SBITS];
   const qk_u32 P = 1u << pbits, S = 1u << sbits, smask = S - 1u;
-  const qk_u32 W = a.words[QK_W_ROWW];
+  const qk_u32 low = 0xffffffffu >> pbits;
   const qk_u32 split = a.words[QK_W_SPLIT];
-  const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 nargs = FIXED ? (qk_u32)NA : (a.words[QK_W_ARGS] & 0xffu);
+  const qk_u32 W = 1u + nargs;
   const qk_u32 types = a.words[QK_W_TYPES];
   const qk_u32 range = a.words[QK_W_RANGE];
   const qk_u32 flags = a.words[QK_W_KEY] >> 16;
-  const bool per_row = (flags & 4u) != 0;
+  const bool do_sum = FIXED ? SUMS : (flags & 1u) != 0;
+  const bool do_mm = FIXED ? MM : (flags & 2u) != 0;
+  const bool per_row = FIXED ? ROW : (flags & 4u) != 0;
   const qk_u32 part = blockIdx.x / split, piece = blockIdx.x % split;
   if (part >= P) return;
-  /* LDS: bases u64[P] | table u64[S] | sums u64[E][R] | cnt u32[R] | min/max u32[E][R] */
-  qk_u64* base = lds_u64;
-  qk_u64* t = base + P;
-  qk_u64* l_sum = t + S;
-  qk_u32* l_cnt = (qk_u32*)(l_sum + ((!per_row && (flags & 1u)) ? (qk_u64)nargs * range : 0));
+  /* LDS: keys u32[S] | pays (u16 group codes or u32 dim rows)[S] |
+   *      sums u64[E][R] | cnt u32[R] | min/max u32[E][R] */
+  qk_u32* tk = (qk_u32*)lds_u64;
+  unsigned short* tp16 = (unsigned short*)(tk + S);
+  qk_u32* tp32 = tk + S;
+  qk_u64* l_sum = (qk_u64*)(per_row ? (void*)(tp32 + S) : (void*)(tp16 + S));
+  qk_u32* l_cnt = (qk_u32*)(l_sum + ((!per_row && do_sum) ? (qk_u64)nargs * range : 0));
   qk_u32* l_min = l_cnt + (per_row ? 0u : range);
-  qk_u32* l_max = l_min + ((!per_row && (flags & 2u)) ? (qk_u64)nargs * range : 0);
-  qk_part_bases(qk_ptr<qk_u32>(a, 4), P, base, s_scan);
-  const qk_u64* gt = qk_ptr<qk_u64>(a, 2) + ((qk_u64)part << sbits);
-  for (qk_u32 i = threadIdx.x; i < S; i += QK_WG) t[i] = gt[i];
+  qk_u32* l_max = l_min + ((!per_row && do_mm) ? (qk_u64)nargs * range : 0);
+  {
+    const qk_u32x4* gk = qk_ptr<qk_u32x4>(a, 2) + (((qk_u64)part << sbits) >> 2);
+    const qk_u32x4* gp = qk_ptr<qk_u32x4>(a, 2) + ((((qk_u64)P + part) << sbits) >> 2);
+    for (qk_u32 i = threadIdx.x; i < S / 4; i += QK_WG) {
+      ((qk_u32x4*)tk)[i] = gk[i];
+      qk_u32x4 v = gp[i];
+      if (per_row) {
+        ((qk_u32x4*)tp32)[i] = v;
+      } else {
+        tp16[4 * i] = (unsigned short)v.x;
+        tp16[4 * i + 1] = (unsigned short)v.y;
+        tp16[4 * i + 2] = (unsigned short)v.z;
+        tp16[4 * i + 3] = (unsigned short)v.w;
+      }
+    }
+  }
   if (!per_row) {
-    const qk_u32 sum_words = (flags & 1u) ? nargs * range : 0u;
-    const qk_u32 mm_words = (flags & 2u) ? nargs * range : 0u;
+    const qk_u32 sum_words = do_sum ? nargs * range : 0u;
+    const qk_u32 mm_words = do_mm ? nargs * range : 0u;
     for (qk_u32 i = threadIdx.x; i < sum_words; i += QK_WG) l_sum[i] = 0;
     for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) l_cnt[i] = 0;
     for (qk_u32 i = threadIdx.x; i < mm_words; i += QK_WG) {
@@ -1075,83 +1177,123 @@ extern "C" __global__ void LB k_join_part_probe(QK_Args a)
       l_max[i] = 0u;
     }
   }
-  __syncthreads();
-  const qk_u64 pstart = base[part];
-  const qk_u64 pend = pstart + qk_ptr<qk_u32>(a, 4)[part];
+  /* this partition's first row: the rows of partitions 0..part-1 precede
+   * it (the reduction's barriers also publish the LDS fills above) */
+  const qk_u32* totals = qk_ptr<qk_u32>(a, 4);
+  qk_u64 before = 0;
+  for (qk_u32 i = threadIdx.x; i < part; i += QK_WG) before += totals[i];
+  const qk_u64 pstart = qk_block_sum_u64(before, s_red);
+  const qk_u64 pend = pstart + totals[part];
   const qk_u64 plen = pend - pstart;
   const qk_u64 chunk = (plen + split - 1) / split;
-  const qk_u64 rs = pstart + chunk * piece;
+  const qk_u64 rs = pstart + chunk * piece < pend ? pstart + chunk * piece : pend;
   const qk_u64 re = rs + chunk < pend ? rs + chunk : pend;
   const qk_u32* rowsbuf = qk_ptr<qk_u32>(a, 3);
   qk_u64* g_cnt = qk_wptr<qk_u64>(a, 1);
   qk_u64* g_sum = g_cnt + range;
   qk_u32* g_min = (qk_u32*)(g_sum + (qk_u64)nargs * range);
   qk_u32* g_max = g_min + (qk_u64)nargs * range;
-  /* PR rows per thread per step: their keys load together, then their
-   * home slots in the LDS table, and only keys whose home slot holds
-   * another key walk on */
-#define QK_PROBE_ROWS 8
-  for (qk_u64 r = rs; r < re; r += (qk_u64)QK_WG * QK_PROBE_ROWS) {
-    qk_u64 rr[QK_PROBE_ROWS];
-    bool ok[QK_PROBE_ROWS];
-    qk_u32 key[QK_PROBE_ROWS], slot[QK_PROBE_ROWS];
-    qk_u64 e[QK_PROBE_ROWS];
+  /* QK_PROBE_ROWS rows per thread per step.  A row's key and first arg
+   * load together, one whole step ahead of their use, so a step's loads are
+   * in flight while the step before probes the LDS table; the rows are then
+   * probed four at a time to bound the registers held per row.  Rows past
+   * re clamp to rs (in bounds: the loop runs only when rs < re). */
+  const qk_u64 pstep = (qk_u64)QK_WG * QK_PROBE_ROWS;
+  qk_u32 nkey[QK_PROBE_ROWS], narg[QK_PROBE_ROWS];
+  auto load_rows = [&](qk_u64 r) {
 #pragma unroll
     for (int j = 0; j < QK_PROBE_ROWS; j++) {
       qk_u64 x = r + (qk_u64)j * QK_WG + threadIdx.x;
-      ok[j] = x < re;
-      rr[j] = ok[j] ? x : rs;
-      key[j] = rowsbuf[rr[j] * W];
-    }
-#pragma unroll
-    for (int j = 0; j < QK_PROBE_ROWS; j++) {
-      slot[j] = (qk_fmix32(key[j]) >> (32 - pbits - sbits)) & smask;
-      e[j] = t[slot[j]];
-    }
-    qk_u32 pay[QK_PROBE_ROWS];
-#pragma unroll
-    for (int j = 0; j < QK_PROBE_ROWS; j++) {
-      qk_u32 pj = 0;
-      qk_u64 ej = e[j];
-      qk_u32 sl = slot[j];
-      for (qk_u32 n = 0; n < S && ej != 0ULL; n++) {
-        if ((qk_u32)ej == key[j]) {
-          pj = (qk_u32)(ej >> 32);
-          break;
-        }
-        sl = (sl + 1) & smask;
-        ej = t[sl];
+      qk_u64 y = x < re ? x : rs;
+      if (W == 2) {
+        uint2 v = ((const uint2*)rowsbuf)[y];
+        nkey[j] = v.x;
+        narg[j] = v.y;
+      } else {
+        nkey[j] = rowsbuf[y * W];
+        narg[j] = W > 1 ? rowsbuf[y * W + 1] : 0u;
       }
-      pay[j] = (ok[j] && pj && pj - 1u < range) ? pj : 0u;
     }
+  };
+  if (rs < re) load_rows(rs);
+  for (qk_u64 r = rs; r < re; r += pstep) {
+    qk_u32 ckey[QK_PROBE_ROWS], carg[QK_PROBE_ROWS];
 #pragma unroll
     for (int j = 0; j < QK_PROBE_ROWS; j++) {
-      if (!pay[j]) continue;
-      if (per_row) atomicAdd(&g_cnt[pay[j] - 1u], 1ULL);
-      else atomicAdd(&l_cnt[pay[j] - 1u], 1u);
+      ckey[j] = nkey[j];
+      carg[j] = narg[j];
     }
-    for (qk_u32 c = 0; c < nargs; c++) {
-      qk_u32 at = (types >> (8 * c)) & 0xffu;
-      qk_u32 v[QK_PROBE_ROWS];
+    if (r + pstep < re) load_rows(r + pstep);
+    /* uniform: every row of the step lies in [rs, re) */
+    const bool full = r + pstep <= re;
 #pragma unroll
-      for (int j = 0; j < QK_PROBE_ROWS; j++) v[j] = rowsbuf[rr[j] * W + 1 + c];
+    for (int sb = 0; sb < QK_PROBE_ROWS; sb += 4) {
+      qk_u32 kw[4], home[4], match[4], pay[4];
+      QkBucket bk[4];
+      bool hit[4];
 #pragma unroll
-      for (int j = 0; j < QK_PROBE_ROWS; j++) {
-        if (!pay[j]) continue;
-        qk_u32 g = pay[j] - 1u;
-        qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)v[j] : (qk_u64)v[j];
-        qk_u32 code = qk_code32(v[j], at);
-        if (per_row) {
-          if (flags & 1u) atomicAdd(&g_sum[(qk_u64)c * range + g], add);
-          if (flags & 2u) {
-            atomicMin(&g_min[(qk_u64)c * range + g], code);
-            atomicMax(&g_max[(qk_u64)c * range + g], code);
+      for (int j = 0; j < 4; j++) {
+        const qk_u32 h = ckey[sb + j];
+        kw[j] = (h & low) | 0x80000000u;
+        home[j] = (h >> (32 - pbits - sbits)) & smask;
+      }
+      /* first buckets of the four rows; a bucket without the key is
+       * followed on only while it has no empty slot at or after the home
+       * slot (rare) */
+#pragma unroll
+      for (int j = 0; j < 4; j++) bk[j] = qk_bucket_read(tk, home[j] & ~(QK_BUCKET - 1u));
+#pragma unroll
+      for (int j = 0; j < 4; j++) match[j] = qk_bucket_match(bk[j], kw[j]);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        qk_u32 b = home[j] & ~(QK_BUCKET - 1u);
+        qk_u32 m = match[j];
+        if (!m) {
+          qk_u32 e = qk_bucket_match(bk[j], 0u) & (0xffu << (home[j] & (QK_BUCKET - 1u)));
+          for (qk_u32 n = QK_BUCKET; !m && !e && n < S; n += QK_BUCKET) {
+            b = (b + QK_BUCKET) & smask;
+            QkBucket nb = qk_bucket_read(tk, b);
+            m = qk_bucket_match(nb, kw[j]);
+            e = qk_bucket_match(nb, 0u);
           }
-        } else {
-          if (flags & 1u) atomicAdd(&l_sum[c * range + g], add);
-          if (flags & 2u) {
-            atomicMin(&l_min[c * range + g], code);
-            atomicMax(&l_max[c * range + g], code);
+        }
+        qk_u32 idx = b + (qk_u32)__builtin_ctz(m | 0x100u);
+        qk_u32 pj = m ? (per_row ? tp32[idx] : (qk_u32)tp16[idx]) : 0u;
+        hit[j] = m && (full || (r + (qk_u64)(sb + j) * QK_WG + threadIdx.x) < re) && pj < range;
+        pay[j] = pj;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (!hit[j]) continue;
+        if (per_row) atomicAdd(&g_cnt[pay[j]], 1ULL);
+        else atomicAdd(&l_cnt[pay[j]], 1u);
+      }
+      for (qk_u32 c = 0; c < nargs; c++) {
+        qk_u32 at = (types >> (8 * c)) & 0xffu;
+        qk_u32 v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          qk_u64 x = r + (qk_u64)(sb + j) * QK_WG + threadIdx.x;
+          v[j] = c == 0 ? carg[sb + j] : rowsbuf[(x < re ? x : rs) * W + 1 + c];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (!hit[j]) continue;
+          qk_u32 g = pay[j];
+          qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)v[j] : (qk_u64)v[j];
+          qk_u32 code = qk_code32(v[j], at);
+          if (per_row) {
+            if (do_sum) atomicAdd(&g_sum[(qk_u64)c * range + g], add);
+            if (do_mm) {
+              atomicMin(&g_min[(qk_u64)c * range + g], code);
+              atomicMax(&g_max[(qk_u64)c * range + g], code);
+            }
+          } else {
+            if (do_sum) atomicAdd(&l_sum[c * range + g], add);
+            if (do_mm) {
+              atomicMin(&l_min[c * range + g], code);
+              atomicMax(&l_max[c * range + g], code);
+            }
           }
         }
       }
@@ -1164,14 +1306,28 @@ extern "C" __global__ void LB k_join_part_probe(QK_Args a)
     if (!cnt) continue;
     atomicAdd(&g_cnt[i], (qk_u64)cnt);
     for (qk_u32 c = 0; c < nargs; c++) {
-      if (flags & 1u) atomicAdd(&g_sum[(qk_u64)c * range + i], l_sum[c * range + i]);
-      if (flags & 2u) {
+      if (do_sum) atomicAdd(&g_sum[(qk_u64)c * range + i], l_sum[c * range + i]);
+      if (do_mm) {
         atomicMin(&g_min[(qk_u64)c * range + i], l_min[c * range + i]);
         atomicMax(&g_max[(qk_u64)c * range + i], l_max[c * range + i]);
       }
     }
   }
 }
+
+/* instances: generic, then group mode with 0-2 args (sums, min/max) and row
+ * mode with 0-1 args; the host picks join_part_probe_<args><s|m|sm>[_row]
+ * when one matches and the generic form otherwise */
+extern "C" __global__ void LB k_join_part_probe(QK_Args a) { qk_part_probe<-1, false, false, false>(a); }
+extern "C" __global__ void LB k_join_part_probe_0(QK_Args a) { qk_part_probe<0, false, false, false>(a); }
+extern "C" __global__ void LB k_join_part_probe_1s(QK_Args a) { qk_part_probe<1, true, false, false>(a); }
+extern "C" __global__ void LB k_join_part_probe_1m(QK_Args a) { qk_part_probe<1, false, true, false>(a); }
+extern "C" __global__ void LB k_join_part_probe_1sm(QK_Args a) { qk_part_probe<1, true, true, false>(a); }
+extern "C" __global__ void LB k_join_part_probe_2s(QK_Args a) { qk_part_probe<2, true, false, false>(a); }
+extern "C" __global__ void LB k_join_part_probe_2sm(QK_Args a) { qk_part_probe<2, true, true, false>(a); }
+extern "C" __global__ void LB k_join_part_probe_0_row(QK_Args a) { qk_part_probe<0, false, false, true>(a); }
+extern "C" __global__ void LB k_join_part_probe_1s_row(QK_Args a) { qk_part_probe<1, true, false, true>(a); }
+extern "C" __global__ void LB k_join_part_probe_1sm_row(QK_Args a) { qk_part_probe<1, true, true, true>(a); }
 
 /* ------------------------------------------------------------------ */
 /* generators: splitmix64 addressed by output index                     */
@@ -1191,10 +1347,10 @@ __device__ __forceinline__ qk_u64 qk_splitmix64_at(qk_u64 seed, qk_u64 k)
 extern "C" __global__ void LB k_gen_events(QK_Args a)
 {
   qk_u64 n = qk_push(a, 0), seed = qk_push(a, 1), row0 = qk_push(a, 2);
-  qk_u32* id = (qk_u32*)a.ptr[6];
-  qk_u32* region = (qk_u32*)a.ptr[7];
-  qk_u32* amount = (qk_u32*)a.ptr[8];
-  qk_u32* ts = (qk_u32*)a.ptr[9];
+  qk_u32* id = qk_gptr<qk_u32>(a.ptr[6]);
+  qk_u32* region = qk_gptr<qk_u32>(a.ptr[7]);
+  qk_u32* amount = qk_gptr<qk_u32>(a.ptr[8]);
+  qk_u32* ts = qk_gptr<qk_u32>(a.ptr[9]);
   for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
     qk_u64 r = row0 + i;
     if (id) id[i] = (qk_u32)r;
@@ -1209,10 +1365,10 @@ extern "C" __global__ void LB k_gen_events(QK_Args a)
 extern "C" __global__ void LB k_gen_scale_events(QK_Args a)
 {
   qk_u64 n = qk_push(a, 0), row0 = qk_push(a, 2);
-  qk_u32* id = (qk_u32*)a.ptr[6];
-  qk_u32* region = (qk_u32*)a.ptr[7];
-  qk_u32* amount = (qk_u32*)a.ptr[8];
-  qk_u32* key = (qk_u32*)a.ptr[9];
+  qk_u32* id = qk_gptr<qk_u32>(a.ptr[6]);
+  qk_u32* region = qk_gptr<qk_u32>(a.ptr[7]);
+  qk_u32* amount = qk_gptr<qk_u32>(a.ptr[8]);
+  qk_u32* key = qk_gptr<qk_u32>(a.ptr[9]);
   for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
     qk_u64 r = row0 + i;
     if (id) id[i] = (qk_u32)(r + 1);
@@ -1228,7 +1384,7 @@ extern "C" __global__ void LB k_gen_scale_events(QK_Args a)
 extern "C" __global__ void LB k_gen_fact_fk(QK_Args a)
 {
   qk_u64 n = qk_push(a, 0), seed = qk_push(a, 1) ^ 0x5851F42D4C957F2DULL, row0 = qk_push(a, 2), dn = qk_push(a, 3);
-  qk_u32* fk = (qk_u32*)a.ptr[6];
+  qk_u32* fk = qk_gptr<qk_u32>(a.ptr[6]);
   const qk_u32* dk = qk_ptr<qk_u32>(a, 0);
   for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
     fk[i] = dk[qk_splitmix64_at(seed, row0 + i) % dn];
@@ -1270,8 +1426,8 @@ extern "C" __global__ void LB k_zonemap_build(QK_Args a)
       }
       mn = min(mn, s_mn[0]);
       mx = max(mx, s_mx[0]);
-      ((double*)a.ptr[1])[z] = mn;
-      ((double*)a.ptr[2])[z] = mx;
+      qk_gptr<double>(a.ptr[1])[z] = mn;
+      qk_gptr<double>(a.ptr[2])[z] = mx;
     }
   }
 }
@@ -1292,7 +1448,19 @@ static const QK_LibraryKernel g_library[] = {
     {"join_part_hist", (const void*)k_join_part_hist, QK_WG},
     {"join_part_offsets", (const void*)k_join_part_offsets, QK_WG},
     {"join_part_scatter", (const void*)k_join_part_scatter, QK_WG},
+    {"join_part_scatter_0", (const void*)k_join_part_scatter_0, QK_WG},
+    {"join_part_scatter_1", (const void*)k_join_part_scatter_1, QK_WG},
+    {"join_part_scatter_2", (const void*)k_join_part_scatter_2, QK_WG},
     {"join_part_probe", (const void*)k_join_part_probe, QK_WG},
+    {"join_part_probe_0", (const void*)k_join_part_probe_0, QK_WG},
+    {"join_part_probe_1s", (const void*)k_join_part_probe_1s, QK_WG},
+    {"join_part_probe_1m", (const void*)k_join_part_probe_1m, QK_WG},
+    {"join_part_probe_1sm", (const void*)k_join_part_probe_1sm, QK_WG},
+    {"join_part_probe_2s", (const void*)k_join_part_probe_2s, QK_WG},
+    {"join_part_probe_2sm", (const void*)k_join_part_probe_2sm, QK_WG},
+    {"join_part_probe_0_row", (const void*)k_join_part_probe_0_row, QK_WG},
+    {"join_part_probe_1s_row", (const void*)k_join_part_probe_1s_row, QK_WG},
+    {"join_part_probe_1sm_row", (const void*)k_join_part_probe_1sm_row, QK_WG},
     {"gen_events", (const void*)k_gen_events, QK_WG},
     {"gen_scale_events", (const void*)k_gen_scale_events, QK_WG},
     {"gen_fact_fk", (const void*)k_gen_fact_fk, QK_WG},
