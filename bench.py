@@ -37,9 +37,19 @@ METRIC = "warm GB/s & % HBM peak at 1B-row u32 agg/filter/join; served QPS at p9
 SEED = 0x1234567890ABCDEF  # bench_common.h:59
 DIM_A_ROWS = 1_000_000
 QUERY_BYTES_PER_ROW = {"q1": 4, "q2": 4, "q3": 8, "q4": 4, "q5": 12, "q6": 8, "q7": 4}
-# the kernel that reads the fact table in each query
-QUERY_KERNEL = {"q1": "aggregate_global", "q2": "scan_simple", "q3": "aggregate_dense", "q4": "aggregate_dense",
-                "q5": "qk_scan*", "q6": "join_probe", "q7": "aggregate_dense", "q8": "scan_simple"}
+# the kernels each query launches over the fact table, with the bytes per
+# fact row each one must move (reads + writes; the partitioned join's
+# scatter reads the key and amount and writes both back in partition order)
+QUERY_KERNELS = {
+    "q1": [("aggregate_global", 4)],
+    "q2": [("scan_simple", 4)],
+    "q3": [("aggregate_dense", 8)],
+    "q4": [("aggregate_dense", 4)],
+    "q5": [("qk_scan*", 12)],
+    "q6": [("join_part_hist", 4), ("join_part_scatter*", 16), ("join_part_probe*", 8), ("join_probe", 8)],
+    "q7": [("aggregate_dense", 4)],
+    "q8": [("scan_simple", 0)],
+}
 
 
 def log(msg: str) -> None:
@@ -130,10 +140,11 @@ def check_invariants(res, rows: int, step: int) -> list:
     return errs
 
 
-def timed_cpu_baseline(sample_rows: int, seed: int, workers: int):
+def timed_cpu_baseline(sample_rows: int, seed: int, workers: int, min_seconds: float):
     """The oracle (C restatement of the reference CPU operator path) over a
-    bounded sample of the same tables on all host cores.  Returns the GB/s
-    of algorithmic bytes and per-query seconds."""
+    bounded sample of the same tables: the q1-q7 suite repeated on all host
+    cores until min_seconds have passed, then once on one core.  Returns
+    (GB/s of algorithmic bytes, repetitions, per-query seconds, 1-core GB/s)."""
     import numpy as np
 
     from computeql_amd.ir import col
@@ -146,23 +157,49 @@ def timed_cpu_baseline(sample_rows: int, seed: int, workers: int):
     battr = dim_b_attr()
     cols = [id_, region, amount, ts]
     r, a, tsl = q5_literals(0, seed)
+    where5 = ((col(1) == r) & (col(2) < a)) | (col(3) > tsl)
+
+    def suite(nt, secs):
+        def timeit(name, fn):
+            t0 = time.perf_counter()
+            fn()
+            secs[name] = secs.get(name, 0.0) + time.perf_counter() - t0
+
+        timeit("q1", lambda: orc.aggregate_global([amount], workers=nt))
+        timeit("q2", lambda: orc.scan_chunked_count(cols, [(2, 1, 200.0)], workers=nt))
+        timeit("q3", lambda: orc.dense_group_u32(region, amount, 0, 50, workers=nt))
+        timeit("q4", lambda: orc.dense_group_u32(amount, amount, 0, 10000, workers=nt))
+        timeit("q5", lambda: orc.scan(cols, where5, nthreads=nt, count_only=True))
+        timeit("q6", lambda: orc.join_aggregate(fk, [amount], [1], dk, dim_group=seg, nthreads=nt))
+        timeit("q7", lambda: orc.join_aggregate(region, [None], [0], bk, dim_pass=(battr < 3), nthreads=nt))
+
+    bytes_per_suite = sum(QUERY_BYTES_PER_ROW.values()) * sample_rows
     secs = {}
+    reps = 0
+    while reps < 1000 and (reps == 0 or sum(secs.values()) < min_seconds):
+        suite(workers, secs)
+        reps += 1
+    one = {}
+    suite(1, one)
+    return (bytes_per_suite * reps / sum(secs.values()) / 1e9, reps, secs,
+            bytes_per_suite / sum(one.values()) / 1e9)
 
-    def timeit(name, fn):
-        t0 = time.perf_counter()
-        fn()
-        secs[name] = time.perf_counter() - t0
 
-    timeit("q1", lambda: orc.aggregate_global([amount], workers=workers))
-    timeit("q2", lambda: orc.scan_chunked_count(cols, [(2, 1, 200.0)], workers=workers))
-    timeit("q3", lambda: orc.dense_group_u32(region, amount, 0, 50, workers=workers))
-    timeit("q4", lambda: orc.dense_group_u32(amount, amount, 0, 10000, workers=workers))
-    timeit("q5", lambda: orc.scan(cols, ((col(1) == r) & (col(2) < a)) | (col(3) > tsl), nthreads=workers,
-                                  count_only=True))
-    timeit("q6", lambda: orc.join_aggregate(fk, [amount], [1], dk, dim_group=seg, nthreads=workers))
-    timeit("q7", lambda: orc.join_aggregate(region, [None], [0], bk, dim_pass=(battr < 3), nthreads=workers))
-    total_bytes = sum(QUERY_BYTES_PER_ROW.values()) * sample_rows
-    return total_bytes / sum(secs.values()) / 1e9, secs
+def measured_traffic(kernel: str, rows: int):
+    """HBM bytes per launch of `kernel` from the rocprofv3 --pmc summary kept
+    in profiles/traffic.json (FETCH_SIZE + WRITE_SIZE, scaled by a plain
+    read of known size), when that summary covers this kernel at this row
+    count; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel.split(":", 1)[-1].rstrip("*"))
+    if not k or rec.get("rows") != rows:
+        return None
+    return int(k["bytes"])
 
 
 def profile_stats(lib, name: str):
@@ -177,8 +214,12 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=1_000_000_000)
-    ap.add_argument("--cpu-sample-rows", type=int, default=16_000_000)
+    ap.add_argument("--cpu-sample-rows", type=int, default=64_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="after the timed steps, read one column once (the known-size read that scales "
+                         "rocprofv3 FETCH_SIZE readings)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}); this build has no multi-GPU path, "
@@ -203,7 +244,7 @@ def main(argv=None) -> int:
         log(f"warmup {w} done")
     lib.gpu_hip_wait()
     lib.gpu_hip_profile_reset()
-    wall = {q: 0.0 for q in QUERY_KERNEL}
+    wall = {q: 0.0 for q in QUERY_KERNELS}
     t_start = time.perf_counter()
     for s in range(a.steps):
         ts0 = time.perf_counter()
@@ -217,23 +258,33 @@ def main(argv=None) -> int:
 
     total_bytes = sum(QUERY_BYTES_PER_ROW.values()) * rows * a.steps
     value = total_bytes / elapsed / 1e9
+    # per-query kernel times from HIP events recorded on the launch stream
     kernels = {}
-    for q, k in QUERY_KERNEL.items():
-        ns, d = profile_stats(lib, f"{q}:{k}")
-        if d:
+    for q, ks in QUERY_KERNELS.items():
+        for k, bpr in ks:
+            ns, d = profile_stats(lib, f"{q}:{k}")
+            if not d:
+                continue
             avg_us = ns / d / 1e3
-            entry = {"kernel": k, "dispatches": d, "avg_us": round(avg_us, 2)}
-            if q in QUERY_BYTES_PER_ROW:
-                entry["gbs"] = round(QUERY_BYTES_PER_ROW[q] * rows / (avg_us * 1e-6) / 1e9, 1)
-            kernels[q] = entry
-    # dominant kernel: the query kernel with the most device time
-    dom_q = max((q for q in kernels if q in QUERY_BYTES_PER_ROW),
-                key=lambda q: kernels[q]["avg_us"] * kernels[q]["dispatches"])
-    dom = kernels[dom_q]
-    achieved = QUERY_BYTES_PER_ROW[dom_q] * rows / (dom["avg_us"] * 1e-6) / 1e9
+            entry = {"dispatches": d, "avg_us": round(avg_us, 2), "bytes_per_row": bpr}
+            if bpr:
+                entry["gbs"] = round(bpr * rows / (avg_us * 1e-6) / 1e9, 1)
+            kernels[f"{q}:{k}"] = entry
+    # dominant kernel: the most device time among kernels that stream the table
+    dom_name = max((n for n, e in kernels.items() if e["bytes_per_row"]),
+                   key=lambda n: kernels[n]["avg_us"] * kernels[n]["dispatches"])
+    dom = kernels[dom_name]
+    bytes_per_launch = dom["bytes_per_row"] * rows
+    achieved = bytes_per_launch / (dom["avg_us"] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_GBS, 4), "traffic": None,
-                "kernel": f"{dom['kernel']} ({dom_q})", "bytes_per_launch": QUERY_BYTES_PER_ROW[dom_q] * rows}
+                "frac": round(achieved / PEAK_GBS, 4), "traffic": measured_traffic(dom_name, rows),
+                "kernel": dom_name, "bytes_per_launch": bytes_per_launch, "avg_us": dom["avg_us"]}
+    if a.calibrate:
+        # a plain read of one known 4-byte column: its counter reading scales
+        # the FETCH_SIZE of the kernels in a --pmc run of this command
+        log("calibration read")
+        cq.bandwidth_read(t, 2)
+        lib.gpu_hip_wait()
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -241,12 +292,14 @@ def main(argv=None) -> int:
             workers = len(os.sched_getaffinity(0))
         except Exception:
             workers = os.cpu_count() or 1
-        log(f"cpu baseline: {a.cpu_sample_rows} rows, {workers} threads")
-        gbs, secs = timed_cpu_baseline(a.cpu_sample_rows, seed, workers)
+        log(f"cpu baseline: {a.cpu_sample_rows} rows, {workers} threads, at least {a.cpu_seconds} s")
+        gbs, reps, secs, gbs1 = timed_cpu_baseline(a.cpu_sample_rows, seed, workers, a.cpu_seconds)
         cpu = {"value": round(gbs, 3), "unit": "GB/s", "cores": workers, "kind": "port",
-               "sample": f"q1-q7 once each over a {a.cpu_sample_rows}-row sample of the same seeded tables "
-                         f"(oracle C restatement of the CPU operator path, {workers} threads)",
-               "per_query_s": {k: round(v, 4) for k, v in secs.items()}}
+               "sample": f"q1-q7 run {reps} times over a {a.cpu_sample_rows}-row sample of the same seeded tables "
+                         f"(the oracle's C restatement of the CPU operator path, {workers} threads, "
+                         f"{sum(secs.values()):.1f} s)",
+               "value_1_core": round(gbs1, 3),
+               "per_query_s": {k: round(v / reps, 4) for k, v in secs.items()}}
 
     out = {
         "metric": METRIC,
