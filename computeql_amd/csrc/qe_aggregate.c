@@ -334,6 +334,223 @@ void qe_dense_totals_free(QE_DenseTotals* d)
   d->block = 0;
 }
 
+/* ---------------- hash GROUP BY (query_exec.c:10258-12300) ---------------- */
+static F64 qe_hash_key_to_f64(U32 type, U64 code)
+{
+  if (type == CQL_ColumnType_U32 || type == CQL_ColumnType_U64) return (F64)code;
+  if (type == CQL_ColumnType_I32 || type == CQL_ColumnType_I64) return (F64)(S64)code;
+  F64 d;
+  memcpy(&d, &code, 8);
+  return d;
+}
+
+/* order-preserving code of an f64 (the kernels' qk_f64_key) back to f64 */
+static F64 qe_f64_from_order_code(U64 key)
+{
+  U64 b = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+  F64 d;
+  memcpy(&d, &b, 8);
+  return d;
+}
+
+static B32 qe_type_int32(U32 t) { return t == CQL_ColumnType_U32 || t == CQL_ColumnType_I32; }
+
+/* GROUP BY over the global hash table of aggregate_hash (key_col < 0: one
+ * group).  Rows: the fused leaves over the pruned range, or `list` (device
+ * row ids) when given.  Groups come out in table-slot order. */
+static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune,
+                             GPU_Buffer* list, U64 list_count, const QE_AggPlan* p, U64 cap, U64* out_group_count,
+                             F64* out_keys, U64* out_cells, QE_Trace* trace)
+{
+  GPU_Kernel* k = qe_kernel("aggregate_hash");
+  GPU_Kernel* kc = qe_kernel("hash_compact");
+  if (!k || !kc) return 0;
+  const U32 E = p->arg_count;
+  QE_ColumnSlots slots;
+  memset(&slots, 0, sizeof(slots));
+  U32 key_slot = key_col >= 0 ? qe_slot_of(&slots, (U32)key_col) : 0;
+  U32 key_type = key_col >= 0 ? t->columns[key_col].type : CQL_ColumnType_U32;
+  U32 slot_bits = 0, type_bits = 0;
+  for (U32 a = 0; a < E; a++) {
+    slot_bits |= qe_slot_of(&slots, p->arg_col[a]) << (4 * a);
+    type_bits |= t->columns[p->arg_col[a]].type << (8 * a);
+  }
+  U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
+  for (U32 l = 0; l < leaf_count; l++) {
+    U32 sl = qe_slot_of(&slots, leaves[l].column);
+    if (sl == ~0u) {
+      qe_set_error("hash group by: too many columns");
+      return 0;
+    }
+    qe_simple_leaf_words(&leaf_words[6 * l], sl, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+  }
+  if (slots.count > QE_MAX_COLUMN_BINDINGS || key_slot == ~0u) {
+    qe_set_error("hash group by: too many columns");
+    return 0;
+  }
+  U64 r0 = prune->row_begin, r1 = prune->row_end;
+  U64 rows = list ? list_count : r1 - r0;
+  /* table size: twice the groups there can be (rows, or the key's value
+   * span from the zone map), grown and rerun if it fills */
+  U64 est = rows ? rows : 1;
+  if (key_col < 0) est = 1;
+  else if (qe_type_int32(key_type) && t->columns[key_col].zonemap_valid) {
+    F64 span = t->columns[key_col].max_value - t->columns[key_col].min_value + 1.0;
+    if (span >= 1.0 && span < (F64)est) est = (U64)span;
+  }
+  U32 log2 = 10;
+  while (log2 < 28 && (1ull << log2) < 2 * est) log2++;
+  U64 forced = qe_settings_u64("QE_AGG_HASH_LOG2", 0); /* initial size override */
+  if (forced >= 4 && forced <= 28) log2 = (U32)forced;
+  U32 L = (U32)qe_settings_u64("QE_AGG_HASH_LDS_SLOTS", 512);
+  if (L & (L - 1)) L = 512;
+  if (L > 4096) L = 4096;
+  U64 lds = (U64)L * (8 + 24 * (U64)E + 4);
+  U32 flags = (p->need_sum ? 1u : 0u) | (p->need_minmax ? 2u : 0u) | (key_col < 0 ? 4u : 0u);
+  U64 cus = gpu_hip_compute_units();
+  U32 blocks;
+  if (list) {
+    U64 need = (list_count + QE_WG - 1) / QE_WG;
+    blocks = (U32)(need < cus * 4 ? (need ? need : 1) : cus * 4);
+  } else {
+    blocks = qe_stream_blocks((r1 + 3) / 4 - r0 / 4);
+  }
+  U32 rec_words = 2 + 3 * E;
+  GPU_Buffer* leaf_buf = leaf_count ? qe_scratch("hash_leaves", sizeof(leaf_words)) : 0;
+  GPU_Buffer* skip_buf = (!list && prune->skip_bits) ? qe_scratch("hash_skip", prune->skip_words * 4) : 0;
+  GPU_Buffer* flag_buf = qe_scratch("hash_flags", 64);
+  B32 ok = (!leaf_count || leaf_buf) && (list || !prune->skip_bits || skip_buf) && flag_buf;
+  U64 groups = 0;
+  U64* recs = 0;
+  B32 err_set = 0;
+  for (U32 attempt = 0; ok && attempt < 4; attempt++, log2++) {
+    U64 C = (1ull << log2) + 1;
+    U64 table_bytes = C * 8 * (2 + 3 * (U64)E);
+    U32 cblocks = (U32)((C + 4095) / 4096);
+    if (cblocks > 1024) cblocks = 1024;
+    GPU_Buffer* table = qe_scratch("hash_table", table_bytes);
+    GPU_Buffer* bc = qe_scratch("hash_block_counts", (U64)cblocks * 8);
+    GPU_Buffer* out = qe_scratch("hash_groups", C * 8 * rec_words);
+    if (!table || !bc || !out) {
+      qe_scratch_release(table);
+      qe_scratch_release(bc);
+      qe_scratch_release(out);
+      ok = 0;
+      break;
+    }
+    if (leaf_buf) gpu_kernel_set_arg_buffer(k, 0, leaf_buf);
+    else gpu_hip_kernel_clear_arg(k, 0);
+    gpu_kernel_set_arg_buffer(k, 1, table);
+    gpu_kernel_set_arg_buffer(k, 2, flag_buf);
+    if (list) gpu_kernel_set_arg_buffer(k, 3, list);
+    else gpu_hip_kernel_clear_arg(k, 3);
+    if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
+    else gpu_hip_kernel_clear_arg(k, 5);
+    for (U32 sl = 0; sl < QE_MAX_COLUMN_BINDINGS; sl++) {
+      if (sl < slots.count) gpu_kernel_set_arg_buffer(k, 6 + sl, t->columns[slots.column[sl]].buffer);
+      else gpu_hip_kernel_clear_arg(k, 6 + sl);
+    }
+    gpu_kernel_set_arg_u64(k, 0, rows);
+    gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
+    gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
+    gpu_kernel_set_arg_u64(k, 5, list ? 0 : r0);
+    gpu_kernel_set_arg_u64(k, 7, list ? 0 : leaf_count);
+    qe_kernel_set_word(k, QK_W_ARGS, E | (slot_bits << 8));
+    qe_kernel_set_word(k, QK_W_TYPES, type_bits);
+    qe_kernel_set_word(k, QK_W_KEY, key_slot | (key_type << 8) | (flags << 16));
+    qe_kernel_set_word(k, QK_W_LOG2, log2);
+    qe_kernel_set_word(k, QK_W_RANGE, L);
+    gpu_hip_kernel_set_lds_bytes(k, (U32)lds);
+    gpu_kernel_set_arg_buffer(kc, 1, table);
+    gpu_kernel_set_arg_buffer(kc, 2, bc);
+    gpu_kernel_set_arg_buffer(kc, 3, out);
+    qe_kernel_set_word(kc, QK_W_ARGS, E);
+    qe_kernel_set_word(kc, QK_W_LOG2, log2);
+    U32 fl[2] = {0, 0};
+    U64* bcounts = (U64*)malloc((U64)cblocks * 8);
+    if (!bcounts) {
+      ok = 0;
+    } else {
+      GPU_Batch* b = gpu_batch_begin(0, (U64)cblocks * 8 + 8);
+      if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
+      if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
+      gpu_batch_buffer_zero(b, flag_buf, 8);
+      /* keys and mins start at all ones: zero everything, then fill the
+       * leading key + min arrays */
+      gpu_batch_buffer_zero(b, table, table_bytes);
+      gpu_batch_buffer_fill(b, table, C * 8 * (1 + (U64)E), 0xffffffffu);
+      gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+      gpu_kernel_set_arg_u64(kc, 6, 1); /* push slot 6: mode, block counts */
+      gpu_batch_kernel_execute(b, kc, cblocks * QE_WG, QE_WG);
+      gpu_kernel_set_arg_u64(kc, 6, 2); /* write records */
+      gpu_batch_kernel_execute(b, kc, cblocks * QE_WG, QE_WG);
+      gpu_batch_buffer_read(b, flag_buf, fl, 8);
+      gpu_batch_buffer_read(b, bc, bcounts, (U64)cblocks * 8);
+      ok = gpu_batch_end(b);
+      gpu_hip_kernel_set_lds_bytes(k, 0);
+    }
+    B32 full = ok && fl[0] != 0;
+    if (ok && !full) {
+      groups = 0;
+      for (U32 i = 0; i < cblocks; i++) groups += bcounts[i];
+      U64 take = groups < cap ? groups : cap;
+      recs = (U64*)malloc((take ? take : 1) * 8 * rec_words);
+      if (!recs) ok = 0;
+      else if (take) gpu_buffer_read(out, recs, take * 8 * rec_words);
+    }
+    free(bcounts);
+    qe_scratch_release(table);
+    qe_scratch_release(bc);
+    qe_scratch_release(out);
+    if (!full) break;
+    if (attempt == 3) {
+      qe_set_error("hash group by: table of 2^%u slots full", log2);
+      err_set = 1;
+      ok = 0;
+    }
+  }
+  qe_scratch_release(leaf_buf);
+  qe_scratch_release(skip_buf);
+  qe_scratch_release(flag_buf);
+  if (!ok) {
+    free(recs);
+    if (!err_set) qe_set_error("aggregate_hash: %s", gpu_hip_last_error());
+    return 0;
+  }
+  U64 take = groups < cap ? groups : cap;
+  for (U64 g = 0; g < take; g++) {
+    const U64* r = recs + g * rec_words;
+    if (out_keys) out_keys[g] = key_col >= 0 ? qe_hash_key_to_f64(key_type, r[0]) : 0.0;
+    U64 cnt = r[1];
+    for (U32 e = 0; e < p->expr_count; e++) {
+      S32 a = p->expr_arg[e];
+      F64 sum = 0.0, mn = 0.0, mx = 0.0;
+      if (a >= 0) {
+        U32 ty = t->columns[p->arg_col[a]].type;
+        U64 sv = r[2 + a];
+        if (qe_type_int32(ty)) sum = (F64)(S64)sv;
+        else memcpy(&sum, &sv, 8);
+        mn = qe_f64_from_order_code(r[2 + E + a]);
+        mx = qe_f64_from_order_code(r[2 + 2 * E + a]);
+      }
+      out_cells[g * p->expr_count + e] = qe_agg_cell(p->funcs[e], cnt, sum, mn, mx);
+    }
+  }
+  free(recs);
+  *out_group_count = groups;
+  if (trace) {
+    trace->path = QE_AggPath_Hash;
+    trace->kernel_count = 3;
+    trace->rows_scanned = rows;
+    trace->rows_after = groups;
+    U64 bytes = 0;
+    for (U32 sl = 0; sl < slots.count; sl++) bytes += rows * t->columns[slots.column[sl]].width;
+    trace->bytes_read = bytes;
+    snprintf(trace->kernel, sizeof(trace->kernel), "aggregate_hash");
+  }
+  return 1;
+}
+
 /* ---------------- entry point ---------------- */
 B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 group_col, const U32* funcs,
                  const S32* arg_cols, U32 expr_count, U64 cap, U64* out_group_count, F64* out_keys, U64* out_cells,
@@ -352,16 +569,53 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
   /* WHERE: numeric AND leaves fuse into the aggregate kernels */
   QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
   U32 leaf_count = 0;
+  B32 general_where = 0;
   if (where && where_count) {
     static __thread QE_Bytecode prog;
     if (!qe_bytecode_program_build(table, where, where_count, &prog)) {
       qe_set_error("qe_aggregate: %s", prog.error);
       return 0;
     }
+    /* an AND of column-literal comparisons fuses into the aggregate
+     * kernels; any other WHERE runs as a scan whose row ids stay on the
+     * device and feed the hash aggregate */
     if (!qe_collect_and_leaves(where, where_count, leaves, &leaf_count, QE_SCAN_MAX_LEAVES)) {
-      qe_set_error("qe_aggregate: WHERE must be an AND of column-literal comparisons to fuse");
-      return 0;
+      general_where = 1;
+      leaf_count = 0;
     }
+  }
+  if (group_col >= 0 && (U32)group_col >= table->column_count) {
+    qe_set_error("qe_aggregate: unknown group column");
+    return 0;
+  }
+  if (general_where) {
+    GPU_Buffer* ids = 0;
+    U64 n = 0;
+    if (!qe_scan_filter_device(table, where, where_count, &ids, &n)) return 0;
+    QE_Prune all;
+    memset(&all, 0, sizeof(all));
+    all.row_end = table->row_count;
+    U64 groups = 0;
+    B32 ok = 1;
+    if (n) ok = qe_aggregate_hash(table, group_col, 0, 0, &all, ids, n, &plan, cap, &groups, out_keys, out_cells, trace);
+    qe_scratch_release(ids);
+    if (!ok) return 0;
+    if (group_col < 0 && groups == 0) {
+      /* an ungrouped aggregate over no rows is one row of zeros
+       * (query_exec.c:10335-10346) */
+      if (cap >= 1) {
+        for (U32 e = 0; e < expr_count; e++) out_cells[e] = qe_agg_cell(plan.funcs[e], 0, 0.0, 0.0, 0.0);
+        if (out_keys) out_keys[0] = 0.0;
+      }
+      groups = 1;
+    }
+    *out_group_count = groups;
+    if (trace) {
+      trace->path = QE_AggPath_Hash;
+      trace->rows_after = groups;
+    }
+    qe_trace_end(trace, t0, p0);
+    return 1;
   }
   QE_Leaf prune_leaves[32];
   U32 prune_count = qe_collect_prune_leaves(where, where_count, prune_leaves, 32);
@@ -390,31 +644,32 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
     return ok;
   }
 
-  if ((U32)group_col >= table->column_count) {
-    qe_prune_free(&prune);
-    qe_set_error("qe_aggregate: unknown group column");
-    return 0;
-  }
   QE_Column* key = &table->columns[group_col];
-  if (key->type != CQL_ColumnType_U32 && key->type != CQL_ColumnType_I32) {
-    qe_prune_free(&prune);
-    qe_set_error("qe_aggregate: GROUP BY key must be U32/I32 on this path");
-    return 0;
-  }
   if (prune.zones_left == 0 || table->row_count == 0) {
     qe_prune_free(&prune);
     *out_group_count = 0;
     qe_trace_end(trace, t0, p0);
     return 1;
   }
-  /* key range from the zone map (query_exec.c:7636-7647) */
+  /* dense LDS tables when the key is a 32-bit integer whose range (from the
+   * zone map, query_exec.c:7636-7647) is small and the args are 32-bit
+   * integers; the hash table otherwise */
   F64 kmin = key->min_value, kmax = key->max_value;
   F64 span = kmax - kmin + 1.0;
   U64 max_range = qe_settings_u64("QE_AGG_DENSE_MAX_RANGE", 65536);
-  if (qe_settings_u64("QE_AGG_DENSE", 1) == 0 || span > (F64)max_range) {
+  B32 dense = qe_settings_u64("QE_AGG_DENSE", 1) != 0 && qe_type_int32(key->type) && key->zonemap_valid &&
+              span <= (F64)max_range;
+  for (U32 a = 0; a < plan.arg_count && dense; a++)
+    if (!qe_type_int32(table->columns[plan.arg_col[a]].type)) dense = 0;
+  if (!dense) {
+    U64 groups = 0;
+    ok = qe_aggregate_hash(table, group_col, leaves, leaf_count, &prune, 0, 0, &plan, cap, &groups, out_keys, out_cells,
+                           trace);
     qe_prune_free(&prune);
-    qe_set_error("qe_aggregate: key range %.0f above the dense limit (hash GROUP BY not built)", span);
-    return 0;
+    if (!ok) return 0;
+    *out_group_count = groups;
+    qe_trace_end(trace, t0, p0);
+    return 1;
   }
   U32 range = (U32)span;
   U32 key_min_code = key->type == CQL_ColumnType_I32 ? ((U32)(S32)kmin ^ 0x80000000u) : (U32)kmin;

@@ -1330,6 +1330,318 @@ extern "C" __global__ void LB k_join_part_probe_1s_row(QK_Args a) { qk_part_prob
 extern "C" __global__ void LB k_join_part_probe_1sm_row(QK_Args a) { qk_part_probe<1, true, true, true>(a); }
 
 /* ------------------------------------------------------------------ */
+/* hash GROUP BY (query_exec.c:10258-12300, general path: slot          */
+/* assignment by CAS, then compaction and reduction)                    */
+/* Keys are 64-bit codes: integers sign- or zero-extended, floats as    */
+/* the bits of the value widened to f64 with -0.0 folded into 0.0.      */
+/* Two levels: each block keeps an LDS table of QK_W_RANGE slots that   */
+/* absorbs frequent keys; rows whose key finds no LDS slot within four  */
+/* probes go to a global open-addressing table of 2^QK_W_LOG2 slots in  */
+/* HBM.  The code ~0 (a 64-bit key only) has its own extra slot.        */
+/* Global layout, C = 2^LOG2 + 1 slots: key u64[C] | min u64[E][C] |    */
+/* count u64[C] | sum u64[E][C] | max u64[E][C]; sums are exact 64-bit  */
+/* integers for U32/I32 args and f64 bits otherwise; min/max are        */
+/* order-preserving codes of the f64 value.                             */
+/* ------------------------------------------------------------------ */
+#define QK_HASH_EMPTY 0xffffffffffffffffULL
+#define QK_HASH_MAX_ARGS 4
+
+__device__ __forceinline__ qk_u64 qk_fmix64(qk_u64 k)
+{
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdULL;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ULL;
+  k ^= k >> 33;
+  return k;
+}
+
+__device__ __forceinline__ qk_u64 qk_key_code_f64(double d) { return d == 0.0 ? 0ULL : (qk_u64)__double_as_longlong(d); }
+
+/* key codes of the four rows of quad q */
+__device__ __forceinline__ void qk_load4_code(const QK_Args& a, qk_u32 slot, qk_u32 type, qk_u64 q, qk_u64 c[4])
+{
+  const qk_u64 base = a.ptr[QK_B_COL0 + slot];
+  if (type == QK_T_U32 || type == QK_T_I32 || type == QK_T_F32) {
+    const qk_u32x4 x = qk_gptr<const qk_u32x4>(base)[q];
+    const qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      c[k] = type == QK_T_U32 ? (qk_u64)w[k]
+                              : (type == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)w[k] : qk_key_code_f64((double)__uint_as_float(w[k])));
+  } else {
+    const qk_u64x2* p = qk_gptr<const qk_u64x2>(base) + 2 * q;
+    const qk_u64x2 x0 = p[0], x1 = p[1];
+    const qk_u64 w[4] = {x0.x, x0.y, x1.x, x1.y};
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = type == QK_T_F64 ? qk_key_code_f64(__longlong_as_double((qk_s64)w[k])) : w[k];
+  }
+}
+
+__device__ __forceinline__ qk_u64 qk_load1_code(const QK_Args& a, qk_u32 slot, qk_u32 type, qk_u64 row)
+{
+  const qk_u64 base = a.ptr[QK_B_COL0 + slot];
+  switch (type) {
+    case QK_T_U32: return (qk_u64)qk_gptr<const qk_u32>(base)[row];
+    case QK_T_I32: return (qk_u64)(qk_s64)qk_gptr<const qk_s32>(base)[row];
+    case QK_T_F32: return qk_key_code_f64((double)qk_gptr<const float>(base)[row]);
+    case QK_T_F64: return qk_key_code_f64(qk_gptr<const double>(base)[row]);
+    default: return qk_gptr<const qk_u64>(base)[row];
+  }
+}
+
+/* the global slot of `key`, inserting it; ~0 when the table is full.  Slots
+ * change only from EMPTY to a key, so a stale read can only show EMPTY, and
+ * the CAS then returns the key that is really there. */
+__device__ __forceinline__ qk_u64 qk_hash_slot(qk_u64* gkey, qk_u32 log2, qk_u64 key, qk_u32* flags)
+{
+  const qk_u64 cap = 1ULL << log2, mask = cap - 1;
+  if (key == QK_HASH_EMPTY) return cap;
+  qk_u64 s = qk_fmix64(key) & mask;
+  for (qk_u64 n = 0; n < cap; n++) {
+    qk_u64 cur = gkey[s];
+    if (cur == key) return s;
+    if (cur == QK_HASH_EMPTY) {
+      qk_u64 prev = atomicCAS((unsigned long long*)&gkey[s], QK_HASH_EMPTY, key);
+      if (prev == QK_HASH_EMPTY || prev == key) return s;
+    }
+    s = (s + 1) & mask;
+  }
+  atomicOr(&flags[0], 1u);
+  return ~0ULL;
+}
+
+struct QkHashCtx {
+  qk_u32 E, types, flags, L, log2;
+  qk_u64 C;
+  qk_u64* gkey;
+  qk_u64* gmin;
+  qk_u64* gcnt;
+  qk_u64* gsum;
+  qk_u64* gmax;
+  qk_u32* gflags;
+  qk_u64* lkey;
+  qk_u32* lcnt;
+  qk_u64* lsum;
+  qk_u64* lmin;
+  qk_u64* lmax;
+};
+
+__device__ __forceinline__ void qk_hash_global_add(const QkHashCtx& h, qk_u64 key, qk_u64 cnt, const qk_u64* sum,
+                                                   const qk_u64* mn, const qk_u64* mx)
+{
+  const qk_u64 s = qk_hash_slot(h.gkey, h.log2, key, h.gflags);
+  if (s == ~0ULL) return;
+  atomicAdd((unsigned long long*)&h.gcnt[s], (unsigned long long)cnt);
+  for (qk_u32 c = 0; c < h.E; c++) {
+    const qk_u32 at = (h.types >> (8 * c)) & 0xffu;
+    if (h.flags & 1u) {
+      if (qk_type_is_int32(at)) atomicAdd((unsigned long long*)&h.gsum[(qk_u64)c * h.C + s], (unsigned long long)sum[c]);
+      else unsafeAtomicAdd((double*)&h.gsum[(qk_u64)c * h.C + s], __longlong_as_double((qk_s64)sum[c]));
+    }
+    if (h.flags & 2u) {
+      atomicMin((unsigned long long*)&h.gmin[(qk_u64)c * h.C + s], (unsigned long long)mn[c]);
+      atomicMax((unsigned long long*)&h.gmax[(qk_u64)c * h.C + s], (unsigned long long)mx[c]);
+    }
+  }
+}
+
+/* one row: into the block's LDS table when its key has (or gets) a slot
+ * there within four probes, else straight into the global table */
+__device__ __forceinline__ void qk_hash_row(const QkHashCtx& h, qk_u64 key, const double* v)
+{
+  qk_u64 sum[QK_HASH_MAX_ARGS], code[QK_HASH_MAX_ARGS];
+#pragma unroll
+  for (int c = 0; c < QK_HASH_MAX_ARGS; c++) {
+    const qk_u32 at = (h.types >> (8 * c)) & 0xffu;
+    sum[c] = qk_type_is_int32(at) ? (qk_u64)(qk_s64)v[c] : (qk_u64)__double_as_longlong(v[c]);
+    code[c] = qk_f64_key(v[c]);
+  }
+  if (h.L && key != QK_HASH_EMPTY) {
+    const qk_u32 lmask = h.L - 1u;
+    qk_u32 s = (qk_u32)qk_fmix64(key) & lmask;
+#pragma unroll 1
+    for (int p = 0; p < 4; p++, s = (s + 1u) & lmask) {
+      qk_u64 cur = h.lkey[s];
+      if (cur == QK_HASH_EMPTY) cur = atomicCAS((unsigned long long*)&h.lkey[s], QK_HASH_EMPTY, key);
+      if (cur != QK_HASH_EMPTY && cur != key) continue;
+      atomicAdd(&h.lcnt[s], 1u);
+      for (qk_u32 c = 0; c < h.E; c++) {
+        const qk_u32 at = (h.types >> (8 * c)) & 0xffu;
+        if (h.flags & 1u) {
+          if (qk_type_is_int32(at)) atomicAdd((unsigned long long*)&h.lsum[c * h.L + s], (unsigned long long)sum[c]);
+          else atomicAdd((double*)&h.lsum[c * h.L + s], v[c]);
+        }
+        if (h.flags & 2u) {
+          atomicMin((unsigned long long*)&h.lmin[c * h.L + s], (unsigned long long)code[c]);
+          atomicMax((unsigned long long*)&h.lmax[c * h.L + s], (unsigned long long)code[c]);
+        }
+      }
+      return;
+    }
+  }
+  qk_hash_global_add(h, key, 1, sum, code, code);
+}
+
+/* aggregate_hash: binding 0 leaves, 1 table (layout above), 2 flags
+ * (u32[0]: global table full), 3 row list (u64 ids, optional: then push 0
+ * is the list length), 5 skip bits, 6.. columns; push 0 rows, 3/4 skip,
+ * 5 first row, 7 leaf count; words ARGS (count | slots << 8), TYPES, KEY
+ * (key slot | key type << 8 | flags << 16: 1 sums, 2 min/max, 4 no key,
+ * every row in group 0), LOG2, RANGE (LDS slots, a power of two or 0).
+ * LDS: key u64[L] | min u64[E][L] | max u64[E][L] | sum u64[E][L] |
+ * count u32[L] */
+extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
+{
+  extern __shared__ qk_u64 lds_u64[];
+  QkHashCtx h;
+  h.E = a.words[QK_W_ARGS] & 0xffu;
+  if (h.E > QK_HASH_MAX_ARGS) h.E = QK_HASH_MAX_ARGS;
+  const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
+  h.types = a.words[QK_W_TYPES];
+  const qk_u32 key_slot = a.words[QK_W_KEY] & 0xffu;
+  const qk_u32 key_type = (a.words[QK_W_KEY] >> 8) & 0xffu;
+  h.flags = a.words[QK_W_KEY] >> 16;
+  h.log2 = a.words[QK_W_LOG2];
+  h.L = a.words[QK_W_RANGE];
+  h.C = (1ULL << h.log2) + 1;
+  h.gkey = qk_wptr<qk_u64>(a, 1);
+  h.gmin = h.gkey + h.C;
+  h.gcnt = h.gmin + (qk_u64)h.E * h.C;
+  h.gsum = h.gcnt + h.C;
+  h.gmax = h.gsum + (qk_u64)h.E * h.C;
+  h.gflags = qk_wptr<qk_u32>(a, 2);
+  h.lkey = lds_u64;
+  h.lmin = h.lkey + h.L;
+  h.lmax = h.lmin + (qk_u64)h.E * h.L;
+  h.lsum = h.lmax + (qk_u64)h.E * h.L;
+  h.lcnt = (qk_u32*)(h.lsum + (qk_u64)h.E * h.L);
+  for (qk_u32 i = threadIdx.x; i < h.L; i += QK_WG) {
+    h.lkey[i] = QK_HASH_EMPTY;
+    h.lcnt[i] = 0;
+    for (qk_u32 c = 0; c < h.E; c++) {
+      h.lsum[c * h.L + i] = 0;
+      h.lmin[c * h.L + i] = QK_HASH_EMPTY;
+      h.lmax[c * h.L + i] = 0;
+    }
+  }
+  __syncthreads();
+  const bool no_key = (h.flags & 4u) != 0;
+  const qk_u64* list = a.ptr[3] ? qk_ptr<qk_u64>(a, 3) : nullptr;
+  if (list) {
+    const qk_u64 n = qk_push(a, QK_P_ROWS);
+    for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
+      const qk_u64 row = list[i];
+      double v[QK_HASH_MAX_ARGS] = {0.0, 0.0, 0.0, 0.0};
+      for (qk_u32 c = 0; c < h.E; c++)
+        v[c] = qk_load1_f64(a, (slots >> (4 * c)) & 0xfu, (h.types >> (8 * c)) & 0xffu, row);
+      qk_hash_row(h, no_key ? 0ULL : qk_load1_code(a, key_slot, key_type, row), v);
+    }
+  } else {
+    const qk_u64 rows = qk_push(a, QK_P_ROWS);
+    const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
+    const qk_u64 r1 = r0 + rows;
+    const QkLeafSet L = qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES));
+    qk_u64 b0, b1;
+    qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
+    for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
+      qk_u64 q[QK_UNROLL];
+      qk_u32 m[QK_UNROLL];
+      qk_step_quads(a, s, b1, r0, r1, q, m);
+      if (L.n) qk_leaves_step(a, L, q, m);
+#pragma unroll 1
+      for (int j = 0; j < QK_UNROLL; j++) {
+        if (!m[j]) continue;
+        qk_u64 kc[4] = {0, 0, 0, 0};
+        if (!no_key) qk_load4_code(a, key_slot, key_type, q[j], kc);
+        double v[QK_HASH_MAX_ARGS][4];
+#pragma unroll
+        for (int c = 0; c < QK_HASH_MAX_ARGS; c++) {
+          if ((qk_u32)c < h.E) qk_load4_f64(a, (slots >> (4 * c)) & 0xfu, (h.types >> (8 * c)) & 0xffu, q[j], v[c]);
+          else v[c][0] = v[c][1] = v[c][2] = v[c][3] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (!(m[j] & (1u << k))) continue;
+          const double vk[QK_HASH_MAX_ARGS] = {v[0][k], v[1][k], v[2][k], v[3][k]};
+          qk_hash_row(h, kc[k], vk);
+        }
+      }
+    }
+  }
+  /* flush the LDS table into the global one */
+  __syncthreads();
+  for (qk_u32 i = threadIdx.x; i < h.L; i += QK_WG) {
+    const qk_u32 cnt = h.lcnt[i];
+    if (!cnt) continue;
+    qk_u64 sum[QK_HASH_MAX_ARGS], mn[QK_HASH_MAX_ARGS], mx[QK_HASH_MAX_ARGS];
+    for (qk_u32 c = 0; c < QK_HASH_MAX_ARGS; c++) {
+      const bool in = c < h.E;
+      sum[c] = in ? h.lsum[c * h.L + i] : 0;
+      mn[c] = in ? h.lmin[c * h.L + i] : QK_HASH_EMPTY;
+      mx[c] = in ? h.lmax[c * h.L + i] : 0;
+    }
+    qk_hash_global_add(h, h.lkey[i], cnt, sum, mn, mx);
+  }
+}
+
+/* hash_compact: the occupied slots of aggregate_hash's table, in slot
+ * order.  binding 1 table, 2 block counts (u64 per block), 3 out records
+ * (u64[2 + 3E] each: key, count, sums, mins, maxs); push 6 mode (1 block
+ * counts, 2 write), words ARGS (count), LOG2.  A slot is occupied when its
+ * count is nonzero (rows only ever add). */
+extern "C" __global__ void LB k_hash_compact(QK_Args a)
+{
+  __shared__ qk_u32 s_scan[QK_WG / 64];
+  __shared__ qk_u64 s_red[QK_WG / 64];
+  const qk_u32 E = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 log2 = a.words[QK_W_LOG2];
+  const qk_u64 C = (1ULL << log2) + 1;
+  const qk_u64* gkey = qk_ptr<qk_u64>(a, 1);
+  const qk_u64* gmin = gkey + C;
+  const qk_u64* gcnt = gmin + (qk_u64)E * C;
+  const qk_u64* gsum = gcnt + C;
+  const qk_u64* gmax = gsum + (qk_u64)E * C;
+  qk_u64* bc = qk_wptr<qk_u64>(a, 2);
+  const qk_u64 mode = qk_push(a, QK_P_MODE);
+  const qk_u64 per = (C + gridDim.x - 1) / gridDim.x;
+  const qk_u64 s0 = per * blockIdx.x < C ? per * blockIdx.x : C;
+  const qk_u64 s1 = s0 + per < C ? s0 + per : C;
+  if (mode == 1) {
+    qk_u64 n = 0;
+    for (qk_u64 s = s0 + threadIdx.x; s < s1; s += QK_WG) n += gcnt[s] ? 1 : 0;
+    n = qk_block_sum_u64(n, s_red);
+    if (threadIdx.x == 0) bc[blockIdx.x] = n;
+    return;
+  }
+  qk_u64 base = 0;
+  for (qk_u32 b = threadIdx.x; b < blockIdx.x; b += QK_WG) base += bc[b];
+  base = qk_block_sum_u64(base, s_red);
+  qk_u64* out = qk_wptr<qk_u64>(a, 3);
+  const qk_u64 cap = a.bytes[3] / (8ULL * (2 + 3 * E));
+  for (qk_u64 c0 = s0; c0 < s1; c0 += QK_WG) {
+    const qk_u64 s = c0 + threadIdx.x;
+    const qk_u32 occ = (s < s1 && gcnt[s]) ? 1u : 0u;
+    qk_u32 total = 0;
+    const qk_u32 pre = qk_block_excl_scan_u32(occ, s_scan, total);
+    if (occ) {
+      const qk_u64 g = base + pre;
+      if (g < cap) {
+        qk_u64* r = out + g * (2 + 3 * (qk_u64)E);
+        r[0] = gkey[s];
+        r[1] = gcnt[s];
+        for (qk_u32 c = 0; c < E; c++) {
+          r[2 + c] = gsum[(qk_u64)c * C + s];
+          r[2 + E + c] = gmin[(qk_u64)c * C + s];
+          r[2 + 2 * E + c] = gmax[(qk_u64)c * C + s];
+        }
+      }
+    }
+    base += total;
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* generators: splitmix64 addressed by output index                     */
 /* (bench_common.h:30-37): d(k) = mix(seed + (k+1) * 0x9E3779B97F4A7C15) */
 /* ------------------------------------------------------------------ */
@@ -1444,6 +1756,8 @@ static const QK_LibraryKernel g_library[] = {
     {"aggregate_dense", (const void*)k_aggregate_dense, QK_WG},
     {"join_build", (const void*)k_join_build, QK_WG},
     {"join_probe", (const void*)k_join_probe, QK_WG},
+    {"aggregate_hash", (const void*)k_aggregate_hash, QK_WG},
+    {"hash_compact", (const void*)k_hash_compact, QK_WG},
     {"join_part_build", (const void*)k_join_part_build, QK_WG},
     {"join_part_hist", (const void*)k_join_part_hist, QK_WG},
     {"join_part_offsets", (const void*)k_join_part_offsets, QK_WG},
