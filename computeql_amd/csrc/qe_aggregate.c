@@ -23,18 +23,7 @@
 
 #include "qe_internal.h"
 
-#define QE_AGG_MAX_ARGS 4
-
-typedef struct QE_AggPlan {
-  U32 expr_count;
-  U32 funcs[QE_AGG_MAX_EXPRS];
-  S32 expr_arg[QE_AGG_MAX_EXPRS]; /* index into args, -1 for COUNT(*) */
-  U32 arg_count;
-  U32 arg_col[QE_AGG_MAX_ARGS];
-  B32 need_sum, need_minmax;
-} QE_AggPlan;
-
-static B32 qe_agg_plan(QE_Table* t, const U32* funcs, const S32* arg_cols, U32 expr_count, QE_AggPlan* p)
+B32 qe_agg_plan(QE_Table* t, const U32* funcs, const S32* arg_cols, U32 expr_count, QE_AggPlan* p)
 {
   memset(p, 0, sizeof(*p));
   if (expr_count == 0 || expr_count > QE_AGG_MAX_EXPRS) {

@@ -1,0 +1,480 @@
+/*
+ * qe_batch.c - batched shared scan: many aggregate queries over one table
+ * in one pass over the union of their columns.
+ *
+ * The reference runs one query per kernel (fused WHERE + aggregate,
+ * query_exec.c:9442-10253 with query_kernel.c:1433-1600) and serialises
+ * queries behind one mutex (application.h:34).  Here a batch of queries is
+ * compiled into a program for the shared_scan kernel: each block walks row
+ * tiles and evaluates every query of the launch on its rows, so a column is
+ * read from HBM once per launch however many queries touch it.  Results
+ * are decoded exactly as qe_aggregate decodes its own (qe_agg_cell), with
+ * integer sums exact and f64 sums in a different (stated) order: for an
+ * ungrouped query, a lane's rows of a tile, then the wave (DPP tree), then
+ * the block's word across tiles in tile order; for a grouped query, the
+ * rows' LDS atomic adds in arrival order; then per word the blocks, lane l
+ * of the reducing wave folding blocks l, l + 64, ... in order and the wave
+ * combining its lanes (DPP tree).
+ */
+#define _GNU_SOURCE
+#include <stdlib.h>
+
+#include "qe_internal.h"
+
+#define QE_BATCH_REC_WORDS 12
+#define QE_BATCH_MAX_LAUNCH_QUERIES 256
+
+typedef struct QE_BatchPlanned {
+  U32 index; /* into the caller's array */
+  QE_AggPlan plan;
+  QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
+  U32 leaf_count;
+  S32 group_col;
+  U32 key_min, range;
+  U32 acc_off, acc_words;
+} QE_BatchPlanned;
+
+static B32 qe_batch_int32(U32 t) { return t == CQL_ColumnType_U32 || t == CQL_ColumnType_I32; }
+
+static F64 qe_batch_f64_from_code(U64 key)
+{
+  U64 b = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+  F64 d;
+  memcpy(&d, &b, 8);
+  return d;
+}
+
+/* can this query run in a shared scan?  fills `out` when it can */
+static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlanned* out)
+{
+  memset(out, 0, sizeof(*out));
+  if (!q->funcs || !q->arg_cols || !q->out_group_count || !q->out_cells) return 0;
+  if (!qe_agg_plan(t, q->funcs, q->arg_cols, q->expr_count, &out->plan)) return 0;
+  if (out->plan.arg_count > 2) return 0;
+  if (q->where && q->where_count) {
+    static __thread QE_Bytecode prog;
+    if (!qe_bytecode_program_build(t, q->where, q->where_count, &prog)) return 0;
+    if (!qe_collect_and_leaves(q->where, q->where_count, out->leaves, &out->leaf_count, QE_SCAN_MAX_LEAVES)) return 0;
+  }
+  out->group_col = q->group_col;
+  out->range = 1;
+  if (q->group_col >= 0) {
+    if ((U32)q->group_col >= t->column_count) return 0;
+    QE_Column* k = &t->columns[q->group_col];
+    if (!qe_batch_int32(k->type) || !k->zonemap_valid || t->row_count == 0) return 0;
+    F64 span = k->max_value - k->min_value + 1.0;
+    if (span < 1.0 || span > (F64)qe_settings_u64("QE_BATCH_MAX_GROUPS", 1024)) return 0;
+    out->range = (U32)span;
+    out->key_min = k->type == CQL_ColumnType_I32 ? ((U32)(S32)k->min_value ^ 0x80000000u) : (U32)k->min_value;
+  }
+  out->acc_words = out->range * (1 + 3 * out->plan.arg_count);
+  return 1;
+}
+
+/* staged columns of a launch: the union of the columns its queries read,
+ * in first-use order; returns the stage index of `column` (adding it) */
+typedef struct QE_BatchStage {
+  U32 count;
+  U32 column[QE_MAX_COLUMNS];
+} QE_BatchStage;
+
+static U32 qe_batch_stage_of(QE_BatchStage* st, U32 column)
+{
+  for (U32 i = 0; i < st->count; i++)
+    if (st->column[i] == column) return i;
+  st->column[st->count] = column;
+  return st->count++;
+}
+
+static void qe_batch_stage_query(QE_BatchStage* st, const QE_BatchPlanned* q)
+{
+  for (U32 l = 0; l < q->leaf_count; l++) qe_batch_stage_of(st, q->leaves[l].column);
+  if (q->group_col >= 0) qe_batch_stage_of(st, (U32)q->group_col);
+  for (U32 a = 0; a < q->plan.arg_count; a++) qe_batch_stage_of(st, q->plan.arg_col[a]);
+}
+
+/* bytes of one stage buffer for tiles of 256 * items rows */
+static U64 qe_batch_stage_bytes(QE_Table* t, const QE_BatchStage* st, U32 items)
+{
+  U64 b = 0;
+  for (U32 i = 0; i < st->count; i++) b += (U64)QE_WG * items * t->columns[st->column[i]].width;
+  return b;
+}
+
+static U64 qe_batch_lds_n(U64 W, U64 stage_bytes, U32 buffers) { return ((W * 8 + 15) & ~15ull) + buffers * stage_bytes; }
+static U64 qe_batch_lds(U64 W, U64 stage_bytes) { return qe_batch_lds_n(W, stage_bytes, 1); }
+
+/* program words of one launch (layout: qk_library.hip, shared scan) */
+static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items, U32* out_words, U32* out_w,
+                             U64* out_stage_bytes)
+{
+  QE_BatchStage st;
+  memset(&st, 0, sizeof(st));
+  U32 W = 0;
+  for (U32 i = 0; i < n; i++) {
+    qs[i].acc_off = W;
+    W += qs[i].acc_words;
+    qe_batch_stage_query(&st, &qs[i]);
+  }
+  U32 rec_total = 0;
+  for (U32 i = 0; i < n; i++) rec_total += QE_BATCH_REC_WORDS + 6 * qs[i].leaf_count;
+  const U32 hdr = 5;
+  U32 words = hdr + n + 3 * st.count + rec_total + W;
+  U32* p = (U32*)calloc(words, 4);
+  if (!p) return 0;
+  U64 stage_bytes = qe_batch_stage_bytes(t, &st, items);
+  p[0] = n;
+  p[1] = W;
+  p[2] = hdr + n + 3 * st.count + rec_total;
+  p[3] = st.count;
+  p[4] = (U32)stage_bytes;
+  U32 off = 0;
+  for (U32 u = 0; u < st.count; u++) {
+    U32 w = t->columns[st.column[u]].width;
+    p[hdr + n + 3 * u] = st.column[u];
+    p[hdr + n + 3 * u + 1] = w;
+    p[hdr + n + 3 * u + 2] = off;
+    off += QE_WG * items * w;
+  }
+  U32 at = hdr + n + 3 * st.count;
+  for (U32 i = 0; i < n; i++) {
+    QE_BatchPlanned* q = &qs[i];
+    p[hdr + i] = at;
+    U32* r = p + at;
+    r[0] = q->leaf_count;
+    r[1] = (q->group_col >= 0 ? 1u : 0u) | (q->plan.need_sum ? 2u : 0u) | (q->plan.need_minmax ? 4u : 0u);
+    r[2] = q->group_col >= 0 ? qe_batch_stage_of(&st, (U32)q->group_col) : 0;
+    r[3] = q->group_col >= 0 ? t->columns[q->group_col].type : 0;
+    r[4] = q->key_min;
+    r[5] = q->range;
+    r[6] = q->plan.arg_count;
+    for (U32 a = 0; a < q->plan.arg_count; a++) {
+      r[7 + 2 * a] = qe_batch_stage_of(&st, q->plan.arg_col[a]);
+      r[8 + 2 * a] = t->columns[q->plan.arg_col[a]].type;
+    }
+    r[11] = q->acc_off;
+    for (U32 l = 0; l < q->leaf_count; l++)
+      qe_simple_leaf_words(&r[QE_BATCH_REC_WORDS + 6 * l], qe_batch_stage_of(&st, q->leaves[l].column),
+                           t->columns[q->leaves[l].column].type, q->leaves[l].op, q->leaves[l].target);
+    at += QE_BATCH_REC_WORDS + 6 * q->leaf_count;
+    /* op of every accumulator word: count add; per arg sum (integer or
+     * f64 add), min, max */
+    U32* ops = p + p[2] + q->acc_off;
+    U32 R = q->range;
+    for (U32 g = 0; g < R; g++) ops[g] = 0;
+    for (U32 a = 0; a < q->plan.arg_count; a++) {
+      U32 sum_op = qe_batch_int32(t->columns[q->plan.arg_col[a]].type) ? 0u : 1u;
+      for (U32 g = 0; g < R; g++) {
+        ops[R + 3 * a * R + g] = sum_op;
+        ops[R + 3 * a * R + R + g] = 2;
+        ops[R + 3 * a * R + 2 * R + g] = 3;
+      }
+    }
+  }
+  *out_words = words;
+  *out_w = W;
+  *out_stage_bytes = stage_bytes;
+  return p;
+}
+
+/* the query's result from its accumulator words */
+static void qe_batch_decode(QE_Table* t, const QE_BatchPlanned* q, const QE_BatchQuery* bq, const U64* acc)
+{
+  const QE_AggPlan* p = &q->plan;
+  const U64* a = acc + q->acc_off;
+  U32 R = q->range;
+  U64 g = 0;
+  for (U32 i = 0; i < R; i++) {
+    U64 cnt = a[i];
+    if (q->group_col >= 0 && !cnt) continue;
+    if (g < bq->cap) {
+      if (bq->out_keys) {
+        F64 kv = 0.0;
+        if (q->group_col >= 0) {
+          U32 code = q->key_min + i;
+          kv = t->columns[q->group_col].type == CQL_ColumnType_I32 ? (F64)(S32)(code ^ 0x80000000u) : (F64)code;
+        }
+        bq->out_keys[g] = kv;
+      }
+      for (U32 e = 0; e < p->expr_count; e++) {
+        S32 ai = p->expr_arg[e];
+        F64 sum = 0.0, mn = 0.0, mx = 0.0;
+        if (ai >= 0 && cnt > 0) {
+          const U64* s = a + R + 3 * (U64)ai * R;
+          if (qe_batch_int32(t->columns[p->arg_col[ai]].type)) sum = (F64)(S64)s[i];
+          else memcpy(&sum, &s[i], 8);
+          mn = qe_batch_f64_from_code(s[R + i]);
+          mx = qe_batch_f64_from_code(s[2 * R + i]);
+        }
+        bq->out_cells[g * p->expr_count + e] = qe_agg_cell(p->funcs[e], cnt, sum, mn, mx);
+      }
+    }
+    g++;
+  }
+  *bq->out_group_count = g;
+}
+
+/* shared_scan instances: block size and rows per thread staged, in order
+ * of preference (more rows per lane per query evaluation first) */
+static const struct {
+  U32 wg, items;
+} qe_batch_shapes[] = {{1024, 1}, {512, 2}, {512, 1}, {256, 4}, {256, 2}, {256, 1}};
+#define QE_BATCH_SHAPES (sizeof(qe_batch_shapes) / sizeof(qe_batch_shapes[0]))
+
+/* one shared-scan launch over queries qs[0..n) */
+static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuery* user, U32 max_items, U32 bpc,
+                           U64 lds_budget, QE_BatchTrace* trace)
+{
+  /* the first instance (block size, rows per thread) whose stage and
+   * accumulators fit one block's LDS; QE_BATCH_WG / QE_BATCH_ITEMS pin
+   * the shape, QE_BATCH_DOUBLE_BUFFER 1 asks for two stage buffers (the
+   * default 2 takes them when they fit) */
+  QE_BatchStage st;
+  memset(&st, 0, sizeof(st));
+  U64 Wq = 0;
+  for (U32 i = 0; i < n; i++) {
+    qe_batch_stage_query(&st, &qs[i]);
+    Wq += qs[i].acc_words;
+  }
+  U64 want_wg = qe_settings_u64("QE_BATCH_WG", 0);
+  U64 dbl_mode = qe_settings_u64("QE_BATCH_DOUBLE_BUFFER", 2);
+  U32 max_lds = gpu_hip_max_lds_bytes();
+  U32 wg = 256, items = 1, buffers = 1;
+  B32 found = 0;
+  for (U32 i = 0; i < QE_BATCH_SHAPES && !found; i++) {
+    if (want_wg && qe_batch_shapes[i].wg != want_wg) continue;
+    if (qe_batch_shapes[i].items > max_items) continue;
+    U64 sb = qe_batch_stage_bytes(t, &st, qe_batch_shapes[i].items * qe_batch_shapes[i].wg / QE_WG);
+    U32 nb = dbl_mode == 1 ? 2 : 1;
+    if (qe_batch_lds_n(Wq, sb, nb) > max_lds) continue;
+    wg = qe_batch_shapes[i].wg;
+    items = qe_batch_shapes[i].items;
+    buffers = (dbl_mode == 1 || (dbl_mode == 2 && qe_batch_lds_n(Wq, sb, 2) <= max_lds)) ? 2 : 1;
+    found = 1;
+  }
+  (void)lds_budget;
+  U32 words = 0, W = 0;
+  U64 stage_bytes = 0;
+  U32* prog = qe_batch_program(t, qs, n, items * wg / QE_WG, &words, &W, &stage_bytes);
+  if (!prog) {
+    qe_set_error("qe_aggregate_batch: out of memory");
+    return 0;
+  }
+  U64 lds = qe_batch_lds_n(W, stage_bytes, buffers);
+  U64 rows = t->row_count;
+  U64 tile = (U64)wg * items;
+  U64 tiles = (rows + tile - 1) / tile;
+  U64 cus = gpu_hip_compute_units();
+  U64 fit = lds ? (U64)max_lds / lds : bpc;
+  U64 wave_fit = 32 / (wg / 64); /* 8 waves per SIMD */
+  if (fit > wave_fit) fit = wave_fit;
+  if (fit < 1) fit = 1;
+  U64 blocks = cus * (bpc < fit ? bpc : fit);
+  if (blocks > tiles) blocks = tiles ? tiles : 1;
+  char kname[32];
+  snprintf(kname, sizeof(kname), "shared_scan_%u_%u", wg, items);
+  GPU_Kernel* k = qe_kernel(kname);
+  GPU_Kernel* kr = qe_kernel("shared_reduce");
+  GPU_Buffer* pb = qe_scratch("batch_program", (U64)words * 4);
+  GPU_Buffer* cb = qe_scratch("batch_columns", (U64)QE_MAX_COLUMNS * 8);
+  GPU_Buffer* part = qe_scratch("batch_partials", blocks * W * 8);
+  GPU_Buffer* res = qe_scratch("batch_result", (U64)W * 8);
+  U64* acc = (U64*)malloc((U64)W * 8 + 8);
+  B32 ok = k && kr && pb && cb && part && res && acc;
+  if (ok) {
+    U64 colptr[QE_MAX_COLUMNS];
+    memset(colptr, 0, sizeof(colptr));
+    for (U32 c = 0; c < t->column_count; c++) colptr[c] = gpu_hip_buffer_device_address(t->columns[c].buffer);
+    gpu_kernel_set_arg_buffer(k, 0, pb);
+    gpu_kernel_set_arg_buffer(k, 1, part);
+    gpu_kernel_set_arg_buffer(k, 2, cb);
+    gpu_kernel_set_arg_u64(k, 0, rows);
+    gpu_kernel_set_arg_u64(k, 5, 0);
+    gpu_kernel_set_arg_u64(k, 6, buffers == 2); /* push slot 6: double-buffered stage */
+    gpu_hip_kernel_set_lds_bytes(k, (U32)lds);
+    gpu_kernel_set_arg_buffer(kr, 0, pb);
+    gpu_kernel_set_arg_buffer(kr, 1, part);
+    gpu_kernel_set_arg_buffer(kr, 3, res);
+    gpu_kernel_set_arg_u64(kr, 0, blocks);
+    U32 rblocks = (W + QE_WG / 64 - 1) / (QE_WG / 64); /* one wave per word */
+    GPU_Batch* b = gpu_batch_begin(0, (U64)W * 8);
+    gpu_batch_buffer_write(b, pb, prog, (U64)words * 4);
+    gpu_batch_buffer_write(b, cb, colptr, (U64)t->column_count * 8);
+    gpu_batch_kernel_execute(b, k, (U32)(blocks * wg), wg);
+    gpu_batch_kernel_execute(b, kr, (rblocks ? rblocks : 1) * QE_WG, QE_WG);
+    gpu_batch_buffer_read(b, res, acc, (U64)W * 8);
+    ok = gpu_batch_end(b);
+    gpu_hip_kernel_set_lds_bytes(k, 0);
+    if (!ok) qe_set_error("qe_aggregate_batch: %s", gpu_hip_last_error());
+  } else if (!k || !kr) {
+    /* qe_kernel set the error */
+  } else {
+    qe_set_error("qe_aggregate_batch: out of memory");
+  }
+  if (ok) {
+    for (U32 i = 0; i < n; i++) {
+      QE_BatchQuery* bq = &user[qs[i].index];
+      qe_batch_decode(t, &qs[i], bq, acc);
+      bq->ok = 1;
+      bq->fused = 1;
+    }
+    if (trace) {
+      U8 used[QE_MAX_COLUMNS];
+      memset(used, 0, sizeof(used));
+      for (U32 i = 0; i < n; i++) {
+        for (U32 l = 0; l < qs[i].leaf_count; l++) used[qs[i].leaves[l].column] = 1;
+        for (U32 a = 0; a < qs[i].plan.arg_count; a++) used[qs[i].plan.arg_col[a]] = 1;
+        if (qs[i].group_col >= 0) used[qs[i].group_col] = 1;
+      }
+      for (U32 c = 0; c < t->column_count; c++)
+        if (used[c]) trace->bytes_read += rows * t->columns[c].width;
+      trace->rows_scanned += rows;
+      trace->launches++;
+      trace->items = items;
+      trace->blocks_per_cu = (U32)(blocks / (cus ? cus : 1));
+    }
+  }
+  free(acc);
+  free(prog);
+  qe_scratch_release(pb);
+  qe_scratch_release(cb);
+  qe_scratch_release(part);
+  qe_scratch_release(res);
+  return ok;
+}
+
+B32 qe_aggregate_batch(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_BatchTrace* trace)
+{
+  U64 t0, p0;
+  qe_trace_begin(0, &t0, &p0);
+  if (!table || (!queries && count)) {
+    qe_set_error("qe_aggregate_batch: null argument");
+    return 0;
+  }
+  if (trace) memset(trace, 0, sizeof(*trace));
+  U32 items = (U32)qe_settings_u64("QE_BATCH_ITEMS", 8);
+  items = items >= 8 ? 8 : (items >= 4 ? 4 : (items >= 2 ? 2 : 1));
+  U32 bpc = (U32)qe_settings_u64("QE_BATCH_BLOCKS_PER_CU", 2);
+  if (bpc < 1) bpc = 1;
+  if (bpc > 16) bpc = 16;
+  /* launches are packed so one block's LDS holds a 1024-row stage of
+   * their columns and their accumulators (QE_BATCH_LDS_BYTES, at most the
+   * LDS of a CU) */
+  U32 max_lds = gpu_hip_max_lds_bytes();
+  U64 lds_budget = qe_settings_u64("QE_BATCH_LDS_BYTES", max_lds);
+  if (lds_budget > max_lds) lds_budget = max_lds;
+  QE_BatchPlanned* planned = (QE_BatchPlanned*)calloc(count ? count : 1, sizeof(QE_BatchPlanned));
+  if (!planned) {
+    qe_set_error("qe_aggregate_batch: out of memory");
+    return 0;
+  }
+  U32 np = 0, fallback = 0;
+  B32 all_ok = 1;
+  for (U32 i = 0; i < count; i++) {
+    queries[i].ok = 0;
+    queries[i].fused = 0;
+    QE_BatchStage one;
+    memset(&one, 0, sizeof(one));
+    if (qe_batch_plan_query(table, &queries[i], &planned[np]) &&
+        (qe_batch_stage_query(&one, &planned[np]),
+         qe_batch_lds(planned[np].acc_words, qe_batch_stage_bytes(table, &one, 1)) <= max_lds)) {
+      planned[np].index = i;
+      np++;
+    } else {
+      /* not shaped for the shared scan: the plain operator, same call */
+      QE_BatchQuery* q = &queries[i];
+      q->ok = qe_aggregate(table, q->where, q->where_count, q->group_col, q->funcs, q->arg_cols, q->expr_count, q->cap,
+                           q->out_group_count, q->out_keys, q->out_cells, 0);
+      if (!q->ok) all_ok = 0;
+      fallback++;
+    }
+  }
+  /* launches by column affinity: a launch takes, one at a time, the
+   * remaining query that adds the fewest bytes to its stage (ties to the
+   * earliest), while its accumulators and the stage buffers of
+   * one-row-per-thread tiles fit the LDS budget; queries sharing columns
+   * thus share launches and each launch reads a small union */
+  {
+    QE_BatchPlanned* ordered = (QE_BatchPlanned*)calloc(np ? np : 1, sizeof(QE_BatchPlanned));
+    U8* taken = (U8*)calloc(np ? np : 1, 1);
+    U32 done = 0;
+    if (!ordered || !taken) {
+      free(ordered);
+      free(taken);
+      free(planned);
+      qe_set_error("qe_aggregate_batch: out of memory");
+      return 0;
+    }
+    while (done < np) {
+      QE_BatchStage st;
+      memset(&st, 0, sizeof(st));
+      U64 words = 0;
+      U32 in_launch = 0;
+      for (;;) {
+        U32 best = np;
+        U64 best_add = ~0ull;
+        U64 base = qe_batch_stage_bytes(table, &st, 4);
+        for (U32 i = 0; i < np; i++) {
+          if (taken[i]) continue;
+          QE_BatchStage grown = st;
+          qe_batch_stage_query(&grown, &planned[i]);
+          U64 add = qe_batch_stage_bytes(table, &grown, 4) - base;
+          if (in_launch && qe_batch_lds(words + planned[i].acc_words, base + add) > lds_budget) continue;
+          if (add < best_add) {
+            best_add = add;
+            best = i;
+          }
+        }
+        if (best == np || in_launch == QE_BATCH_MAX_LAUNCH_QUERIES) break;
+        qe_batch_stage_query(&st, &planned[best]);
+        words += planned[best].acc_words;
+        taken[best] = 1;
+        ordered[done++] = planned[best];
+        in_launch++;
+      }
+      /* mark the launch boundary in the sequence */
+      ordered[done - 1].acc_off = ~0u;
+    }
+    memcpy(planned, ordered, sizeof(QE_BatchPlanned) * np);
+    free(ordered);
+    free(taken);
+  }
+  U32 start = 0;
+  while (start < np) {
+    U32 end = start;
+    while (planned[end].acc_off != ~0u) end++;
+    end++;
+    if (table->row_count == 0) {
+      /* no rows: no groups, or the one all-zero row of an ungrouped
+       * aggregate (query_exec.c:10335-10346) */
+      for (U32 i = start; i < end; i++) {
+        QE_BatchQuery* bq = &queries[planned[i].index];
+        if (planned[i].group_col >= 0) {
+          *bq->out_group_count = 0;
+        } else {
+          U64 acc[1 + 3 * 2];
+          memset(acc, 0, sizeof(acc));
+          planned[i].acc_off = 0;
+          qe_batch_decode(table, &planned[i], bq, acc);
+        }
+        bq->ok = 1;
+        bq->fused = 1;
+      }
+    } else if (!qe_batch_launch(table, planned + start, end - start, queries, items, bpc, lds_budget, trace)) {
+      all_ok = 0;
+      break;
+    }
+    start = end;
+  }
+  free(planned);
+  if (trace) {
+    trace->queries = count;
+    trace->fallback = fallback;
+    trace->fused = count - fallback;
+    QE_Trace tmp;
+    qe_trace_end(&tmp, t0, p0);
+    trace->gpu_kernel_ns = tmp.gpu_kernel_ns;
+    trace->wall_ns = tmp.wall_ns;
+  }
+  return all_ok;
+}

@@ -142,6 +142,18 @@ typedef struct QE_GroupTotals {
 
 U64 qe_agg_cell(U32 func, U64 count, F64 sum, F64 mn, F64 mx);
 
+/* an aggregate's expressions resolved to distinct argument columns */
+#define QE_AGG_MAX_ARGS 4
+typedef struct QE_AggPlan {
+  U32 expr_count;
+  U32 funcs[QE_AGG_MAX_EXPRS];
+  S32 expr_arg[QE_AGG_MAX_EXPRS]; /* index into args, -1 for COUNT(*) */
+  U32 arg_count;
+  U32 arg_col[QE_AGG_MAX_ARGS];
+  B32 need_sum, need_minmax;
+} QE_AggPlan;
+B32 qe_agg_plan(QE_Table* t, const U32* funcs, const S32* arg_cols, U32 expr_count, QE_AggPlan* p);
+
 /* per-group device totals of aggregate_dense / join_probe: count u64[R];
  * sum u64[E][R]; min u32[E][R]; max u32[E][R] */
 typedef struct QE_DenseTotals {

@@ -1642,6 +1642,414 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 }
 
 /* ------------------------------------------------------------------ */
+/* shared scan: many aggregate queries over the same resident columns  */
+/* in one launch (the reference fuses one WHERE with one aggregate per */
+/* kernel, query_kernel.c:1433-1600; here one pass serves a batch).    */
+/* Program (u32 words, binding 0):                                      */
+/*   [0] queries Q  [1] accumulator words W  [2] offset of the op table */
+/*   [3] staged columns U  [4] bytes of one stage buffer                */
+/*   [5 + q] offset of query q's record                                 */
+/*   [5 + Q + 3u] staged column u: table column, width in bytes (4 or   */
+/*   8), byte offset of its rows in a stage buffer (16-byte aligned)    */
+/* query record: [0] leaves L (<= 8), [1] flags (1 grouped, 2 sums,     */
+/*   4 min/max), [2] key (staged index), [3] key type, [4] key_min code,*/
+/*   [5] groups R (1 when ungrouped), [6] args A (<= 2), [7] arg 0      */
+/*   (staged index), [8] arg type 0, [9] arg 1, [10] arg type 1,        */
+/*   [11] accumulator offset, [12..] leaves, 6 words each               */
+/*   (qe_simple_leaf_words layout, slot = staged index).                */
+/* Columns: binding 2 holds the base address of every table column.     */
+/* A block walks tiles of 256 x ITEMS rows.  The rows of every staged   */
+/* column of a tile are copied straight from HBM into an LDS stage      */
+/* buffer (global_load_lds, 16 bytes per lane, no registers in          */
+/* between), the copies of the next tile are issued into the other      */
+/* buffer before the current one is evaluated, and the queries are then */
+/* evaluated on the tile from LDS, wave w taking queries w, w + 4, ...  */
+/* over all the tile's rows, so each column is read from HBM once per   */
+/* launch however many queries read it and each query's record and     */
+/* reductions are paid once per tile and wave.                          */
+/* Accumulators of a query, u64 words from its offset: count[R], then   */
+/* per arg sum[R], min[R], max[R] (sums exact integers for U32/I32 args */
+/* and f64 bits otherwise; min/max order-preserving codes of the f64    */
+/* value).  op table: one u32 per accumulator word (0 integer add,      */
+/* 1 f64 add, 2 min, 3 max).  Each block writes its W words to          */
+/* binding 1 [block][W]; shared_reduce folds them in block order.       */
+/* LDS: accumulators u64[W] (padded to 16 bytes) | stage buffer 0 |     */
+/* stage buffer 1 (when double-buffered, push slot 6).                  */
+/* ------------------------------------------------------------------ */
+#define QK_SS_MAX_ARGS 2
+#define QK_SS_HDR 5
+
+__device__ __forceinline__ bool qk_type_is_wide(qk_u32 t) { return t == QK_T_F64 || t == QK_T_I64 || t == QK_T_U64; }
+
+/* wave64 reductions on DPP: xor-1 and xor-2 quad permutes, then the
+ * half-row and row mirrors leave each 16-lane row's total in all of its
+ * lanes, and four readlanes add the rows.  Every lane of the wave must be
+ * active (callers are in uniform control flow). */
+template <int CTRL>
+__device__ __forceinline__ qk_u64 qk_dpp64(qk_u64 v)
+{
+  const qk_u32 lo = (qk_u32)__builtin_amdgcn_update_dpp(0, (int)(qk_u32)v, CTRL, 0xf, 0xf, false);
+  const qk_u32 hi = (qk_u32)__builtin_amdgcn_update_dpp(0, (int)(qk_u32)(v >> 32), CTRL, 0xf, 0xf, false);
+  return ((qk_u64)hi << 32) | lo;
+}
+
+__device__ __forceinline__ qk_u64 qk_readlane64(qk_u64 v, int lane)
+{
+  const qk_u32 lo = (qk_u32)__builtin_amdgcn_readlane((int)(qk_u32)v, lane);
+  const qk_u32 hi = (qk_u32)__builtin_amdgcn_readlane((int)(qk_u32)(v >> 32), lane);
+  return ((qk_u64)hi << 32) | lo;
+}
+
+#define QK_DPP_XOR1 0xB1
+#define QK_DPP_XOR2 0x4E
+#define QK_DPP_HALF_MIRROR 0x141
+#define QK_DPP_MIRROR 0x140
+
+__device__ __forceinline__ qk_u64 qk_wave_add_u64(qk_u64 v)
+{
+  v += qk_dpp64<QK_DPP_XOR1>(v);
+  v += qk_dpp64<QK_DPP_XOR2>(v);
+  v += qk_dpp64<QK_DPP_HALF_MIRROR>(v);
+  v += qk_dpp64<QK_DPP_MIRROR>(v);
+  return qk_readlane64(v, 0) + qk_readlane64(v, 16) + qk_readlane64(v, 32) + qk_readlane64(v, 48);
+}
+
+__device__ __forceinline__ double qk_wave_add_f64(double d)
+{
+  qk_u64 v = (qk_u64)__double_as_longlong(d);
+  d += __longlong_as_double((qk_s64)qk_dpp64<QK_DPP_XOR1>(v));
+  v = (qk_u64)__double_as_longlong(d);
+  d += __longlong_as_double((qk_s64)qk_dpp64<QK_DPP_XOR2>(v));
+  v = (qk_u64)__double_as_longlong(d);
+  d += __longlong_as_double((qk_s64)qk_dpp64<QK_DPP_HALF_MIRROR>(v));
+  v = (qk_u64)__double_as_longlong(d);
+  d += __longlong_as_double((qk_s64)qk_dpp64<QK_DPP_MIRROR>(v));
+  v = (qk_u64)__double_as_longlong(d);
+  return (__longlong_as_double((qk_s64)qk_readlane64(v, 0)) + __longlong_as_double((qk_s64)qk_readlane64(v, 16))) +
+         (__longlong_as_double((qk_s64)qk_readlane64(v, 32)) + __longlong_as_double((qk_s64)qk_readlane64(v, 48)));
+}
+
+__device__ __forceinline__ void qk_wave_minmax_u64(qk_u64& mn, qk_u64& mx)
+{
+#define QK_MM_STEP(C)                            \
+  {                                              \
+    const qk_u64 a2 = qk_dpp64<C>(mn);           \
+    const qk_u64 b2 = qk_dpp64<C>(mx);           \
+    mn = a2 < mn ? a2 : mn;                      \
+    mx = b2 > mx ? b2 : mx;                      \
+  }
+  QK_MM_STEP(QK_DPP_XOR1)
+  QK_MM_STEP(QK_DPP_XOR2)
+  QK_MM_STEP(QK_DPP_HALF_MIRROR)
+  QK_MM_STEP(QK_DPP_MIRROR)
+#undef QK_MM_STEP
+  qk_u64 n0 = qk_readlane64(mn, 0), n1 = qk_readlane64(mn, 16), n2 = qk_readlane64(mn, 32), n3 = qk_readlane64(mn, 48);
+  qk_u64 x0 = qk_readlane64(mx, 0), x1 = qk_readlane64(mx, 16), x2 = qk_readlane64(mx, 32), x3 = qk_readlane64(mx, 48);
+  n0 = n1 < n0 ? n1 : n0;
+  n2 = n3 < n2 ? n3 : n2;
+  mn = n2 < n0 ? n2 : n0;
+  x0 = x1 > x0 ? x1 : x0;
+  x2 = x3 > x2 ? x3 : x2;
+  mx = x2 > x0 ? x2 : x0;
+}
+
+/* value i of a staged column (its bytes at `p`, type `t`) as f64 */
+__device__ __forceinline__ double qk_ss_f64(const unsigned char* p, qk_u32 t, qk_u32 i)
+{
+  if (qk_type_is_wide(t)) {
+    const qk_u64 x = ((const qk_u64*)p)[i];
+    return t == QK_T_F64 ? __longlong_as_double((qk_s64)x) : (t == QK_T_I64 ? (double)(qk_s64)x : (double)x);
+  }
+  const qk_u32 x = ((const qk_u32*)p)[i];
+  return t == QK_T_U32 ? (double)x : (t == QK_T_I32 ? (double)(qk_s32)x : (double)__uint_as_float(x));
+}
+
+/* the lane's RPL rows (i * 64 + lane) of a staged column as f64, with the
+ * column type dispatched once for all of them */
+template <int RPL, typename T>
+__device__ __forceinline__ void qk_ss_rows_t(const unsigned char* p, qk_u32 lane, double* v)
+{
+  const T* sv = (const T*)p;
+#pragma unroll
+  for (int i = 0; i < RPL; i++) v[i] = (double)sv[i * 64 + lane];
+}
+
+template <int RPL>
+__device__ __forceinline__ void qk_ss_rows(const unsigned char* p, qk_u32 t, qk_u32 lane, double* v)
+{
+  switch (t) {
+    case QK_T_U32: qk_ss_rows_t<RPL, qk_u32>(p, lane, v); break;
+    case QK_T_I32: qk_ss_rows_t<RPL, qk_s32>(p, lane, v); break;
+    case QK_T_F32: qk_ss_rows_t<RPL, float>(p, lane, v); break;
+    case QK_T_F64: qk_ss_rows_t<RPL, double>(p, lane, v); break;
+    case QK_T_I64: qk_ss_rows_t<RPL, qk_s64>(p, lane, v); break;
+    default: qk_ss_rows_t<RPL, qk_u64>(p, lane, v); break;
+  }
+}
+
+/* clear the bits of rows whose value fails `x op tv` (op dispatched once) */
+template <int RPL>
+__device__ __forceinline__ qk_u32 qk_ss_cmp(const double* x, qk_u32 op, double tv, qk_u32 m)
+{
+  switch (op) {
+    case 0:
+#pragma unroll
+      for (int i = 0; i < RPL; i++) m &= (x[i] == tv) ? ~0u : ~(1u << i);
+      break;
+    case 1:
+#pragma unroll
+      for (int i = 0; i < RPL; i++) m &= (x[i] < tv) ? ~0u : ~(1u << i);
+      break;
+    case 2:
+#pragma unroll
+      for (int i = 0; i < RPL; i++) m &= (x[i] <= tv) ? ~0u : ~(1u << i);
+      break;
+    case 3:
+#pragma unroll
+      for (int i = 0; i < RPL; i++) m &= (x[i] > tv) ? ~0u : ~(1u << i);
+      break;
+    default:
+#pragma unroll
+      for (int i = 0; i < RPL; i++) m &= (x[i] >= tv) ? ~0u : ~(1u << i);
+      break;
+  }
+  return m;
+}
+
+/* the program and the column table do not change during a launch: read
+ * through the constant address space, whose uniform loads the compiler
+ * makes scalar loads */
+#define QK_CONST __attribute__((address_space(4)))
+typedef const QK_CONST qk_u32* qk_cu32p;
+typedef const QK_CONST qk_u64* qk_cu64p;
+
+/* issue the copies of tile `t` (rows base..base+tile) of every staged
+ * column into the stage buffer at `buf`: 1 KB per wave instruction, the
+ * chunks dealt round-robin over the block's waves; lanes past the
+ * column's end read its last 16 bytes instead (those rows are masked) */
+template <int WG>
+__device__ __forceinline__ void qk_ss_stage_issue(qk_cu32p st, qk_u32 U, qk_cu64p cols, qk_u64 base,
+                                                  qk_u64 tile, qk_u64 rows_end, unsigned char* buf)
+{
+  /* the wave index is uniform; readfirstlane tells the compiler so (the
+   * LDS destination of global_load_lds must be uniform) */
+  const qk_u32 wave = (qk_u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63u;
+  for (qk_u32 u = 0; u < U; u++) {
+    const qk_u32 c = st[3 * u], width = st[3 * u + 1], off = st[3 * u + 2];
+    const qk_u64 col_bytes = rows_end * width;
+    const qk_u64 last = (col_bytes - 1) & ~15ULL;
+    const qk_u64 start = base * width;
+    const qk_u32 chunks = (qk_u32)((tile * width) >> 10);
+    for (qk_u32 k = wave; k < chunks; k += WG / 64) {
+      const qk_u64 cb = start + ((qk_u64)k << 10);
+      if (cb >= col_bytes) break;
+      qk_u64 src = cb + lane * 16u;
+      src = src <= last ? src : last;
+      __builtin_amdgcn_global_load_lds((const QK_GLOBAL void*)(cols[c] + src),
+                                       (__attribute__((address_space(3))) void*)(buf + off + (k << 10)), 16, 0, 0);
+    }
+  }
+}
+
+template <int WG, int ITEMS>
+__device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
+{
+  constexpr int RPL = WG * ITEMS / 64; /* rows per lane of a tile */
+  static_assert(RPL <= 32, "row masks are 32 bits");
+  extern __shared__ __attribute__((aligned(16))) qk_u64 lds_u64[];
+  const qk_cu32p prog = (qk_cu32p)a.ptr[0];
+  const qk_cu64p cols = (qk_cu64p)a.ptr[2];
+  const qk_u32 Q = prog[0], W = prog[1], U = prog[3], stage_bytes = prog[4];
+  const qk_cu32p ops = prog + prog[2];
+  const qk_cu32p st = prog + QK_SS_HDR + Q;
+  qk_u64* acc = lds_u64;
+  unsigned char* buf0 = (unsigned char*)lds_u64 + (((qk_u64)W * 8 + 15) & ~15ULL);
+  for (qk_u32 i = threadIdx.x; i < W; i += WG) acc[i] = ops[i] == 2u ? ~0ULL : 0ULL;
+  const qk_u64 rows = qk_push(a, QK_P_ROWS);
+  const qk_u64 tile = (qk_u64)WG * ITEMS;
+  const qk_u64 tiles = (rows + tile - 1) / tile;
+  const qk_u64 per = (tiles + gridDim.x - 1) / gridDim.x;
+  const qk_u64 t_begin = per * blockIdx.x, t_end = t_begin + per < tiles ? t_begin + per : tiles;
+  /* push slot 6: 1 = two stage buffers (tile t+1 loads while tile t is
+   * evaluated), 0 = one (a larger tile in the same LDS; other blocks on
+   * the CU cover the load) */
+  const bool dbl = qk_push(a, QK_P_MODE) != 0;
+  if (dbl && t_begin < t_end) qk_ss_stage_issue<WG>(st, U, cols, t_begin * tile, tile, rows, buf0);
+  for (qk_u64 t = t_begin; t < t_end; t++) {
+    if (!dbl) {
+      if (t != t_begin) __syncthreads(); /* every wave is done with tile t-1 */
+      qk_ss_stage_issue<WG>(st, U, cols, t * tile, tile, rows, buf0);
+    }
+    /* tile t has landed in its buffer, and every wave is done with tile
+     * t-1, whose buffer the copies of tile t+1 now take */
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    const unsigned char* sb = buf0 + (dbl ? ((t - t_begin) & 1u) * stage_bytes : 0);
+    if (dbl && t + 1 < t_end)
+      qk_ss_stage_issue<WG>(st, U, cols, (t + 1) * tile, tile, rows, buf0 + ((t + 1 - t_begin) & 1u) * stage_bytes);
+    /* wave w evaluates queries w, w + 4, ... over all rows of the tile:
+     * lane l takes rows l, l + 64, ... (RPL rows) */
+    const qk_u64 base = t * tile;
+    const qk_u32 lane = threadIdx.x & 63u;
+    qk_u32 valid = 0;
+#pragma unroll
+    for (int i = 0; i < RPL; i++) valid |= (base + (qk_u64)i * 64 + lane < rows) ? (1u << i) : 0u;
+    const qk_u32 wave = (qk_u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+#pragma unroll 1
+    for (qk_u32 q = wave; q < Q; q += WG / 64) {
+      const qk_cu32p rec = prog + prog[QK_SS_HDR + q];
+      const qk_u32 L = rec[0] < 8u ? rec[0] : 8u;
+      const qk_u32 flags = rec[1];
+      qk_u32 m = valid;
+#pragma unroll 1
+      for (qk_u32 l = 0; l < L; l++) {
+        const qk_cu32p w = rec + 12 + 6 * l;
+        const qk_u32 u = w[0], tw = w[1], op = w[2], type = tw & 0xffu;
+        const unsigned char* cp = sb + st[3 * u + 2];
+        if (tw & QK_LEAF_NEVER) {
+          m = 0;
+          break;
+        }
+        if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
+          const qk_u32 lo = w[3], span = w[4], bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
+          const qk_u32* sv = (const qk_u32*)cp;
+#pragma unroll
+          for (int i = 0; i < RPL; i++)
+            if (((sv[i * 64 + lane] ^ bias) - lo) > span) m &= ~(1u << i);
+        } else {
+          const double tv = __longlong_as_double((qk_s64)((qk_u64)w[3] | ((qk_u64)w[4] << 32)));
+          double x[RPL];
+          qk_ss_rows<RPL>(cp, type, lane, x);
+          m = qk_ss_cmp<RPL>(x, op, tv, m);
+        }
+      }
+      const qk_u32 R = rec[5], A = rec[6] < QK_SS_MAX_ARGS ? rec[6] : QK_SS_MAX_ARGS;
+      qk_u64* qa = acc + rec[11];
+      if (flags & 1u) {
+        /* grouped: LDS atomics into the query's group table (only this
+         * wave writes it) */
+        if (!m) continue;
+        const qk_u32* kp = (const qk_u32*)(sb + st[3 * rec[2] + 2]);
+        const qk_u32 kt = rec[3], kmin = rec[4];
+        for (qk_u32 mm = m; mm; mm &= mm - 1) {
+          const qk_u32 i = (qk_u32)__builtin_ctz(mm);
+          const qk_u32 kv = kp[i * 64 + lane];
+          const qk_u32 g = (kt == QK_T_I32 ? (kv ^ 0x80000000u) : kv) - kmin;
+          if (g >= R) continue;
+          atomicAdd((unsigned long long*)&qa[g], 1ULL);
+          for (qk_u32 k = 0; k < A; k++) {
+            const qk_u32 at = rec[8 + 2 * k];
+            const double v = qk_ss_f64(sb + st[3 * rec[7 + 2 * k] + 2], at, i * 64 + lane);
+            qk_u64* qs = qa + R + (qk_u64)k * 3 * R;
+            if (flags & 2u) {
+              if (qk_type_is_int32(at)) atomicAdd((unsigned long long*)&qs[g], (unsigned long long)(qk_s64)v);
+              else atomicAdd((double*)&qs[g], v);
+            }
+            if (flags & 4u) {
+              atomicMin((unsigned long long*)&qs[R + g], (unsigned long long)qk_f64_key(v));
+              atomicMax((unsigned long long*)&qs[2 * R + g], (unsigned long long)qk_f64_key(v));
+            }
+          }
+        }
+      } else {
+        /* ungrouped: the lane's rows in registers, then the wave (DPP),
+         * then lane 0 into the query's words (only this wave writes them) */
+        if (!__any((int)(m != 0))) continue;
+        const qk_u64 cnt = qk_wave_add_u64((qk_u64)__popc(m));
+        if (lane == 0) qa[0] += cnt;
+        for (qk_u32 k = 0; k < A; k++) {
+          const qk_u32 at = rec[8 + 2 * k];
+          const unsigned char* ap = sb + st[3 * rec[7 + 2 * k] + 2];
+          qk_u64* qs = qa + 1 + (qk_u64)k * 3;
+          double v[RPL];
+          qk_ss_rows<RPL>(ap, at, lane, v);
+          if (flags & 2u) {
+            if (qk_type_is_int32(at)) {
+              qk_u64 su = 0;
+#pragma unroll
+              for (int i = 0; i < RPL; i++) su += (m & (1u << i)) ? (qk_u64)(qk_s64)v[i] : 0ULL;
+              su = qk_wave_add_u64(su);
+              if (lane == 0) qs[0] += su;
+            } else {
+              double sd = 0.0;
+#pragma unroll
+              for (int i = 0; i < RPL; i++) sd += (m & (1u << i)) ? v[i] : 0.0;
+              sd = qk_wave_add_f64(sd);
+              if (lane == 0) qs[0] = (qk_u64)__double_as_longlong(__longlong_as_double((qk_s64)qs[0]) + sd);
+            }
+          }
+          if (flags & 4u) {
+            qk_u64 mn = ~0ULL, mx = 0ULL;
+#pragma unroll
+            for (int i = 0; i < RPL; i++) {
+              const qk_u64 kk = qk_f64_key(v[i]);
+              const bool in = (m >> i) & 1u;
+              mn = (in && kk < mn) ? kk : mn;
+              mx = (in && kk > mx) ? kk : mx;
+            }
+            qk_wave_minmax_u64(mn, mx);
+            if (lane == 0) {
+              qs[1] = mn < qs[1] ? mn : qs[1];
+              qs[2] = mx > qs[2] ? mx : qs[2];
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  qk_u64* out = qk_wptr<qk_u64>(a, 1) + (qk_u64)blockIdx.x * W;
+  for (qk_u32 i = threadIdx.x; i < W; i += WG) out[i] = acc[i];
+}
+
+/* instances: shared_scan_<block size>_<rows per thread staged> */
+extern "C" __global__ void LB k_shared_scan_256_1(QK_Args a) { qk_shared_scan<256, 1>(a); }
+extern "C" __global__ void LB k_shared_scan_256_2(QK_Args a) { qk_shared_scan<256, 2>(a); }
+extern "C" __global__ void LB k_shared_scan_256_4(QK_Args a) { qk_shared_scan<256, 4>(a); }
+extern "C" __global__ void __launch_bounds__(512) k_shared_scan_512_1(QK_Args a) { qk_shared_scan<512, 1>(a); }
+extern "C" __global__ void __launch_bounds__(512) k_shared_scan_512_2(QK_Args a) { qk_shared_scan<512, 2>(a); }
+extern "C" __global__ void __launch_bounds__(1024) k_shared_scan_1024_1(QK_Args a) { qk_shared_scan<1024, 1>(a); }
+
+/* shared_reduce: word w of the result folds the blocks' words w with the
+ * word's op.  One wave per word: lane l folds blocks l, l + 64, ... in
+ * order, then the wave combines its lanes (DPP tree), so an f64 sum has
+ * one fixed order for a given grid.  binding 0 program, 1 partials
+ * [blocks][W], 3 result u64[W]; push 0 blocks */
+extern "C" __global__ void LB k_shared_reduce(QK_Args a)
+{
+  const qk_cu32p prog = (qk_cu32p)a.ptr[0];
+  const qk_u32 W = prog[1];
+  const qk_cu32p ops = prog + prog[2];
+  const qk_u64 nb = qk_push(a, 0);
+  const qk_u64* part = qk_ptr<qk_u64>(a, 1);
+  qk_u64* res = qk_wptr<qk_u64>(a, 3);
+  const qk_u32 wave = (qk_u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63u;
+  const qk_u32 w = blockIdx.x * (QK_WG / 64) + wave;
+  if (w >= W) return;
+  const qk_u32 op = ops[w];
+  qk_u64 x = op == 2u ? ~0ULL : 0ULL;
+  double d = 0.0;
+  for (qk_u64 b = lane; b < nb; b += 64) {
+    const qk_u64 v = part[b * W + w];
+    if (op == 0u) x += v;
+    else if (op == 1u) d += __longlong_as_double((qk_s64)v);
+    else if (op == 2u) x = v < x ? v : x;
+    else x = v > x ? v : x;
+  }
+  qk_u64 out;
+  if (op == 0u) {
+    out = qk_wave_add_u64(x);
+  } else if (op == 1u) {
+    out = (qk_u64)__double_as_longlong(qk_wave_add_f64(d));
+  } else {
+    qk_u64 mn = x, mx = x;
+    qk_wave_minmax_u64(mn, mx);
+    out = op == 2u ? mn : mx;
+  }
+  if (lane == 0) res[w] = out;
+}
+
+/* ------------------------------------------------------------------ */
 /* generators: splitmix64 addressed by output index                     */
 /* (bench_common.h:30-37): d(k) = mix(seed + (k+1) * 0x9E3779B97F4A7C15) */
 /* ------------------------------------------------------------------ */
@@ -1757,6 +2165,13 @@ static const QK_LibraryKernel g_library[] = {
     {"join_build", (const void*)k_join_build, QK_WG},
     {"join_probe", (const void*)k_join_probe, QK_WG},
     {"aggregate_hash", (const void*)k_aggregate_hash, QK_WG},
+    {"shared_scan_256_1", (const void*)k_shared_scan_256_1, 256},
+    {"shared_scan_256_2", (const void*)k_shared_scan_256_2, 256},
+    {"shared_scan_256_4", (const void*)k_shared_scan_256_4, 256},
+    {"shared_scan_512_1", (const void*)k_shared_scan_512_1, 512},
+    {"shared_scan_512_2", (const void*)k_shared_scan_512_2, 512},
+    {"shared_scan_1024_1", (const void*)k_shared_scan_1024_1, 1024},
+    {"shared_reduce", (const void*)k_shared_reduce, QK_WG},
     {"hash_compact", (const void*)k_hash_compact, QK_WG},
     {"join_part_build", (const void*)k_join_part_build, QK_WG},
     {"join_part_hist", (const void*)k_join_part_hist, QK_WG},

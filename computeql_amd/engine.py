@@ -14,7 +14,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import ir
-from .native import QE_Trace, load
+from .native import QE_BatchQuery, QE_BatchTrace, QE_Trace, load
 
 U32, U64, F32, F64, I32, I64 = 1, 2, 3, 4, 7, 8
 COUNT, SUM, AVG, MIN, MAX = 0, 1, 2, 3, 4
@@ -189,8 +189,8 @@ class Table:
         ncount = len(nodes) if nodes is not None else 0
         if cap is None:
             cap = 1 if group is None else 1 << 17
-        keys = np.zeros(max(cap, 1), dtype=np.float64)
-        cells = np.zeros((max(cap, 1), e), dtype=np.uint64)
+        keys = np.empty(max(cap, 1), dtype=np.float64)
+        cells = np.empty((max(cap, 1), e), dtype=np.uint64)
         g = C.c_uint64(0)
         tr = QE_Trace()
         ok = self._lib.qe_aggregate(self.handle, nptr, ncount, -1 if group is None else int(group), fa, aa, e, cap,
@@ -201,6 +201,68 @@ class Table:
         if n > cap:
             return self.aggregate(funcs, args, where, group, cap=n)
         return AggResult(funcs, keys[:n].copy(), cells[:n].copy(), tr.as_dict())
+
+
+@dataclass
+class AggSpec:
+    """One query of a batch: what Table.aggregate takes."""
+    funcs: list
+    args: list
+    where: object = None
+    group: int | None = None
+    cap: int | None = None
+
+
+def aggregate_batch(table: Table, specs, with_trace: bool = False):
+    """qe_aggregate_batch: many aggregate queries over one table in shared
+    scans.  Returns one AggResult per spec (trace holds 'fused'), and the
+    batch trace when with_trace is set."""
+    lib = load()
+    specs = [s if isinstance(s, AggSpec) else AggSpec(**s) for s in specs]
+    n = len(specs)
+    qs = (QE_BatchQuery * max(n, 1))()
+    keep = []
+    outs = []
+    for i, sp in enumerate(specs):
+        funcs = list(sp.funcs)
+        e = len(funcs)
+        fa = (C.c_uint32 * e)(*funcs)
+        aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in sp.args])
+        nodes = ir.to_nodes(sp.where)
+        cap = sp.cap if sp.cap is not None else (1 if sp.group is None else 1 << 17)
+        keys = np.empty(max(cap, 1), dtype=np.float64)
+        cells = np.empty((max(cap, 1), e), dtype=np.uint64)
+        g = C.c_uint64(0)
+        keep += [fa, aa, nodes, keys, cells, g]
+        q = qs[i]
+        q.where = nodes.ctypes.data if nodes is not None else None
+        q.where_count = len(nodes) if nodes is not None else 0
+        q.group_col = -1 if sp.group is None else int(sp.group)
+        q.funcs = C.addressof(fa)
+        q.arg_cols = C.addressof(aa)
+        q.expr_count = e
+        q.cap = cap
+        q.out_group_count = C.addressof(g)
+        q.out_keys = keys.ctypes.data
+        q.out_cells = cells.ctypes.data
+        outs.append((funcs, keys, cells, g, cap))
+    tr = QE_BatchTrace()
+    ok = lib.qe_aggregate_batch(table.handle, qs, n, C.byref(tr))
+    results = []
+    for i, (funcs, keys, cells, g, cap) in enumerate(outs):
+        if not qs[i].ok:
+            raise _err(f"aggregate_batch query {i}")
+        ng = int(g.value)
+        if ng > cap:
+            sp = specs[i]
+            r = table.aggregate(sp.funcs, sp.args, sp.where, sp.group, cap=ng)
+            r.trace["fused"] = 0
+            results.append(r)
+            continue
+        results.append(AggResult(funcs, keys[:ng].copy(), cells[:ng].copy(), {"fused": int(qs[i].fused)}))
+    if not ok:
+        raise _err("aggregate_batch")
+    return (results, tr.as_dict()) if with_trace else results
 
 
 def aggregate_join(fact: Table, fact_key: int, dim: Table, dim_key: int, funcs, fact_args, dim_where=None,
@@ -214,8 +276,8 @@ def aggregate_join(fact: Table, fact_key: int, dim: Table, dim_key: int, funcs, 
     nodes = ir.to_nodes(dim_where)
     nptr = nodes.ctypes.data if nodes is not None else None
     ncount = len(nodes) if nodes is not None else 0
-    keys = np.zeros(max(cap, 1), dtype=np.float64)
-    cells = np.zeros((max(cap, 1), e), dtype=np.uint64)
+    keys = np.empty(max(cap, 1), dtype=np.float64)
+    cells = np.empty((max(cap, 1), e), dtype=np.uint64)
     g = C.c_uint64(0)
     tr = QE_Trace()
     ok = lib.qe_aggregate_join(fact.handle, fact_key, dim.handle, dim_key, nptr, ncount,

@@ -33,6 +33,31 @@ class QE_Trace(C.Structure):
         return {f: (getattr(self, f).decode() if f == "kernel" else getattr(self, f)) for f, _ in self._fields_}
 
 
+class QE_BatchQuery(C.Structure):
+    _fields_ = [
+        ("where", C.c_void_p),
+        ("where_count", U32),
+        ("group_col", C.c_int32),
+        ("funcs", C.c_void_p),
+        ("arg_cols", C.c_void_p),
+        ("expr_count", U32),
+        ("cap", U64),
+        ("out_group_count", C.c_void_p),
+        ("out_keys", C.c_void_p),
+        ("out_cells", C.c_void_p),
+        ("ok", U32),
+        ("fused", U32),
+    ]
+
+
+class QE_BatchTrace(C.Structure):
+    _fields_ = [(n, U32) for n in ("queries", "fused", "fallback", "launches", "items", "blocks_per_cu")] + [
+        (n, U64) for n in ("rows_scanned", "bytes_read", "gpu_kernel_ns", "wall_ns")]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class GPU_MemoryStats(C.Structure):
     _fields_ = [(n, U64) for n in ("bytes_in_use", "bytes_peak", "alloc_failures", "alloc_retry_successes",
                                     "pool_evictions", "pool_evicted_bytes", "alloc_calls", "unrecovered_alloc_failures")]
@@ -126,6 +151,7 @@ SIGNATURES = [
     ("qe_scan_filter", B32, [P, P, U32, B32, P, U64, P, P]),
     ("qe_aggregate", B32, [P, P, U32, S32, P, P, U32, U64, P, P, P, P]),
     ("qe_aggregate_join", B32, [P, U32, P, U32, P, U32, S32, P, P, U32, U64, P, P, P, P]),
+    ("qe_aggregate_batch", B32, [P, P, U32, P]),
     ("qe_bandwidth_read", B32, [P, U64, U32, P, P]),
 ]
 

@@ -345,7 +345,9 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
  * WHERE, no key or one GROUP BY column, up to 8 aggregates (arg_cols[e] < 0
  * = COUNT(*)).  Writes out_group_count groups: out_keys[g] (key value as
  * F64), out_cells[g*E+e] (U64 bits for COUNT, F64 bits otherwise).  Groups
- * are in ascending key order. */
+ * are in ascending key order on the dense path and in hash-table slot order
+ * on the hash path (trace->path QE_AggPath_Hash: keys outside the dense
+ * range, non-32-bit keys, or a WHERE that is not an AND of comparisons). */
 B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 group_col,
                  const U32* funcs, const S32* arg_cols, U32 expr_count, U64 cap,
                  U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace);
@@ -359,6 +361,46 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
                       const QE_IRNode* dim_where, U32 dim_where_count, S32 dim_group_col,
                       const U32* funcs, const S32* fact_arg_cols, U32 expr_count, U64 cap,
                       U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace);
+
+/* Batched shared scan.  No counterpart in the reference, whose fused
+ * kernels serve one query each (query_kernel.c:1433-1600): many aggregate
+ * queries over the same table run as one pass over the union of their
+ * columns.  Each query is what qe_aggregate takes, and each result is what
+ * qe_aggregate returns for it alone (f64 sums within 1e-12 relative: the
+ * summation order differs).  Queries fuse when the WHERE is an AND of at
+ * most 8 comparisons, the GROUP BY key (if any) is a 32-bit integer column
+ * whose zone-map range is at most QE_BATCH_MAX_GROUPS (1024), and they
+ * take at most 2 distinct argument columns; other queries run through
+ * qe_aggregate inside the same call.  `ok` is set per query. */
+typedef struct QE_BatchQuery {
+  const QE_IRNode* where;
+  U32 where_count;
+  S32 group_col;
+  const U32* funcs;
+  const S32* arg_cols;
+  U32 expr_count;
+  U64 cap;
+  U64* out_group_count;
+  F64* out_keys;
+  U64* out_cells;
+  B32 ok;
+  U32 fused; /* out: 1 when the query ran in a shared scan */
+} QE_BatchQuery;
+
+typedef struct QE_BatchTrace {
+  U32 queries;
+  U32 fused;
+  U32 fallback;
+  U32 launches;     /* shared-scan launches made */
+  U32 items;        /* rows per thread per tile of the shared scan */
+  U32 blocks_per_cu;
+  U64 rows_scanned; /* rows read per launch */
+  U64 bytes_read;   /* column bytes the launches read (union per launch) */
+  U64 gpu_kernel_ns;
+  U64 wall_ns;
+} QE_BatchTrace;
+
+B32 qe_aggregate_batch(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_BatchTrace* trace);
 
 /* bandwidth probe (bandwidth_read.comp): reads `bytes` of a buffer with
  * 16-byte loads; returns a sink value.  For measuring the HBM ceiling. */

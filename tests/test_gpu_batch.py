@@ -1,0 +1,126 @@
+"""GPU parity of the batched shared scan: every query of a 64-query mixed
+batch over the 30-column event table equals the plain operator's result
+for it alone and the CPU oracle's.  Counts, integer sums, min/max and
+keys bit-exact; f64 sums and averages within 1e-12 relative (a batch adds a
+query's values per thread, wave and block, the plain kernels in their own
+order).  Groups are compared sorted by key."""
+import numpy as np
+import pytest
+
+import computeql_amd as cq
+from computeql_amd import workloads as wl
+from computeql_amd.ir import col
+
+pytestmark = pytest.mark.gpu
+
+ROWS = 1_000_003
+RTOL = 1e-12
+
+
+@pytest.fixture(autouse=True)
+def _settings():
+    cq.reset_settings()
+    yield
+    cq.reset_settings()
+
+
+@pytest.fixture(scope="module")
+def events():
+    cols = wl.event_table(ROWS, seed=7)
+    t = wl.upload(cols)
+    yield t, cols
+    t.close()
+
+
+FLOAT_ARGS = {cq.F64, cq.I64}
+
+
+def _is_float(spec, e):
+    a = spec.args[e]
+    return a is not None and wl.EVENT_COLUMNS[a][1] in FLOAT_ARGS and spec.funcs[e] in (cq.SUM, cq.AVG)
+
+
+def _same(spec, got_keys, got_vals, ref_keys, ref_vals, what):
+    og, orf = np.argsort(got_keys, kind="stable"), np.argsort(ref_keys, kind="stable")
+    assert len(got_keys) == len(ref_keys), what
+    assert np.array_equal(got_keys[og], ref_keys[orf]), what
+    for e in range(len(spec.funcs)):
+        g, r = got_vals[og, e], ref_vals[orf, e]
+        if _is_float(spec, e):
+            assert np.allclose(g, r, rtol=RTOL, atol=0.0), (what, e)
+        else:
+            assert np.array_equal(g, r), (what, e)
+
+
+def _vals(res):
+    return np.stack([res.column(e).astype(np.float64) for e in range(len(res.funcs))], axis=1) if res.groups else \
+        np.zeros((0, len(res.funcs)))
+
+
+def test_mixed_batch_matches_plain_and_oracle(events, orc):
+    t, cols = events
+    specs = wl.mixed_queries(64, seed=11)
+    results, tr = cq.aggregate_batch(t, specs, with_trace=True)
+    assert tr["fused"] == 64 and tr["fallback"] == 0 and tr["launches"] >= 1
+    for i, (sp, res) in enumerate(zip(specs, results)):
+        assert res.trace["fused"] == 1
+        plain = t.aggregate(sp.funcs, sp.args, where=sp.where, group=sp.group)
+        _same(sp, res.keys, _vals(res), plain.keys, _vals(plain), f"plain q{i}")
+        if i % 4 == 0:
+            rk, rv = orc.group_aggregate(cols, sp.group, sp.funcs, sp.args, where=sp.where)
+            _same(sp, res.keys, _vals(res), rk, rv, f"oracle q{i}")
+
+
+@pytest.mark.parametrize("items,bpc,lds,wg,dbl", [(1, 1, 81920, 0, 2), (8, 4, 81920, 0, 2), (2, 2, 24576, 0, 2),
+                                                  (4, 2, 0, 256, 1), (2, 1, 0, 512, 0), (1, 2, 0, 1024, 0)])
+def test_batch_configs_agree(events, items, bpc, lds, wg, dbl):
+    t, _ = events
+    specs = wl.mixed_queries(24, seed=5)
+    base = cq.aggregate_batch(t, specs)
+    cq.set_setting("QE_BATCH_ITEMS", items)
+    cq.set_setting("QE_BATCH_BLOCKS_PER_CU", bpc)
+    cq.set_setting("QE_BATCH_DOUBLE_BUFFER", dbl)
+    if lds:
+        cq.set_setting("QE_BATCH_LDS_BYTES", lds)
+    if wg:
+        cq.set_setting("QE_BATCH_WG", wg)
+    got, tr = cq.aggregate_batch(t, specs, with_trace=True)
+    if lds == 24576:
+        assert tr["launches"] > 1
+    for i, (sp, a, b) in enumerate(zip(specs, base, got)):
+        _same(sp, b.keys, _vals(b), a.keys, _vals(a), f"q{i}")
+
+
+def test_alone_equals_in_the_mix(events):
+    t, _ = events
+    specs = wl.mixed_queries(16, seed=9)
+    mix = cq.aggregate_batch(t, specs)
+    for i, sp in enumerate(specs):
+        alone = cq.aggregate_batch(t, [sp])[0]
+        _same(sp, alone.keys, _vals(alone), mix[i].keys, _vals(mix[i]), f"q{i}")
+
+
+def test_unfusable_queries_fall_back(events, orc):
+    t, cols = events
+    specs = [
+        cq.AggSpec([cq.COUNT, cq.SUM], [None, wl.M0], col(wl.STATUS) == 2, wl.USER),  # 1M-range key: hash path
+        cq.AggSpec([cq.COUNT], [None], (col(wl.REGION) == 3) | (col(wl.STATUS) == 1)),  # OR: not an AND of leaves
+        cq.AggSpec([cq.COUNT, cq.SUM], [None, wl.U0], col(wl.REGION) < 10, wl.STATUS),
+    ]
+    res, tr = cq.aggregate_batch(t, specs, with_trace=True)
+    assert tr["fallback"] == 2 and tr["fused"] == 1
+    assert [r.trace["fused"] for r in res] == [0, 0, 1]
+    for i, (sp, r) in enumerate(zip(specs, res)):
+        rk, rv = orc.group_aggregate(cols, sp.group, sp.funcs, sp.args, where=sp.where)
+        _same(sp, r.keys, _vals(r), rk, rv, f"q{i}")
+
+
+def test_empty_table_batch():
+    t = cq.Table(0)
+    t.add_column("a", cq.U32)
+    t.add_column("b", cq.F64)
+    specs = [cq.AggSpec([cq.COUNT, cq.SUM, cq.MIN], [None, 1, 1]), cq.AggSpec([cq.COUNT], [None], col(0) < 5)]
+    res = cq.aggregate_batch(t, specs)
+    assert res[0].groups == 1 and int(res[0].cells[0, 0]) == 0 and res[0].column(1)[0] == 0.0
+    assert res[1].groups == 1 and int(res[1].cells[0, 0]) == 0
+    t.close()
