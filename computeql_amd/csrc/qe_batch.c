@@ -17,7 +17,9 @@
  * combining its lanes (DPP tree).
  */
 #define _GNU_SOURCE
+#include <pthread.h>
 #include <stdlib.h>
+#include <sys/stat.h>
 
 #include "qe_internal.h"
 
@@ -222,9 +224,10 @@ static const struct {
 #define QE_BATCH_SHAPES (sizeof(qe_batch_shapes) / sizeof(qe_batch_shapes[0]))
 
 /* one shared-scan launch over queries qs[0..n) */
-static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuery* user, U32 max_items, U32 bpc,
-                           U64 lds_budget, QE_BatchTrace* trace)
+static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuery* user, const QE_BatchConfig* cfg,
+                           QE_BatchTrace* trace)
 {
+  const U32 max_items = cfg->items, bpc = cfg->blocks_per_cu;
   /* the first instance (block size, rows per thread) whose stage and
    * accumulators fit one block's LDS; QE_BATCH_WG / QE_BATCH_ITEMS pin
    * the shape, QE_BATCH_DOUBLE_BUFFER 1 asks for two stage buffers (the
@@ -236,8 +239,8 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
     qe_batch_stage_query(&st, &qs[i]);
     Wq += qs[i].acc_words;
   }
-  U64 want_wg = qe_settings_u64("QE_BATCH_WG", 0);
-  U64 dbl_mode = qe_settings_u64("QE_BATCH_DOUBLE_BUFFER", 2);
+  U64 want_wg = cfg->wg;
+  U64 dbl_mode = cfg->buffers;
   U32 max_lds = gpu_hip_max_lds_bytes();
   U32 wg = 256, items = 1, buffers = 1;
   B32 found = 0;
@@ -252,7 +255,6 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
     buffers = (dbl_mode == 1 || (dbl_mode == 2 && qe_batch_lds_n(Wq, sb, 2) <= max_lds)) ? 2 : 1;
     found = 1;
   }
-  (void)lds_budget;
   U32 words = 0, W = 0;
   U64 stage_bytes = 0;
   U32* prog = qe_batch_program(t, qs, n, items * wg / QE_WG, &words, &W, &stage_bytes);
@@ -343,7 +345,19 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
   return ok;
 }
 
-B32 qe_aggregate_batch(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_BatchTrace* trace)
+/* the configuration the settings give (QE_BATCH_WG, QE_BATCH_ITEMS,
+ * QE_BATCH_DOUBLE_BUFFER, QE_BATCH_BLOCKS_PER_CU, QE_BATCH_LDS_BYTES) */
+static void qe_batch_config_from_settings(QE_BatchConfig* c)
+{
+  c->wg = (U32)qe_settings_u64("QE_BATCH_WG", 0);
+  c->items = (U32)qe_settings_u64("QE_BATCH_ITEMS", 8);
+  c->buffers = (U32)qe_settings_u64("QE_BATCH_DOUBLE_BUFFER", 2);
+  c->blocks_per_cu = (U32)qe_settings_u64("QE_BATCH_BLOCKS_PER_CU", 2);
+  c->lds_budget = qe_settings_u64("QE_BATCH_LDS_BYTES", 0);
+}
+
+static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 count, const QE_BatchConfig* cfg_in,
+                                  QE_BatchTrace* trace)
 {
   U64 t0, p0;
   qe_trace_begin(0, &t0, &p0);
@@ -352,17 +366,15 @@ B32 qe_aggregate_batch(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_Ba
     return 0;
   }
   if (trace) memset(trace, 0, sizeof(*trace));
-  U32 items = (U32)qe_settings_u64("QE_BATCH_ITEMS", 8);
-  items = items >= 8 ? 8 : (items >= 4 ? 4 : (items >= 2 ? 2 : 1));
-  U32 bpc = (U32)qe_settings_u64("QE_BATCH_BLOCKS_PER_CU", 2);
-  if (bpc < 1) bpc = 1;
-  if (bpc > 16) bpc = 16;
+  QE_BatchConfig cfg = *cfg_in;
+  cfg.items = cfg.items >= 8 ? 8 : (cfg.items >= 4 ? 4 : (cfg.items >= 2 ? 2 : 1));
+  if (cfg.blocks_per_cu < 1) cfg.blocks_per_cu = 1;
+  if (cfg.blocks_per_cu > 16) cfg.blocks_per_cu = 16;
   /* launches are packed so one block's LDS holds a 1024-row stage of
-   * their columns and their accumulators (QE_BATCH_LDS_BYTES, at most the
-   * LDS of a CU) */
+   * their columns and their accumulators (lds_budget, at most the LDS of
+   * a CU; 0 = all of it) */
   U32 max_lds = gpu_hip_max_lds_bytes();
-  U64 lds_budget = qe_settings_u64("QE_BATCH_LDS_BYTES", max_lds);
-  if (lds_budget > max_lds) lds_budget = max_lds;
+  U64 lds_budget = cfg.lds_budget && cfg.lds_budget < max_lds ? cfg.lds_budget : max_lds;
   QE_BatchPlanned* planned = (QE_BatchPlanned*)calloc(count ? count : 1, sizeof(QE_BatchPlanned));
   if (!planned) {
     qe_set_error("qe_aggregate_batch: out of memory");
@@ -460,7 +472,7 @@ B32 qe_aggregate_batch(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_Ba
         bq->ok = 1;
         bq->fused = 1;
       }
-    } else if (!qe_batch_launch(table, planned + start, end - start, queries, items, bpc, lds_budget, trace)) {
+    } else if (!qe_batch_launch(table, planned + start, end - start, queries, &cfg, trace)) {
       all_ok = 0;
       break;
     }
@@ -477,4 +489,172 @@ B32 qe_aggregate_batch(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_Ba
     trace->wall_ns = tmp.wall_ns;
   }
   return all_ok;
+}
+
+/* ---------------- autotuner ---------------- */
+/* A batch's best configuration depends on the table's size, how wide the
+ * union of its columns is and how many queries it has.  The tuner times
+ * the candidate configurations on the caller's batch (a warm run, then a
+ * timed run, wall clock: what a caller waits for), keeps the fastest per
+ * workload key in memory and appends it to a text file
+ * ($CQLHIP_TUNE_FILE, else ~/.cache/cqlhip/batch_tune.txt) that later
+ * processes read back. */
+typedef struct QE_TuneEntry {
+  U32 rows_log2, width, count_log2;
+  QE_BatchConfig cfg;
+} QE_TuneEntry;
+
+#define QE_TUNE_CAP 256
+static QE_TuneEntry g_tune[QE_TUNE_CAP];
+static U32 g_tune_count;
+static B32 g_tune_loaded;
+static pthread_mutex_t g_tune_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static U32 qe_log2_ceil(U64 x)
+{
+  U32 l = 0;
+  while (((U64)1 << l) < x && l < 63) l++;
+  return l;
+}
+
+static void qe_tune_path(char* out, size_t n)
+{
+  const char* env = getenv("CQLHIP_TUNE_FILE");
+  if (env && env[0]) {
+    snprintf(out, n, "%s", env);
+    return;
+  }
+  const char* home = getenv("HOME");
+  snprintf(out, n, "%s/.cache/cqlhip/batch_tune.txt", home ? home : "/tmp");
+}
+
+/* the workload key of a batch: rows, the byte width of the union of the
+ * fusable queries' columns (per row), and the query count */
+static void qe_tune_key(QE_Table* t, QE_BatchQuery* q, U32 count, U32* rows_log2, U32* width, U32* count_log2)
+{
+  QE_BatchStage st;
+  memset(&st, 0, sizeof(st));
+  QE_BatchPlanned p;
+  for (U32 i = 0; i < count; i++)
+    if (qe_batch_plan_query(t, &q[i], &p)) qe_batch_stage_query(&st, &p);
+  U64 w = 0;
+  for (U32 i = 0; i < st.count; i++) w += t->columns[st.column[i]].width;
+  *rows_log2 = qe_log2_ceil(t->row_count ? t->row_count : 1);
+  *width = (U32)((w + 15) / 16);
+  *count_log2 = qe_log2_ceil(count ? count : 1);
+}
+
+static void qe_tune_load_locked(void)
+{
+  if (g_tune_loaded) return;
+  g_tune_loaded = 1;
+  char path[512];
+  qe_tune_path(path, sizeof(path));
+  FILE* f = fopen(path, "r");
+  if (!f) return;
+  QE_TuneEntry e;
+  unsigned long long budget;
+  while (g_tune_count < QE_TUNE_CAP &&
+         fscanf(f, "%u %u %u %u %u %u %u %llu", &e.rows_log2, &e.width, &e.count_log2, &e.cfg.wg, &e.cfg.items,
+                &e.cfg.buffers, &e.cfg.blocks_per_cu, &budget) == 8) {
+    e.cfg.lds_budget = budget;
+    g_tune[g_tune_count++] = e;
+  }
+  fclose(f);
+}
+
+static B32 qe_tune_lookup(U32 rl, U32 w, U32 cl, QE_BatchConfig* out)
+{
+  pthread_mutex_lock(&g_tune_lock);
+  qe_tune_load_locked();
+  B32 found = 0;
+  for (U32 i = g_tune_count; i-- > 0;) {
+    if (g_tune[i].rows_log2 == rl && g_tune[i].width == w && g_tune[i].count_log2 == cl) {
+      *out = g_tune[i].cfg;
+      found = 1;
+      break;
+    }
+  }
+  pthread_mutex_unlock(&g_tune_lock);
+  return found;
+}
+
+static void qe_tune_store(U32 rl, U32 w, U32 cl, const QE_BatchConfig* cfg)
+{
+  pthread_mutex_lock(&g_tune_lock);
+  qe_tune_load_locked();
+  QE_TuneEntry e = {rl, w, cl, *cfg};
+  if (g_tune_count < QE_TUNE_CAP) g_tune[g_tune_count++] = e;
+  char path[512];
+  qe_tune_path(path, sizeof(path));
+  char dir[512];
+  snprintf(dir, sizeof(dir), "%s", path);
+  char* slash = strrchr(dir, '/');
+  if (slash) {
+    /* create the directory chain one level at a time */
+    *slash = 0;
+    for (char* p = dir + 1; *p; p++) {
+      if (*p == '/') {
+        *p = 0;
+        mkdir(dir, 0755);
+        *p = '/';
+      }
+    }
+    mkdir(dir, 0755);
+  }
+  FILE* f = fopen(path, "a");
+  if (f) {
+    fprintf(f, "%u %u %u %u %u %u %u %llu\n", rl, w, cl, cfg->wg, cfg->items, cfg->buffers, cfg->blocks_per_cu,
+            (unsigned long long)cfg->lds_budget);
+    fclose(f);
+  }
+  pthread_mutex_unlock(&g_tune_lock);
+}
+
+B32 qe_aggregate_batch_tune(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_BatchConfig* best, F64* best_ms)
+{
+  static const U32 shapes[][2] = {{1024, 1}, {512, 2}, {512, 1}, {256, 4}, {256, 1}};
+  static const U32 buffers[] = {0, 2};
+  static const U64 budgets[] = {64 * 1024, 0};
+  F64 best_t = 1e300;
+  QE_BatchConfig b;
+  memset(&b, 0, sizeof(b));
+  for (U32 i = 0; i < sizeof(shapes) / sizeof(shapes[0]); i++)
+    for (U32 j = 0; j < 2; j++)
+      for (U32 k = 0; k < 2; k++) {
+        QE_BatchConfig c = {shapes[i][0], shapes[i][1], buffers[j], 2, budgets[k]};
+        if (!qe_aggregate_batch_run(table, queries, count, &c, 0)) return 0;
+        U64 t0 = qe_now_ns();
+        if (!qe_aggregate_batch_run(table, queries, count, &c, 0)) return 0;
+        F64 ms = (F64)(qe_now_ns() - t0) / 1e6;
+        if (ms < best_t) {
+          best_t = ms;
+          b = c;
+        }
+      }
+  U32 rl, w, cl;
+  qe_tune_key(table, queries, count, &rl, &w, &cl);
+  qe_tune_store(rl, w, cl, &b);
+  if (best) *best = b;
+  if (best_ms) *best_ms = best_t;
+  return 1;
+}
+
+B32 qe_aggregate_batch(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_BatchTrace* trace)
+{
+  QE_BatchConfig cfg;
+  qe_batch_config_from_settings(&cfg);
+  if (table && queries && count && qe_settings_u64("QE_BATCH_AUTOTUNE", 0)) {
+    /* the tuned configuration of this workload, tuning it on this batch the
+     * first time */
+    U32 rl, w, cl;
+    qe_tune_key(table, queries, count, &rl, &w, &cl);
+    if (!qe_tune_lookup(rl, w, cl, &cfg) && !qe_aggregate_batch_tune(table, queries, count, &cfg, 0)) return 0;
+  }
+  B32 ok = qe_aggregate_batch_run(table, queries, count, &cfg, trace);
+  if (trace) {
+    trace->tuned_wg = cfg.wg;
+    trace->tuned_buffers = cfg.buffers;
+  }
+  return ok;
 }

@@ -14,7 +14,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import ir
-from .native import QE_BatchQuery, QE_BatchTrace, QE_Trace, load
+from .native import QE_BatchConfig, QE_BatchQuery, QE_BatchTrace, QE_Trace, load
 
 U32, U64, F32, F64, I32, I64 = 1, 2, 3, 4, 7, 8
 COUNT, SUM, AVG, MIN, MAX = 0, 1, 2, 3, 4
@@ -263,6 +263,40 @@ def aggregate_batch(table: Table, specs, with_trace: bool = False):
     if not ok:
         raise _err("aggregate_batch")
     return (results, tr.as_dict()) if with_trace else results
+
+
+def autotune_batch(table: Table, specs) -> dict:
+    """qe_aggregate_batch_tune: times the shared-scan configurations on this
+    batch and records the fastest for its workload; returns it with its
+    time in ms (used by aggregate_batch when QE_BATCH_AUTOTUNE is 1)."""
+    lib = load()
+    specs = [s if isinstance(s, AggSpec) else AggSpec(**s) for s in specs]
+    qs = (QE_BatchQuery * max(len(specs), 1))()
+    keep = []
+    for i, sp in enumerate(specs):
+        funcs = list(sp.funcs)
+        e = len(funcs)
+        fa = (C.c_uint32 * e)(*funcs)
+        aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in sp.args])
+        nodes = ir.to_nodes(sp.where)
+        cap = sp.cap if sp.cap is not None else (1 if sp.group is None else 1 << 17)
+        keys = np.empty(max(cap, 1), dtype=np.float64)
+        cells = np.empty((max(cap, 1), e), dtype=np.uint64)
+        g = C.c_uint64(0)
+        keep += [fa, aa, nodes, keys, cells, g]
+        q = qs[i]
+        q.where = nodes.ctypes.data if nodes is not None else None
+        q.where_count = len(nodes) if nodes is not None else 0
+        q.group_col = -1 if sp.group is None else int(sp.group)
+        q.funcs, q.arg_cols, q.expr_count, q.cap = C.addressof(fa), C.addressof(aa), e, cap
+        q.out_group_count, q.out_keys, q.out_cells = C.addressof(g), keys.ctypes.data, cells.ctypes.data
+    best = QE_BatchConfig()
+    ms = C.c_double(0.0)
+    if not lib.qe_aggregate_batch_tune(table.handle, qs, len(specs), C.byref(best), C.byref(ms)):
+        raise _err("autotune_batch")
+    out = best.as_dict()
+    out["ms"] = ms.value
+    return out
 
 
 def aggregate_join(fact: Table, fact_key: int, dim: Table, dim_key: int, funcs, fact_args, dim_where=None,

@@ -52,7 +52,31 @@ class QE_BatchQuery(C.Structure):
 
 class QE_BatchTrace(C.Structure):
     _fields_ = [(n, U32) for n in ("queries", "fused", "fallback", "launches", "items", "blocks_per_cu")] + [
-        (n, U64) for n in ("rows_scanned", "bytes_read", "gpu_kernel_ns", "wall_ns")]
+        (n, U64) for n in ("rows_scanned", "bytes_read", "gpu_kernel_ns", "wall_ns")] + [
+        (n, U32) for n in ("tuned_wg", "tuned_buffers")]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class QE_BatchConfig(C.Structure):
+    _fields_ = [("wg", U32), ("items", U32), ("buffers", U32), ("blocks_per_cu", U32), ("lds_budget", U64)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class QE_ServerConfig(C.Structure):
+    _fields_ = [("workers", U32), ("max_batch", U32), ("hbm_budget", U64)]
+
+
+class QE_ServerStats(C.Structure):
+    _fields_ = [(n, U64) for n in ("submitted", "completed", "failed", "batches", "batched_queries", "p50_ns",
+                                    "p99_ns", "max_ns", "mean_ns", "elapsed_ns", "inflight_peak_bytes",
+                                    "queue_peak")]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -152,6 +176,14 @@ SIGNATURES = [
     ("qe_aggregate", B32, [P, P, U32, S32, P, P, U32, U64, P, P, P, P]),
     ("qe_aggregate_join", B32, [P, U32, P, U32, P, U32, S32, P, P, U32, U64, P, P, P, P]),
     ("qe_aggregate_batch", B32, [P, P, U32, P]),
+    ("qe_aggregate_batch_tune", B32, [P, P, U32, P, P]),
+    ("qe_server_start", P, [P]),
+    ("qe_server_submit", U64, [P, P, P]),
+    ("qe_server_wait", B32, [P, U64, U64]),
+    ("qe_server_stats", None, [P, P]),
+    ("qe_server_stats_reset", None, [P]),
+    ("qe_server_stop", None, [P]),
+    ("qe_server_drive", B32, [P, P, P, U32, C.c_double, U64, U64, P]),
     ("qe_bandwidth_read", B32, [P, U64, U32, P, P]),
 ]
 

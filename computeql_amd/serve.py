@@ -1,0 +1,133 @@
+"""The served request loop (qe_server_* of include/cqlhip.h) from Python.
+
+    srv = Server(workers=2, max_batch=64)
+    ticket = srv.submit(table, AggSpec([COUNT], [None], col(1) < 5))
+    result = ticket.result()          # AggResult, as Table.aggregate returns
+    stats = srv.drive(table, specs, rate_qps=20000, seconds=2.0)
+    srv.close()
+
+drive() runs the native Poisson load driver (arrivals generated in C, so the
+offered rate is not bounded by Python) and returns the server's stats over
+the run: completed, batches, p50/p99/max latency (ms) and achieved QPS.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import ir
+from .engine import AggResult, AggSpec, QueryError, Table, _err
+from .native import QE_BatchQuery, QE_ServerConfig, QE_ServerStats, load
+
+
+def _query(spec: AggSpec, keep: list, cap: int | None = None):
+    funcs = list(spec.funcs)
+    e = len(funcs)
+    fa = (C.c_uint32 * e)(*funcs)
+    aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in spec.args])
+    nodes = ir.to_nodes(spec.where)
+    if cap is None:
+        cap = spec.cap if spec.cap is not None else (1 if spec.group is None else 1 << 17)
+    q = QE_BatchQuery()
+    q.where = nodes.ctypes.data if nodes is not None else None
+    q.where_count = len(nodes) if nodes is not None else 0
+    q.group_col = -1 if spec.group is None else int(spec.group)
+    q.funcs = C.addressof(fa)
+    q.arg_cols = C.addressof(aa)
+    q.expr_count = e
+    q.cap = cap
+    keep += [fa, aa, nodes]
+    return q, funcs, cap
+
+
+class Ticket:
+    def __init__(self, server: "Server", ticket: int, q, funcs, keys, cells, count, keep):
+        self._server, self._ticket, self._q = server, ticket, q
+        self._funcs, self._keys, self._cells, self._count, self._keep = funcs, keys, cells, count, keep
+        self._done = False
+
+    def result(self, timeout: float = 60.0) -> AggResult:
+        if not self._done:
+            if not self._server._lib.qe_server_wait(self._server.handle, self._ticket, int(timeout * 1e9)):
+                raise TimeoutError("request not finished")
+            self._done = True
+        if not self._q.ok:
+            raise QueryError("served request failed")
+        n = int(self._count.value)
+        if n > len(self._keys):
+            raise QueryError(f"{n} groups exceed the request's capacity {len(self._keys)}")
+        return AggResult(self._funcs, self._keys[:n].copy(), self._cells[:n].copy(), {"fused": int(self._q.fused)})
+
+
+class Server:
+    def __init__(self, workers: int = 2, max_batch: int = 64, hbm_budget: int = 1 << 30):
+        self._lib = load()
+        cfg = QE_ServerConfig(workers, max_batch, hbm_budget)
+        self.handle = self._lib.qe_server_start(C.byref(cfg))
+        if not self.handle:
+            raise _err("qe_server_start")
+
+    def submit(self, table: Table, spec: AggSpec) -> Ticket:
+        keep: list = []
+        q, funcs, cap = _query(spec, keep)
+        keys = np.empty(max(cap, 1), dtype=np.float64)
+        cells = np.empty((max(cap, 1), len(funcs)), dtype=np.uint64)
+        count = C.c_uint64(0)
+        q.out_group_count = C.addressof(count)
+        q.out_keys = keys.ctypes.data
+        q.out_cells = cells.ctypes.data
+        t = self._lib.qe_server_submit(self.handle, table.handle, C.byref(q))
+        if not t:
+            raise _err("qe_server_submit")
+        return Ticket(self, t, q, funcs, keys, cells, count, keep)
+
+    def stats(self) -> dict:
+        st = QE_ServerStats()
+        self._lib.qe_server_stats(self.handle, C.byref(st))
+        return _stats_dict(st)
+
+    def reset_stats(self) -> None:
+        self._lib.qe_server_stats_reset(self.handle)
+
+    def drive(self, table: Table, specs, rate_qps: float, seconds: float, seed: int = 1, cap: int = 1024) -> dict:
+        keep: list = []
+        pool = (QE_BatchQuery * len(specs))()
+        for i, sp in enumerate(specs):
+            q, _, _ = _query(sp, keep, cap=cap)
+            pool[i] = q
+        st = QE_ServerStats()
+        ok = self._lib.qe_server_drive(self.handle, table.handle, pool, len(specs), float(rate_qps),
+                                       int(seconds * 1e9), int(seed), C.byref(st))
+        if not ok:
+            raise _err("qe_server_drive")
+        d = _stats_dict(st)
+        d["offered_qps"] = rate_qps
+        return d
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.qe_server_stop(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stats_dict(st: QE_ServerStats) -> dict:
+    d = st.as_dict()
+    secs = d["elapsed_ns"] / 1e9
+    d["qps"] = d["completed"] / secs if secs > 0 else 0.0
+    for k in ("p50_ns", "p99_ns", "max_ns", "mean_ns"):
+        d[k.replace("_ns", "_ms")] = d.pop(k) / 1e6
+    d["mean_batch"] = d["batched_queries"] / d["batches"] if d["batches"] else 0.0
+    return d

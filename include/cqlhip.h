@@ -398,9 +398,74 @@ typedef struct QE_BatchTrace {
   U64 bytes_read;   /* column bytes the launches read (union per launch) */
   U64 gpu_kernel_ns;
   U64 wall_ns;
+  U32 tuned_wg;      /* configuration used: block size asked for (0 = first that fits) */
+  U32 tuned_buffers; /* stage buffers asked for (0 one, 1 two, 2 two when they fit) */
 } QE_BatchTrace;
 
+/* launch configuration of the shared scan: block size (256/512/1024, 0 =
+ * the largest whose stage fits), rows per thread staged (at most), stage
+ * buffers (0 one, 1 two, 2 two when they fit), blocks per CU, and the LDS
+ * bytes a launch's stage and accumulators may take (0 = a CU's LDS) */
+typedef struct QE_BatchConfig {
+  U32 wg, items, buffers, blocks_per_cu;
+  U64 lds_budget;
+} QE_BatchConfig;
+
+/* With setting QE_BATCH_AUTOTUNE 1 the configuration comes from the
+ * autotuner (tuned on the first batch of each workload: table size, width
+ * of the queries' columns, query count); otherwise from the QE_BATCH_*
+ * settings. */
 B32 qe_aggregate_batch(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_BatchTrace* trace);
+/* Autotuner: times the candidate configurations on this batch, records the
+ * fastest for its workload (in memory and in $CQLHIP_TUNE_FILE, default
+ * ~/.cache/cqlhip/batch_tune.txt) and returns it with its time. */
+B32 qe_aggregate_batch_tune(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_BatchConfig* best,
+                            F64* best_ms);
+
+/* Served request loop.  Replaces, for aggregate queries, the one-query-
+ * at-a-time execution of the reference's Postgres-wire server (pg_server.c:
+ * 200-250, :643-760, serialised by g_query_exec_mutex, application.h:34):
+ * queued requests are batched into shared scans by `workers` threads (one
+ * batch in flight each), up to `max_batch` queries per batch, admitting
+ * requests while the device memory they need stays within `hbm_budget`;
+ * requests arriving while batches run join the next batch.  max_batch 1
+ * runs each request alone through qe_aggregate (the plain path).  A
+ * request's QE_BatchQuery and the arrays it points to stay owned by the
+ * caller and must live until qe_server_wait returns 1 for its ticket;
+ * results are what qe_aggregate_batch returns for them. */
+typedef struct QE_Server QE_Server;
+
+typedef struct QE_ServerConfig {
+  U32 workers;    /* default 2 */
+  U32 max_batch;  /* default 64 */
+  U64 hbm_budget; /* default 1 GiB */
+} QE_ServerConfig;
+
+typedef struct QE_ServerStats {
+  U64 submitted, completed, failed;
+  U64 batches, batched_queries;
+  U64 p50_ns, p99_ns, max_ns, mean_ns; /* submit to completion */
+  U64 elapsed_ns;                      /* first submit to last completion */
+  U64 inflight_peak_bytes, queue_peak;
+} QE_ServerStats;
+
+QE_Server* qe_server_start(const QE_ServerConfig* config);
+/* returns a ticket, 0 on error */
+U64 qe_server_submit(QE_Server* server, QE_Table* table, QE_BatchQuery* query);
+/* 1 when the request is done (query->ok says whether it succeeded; the
+ * ticket is then released), 0 on timeout */
+B32 qe_server_wait(QE_Server* server, U64 ticket, U64 timeout_ns);
+void qe_server_stats(QE_Server* server, QE_ServerStats* out);
+void qe_server_stats_reset(QE_Server* server);
+/* runs what is queued, then stops the workers and frees the server */
+void qe_server_stop(QE_Server* server);
+/* load driver: Poisson arrivals at rate_qps for duration_ns (seeded),
+ * request k a copy of pool[k % pool_count] with output buffers of its own
+ * (pool entries' out_* fields are ignored; cap sizes the buffers);
+ * returns once every request has finished, with the server's stats over
+ * the run (they are reset at its start). */
+B32 qe_server_drive(QE_Server* server, QE_Table* table, const QE_BatchQuery* pool, U32 pool_count, F64 rate_qps,
+                    U64 duration_ns, U64 seed, QE_ServerStats* out);
 
 /* bandwidth probe (bandwidth_read.comp): reads `bytes` of a buffer with
  * 16-byte loads; returns a sink value.  For measuring the HBM ceiling. */

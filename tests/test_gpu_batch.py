@@ -124,3 +124,19 @@ def test_empty_table_batch():
     assert res[0].groups == 1 and int(res[0].cells[0, 0]) == 0 and res[0].column(1)[0] == 0.0
     assert res[1].groups == 1 and int(res[1].cells[0, 0]) == 0
     t.close()
+
+
+def test_autotuner_records_and_applies(events, tmp_path, monkeypatch):
+    t, _ = events
+    monkeypatch.setenv("CQLHIP_TUNE_FILE", str(tmp_path / "tune.txt"))
+    specs = wl.mixed_queries(16, seed=13)
+    best = cq.autotune_batch(t, specs)
+    assert best["wg"] in (256, 512, 1024) and best["ms"] > 0
+    lines = (tmp_path / "tune.txt").read_text().split("\n")
+    assert any(line.split()[3:5] == [str(best["wg"]), str(best["items"])] for line in lines if line)
+    base = cq.aggregate_batch(t, specs)
+    cq.set_setting("QE_BATCH_AUTOTUNE", 1)
+    got, tr = cq.aggregate_batch(t, specs, with_trace=True)
+    assert tr["tuned_wg"] == best["wg"] and tr["tuned_buffers"] == best["buffers"]
+    for i, (sp, a, b) in enumerate(zip(specs, base, got)):
+        _same(sp, b.keys, _vals(b), a.keys, _vals(a), f"q{i}")
