@@ -18,7 +18,16 @@ columns the query reads in full):
 value = algorithmic bytes of q1..q7 / wall time of the timed steps (GB/s),
 inputs resident in HBM before the timed region starts.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
+The same JSON line also carries, measured after the timed steps:
+  batched  a 64-query mixed batch over the 10M-row 30-column event table,
+           one query per launch vs one shared-scan batch (QPS, speedup)
+  served   the request loop under Poisson arrivals at rising rates, plain
+           (one query per launch) and batched: the highest rate whose p99
+           latency stays within 50 ms, and the ratio of the two
+  cpu_baseline  the oracle (C restatement of the CPU operator path) on a
+           bounded sample of the q1-q7 tables
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N] [--no-served] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -202,6 +211,69 @@ def measured_traffic(kernel: str, rows: int):
     return int(k["bytes"])
 
 
+EVENT_ROWS = 10_000_000
+P99_LIMIT_MS = 50.0
+
+
+def batched_and_served(cq, lib, seconds_per_rate: float):
+    """The batched and served forms on the 10M-row 30-column event table
+    (synthetic, seeded): one 64-query mixed batch run plain (one query per
+    launch) and as a shared scan, then the request loop under Poisson load
+    at rising rates, plain (max_batch 1) and batched, each up to the last
+    rate whose p99 stays within 50 ms and whose achieved rate is within 10%
+    of the offered one."""
+    from computeql_amd import workloads as wl
+    from computeql_amd.serve import Server
+
+    log(f"batched/served: building the {EVENT_ROWS}-row 30-column event table")
+    ev = wl.upload(wl.event_table(EVENT_ROWS, seed=7))
+    specs = wl.mixed_queries(64, seed=11)
+    for _ in range(2):
+        for sp in specs:
+            ev.aggregate(sp.funcs, sp.args, where=sp.where, group=sp.group)
+        cq.aggregate_batch(ev, specs)
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for sp in specs:
+            ev.aggregate(sp.funcs, sp.args, where=sp.where, group=sp.group)
+    plain_s = (time.perf_counter() - t0) / reps
+    lib.gpu_hip_profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, tr = cq.aggregate_batch(ev, specs, with_trace=True)
+    batch_s = (time.perf_counter() - t0) / reps
+    scan_ns, _ = profile_stats(lib, "shared_scan*")
+    batched = {"workload": f"{EVENT_ROWS}-row 30-column event table, 64 mixed filter/GROUP BY aggregates",
+               "plain_ms_per_batch": round(plain_s * 1e3, 3), "batched_ms_per_batch": round(batch_s * 1e3, 3),
+               "plain_qps": round(64 / plain_s, 1), "batched_qps": round(64 / batch_s, 1),
+               "speedup": round(plain_s / batch_s, 2), "launches": tr["launches"],
+               "union_bytes_per_batch": tr["bytes_read"],
+               "shared_scan_gbs": round(tr["bytes_read"] / (scan_ns / reps) if scan_ns else 0.0, 1)}
+    served = {"workload": batched["workload"] + ", Poisson arrivals cycling through the 64 queries",
+              "p99_limit_ms": P99_LIMIT_MS, "seconds_per_rate": seconds_per_rate}
+    for name, kw, rates in (("plain", dict(workers=2, max_batch=1), (2000, 4000, 6000, 8000, 10000, 12000, 16000)),
+                            ("batched", dict(workers=3, max_batch=256),
+                             (8000, 16000, 32000, 48000, 64000, 80000, 96000))):
+        points, best = [], None
+        with Server(**kw) as srv:
+            srv.drive(ev, specs, rate_qps=rates[0], seconds=0.3, seed=1)
+            for r in rates:
+                st = srv.drive(ev, specs, rate_qps=r, seconds=seconds_per_rate, seed=r)
+                points.append({"offered": r, "qps": round(st["qps"], 1), "p50_ms": round(st["p50_ms"], 3),
+                               "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 1)})
+                log(f"served {name} offered {r}: {st['qps']:.0f} qps p99 {st['p99_ms']:.2f} ms")
+                if st["p99_ms"] > P99_LIMIT_MS or st["qps"] < 0.9 * r:
+                    break
+                best = points[-1]
+        served[name] = {"config": kw, "rates": points,
+                        "max_qps_at_p99": best["qps"] if best else 0.0, "p99_ms_there": best["p99_ms"] if best else None}
+    p = served["plain"]["max_qps_at_p99"]
+    served["speedup_at_p99"] = round(served["batched"]["max_qps_at_p99"] / p, 2) if p else None
+    ev.close()
+    return batched, served
+
+
 def profile_stats(lib, name: str):
     d = ctypes.c_uint64(0)
     ns = lib.gpu_hip_profile_query(name.encode(), ctypes.byref(d))
@@ -217,6 +289,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-sample-rows", type=int, default=64_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-served", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--served-seconds", type=float, default=1.0)
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, read one column once (the known-size read that scales "
                          "rocprofv3 FETCH_SIZE readings)")
@@ -286,6 +360,13 @@ def main(argv=None) -> int:
         cq.bandwidth_read(t, 2)
         lib.gpu_hip_wait()
 
+    batched = served = None
+    if not a.no_served:
+        t.close()
+        dima.close()
+        dimb.close()
+        batched, served = batched_and_served(cq, lib, a.served_seconds)
+
     cpu = None
     if not a.no_cpu_baseline:
         try:
@@ -320,11 +401,14 @@ def main(argv=None) -> int:
                    "kernels": kernels, "invariants": "ok" if not errs else sorted(set(errs))},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
-    t.close()
-    dima.close()
-    dimb.close()
+    if a.no_served:
+        t.close()
+        dima.close()
+        dimb.close()
     return 0 if not errs else 1
 
 
