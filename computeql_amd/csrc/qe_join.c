@@ -136,20 +136,30 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     return 0;
   }
   const U64 rows = fact->row_count;
-  /* hist and scatter blocks: each resident scatter block keeps up to P
-   * partially written output lines open in L2, so their number is bounded
-   * separately from the scan kernels' */
+  const U32 W = 1 + arg_count;
+  /* rows of 1 or 2 words go through join_part_scatter_wc (whole 128-byte
+   * lines, partial ones carried in LDS) when two of its 512-thread blocks
+   * fit a CU: next u64[P] | 4 u32[P] | lines 16 B[P + 4096 / CH] | carry
+   * 128 B[P] | rows u32[4096 * W], CH = 32 / W */
+  const U64 wc_lds = P * 24 + 16 * (P + 4096 / (32 / (W > 2 ? 1 : W))) + P * 128 + 4096 * 4 * (U64)W;
+  const B32 wc = W <= 2 && rows < (1ull << 32) && qe_settings_u64("QE_JOIN_SCATTER_LINES", 1) &&
+                 2 * (wc_lds + 64) <= gpu_hip_max_lds_bytes();
+  const U32 swg = wc ? 512 : QE_WG;
+  /* hist and scatter blocks (the same block ranges): the scatter's blocks
+   * per CU are bounded by its LDS, separately from the scan kernels' */
   U32 B = qe_stream_blocks((rows + 3) / 4);
   {
-    U64 cap = gpu_hip_compute_units() * qe_settings_u64("QE_JOIN_SCATTER_BLOCKS_PER_CU", 4);
+    U64 cap = gpu_hip_compute_units() * (wc ? qe_settings_u64("QE_JOIN_SCATTER_WC_BLOCKS_PER_CU", 2)
+                                            : qe_settings_u64("QE_JOIN_SCATTER_BLOCKS_PER_CU", 4));
     if (cap >= 1 && B > cap) B = (U32)cap;
   }
-  const U32 W = 1 + arg_count;
   GPU_Kernel* kb = qe_kernel("join_part_build");
-  GPU_Kernel* kh = qe_kernel("join_part_hist");
+  /* the histogram runs with the scatter's block count and block size */
+  GPU_Kernel* kh = qe_kernel(wc ? "join_part_hist_512" : "join_part_hist");
   GPU_Kernel* ko = qe_kernel("join_part_offsets");
   char sname[40] = "join_part_scatter";
-  if (arg_count <= 2 && qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1))
+  if (wc) snprintf(sname, sizeof(sname), "join_part_scatter_wc_%u", arg_count);
+  else if (arg_count <= 2 && qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1))
     snprintf(sname, sizeof(sname), "join_part_scatter_%u", arg_count);
   GPU_Kernel* ks = qe_kernel(sname);
   /* a probe instance specialised for this shape when one exists */
@@ -169,7 +179,8 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   GPU_Buffer* hist = qe_scratch("jp_hist", (U64)B * P * 4);
   GPU_Buffer* off = qe_scratch("jp_off", (U64)B * P * 4);
   GPU_Buffer* totals = qe_scratch("jp_totals", P * 4);
-  GPU_Buffer* rowsbuf = qe_scratch("jp_rows", rows * W * 4);
+  /* one row past the partitions' rows takes the scatter's discarded stores */
+  GPU_Buffer* rowsbuf = qe_scratch("jp_rows", (rows + 1) * W * 4);
   B32 ok = kb && kh && ko && ks && kp && table && flags && hist && off && totals && rowsbuf;
   U64 cus = gpu_hip_compute_units();
   U32 split = (U32)((cus * qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU", 6) + P - 1) / P);
@@ -179,6 +190,13 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   U64 probe_lds = (U64)S * 4 + (U64)S * (group_mode ? 2 : 4) + acc_lds;
   if (ok && probe_lds > gpu_hip_max_lds_bytes()) {
     qe_set_error("qe_aggregate_join: partition probe needs %llu bytes of LDS", (unsigned long long)probe_lds);
+    ok = 0;
+  }
+  /* scatter: next, dst u64[P] | cnt, start u32[P] | rows u32[4096 * W], plus
+   * its 16 static bytes */
+  const U64 scatter_lds = wc ? wc_lds : P * 24 + 4096 * 4 * (U64)W;
+  if (ok && scatter_lds + 64 > gpu_hip_max_lds_bytes()) {
+    qe_set_error("qe_aggregate_join: partition scatter needs %llu bytes of LDS", (unsigned long long)scatter_lds);
     ok = 0;
   }
   if (ok) {
@@ -229,8 +247,7 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     gpu_kernel_set_arg_buffer(ks, 2, rowsbuf);
     gpu_kernel_set_arg_buffer(ks, 3, off);
     gpu_kernel_set_arg_buffer(ks, 4, totals);
-    /* base u64[P] | cursor, cnt, start u32[P] | part u16[4096] | rows u32[4096 * W] */
-    gpu_hip_kernel_set_lds_bytes(ks, (U32)(P * 20 + 4096 * 2 + 4096 * 4 * (U64)W));
+    gpu_hip_kernel_set_lds_bytes(ks, (U32)scatter_lds);
     U32 kflags = (need_sum ? 1u : 0u) | (need_minmax ? 2u : 0u) | (group_mode ? 0u : 4u);
     gpu_kernel_set_arg_buffer(kp, 1, acc);
     gpu_kernel_set_arg_buffer(kp, 2, table);
@@ -257,9 +274,9 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
       gpu_batch_buffer_zero(b, acc, zero_bytes);
     }
     gpu_batch_kernel_execute(b, kb, (U32)build_threads, QE_WG);
-    gpu_batch_kernel_execute(b, kh, B * QE_WG, QE_WG);
+    gpu_batch_kernel_execute(b, kh, B * swg, swg);
     gpu_batch_kernel_execute(b, ko, (U32)(P * QE_WG), QE_WG);
-    gpu_batch_kernel_execute(b, ks, B * QE_WG, QE_WG);
+    gpu_batch_kernel_execute(b, ks, B * swg, swg);
     gpu_batch_kernel_execute(b, kp, (U32)(P * split * QE_WG), QE_WG);
     gpu_batch_buffer_read(b, flags, fl, 8);
     gpu_batch_buffer_read(b, acc, host_acc, acc_bytes);

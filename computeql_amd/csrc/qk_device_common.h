@@ -327,10 +327,10 @@ __device__ __forceinline__ qk_u64 qk_block_sum_u64(qk_u64 v, qk_u64* lds4)
   return t;
 }
 
-/* block exclusive scan of a u32 per thread; returns the exclusive prefix and
- * the block total */
-template <bool LDS_ONLY = false>
-__device__ __forceinline__ qk_u32 qk_block_excl_scan_u32(qk_u32 v, qk_u32* lds4, qk_u32& total)
+/* block exclusive scan of a u32 per thread over a WG-thread block; returns
+ * the exclusive prefix and the block total; lds holds WG / 64 words */
+template <int WG, bool LDS_ONLY>
+__device__ __forceinline__ qk_u32 qk_block_excl_scan_wg(qk_u32 v, qk_u32* lds, qk_u32& total)
 {
   qk_u32 incl = qk_wave_incl_scan_u32(v);
   qk_u32 wave = threadIdx.x >> 6;
@@ -338,18 +338,25 @@ __device__ __forceinline__ qk_u32 qk_block_excl_scan_u32(qk_u32 v, qk_u32* lds4,
    * whose global stores need no ordering against the scan */
   if (LDS_ONLY) qk_lds_barrier();
   else __syncthreads();
-  if (__lane_id() == 63) lds4[wave] = incl;
+  if (__lane_id() == 63) lds[wave] = incl;
   if (LDS_ONLY) qk_lds_barrier();
   else __syncthreads();
   qk_u32 before = 0, all = 0;
 #pragma unroll
-  for (int w = 0; w < QK_WG / 64; w++) {
-    qk_u32 x = lds4[w];
+  for (int w = 0; w < WG / 64; w++) {
+    qk_u32 x = lds[w];
     before += ((qk_u32)w < wave) ? x : 0u;
     all += x;
   }
   total = all;
   return before + incl - v;
+}
+
+/* the same over a QK_WG-thread block; lds4 holds QK_WG / 64 words */
+template <bool LDS_ONLY = false>
+__device__ __forceinline__ qk_u32 qk_block_excl_scan_u32(qk_u32 v, qk_u32* lds4, qk_u32& total)
+{
+  return qk_block_excl_scan_wg<QK_WG, LDS_ONLY>(v, lds4, total);
 }
 
 /* ---- the scan driver: count / per-block counts / ordered row ids ----

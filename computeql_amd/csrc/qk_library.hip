@@ -860,9 +860,12 @@ extern "C" __global__ void LB k_join_part_build(QK_Args a)
 
 /* join_part_hist: binding 6 fact key column, 2 histogram out (u32
  * [blocks][P]); push 0 rows, 5 first row; words PBITS.  The block ranges
- * are those of qk_block_range, shared with join_part_scatter. */
-extern "C" __global__ void LB k_join_part_hist(QK_Args a)
+ * are those of qk_block_range, shared with join_part_scatter; the WG-thread
+ * form walks them in steps of WG x U quads (1024 quads for both forms). */
+template <int WG, int U>
+__device__ __forceinline__ void qk_part_hist(const QK_Args& a)
 {
+  static_assert(WG * U == QK_WG * QK_UNROLL, "block ranges step in 1024 quads");
   extern __shared__ qk_u64 lds_u64[];
   qk_u32* hist = (qk_u32*)lds_u64;
   const qk_u64 rows = qk_push(a, QK_P_ROWS);
@@ -870,20 +873,25 @@ extern "C" __global__ void LB k_join_part_hist(QK_Args a)
   const qk_u64 r1 = r0 + rows;
   const qk_u32 pbits = a.words[QK_W_PBITS];
   const qk_u32 P = 1u << pbits;
-  for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) hist[i] = 0;
+  for (qk_u32 i = threadIdx.x; i < P; i += WG) hist[i] = 0;
   __syncthreads();
   const qk_u32x4* keys = qk_ptr<qk_u32x4>(a, QK_B_COL0);
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
-  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
-    qk_u64 q[QK_UNROLL];
-    qk_u32 m[QK_UNROLL];
-    qk_u32x4 kv[QK_UNROLL];
-    qk_step_quads(a, s, b1, r0, r1, q, m);
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)WG * U) {
+    qk_u32 m[U];
+    qk_u32x4 kv[U];
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
+    for (int j = 0; j < U; j++) {
+      const qk_u64 qq = s + (qk_u64)j * WG + threadIdx.x;
+      const bool in = qq < b1;
+      qk_u32 v = in ? qk_quad_valid(qq, r0, r1) : 0u;
+      if (v && qk_quad_skipped(a, qq)) v = 0u;
+      m[j] = v;
+      kv[j] = keys[in ? qq : b1 - 1];
+    }
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
+    for (int j = 0; j < U; j++) {
       qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
 #pragma unroll
       for (int k = 0; k < 4; k++)
@@ -892,8 +900,11 @@ extern "C" __global__ void LB k_join_part_hist(QK_Args a)
   }
   __syncthreads();
   qk_u32* out = qk_wptr<qk_u32>(a, 2) + (qk_u64)blockIdx.x * P;
-  for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) out[i] = hist[i];
+  for (qk_u32 i = threadIdx.x; i < P; i += WG) out[i] = hist[i];
 }
+
+extern "C" __global__ void LB k_join_part_hist(QK_Args a) { qk_part_hist<QK_WG, QK_UNROLL>(a); }
+extern "C" __global__ void __launch_bounds__(512) k_join_part_hist_512(QK_Args a) { qk_part_hist<512, 2>(a); }
 
 /* join_part_offsets: one block per partition p: exclusive scan over the
  * blocks of hist[b][p] -> off[b][p]; total[p].  binding 2 hist, 3 off,
@@ -938,59 +949,75 @@ __device__ __forceinline__ qk_u64 qk_part_bases(const qk_u32* totals, qk_u32 P, 
 
 /* join_part_scatter: binding 6 fact key, 7.. arg columns (slots in
  * QK_W_ARGS as join_probe), 3 off, 4 totals, 2 out rows (u32, ROWW words
- * per row: fmix32(key) then args); push 0 rows, 5 first row; words PBITS, ROWW.
- * Each step's 4096 rows are first sorted by partition in LDS (counting
- * sort), then written out in partition runs so consecutive threads store
- * consecutive rows of one partition.
- * LDS: base u64[P] | cursor u32[P] | cnt u32[P] | start u32[P] |
- *      part u16[4096] | buf u32[4096 * ROWW] */
+ * per row: fmix32(key) then args; the buffer's last row is a discard row
+ * that no partition owns); push 0 rows, 5 first row; words PBITS, ROWW.
+ * Each step's 4096 rows are sorted by partition in LDS (counting sort),
+ * then written out in partition runs so consecutive threads store
+ * consecutive rows of one partition.  A sorted row's partition is the top
+ * bits of its first word, so no partition array is kept.
+ * LDS: next u64[P] (the block's next output row per partition) |
+ *      dst u64[P] (output row of sorted position 0 of the partition) |
+ *      cnt u32[P] | start u32[P] | buf u32[4096 * ROWW] */
 #define QK_SCATTER_STEP (QK_WG * QK_UNROLL * 4)
+template <int W>
+struct QkRowW {
+  qk_u32 w[W];
+};
+
 template <int NA>
 __device__ __forceinline__ void qk_part_scatter(const QK_Args& a)
 {
   extern __shared__ qk_u64 lds_u64[];
   __shared__ qk_u32 s_scan[QK_WG / 64];
+  constexpr bool FIXED = NA >= 0 && NA <= 2;
   const qk_u64 rows = qk_push(a, QK_P_ROWS);
   const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
   const qk_u64 r1 = r0 + rows;
   const qk_u32 pbits = a.words[QK_W_PBITS];
   const qk_u32 P = 1u << pbits;
-  const qk_u32 nargs = NA >= 0 ? (qk_u32)NA : (a.words[QK_W_ARGS] & 0xffu);
+  const qk_u32 nargs = FIXED ? (qk_u32)NA : (a.words[QK_W_ARGS] & 0xffu);
   const qk_u32 W = 1u + nargs;
   const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
-  qk_u64* base = lds_u64;
-  qk_u32* cursor = (qk_u32*)(base + P);
-  qk_u32* cnt = cursor + P;
+  qk_u64* next = lds_u64;
+  qk_u64* dst = next + P;
+  qk_u32* cnt = (qk_u32*)(dst + P);
   qk_u32* start = cnt + P;
-  unsigned short* rpart = (unsigned short*)(start + P);
-  qk_u32* buf = (qk_u32*)(rpart + QK_SCATTER_STEP);
-  qk_part_bases(qk_ptr<qk_u32>(a, 4), P, base, s_scan);
-  const qk_u32* off = qk_ptr<qk_u32>(a, 3) + (qk_u64)blockIdx.x * P;
-  for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cursor[i] = off[i];
+  qk_u32* buf = start + P;
+  qk_part_bases(qk_ptr<qk_u32>(a, 4), P, next, s_scan);
+  {
+    const qk_u32* off = qk_ptr<qk_u32>(a, 3) + (qk_u64)blockIdx.x * P;
+    for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) {
+      next[i] += off[i];
+      cnt[i] = 0;
+    }
+  }
   qk_u32* out = qk_wptr<qk_u32>(a, 2);
-  const qk_u64 cap = a.bytes[2] / (4ULL * W);
+  /* rows [0, cap) belong to the partitions; row cap takes discarded stores */
+  const qk_u64 nrows = a.bytes[2] / (4ULL * W);
+  const qk_u64 cap = nrows ? nrows - 1 : 0;
+  const qk_u32 pshift = 32 - pbits;
   const qk_u32x4* keys = qk_ptr<qk_u32x4>(a, QK_B_COL0);
   const qk_u32x4* col0 = nargs ? qk_ptr<qk_u32x4>(a, QK_B_COL0 + (slots & 0xfu)) : keys;
   qk_u64 b0, b1;
   qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
   const qk_u64 qstep = (qk_u64)QK_WG * QK_UNROLL;
-  /* the quads of step s+1 (keys and the first arg column) load while step
-   * s sorts and writes; the loop's barriers order LDS only, so the stores
-   * of one step stay in flight through the next */
+  /* the keys of step s+1 load while step s sorts and writes, and the first
+   * arg column of step s loads at its start (it is first needed after the
+   * ranking and the scan).  The loads are unconditional (past the range
+   * they reread the last quad, masked off), and with the write-out's store
+   * count fixed the wait for them does not also wait for this step's
+   * stores.  The loop's barriers order LDS only. */
   qk_u64 nq[QK_UNROLL];
   qk_u32 nm[QK_UNROLL];
-  qk_u32x4 nkv[QK_UNROLL], nx0[QK_UNROLL];
+  qk_u32x4 nkv[QK_UNROLL];
   auto load_step = [&](qk_u64 s) {
     qk_step_quads(a, s, b1, r0, r1, nq, nm);
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
-      nkv[j] = keys[nq[j]];
-      if (nargs) nx0[j] = col0[nq[j]];
-    }
+    for (int j = 0; j < QK_UNROLL; j++) nkv[j] = keys[nq[j]];
   };
   if (b0 < b1) load_step(b0);
-  for (qk_u64 s = b0; s < b1; s += qstep) {
-    for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cnt[i] = 0;
+  qk_lds_barrier();
+  auto step = [&](qk_u64 s) {
     qk_u64 q[QK_UNROLL];
     qk_u32 m[QK_UNROLL];
     qk_u32x4 kv[QK_UNROLL], x0[QK_UNROLL];
@@ -999,10 +1026,9 @@ __device__ __forceinline__ void qk_part_scatter(const QK_Args& a)
       q[j] = nq[j];
       m[j] = nm[j];
       kv[j] = nkv[j];
-      x0[j] = nx0[j];
+      if (nargs) x0[j] = col0[q[j]];
     }
-    if (s + qstep < b1) load_step(s + qstep);
-    qk_lds_barrier();
+    load_step(s + qstep < b1 ? s + qstep : s);
     /* a row is stored with its key's hash h = fmix32(key) in place of the
      * key (a bijection, so nothing is lost): the probe reads the partition,
      * slot and key word straight from it */
@@ -1013,76 +1039,328 @@ __device__ __forceinline__ void qk_part_scatter(const QK_Args& a)
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         hv[j][k] = qk_fmix32(w[k]);
-        rank[j][k] = (m[j] & (1u << k)) ? atomicAdd(&cnt[pbits ? hv[j][k] >> (32 - pbits) : 0u], 1u) : 0u;
+        rank[j][k] = (m[j] & (1u << k)) ? atomicAdd(&cnt[pbits ? hv[j][k] >> pshift : 0u], 1u) : 0u;
       }
     }
     qk_lds_barrier();
-    /* start[p] = exclusive scan of cnt over the partitions */
+    /* start[p] = exclusive scan of cnt; dst[p] = output row of sorted
+     * position 0 of partition p (u64 arithmetic modulo 2^64: dst[p] + r
+     * is the row of sorted position r >= start[p]); cnt reset for the next
+     * step (its atomics follow a barrier that this scan's readers pass) */
     qk_u32 running = 0;
     for (qk_u32 c = 0; c < P; c += QK_WG) {
       qk_u32 i = c + threadIdx.x;
       qk_u32 v = i < P ? cnt[i] : 0u;
       qk_u32 total = 0;
       qk_u32 pre = qk_block_excl_scan_u32<true>(v, s_scan, total);
-      if (i < P) start[i] = running + pre;
+      if (i < P) {
+        const qk_u32 st = running + pre;
+        start[i] = st;
+        dst[i] = next[i] - (qk_u64)st;
+        next[i] += v;
+        cnt[i] = 0;
+      }
       running += total;
     }
     const qk_u32 step_rows = running;
     qk_lds_barrier();
-    /* place keys, then each arg column, into partition order */
+    /* place rows into partition order: every start[] read of the step is
+     * issued before the first write */
+    qk_u32 idx[QK_UNROLL][4];
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
-      qk_u32 x[4] = {x0[j].x, x0[j].y, x0[j].z, x0[j].w};
+    for (int j = 0; j < QK_UNROLL; j++)
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        if (!(m[j] & (1u << k))) continue;
-        qk_u32 p = pbits ? hv[j][k] >> (32 - pbits) : 0u;
-        qk_u32 idx = start[p] + rank[j][k];
-        rpart[idx] = (unsigned short)p;
-        buf[idx * W] = hv[j][k];
-        if (nargs) buf[idx * W + 1] = x[k];
-      }
-    }
-    for (qk_u32 c = 1; c < nargs; c++) {
-      const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + ((slots >> (4 * c)) & 0xfu));
-      qk_u32x4 x[QK_UNROLL];
-#pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+      for (int k = 0; k < 4; k++) idx[j][k] = start[pbits ? hv[j][k] >> pshift : 0u] + rank[j][k];
+    if (FIXED && W <= 2) {
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) {
-        qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
+        qk_u32 x[4] = {x0[j].x, x0[j].y, x0[j].z, x0[j].w};
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-          if (m[j] & (1u << k)) buf[(start[pbits ? hv[j][k] >> (32 - pbits) : 0u] + rank[j][k]) * W + 1 + c] = w[k];
+        for (int k = 0; k < 4; k++) {
+          if (!(m[j] & (1u << k))) continue;
+          if (W == 2) ((uint2*)buf)[idx[j][k]] = make_uint2(hv[j][k], x[k]);
+          else buf[idx[j][k]] = hv[j][k];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) {
+        qk_u32 x[4] = {x0[j].x, x0[j].y, x0[j].z, x0[j].w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (!(m[j] & (1u << k))) continue;
+          buf[idx[j][k] * W] = hv[j][k];
+          buf[idx[j][k] * W + 1] = x[k];
+        }
+      }
+      for (qk_u32 c = 1; c < nargs; c++) {
+        const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + ((slots >> (4 * c)) & 0xfu));
+        qk_u32x4 x[QK_UNROLL];
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) {
+          qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            if (m[j] & (1u << k)) buf[idx[j][k] * W + 1 + c] = w[k];
+        }
       }
     }
     qk_lds_barrier();
-    /* write out: consecutive threads take consecutive rows of the sorted
-     * step, so a partition's run goes out as contiguous stores */
-    if (W == 2) {
-      for (qk_u32 r = threadIdx.x; r < step_rows; r += QK_WG) {
-        qk_u32 p = rpart[r];
-        qk_u64 dst = base[p] + cursor[p] + (r - start[p]);
-        if (dst < cap) ((uint2*)out)[dst] = make_uint2(buf[2 * r], buf[2 * r + 1]);
+    /* write out: consecutive threads take consecutive sorted rows, so a
+     * partition's run goes out as contiguous stores.  Fixed shapes issue
+     * every read of the step, then every store (a fixed count of stores,
+     * positions past the step's rows going to the discard row). */
+    if (FIXED) {
+      constexpr int WF = FIXED ? NA + 1 : 1;
+      constexpr int PER = QK_SCATTER_STEP / QK_WG;
+      const QkRowW<WF>* rb = (const QkRowW<WF>*)buf;
+      QkRowW<WF>* ro = (QkRowW<WF>*)out;
+      constexpr int HALF = PER / 2;
+#pragma unroll
+      for (int h = 0; h < PER; h += HALF) {
+        QkRowW<WF> row[HALF];
+        qk_u64 d[HALF];
+#pragma unroll
+        for (int i = 0; i < HALF; i++) row[i] = rb[(h + i) * QK_WG + threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < HALF; i++) {
+          qk_u32 r = (qk_u32)(h + i) * QK_WG + threadIdx.x;
+          /* opaque to the optimiser: otherwise the 16 loop-invariant
+           * (out + r) address bases are hoisted out of the step loop and
+           * held in (spilled) registers */
+          asm volatile("" : "+v"(r));
+          d[i] = dst[pbits ? row[i].w[0] >> pshift : 0u] + r;
+          if (r >= step_rows || d[i] >= cap) d[i] = cap;
+        }
+#pragma unroll
+        for (int i = 0; i < HALF; i++) ro[d[i]] = row[i];
       }
     } else {
       for (qk_u32 i = threadIdx.x; i < step_rows * W; i += QK_WG) {
         qk_u32 r = i / W, wd = i - r * W;
-        qk_u32 p = rpart[r];
-        qk_u64 dst = base[p] + cursor[p] + (r - start[p]);
-        if (dst < cap) out[dst * W + wd] = buf[i];
+        qk_u64 dr = dst[pbits ? buf[r * W] >> pshift : 0u] + r;
+        if (dr < cap) out[dr * W + wd] = buf[i];
       }
     }
-    qk_lds_barrier();
-    for (qk_u32 i = threadIdx.x; i < P; i += QK_WG) cursor[i] += cnt[i];
-  }
+  };
+  /* the first step is peeled so the loop is entered, like its back edge,
+   * with a step's stores in flight: the waits for the next keys then count
+   * past those stores instead of draining them */
+  if (b0 < b1) step(b0);
+  for (qk_u64 s = b0 + qstep; s < b1; s += qstep) step(s);
 }
 
 /* instances over the arg count (rows of 1 + NA words); NA < 0 reads it */
+/* four blocks per CU fit the LDS of the 1- and 2-word rows at P <= 256;
+ * the register budget is held to match (128 VGPRs: 4 waves per SIMD) */
+#define LB_SCATTER LB __attribute__((amdgpu_waves_per_eu(4)))
 extern "C" __global__ void LB k_join_part_scatter(QK_Args a) { qk_part_scatter<-1>(a); }
-extern "C" __global__ void LB k_join_part_scatter_0(QK_Args a) { qk_part_scatter<0>(a); }
-extern "C" __global__ void LB k_join_part_scatter_1(QK_Args a) { qk_part_scatter<1>(a); }
+extern "C" __global__ void LB_SCATTER k_join_part_scatter_0(QK_Args a) { qk_part_scatter<0>(a); }
+extern "C" __global__ void LB_SCATTER k_join_part_scatter_1(QK_Args a) { qk_part_scatter<1>(a); }
 extern "C" __global__ void LB k_join_part_scatter_2(QK_Args a) { qk_part_scatter<2>(a); }
+
+/* join_part_scatter_wc_<NA>: join_part_scatter for rows of 1 or 2 words
+ * (NA = 0, 1) that writes the partitions' rows as whole aligned 128-byte
+ * lines.  A line left partly filled at the end of a step is held in LDS
+ * (the partition's carry) and written when a later step completes it, or
+ * at the block's end; written out partly filled between steps, such lines
+ * cost HBM write bandwidth.  The block's first line of each partition may
+ * hold rows of the previous block: those rows are never written here.
+ * Bindings, push and words as join_part_scatter; 512-thread blocks over
+ * the block ranges of join_part_hist (4096-row steps, 8 rows a thread);
+ * row indices < 2^32.
+ * LDS: next u64[P] | cnt, start, nf, vc u32[P] | lines QkWcLine[P +
+ *      4096 / CH] | carry u32[P * 32] | buf u32[4096 * ROWW], where CH =
+ *      32 / ROWW rows make one line. */
+#define QK_WC_WG 512
+struct QkWcLine {
+  qk_u32 dst;   /* output line index (row / CH) */
+  qk_u32 p;     /* partition */
+  qk_s32 bo;    /* buf row of the line's row 0 (rows below c are carried) */
+  qk_u32 cs;    /* c (carried rows leading the line) | skip << 16 */
+};
+
+template <int NA>
+__device__ __forceinline__ void qk_part_scatter_wc(const QK_Args& a)
+{
+  constexpr int WG = QK_WC_WG, U = 2;
+  constexpr qk_u32 W = 1 + NA;
+  constexpr qk_u32 CH = 32 / W, CHB = W == 1 ? 5 : 4;
+  static_assert(NA == 0 || NA == 1, "rows of 1 or 2 words");
+  using Row = QkRowW<W>;
+  extern __shared__ qk_u64 lds_u64[];
+  __shared__ qk_u32 s_scan[WG / 64];
+  const qk_u64 rows = qk_push(a, QK_P_ROWS);
+  const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
+  const qk_u64 r1 = r0 + rows;
+  const qk_u32 pbits = a.words[QK_W_PBITS];
+  const qk_u32 P = 1u << pbits;
+  const qk_u32 pshift = 32 - pbits;
+  const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
+  qk_u64* next = lds_u64;
+  qk_u32* cnt = (qk_u32*)(next + P);
+  qk_u32* start = cnt + P;
+  qk_u32* nf = start + P;
+  qk_u32* vc = nf + P;
+  QkWcLine* lines = (QkWcLine*)(vc + P);
+  Row* carry = (Row*)(lines + P + 4096 / CH);
+  Row* buf = carry + (qk_u64)P * CH;
+  Row* out = (Row*)qk_wptr<qk_u32>(a, 2);
+  const qk_u64 cap = a.bytes[2] / (4ULL * W);
+  {
+    /* next[p] = the block's first row of partition p; its leading rows in
+     * that row's line belong to earlier blocks (vc[p] of them) */
+    const qk_u32* totals = qk_ptr<qk_u32>(a, 4);
+    const qk_u32* off = qk_ptr<qk_u32>(a, 3) + (qk_u64)blockIdx.x * P;
+    qk_u64 running = 0;
+    for (qk_u32 c0 = 0; c0 < P; c0 += WG) {
+      const qk_u32 i = c0 + threadIdx.x;
+      const qk_u32 v = i < P ? totals[i] : 0u;
+      qk_u32 total = 0;
+      const qk_u32 pre = qk_block_excl_scan_wg<WG, true>(v, s_scan, total);
+      if (i < P) {
+        const qk_u64 n0 = running + pre + off[i];
+        next[i] = n0;
+        vc[i] = (qk_u32)(n0 & (CH - 1));
+        cnt[i] = 0;
+      }
+      running += total;
+    }
+  }
+  const qk_u32x4* keys = qk_ptr<qk_u32x4>(a, QK_B_COL0);
+  const qk_u32x4* col0 = NA ? qk_ptr<qk_u32x4>(a, QK_B_COL0 + (slots & 0xfu)) : keys;
+  qk_u64 b0, b1;
+  qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
+  const qk_u64 qstep = (qk_u64)WG * U;
+  qk_u32 nm[U];
+  qk_u32x4 nkv[U], nx[U];
+  /* quads s + j * WG + tid; past the range the last quad, masked off */
+  auto load_step = [&](qk_u64 s) {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const qk_u64 qq = s + (qk_u64)j * WG + threadIdx.x;
+      const bool in = qq < b1;
+      qk_u32 v = in ? qk_quad_valid(qq, r0, r1) : 0u;
+      if (v && qk_quad_skipped(a, qq)) v = 0u;
+      nm[j] = v;
+      const qk_u64 q = in ? qq : b1 - 1;
+      nkv[j] = keys[q];
+      if (NA) nx[j] = col0[q];
+    }
+  };
+  if (b0 < b1) load_step(b0);
+  qk_lds_barrier();
+  for (qk_u64 s = b0; s < b1; s += qstep) {
+    qk_u32 m[U];
+    qk_u32x4 kv[U], xv[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      m[j] = nm[j];
+      kv[j] = nkv[j];
+      xv[j] = nx[j];
+    }
+    load_step(s + qstep < b1 ? s + qstep : s);
+    /* rank each row within its partition's rows of the step */
+    qk_u32 hv[U][4], rank[U][4];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        hv[j][k] = qk_fmix32(w[k]);
+        rank[j][k] = (m[j] & (1u << k)) ? atomicAdd(&cnt[pbits ? hv[j][k] >> pshift : 0u], 1u) : 0u;
+      }
+    }
+    qk_lds_barrier();
+    /* per partition: c carried rows + cnt new rows; the whole lines among
+     * them are written this step.  One scan of cnt << 16 | lines gives the
+     * new rows' buf offsets and the lines' numbering (a step has at most
+     * 4096 rows and P + 4096 / CH lines, so neither half overflows). */
+    qk_u32 run_rows = 0, run_lines = 0;
+    for (qk_u32 c0 = 0; c0 < P; c0 += WG) {
+      const qk_u32 i = c0 + threadIdx.x;
+      qk_u32 cn = 0, c = 0, f = 0;
+      qk_u64 nx0 = 0;
+      if (i < P) {
+        cn = cnt[i];
+        nx0 = next[i];
+        c = (qk_u32)(nx0 & (CH - 1));
+        f = (c + cn) >> CHB;
+      }
+      qk_u32 total = 0;
+      const qk_u32 pre = qk_block_excl_scan_wg<WG, true>((cn << 16) | f, s_scan, total);
+      if (i < P) {
+        const qk_u32 st = run_rows + (pre >> 16);
+        const qk_u32 l0 = run_lines + (pre & 0xffffu);
+        start[i] = st;
+        nf[i] = (c + cn) | (c << 16);
+        const qk_u32 skip = vc[i];
+        const qk_u32 d0 = (qk_u32)((nx0 - c) >> CHB);
+        for (qk_u32 k = 0; k < f; k++) {
+          QkWcLine L;
+          L.dst = d0 + k;
+          L.p = i;
+          L.bo = (qk_s32)(st + k * CH) - (qk_s32)c;
+          L.cs = k ? 0u : (c | (skip << 16));
+          lines[l0 + k] = L;
+        }
+        if (f) vc[i] = 0;
+        next[i] = nx0 + cn;
+        cnt[i] = 0;
+      }
+      run_rows += total >> 16;
+      run_lines += total & 0xffffu;
+    }
+    const qk_u32 nlines = run_lines;
+    qk_lds_barrier();
+    /* place the new rows in partition order */
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const qk_u32 x[4] = {xv[j].x, xv[j].y, xv[j].z, xv[j].w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!(m[j] & (1u << k))) continue;
+        Row r;
+        r.w[0] = hv[j][k];
+        if (W == 2) r.w[W - 1] = x[k];
+        buf[start[pbits ? hv[j][k] >> pshift : 0u] + rank[j][k]] = r;
+      }
+    }
+    qk_lds_barrier();
+    /* write the whole lines: CH consecutive threads take one line */
+    for (qk_u32 idx = threadIdx.x; idx < nlines * CH; idx += WG) {
+      const QkWcLine L = lines[idx >> CHB];
+      const qk_u32 jj = idx & (CH - 1);
+      const qk_u32 c = L.cs & 0xffffu, skip = L.cs >> 16;
+      const Row r = jj < c ? carry[(qk_u64)L.p * CH + jj] : buf[L.bo + (qk_s32)jj];
+      const qk_u64 d = ((qk_u64)L.dst << CHB) + jj;
+      if (jj >= skip && d < cap) out[d] = r;
+    }
+    qk_lds_barrier();
+    /* carry the rows past each partition's last whole line */
+    for (qk_u32 t = threadIdx.x; t < P * CH; t += WG) {
+      const qk_u32 p = t >> CHB, jj = t & (CH - 1);
+      const qk_u32 info = nf[p];
+      const qk_u32 n = info & 0xffffu, c = info >> 16;
+      const qk_u32 j = ((n >> CHB) << CHB) + jj;
+      if (j < n && j >= c) carry[t] = buf[start[p] + j - c];
+    }
+  }
+  qk_lds_barrier();
+  /* the partly filled last lines */
+  for (qk_u32 t = threadIdx.x; t < P * CH; t += WG) {
+    const qk_u32 p = t >> CHB, jj = t & (CH - 1);
+    const qk_u64 nx0 = next[p];
+    const qk_u32 c = (qk_u32)(nx0 & (CH - 1));
+    const qk_u64 d = nx0 - c + jj;
+    if (jj < c && jj >= vc[p] && d < cap) out[d] = carry[t];
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(QK_WC_WG) k_join_part_scatter_wc_0(QK_Args a) { qk_part_scatter_wc<0>(a); }
+extern "C" __global__ void __launch_bounds__(QK_WC_WG) k_join_part_scatter_wc_1(QK_Args a) { qk_part_scatter_wc<1>(a); }
 
 /* join_part_probe: blocks (partition p, split i) for i < SPLIT; loads
  * table p (join_part_build's layout) into LDS and probes its share of the
@@ -2180,6 +2458,9 @@ static const QK_LibraryKernel g_library[] = {
     {"join_part_scatter_0", (const void*)k_join_part_scatter_0, QK_WG},
     {"join_part_scatter_1", (const void*)k_join_part_scatter_1, QK_WG},
     {"join_part_scatter_2", (const void*)k_join_part_scatter_2, QK_WG},
+    {"join_part_hist_512", (const void*)k_join_part_hist_512, 512},
+    {"join_part_scatter_wc_0", (const void*)k_join_part_scatter_wc_0, QK_WC_WG},
+    {"join_part_scatter_wc_1", (const void*)k_join_part_scatter_wc_1, QK_WC_WG},
     {"join_part_probe", (const void*)k_join_part_probe, QK_WG},
     {"join_part_probe_0", (const void*)k_join_part_probe_0, QK_WG},
     {"join_part_probe_1s", (const void*)k_join_part_probe_1s, QK_WG},
