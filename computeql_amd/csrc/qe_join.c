@@ -173,6 +173,9 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     else if (arg_count == 1 && need_sum) snprintf(pname, sizeof(pname), "join_part_probe_1%s_row", need_minmax ? "sm" : "s");
   }
   if (!qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1)) snprintf(pname, sizeof(pname), "join_part_probe");
+  /* QE_JOIN_PROBE_WG: 512 (default) or 256 threads a probe block */
+  const U32 pwg = qe_settings_u64("QE_JOIN_PROBE_WG", 512) == 256 ? 256 : 512;
+  if (pwg == 512) snprintf(pname + strlen(pname), sizeof(pname) - strlen(pname), "_512");
   GPU_Kernel* kp = qe_kernel(pname);
   GPU_Buffer* table = qe_scratch("jp_table", P * S * 8);
   GPU_Buffer* flags = qe_scratch("join_flags", 64);
@@ -183,11 +186,26 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   GPU_Buffer* rowsbuf = qe_scratch("jp_rows", (rows + 1) * W * 4);
   B32 ok = kb && kh && ko && ks && kp && table && flags && hist && off && totals && rowsbuf;
   U64 cus = gpu_hip_compute_units();
-  U32 split = (U32)((cus * qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU", 6) + P - 1) / P);
+  U32 split = (U32)((cus * qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU", 12) + P - 1) / P);
   if (split < 1) split = 1;
-  U64 acc_lds = group_mode ? R * 4 + (need_sum ? R * 8 * arg_count : 0) + (need_minmax ? R * 8 * arg_count : 0) : 0;
-  /* key words u32[S], then payloads: u16 group codes or u32 dim rows */
-  U64 probe_lds = (U64)S * 4 + (U64)S * (group_mode ? 2 : 4) + acc_lds;
+  /* key words u32[S], then payloads: u16 group codes or u32 dim rows; in
+   * group mode 2^clog copies of the group tables (cnt u32[R], sums u64[E][R],
+   * min/max u32[E][R], each with an odd stride).  The copies are as many as
+   * QE_JOIN_PROBE_COPIES (default 2) allows while QE_JOIN_PROBE_BLOCKS_PER_CU_LDS
+   * (default 3) probe blocks still fit a CU's LDS, at least one. */
+  const U64 table_lds = (U64)S * 4 + (U64)S * (group_mode ? 2 : 4);
+  U32 clog = 0;
+  U64 acc_lds = 0;
+  if (group_mode) {
+    const U64 ew = R * arg_count;
+    const U64 copy_bytes = (R | 1) * 4 + (need_sum && ew ? (ew | 1) * 8 : 0) + (need_minmax && ew ? 2 * (ew | 1) * 4 : 0);
+    const U64 max_copies = qe_settings_u64("QE_JOIN_PROBE_COPIES", 2);
+    U64 fit = qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU_LDS", 3);
+    const U64 per_block = gpu_hip_max_lds_bytes() / (fit ? fit : 1);
+    while (clog < 4 && (2ull << clog) <= max_copies && table_lds + (2ull << clog) * copy_bytes + 64 <= per_block) clog++;
+    acc_lds = ((U64)1 << clog) * copy_bytes;
+  }
+  U64 probe_lds = table_lds + acc_lds;
   if (ok && probe_lds > gpu_hip_max_lds_bytes()) {
     qe_set_error("qe_aggregate_join: partition probe needs %llu bytes of LDS", (unsigned long long)probe_lds);
     ok = 0;
@@ -257,7 +275,7 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     qe_kernel_set_word(kp, QK_W_SBITS, sbits);
     qe_kernel_set_word(kp, QK_W_ROWW, W);
     qe_kernel_set_word(kp, QK_W_SPLIT, split);
-    qe_kernel_set_word(kp, QK_W_ARGS, arg_count);
+    qe_kernel_set_word(kp, QK_W_ARGS, arg_count | (clog << 16));
     qe_kernel_set_word(kp, QK_W_TYPES, type_bits);
     qe_kernel_set_word(kp, QK_W_RANGE, (U32)R);
     qe_kernel_set_word(kp, QK_W_KEY, kflags << 16);
@@ -277,7 +295,7 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     gpu_batch_kernel_execute(b, kh, B * swg, swg);
     gpu_batch_kernel_execute(b, ko, (U32)(P * QE_WG), QE_WG);
     gpu_batch_kernel_execute(b, ks, B * swg, swg);
-    gpu_batch_kernel_execute(b, kp, (U32)(P * split * QE_WG), QE_WG);
+    gpu_batch_kernel_execute(b, kp, (U32)(P * split * pwg), pwg);
     gpu_batch_buffer_read(b, flags, fl, 8);
     gpu_batch_buffer_read(b, acc, host_acc, acc_bytes);
     ok = gpu_batch_end(b);

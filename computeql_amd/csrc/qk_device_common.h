@@ -314,18 +314,21 @@ __device__ __forceinline__ qk_u32 qk_wave_incl_scan_u32(qk_u32 v)
 }
 
 /* block-wide sum of a u64, result valid in every thread */
-__device__ __forceinline__ qk_u64 qk_block_sum_u64(qk_u64 v, qk_u64* lds4)
+template <int WG>
+__device__ __forceinline__ qk_u64 qk_block_sum_u64_wg(qk_u64 v, qk_u64* lds)
 {
   v = qk_wave_sum_u64(v);
   qk_u32 wave = threadIdx.x >> 6;
   __syncthreads();
-  if (__lane_id() == 0) lds4[wave] = v;
+  if (__lane_id() == 0) lds[wave] = v;
   __syncthreads();
   qk_u64 t = 0;
 #pragma unroll
-  for (int w = 0; w < QK_WG / 64; w++) t += lds4[w];
+  for (int w = 0; w < WG / 64; w++) t += lds[w];
   return t;
 }
+
+__device__ __forceinline__ qk_u64 qk_block_sum_u64(qk_u64 v, qk_u64* lds4) { return qk_block_sum_u64_wg<QK_WG>(v, lds4); }
 
 /* block exclusive scan of a u32 per thread over a WG-thread block; returns
  * the exclusive prefix and the block total; lds holds WG / 64 words */

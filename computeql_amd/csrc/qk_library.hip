@@ -807,6 +807,9 @@ __device__ __forceinline__ qk_u32 qk_fmix32(qk_u32 h)
   return h;
 }
 
+/* partition tables: buckets of QK_BUCKET slots (one 16-byte LDS read) */
+#define QK_BUCKET 4
+
 /* join_part_build: binding 0 dim keys, 1 dim group column, 2 partition
  * tables, 3 flags (u32[2]: duplicate, full), 4 row list (u64 ids,
  * optional); push 0 rows; words PMODE/GMIN/GTYPE as join_build, PBITS
@@ -816,7 +819,9 @@ __device__ __forceinline__ qk_u32 qk_fmix32(qk_u32 h)
  * key is fully given by the low 32-PBITS bits of h, so a slot stores the key
  * word kw = (h & low) | 0x80000000 (never 0: 0 marks an empty slot) in
  * keys u32[P][S], and the payload (dim row or group code) in pays u32[P][S]
- * right after.  Collisions walk on by linear probing over slots. */
+ * right after.  The slots form aligned 4-slot buckets; a key goes to the
+ * first bucket from its home slot's bucket on with an empty slot, so a
+ * lookup that meets a bucket with an empty slot can stop there. */
 extern "C" __global__ void LB k_join_part_build(QK_Args a)
 {
   const qk_u64 n = qk_push(a, 0);
@@ -837,9 +842,18 @@ extern "C" __global__ void LB k_join_part_build(QK_Args a)
     qk_u32 h = qk_fmix32(key);
     qk_u32 kw = (h & low) | 0x80000000u;
     qk_u64 tbase = (qk_u64)(h >> (32 - pbits)) << sbits;
-    qk_u32 slot = (h >> (32 - pbits - sbits)) & smask;
-    qk_u32 probes = 0;
-    for (;;) {
+    /* probe i takes bucket i / 4 after the home slot's bucket, and in it
+     * the slot (home + i) mod 4: the home bucket's slots are all tried
+     * before any later bucket's, starting at the home slot so that keys of
+     * one bucket do not all contend for its first slot */
+    const qk_u32 home = (h >> (32 - pbits - sbits)) & smask;
+    const qk_u32 hb = home & ~(QK_BUCKET - 1u);
+    for (qk_u32 i = 0;; i++) {
+      if (i > smask) {
+        atomicOr(&flags[1], 1u);
+        break;
+      }
+      const qk_u32 slot = ((hb + (i & ~(QK_BUCKET - 1u))) & smask) + ((home + i) & (QK_BUCKET - 1u));
       qk_u32 prev = atomicCAS(&tkeys[tbase + slot], 0u, kw);
       if (prev == 0u) {
         tpays[tbase + slot] = pay;
@@ -847,11 +861,6 @@ extern "C" __global__ void LB k_join_part_build(QK_Args a)
       }
       if (prev == kw) {
         atomicOr(&flags[0], 1u);
-        break;
-      }
-      slot = (slot + 1) & smask;
-      if (++probes > smask) {
-        atomicOr(&flags[1], 1u);
         break;
       }
     }
@@ -1368,19 +1377,21 @@ extern "C" __global__ void __launch_bounds__(QK_WC_WG) k_join_part_scatter_wc_1(
  * payload = group code -> LDS group tables; flag 4 payload = dim row ->
  * global atomics).  binding 1 accumulators, 2 partition tables, 3 rows,
  * 4 totals; words PBITS (>= 1), SBITS (>= 3), ROWW, SPLIT, ARGS (arg
- * count), TYPES, RANGE, KEY (flags << 16).
- * A lookup reads the aligned 4-slot bucket holding the home slot (one
- * 16-byte LDS read) and compares its key words: a key sits between
- * its home slot and the first empty slot after it, so the bucket settles
- * the lookup unless every slot from the home slot to the bucket's end is
- * taken, and only then does the next bucket follow.
+ * count | log2 of the group-table copies << 16), TYPES, RANGE, KEY
+ * (flags << 16).
+ * A lookup reads the key's home bucket and the bucket after it (two
+ * 16-byte LDS reads issued together) and compares their key words: a key
+ * sits between its home bucket's first slot and the first empty slot at or
+ * after it, so the two buckets settle the lookup unless both are full
+ * without the key, and only then do further buckets follow.
+ * In group mode the LDS group tables are replicated (2^k copies, lane i
+ * adding into copy i mod 2^k, each copy's stride odd so one group's copies
+ * fall in different banks): lanes of a wave adding into one group then
+ * mostly hit different addresses.  The copies are summed at the end.
  * The body is a template over the arg count and the aggregate flags so the
  * common shapes run without per-row branches on them; NA < 0 is the
  * generic form that reads them from the push words.  Group codes
  * (< 65536) sit in LDS as u16, dim rows as u32. */
-#define QK_PROBE_ROWS 16
-/* QK_BUCKET slots (QK_BUCKET * 4 bytes, one 16-byte LDS read) */
-#define QK_BUCKET 4
 struct QkBucket {
   qk_u32 w[QK_BUCKET];
 };
@@ -1399,12 +1410,12 @@ __device__ __forceinline__ qk_u32 qk_bucket_match(const QkBucket& k, qk_u32 v)
   return m;
 }
 
-template <int NA, bool SUMS, bool MM, bool ROW>
+template <int NA, bool SUMS, bool MM, bool ROW, int WG, int ROWS, int G>
 __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
 {
   /* 16-byte aligned: the bucket reads are 16-byte LDS loads */
   extern __shared__ __attribute__((aligned(16))) qk_u64 lds_u64[];
-  __shared__ qk_u64 s_red[QK_WG / 64];
+  __shared__ qk_u64 s_red[WG / 64];
   constexpr bool FIXED = NA >= 0;
   const qk_u32 pbits = a.words[QK_W_PBITS], sbits = a.words[QK_W_SBITS];
   const qk_u32 P = 1u << pbits, S = 1u << sbits, smask = S - 1u;
@@ -1420,19 +1431,29 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
   const bool per_row = FIXED ? ROW : (flags & 4u) != 0;
   const qk_u32 part = blockIdx.x / split, piece = blockIdx.x % split;
   if (part >= P) return;
-  /* LDS: keys u32[S] | pays (u16 group codes or u32 dim rows)[S] |
-   *      sums u64[E][R] | cnt u32[R] | min/max u32[E][R] */
+  /* LDS: keys u32[S] | pays (u16 group codes or u32 dim rows)[S] | then,
+   * in group mode, C copies each of sums u64[E][R] | cnt u32[R] |
+   * min u32[E][R] | max u32[E][R], every copy's stride odd */
   qk_u32* tk = (qk_u32*)lds_u64;
   unsigned short* tp16 = (unsigned short*)(tk + S);
   qk_u32* tp32 = tk + S;
+  const qk_u32 C = per_row ? 1u : (1u << ((a.words[QK_W_ARGS] >> 16) & 0xfu));
+  const qk_u32 sw = (!per_row && do_sum) ? nargs * range : 0u;
+  const qk_u32 mw = (!per_row && do_mm) ? nargs * range : 0u;
+  const qk_u32 sstr = sw ? (sw | 1u) : 0u, cstr = per_row ? 0u : (range | 1u), mstr = mw ? (mw | 1u) : 0u;
   qk_u64* l_sum = (qk_u64*)(per_row ? (void*)(tp32 + S) : (void*)(tp16 + S));
-  qk_u32* l_cnt = (qk_u32*)(l_sum + ((!per_row && do_sum) ? (qk_u64)nargs * range : 0));
-  qk_u32* l_min = l_cnt + (per_row ? 0u : range);
-  qk_u32* l_max = l_min + ((!per_row && do_mm) ? (qk_u64)nargs * range : 0);
+  qk_u32* l_cnt = (qk_u32*)(l_sum + (qk_u64)C * sstr);
+  qk_u32* l_min = l_cnt + C * cstr;
+  qk_u32* l_max = l_min + C * mstr;
+  const qk_u32 copy = __lane_id() & (C - 1u);
+  qk_u64* my_sum = l_sum + copy * sstr;
+  qk_u32* my_cnt = l_cnt + copy * cstr;
+  qk_u32* my_min = l_min + copy * mstr;
+  qk_u32* my_max = l_max + copy * mstr;
   {
     const qk_u32x4* gk = qk_ptr<qk_u32x4>(a, 2) + (((qk_u64)part << sbits) >> 2);
     const qk_u32x4* gp = qk_ptr<qk_u32x4>(a, 2) + ((((qk_u64)P + part) << sbits) >> 2);
-    for (qk_u32 i = threadIdx.x; i < S / 4; i += QK_WG) {
+    for (qk_u32 i = threadIdx.x; i < S / 4; i += WG) {
       ((qk_u32x4*)tk)[i] = gk[i];
       qk_u32x4 v = gp[i];
       if (per_row) {
@@ -1446,11 +1467,9 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
     }
   }
   if (!per_row) {
-    const qk_u32 sum_words = do_sum ? nargs * range : 0u;
-    const qk_u32 mm_words = do_mm ? nargs * range : 0u;
-    for (qk_u32 i = threadIdx.x; i < sum_words; i += QK_WG) l_sum[i] = 0;
-    for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) l_cnt[i] = 0;
-    for (qk_u32 i = threadIdx.x; i < mm_words; i += QK_WG) {
+    for (qk_u32 i = threadIdx.x; i < C * sstr; i += WG) l_sum[i] = 0;
+    for (qk_u32 i = threadIdx.x; i < C * cstr; i += WG) l_cnt[i] = 0;
+    for (qk_u32 i = threadIdx.x; i < C * mstr; i += WG) {
       l_min[i] = 0xffffffffu;
       l_max[i] = 0u;
     }
@@ -1459,8 +1478,8 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
    * it (the reduction's barriers also publish the LDS fills above) */
   const qk_u32* totals = qk_ptr<qk_u32>(a, 4);
   qk_u64 before = 0;
-  for (qk_u32 i = threadIdx.x; i < part; i += QK_WG) before += totals[i];
-  const qk_u64 pstart = qk_block_sum_u64(before, s_red);
+  for (qk_u32 i = threadIdx.x; i < part; i += WG) before += totals[i];
+  const qk_u64 pstart = qk_block_sum_u64_wg<WG>(before, s_red);
   const qk_u64 pend = pstart + totals[part];
   const qk_u64 plen = pend - pstart;
   const qk_u64 chunk = (plen + split - 1) / split;
@@ -1471,17 +1490,17 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
   qk_u64* g_sum = g_cnt + range;
   qk_u32* g_min = (qk_u32*)(g_sum + (qk_u64)nargs * range);
   qk_u32* g_max = g_min + (qk_u64)nargs * range;
-  /* QK_PROBE_ROWS rows per thread per step.  A row's key and first arg
+  /* ROWS rows per thread per step.  A row's key and first arg
    * load together, one whole step ahead of their use, so a step's loads are
    * in flight while the step before probes the LDS table; the rows are then
-   * probed four at a time to bound the registers held per row.  Rows past
+   * probed G at a time to bound the registers held per row.  Rows past
    * re clamp to rs (in bounds: the loop runs only when rs < re). */
-  const qk_u64 pstep = (qk_u64)QK_WG * QK_PROBE_ROWS;
-  qk_u32 nkey[QK_PROBE_ROWS], narg[QK_PROBE_ROWS];
+  const qk_u64 pstep = (qk_u64)WG * ROWS;
+  qk_u32 nkey[ROWS], narg[ROWS];
   auto load_rows = [&](qk_u64 r) {
 #pragma unroll
-    for (int j = 0; j < QK_PROBE_ROWS; j++) {
-      qk_u64 x = r + (qk_u64)j * QK_WG + threadIdx.x;
+    for (int j = 0; j < ROWS; j++) {
+      qk_u64 x = r + (qk_u64)j * WG + threadIdx.x;
       qk_u64 y = x < re ? x : rs;
       if (W == 2) {
         uint2 v = ((const uint2*)rowsbuf)[y];
@@ -1495,9 +1514,9 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
   };
   if (rs < re) load_rows(rs);
   for (qk_u64 r = rs; r < re; r += pstep) {
-    qk_u32 ckey[QK_PROBE_ROWS], carg[QK_PROBE_ROWS];
+    qk_u32 ckey[ROWS], carg[ROWS];
 #pragma unroll
-    for (int j = 0; j < QK_PROBE_ROWS; j++) {
+    for (int j = 0; j < ROWS; j++) {
       ckey[j] = nkey[j];
       carg[j] = narg[j];
     }
@@ -1505,30 +1524,31 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
     /* uniform: every row of the step lies in [rs, re) */
     const bool full = r + pstep <= re;
 #pragma unroll
-    for (int sb = 0; sb < QK_PROBE_ROWS; sb += 4) {
-      qk_u32 kw[4], home[4], match[4], pay[4];
-      QkBucket bk[4];
-      bool hit[4];
+    for (int sb = 0; sb < ROWS; sb += G) {
+      qk_u32 kw[G], home[G], pay[G];
+      QkBucket bk0[G], bk1[G];
+      bool hit[G];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < G; j++) {
         const qk_u32 h = ckey[sb + j];
         kw[j] = (h & low) | 0x80000000u;
-        home[j] = (h >> (32 - pbits - sbits)) & smask;
+        home[j] = (h >> (32 - pbits - sbits)) & smask & ~(QK_BUCKET - 1u);
       }
-      /* first buckets of the four rows; a bucket without the key is
-       * followed on only while it has no empty slot at or after the home
-       * slot (rare) */
+      /* the home bucket and the next of each of the G rows; further
+       * buckets follow only while both are full without the key (rare) */
 #pragma unroll
-      for (int j = 0; j < 4; j++) bk[j] = qk_bucket_read(tk, home[j] & ~(QK_BUCKET - 1u));
+      for (int j = 0; j < G; j++) {
+        bk0[j] = qk_bucket_read(tk, home[j]);
+        bk1[j] = qk_bucket_read(tk, (home[j] + QK_BUCKET) & smask);
+      }
 #pragma unroll
-      for (int j = 0; j < 4; j++) match[j] = qk_bucket_match(bk[j], kw[j]);
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        qk_u32 b = home[j] & ~(QK_BUCKET - 1u);
-        qk_u32 m = match[j];
+      for (int j = 0; j < G; j++) {
+        const qk_u32 m0 = qk_bucket_match(bk0[j], kw[j]), m1 = qk_bucket_match(bk1[j], kw[j]);
+        qk_u32 b = m0 ? home[j] : (home[j] + QK_BUCKET) & smask;
+        qk_u32 m = m0 ? m0 : m1;
         if (!m) {
-          qk_u32 e = qk_bucket_match(bk[j], 0u) & (0xffu << (home[j] & (QK_BUCKET - 1u)));
-          for (qk_u32 n = QK_BUCKET; !m && !e && n < S; n += QK_BUCKET) {
+          qk_u32 e = qk_bucket_match(bk0[j], 0u) | qk_bucket_match(bk1[j], 0u);
+          for (qk_u32 n = 2 * QK_BUCKET; !m && !e && n < S; n += QK_BUCKET) {
             b = (b + QK_BUCKET) & smask;
             QkBucket nb = qk_bucket_read(tk, b);
             m = qk_bucket_match(nb, kw[j]);
@@ -1537,25 +1557,25 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
         }
         qk_u32 idx = b + (qk_u32)__builtin_ctz(m | 0x100u);
         qk_u32 pj = m ? (per_row ? tp32[idx] : (qk_u32)tp16[idx]) : 0u;
-        hit[j] = m && (full || (r + (qk_u64)(sb + j) * QK_WG + threadIdx.x) < re) && pj < range;
+        hit[j] = m && (full || (r + (qk_u64)(sb + j) * WG + threadIdx.x) < re) && pj < range;
         pay[j] = pj;
       }
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < G; j++) {
         if (!hit[j]) continue;
         if (per_row) atomicAdd(&g_cnt[pay[j]], 1ULL);
-        else atomicAdd(&l_cnt[pay[j]], 1u);
+        else atomicAdd(&my_cnt[pay[j]], 1u);
       }
       for (qk_u32 c = 0; c < nargs; c++) {
         qk_u32 at = (types >> (8 * c)) & 0xffu;
-        qk_u32 v[4];
+        qk_u32 v[G];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          qk_u64 x = r + (qk_u64)(sb + j) * QK_WG + threadIdx.x;
+        for (int j = 0; j < G; j++) {
+          qk_u64 x = r + (qk_u64)(sb + j) * WG + threadIdx.x;
           v[j] = c == 0 ? carg[sb + j] : rowsbuf[(x < re ? x : rs) * W + 1 + c];
         }
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < G; j++) {
           if (!hit[j]) continue;
           qk_u32 g = pay[j];
           qk_u64 add = at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)v[j] : (qk_u64)v[j];
@@ -1567,10 +1587,10 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
               atomicMax(&g_max[(qk_u64)c * range + g], code);
             }
           } else {
-            if (do_sum) atomicAdd(&l_sum[c * range + g], add);
+            if (do_sum) atomicAdd(&my_sum[c * range + g], add);
             if (do_mm) {
-              atomicMin(&l_min[c * range + g], code);
-              atomicMax(&l_max[c * range + g], code);
+              atomicMin(&my_min[c * range + g], code);
+              atomicMax(&my_max[c * range + g], code);
             }
           }
         }
@@ -1579,15 +1599,26 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
   }
   if (per_row) return;
   __syncthreads();
-  for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) {
-    qk_u32 cnt = l_cnt[i];
+  for (qk_u32 i = threadIdx.x; i < range; i += WG) {
+    qk_u32 cnt = 0;
+    for (qk_u32 k = 0; k < C; k++) cnt += l_cnt[k * cstr + i];
     if (!cnt) continue;
     atomicAdd(&g_cnt[i], (qk_u64)cnt);
     for (qk_u32 c = 0; c < nargs; c++) {
-      if (do_sum) atomicAdd(&g_sum[(qk_u64)c * range + i], l_sum[c * range + i]);
+      const qk_u32 w = c * range + i;
+      if (do_sum) {
+        qk_u64 sum = 0;
+        for (qk_u32 k = 0; k < C; k++) sum += l_sum[k * sstr + w];
+        atomicAdd(&g_sum[(qk_u64)c * range + i], sum);
+      }
       if (do_mm) {
-        atomicMin(&g_min[(qk_u64)c * range + i], l_min[c * range + i]);
-        atomicMax(&g_max[(qk_u64)c * range + i], l_max[c * range + i]);
+        qk_u32 mn = 0xffffffffu, mx = 0u;
+        for (qk_u32 k = 0; k < C; k++) {
+          mn = min(mn, l_min[k * mstr + w]);
+          mx = max(mx, l_max[k * mstr + w]);
+        }
+        atomicMin(&g_min[(qk_u64)c * range + i], mn);
+        atomicMax(&g_max[(qk_u64)c * range + i], mx);
       }
     }
   }
@@ -1595,17 +1626,31 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
 
 /* instances: generic, then group mode with 0-2 args (sums, min/max) and row
  * mode with 0-1 args; the host picks join_part_probe_<args><s|m|sm>[_row]
- * when one matches and the generic form otherwise */
-extern "C" __global__ void LB k_join_part_probe(QK_Args a) { qk_part_probe<-1, false, false, false>(a); }
-extern "C" __global__ void LB k_join_part_probe_0(QK_Args a) { qk_part_probe<0, false, false, false>(a); }
-extern "C" __global__ void LB k_join_part_probe_1s(QK_Args a) { qk_part_probe<1, true, false, false>(a); }
-extern "C" __global__ void LB k_join_part_probe_1m(QK_Args a) { qk_part_probe<1, false, true, false>(a); }
-extern "C" __global__ void LB k_join_part_probe_1sm(QK_Args a) { qk_part_probe<1, true, true, false>(a); }
-extern "C" __global__ void LB k_join_part_probe_2s(QK_Args a) { qk_part_probe<2, true, false, false>(a); }
-extern "C" __global__ void LB k_join_part_probe_2sm(QK_Args a) { qk_part_probe<2, true, true, false>(a); }
-extern "C" __global__ void LB k_join_part_probe_0_row(QK_Args a) { qk_part_probe<0, false, false, true>(a); }
-extern "C" __global__ void LB k_join_part_probe_1s_row(QK_Args a) { qk_part_probe<1, true, false, true>(a); }
-extern "C" __global__ void LB k_join_part_probe_1sm_row(QK_Args a) { qk_part_probe<1, true, true, true>(a); }
+ * when one matches and the generic form otherwise.  Each comes as a
+ * 256-thread block of 16 rows a thread per step, looked up 4 at a time,
+ * and, suffixed _512, as a 512-thread block of 8 rows a thread looked up 2
+ * at a time (the same LDS per block, fewer registers a thread, held to 80
+ * so that three blocks, 6 waves a SIMD, fit a CU: twice the waves per CU). */
+#define QK_PROBE_INSTANCE(name, na, sums, mm, row)                                                                  \
+  extern "C" __global__ void LB k_join_part_probe##name(QK_Args a)                                                 \
+  {                                                                                                                \
+    qk_part_probe<na, sums, mm, row, QK_WG, 16, 4>(a);                                                                \
+  }                                                                                                                \
+  extern "C" __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6)))                        \
+      k_join_part_probe##name##_512(QK_Args a)                                                                     \
+  {                                                                                                                \
+    qk_part_probe<na, sums, mm, row, 512, 8, 2>(a);                                                                   \
+  }
+QK_PROBE_INSTANCE(, -1, false, false, false)
+QK_PROBE_INSTANCE(_0, 0, false, false, false)
+QK_PROBE_INSTANCE(_1s, 1, true, false, false)
+QK_PROBE_INSTANCE(_1m, 1, false, true, false)
+QK_PROBE_INSTANCE(_1sm, 1, true, true, false)
+QK_PROBE_INSTANCE(_2s, 2, true, false, false)
+QK_PROBE_INSTANCE(_2sm, 2, true, true, false)
+QK_PROBE_INSTANCE(_0_row, 0, false, false, true)
+QK_PROBE_INSTANCE(_1s_row, 1, true, false, true)
+QK_PROBE_INSTANCE(_1sm_row, 1, true, true, true)
 
 /* ------------------------------------------------------------------ */
 /* hash GROUP BY (query_exec.c:10258-12300, general path: slot          */
@@ -2462,15 +2507,25 @@ static const QK_LibraryKernel g_library[] = {
     {"join_part_scatter_wc_0", (const void*)k_join_part_scatter_wc_0, QK_WC_WG},
     {"join_part_scatter_wc_1", (const void*)k_join_part_scatter_wc_1, QK_WC_WG},
     {"join_part_probe", (const void*)k_join_part_probe, QK_WG},
+    {"join_part_probe_512", (const void*)k_join_part_probe_512, 512},
     {"join_part_probe_0", (const void*)k_join_part_probe_0, QK_WG},
+    {"join_part_probe_0_512", (const void*)k_join_part_probe_0_512, 512},
     {"join_part_probe_1s", (const void*)k_join_part_probe_1s, QK_WG},
+    {"join_part_probe_1s_512", (const void*)k_join_part_probe_1s_512, 512},
     {"join_part_probe_1m", (const void*)k_join_part_probe_1m, QK_WG},
+    {"join_part_probe_1m_512", (const void*)k_join_part_probe_1m_512, 512},
     {"join_part_probe_1sm", (const void*)k_join_part_probe_1sm, QK_WG},
+    {"join_part_probe_1sm_512", (const void*)k_join_part_probe_1sm_512, 512},
     {"join_part_probe_2s", (const void*)k_join_part_probe_2s, QK_WG},
+    {"join_part_probe_2s_512", (const void*)k_join_part_probe_2s_512, 512},
     {"join_part_probe_2sm", (const void*)k_join_part_probe_2sm, QK_WG},
+    {"join_part_probe_2sm_512", (const void*)k_join_part_probe_2sm_512, 512},
     {"join_part_probe_0_row", (const void*)k_join_part_probe_0_row, QK_WG},
+    {"join_part_probe_0_row_512", (const void*)k_join_part_probe_0_row_512, 512},
     {"join_part_probe_1s_row", (const void*)k_join_part_probe_1s_row, QK_WG},
+    {"join_part_probe_1s_row_512", (const void*)k_join_part_probe_1s_row_512, 512},
     {"join_part_probe_1sm_row", (const void*)k_join_part_probe_1sm_row, QK_WG},
+    {"join_part_probe_1sm_row_512", (const void*)k_join_part_probe_1sm_row_512, 512},
     {"gen_events", (const void*)k_gen_events, QK_WG},
     {"gen_scale_events", (const void*)k_gen_scale_events, QK_WG},
     {"gen_fact_fk", (const void*)k_gen_fact_fk, QK_WG},
