@@ -97,7 +97,8 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
                                const QE_AggPlan* p, U64* out_cells, QE_Trace* trace)
 {
   GPU_Kernel* k = qe_kernel("aggregate_global");
-  if (!k) return 0;
+  GPU_Kernel* kf = qe_kernel("aggregate_global_fold");
+  if (!k || !kf) return 0;
   QE_ColumnSlots slots;
   memset(&slots, 0, sizeof(slots));
   U32 arg_slot[QE_AGG_MAX_ARGS];
@@ -144,12 +145,17 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
     gpu_kernel_set_arg_u64(k, 7, leaf_count);
     qe_kernel_set_word(k, QK_W_ARGS, p->arg_count | (slot_bits << 8));
     qe_kernel_set_word(k, QK_W_TYPES, type_bits);
+    /* per-block partials, then one block folds them in block order */
+    gpu_kernel_set_arg_buffer(kf, 1, result);
+    gpu_kernel_set_arg_buffer(kf, 2, partials);
+    gpu_kernel_set_arg_u64(kf, 0, blocks);
+    qe_kernel_set_word(kf, QK_W_ARGS, p->arg_count);
+    qe_kernel_set_word(kf, QK_W_TYPES, type_bits);
     GPU_Batch* b = gpu_batch_begin(0, sizeof(res));
     if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
     if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
-    /* the ticket word of the result starts at zero (new scratch is zeroed)
-     * and the kernel's last block leaves it at zero */
     gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+    gpu_batch_kernel_execute(b, kf, QE_WG, QE_WG);
     gpu_batch_buffer_read(b, result, res, sizeof(res));
     ok = gpu_batch_end(b);
   }
@@ -175,7 +181,7 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
   }
   if (trace) {
     trace->path = leaf_count ? QE_AggPath_FusedGlobal : QE_AggPath_Global;
-    trace->kernel_count = 1;
+    trace->kernel_count = 2;
     trace->rows_scanned = r1 - r0;
     trace->rows_after = count;
     U64 bytes = 0;

@@ -608,11 +608,16 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
     if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune.skip_bits, prune.skip_words * 4);
   }
   if (ok && count_only) {
-    /* count mode: the kernel's last block leaves the total in counter[0] */
+    /* count mode: one count per block, added here (as the reference adds
+     * its per-chunk counts, query_exec.c:2184-2190) */
+    U64* counts = (U64*)malloc(sizeof(U64) * blocks);
     gpu_kernel_set_arg_u64(k, 6, 0); /* QK_MODE_COUNT */
     gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
-    gpu_batch_buffer_read(b, counter, &total, 8);
-    ok = gpu_batch_end(b);
+    if (counts) gpu_batch_buffer_read(b, block_counts, counts, (U64)blocks * 8);
+    ok = gpu_batch_end(b) && counts;
+    if (ok)
+      for (U32 i = 0; i < blocks; i++) total += counts[i];
+    free(counts);
     kernels = 1;
   } else if (ok) {
     U64* counts = (U64*)malloc(sizeof(U64) * blocks);

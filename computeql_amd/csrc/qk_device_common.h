@@ -70,6 +70,8 @@ typedef struct QK_Args {
 #define QK_P_MODE 6
 #define QK_P_LEAVES 7
 
+/* modes 0 (count) and 1 (block counts) both leave one count per block in
+ * binding 2; mode 2 writes the ordered row ids */
 #define QK_MODE_COUNT 0
 #define QK_MODE_BLOCK_COUNTS 1
 #define QK_MODE_WRITE_IDS 2
@@ -399,31 +401,13 @@ __device__ __forceinline__ void qk_scan_driver_step(const QK_Args& a, Pred pred)
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) count += (qk_u64)__popc(m[j]);
     }
+    /* count and block-count modes: one count per block, added up by the
+     * caller.  (A last-block fold inside the kernel would need an agent-
+     * scope release/acquire per block, which on this part writes back and
+     * invalidates the XCD's L2 under the blocks still streaming.) */
     qk_u64 total = qk_block_sum_u64(count, s_red);
     qk_u64* bc = qk_wptr<qk_u64>(a, QK_B_BLOCK_COUNTS);
     if (threadIdx.x == 0) bc[blockIdx.x] = total;
-    if (mode != QK_MODE_COUNT) return;
-    /* count mode: the last block to finish adds the block counts (in block
-     * order) into out_count[0]; out_count[1] is its ticket, left at zero */
-    __shared__ qk_u32 s_last;
-    __threadfence();
-    __syncthreads();
-    qk_u64* oc = qk_wptr<qk_u64>(a, QK_B_OUT_COUNT);
-    if (threadIdx.x == 0) {
-      qk_u32 t = __hip_atomic_fetch_add((qk_u32*)&oc[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      s_last = t == gridDim.x - 1 ? 1u : 0u;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    qk_u64 part = 0;
-    for (qk_u32 b = threadIdx.x; b < gridDim.x; b += QK_WG)
-      part += __hip_atomic_load(&bc[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    qk_u64 all = qk_block_sum_u64(part, s_red);
-    if (threadIdx.x == 0) {
-      oc[0] = all;
-      __hip_atomic_store((qk_u32*)&oc[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     return;
   }
 

@@ -81,17 +81,17 @@ extern "C" __global__ void LB k_bandwidth_read(QK_Args a)
 /* ------------------------------------------------------------------ */
 /* aggregate_global: COUNT and SUM/MIN/MAX of up to 4 columns with an   */
 /* optional numeric-AND WHERE, one pass.                                */
-/*  binding 0 leaves (6 words each), 1 result (16 x u64), 2 partials    */
-/*  (grid x 13 u64), 5 skip bitmap, 6.. column slots                    */
+/*  binding 0 leaves (6 words each), 2 partials (grid x 13 u64),       */
+/*  5 skip bitmap, 6.. column slots                                     */
 /*  push 0 rows, 3 skip shift, 4 skip on, 5 first row, 7 leaf count     */
 /*  words[12] = arg count | arg slots << 8 (4 bits each)                */
 /*  words[13] = arg types (8 bits each)                                 */
-/* result: [0] count, [1+3c] sum, [2+3c] min key, [3+3c] max key,       */
-/* [15] ticket.  Sums: exact 64-bit integers for U32/I32 (I32 as two's  */
+/* partials, per block: [0] count, [1+3c] sum, [2+3c] min key, [3+3c]  */
+/* max key.  Sums: exact 64-bit integers for U32/I32 (I32 as two's      */
 /* complement), f64 bits otherwise.  Min/max keys are order-preserving  */
 /* u64 encodings (u32: value; i32/i64: value ^ sign; f64: IEEE flip).   */
-/* The last block to finish folds the partials in a fixed order, so the */
-/* f64 result depends only on the grid size.                            */
+/* aggregate_global_fold then folds the partials into the result in a   */
+/* fixed order, so the f64 result depends only on the grid size.        */
 /* ------------------------------------------------------------------ */
 __device__ __forceinline__ qk_u64 qk_f64_key(double d)
 {
@@ -175,7 +175,6 @@ __device__ __forceinline__ qk_u64 agg_combine_sum(qk_u32 type, qk_u64 x, qk_u64 
 extern "C" __global__ void LB k_aggregate_global(QK_Args a)
 {
   __shared__ qk_u64 s_part[QK_WG / 64][13];
-  __shared__ qk_u32 s_last;
   const qk_u64 rows = qk_push(a, QK_P_ROWS);
   const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
   const qk_u64 r1 = r0 + rows;
@@ -258,7 +257,6 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
   }
   __syncthreads();
   qk_u64* partials = qk_wptr<qk_u64>(a, 2);
-  qk_u64* result = qk_wptr<qk_u64>(a, 1);
   if (threadIdx.x < 13) {
     int f = threadIdx.x;
     qk_u64 v = s_part[0][f];
@@ -273,18 +271,26 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
     }
     partials[(qk_u64)blockIdx.x * 13 + f] = v;
   }
-  /* last block folds (release the partials, then take a ticket) */
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    qk_u32 ticket = __hip_atomic_fetch_add((qk_u32*)&result[15], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = ticket == gridDim.x - 1 ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  /* fixed-order fold, all threads: thread t walks blocks t, t+256, ... in
-   * order, then the 256 partials meet in a fixed butterfly + wave order */
+}
+
+/* aggregate_global_fold: one block folds aggregate_global's per-block
+ * partials (binding 2, 13 words a block; push 0 = blocks) into binding 1's
+ * words 0..12, in a fixed order: thread t walks blocks t, t+256, ... in
+ * order, then the 256 partials meet in a fixed butterfly + wave order.  It
+ * runs as its own launch after aggregate_global: the kernel boundary makes
+ * the partials visible, where a last-block fold inside aggregate_global
+ * would need an agent-scope release/acquire per block (an L2 write-back
+ * and invalidate on this part, under the blocks still streaming).
+ * Words ARGS, TYPES as aggregate_global. */
+extern "C" __global__ void LB k_aggregate_global_fold(QK_Args a)
+{
+  __shared__ qk_u64 s_part[QK_WG / 64][13];
+  const qk_u32 nblocks = (qk_u32)qk_push(a, 0);
+  const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 types = a.words[QK_W_TYPES];
+  const qk_u64* partials = qk_ptr<qk_u64>(a, 2);
+  qk_u64* result = qk_wptr<qk_u64>(a, 1);
+  const qk_u32 lane = __lane_id(), wave = threadIdx.x >> 6;
   qk_u64 v[13];
   v[0] = 0;
 #pragma unroll
@@ -293,16 +299,16 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
     v[2 + 3 * c] = ~0ULL;
     v[3 + 3 * c] = 0;
   }
-  for (qk_u32 b = threadIdx.x; b < gridDim.x; b += QK_WG) {
+  for (qk_u32 b = threadIdx.x; b < nblocks; b += QK_WG) {
     const qk_u64* pb = &partials[(qk_u64)b * 13];
-    v[0] += __hip_atomic_load(&pb[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v[0] += pb[0];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       if ((qk_u32)c >= nargs) continue;
       qk_u32 t = (types >> (8 * c)) & 0xffu;
-      v[1 + 3 * c] = agg_combine_sum(t, v[1 + 3 * c], __hip_atomic_load(&pb[1 + 3 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      v[2 + 3 * c] = min(v[2 + 3 * c], (qk_u64)__hip_atomic_load(&pb[2 + 3 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      v[3 + 3 * c] = max(v[3 + 3 * c], (qk_u64)__hip_atomic_load(&pb[3 + 3 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      v[1 + 3 * c] = agg_combine_sum(t, v[1 + 3 * c], pb[1 + 3 * c]);
+      v[2 + 3 * c] = min(v[2 + 3 * c], pb[2 + 3 * c]);
+      v[3 + 3 * c] = max(v[3 + 3 * c], pb[3 + 3 * c]);
     }
   }
 #pragma unroll
@@ -316,7 +322,6 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
       v[3 + 3 * c] = max(v[3 + 3 * c], (qk_u64)__shfl_xor(v[3 + 3 * c], o, 64));
     }
   }
-  __syncthreads();
   if (lane == 0)
     for (int f = 0; f < 13; f++) s_part[wave][f] = v[f];
   __syncthreads();
@@ -334,7 +339,6 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
     }
     result[f] = r;
   }
-  if (threadIdx.x == 0) __hip_atomic_store((qk_u32*)&result[15], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* ------------------------------------------------------------------ */
@@ -2482,6 +2486,7 @@ static const QK_LibraryKernel g_library[] = {
     {"fill_u32", (const void*)k_fill_u32, QK_WG},
     {"bandwidth_read", (const void*)k_bandwidth_read, QK_WG},
     {"aggregate_global", (const void*)k_aggregate_global, QK_WG},
+    {"aggregate_global_fold", (const void*)k_aggregate_global_fold, QK_WG},
     {"scan_simple", (const void*)k_scan_simple, QK_WG},
     {"scan_filter", (const void*)k_scan_filter, QK_WG},
     {"aggregate_dense", (const void*)k_aggregate_dense, QK_WG},
