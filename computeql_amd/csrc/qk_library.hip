@@ -2377,6 +2377,283 @@ extern "C" __global__ void LB k_shared_reduce(QK_Args a)
 }
 
 /* ------------------------------------------------------------------ */
+/* exclusive scan of u32 counts (any length): scan_u32_blocks scans each */
+/* 2048-element tile and leaves its total, scan_u32_top scans the tile  */
+/* totals in one block, scan_u32_add adds them back.  binding 0 in, 1   */
+/* out (may alias in), 2 tile totals; push 0 = n.                       */
+/* ------------------------------------------------------------------ */
+#define QK_SCAN_TILE (QK_WG * 8)
+extern "C" __global__ void LB k_scan_u32_blocks(QK_Args a)
+{
+  __shared__ qk_u32 s_scan[QK_WG / 64];
+  const qk_u64 n = qk_push(a, 0);
+  const qk_u32* in = qk_ptr<qk_u32>(a, 0);
+  qk_u32* out = qk_wptr<qk_u32>(a, 1);
+  const qk_u64 t0 = (qk_u64)blockIdx.x * QK_SCAN_TILE + (qk_u64)threadIdx.x * 8;
+  qk_u32 v[8], run = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    v[k] = t0 + k < n ? in[t0 + k] : 0u;
+    const qk_u32 x = v[k];
+    v[k] = run;
+    run += x;
+  }
+  qk_u32 total = 0;
+  const qk_u32 pre = qk_block_excl_scan_u32(run, s_scan, total);
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (t0 + k < n) out[t0 + k] = pre + v[k];
+  if (threadIdx.x == 0) qk_wptr<qk_u32>(a, 2)[blockIdx.x] = total;
+}
+
+extern "C" __global__ void LB k_scan_u32_top(QK_Args a)
+{
+  __shared__ qk_u32 s_scan[QK_WG / 64];
+  const qk_u64 tiles = qk_push(a, 0);
+  qk_u32* t = qk_wptr<qk_u32>(a, 2);
+  qk_u32 running = 0;
+  for (qk_u64 c = 0; c < tiles; c += QK_WG) {
+    const qk_u64 i = c + threadIdx.x;
+    const qk_u32 v = i < tiles ? t[i] : 0u;
+    qk_u32 total = 0;
+    const qk_u32 pre = qk_block_excl_scan_u32(v, s_scan, total);
+    if (i < tiles) t[i] = running + pre;
+    running += total;
+  }
+}
+
+extern "C" __global__ void LB k_scan_u32_add(QK_Args a)
+{
+  const qk_u64 n = qk_push(a, 0);
+  qk_u32* out = qk_wptr<qk_u32>(a, 1);
+  const qk_u32 add = qk_ptr<qk_u32>(a, 2)[blockIdx.x];
+  const qk_u64 t0 = (qk_u64)blockIdx.x * QK_SCAN_TILE;
+  for (qk_u32 k = threadIdx.x; k < QK_SCAN_TILE; k += QK_WG)
+    if (t0 + k < n) out[t0 + k] += add;
+}
+
+/* ------------------------------------------------------------------ */
+/* hash join producing row pairs (query_exec.c:14541-15358,             */
+/* hash_join_build_count.comp / csr_scatter.comp / hash_join_probe.comp) */
+/* Keys compare as f64 values (u32/i32/u64/i64/f32/f64 columns), -0.0   */
+/* as 0.0, NaN never equal.  The build side goes into an open-addressing */
+/* table of distinct key codes (2^LOG2 >= 2 x build rows, linear        */
+/* probing from fmix64(code)) with a count per key; a scan of the counts */
+/* and a scatter give each key's build positions as one contiguous      */
+/* group, which is then sorted, so a probe row's matches come out in    */
+/* build-position order.                                                */
+/* Common bindings: 2 table keys u64[T], 3 counts u32[T], 5 group starts */
+/* u32[T], 8 groups u32[build rows]; word LOG2.                         */
+/* ------------------------------------------------------------------ */
+#define QK_HJ_NONE 0xffffffffu
+
+__device__ __forceinline__ qk_u32 qk_hj_lookup(const qk_u64* keys, qk_u64 code, qk_u64 mask)
+{
+  qk_u64 s = qk_fmix64(code) & mask;
+  for (qk_u64 i = 0; i <= mask; i++) {
+    const qk_u64 k = keys[s];
+    if (k == code) return (qk_u32)s;
+    if (k == QK_HASH_EMPTY) return QK_HJ_NONE;
+    s = (s + 1) & mask;
+  }
+  return QK_HJ_NONE;
+}
+
+/* hjoin_build: binding 0 build position -> row (u64, optional), 4 slot of
+ * each build position (u32, NONE for a NaN key), 6 build key column; push
+ * 0 = build positions; word KEY = key column type.  Keys start all-ones
+ * (never a code: it is a NaN pattern), counts at zero. */
+extern "C" __global__ void LB k_hjoin_build(QK_Args a)
+{
+  const qk_u64 nb = qk_push(a, 0);
+  const qk_u32 type = a.words[QK_W_KEY] & 0xffu;
+  const qk_u64 mask = (1ULL << a.words[QK_W_LOG2]) - 1;
+  const qk_u64* list = a.ptr[0] ? qk_ptr<qk_u64>(a, 0) : nullptr;
+  qk_u64* keys = qk_wptr<qk_u64>(a, 2);
+  qk_u32* cnt = qk_wptr<qk_u32>(a, 3);
+  qk_u32* slot_of = qk_wptr<qk_u32>(a, 4);
+  for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < nb; i += (qk_u64)gridDim.x * QK_WG) {
+    const double d = qk_load1_f64(a, 0, type, list ? list[i] : i);
+    qk_u32 slot = QK_HJ_NONE;
+    if (d == d) {
+      const qk_u64 code = qk_key_code_f64(d);
+      qk_u64 s = qk_fmix64(code) & mask;
+      for (qk_u64 t = 0; t <= mask; t++) {
+        const qk_u64 prev = atomicCAS((unsigned long long*)&keys[s], (unsigned long long)QK_HASH_EMPTY,
+                                      (unsigned long long)code);
+        if (prev == QK_HASH_EMPTY || prev == code) {
+          atomicAdd(&cnt[s], 1u);
+          slot = (qk_u32)s;
+          break;
+        }
+        s = (s + 1) & mask;
+      }
+    }
+    slot_of[i] = slot;
+  }
+}
+
+/* hjoin_scatter: build position i goes to groups[start[slot] + rank], the
+ * rank from an atomic cursor per slot (binding 7, zeroed). */
+extern "C" __global__ void LB k_hjoin_scatter(QK_Args a)
+{
+  const qk_u64 nb = qk_push(a, 0);
+  const qk_u32* slot_of = qk_ptr<qk_u32>(a, 4);
+  const qk_u32* start = qk_ptr<qk_u32>(a, 5);
+  qk_u32* cursor = qk_wptr<qk_u32>(a, 7);
+  qk_u32* groups = qk_wptr<qk_u32>(a, 8);
+  for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < nb; i += (qk_u64)gridDim.x * QK_WG) {
+    const qk_u32 s = slot_of[i];
+    if (s == QK_HJ_NONE) continue;
+    groups[start[s] + atomicAdd(&cursor[s], 1u)] = (qk_u32)i;
+  }
+}
+
+/* hjoin_sort_small: one thread per slot sorts a group of 2..32 positions
+ * in place (insertion sort); larger groups are listed in binding 9 (u32,
+ * count in binding 10 [0], zeroed) for hjoin_sort_big.  push 0 = T. */
+#define QK_HJ_SMALL 32
+extern "C" __global__ void LB k_hjoin_sort_small(QK_Args a)
+{
+  const qk_u64 T = qk_push(a, 0);
+  const qk_u32* cnt = qk_ptr<qk_u32>(a, 3);
+  const qk_u32* start = qk_ptr<qk_u32>(a, 5);
+  qk_u32* groups = qk_wptr<qk_u32>(a, 8);
+  qk_u32* big = qk_wptr<qk_u32>(a, 9);
+  qk_u32* big_count = qk_wptr<qk_u32>(a, 10);
+  for (qk_u64 s = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; s < T; s += (qk_u64)gridDim.x * QK_WG) {
+    const qk_u32 c = cnt[s];
+    if (c < 2) continue;
+    if (c > QK_HJ_SMALL) {
+      big[atomicAdd(big_count, 1u)] = (qk_u32)s;
+      continue;
+    }
+    qk_u32* g = groups + start[s];
+    qk_u32 v[QK_HJ_SMALL];
+    for (qk_u32 k = 0; k < c; k++) v[k] = g[k];
+    for (qk_u32 k = 1; k < c; k++) {
+      const qk_u32 x = v[k];
+      qk_u32 j = k;
+      while (j > 0 && v[j - 1] > x) {
+        v[j] = v[j - 1];
+        j--;
+      }
+      v[j] = x;
+    }
+    for (qk_u32 k = 0; k < c; k++) g[k] = v[k];
+  }
+}
+
+/* hjoin_sort_big: block b sorts the groups listed at b, b + grid, ... with
+ * a bitonic network in which every compare-exchange puts the smaller value
+ * at the lower index (each merge starts by comparing mirrored positions),
+ * so positions past the group's end act as +infinity and are never read
+ * or written. */
+extern "C" __global__ void LB k_hjoin_sort_big(QK_Args a)
+{
+  const qk_u32* cnt = qk_ptr<qk_u32>(a, 3);
+  const qk_u32* start = qk_ptr<qk_u32>(a, 5);
+  qk_u32* groups = qk_wptr<qk_u32>(a, 8);
+  const qk_u32* big = qk_ptr<qk_u32>(a, 9);
+  const qk_u32 nbig = qk_ptr<qk_u32>(a, 10)[0];
+  for (qk_u32 gi = blockIdx.x; gi < nbig; gi += gridDim.x) {
+    const qk_u32 s = big[gi];
+    const qk_u32 c = cnt[s];
+    qk_u32* g = groups + start[s];
+    qk_u32 N = 1;
+    while (N < c) N <<= 1;
+    for (qk_u32 half = 1; half < N; half <<= 1) {
+      for (qk_u32 dist = half; dist > 0; dist >>= 1) {
+        __syncthreads();
+        for (qk_u32 i = threadIdx.x; i < N; i += QK_WG) {
+          /* first stage of a merge: mirror within the 2*half block */
+          const qk_u32 l = dist == half ? (i ^ (2 * half - 1)) : (i ^ dist);
+          if (l <= i || l >= c) continue;
+          const qk_u32 x = g[i], y = g[l];
+          if (x > y) {
+            g[i] = y;
+            g[l] = x;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+/* hjoin_probe: binding 0 probe position -> row (u64, optional), 1 build
+ * position -> row (u64, optional), 6 probe key column, 9 out records
+ * (u64), 10 per-block totals (mode 0) / first record of each block (mode
+ * 1); push 0 = probe positions, 1 = mode, 2 = record capacity; words KEY
+ * = probe key type, ARGS = join type (0 inner, 1 left, 2 semi, 3 anti).
+ * Each block owns a contiguous range of probe positions.  inner/left
+ * records are (probe position, build row) pairs, a left join's unmatched
+ * row paired with all ones; semi/anti records are probe positions. */
+__device__ __forceinline__ qk_u32 qk_hj_outputs(qk_u32 jt, qk_u32 c)
+{
+  return jt == 0 ? c : (jt == 1 ? (c ? c : 1u) : (jt == 2 ? (c ? 1u : 0u) : (c ? 0u : 1u)));
+}
+
+extern "C" __global__ void LB k_hjoin_probe(QK_Args a)
+{
+  __shared__ qk_u64 s_red[QK_WG / 64];
+  __shared__ qk_u32 s_scan[QK_WG / 64];
+  const qk_u64 np = qk_push(a, 0);
+  const qk_u32 mode = (qk_u32)qk_push(a, 1);
+  const qk_u64 cap = qk_push(a, 2);
+  const qk_u32 type = a.words[QK_W_KEY] & 0xffu;
+  const qk_u32 jt = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u64 mask = (1ULL << a.words[QK_W_LOG2]) - 1;
+  const qk_u64* plist = a.ptr[0] ? qk_ptr<qk_u64>(a, 0) : nullptr;
+  const qk_u64* blist = a.ptr[1] ? qk_ptr<qk_u64>(a, 1) : nullptr;
+  const qk_u64* keys = qk_ptr<qk_u64>(a, 2);
+  const qk_u32* cnt = qk_ptr<qk_u32>(a, 3);
+  const qk_u32* start = qk_ptr<qk_u32>(a, 5);
+  const qk_u32* groups = qk_ptr<qk_u32>(a, 8);
+  qk_u64* out = qk_wptr<qk_u64>(a, 9);
+  qk_u64* blk = qk_wptr<qk_u64>(a, 10);
+  qk_u64 per = (np + gridDim.x - 1) / gridDim.x;
+  per = (per + QK_WG - 1) / QK_WG * QK_WG;
+  const qk_u64 p0 = min(np, per * blockIdx.x), p1 = min(np, p0 + per);
+  qk_u64 total = 0, o = mode ? blk[blockIdx.x] : 0;
+  for (qk_u64 base = p0; base < p1; base += QK_WG) {
+    const qk_u64 p = base + threadIdx.x;
+    qk_u32 slot = QK_HJ_NONE, c = 0;
+    if (p < p1) {
+      const double d = qk_load1_f64(a, 0, type, plist ? plist[p] : p);
+      if (d == d) slot = qk_hj_lookup(keys, qk_key_code_f64(d), mask);
+      c = slot != QK_HJ_NONE ? cnt[slot] : 0u;
+    }
+    const qk_u32 n_out = p < p1 ? qk_hj_outputs(jt, c) : 0u;
+    if (!mode) {
+      total += n_out;
+      continue;
+    }
+    qk_u32 step_total = 0;
+    const qk_u32 pre = qk_block_excl_scan_u32(n_out, s_scan, step_total);
+    const qk_u64 w = o + pre;
+    if (jt >= 2) {
+      if (n_out && w < cap) out[w] = p;
+    } else if (c) {
+      const qk_u32 g0 = start[slot];
+      for (qk_u32 k = 0; k < c && w + k < cap; k++) {
+        const qk_u32 bpos = groups[g0 + k];
+        out[2 * (w + k)] = p;
+        out[2 * (w + k) + 1] = blist ? blist[bpos] : (qk_u64)bpos;
+      }
+    } else if (n_out && w < cap) {
+      out[2 * w] = p;
+      out[2 * w + 1] = ~0ULL;
+    }
+    o += step_total;
+  }
+  if (!mode) {
+    const qk_u64 t = qk_block_sum_u64(total, s_red);
+    if (threadIdx.x == 0) blk[blockIdx.x] = t;
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* generators: splitmix64 addressed by output index                     */
 /* (bench_common.h:30-37): d(k) = mix(seed + (k+1) * 0x9E3779B97F4A7C15) */
 /* ------------------------------------------------------------------ */
@@ -2531,6 +2808,14 @@ static const QK_LibraryKernel g_library[] = {
     {"join_part_probe_1s_row_512", (const void*)k_join_part_probe_1s_row_512, 512},
     {"join_part_probe_1sm_row", (const void*)k_join_part_probe_1sm_row, QK_WG},
     {"join_part_probe_1sm_row_512", (const void*)k_join_part_probe_1sm_row_512, 512},
+    {"scan_u32_blocks", (const void*)k_scan_u32_blocks, QK_WG},
+    {"scan_u32_top", (const void*)k_scan_u32_top, QK_WG},
+    {"scan_u32_add", (const void*)k_scan_u32_add, QK_WG},
+    {"hjoin_build", (const void*)k_hjoin_build, QK_WG},
+    {"hjoin_scatter", (const void*)k_hjoin_scatter, QK_WG},
+    {"hjoin_sort_small", (const void*)k_hjoin_sort_small, QK_WG},
+    {"hjoin_sort_big", (const void*)k_hjoin_sort_big, QK_WG},
+    {"hjoin_probe", (const void*)k_hjoin_probe, QK_WG},
     {"gen_events", (const void*)k_gen_events, QK_WG},
     {"gen_scale_events", (const void*)k_gen_scale_events, QK_WG},
     {"gen_fact_fk", (const void*)k_gen_fact_fk, QK_WG},

@@ -325,6 +325,51 @@ def aggregate_join(fact: Table, fact_key: int, dim: Table, dim_key: int, funcs, 
     return AggResult(funcs, keys[:n].copy(), cells[:n].copy(), tr.as_dict())
 
 
+JOIN_TYPES = {"inner": 0, "left": 1, "semi": 2, "anti": 3}
+BUILD_NULL = (1 << 64) - 1
+
+
+def hash_join(probe: Table, probe_key: int, build: Table, build_key: int, join_type: str = "inner",
+              probe_rows=None, build_rows=None, cap: int | None = None, with_trace: bool = False):
+    """qe_hash_join_impl: the equi-join probe.key = build.key.
+
+    inner / left: an (n, 2) u64 array of (probe position, build row) pairs,
+    a left join's unmatched position paired with BUILD_NULL; semi / anti:
+    the probe positions kept, ascending.  Sorted by probe position, then
+    build position.  probe_rows / build_rows (u64 arrays) restrict the
+    sides; probe positions index probe_rows when it is given."""
+    lib = load()
+    jt = JOIN_TYPES[join_type]
+    keep = []
+
+    def rows_arg(rows):
+        if rows is None:
+            return None, 0
+        arr = np.ascontiguousarray(rows, dtype=np.uint64)
+        keep.append(arr)
+        return arr.ctypes.data, len(arr)
+
+    pr, pc = rows_arg(probe_rows)
+    br, bc = rows_arg(build_rows)
+    rec = 1 if jt >= 2 else 2
+    cap = (pc if probe_rows is not None else probe.rows) if cap is None else cap
+    out = np.empty(max(cap, 1) * rec, dtype=np.uint64)
+    n = C.c_uint64(0)
+    tr = QE_Trace()
+    ok = lib.qe_hash_join(probe.handle, probe_key, pr, pc, build.handle, build_key, br, bc, jt, out.ctypes.data, cap,
+                          C.byref(n), C.byref(tr))
+    if not ok:
+        raise _err("hash_join")
+    total = int(n.value)
+    if total > cap:
+        return hash_join(probe, probe_key, build, build_key, join_type, probe_rows, build_rows, cap=total,
+                         with_trace=with_trace)
+    res = out[: total * rec].copy()
+    if rec == 2:
+        res = res.reshape(total, 2)
+    return (res, tr.as_dict()) if with_trace else res
+
+
 def bandwidth_read(table: Table, column: int, blocks: int = 0) -> dict:
     lib = load()
     buf = lib.qe_table_column_buffer(table.handle, column)

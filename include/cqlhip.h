@@ -362,6 +362,32 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
                       const U32* funcs, const S32* fact_arg_cols, U32 expr_count, U64 cap,
                       U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace);
 
+/* Join types of qe_hash_join (the reference's join_type strings "inner",
+ * "left", "semi", "anti", query_exec.c:14613-14650). */
+enum {
+  QE_JoinType_Inner = 0,
+  QE_JoinType_Left = 1,
+  QE_JoinType_Semi = 2,
+  QE_JoinType_Anti = 3,
+};
+
+/* qe_hash_join_impl (query_exec.c:14541-15358) for numeric keys: the
+ * equi-join probe.key = build.key producing row pairs.  Keys compare as
+ * f64 values (-0.0 equal to 0.0, NaN equal to nothing).  probe_rows /
+ * build_rows list the rows taking part (NULL = every row of the table;
+ * probe positions index probe_rows, or are rows when it is NULL).
+ *   inner / left: out holds (probe position, build row) u64 pairs, a left
+ *     join's unmatched probe position paired with ~0 (the reference's
+ *     BUILD_NULL); every build row with the key fans out.
+ *   semi / anti: out holds the probe positions with / without a match, once.
+ * Records are sorted by probe position, then by build position.  At most
+ * cap records are written; *out_count is the full count.  Returns 0 with
+ * qe_last_error() on failure (non-numeric keys, rows out of range, more
+ * than 2^31 build rows or 2^32 records). */
+U32 qe_hash_join(QE_Table* probe, U32 probe_key_col, const U64* probe_rows, U64 probe_count,
+                 QE_Table* build, U32 build_key_col, const U64* build_rows, U64 build_count,
+                 U32 join_type, U64* out, U64 cap, U64* out_count, QE_Trace* trace);
+
 /* Batched shared scan.  No counterpart in the reference, whose fused
  * kernels serve one query each (query_kernel.c:1433-1600): many aggregate
  * queries over the same table run as one pass over the union of their

@@ -935,3 +935,88 @@ void orc_dense_group_u32(const orc_u32* key, const orc_u32* arg, orc_u64 rows, o
   free(c.mn);
   free(c.mx);
 }
+
+/* ------------------------------------------------------------------ */
+/* hash join producing row pairs                                        */
+/* ------------------------------------------------------------------ */
+/* qe_hash_join_impl, query_exec.c:14541-15358 with hash_join_probe.comp:
+ * numeric keys are compared as f64 values (the key columns are converted
+ * to f64, :14704-14725), every build row is kept so duplicate build keys
+ * fan out, a LEFT join pairs an unmatched probe row with all ones
+ * (BUILD_NULL), SEMI / ANTI keep the probe positions with / without a
+ * match, once each, in probe order (qe_join_filter_left_rows,
+ * :14470-14520).  NaN keys match nothing (f64 ==), -0.0 matches 0.0.
+ * Records are written in probe-position order and, within one probe row,
+ * build-position order (the reference's own order is that of its atomics).
+ * join_type: 0 inner, 1 left, 2 semi, 3 anti.  Returns the record count;
+ * writes at most cap records (pairs for inner/left, positions otherwise). */
+orc_u64 orc_hash_join(const orc_column* probe_key, const orc_u64* probe_rows, orc_u64 np,
+                      const orc_column* build_key, const orc_u64* build_rows, orc_u64 nb, orc_u32 join_type,
+                      orc_u64* out, orc_u64 cap)
+{
+  orc_u64 size = 16;
+  while (size < 2 * nb) size <<= 1;
+  /* distinct keys: first and last build position of each, chained through
+   * next[] in ascending position order */
+  orc_f64* skey = (orc_f64*)malloc(size * sizeof(orc_f64));
+  orc_u64* first = (orc_u64*)malloc(size * sizeof(orc_u64));
+  orc_u64* last = (orc_u64*)malloc(size * sizeof(orc_u64));
+  orc_u64* next = (orc_u64*)malloc((nb ? nb : 1) * sizeof(orc_u64));
+  for (orc_u64 i = 0; i < size; i++) first[i] = ~0ULL;
+  for (orc_u64 i = 0; i < nb; i++) {
+    orc_f64 v = orc_load(build_key, build_rows ? build_rows[i] : i);
+    next[i] = ~0ULL;
+    if (v != v) continue;
+    orc_f64 kv = v;
+    orc_u64 h = orc_hash_keys((const orc_f64* const[]){&kv}, 1, 0) & (size - 1);
+    while (first[h] != ~0ULL && skey[h] != v) h = (h + 1) & (size - 1);
+    if (first[h] == ~0ULL) {
+      first[h] = last[h] = i;
+      skey[h] = v;
+    } else {
+      next[last[h]] = i;
+      last[h] = i;
+    }
+  }
+  orc_u64 total = 0;
+  for (orc_u64 p = 0; p < np; p++) {
+    orc_f64 v = orc_load(probe_key, probe_rows ? probe_rows[p] : p);
+    orc_u64 chain = ~0ULL;
+    if (v == v) {
+      orc_f64 kv = v;
+      orc_u64 h = orc_hash_keys((const orc_f64* const[]){&kv}, 1, 0) & (size - 1);
+      while (first[h] != ~0ULL && skey[h] != v) h = (h + 1) & (size - 1);
+      chain = first[h];
+    }
+    if (join_type >= 2) {
+      int keep = join_type == 2 ? chain != ~0ULL : chain == ~0ULL;
+      if (keep) {
+        if (total < cap) out[total] = p;
+        total++;
+      }
+      continue;
+    }
+    if (chain == ~0ULL) {
+      if (join_type == 1) {
+        if (total < cap) {
+          out[2 * total] = p;
+          out[2 * total + 1] = ~0ULL;
+        }
+        total++;
+      }
+      continue;
+    }
+    for (orc_u64 b = chain; b != ~0ULL; b = next[b]) {
+      if (total < cap) {
+        out[2 * total] = p;
+        out[2 * total + 1] = build_rows ? build_rows[b] : b;
+      }
+      total++;
+    }
+  }
+  free(skey);
+  free(first);
+  free(last);
+  free(next);
+  return total;
+}

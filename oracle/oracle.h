@@ -148,6 +148,9 @@ orc_u64 orc_join_aggregate(orc_u64 fact_rows, const orc_u32* fact_key, const orc
 /* multithreaded numeric-AND bitmask count (qe_scan_filter_chunked_cpu,
  * query_exec.c:1240-1367, count_only) for leaves col op target (op 0..4).
  * Used as the timed CPU baseline of a filter count. */
+orc_u64 orc_hash_join(const orc_column* probe_key, const orc_u64* probe_rows, orc_u64 np,
+                      const orc_column* build_key, const orc_u64* build_rows, orc_u64 nb, orc_u32 join_type,
+                      orc_u64* out, orc_u64 cap);
 orc_u64 orc_scan_chunked_count(const orc_column* cols, const orc_u32* leaf_col, const orc_u32* leaf_op,
                                const orc_f64* leaf_target, orc_u32 leaf_count, orc_u64 rows, int workers);
 

@@ -60,6 +60,7 @@ def load():
         "orc_group_aggregate": (u64, [P, u64, P, u32, i32, P, P, u32, P, P, u64]),
         "orc_join_aggregate": (u64, [u64, P, P, P, u32, u64, P, P, P, P, P, u64, i32]),
         "orc_scan_chunked_count": (u64, [P, P, P, P, u32, u64, i32]),
+        "orc_hash_join": (u64, [P, P, u64, P, P, u64, u32, P, u64]),
         "orc_dense_group_u32": (None, [P, P, u64, u32, u32, i32, P, P, P, P]),
         "orc_hw_threads": (i32, []),
     }
@@ -196,6 +197,29 @@ def join_aggregate(fact_key, fact_args, funcs, dim_key, dim_pass=None, dim_group
                                   None if dp is None else dp.ctypes.data, None if dg is None else dg.ctypes.data,
                                   keys.ctypes.data, res.ctypes.data, max_groups, nthreads or threads())
     return keys[:g].copy(), res[:g].copy()
+
+
+JOIN_TYPES = {"inner": 0, "left": 1, "semi": 2, "anti": 3}
+
+
+def hash_join(probe_key, build_key, join_type="inner", probe_rows=None, build_rows=None):
+    """Row-pair equi-join: (n, 2) u64 (probe position, build row) pairs for
+    inner/left, u64 probe positions for semi/anti (orc_hash_join)."""
+    jt = JOIN_TYPES[join_type]
+    cols, keep = _columns([probe_key, build_key])
+    pr = None if probe_rows is None else np.ascontiguousarray(probe_rows, dtype=np.uint64)
+    br = None if build_rows is None else np.ascontiguousarray(build_rows, dtype=np.uint64)
+    npr = len(pr) if pr is not None else len(probe_key)
+    nbr = len(br) if br is not None else len(build_key)
+    lib = load()
+    args = (C.byref(cols[0]), None if pr is None else pr.ctypes.data, npr, C.byref(cols[1]),
+            None if br is None else br.ctypes.data, nbr, jt)
+    n = lib.orc_hash_join(*args, None, 0)
+    rec = 1 if jt >= 2 else 2
+    out = np.empty(max(n, 1) * rec, dtype=np.uint64)
+    lib.orc_hash_join(*args, out.ctypes.data, n)
+    out = out[: n * rec]
+    return out.reshape(n, 2) if rec == 2 else out
 
 
 def scan_chunked_count(cols, leaves, workers=None):
