@@ -2377,6 +2377,34 @@ extern "C" __global__ void LB k_shared_reduce(QK_Args a)
 }
 
 /* ------------------------------------------------------------------ */
+/* gather: column values through row ids (selection_gather.comp modes   */
+/* F32/F64): out_c[i] = column c at rows[i * STRIDE + OFFSET], 0 where  */
+/* the id is all ones (no row).  binding 0 row ids (u64), 6..9 columns, */
+/* 10..13 outputs (the column's width); push 0 count, 1 stride, 2       */
+/* offset (u64 units); words ARGS = columns (1..4), TYPES = widths,     */
+/* 8 bits each (4 or 8 bytes).                                          */
+/* ------------------------------------------------------------------ */
+extern "C" __global__ void LB k_gather(QK_Args a)
+{
+  const qk_u64 n = qk_push(a, 0), stride = qk_push(a, 1), offset = qk_push(a, 2);
+  const qk_u32 ncols = a.words[QK_W_ARGS] & 0xffu;
+  const qk_u32 widths = a.words[QK_W_TYPES];
+  const qk_u64* rows = qk_ptr<qk_u64>(a, 0);
+  for (qk_u64 i = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; i < n; i += (qk_u64)gridDim.x * QK_WG) {
+    const qk_u64 id = rows[i * stride + offset];
+#pragma unroll
+    for (qk_u32 c = 0; c < 4; c++) {
+      if (c >= ncols) break;
+      if (((widths >> (8 * c)) & 0xffu) == 8) {
+        qk_wptr<qk_u64>(a, 10 + c)[i] = id == ~0ULL ? 0ULL : qk_ptr<qk_u64>(a, QK_B_COL0 + c)[id];
+      } else {
+        qk_wptr<qk_u32>(a, 10 + c)[i] = id == ~0ULL ? 0u : qk_ptr<qk_u32>(a, QK_B_COL0 + c)[id];
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* exclusive scan of u32 counts (any length): scan_u32_blocks scans each */
 /* 2048-element tile and leaves its total, scan_u32_top scans the tile  */
 /* totals in one block, scan_u32_add adds them back.  binding 0 in, 1   */
@@ -2808,6 +2836,7 @@ static const QK_LibraryKernel g_library[] = {
     {"join_part_probe_1s_row_512", (const void*)k_join_part_probe_1s_row_512, 512},
     {"join_part_probe_1sm_row", (const void*)k_join_part_probe_1sm_row, QK_WG},
     {"join_part_probe_1sm_row_512", (const void*)k_join_part_probe_1sm_row_512, 512},
+    {"gather", (const void*)k_gather, QK_WG},
     {"scan_u32_blocks", (const void*)k_scan_u32_blocks, QK_WG},
     {"scan_u32_top", (const void*)k_scan_u32_top, QK_WG},
     {"scan_u32_add", (const void*)k_scan_u32_add, QK_WG},

@@ -178,6 +178,39 @@ class Table:
     def count(self, where) -> int:
         return self.scan(where, count_only=True)[0]
 
+    def _col_index(self, c):
+        return self.index(c) if isinstance(c, str) else int(c)
+
+    def gather(self, columns, rows, stride: int = 1, offset: int = 0) -> list:
+        """qe_gather: the values of `columns` at rows[i * stride + offset]
+        (0 where the id is all ones), one array per column."""
+        cols = [self._col_index(c) for c in columns]
+        rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+        n = (len(rows) - offset + stride - 1) // stride if len(rows) > offset else 0
+        outs = [np.empty(max(n, 1), dtype=_NP[self.types[c]]) for c in cols]
+        ca = (C.c_uint32 * max(len(cols), 1))(*cols)
+        pa = (C.c_void_p * max(len(cols), 1))(*[o.ctypes.data for o in outs])
+        if not self._lib.qe_gather(self.handle, ca, len(cols), rows.ctypes.data, n, stride, offset, pa):
+            raise _err("gather")
+        return [o[:n] for o in outs]
+
+    def select(self, columns, where=None, cap: int | None = None) -> list:
+        """qe_scan_materialize: SELECT columns WHERE ..., in row order."""
+        cols = [self._col_index(c) for c in columns]
+        nodes = ir.to_nodes(where)
+        cap = self.rows if cap is None else cap
+        outs = [np.empty(max(cap, 1), dtype=_NP[self.types[c]]) for c in cols]
+        ca = (C.c_uint32 * max(len(cols), 1))(*cols)
+        pa = (C.c_void_p * max(len(cols), 1))(*[o.ctypes.data for o in outs])
+        cnt = C.c_uint64(0)
+        ok = self._lib.qe_scan_materialize(self.handle, nodes.ctypes.data if nodes is not None else None,
+                                           len(nodes) if nodes is not None else 0, ca, len(cols), cap, C.byref(cnt),
+                                           pa)
+        if not ok:
+            raise _err("select")
+        n = min(int(cnt.value), cap)
+        return [o[:n] for o in outs]
+
     def aggregate(self, funcs, args, where=None, group: int | None = None, cap: int | None = None) -> AggResult:
         """qe_aggregate_impl for the hot-path shapes (args[e] < 0 or None = COUNT(*))."""
         funcs = list(funcs)

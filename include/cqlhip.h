@@ -388,6 +388,22 @@ U32 qe_hash_join(QE_Table* probe, U32 probe_key_col, const U64* probe_rows, U64 
                  QE_Table* build, U32 build_key_col, const U64* build_rows, U64 build_count,
                  U32 join_type, U64* out, U64 cap, U64* out_count, QE_Trace* trace);
 
+/* Row materialisation (selection_gather.comp modes F32/F64,
+ * qe_selection_gather_column): out[c][i] = column cols[c] at row
+ * rows[i * stride + offset], at the column's own width (4 or 8 bytes), 0
+ * where that id is ~0 (no row, e.g. a left join's unmatched side).  With
+ * stride 2 the (probe, build) pairs of qe_hash_join serve as rows
+ * directly (offset 0 probe side, 1 build side). */
+U32 qe_gather(QE_Table* table, const U32* cols, U32 col_count, const U64* rows, U64 count, U64 stride, U64 offset,
+              void* const* out);
+
+/* SELECT cols FROM table WHERE ...: the matching rows' values, in row
+ * order, with the row ids kept on the device between the scan and the
+ * gather (the reference's device selections, query_exec.h:383-415).  At
+ * most cap rows are written per column; *out_count is the full count. */
+U32 qe_scan_materialize(QE_Table* table, const QE_IRNode* where, U32 where_count, const U32* cols, U32 col_count,
+                        U64 cap, U64* out_count, void* const* out);
+
 /* Batched shared scan.  No counterpart in the reference, whose fused
  * kernels serve one query each (query_kernel.c:1433-1600): many aggregate
  * queries over the same table run as one pass over the union of their
