@@ -65,6 +65,26 @@ struct PendingRead {
   U64 size;
 };
 
+/* a validated launch: kernel, args snapshot, grid */
+struct PreparedLaunch {
+  GPU_Kernel* k;
+  QK_Args args;
+  U32 blocks;
+  U32 block;
+  U32 lds;
+};
+
+/* one operation of a deferred (graph) batch; writes and reads go through
+ * pinned staging so a captured graph's copies have fixed host addresses */
+struct BatchOp {
+  int kind; /* 0 write, 1 read, 2 kernel */
+  void* dev;
+  U64 size;
+  char* staged;
+  void* user;
+  PreparedLaunch launch;
+};
+
 struct GPU_Batch {
   hipStream_t stream;
   bool failed;
@@ -72,6 +92,12 @@ struct GPU_Batch {
   std::vector<char*> blocks; /* pinned staging blocks owned by this batch */
   U64 block_used;
   std::vector<PendingRead> reads;
+  /* deferred batches (gpu_hip_batch_begin_graph): operations are recorded
+   * and run at batch_end as a cached HIP graph */
+  bool deferred = false;
+  std::string key;
+  std::vector<BatchOp> ops;
+  std::vector<char*> big_blocks; /* staging larger than one block */
 };
 
 struct GPU_StreamSlot {
@@ -196,9 +222,10 @@ void flush_profile(std::vector<PendingProfile>& prof)
   prof.clear();
 }
 
-bool launch(GPU_Kernel* k, U32 global_work_size, U32 local_work_size, hipStream_t stream,
-            std::vector<PendingProfile>* prof)
+/* validate a dispatch and snapshot its args; *empty when the grid is 0 */
+bool prepare(GPU_Kernel* k, U32 global_work_size, U32 local_work_size, PreparedLaunch& out, bool& empty)
 {
+  empty = false;
   if (!k) {
     set_error("dispatch of a null kernel");
     return false;
@@ -215,14 +242,39 @@ bool launch(GPU_Kernel* k, U32 global_work_size, U32 local_work_size, hipStream_
     return false;
   }
   U64 blocks = ((U64)global_work_size + block - 1) / block;
-  if (blocks == 0) return true;
+  if (blocks == 0) {
+    empty = true;
+    return true;
+  }
   if (blocks > 0x7fffffffULL) {
     set_error("%s: grid too large", k->name.c_str());
     return false;
   }
-  QK_Args args = k->args; /* snapshot */
-  args.global_work_size = global_work_size;
-  args.grid_blocks = (U32)blocks;
+  out.k = k;
+  out.args = k->args; /* snapshot */
+  out.args.global_work_size = global_work_size;
+  out.args.grid_blocks = (U32)blocks;
+  out.blocks = (U32)blocks;
+  out.block = block;
+  out.lds = k->lds_bytes;
+  return true;
+}
+
+hipError_t enqueue(const PreparedLaunch& p, hipStream_t stream)
+{
+  QK_Args args = p.args;
+  void* params[1] = {&args};
+  if (p.k->lib) return hipLaunchKernel(p.k->lib->host_fn, dim3(p.blocks), dim3(p.block), params, p.lds, stream);
+  return hipModuleLaunchKernel(p.k->fn, p.blocks, 1, 1, p.block, 1, 1, p.lds, stream, params, nullptr);
+}
+
+bool launch(GPU_Kernel* k, U32 global_work_size, U32 local_work_size, hipStream_t stream,
+            std::vector<PendingProfile>* prof)
+{
+  PreparedLaunch p;
+  bool empty = false;
+  if (!prepare(k, global_work_size, local_work_size, p, empty)) return false;
+  if (empty) return true;
   PendingProfile pp;
   bool timed = st().profile && prof;
   if (timed) {
@@ -231,14 +283,7 @@ bool launch(GPU_Kernel* k, U32 global_work_size, U32 local_work_size, hipStream_
     hipEventCreate(&pp.stop);
     hipEventRecord(pp.start, stream);
   }
-  hipError_t e;
-  if (k->lib) {
-    void* params[1] = {&args};
-    e = hipLaunchKernel(k->lib->host_fn, dim3((U32)blocks), dim3(block), params, k->lds_bytes, stream);
-  } else {
-    void* params[1] = {&args};
-    e = hipModuleLaunchKernel(k->fn, (U32)blocks, 1, 1, block, 1, 1, k->lds_bytes, stream, params, nullptr);
-  }
+  hipError_t e = enqueue(p, stream);
   if (timed) {
     hipEventRecord(pp.stop, stream);
     prof->push_back(pp);
@@ -698,6 +743,16 @@ std::vector<char*> g_staging_free;
 char* staging_take(GPU_Batch* b, U64 size)
 {
   size = (size + 63) & ~(U64)63;
+  if (size > kStagingBlock) {
+    /* larger than a staging block: a pinned block of its own */
+    char* big = nullptr;
+    if (hipHostMalloc((void**)&big, size, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    b->big_blocks.push_back(big);
+    return big;
+  }
   if (b->blocks.empty() || b->block_used + size > kStagingBlock) {
     char* blk = nullptr;
     {
@@ -719,11 +774,77 @@ char* staging_take(GPU_Batch* b, U64 size)
   return p;
 }
 
-void staging_return(GPU_Batch* b)
+void staging_return_blocks(std::vector<char*>& blocks, std::vector<char*>& big)
 {
-  std::lock_guard<std::mutex> g(g_staging_lock);
-  for (char* blk : b->blocks) g_staging_free.push_back(blk);
-  b->blocks.clear();
+  {
+    std::lock_guard<std::mutex> g(g_staging_lock);
+    for (char* blk : blocks) g_staging_free.push_back(blk);
+  }
+  blocks.clear();
+  for (char* p : big) hipHostFree(p);
+  big.clear();
+}
+
+void staging_return(GPU_Batch* b) { staging_return_blocks(b->blocks, b->big_blocks); }
+
+/* HIP graphs of deferred batches, per host thread (a graph's staging is
+ * its own, so concurrent threads never share one), least recently used
+ * evicted past kMaxGraphs */
+struct GraphEntry {
+  std::string key;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  std::vector<BatchOp> ops;
+  std::vector<char*> blocks, big_blocks;
+  U64 tick;
+};
+const size_t kMaxGraphs = 16;
+thread_local std::vector<GraphEntry>* t_graphs;
+thread_local U64 t_graph_tick;
+std::atomic<U64> g_graph_replays{0}, g_graph_captures{0};
+
+std::vector<GraphEntry>& graphs()
+{
+  if (!t_graphs) t_graphs = new std::vector<GraphEntry>();
+  return *t_graphs;
+}
+
+void graph_destroy(GraphEntry& e)
+{
+  hipGraphExecDestroy(e.exec);
+  hipGraphDestroy(e.graph);
+  staging_return_blocks(e.blocks, e.big_blocks);
+}
+
+/* the same operations up to the bytes written: same buffers, sizes,
+ * kernels, args and grids */
+bool ops_match(const std::vector<BatchOp>& a, const std::vector<BatchOp>& b)
+{
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); i++) {
+    const BatchOp &x = a[i], &y = b[i];
+    if (x.kind != y.kind) return false;
+    if (x.kind != 2) {
+      if (x.dev != y.dev || x.size != y.size) return false;
+    } else if (x.launch.k != y.launch.k || x.launch.blocks != y.launch.blocks || x.launch.block != y.launch.block ||
+               x.launch.lds != y.launch.lds || memcmp(&x.launch.args, &y.launch.args, sizeof(QK_Args)) != 0) {
+      return false;
+    }
+  }
+  return true;
+}
+
+/* issue the recorded operations on a stream */
+bool issue_ops(const std::vector<BatchOp>& ops, hipStream_t stream)
+{
+  for (const BatchOp& op : ops) {
+    hipError_t e;
+    if (op.kind == 0) e = hipMemcpyAsync(op.dev, op.staged, op.size, hipMemcpyHostToDevice, stream);
+    else if (op.kind == 1) e = hipMemcpyAsync(op.staged, op.dev, op.size, hipMemcpyDeviceToHost, stream);
+    else e = enqueue(op.launch, stream);
+    if (!check(e, op.kind == 2 ? op.launch.k->name.c_str() : "graph batch copy")) return false;
+  }
+  return true;
 }
 }  // namespace
 
@@ -731,6 +852,16 @@ void gpu_hip_batch_buffer_write(GPU_Batch* batch, GPU_Buffer* buffer, void* data
 {
   if (!batch || !buffer || !data) return;
   if (size > buffer->alloc_size) size = buffer->alloc_size;
+  if (batch->deferred) {
+    char* p = staging_take(batch, size);
+    if (!p) {
+      batch->failed = true;
+      return;
+    }
+    memcpy(p, data, size);
+    batch->ops.push_back(BatchOp{0, buffer->ptr, size, p, nullptr, {}});
+    return;
+  }
   const void* src = data;
   if (size <= kStagingMax) {
     char* p = staging_take(batch, size);
@@ -764,7 +895,14 @@ void gpu_hip_batch_buffer_fill(GPU_Batch* batch, GPU_Buffer* buffer, U64 size, U
   U64 cap = (U64)st().props.multiProcessorCount * 8 * 256;
   if (threads > cap) threads = cap;
   if (threads == 0) threads = 1;
-  if (!launch(k, (U32)threads, 256, batch->stream, &batch->prof)) batch->failed = true;
+  if (batch->deferred) {
+    PreparedLaunch pl;
+    bool empty = false;
+    if (!prepare(k, (U32)threads, 256, pl, empty)) batch->failed = true;
+    else if (!empty) batch->ops.push_back(BatchOp{2, nullptr, 0, nullptr, nullptr, pl});
+  } else if (!launch(k, (U32)threads, 256, batch->stream, &batch->prof)) {
+    batch->failed = true;
+  }
   k->args = saved;
   k->missing_mask = saved_missing;
 }
@@ -777,6 +915,13 @@ void gpu_hip_batch_buffer_zero(GPU_Batch* batch, GPU_Buffer* buffer, U64 size)
 void gpu_hip_batch_kernel_execute(GPU_Batch* batch, GPU_Kernel* kernel, U32 global_work_size, U32 local_work_size)
 {
   if (!batch) return;
+  if (batch->deferred) {
+    PreparedLaunch pl;
+    bool empty = false;
+    if (!prepare(kernel, global_work_size, local_work_size, pl, empty)) batch->failed = true;
+    else if (!empty) batch->ops.push_back(BatchOp{2, nullptr, 0, nullptr, nullptr, pl});
+    return;
+  }
   if (!launch(kernel, global_work_size, local_work_size, batch->stream, &batch->prof)) batch->failed = true;
 }
 
@@ -784,6 +929,15 @@ void gpu_hip_batch_buffer_read(GPU_Batch* batch, GPU_Buffer* buffer, void* out_d
 {
   if (!batch || !buffer || !out_data) return;
   if (size > buffer->alloc_size) size = buffer->alloc_size;
+  if (batch->deferred) {
+    char* p = staging_take(batch, size);
+    if (!p) {
+      batch->failed = true;
+      return;
+    }
+    batch->ops.push_back(BatchOp{1, buffer->ptr, size, p, out_data, {}});
+    return;
+  }
   void* dst = out_data;
   if (size <= kStagingMax) {
     char* p = staging_take(batch, size);
@@ -796,9 +950,117 @@ void gpu_hip_batch_buffer_read(GPU_Batch* batch, GPU_Buffer* buffer, void* out_d
     batch->failed = true;
 }
 
+/* a deferred batch: replay the thread's graph for this key when the
+ * recorded operations match it (only the bytes written may differ), else
+ * capture the operations into a new graph; either way the work runs as one
+ * graph launch */
+static B32 batch_end_deferred(GPU_Batch* batch)
+{
+  bool ok = !batch->failed;
+  std::vector<GraphEntry>& gs = graphs();
+  GraphEntry* hit = nullptr;
+  for (auto& e : gs)
+    if (e.key == batch->key) hit = &e;
+  const std::vector<BatchOp>* src = &batch->ops;
+  PendingProfile pp;
+  bool timed = st().profile;
+  if (timed) {
+    pp.name = "graph:" + batch->key.substr(0, batch->key.find(':', 6));
+    hipEventCreate(&pp.start);
+    hipEventCreate(&pp.stop);
+  }
+  if (ok && hit && ops_match(hit->ops, batch->ops)) {
+    for (size_t i = 0; i < batch->ops.size(); i++)
+      if (batch->ops[i].kind == 0) memcpy(hit->ops[i].staged, batch->ops[i].staged, batch->ops[i].size);
+    if (timed) hipEventRecord(pp.start, batch->stream);
+    ok = check(hipGraphLaunch(hit->exec, batch->stream), "hipGraphLaunch");
+    if (timed) hipEventRecord(pp.stop, batch->stream);
+    hit->tick = ++t_graph_tick;
+    src = &hit->ops;
+    g_graph_replays++;
+  } else if (ok) {
+    if (hit) {
+      graph_destroy(*hit);
+      gs.erase(gs.begin() + (hit - gs.data()));
+      hit = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ok = check(hipStreamBeginCapture(batch->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (ok) {
+      bool issued = issue_ops(batch->ops, batch->stream);
+      ok = check(hipStreamEndCapture(batch->stream, &graph), "hipStreamEndCapture") && issued;
+    }
+    if (ok) ok = check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (ok) {
+      if (timed) hipEventRecord(pp.start, batch->stream);
+      ok = check(hipGraphLaunch(exec, batch->stream), "hipGraphLaunch");
+      if (timed) hipEventRecord(pp.stop, batch->stream);
+    }
+    if (ok) {
+      if (gs.size() >= kMaxGraphs) {
+        size_t lru = 0;
+        for (size_t i = 1; i < gs.size(); i++)
+          if (gs[i].tick < gs[lru].tick) lru = i;
+        graph_destroy(gs[lru]);
+        gs.erase(gs.begin() + lru);
+      }
+      GraphEntry e;
+      e.key = batch->key;
+      e.graph = graph;
+      e.exec = exec;
+      e.ops = batch->ops;
+      e.blocks.swap(batch->blocks);
+      e.big_blocks.swap(batch->big_blocks);
+      e.tick = ++t_graph_tick;
+      gs.push_back(e);
+      src = &gs.back().ops;
+      g_graph_captures++;
+    } else {
+      if (exec) hipGraphExecDestroy(exec);
+      if (graph) hipGraphDestroy(graph);
+    }
+  }
+  ok = check(hipStreamSynchronize(batch->stream), "batch_end") && ok;
+  for (size_t i = 0; i < batch->ops.size(); i++) {
+    const BatchOp& op = batch->ops[i];
+    if (op.kind != 1) continue;
+    if (ok) memcpy(op.user, (*src)[i].staged, op.size);
+    else memset(op.user, 0, op.size);
+  }
+  if (timed) {
+    if (ok) batch->prof.push_back(pp);
+    else {
+      hipEventDestroy(pp.start);
+      hipEventDestroy(pp.stop);
+    }
+  }
+  staging_return(batch);
+  flush_profile(batch->prof);
+  give_stream(batch->stream);
+  delete batch;
+  return ok ? 1 : 0;
+}
+
+GPU_Batch* gpu_hip_batch_begin_graph(const char* key)
+{
+  GPU_Batch* b = gpu_hip_batch_begin(0, 0);
+  if (!b) return nullptr;
+  b->deferred = true;
+  b->key = key ? key : "";
+  return b;
+}
+
+void gpu_hip_graph_stats(U64* replays, U64* captures)
+{
+  if (replays) *replays = g_graph_replays.load();
+  if (captures) *captures = g_graph_captures.load();
+}
+
 B32 gpu_hip_batch_end(GPU_Batch* batch)
 {
   if (!batch) return 0;
+  if (batch->deferred) return batch_end_deferred(batch);
   bool ok = check(hipStreamSynchronize(batch->stream), "batch_end") && !batch->failed;
   for (auto& r : batch->reads) {
     if (ok) memcpy(r.user, r.pinned, r.size);

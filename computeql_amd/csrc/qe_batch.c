@@ -299,7 +299,18 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
     gpu_kernel_set_arg_buffer(kr, 3, res);
     gpu_kernel_set_arg_u64(kr, 0, blocks);
     U32 rblocks = (W + QE_WG / 64 - 1) / (QE_WG / 64); /* one wave per word */
-    GPU_Batch* b = gpu_batch_begin(0, (U64)W * 8);
+    /* QE_BATCH_GRAPH (default 1): the launch runs as a HIP graph that the
+     * next batch of the same shape over the same table replays (only the
+     * program bytes change) */
+    GPU_Batch* b;
+    if (qe_settings_u64("QE_BATCH_GRAPH", 1)) {
+      char key[160];
+      snprintf(key, sizeof(key), "batch:%s:%p:%u:%u:%u:%llu", kname, (void*)t, words, W, buffers,
+               (unsigned long long)blocks);
+      b = gpu_hip_batch_begin_graph(key);
+    } else {
+      b = gpu_batch_begin(0, (U64)W * 8);
+    }
     gpu_batch_buffer_write(b, pb, prog, (U64)words * 4);
     gpu_batch_buffer_write(b, cb, colptr, (U64)t->column_count * 8);
     gpu_batch_kernel_execute(b, k, (U32)(blocks * wg), wg);

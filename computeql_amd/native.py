@@ -155,6 +155,8 @@ SIGNATURES = [
     ("gpu_hip_profile_reset", None, []),
     ("gpu_hip_profile_query", U64, [C.c_char_p, P]),
     ("gpu_hip_profile_tag", None, [C.c_char_p]),
+    ("gpu_hip_batch_begin_graph", P, [C.c_char_p]),
+    ("gpu_hip_graph_stats", None, [P, P]),
     ("gpu_hip_fault_alloc_at", None, [U64]),
     ("gpu_hip_last_error", C.c_char_p, []),
     ("qe_settings_set_u64", None, [C.c_char_p, U64]),

@@ -206,6 +206,17 @@ U64 gpu_hip_profile_query(const char* name, U64* out_dispatches);
 /* prefix "tag:" to the profile names of this thread's next dispatches
  * ("" clears), so one kernel used by several operators is told apart */
 void gpu_hip_profile_tag(const char* tag);
+/* A batch run as a HIP graph (the reference submits each batch as a
+ * recorded Vulkan command buffer, gpu_vulkan.c batch_begin/end; graphs
+ * have no counterpart there).  The batch's operations are recorded and,
+ * at gpu_batch_end, replayed as the calling thread's graph for `key` when
+ * they match it (same buffers, sizes, kernels, args and grids; only the
+ * bytes written may differ), else captured into a new graph for `key`.
+ * Writes and reads go through pinned staging that the graph owns.  Graphs
+ * are per host thread, at most 16, least recently used evicted. */
+GPU_Batch* gpu_hip_batch_begin_graph(const char* key);
+/* graph replays and captures since the process started */
+void gpu_hip_graph_stats(U64* replays, U64* captures);
 /* fault injection: fail the k-th device allocation from now (gpu.h:68) */
 void gpu_hip_fault_alloc_at(U64 k);
 const char* gpu_hip_last_error(void);

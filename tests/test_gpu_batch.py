@@ -140,3 +140,40 @@ def test_autotuner_records_and_applies(events, tmp_path, monkeypatch):
     assert tr["tuned_wg"] == best["wg"] and tr["tuned_buffers"] == best["buffers"]
     for i, (sp, a, b) in enumerate(zip(specs, base, got)):
         _same(sp, b.keys, _vals(b), a.keys, _vals(a), f"q{i}")
+
+
+def _graph_specs(r):
+    """A batch whose shape is fixed and whose literals change with r."""
+    return [
+        cq.AggSpec([cq.COUNT, cq.SUM], [None, wl.U0], (col(wl.REGION) < 5 + r) & (col(wl.STATUS) == r % 5), wl.STATUS),
+        cq.AggSpec([cq.COUNT, cq.MIN, cq.MAX], [None, wl.M0, wl.M0], col(wl.U0 + 1) < 1000 * r + 7),
+        cq.AggSpec([cq.SUM, cq.AVG], [wl.M0 + 1, wl.U0 + 2], col(wl.REGION) == 7 + r, wl.REGION),
+        cq.AggSpec([cq.COUNT], [None], (col(wl.STATUS) >= r % 3) & (col(wl.U0 + 3) > 123 * r)),
+    ]
+
+
+def test_graph_replays_match_eager(events):
+    """The batch runs as a HIP graph: the first round captures it, the later
+    rounds (new literals, same shape) replay it, with eager batches and
+    plain queries run in between; every replay equals the eager result
+    bit for bit (the same kernels and args run in both)."""
+    t, _ = events
+    lib = cq.load()
+    import ctypes as C
+    r0, c0 = C.c_uint64(0), C.c_uint64(0)
+    lib.gpu_hip_graph_stats(C.byref(r0), C.byref(c0))
+    for r in range(4):
+        specs = _graph_specs(r)
+        cq.set_setting("QE_BATCH_GRAPH", 1)
+        graph = cq.aggregate_batch(t, specs)
+        cq.set_setting("QE_BATCH_GRAPH", 0)
+        eager = cq.aggregate_batch(t, specs)
+        plain = t.aggregate(specs[1].funcs, specs[1].args, where=specs[1].where)
+        for i, (sp, g, e) in enumerate(zip(specs, graph, eager)):
+            assert np.array_equal(g.keys, e.keys), (r, i)
+            assert np.array_equal(g.cells, e.cells), (r, i)
+        _same(specs[1], graph[1].keys, _vals(graph[1]), plain.keys, _vals(plain), f"plain r{r}")
+    r1, c1 = C.c_uint64(0), C.c_uint64(0)
+    lib.gpu_hip_graph_stats(C.byref(r1), C.byref(c1))
+    assert c1.value - c0.value >= 1
+    assert r1.value - r0.value >= 3
