@@ -103,10 +103,13 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
   memset(&slots, 0, sizeof(slots));
   U32 arg_slot[QE_AGG_MAX_ARGS];
   U32 slot_bits = 0, type_bits = 0;
+  /* encoded columns (a3) are read as codes: an encoded arg is a U32 column
+   * of codes (type byte U32 | encoding << 4) whose base is added back below */
   for (U32 a = 0; a < p->arg_count; a++) {
     arg_slot[a] = qe_slot_of(&slots, p->arg_col[a]);
     slot_bits |= arg_slot[a] << (4 * a);
-    type_bits |= t->columns[p->arg_col[a]].type << (8 * a);
+    U32 enc = qe_col_enc(t, p->arg_col[a]);
+    type_bits |= (enc ? (CQL_ColumnType_U32 | (enc << 4)) : t->columns[p->arg_col[a]].type) << (8 * a);
   }
   U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
   for (U32 l = 0; l < leaf_count; l++) {
@@ -116,6 +119,7 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
       return 0;
     }
     qe_simple_leaf_words(&leaf_words[6 * l], slot, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+    qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(t, leaves[l].column), t->columns[leaves[l].column].enc_base);
   }
   U64 r0 = prune->row_begin, r1 = prune->row_end;
   U64 quads = (r1 + 3) / 4 - r0 / 4;
@@ -135,7 +139,7 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
     if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
     else gpu_hip_kernel_clear_arg(k, 5);
     for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
-      if (s < slots.count) gpu_kernel_set_arg_buffer(k, 6 + s, t->columns[slots.column[s]].buffer);
+      if (s < slots.count) gpu_kernel_set_arg_buffer(k, 6 + s, qe_col_kbuf(t, slots.column[s]));
       else gpu_hip_kernel_clear_arg(k, 6 + s);
     }
     gpu_kernel_set_arg_u64(k, 0, r1 - r0);
@@ -168,6 +172,21 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
     return 0;
   }
   U64 count = res[0];
+  /* encoded args: the kernel saw codes = value - base */
+  for (U32 a = 0; a < p->arg_count && count; a++) {
+    const QE_Column* col = &t->columns[p->arg_col[a]];
+    if (!qe_col_enc(t, p->arg_col[a])) continue;
+    const U32 base = col->enc_base;
+    if (col->type == CQL_ColumnType_I32) {
+      res[1 + 3 * a] = (U64)((S64)res[1 + 3 * a] + (S64)count * (S64)(S32)base);
+      res[2 + 3 * a] = (U32)(base + (U32)res[2 + 3 * a]) ^ 0x80000000u;
+      res[3 + 3 * a] = (U32)(base + (U32)res[3 + 3 * a]) ^ 0x80000000u;
+    } else {
+      res[1 + 3 * a] += count * (U64)base;
+      res[2 + 3 * a] = (U32)(base + (U32)res[2 + 3 * a]);
+      res[3 + 3 * a] = (U32)(base + (U32)res[3 + 3 * a]);
+    }
+  }
   for (U32 e = 0; e < p->expr_count; e++) {
     S32 a = p->expr_arg[e];
     F64 sum = 0.0, mn = 0.0, mx = 0.0;
@@ -230,6 +249,8 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
   memset(&slots, 0, sizeof(slots));
   U32 key_slot = qe_slot_of(&slots, key_col);
   U32 slot_bits = 0, type_bits = 0;
+  /* encoded columns (a3) are read as codes: type byte U32 | encoding << 4,
+   * the key's minimum rewritten to codes, the base added back below */
   for (U32 a = 0; a < arg_count; a++) {
     U32 s = qe_slot_of(&slots, arg_cols[a]);
     U32 ty = t->columns[arg_cols[a]].type;
@@ -237,8 +258,9 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
       qe_set_error("dense group by: arguments must be U32/I32");
       return 0;
     }
+    U32 enc = qe_col_enc(t, arg_cols[a]);
     slot_bits |= s << (4 * a);
-    type_bits |= ty << (8 * a);
+    type_bits |= (enc ? (CQL_ColumnType_U32 | (enc << 4)) : ty) << (8 * a);
   }
   U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
   for (U32 l = 0; l < leaf_count; l++) {
@@ -248,6 +270,13 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
       return 0;
     }
     qe_simple_leaf_words(&leaf_words[6 * l], s, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+    qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(t, leaves[l].column), t->columns[leaves[l].column].enc_base);
+  }
+  const U32 key_enc = qe_col_enc(t, key_col);
+  U32 key_byte = t->columns[key_col].type, kmin = key_min_code;
+  if (key_enc) {
+    key_byte = CQL_ColumnType_U32 | (key_enc << 4);
+    kmin = key_min_code - t->columns[key_col].enc_base - (t->columns[key_col].type == CQL_ColumnType_I32 ? 0x80000000u : 0u);
   }
   if (!qe_dense_totals_alloc(out, range, arg_count)) return 0;
   U64 r0 = prune->row_begin, r1 = prune->row_end;
@@ -273,7 +302,7 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
     if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
     else gpu_hip_kernel_clear_arg(k, 5);
     for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
-      if (s < slots.count) gpu_kernel_set_arg_buffer(k, 6 + s, t->columns[slots.column[s]].buffer);
+      if (s < slots.count) gpu_kernel_set_arg_buffer(k, 6 + s, qe_col_kbuf(t, slots.column[s]));
       else gpu_hip_kernel_clear_arg(k, 6 + s);
     }
     gpu_kernel_set_arg_u64(k, 0, r1 - r0);
@@ -281,11 +310,11 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
     gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
     gpu_kernel_set_arg_u64(k, 5, r0);
     gpu_kernel_set_arg_u64(k, 7, leaf_count);
-    qe_kernel_set_word(k, QK_W_KEY_MIN, key_min_code);
+    qe_kernel_set_word(k, QK_W_KEY_MIN, kmin);
     qe_kernel_set_word(k, QK_W_RANGE, range);
     qe_kernel_set_word(k, QK_W_ARGS, arg_count | (slot_bits << 8));
     qe_kernel_set_word(k, QK_W_TYPES, type_bits);
-    qe_kernel_set_word(k, QK_W_KEY, key_slot | (t->columns[key_col].type << 8) | (flags << 16));
+    qe_kernel_set_word(k, QK_W_KEY, key_slot | (key_byte << 8) | (flags << 16));
     gpu_hip_kernel_set_lds_bytes(k, (U32)lds);
     GPU_Batch* b = gpu_batch_begin(0, gbytes);
     if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
@@ -311,6 +340,21 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
     out->block = 0;
     qe_set_error("aggregate_dense: %s", gpu_hip_last_error());
     return 0;
+  }
+  /* encoded args: per group, the kernel saw codes = value - base */
+  for (U32 a = 0; a < arg_count; a++) {
+    if (!qe_col_enc(t, arg_cols[a])) continue;
+    const QE_Column* col = &t->columns[arg_cols[a]];
+    const U32 base = col->enc_base, flip = col->type == CQL_ColumnType_I32 ? 0x80000000u : 0u;
+    for (U32 g = 0; g < range; g++) {
+      const U64 cnt = out->count[g];
+      if (!cnt) continue;
+      const U64 i = (U64)a * range + g;
+      if (flip) out->sum[i] = (U64)((S64)out->sum[i] + (S64)cnt * (S64)(S32)base);
+      else out->sum[i] += cnt * (U64)base;
+      out->mn[i] = (base + out->mn[i]) ^ flip;
+      out->mx[i] = (base + out->mx[i]) ^ flip;
+    }
   }
   if (trace) {
     trace->kernel_count = 1;
@@ -656,6 +700,16 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
               span <= (F64)max_range;
   for (U32 a = 0; a < plan.arg_count && dense; a++)
     if (!qe_type_int32(table->columns[plan.arg_col[a]].type)) dense = 0;
+  /* the block's LDS tables must fit: counts u32[R], sums u64[E][R],
+   * min/max u32[E][R] (args equal to the key fold into the counts) */
+  if (dense) {
+    B32 all_key = 1;
+    for (U32 a = 0; a < plan.arg_count; a++)
+      if (plan.arg_col[a] != (U32)group_col) all_key = 0;
+    const U64 e = (all_key && qe_settings_u64("QE_AGG_KEY_FOLD", 1)) ? 0 : plan.arg_count;
+    const U64 lds = (U64)span * 4 + (plan.need_sum ? (U64)span * 8 * e : 0) + (plan.need_minmax ? (U64)span * 8 * e : 0);
+    if (lds > gpu_hip_max_lds_bytes()) dense = 0;
+  }
   if (!dense) {
     U64 groups = 0;
     ok = qe_aggregate_hash(table, group_col, leaves, leaf_count, &prune, 0, 0, &plan, cap, &groups, out_keys, out_cells,

@@ -187,6 +187,7 @@ void qe_table_release(QE_Table* table)
   for (U32 c = 0; c < table->column_count; c++) {
     QE_Column* col = &table->columns[c];
     if (col->buffer) gpu_buffer_release(col->buffer);
+    if (col->enc_buffer) gpu_buffer_release(col->enc_buffer);
     free(col->zone_min);
     free(col->zone_max);
   }
@@ -265,6 +266,11 @@ B32 qe_table_build_zonemap(QE_Table* table, U32 column)
 {
   if (!table || column >= table->column_count) return 0;
   QE_Column* col = &table->columns[column];
+  /* the data changed: an encoded copy is stale */
+  if (col->enc_buffer) gpu_buffer_release(col->enc_buffer);
+  col->enc_buffer = 0;
+  col->enc = 0;
+  col->enc_base = 0;
   U64 zones = (table->row_count + (1ull << QE_ZONE_SHIFT) - 1) >> QE_ZONE_SHIFT;
   free(col->zone_min);
   free(col->zone_max);
@@ -315,6 +321,116 @@ B32 qe_table_build_zonemap(QE_Table* table, U32 column)
   col->max_value = mx;
   col->zonemap_valid = 1;
   return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* narrow encoding (qe_column_encoding, query_exec.c:1555-1589)         */
+/* ------------------------------------------------------------------ */
+U32 qe_col_enc(const QE_Table* t, U32 col)
+{
+  return qe_settings_u64("QE_ENCODE", 1) ? t->columns[col].enc : 0;
+}
+
+GPU_Buffer* qe_col_kbuf(const QE_Table* t, U32 col)
+{
+  const QE_Column* c = &t->columns[col];
+  U32 e = qe_col_enc(t, col);
+  return (e == 1 || e == 2) ? c->enc_buffer : c->buffer;
+}
+
+void qe_leaf_encode(U32* words, U32 enc, U32 base)
+{
+  if (!enc || !(words[1] & QE_SCAN_INT_FLAG)) return;
+  U32 lo = words[3] - base;
+  if (words[1] & QE_SCAN_INT_BIAS) {
+    lo -= 0x80000000u;
+    words[1] &= ~QE_SCAN_INT_BIAS;
+  }
+  words[3] = lo;
+  words[1] |= enc << 8;
+}
+
+/* one u32 flag word set by a kernel, read back */
+static B32 qe_flag_kernel(GPU_Kernel* k, QE_Column* col, U64 rows, U32* flag)
+{
+  GPU_Buffer* f = qe_scratch("enc_flag", 64);
+  if (!f) return 0;
+  gpu_kernel_set_arg_buffer(k, 6, col->buffer);
+  gpu_kernel_set_arg_buffer(k, 3, f);
+  gpu_kernel_set_arg_u64(k, 0, rows);
+  U64 blocks = (rows / 4 + QE_WG - 1) / QE_WG;
+  U64 cap = (U64)gpu_hip_compute_units() * 8;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  GPU_Batch* b = gpu_batch_begin(0, 4);
+  gpu_batch_buffer_zero(b, f, 64);
+  gpu_batch_kernel_execute(b, k, (U32)(blocks * QE_WG), QE_WG);
+  gpu_batch_buffer_read(b, f, flag, 4);
+  B32 ok = gpu_batch_end(b);
+  qe_scratch_release(f);
+  if (!ok) qe_set_error("encode: %s", gpu_hip_last_error());
+  return ok;
+}
+
+S32 qe_table_encode_column(QE_Table* table, U32 column, B32 allow_sequence)
+{
+  if (!table || column >= table->column_count) {
+    qe_set_error("qe_table_encode_column: bad column");
+    return -1;
+  }
+  QE_Column* col = &table->columns[column];
+  const U64 rows = table->row_count;
+  if (col->enc) return (S32)col->enc;
+  if (rows == 0 || (col->type != CQL_ColumnType_U32 && col->type != CQL_ColumnType_I32)) return 0;
+  /* a sequence: value = first value + row, the last one in the type's range */
+  if (allow_sequence && qe_settings_u64("QE_ENCODE_SEQUENCE", 1) && rows < (1ull << 32)) {
+    U32 first = 0;
+    gpu_buffer_read(col->buffer, &first, 4);
+    B32 fits = col->type == CQL_ColumnType_I32 ? (S64)(S32)first + (S64)rows - 1 <= 2147483647ll
+                                               : (U64)first + rows - 1 <= 0xffffffffull;
+    GPU_Kernel* k = qe_kernel("seq_check");
+    U32 bad = 1;
+    if (fits && k) {
+      qe_kernel_set_word(k, QK_W_KEY_MIN, first);
+      if (!qe_flag_kernel(k, col, rows, &bad)) return -1;
+    }
+    if (fits && !bad) {
+      col->enc = 3;
+      col->enc_base = first;
+      return 3;
+    }
+  }
+  if (!col->zonemap_valid) return 0;
+  const F64 lo = col->min_value, hi = col->max_value;
+  const U32 width = hi - lo < 256.0 ? 1 : (hi - lo < 65536.0 ? 2 : 0);
+  if (!width) return 0;
+  const U32 base = (U32)(S64)lo;
+  GPU_Kernel* k = qe_kernel("encode_column");
+  /* codes of 4 rows a quad, with 64 bytes of slack past the last quad */
+  U64 bytes = ((rows + 3) / 4) * 4 * width + 64;
+  GPU_Buffer* out = gpu_buffer_alloc(bytes, GPU_BufferFlag_ReadWrite | GPU_BufferFlag_DeviceLocal, 0);
+  if (!k || !out) {
+    if (out) gpu_buffer_release(out);
+    qe_set_error("qe_table_encode_column: allocation failed");
+    return -1;
+  }
+  gpu_kernel_set_arg_buffer(k, 2, out);
+  qe_kernel_set_word(k, QK_W_ARGS, width);
+  qe_kernel_set_word(k, QK_W_KEY_MIN, base);
+  U32 bad = 1;
+  if (!qe_flag_kernel(k, col, rows, &bad)) {
+    gpu_buffer_release(out);
+    return -1;
+  }
+  if (bad) {
+    gpu_buffer_release(out);
+    qe_set_error("qe_table_encode_column: values outside the zone-map range (stale zone map)");
+    return -1;
+  }
+  col->enc_buffer = out;
+  col->enc = width;
+  col->enc_base = base;
+  return (S32)width;
 }
 
 /* ------------------------------------------------------------------ */

@@ -32,6 +32,12 @@ typedef struct QE_Column {
   B32 zonemap_valid;
   F64 min_value;
   F64 max_value;
+  /* narrow encoding (qe_table_encode_column): enc 0 none, 1 u8 codes, 2 u16
+   * codes (in enc_buffer), 3 sequence (value = enc_base + row); dropped
+   * whenever the column's data is rewritten */
+  U32 enc;
+  U32 enc_base;
+  GPU_Buffer* enc_buffer;
 } QE_Column;
 
 struct QE_Table {
@@ -83,6 +89,20 @@ typedef struct QE_ColumnSlots {
 } QE_ColumnSlots;
 
 U32 qe_slot_of(QE_ColumnSlots* slots, U32 column); /* ~0u when out of slots */
+
+/* integer-leaf flags of word 1 (query_exec.c:1372-1374) */
+#define QE_SCAN_INT_FLAG 0x10000u
+#define QE_SCAN_INT_NEVER 0x20000u
+#define QE_SCAN_INT_BIAS 0x40000u
+
+/* the encoding kernels are given for a column (0 unless it is encoded and
+ * QE_ENCODE is on), and the buffer they read for it */
+U32 qe_col_enc(const QE_Table* t, U32 col);
+GPU_Buffer* qe_col_kbuf(const QE_Table* t, U32 col);
+/* rewrite an integer leaf (qe_simple_leaf_words) to compare codes of an
+ * encoded column: lo -= base (and the i32 bias), width in bits 8-15 of
+ * word 1 (the reference's leaf layout, query_exec.c:1442-1456) */
+void qe_leaf_encode(U32* words, U32 enc, U32 base);
 
 typedef struct QE_Bytecode {
   U32 words[4096];

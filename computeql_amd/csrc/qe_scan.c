@@ -306,11 +306,8 @@ B32 qe_scan_int_interval(U32 op, F64 target, F64 code_offset, F64 max_code, U32*
   return 1;
 }
 
-#define QE_SCAN_INT_FLAG 0x10000u
-#define QE_SCAN_INT_NEVER 0x20000u
-#define QE_SCAN_INT_BIAS 0x40000u
-
-/* query_exec.c:1408-1456, full-width columns (width 0) */
+/* query_exec.c:1408-1456, full-width columns (width 0; qe_leaf_encode
+ * rewrites a leaf for an encoded column) */
 void qe_simple_leaf_words(U32* words, U32 slot, U32 type, U32 op, F64 target)
 {
   U64 bits;
@@ -400,10 +397,13 @@ void qe_prune_free(QE_Prune* p)
 }
 
 /* ---------------- the scan driver ---------------- */
-static void qe_bind_columns(GPU_Kernel* k, QE_Table* t, const QE_ColumnSlots* slots)
+/* the slot columns; encoded copies where `encoded` (kernels whose leaves
+ * were rewritten for codes) */
+static void qe_bind_columns(GPU_Kernel* k, QE_Table* t, const QE_ColumnSlots* slots, B32 encoded)
 {
   for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
-    if (s < slots->count) gpu_kernel_set_arg_buffer(k, 6 + s, t->columns[slots->column[s]].buffer);
+    if (s < slots->count)
+      gpu_kernel_set_arg_buffer(k, 6 + s, encoded ? qe_col_kbuf(t, slots->column[s]) : t->columns[slots->column[s]].buffer);
     else gpu_hip_kernel_clear_arg(k, 6 + s);
   }
 }
@@ -539,6 +539,9 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
     for (U32 l = 0; l < leaf_count; l++) {
       U32 slot = qe_slot_of(&slots, leaves[l].column);
       qe_simple_leaf_words(&leaf_words[6 * l], slot, table->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+      /* scan_simple reads encoded columns (a3) as codes */
+      if (path == QE_ScanPath_Simple)
+        qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(table, leaves[l].column), table->columns[leaves[l].column].enc_base);
     }
   }
   if (path == QE_ScanPath_Interpreter) {
@@ -591,7 +594,7 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
     gpu_kernel_set_arg_buffer(k, 4, counter);
     if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
     else gpu_hip_kernel_clear_arg(k, 5);
-    qe_bind_columns(k, table, &slots);
+    qe_bind_columns(k, table, &slots, path == QE_ScanPath_Simple);
     gpu_kernel_set_arg_u64(k, 0, scan_rows);
     gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
     gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);

@@ -166,6 +166,27 @@ __device__ __forceinline__ void qk_load4_f64(const QK_Args& a, qk_u32 slot, qk_u
   }
 }
 
+/* ---- narrow column encodings (qe_column_encoding / qe_encoding_shader_width,
+ * query_exec.c:1555-1600): 0 full width, 1 u8 codes, 2 u16 codes, 3 a
+ * sequence (the row number is the code, nothing stored).  A kernel given an
+ * encoded column works on codes = value - base: integer leaves arrive with
+ * their interval rewritten to codes, dense keys with key_min rewritten, and
+ * the host adds the base back to sums, minima and maxima. ---- */
+__device__ __forceinline__ qk_u32x4 qk_quad_codes(qk_u64 col, qk_u32 enc, qk_u64 q)
+{
+  if (enc == 0) return qk_gptr<const qk_u32x4>(col)[q];
+  if (enc == 1) {
+    const qk_u32 x = qk_gptr<const qk_u32>(col)[q];
+    return qk_u32x4{x & 0xffu, (x >> 8) & 0xffu, (x >> 16) & 0xffu, x >> 24};
+  }
+  if (enc == 2) {
+    const uint2 x = qk_gptr<const uint2>(col)[q];
+    return qk_u32x4{x.x & 0xffffu, x.x >> 16, x.y & 0xffffu, x.y >> 16};
+  }
+  const qk_u32 r = (qk_u32)(q * 4);
+  return qk_u32x4{r, r + 1u, r + 2u, r + 3u};
+}
+
 /* one numeric leaf over a quad (qe_simple_leaf_words layout,
  * query_exec.c:1442-1456; integer form :1408-1438) */
 __device__ __forceinline__ qk_u32 qk_leaf_quad(const QK_Args& a, const qk_u32* w, qk_u64 q)
@@ -177,7 +198,7 @@ __device__ __forceinline__ qk_u32 qk_leaf_quad(const QK_Args& a, const qk_u32* w
   if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
     qk_u32 lo = w[3], span = w[4];
     qk_u32 bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
-    qk_u32x4 x = (qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot))[q];
+    qk_u32x4 x = qk_quad_codes(a.ptr[QK_B_COL0 + slot], (tw >> 8) & 0xffu, q);
     m |= (((x.x ^ bias) - lo) <= span) ? 1u : 0u;
     m |= (((x.y ^ bias) - lo) <= span) ? 2u : 0u;
     m |= (((x.z ^ bias) - lo) <= span) ? 4u : 0u;
@@ -247,10 +268,16 @@ __device__ __forceinline__ void qk_leaves_step(const QK_Args& a, const QkLeafSet
     if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
       qk_u32 lo = w[3], span = w[4];
       qk_u32 bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
-      const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot);
+      const qk_u32 enc = (tw >> 8) & 0xffu;
       qk_u32x4 x[QK_UNROLL];
+      if (enc == 0) {
+        const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot);
 #pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+        for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+      } else {
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_quad_codes(a.ptr[QK_B_COL0 + slot], enc, q[j]);
+      }
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) {
         qk_u32 mm = 0;

@@ -204,13 +204,19 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       if ((qk_u32)c >= nargs) continue;
-      qk_u32 slot = (slots >> (4 * c)) & 0xfu, type = (types >> (8 * c)) & 0xffu;
+      /* type byte: column type, encoding << 4 (encoded args are codes, U32) */
+      qk_u32 slot = (slots >> (4 * c)) & 0xfu, type = (types >> (8 * c)) & 0xfu, enc = (types >> (8 * c + 4)) & 3u;
       if (type == QK_T_U32) {
         /* the hot case: the U loads of the step in flight together */
-        const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot);
         qk_u32x4 x[QK_UNROLL];
+        if (enc == 0) {
+          const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot);
 #pragma unroll
-        for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+          for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+        } else {
+#pragma unroll
+          for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_quad_codes(a.ptr[QK_B_COL0 + slot], enc, q[j]);
+        }
 #pragma unroll
         for (int j = 0; j < QK_UNROLL; j++) {
           qk_u32 mm = m[j];
@@ -240,7 +246,7 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
     count += __shfl_xor(count, o, 64);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      qk_u32 t = (types >> (8 * c)) & 0xffu;
+      qk_u32 t = (types >> (8 * c)) & 0xfu;
       qk_u64 os = __shfl_xor(acc[c].sum, o, 64);
       acc[c].sum = agg_combine_sum(t, acc[c].sum, os);
       acc[c].mn = min(acc[c].mn, (qk_u64)__shfl_xor(acc[c].mn, o, 64));
@@ -265,7 +271,7 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
       if (f == 0) v += x;
       else {
         int c = (f - 1) / 3, kind = (f - 1) % 3;
-        qk_u32 t = (types >> (8 * c)) & 0xffu;
+        qk_u32 t = (types >> (8 * c)) & 0xfu;
         v = kind == 0 ? agg_combine_sum(t, v, x) : (kind == 1 ? min(v, x) : max(v, x));
       }
     }
@@ -305,7 +311,7 @@ extern "C" __global__ void LB k_aggregate_global_fold(QK_Args a)
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       if ((qk_u32)c >= nargs) continue;
-      qk_u32 t = (types >> (8 * c)) & 0xffu;
+      qk_u32 t = (types >> (8 * c)) & 0xfu;
       v[1 + 3 * c] = agg_combine_sum(t, v[1 + 3 * c], pb[1 + 3 * c]);
       v[2 + 3 * c] = min(v[2 + 3 * c], pb[2 + 3 * c]);
       v[3 + 3 * c] = max(v[3 + 3 * c], pb[3 + 3 * c]);
@@ -316,7 +322,7 @@ extern "C" __global__ void LB k_aggregate_global_fold(QK_Args a)
     v[0] += __shfl_xor(v[0], o, 64);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      qk_u32 t = (types >> (8 * c)) & 0xffu;
+      qk_u32 t = (types >> (8 * c)) & 0xfu;
       v[1 + 3 * c] = agg_combine_sum(t, v[1 + 3 * c], __shfl_xor(v[1 + 3 * c], o, 64));
       v[2 + 3 * c] = min(v[2 + 3 * c], (qk_u64)__shfl_xor(v[2 + 3 * c], o, 64));
       v[3 + 3 * c] = max(v[3 + 3 * c], (qk_u64)__shfl_xor(v[3 + 3 * c], o, 64));
@@ -333,7 +339,7 @@ extern "C" __global__ void LB k_aggregate_global_fold(QK_Args a)
       if (f == 0) r += x;
       else {
         int c = (f - 1) / 3, kind = (f - 1) % 3;
-        qk_u32 t = (types >> (8 * c)) & 0xffu;
+        qk_u32 t = (types >> (8 * c)) & 0xfu;
         r = kind == 0 ? agg_combine_sum(t, r, x) : (kind == 1 ? min(r, x) : max(r, x));
       }
     }
@@ -494,7 +500,9 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
   const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
   const qk_u32 types = a.words[QK_W_TYPES];
   const qk_u32 key_slot = a.words[QK_W_KEY] & 0xffu;
-  const qk_u32 key_type = (a.words[QK_W_KEY] >> 8) & 0xffu;
+  /* type bytes: column type, encoding << 4 (encoded columns are codes, U32) */
+  const qk_u32 key_type = (a.words[QK_W_KEY] >> 8) & 0xfu;
+  const qk_u32 key_enc = (a.words[QK_W_KEY] >> 12) & 3u;
   const qk_u32 flags = a.words[QK_W_KEY] >> 16;
   const qk_u32 key_min = a.words[QK_W_KEY_MIN];
   const qk_u32 range = a.words[QK_W_RANGE];
@@ -524,8 +532,13 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
     qk_u32x4 kv[QK_UNROLL];
     qk_step_quads(a, s, b1, r0, r1, q, m);
     if (L.n) qk_leaves_step(a, L, q, m);
+    if (key_enc == 0) {
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
+      for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
+    } else {
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) kv[j] = qk_quad_codes(a.ptr[QK_B_COL0 + key_slot], key_enc, q[j]);
+    }
     qk_u32 g[QK_UNROLL][4];
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) {
@@ -543,11 +556,16 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
         if (m[j] & (1u << k)) atomicAdd(&l_cnt[g[j][k]], 1u);
     }
     for (qk_u32 c = 0; c < nargs; c++) {
-      qk_u32 at = (types >> (8 * c)) & 0xffu;
-      const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + ((slots >> (4 * c)) & 0xfu));
+      qk_u32 at = (types >> (8 * c)) & 0xfu, aenc = (types >> (8 * c + 4)) & 3u;
+      const qk_u64 cp = a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)];
       qk_u32x4 x[QK_UNROLL];
+      if (aenc == 0) {
 #pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+        for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_gptr<const qk_u32x4>(cp)[q[j]];
+      } else {
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_quad_codes(cp, aenc, q[j]);
+      }
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) {
         qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
@@ -1827,7 +1845,9 @@ extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
   const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
   h.types = a.words[QK_W_TYPES];
   const qk_u32 key_slot = a.words[QK_W_KEY] & 0xffu;
-  const qk_u32 key_type = (a.words[QK_W_KEY] >> 8) & 0xffu;
+  /* type bytes: column type, encoding << 4 (encoded columns are codes, U32) */
+  const qk_u32 key_type = (a.words[QK_W_KEY] >> 8) & 0xfu;
+  const qk_u32 key_enc = (a.words[QK_W_KEY] >> 12) & 3u;
   h.flags = a.words[QK_W_KEY] >> 16;
   h.log2 = a.words[QK_W_LOG2];
   h.L = a.words[QK_W_RANGE];
@@ -2377,6 +2397,56 @@ extern "C" __global__ void LB k_shared_reduce(QK_Args a)
 }
 
 /* ------------------------------------------------------------------ */
+/* column encoding (qe_encode_chunk / qe_column_is_sequence,            */
+/* query_exec.c:1506-1672), on the device from the resident column.     */
+/* encode_column: binding 6 the u32/i32 column, 2 codes out (u8 or u16, */
+/* 4 rows a quad), 3 flags (u32: set when a value - base exceeds the    */
+/* width's range); push 0 rows; words ARGS = width (1, 2), KEY_MIN =    */
+/* base.  seq_check: binding 6 the column, 3 flags (set when a row's    */
+/* value is not base + row); push 0 rows; word KEY_MIN = base.          */
+/* ------------------------------------------------------------------ */
+extern "C" __global__ void LB k_encode_column(QK_Args a)
+{
+  const qk_u64 rows = qk_push(a, 0);
+  const qk_u32 width = a.words[QK_W_ARGS], base = a.words[QK_W_KEY_MIN];
+  const qk_u32 maxc = width == 1 ? 0xffu : 0xffffu;
+  const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0);
+  qk_u32* flags = qk_wptr<qk_u32>(a, 3);
+  const qk_u64 quads = (rows + 3) / 4;
+  bool bad = false;
+  for (qk_u64 q = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; q < quads; q += (qk_u64)gridDim.x * QK_WG) {
+    const qk_u32x4 v = col[q];
+    const qk_u32 w[4] = {v.x, v.y, v.z, v.w};
+    qk_u32 c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const bool live = q * 4 + k < rows;
+      c[k] = live ? w[k] - base : 0u;
+      bad |= live && c[k] > maxc;
+    }
+    if (width == 1) qk_wptr<qk_u32>(a, 2)[q] = c[0] | (c[1] << 8) | (c[2] << 16) | (c[3] << 24);
+    else qk_wptr<uint2>(a, 2)[q] = make_uint2(c[0] | (c[1] << 16), c[2] | (c[3] << 16));
+  }
+  if (bad) atomicOr(flags, 1u);
+}
+
+extern "C" __global__ void LB k_seq_check(QK_Args a)
+{
+  const qk_u64 rows = qk_push(a, 0);
+  const qk_u32 base = a.words[QK_W_KEY_MIN];
+  const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0);
+  const qk_u64 quads = (rows + 3) / 4;
+  bool bad = false;
+  for (qk_u64 q = (qk_u64)blockIdx.x * QK_WG + threadIdx.x; q < quads; q += (qk_u64)gridDim.x * QK_WG) {
+    const qk_u32x4 v = col[q];
+    const qk_u32 w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) bad |= q * 4 + k < rows && w[k] != base + (qk_u32)(q * 4 + k);
+  }
+  if (bad) atomicOr(qk_wptr<qk_u32>(a, 3), 1u);
+}
+
+/* ------------------------------------------------------------------ */
 /* gather: column values through row ids (selection_gather.comp modes   */
 /* F32/F64): out_c[i] = column c at rows[i * STRIDE + OFFSET], 0 where  */
 /* the id is all ones (no row).  binding 0 row ids (u64), 6..9 columns, */
@@ -2836,6 +2906,8 @@ static const QK_LibraryKernel g_library[] = {
     {"join_part_probe_1s_row_512", (const void*)k_join_part_probe_1s_row_512, 512},
     {"join_part_probe_1sm_row", (const void*)k_join_part_probe_1sm_row, QK_WG},
     {"join_part_probe_1sm_row_512", (const void*)k_join_part_probe_1sm_row_512, 512},
+    {"encode_column", (const void*)k_encode_column, QK_WG},
+    {"seq_check", (const void*)k_seq_check, QK_WG},
     {"gather", (const void*)k_gather, QK_WG},
     {"scan_u32_blocks", (const void*)k_scan_u32_blocks, QK_WG},
     {"scan_u32_top", (const void*)k_scan_u32_top, QK_WG},

@@ -133,6 +133,14 @@ class Table:
             raise _err("column_read")
         return out
 
+    def encode(self, column, allow_sequence: bool = True) -> int:
+        """qe_table_encode_column: keep the column also as narrow codes (1 u8,
+        2 u16) or as a sequence (3); 0 when none applies."""
+        r = self._lib.qe_table_encode_column(self.handle, self._col_index(column), 1 if allow_sequence else 0)
+        if r < 0:
+            raise _err("encode")
+        return int(r)
+
     def build_zonemap(self, column: int) -> None:
         if not self._lib.qe_table_build_zonemap(self.handle, column):
             raise _err("build_zonemap")

@@ -179,6 +179,7 @@ SIGNATURES = [
     ("qe_aggregate_join", B32, [P, U32, P, U32, P, U32, S32, P, P, U32, U64, P, P, P, P]),
     ("qe_hash_join", U32, [P, U32, P, U64, P, U32, P, U64, U32, P, U64, P, P]),
     ("qe_gather", U32, [P, P, U32, P, U64, U64, U64, P]),
+    ("qe_table_encode_column", S32, [P, U32, B32]),
     ("qe_scan_materialize", U32, [P, P, U32, P, U32, U64, P, P]),
     ("qe_aggregate_batch", B32, [P, P, U32, P]),
     ("qe_aggregate_batch_tune", B32, [P, P, U32, P, P]),

@@ -399,6 +399,17 @@ U32 qe_hash_join(QE_Table* probe, U32 probe_key_col, const U64* probe_rows, U64 
                  QE_Table* build, U32 build_key_col, const U64* build_rows, U64 build_count,
                  U32 join_type, U64* out, U64 cap, U64* out_count, QE_Trace* trace);
 
+/* Narrow column encoding (qe_column_encoding / qe_encode_chunk,
+ * query_exec.c:1506-1672): keep a copy of a u32/i32 column as 1-byte or
+ * 2-byte codes (value - base, base = the zone-map minimum, when the range is
+ * below 256 / 65536), or, with allow_sequence, recognise a sequence (value
+ * = first value + row, nothing stored).  scan_simple, the global aggregate
+ * and the dense GROUP BY then read the codes while QE_ENCODE is on (the
+ * default); other kernels read the full column.  Returns the encoding (0
+ * none applies, 1 u8, 2 u16, 3 sequence) or -1 on error (values outside
+ * the zone-map range).  Rewriting the column's data drops the encoding. */
+S32 qe_table_encode_column(QE_Table* table, U32 column, B32 allow_sequence);
+
 /* Row materialisation (selection_gather.comp modes F32/F64,
  * qe_selection_gather_column): out[c][i] = column cols[c] at row
  * rows[i * stride + offset], at the column's own width (4 or 8 bytes), 0
