@@ -27,7 +27,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["qk_library.hip", "gpu_hip.cpp"]
-C_SOURCES = ["qe_core.c", "qe_scan.c", "qe_aggregate.c", "qe_join.c", "qe_hjoin.c", "qe_gather.c", "qe_batch.c", "qe_serve.c",
+C_SOURCES = ["qe_core.c", "qe_scan.c", "qe_aggregate.c", "qe_join.c", "qe_hjoin.c", "qe_gather.c", "qe_stream.c", "qe_batch.c", "qe_serve.c",
              "query_kernel.c"]
 
 # packed-fp32 instructions stay off in every HIP build

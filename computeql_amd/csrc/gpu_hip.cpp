@@ -139,6 +139,7 @@ struct HipState {
   std::map<std::string, std::pair<U64, U64>> profile_table; /* ns, dispatches */
   U64 fault_alloc_at = 0;
   U64 fault_alloc_counter = 0;
+  U64 (*evict_hook)(U64) = nullptr;
 };
 
 HipState& st()
@@ -331,6 +332,14 @@ GPU_Buffer* alloc_impl(U64 size, U32 flags, void* data)
       delete v;
     }
     e = hipMalloc(&p, padded);
+    if (e != hipSuccess) {
+      /* then the owner of cached device data (the resident chunk cache)
+       * frees what it can, and one more try (g_gpu_evict_hook,
+       * query_exec.c:1817-1822) */
+      (void)hipGetLastError();
+      U64 (*hook)(U64) = s.evict_hook;
+      if (hook && hook(padded) > 0) e = hipMalloc(&p, padded);
+    }
     std::lock_guard<std::mutex> g(s.lock);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -1041,6 +1050,21 @@ static B32 batch_end_deferred(GPU_Batch* batch)
   delete batch;
   return ok ? 1 : 0;
 }
+
+void* gpu_hip_host_alloc(U64 bytes)
+{
+  if (!st().initialised) gpu_hip_init();
+  void* p = nullptr;
+  if (!check(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+  return p;
+}
+
+void gpu_hip_host_free(void* p)
+{
+  if (p) hipHostFree(p);
+}
+
+void gpu_hip_set_evict_hook(U64 (*hook)(U64 bytes_wanted)) { st().evict_hook = hook; }
 
 GPU_Batch* gpu_hip_batch_begin_graph(const char* key)
 {

@@ -67,6 +67,8 @@ B32 qe_agg_plan(QE_Table* t, const U32* funcs, const S32* arg_cols, U32 expr_cou
   return 1;
 }
 
+static B32 qe_type_int32(U32 t) { return t == CQL_ColumnType_U32 || t == CQL_ColumnType_I32; }
+
 static F64 qe_key_to_f64(U32 type, U64 key)
 {
   switch (type) {
@@ -195,6 +197,10 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
       sum = qe_sum_to_f64(type, res[1 + 3 * a]);
       mn = qe_key_to_f64(type, res[2 + 3 * a]);
       mx = qe_key_to_f64(type, res[3 + 3 * a]);
+      if (qe_type_int32(type)) {
+        out_cells[e] = qe_agg_cell_int(p->funcs[e], count, (S64)res[1 + 3 * a], mn, mx);
+        continue;
+      }
     }
     out_cells[e] = qe_agg_cell(p->funcs[e], count, sum, mn, mx);
   }
@@ -392,8 +398,6 @@ static F64 qe_f64_from_order_code(U64 key)
   return d;
 }
 
-static B32 qe_type_int32(U32 t) { return t == CQL_ColumnType_U32 || t == CQL_ColumnType_I32; }
-
 /* GROUP BY over the global hash table of aggregate_hash (key_col < 0: one
  * group).  Rows: the fused leaves over the pruned range, or `list` (device
  * row ids) when given.  Groups come out in table-slot order. */
@@ -571,6 +575,10 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
         else memcpy(&sum, &sv, 8);
         mn = qe_f64_from_order_code(r[2 + E + a]);
         mx = qe_f64_from_order_code(r[2 + 2 * E + a]);
+        if (qe_type_int32(ty)) {
+          out_cells[g * p->expr_count + e] = qe_agg_cell_int(p->funcs[e], cnt, (S64)sv, mn, mx);
+          continue;
+        }
       }
       out_cells[g * p->expr_count + e] = qe_agg_cell(p->funcs[e], cnt, sum, mn, mx);
     }
@@ -601,6 +609,9 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
     qe_set_error("qe_aggregate: null argument");
     return 0;
   }
+  if (table->host_backed)
+    return qe_stream_aggregate(table, where, where_count, group_col, funcs, arg_cols, expr_count, cap, out_group_count,
+                               out_keys, out_cells, trace);
   QE_AggPlan plan;
   if (!qe_agg_plan(table, funcs, arg_cols, expr_count, &plan)) return 0;
   if (trace) trace->rows_before = table->row_count;
@@ -748,18 +759,23 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
       for (U32 e = 0; e < expr_count; e++) {
         S32 a = plan.expr_arg[e];
         F64 sum = 0.0, mn = 0.0, mx = 0.0;
+        S64 isum = 0;
         if (a >= 0) {
           if (fold) {
+            isum = (S64)kv * (S64)cnt;
             sum = kv * (F64)cnt;
             mn = mx = kv;
           } else {
             U32 ty = table->columns[plan.arg_col[a]].type;
             U64 s = tot.sum[(U64)a * range + i];
+            isum = (S64)s;
             sum = ty == CQL_ColumnType_I32 ? (F64)(S64)s : (F64)s;
             U32 lo = tot.mn[(U64)a * range + i], hi = tot.mx[(U64)a * range + i];
             mn = ty == CQL_ColumnType_I32 ? (F64)(S32)(lo ^ 0x80000000u) : (F64)lo;
             mx = ty == CQL_ColumnType_I32 ? (F64)(S32)(hi ^ 0x80000000u) : (F64)hi;
           }
+          out_cells[g * expr_count + e] = qe_agg_cell_int(plan.funcs[e], cnt, isum, mn, mx);
+          continue;
         }
         out_cells[g * expr_count + e] = qe_agg_cell(plan.funcs[e], cnt, sum, mn, mx);
       }

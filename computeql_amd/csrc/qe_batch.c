@@ -630,6 +630,7 @@ B32 qe_aggregate_batch_tune(QE_Table* table, QE_BatchQuery* queries, U32 count, 
   F64 best_t = 1e300;
   QE_BatchConfig b;
   memset(&b, 0, sizeof(b));
+  if (!qe_require_device(table, "qe_aggregate_batch_tune")) return 0;
   for (U32 i = 0; i < sizeof(shapes) / sizeof(shapes[0]); i++)
     for (U32 j = 0; j < 2; j++)
       for (U32 k = 0; k < 2; k++) {
@@ -654,6 +655,7 @@ B32 qe_aggregate_batch_tune(QE_Table* table, QE_BatchQuery* queries, U32 count, 
 B32 qe_aggregate_batch(QE_Table* table, QE_BatchQuery* queries, U32 count, QE_BatchTrace* trace)
 {
   QE_BatchConfig cfg;
+  if (!qe_require_device(table, "qe_aggregate_batch")) return 0;
   qe_batch_config_from_settings(&cfg);
   if (table && queries && count && qe_settings_u64("QE_BATCH_AUTOTUNE", 0)) {
     /* the tuned configuration of this workload, tuning it on this batch the

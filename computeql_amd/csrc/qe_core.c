@@ -161,7 +161,7 @@ U32 qe_stream_blocks(U64 quads)
 /* ------------------------------------------------------------------ */
 /* tables                                                               */
 /* ------------------------------------------------------------------ */
-static U32 qe_type_width(U32 type)
+U32 qe_type_width(U32 type)
 {
   switch (type) {
     case CQL_ColumnType_U32:
@@ -188,6 +188,10 @@ void qe_table_release(QE_Table* table)
     QE_Column* col = &table->columns[c];
     if (col->buffer) gpu_buffer_release(col->buffer);
     if (col->enc_buffer) gpu_buffer_release(col->enc_buffer);
+    if (col->host_data) {
+      qe_resident_drop_column(col->uid);
+      gpu_hip_host_free(col->host_data);
+    }
     free(col->zone_min);
     free(col->zone_max);
   }
@@ -202,6 +206,7 @@ S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void*
     qe_set_error("qe_table_add_column: table full");
     return -1;
   }
+  if (!qe_require_device(table, "qe_table_add_column")) return -1;
   U32 width = qe_type_width(type);
   if (!width) {
     qe_set_error("qe_table_add_column: unsupported type %u", type);
@@ -248,6 +253,10 @@ B32 qe_table_column_read(QE_Table* table, U32 column, U64 row0, U64 rows, void* 
     return 0;
   }
   QE_Column* col = &table->columns[column];
+  if (col->host_data) {
+    memcpy(out, (const U8*)col->host_data + row0 * col->width, rows * col->width);
+    return 1;
+  }
   if (row0 == 0) {
     gpu_buffer_read(col->buffer, out, rows * col->width);
     return 1;
@@ -264,7 +273,7 @@ B32 qe_table_column_read(QE_Table* table, U32 column, U64 row0, U64 rows, void* 
 
 B32 qe_table_build_zonemap(QE_Table* table, U32 column)
 {
-  if (!table || column >= table->column_count) return 0;
+  if (!table || column >= table->column_count || !qe_require_device(table, "qe_table_build_zonemap")) return 0;
   QE_Column* col = &table->columns[column];
   /* the data changed: an encoded copy is stale */
   if (col->enc_buffer) gpu_buffer_release(col->enc_buffer);
@@ -378,6 +387,7 @@ S32 qe_table_encode_column(QE_Table* table, U32 column, B32 allow_sequence)
     qe_set_error("qe_table_encode_column: bad column");
     return -1;
   }
+  if (!qe_require_device(table, "qe_table_encode_column")) return -1;
   QE_Column* col = &table->columns[column];
   const U64 rows = table->row_count;
   if (col->enc) return (S32)col->enc;
@@ -454,7 +464,7 @@ static B32 qe_run_gen(QE_Table* table, GPU_Kernel* k, U64 rows)
 
 B32 qe_gen_events(QE_Table* table, U64 seed, S32 id_col, S32 region_col, S32 amount_col, S32 ts_col)
 {
-  if (!table) return 0;
+  if (!table || !qe_require_device(table, "qe_gen")) return 0;
   GPU_Kernel* k = qe_kernel("gen_events");
   if (!k) return 0;
   S32 cols[4] = {id_col, region_col, amount_col, ts_col};
@@ -479,7 +489,7 @@ B32 qe_gen_events(QE_Table* table, U64 seed, S32 id_col, S32 region_col, S32 amo
 
 B32 qe_gen_scale_events(QE_Table* table, S32 id_col, S32 region_col, S32 amount_col, S32 key_col)
 {
-  if (!table) return 0;
+  if (!table || !qe_require_device(table, "qe_gen")) return 0;
   GPU_Kernel* k = qe_kernel("gen_scale_events");
   if (!k) return 0;
   S32 cols[4] = {id_col, region_col, amount_col, key_col};
@@ -522,6 +532,7 @@ B32 qe_gen_dim(QE_Table* table, U64 seed, U32 attr_range, B32 spread_keys, S32 k
     qe_set_error("qe_gen_dim: bad arguments");
     return 0;
   }
+  if (!qe_require_device(table, "qe_gen_dim")) return 0;
   U64 n = table->row_count;
   U32* keys = (U32*)malloc(sizeof(U32) * (n ? n : 1));
   if (!keys) return 0;
@@ -554,6 +565,7 @@ B32 qe_gen_fact_fk(QE_Table* table, U32 fk_col, U64 seed, QE_Table* dim, U32 dim
     qe_set_error("qe_gen_fact_fk: bad arguments");
     return 0;
   }
+  if (!qe_require_device(table, "qe_gen_fact_fk") || !qe_require_device(dim, "qe_gen_fact_fk")) return 0;
   GPU_Kernel* k = qe_kernel("gen_fact_fk");
   if (!k) return 0;
   gpu_kernel_set_arg_buffer(k, 6, table->columns[fk_col].buffer);
@@ -571,6 +583,14 @@ B32 qe_gen_fact_fk(QE_Table* table, U32 fk_col, U64 seed, QE_Table* dim, U32 dim
 /* (qe_agg_func_policy, query_exec.h:917-926; qe_aggregate_finish,      */
 /* query_exec.c:7269-7293)                                              */
 /* ------------------------------------------------------------------ */
+__thread B32 qe_t_raw_int_sums;
+
+U64 qe_agg_cell_int(U32 func, U64 count, S64 isum, F64 mn, F64 mx)
+{
+  if (qe_t_raw_int_sums && func == QE_AGG_FUNC_SUM) return (U64)isum;
+  return qe_agg_cell(func, count, (F64)isum, mn, mx);
+}
+
 U64 qe_agg_cell(U32 func, U64 count, F64 sum, F64 mn, F64 mx)
 {
   F64 v;

@@ -73,6 +73,7 @@ static B32 qe_gather_check(QE_Table* t, const U32* cols, U32 col_count, void* co
     qe_set_error("gather: null argument");
     return 0;
   }
+  if (!qe_require_device(t, "gather")) return 0;
   for (U32 c = 0; c < col_count; c++)
     if (cols[c] >= t->column_count || !out[c]) {
       qe_set_error("gather: bad column %u", c);

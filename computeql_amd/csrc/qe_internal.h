@@ -38,13 +38,33 @@ typedef struct QE_Column {
   U32 enc;
   U32 enc_base;
   GPU_Buffer* enc_buffer;
+  /* host-backed column (qe_table_add_host_column): the rows live in pinned
+   * host memory and reach the device chunk by chunk through the resident
+   * chunk cache, under `uid` (buffer is 0) */
+  void* host_data;
+  U64 uid;
 } QE_Column;
 
 struct QE_Table {
   U64 row_count;
   U32 column_count;
+  B32 host_backed; /* every column is host-backed; operators stream it */
   QE_Column columns[QE_MAX_COLUMNS];
 };
+
+U32 qe_type_width(U32 type); /* bytes per row of a column type, 0 if unsupported */
+
+/* operators that need every column whole in HBM refuse host-backed tables */
+B32 qe_require_device(const QE_Table* t, const char* op);
+
+/* ---- streamed execution over host-backed tables (qe_stream.c) ---- */
+B32 qe_stream_scan(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only, U64* out_ids, U64 cap,
+                   U64* out_count, QE_Trace* trace);
+B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 group_col, const U32* funcs,
+                        const S32* arg_cols, U32 expr_count, U64 cap, U64* out_group_count, F64* out_keys,
+                        U64* out_cells, QE_Trace* trace);
+/* drop the cached chunks of a column (its table is being released) */
+void qe_resident_drop_column(U64 uid);
 
 /* ---- gpu_* wrappers over the active backend (gpu_dispatch.c:153-603) ---- */
 GPU_Backend* qe_gpu(void);
@@ -161,6 +181,11 @@ typedef struct QE_GroupTotals {
 } QE_GroupTotals;
 
 U64 qe_agg_cell(U32 func, U64 count, F64 sum, F64 mn, F64 mx);
+/* the cell of an aggregate over a 32-bit integer column, its sum exact;
+ * while qe_t_raw_int_sums is set on this thread (streamed aggregation
+ * merging chunk results) a SUM cell holds the S64 sum, not its F64 */
+U64 qe_agg_cell_int(U32 func, U64 count, S64 isum, F64 mn, F64 mx);
+extern __thread B32 qe_t_raw_int_sums;
 
 /* an aggregate's expressions resolved to distinct argument columns */
 #define QE_AGG_MAX_ARGS 4

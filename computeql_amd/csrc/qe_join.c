@@ -331,6 +331,7 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     qe_set_error("qe_aggregate_join: bad arguments");
     return 0;
   }
+  if (!qe_require_device(fact, "qe_aggregate_join") || !qe_require_device(dim, "qe_aggregate_join")) return 0;
   if (qe_settings_u64("QE_JOIN_PUSHDOWN", 1) == 0) {
     qe_set_error("qe_aggregate_join: QE_JOIN_PUSHDOWN is off");
     return 0;

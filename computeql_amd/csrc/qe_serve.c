@@ -203,6 +203,7 @@ U64 qe_server_submit(QE_Server* s, QE_Table* table, QE_BatchQuery* query)
     qe_set_error("qe_server_submit: null argument");
     return 0;
   }
+  if (!qe_require_device(table, "qe_server_submit")) return 0;
   QE_Request* r = (QE_Request*)calloc(1, sizeof(QE_Request));
   if (!r) {
     qe_set_error("qe_server_submit: out of memory");

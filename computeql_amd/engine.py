@@ -14,7 +14,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import ir
-from .native import QE_BatchConfig, QE_BatchQuery, QE_BatchTrace, QE_Trace, load
+from .native import QE_BatchConfig, QE_BatchQuery, QE_BatchTrace, QE_ResidentStats, QE_StreamStats, QE_Trace, load
 
 U32, U64, F32, F64, I32, I64 = 1, 2, 3, 4, 7, 8
 COUNT, SUM, AVG, MIN, MAX = 0, 1, 2, 3, 4
@@ -115,6 +115,20 @@ class Table:
         idx = self._lib.qe_table_add_column(self.handle, name.encode(), ctype, ptr, None)
         if idx < 0:
             raise _err("add_column")
+        self.names.append(name)
+        self.types.append(ctype)
+        return int(idx)
+
+    def add_host_column(self, name: str, ctype: int, data: np.ndarray) -> int:
+        """qe_table_add_host_column: the rows stay in pinned host memory and
+        scans/aggregates stream them through the resident chunk cache.  A
+        table holds either host-backed or device columns."""
+        data = np.ascontiguousarray(data, dtype=_NP[ctype])
+        if data.shape != (self.rows,):
+            raise ValueError(f"column {name}: expected {self.rows} rows, got {data.shape}")
+        idx = self._lib.qe_table_add_host_column(self.handle, name.encode(), ctype, data.ctypes.data)
+        if idx < 0:
+            raise _err("add_host_column")
         self.names.append(name)
         self.types.append(ctype)
         return int(idx)
@@ -419,3 +433,22 @@ def bandwidth_read(table: Table, column: int, blocks: int = 0) -> dict:
     if not lib.qe_bandwidth_read(buf, table.rows * 4, blocks, C.byref(sink), C.byref(tr)):
         raise _err("bandwidth_read")
     return tr.as_dict()
+
+
+def resident_stats() -> dict:
+    """Counters of the resident chunk cache (qe_resident_stats)."""
+    st = QE_ResidentStats()
+    load().qe_resident_stats(C.byref(st))
+    return st.as_dict()
+
+
+def resident_clear() -> None:
+    """Free every cached chunk no query holds (a cold start)."""
+    load().qe_resident_clear()
+
+
+def stream_stats() -> dict:
+    """What the calling thread's last streamed scan or aggregate did."""
+    st = QE_StreamStats()
+    load().qe_stream_last_stats(C.byref(st))
+    return st.as_dict()

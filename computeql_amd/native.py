@@ -87,6 +87,22 @@ class GPU_MemoryStats(C.Structure):
                                     "pool_evictions", "pool_evicted_bytes", "alloc_calls", "unrecovered_alloc_failures")]
 
 
+class QE_ResidentStats(C.Structure):
+    _fields_ = [(n, U64) for n in ("hit_bytes", "miss_bytes", "evicted_bytes", "bytes_in_use", "entries",
+                                    "budget_bytes")]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class QE_StreamStats(C.Structure):
+    _fields_ = [(n, U64) for n in ("chunks", "chunks_skipped", "chunks_run", "chunks_resident", "transient_chunks",
+                                    "bytes_copied", "copy_wait_ns", "wall_ns")]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class String8(C.Structure):
     _fields_ = [("str", C.c_void_p), ("size", U64)]
 
@@ -157,6 +173,9 @@ SIGNATURES = [
     ("gpu_hip_profile_tag", None, [C.c_char_p]),
     ("gpu_hip_batch_begin_graph", P, [C.c_char_p]),
     ("gpu_hip_graph_stats", None, [P, P]),
+    ("gpu_hip_host_alloc", P, [U64]),
+    ("gpu_hip_host_free", None, [P]),
+    ("gpu_hip_set_evict_hook", None, [P]),
     ("gpu_hip_fault_alloc_at", None, [U64]),
     ("gpu_hip_last_error", C.c_char_p, []),
     ("qe_settings_set_u64", None, [C.c_char_p, U64]),
@@ -170,6 +189,10 @@ SIGNATURES = [
     ("qe_table_column_buffer", P, [P, U32]),
     ("qe_table_column_read", B32, [P, U32, U64, U64, P]),
     ("qe_table_build_zonemap", B32, [P, U32]),
+    ("qe_table_add_host_column", S32, [P, C.c_char_p, U32, P]),
+    ("qe_resident_stats", None, [P]),
+    ("qe_resident_clear", None, []),
+    ("qe_stream_last_stats", None, [P]),
     ("qe_gen_events", B32, [P, U64, S32, S32, S32, S32]),
     ("qe_gen_scale_events", B32, [P, S32, S32, S32, S32]),
     ("qe_gen_fact_fk", B32, [P, U32, U64, P, U32]),

@@ -217,6 +217,16 @@ void gpu_hip_profile_tag(const char* tag);
 GPU_Batch* gpu_hip_batch_begin_graph(const char* key);
 /* graph replays and captures since the process started */
 void gpu_hip_graph_stats(U64* replays, U64* captures);
+/* pinned host memory (hipHostMalloc): what host-backed columns live in, so
+ * their chunk copies are asynchronous DMA (registering existing memory,
+ * the reference's host-pointer import, gpu_vulkan.c:2213-2358, is not
+ * used: it is refused on some systems) */
+void* gpu_hip_host_alloc(U64 bytes);
+void gpu_hip_host_free(void* p);
+/* called when a device allocation fails even after the pooled buffers were
+ * freed: returns the bytes it released, and the allocation is tried once
+ * more (g_gpu_evict_hook, query_exec.c:1817-1822) */
+void gpu_hip_set_evict_hook(U64 (*hook)(U64 bytes_wanted));
 /* fault injection: fail the k-th device allocation from now (gpu.h:68) */
 void gpu_hip_fault_alloc_at(U64 k);
 const char* gpu_hip_last_error(void);
@@ -339,6 +349,44 @@ GPU_Buffer* qe_table_column_buffer(QE_Table* table, U32 column);
 B32 qe_table_column_read(QE_Table* table, U32 column, U64 row0, U64 rows, void* out);
 /* recompute zone maps (8192-row min/max) of a column on the device */
 B32 qe_table_build_zonemap(QE_Table* table, U32 column);
+
+/* host-backed tables (the reference's streamed cold path, query_exec.c:
+ * 1674-2491; columns in host memory, cql.h:283-322).  The rows are copied
+ * into pinned host memory; zone maps are built on the device as the rows
+ * pass through once.  A table holds either host-backed or device columns.
+ * qe_scan_filter and qe_aggregate over such a table stream it in chunks of
+ * QE_STREAM_CHUNK_ROWS rows (default 16,777,216, query_exec.c:2096-2099)
+ * through the resident chunk cache, copies of one chunk overlapping the
+ * kernels of the one before; the other operators refuse it. */
+S32 qe_table_add_host_column(QE_Table* table, const char* name, U32 type, const void* data);
+
+/* the resident chunk cache (QE_ResidentStats, query_exec.c:1700-1754):
+ * budget QE_RESIDENT_BUDGET_PERCENT (70) of HBM, at most
+ * QE_RESIDENT_BUDGET_BYTES when set; QE_RESIDENT 0 turns it off */
+typedef struct QE_ResidentStats {
+  U64 hit_bytes;
+  U64 miss_bytes;
+  U64 evicted_bytes;
+  U64 bytes_in_use;
+  U64 entries;
+  U64 budget_bytes;
+} QE_ResidentStats;
+void qe_resident_stats(QE_ResidentStats* out);
+/* free every chunk no query holds (a cold start, bench_common.h:516-560) */
+void qe_resident_clear(void);
+
+/* the last streamed operator of the calling thread */
+typedef struct QE_StreamStats {
+  U64 chunks;           /* chunks of the table */
+  U64 chunks_skipped;   /* every zone pruned: neither copied nor run */
+  U64 chunks_run;
+  U64 chunks_resident;  /* every column read from the cache */
+  U64 transient_chunks; /* column chunks streamed outside the cache */
+  U64 bytes_copied;     /* host to device */
+  U64 copy_wait_ns;     /* time the compute side waited for copies */
+  U64 wall_ns;
+} QE_StreamStats;
+void qe_stream_last_stats(QE_StreamStats* out);
 
 /* synthetic tables generated in HBM (same definitions as the oracle's) */
 B32 qe_gen_events(QE_Table* table, U64 seed, S32 id_col, S32 region_col, S32 amount_col, S32 ts_col);

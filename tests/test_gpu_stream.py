@@ -1,0 +1,294 @@
+"""GPU parity of host-backed tables streamed through the resident chunk
+cache (the reference's cold path: qe_scan_filter_streamed and the streamed
+aggregates, query_exec.c:2023-2491, :8842-9110; the resident cache,
+query_exec.c:1674-1968).  Every scan and aggregate over a host-backed table
+equals the same query over the same rows held whole in HBM: row ids and
+32-bit integer results bit for bit, F32/F64 sums to 1e-12 relative (the
+streamed path adds the per-chunk sums in chunk order); and the integer
+results equal the CPU oracle's.  Chunks are 131072 rows here so that the
+1,000,003-row table spans eight of them, the last one ragged."""
+import threading
+
+import numpy as np
+import pytest
+
+import computeql_amd as cq
+from computeql_amd.ir import col
+
+pytestmark = pytest.mark.gpu
+
+N = 1_000_003
+CHUNK = 131_072
+CHUNKS = (N + CHUNK - 1) // CHUNK
+
+
+@pytest.fixture(autouse=True)
+def _settings():
+    cq.reset_settings()
+    cq.set_setting("QE_STREAM_CHUNK_ROWS", CHUNK)
+    yield
+    cq.reset_settings()
+
+
+def _columns():
+    r = np.random.default_rng(33)
+    return [
+        r.integers(0, 50, N).astype(np.uint32),                        # 0 region
+        r.integers(0, 10_000, N).astype(np.uint32),                    # 1 amount
+        np.arange(N, dtype=np.uint32),                                 # 2 id (sorted: zone maps prune)
+        r.integers(-1_000_000, 1_000_000, N).astype(np.int32),         # 3 signed
+        r.integers(0, 2**32, N, dtype=np.uint64).astype(np.uint32),    # 4 full range
+        (r.random(N) * 1000.0).astype(np.float32),                     # 5 f32
+        r.random(N) * 1e6 + 1.0,                                       # 6 f64 (positive: no cancellation)
+        r.integers(0, 300_000, N).astype(np.uint32),                   # 7 wide key
+    ]
+
+
+TYPES = [cq.U32, cq.U32, cq.U32, cq.I32, cq.U32, cq.F32, cq.F64, cq.U32]
+
+
+@pytest.fixture(scope="module")
+def tables():
+    cq.reset_settings()
+    cq.set_setting("QE_STREAM_CHUNK_ROWS", CHUNK)
+    cols = _columns()
+    dev = cq.Table(N)
+    host = cq.Table(N)
+    for i, (c, ty) in enumerate(zip(cols, TYPES)):
+        dev.add_column(f"c{i}", ty, c)
+        host.add_host_column(f"c{i}", ty, c)
+    cq.reset_settings()
+    yield dev, host, cols
+    host.close()
+    dev.close()
+
+
+WHERES = [
+    None,
+    col(0) < 7,
+    (col(1) >= 123) & (col(1) <= 4567),
+    col(2) < 200_000,                              # six of eight chunks pruned
+    (col(2) >= 500_000) & (col(3) > 0),
+    col(2) > 5_000_000,                            # every chunk pruned
+    (col(0) == 3) | (col(1) > 9_990),              # not an AND: the general scan
+    (col(3) < -900_000) & (col(5) > 500.0),
+    col(6) < 1_000.0,
+]
+
+
+@pytest.mark.parametrize("wi", range(len(WHERES)))
+def test_stream_scan_equals_resident(orc, tables, wi):
+    dev, host, cols = tables
+    w = WHERES[wi]
+    if w is None:
+        nh = host.count(None)
+        assert nh == N == dev.count(None)
+        return
+    nd, idd, _ = dev.scan(w)
+    nh, idh, tr = host.scan(w)
+    assert nh == nd and np.array_equal(idh, idd)
+    assert np.array_equal(idh, orc.scan(cols, w))
+    assert host.count(w) == nd
+    st = cq.stream_stats()
+    assert st["chunks"] == CHUNKS and st["chunks_run"] + st["chunks_skipped"] == CHUNKS
+
+
+def test_scan_cap_keeps_the_first_ids(tables):
+    dev, host, _ = tables
+    w = col(0) < 25
+    nd, idd, _ = dev.scan(w)
+    for cap in (1, 1000, 140_000, nd - 1):
+        nh, idh, _ = host.scan(w, cap=cap)
+        assert nh == nd and np.array_equal(idh, idd[:cap])
+
+
+def _agg_equal(a, b, funcs, args, cols):
+    """a: streamed, b: resident (groups sorted by key on both sides)."""
+    oa, ob = np.argsort(a.keys, kind="stable"), np.argsort(b.keys, kind="stable")
+    assert np.array_equal(a.keys[oa], b.keys[ob])
+    for e, (f, ar) in enumerate(zip(funcs, args)):
+        x, y = a.cells[oa, e], b.cells[ob, e]
+        exact = f in (cq.COUNT, cq.MIN, cq.MAX) or ar is None or cols[ar].dtype in (np.uint32, np.int32)
+        if exact:
+            assert np.array_equal(x, y), (f, ar)
+        else:
+            xf, yf = x.view(np.float64), y.view(np.float64)
+            assert np.allclose(xf, yf, rtol=1e-12, atol=0.0), (f, ar)
+
+
+AGGS = [
+    ([cq.COUNT, cq.SUM, cq.MIN, cq.MAX, cq.AVG], [None, 1, 1, 1, 3]),
+    ([cq.SUM, cq.MIN, cq.MAX, cq.SUM, cq.MIN, cq.MAX, cq.AVG, cq.MAX], [3, 3, 3, 4, 4, 4, 1, 7]),  # two packs
+    ([cq.SUM, cq.MIN, cq.MAX, cq.AVG, cq.SUM], [5, 5, 6, 6, 6]),
+]
+
+
+@pytest.mark.parametrize("wi", [None, 1, 3, 4, 5, 6])
+@pytest.mark.parametrize("ai", range(len(AGGS)))
+def test_stream_global_aggregate(orc, tables, wi, ai):
+    dev, host, cols = tables
+    funcs, args = AGGS[ai]
+    w = None if wi is None else WHERES[wi]
+    a = host.aggregate(funcs, args, where=w)
+    b = dev.aggregate(funcs, args, where=w)
+    assert a.groups == b.groups == 1
+    _agg_equal(a, b, funcs, args, cols)
+    if ai == 0:
+        _, rv = orc.group_aggregate(cols, None, funcs, args, where=w)
+        for e in range(len(funcs)):
+            assert float(a.column(e)[0]) == float(rv[0, e]), e
+
+
+@pytest.mark.parametrize("key,wi", [(0, None), (0, 2), (3, 1), (7, None), (7, 4), (0, 6), (4, 3), (0, 5)])
+@pytest.mark.parametrize("ai", range(len(AGGS)))
+def test_stream_group_by(orc, tables, key, wi, ai):
+    dev, host, cols = tables
+    funcs, args = AGGS[ai]
+    w = None if wi is None else WHERES[wi]
+    a = host.aggregate(funcs, args, where=w, group=key)
+    b = dev.aggregate(funcs, args, where=w, group=key)
+    assert a.groups == b.groups
+    assert np.all(np.diff(a.keys) > 0)        # ascending key order
+    _agg_equal(a, b, funcs, args, cols)
+    if ai == 0:
+        rk, rv = orc.group_aggregate(cols, key, funcs, args, where=w)
+        o = np.argsort(rk, kind="stable")
+        assert np.array_equal(a.keys, rk[o])
+        for e in range(len(funcs)):
+            assert np.array_equal(a.column(e).astype(np.float64), rv[o, e]), e
+
+
+def test_cold_then_warm(tables):
+    dev, host, _ = tables
+    funcs, args = AGGS[0]
+    cq.resident_clear()
+    s0 = cq.resident_stats()
+    a = host.aggregate(funcs, args, group=0)
+    cold = cq.stream_stats()
+    assert cold["chunks_run"] == CHUNKS and cold["bytes_copied"] == N * 4 * 3     # region, amount, signed
+    s1 = cq.resident_stats()
+    assert s1["miss_bytes"] > s0["miss_bytes"] and s1["entries"] == 3 * CHUNKS
+    b = host.aggregate(funcs, args, group=0)
+    warm = cq.stream_stats()
+    assert warm["bytes_copied"] == 0 and warm["chunks_resident"] == CHUNKS
+    assert cq.resident_stats()["hit_bytes"] > s1["hit_bytes"]
+    assert np.array_equal(a.cells, b.cells) and np.array_equal(a.keys, b.keys)
+
+
+def test_zone_pruned_chunks_are_not_copied(tables):
+    _, host, _ = tables
+    cq.resident_clear()
+    host.scan(col(2) < 200_000)
+    st = cq.stream_stats()
+    assert st["chunks_skipped"] == CHUNKS - 2 and st["bytes_copied"] == 2 * CHUNK * 4
+
+
+def test_small_budget_evicts_and_streams(tables):
+    dev, host, cols = tables
+    funcs, args = AGGS[1]
+    cq.resident_clear()
+    one = CHUNK * 4 + 64
+    cq.set_setting("QE_RESIDENT_BUDGET_BYTES", 5 * one)
+    s0 = cq.resident_stats()
+    for _ in range(2):
+        a = host.aggregate(funcs, args, group=0, where=WHERES[1])
+        b = dev.aggregate(funcs, args, group=0, where=WHERES[1])
+        _agg_equal(a, b, funcs, args, cols)
+        s = cq.resident_stats()
+        assert s["bytes_in_use"] <= 5 * one
+    st = cq.stream_stats()
+    assert st["bytes_copied"] > 0
+    assert s["evicted_bytes"] > s0["evicted_bytes"] or st["transient_chunks"] > 0
+
+
+def test_cache_off_streams_through_own_buffers(tables):
+    dev, host, cols = tables
+    cq.set_setting("QE_RESIDENT", 0)
+    funcs, args = AGGS[0]
+    a = host.aggregate(funcs, args, group=3, where=WHERES[2])
+    b = dev.aggregate(funcs, args, group=3, where=WHERES[2])
+    _agg_equal(a, b, funcs, args, cols)
+    st = cq.stream_stats()
+    assert st["transient_chunks"] == 2 * CHUNKS and st["chunks_resident"] == 0      # amount, signed
+
+
+def test_failed_cache_allocation_falls_back(tables):
+    dev, host, cols = tables
+    cq.resident_clear()
+    lib = cq.load()
+    lib.gpu_hip_fault_alloc_at(1)         # the first chunk buffer the cache asks for
+    try:
+        nh, idh, _ = host.scan(WHERES[2])
+    finally:
+        lib.gpu_hip_fault_alloc_at(0)
+    assert cq.stream_stats()["transient_chunks"] >= 1
+    nd, idd, _ = dev.scan(WHERES[2])
+    assert nh == nd and np.array_equal(idh, idd)
+
+
+def test_concurrent_streamed_queries(tables):
+    dev, host, cols = tables
+    cq.resident_clear()
+    funcs, args = AGGS[0]
+    want = {k: dev.aggregate(funcs, args, group=k, where=WHERES[1]) for k in (0, 3)}
+    errors = []
+
+    def worker(k):
+        try:
+            for _ in range(3):
+                got = host.aggregate(funcs, args, group=k, where=WHERES[1])
+                _agg_equal(got, want[k], funcs, args, cols)
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(e)
+
+    ts = [threading.Thread(target=worker, args=(k,)) for k in (0, 3)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+
+
+def test_small_and_empty_host_tables(orc):
+    for n in (0, 5, 8192, 8193):
+        c = np.arange(n, dtype=np.uint32) % 7
+        dev, host = cq.Table(n), cq.Table(n)
+        dev.add_column("a", cq.U32, c)
+        host.add_host_column("a", cq.U32, c)
+        assert host.count(col(0) < 3) == dev.count(col(0) < 3)
+        a = host.aggregate([cq.COUNT, cq.SUM, cq.MIN], [None, 0, 0])
+        b = dev.aggregate([cq.COUNT, cq.SUM, cq.MIN], [None, 0, 0])
+        assert np.array_equal(a.cells, b.cells), n
+        a = host.aggregate([cq.COUNT, cq.MAX], [None, 0], group=0)
+        b = dev.aggregate([cq.COUNT, cq.MAX], [None, 0], group=0)
+        assert a.groups == b.groups
+        _agg_equal(a, b, [cq.COUNT, cq.MAX], [None, 0], [c])
+        host.close()
+        dev.close()
+
+
+def test_release_drops_cached_chunks():
+    c = np.arange(300_000, dtype=np.uint32)
+    t = cq.Table(len(c))
+    t.add_host_column("a", cq.U32, c)
+    cq.resident_clear()
+    t.count(col(0) < 10)
+    before = cq.resident_stats()["entries"]
+    assert before >= 1
+    t.close()
+    assert cq.resident_stats()["entries"] == before - 1
+
+
+def test_host_tables_refuse_device_only_operators(tables):
+    dev, host, _ = tables
+    with pytest.raises(cq.QueryError):
+        host.gather([0], np.arange(10, dtype=np.uint64))
+    with pytest.raises(cq.QueryError):
+        host.encode(0)
+    with pytest.raises(cq.QueryError):
+        cq.hash_join(host, 0, dev, 0)
+    with pytest.raises(cq.QueryError):
+        host.add_column("x", cq.U32, np.zeros(N, dtype=np.uint32))
+    with pytest.raises(cq.QueryError):
+        dev.add_host_column("x", cq.U32, np.zeros(N, dtype=np.uint32))
+    assert np.array_equal(host.read(2, 10, 5), np.arange(10, 15, dtype=np.uint32))
