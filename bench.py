@@ -22,6 +22,10 @@ The same JSON line also carries, measured after the timed steps:
   encoded  the same steps with id, region and amount also kept narrow
            (sequence, 1-byte, 2-byte codes): effective GB/s of the same
            algorithmic bytes, never a fraction of HBM peak
+  streamed q1-q3 over a host-backed copy of region and amount (pinned host
+           memory): cold (every 16M-row chunk copied over PCIe, copies
+           overlapping the previous chunk's kernels; PCIe-inclusive GB/s)
+           and warm (chunks held by the resident chunk cache)
   batched  a 64-query mixed batch over the 10M-row 30-column event table,
            one query per launch vs one shared-scan batch (QPS, speedup)
   served   the request loop under Poisson arrivals at rising rates, plain
@@ -31,7 +35,7 @@ The same JSON line also carries, measured after the timed steps:
            bounded sample of the q1-q7 tables
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N] [--no-served] [--no-encoded]
-                    [--no-cpu-baseline]
+                    [--no-streamed] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -315,6 +319,71 @@ def encoded_run(cq, lib, t, dima, dimb, seed: int, rows: int, steps: int):
                     "q8); q5's generated kernel and q6's join read full width"}
 
 
+def streamed_run(cq, lib, t, rows: int, repeats: int = 3):
+    """The cold path: region and amount copied out of the fact table into a
+    host-backed table (pinned host memory), then q1-q3 over it cold (resident
+    chunk cache emptied, every chunk copied over PCIe while the previous
+    chunk's kernels run) and warm (every chunk in the cache).  Each result
+    must equal the same query over the resident table.  Cold GB/s include the
+    PCIe copies and are never a fraction of HBM peak."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    region, amount = t.read(1), t.read(2)
+    h = cq.Table(rows)
+    h.add_host_column("region", cq.U32, region)
+    h.add_host_column("amount", cq.U32, amount)
+    del region, amount
+    lib.gpu_hip_wait()
+    build_s = time.perf_counter() - t0
+    from computeql_amd.ir import col
+
+    queries = {
+        "q1": (lambda x, reg, amt: x.aggregate([cq.SUM, cq.MIN, cq.MAX], [amt] * 3), 4),
+        "q2": (lambda x, reg, amt: x.scan(col(amt) < 200, count_only=True), 4),
+        "q3": (lambda x, reg, amt: x.aggregate([cq.COUNT, cq.SUM], [None, amt], group=reg), 8),
+    }
+
+    def same(a, b):
+        if isinstance(a, tuple):
+            return a[0] == b[0]
+        oa, ob = np.argsort(a.keys, kind="stable"), np.argsort(b.keys, kind="stable")
+        return np.array_equal(a.keys[oa], b.keys[ob]) and np.array_equal(a.cells[oa], b.cells[ob])
+
+    out = {"rows": rows, "chunk_rows": int(cq.get_setting("QE_STREAM_CHUNK_ROWS", 16777216)),
+           "build_s": round(build_s, 2)}
+    errs = []
+    for q, (fn, bpr) in queries.items():
+        ref = fn(t, 1, 2)
+        t1 = time.perf_counter()
+        fn(t, 1, 2)
+        resident_ms = 1e3 * (time.perf_counter() - t1)
+        cq.resident_clear()
+        t1 = time.perf_counter()
+        cold = fn(h, 0, 1)
+        cold_ms = 1e3 * (time.perf_counter() - t1)
+        st = cq.stream_stats()
+        warm_ms = []
+        for _ in range(repeats):
+            t1 = time.perf_counter()
+            warm = fn(h, 0, 1)
+            warm_ms.append(1e3 * (time.perf_counter() - t1))
+        wst = cq.stream_stats()
+        if not (same(cold, ref) and same(warm, ref)):
+            errs.append(f"streamed {q} differs from resident")
+        nbytes = bpr * rows
+        w = min(warm_ms)
+        out[q] = {"cold_ms": round(cold_ms, 2), "cold_gbs_pcie_inclusive": round(nbytes / cold_ms / 1e6, 1),
+                  "bytes_copied": st["bytes_copied"], "copy_wait_ms": round(st["copy_wait_ns"] / 1e6, 2),
+                  "warm_ms": round(w, 3), "warm_gbs": round(nbytes / w / 1e6, 1),
+                  "warm_bytes_copied": wst["bytes_copied"], "resident_ms": round(resident_ms, 3)}
+        log(f"streamed {q}: cold {cold_ms:.1f} ms, warm {w:.2f} ms, resident {resident_ms:.2f} ms")
+    h.close()
+    cq.resident_clear()
+    out["invariants"] = "ok" if not errs else errs
+    return out
+
+
 def profile_stats(lib, name: str):
     d = ctypes.c_uint64(0)
     ns = lib.gpu_hip_profile_query(name.encode(), ctypes.byref(d))
@@ -333,6 +402,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-served", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--served-seconds", type=float, default=1.0)
     ap.add_argument("--no-encoded", action="store_true", help="skip the narrow-encoded rerun of the steps")
+    ap.add_argument("--no-streamed", action="store_true", help="skip the host-backed (cold/warm) streamed runs")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, read one column once (the known-size read that scales "
                          "rocprofv3 FETCH_SIZE readings)")
@@ -407,6 +477,11 @@ def main(argv=None) -> int:
         encoded = encoded_run(cq, lib, t, dima, dimb, seed, rows, a.steps)
         errs += [] if encoded["invariants"] == "ok" else [f"encoded {e}" for e in encoded["invariants"]]
 
+    streamed = None
+    if not a.no_streamed:
+        streamed = streamed_run(cq, lib, t, rows)
+        errs += [] if streamed["invariants"] == "ok" else streamed["invariants"]
+
     batched = served = None
     if not a.no_served:
         t.close()
@@ -449,6 +524,7 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "encoded": encoded,
+        "streamed": streamed,
         "batched": batched,
         "served": served,
     }
