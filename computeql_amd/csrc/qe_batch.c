@@ -23,7 +23,7 @@
 
 #include "qe_internal.h"
 
-#define QE_BATCH_REC_WORDS 12
+#define QE_BATCH_REC_WORDS 14
 #define QE_BATCH_MAX_LAUNCH_QUERIES 256
 
 typedef struct QE_BatchPlanned {
@@ -34,7 +34,22 @@ typedef struct QE_BatchPlanned {
   S32 group_col;
   U32 key_min, range;
   U32 acc_off, acc_words;
+  U32 reps, lds_off, lds_words; /* LDS replicas of the words (grouped queries) */
 } QE_BatchPlanned;
+
+/* LDS replicas of a grouped query's words: lanes of the owning wave whose
+ * rows fall in the same group contend on one LDS word; with `reps` copies
+ * (lane l adding into copy l mod reps) about QE_BATCH_REPLICA_LANES (32)
+ * active lanes spread over range x reps words.  At most 16 copies and
+ * 16 KB per query. */
+static U32 qe_batch_reps(U32 range, U32 words)
+{
+  U64 lanes = qe_settings_u64("QE_BATCH_REPLICA_LANES", 32);
+  U32 reps = 1;
+  while (reps < 16 && (U64)range * reps < lanes) reps <<= 1;
+  while (reps > 1 && (U64)words * reps * 8 > 16384) reps >>= 1;
+  return reps;
+}
 
 static B32 qe_batch_int32(U32 t) { return t == CQL_ColumnType_U32 || t == CQL_ColumnType_I32; }
 
@@ -70,6 +85,8 @@ static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlan
     out->key_min = k->type == CQL_ColumnType_I32 ? ((U32)(S32)k->min_value ^ 0x80000000u) : (U32)k->min_value;
   }
   out->acc_words = out->range * (1 + 3 * out->plan.arg_count);
+  out->reps = q->group_col >= 0 ? qe_batch_reps(out->range, out->acc_words) : 1;
+  out->lds_words = out->acc_words * out->reps;
   return 1;
 }
 
@@ -108,19 +125,21 @@ static U64 qe_batch_lds(U64 W, U64 stage_bytes) { return qe_batch_lds_n(W, stage
 
 /* program words of one launch (layout: qk_library.hip, shared scan) */
 static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items, U32* out_words, U32* out_w,
-                             U64* out_stage_bytes)
+                             U32* out_wl, U64* out_stage_bytes)
 {
   QE_BatchStage st;
   memset(&st, 0, sizeof(st));
-  U32 W = 0;
+  U32 W = 0, WL = 0;
   for (U32 i = 0; i < n; i++) {
     qs[i].acc_off = W;
     W += qs[i].acc_words;
+    qs[i].lds_off = WL;
+    WL += qs[i].lds_words;
     qe_batch_stage_query(&st, &qs[i]);
   }
   U32 rec_total = 0;
   for (U32 i = 0; i < n; i++) rec_total += QE_BATCH_REC_WORDS + 6 * qs[i].leaf_count;
-  const U32 hdr = 5;
+  const U32 hdr = 6;
   U32 words = hdr + n + 3 * st.count + rec_total + W;
   U32* p = (U32*)calloc(words, 4);
   if (!p) return 0;
@@ -130,6 +149,7 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
   p[2] = hdr + n + 3 * st.count + rec_total;
   p[3] = st.count;
   p[4] = (U32)stage_bytes;
+  p[5] = WL;
   U32 off = 0;
   for (U32 u = 0; u < st.count; u++) {
     U32 w = t->columns[st.column[u]].width;
@@ -155,6 +175,8 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
       r[8 + 2 * a] = t->columns[q->plan.arg_col[a]].type;
     }
     r[11] = q->acc_off;
+    r[12] = q->lds_off;
+    r[13] = q->reps;
     for (U32 l = 0; l < q->leaf_count; l++)
       qe_simple_leaf_words(&r[QE_BATCH_REC_WORDS + 6 * l], qe_batch_stage_of(&st, q->leaves[l].column),
                            t->columns[q->leaves[l].column].type, q->leaves[l].op, q->leaves[l].target);
@@ -175,6 +197,7 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
   }
   *out_words = words;
   *out_w = W;
+  *out_wl = WL;
   *out_stage_bytes = stage_bytes;
   return p;
 }
@@ -237,7 +260,7 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
   U64 Wq = 0;
   for (U32 i = 0; i < n; i++) {
     qe_batch_stage_query(&st, &qs[i]);
-    Wq += qs[i].acc_words;
+    Wq += qs[i].lds_words;
   }
   U64 want_wg = cfg->wg;
   U64 dbl_mode = cfg->buffers;
@@ -255,14 +278,14 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
     buffers = (dbl_mode == 1 || (dbl_mode == 2 && qe_batch_lds_n(Wq, sb, 2) <= max_lds)) ? 2 : 1;
     found = 1;
   }
-  U32 words = 0, W = 0;
+  U32 words = 0, W = 0, WL = 0;
   U64 stage_bytes = 0;
-  U32* prog = qe_batch_program(t, qs, n, items * wg / QE_WG, &words, &W, &stage_bytes);
+  U32* prog = qe_batch_program(t, qs, n, items * wg / QE_WG, &words, &W, &WL, &stage_bytes);
   if (!prog) {
     qe_set_error("qe_aggregate_batch: out of memory");
     return 0;
   }
-  U64 lds = qe_batch_lds_n(W, stage_bytes, buffers);
+  U64 lds = qe_batch_lds_n(WL, stage_bytes, buffers);
   U64 rows = t->row_count;
   U64 tile = (U64)wg * items;
   U64 tiles = (rows + tile - 1) / tile;
@@ -301,12 +324,13 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
     U32 rblocks = (W + QE_WG / 64 - 1) / (QE_WG / 64); /* one wave per word */
     /* QE_BATCH_GRAPH (default 1): the launch runs as a HIP graph that the
      * next batch of the same shape over the same table replays (only the
-     * program bytes change) */
+     * program bytes change); the shape includes the dynamic LDS size,
+     * which the captured launch keeps */
     GPU_Batch* b;
     if (qe_settings_u64("QE_BATCH_GRAPH", 1)) {
       char key[160];
-      snprintf(key, sizeof(key), "batch:%s:%p:%u:%u:%u:%llu", kname, (void*)t, words, W, buffers,
-               (unsigned long long)blocks);
+      snprintf(key, sizeof(key), "batch:%s:%p:%u:%u:%u:%llu:%llu", kname, (void*)t, words, W, buffers,
+               (unsigned long long)blocks, (unsigned long long)lds);
       b = gpu_hip_batch_begin_graph(key);
     } else {
       b = gpu_batch_begin(0, (U64)W * 8);
@@ -400,7 +424,7 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
     memset(&one, 0, sizeof(one));
     if (qe_batch_plan_query(table, &queries[i], &planned[np]) &&
         (qe_batch_stage_query(&one, &planned[np]),
-         qe_batch_lds(planned[np].acc_words, qe_batch_stage_bytes(table, &one, 1)) <= max_lds)) {
+         qe_batch_lds(planned[np].lds_words, qe_batch_stage_bytes(table, &one, 1)) <= max_lds)) {
       planned[np].index = i;
       np++;
     } else {
@@ -442,7 +466,7 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
           QE_BatchStage grown = st;
           qe_batch_stage_query(&grown, &planned[i]);
           U64 add = qe_batch_stage_bytes(table, &grown, 4) - base;
-          if (in_launch && qe_batch_lds(words + planned[i].acc_words, base + add) > lds_budget) continue;
+          if (in_launch && qe_batch_lds(words + planned[i].lds_words, base + add) > lds_budget) continue;
           if (add < best_add) {
             best_add = add;
             best = i;
@@ -450,7 +474,7 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
         }
         if (best == np || in_launch == QE_BATCH_MAX_LAUNCH_QUERIES) break;
         qe_batch_stage_query(&st, &planned[best]);
-        words += planned[best].acc_words;
+        words += planned[best].lds_words;
         taken[best] = 1;
         ordered[done++] = planned[best];
         in_launch++;

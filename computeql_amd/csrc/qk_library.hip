@@ -1995,15 +1995,19 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /* Program (u32 words, binding 0):                                      */
 /*   [0] queries Q  [1] accumulator words W  [2] offset of the op table */
 /*   [3] staged columns U  [4] bytes of one stage buffer                */
-/*   [5 + q] offset of query q's record                                 */
-/*   [5 + Q + 3u] staged column u: table column, width in bytes (4 or   */
+/*   [5] LDS accumulator words WL (W with every replica)                */
+/*   [6 + q] offset of query q's record                                 */
+/*   [6 + Q + 3u] staged column u: table column, width in bytes (4 or   */
 /*   8), byte offset of its rows in a stage buffer (16-byte aligned)    */
 /* query record: [0] leaves L (<= 8), [1] flags (1 grouped, 2 sums,     */
 /*   4 min/max), [2] key (staged index), [3] key type, [4] key_min code,*/
 /*   [5] groups R (1 when ungrouped), [6] args A (<= 2), [7] arg 0      */
 /*   (staged index), [8] arg type 0, [9] arg 1, [10] arg type 1,        */
-/*   [11] accumulator offset, [12..] leaves, 6 words each               */
-/*   (qe_simple_leaf_words layout, slot = staged index).                */
+/*   [11] offset of its words among the W, [12] offset of its LDS      */
+/*   words, [13] replicas (a power of two; lane l of the owning wave    */
+/*   adds into replica l mod replicas, the block folds them at its end),*/
+/*   [14..] leaves, 6 words each (qe_simple_leaf_words layout, slot =   */
+/*   staged index).                                                     */
 /* Columns: binding 2 holds the base address of every table column.     */
 /* A block walks tiles of 256 x ITEMS rows.  The rows of every staged   */
 /* column of a tile are copied straight from HBM into an LDS stage      */
@@ -2020,11 +2024,12 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /* value).  op table: one u32 per accumulator word (0 integer add,      */
 /* 1 f64 add, 2 min, 3 max).  Each block writes its W words to          */
 /* binding 1 [block][W]; shared_reduce folds them in block order.       */
-/* LDS: accumulators u64[W] (padded to 16 bytes) | stage buffer 0 |     */
+/* LDS: accumulators u64[WL] (padded to 16 bytes) | stage buffer 0 |    */
 /* stage buffer 1 (when double-buffered, push slot 6).                  */
 /* ------------------------------------------------------------------ */
 #define QK_SS_MAX_ARGS 2
-#define QK_SS_HDR 5
+#define QK_SS_HDR 6
+#define QK_SS_REC 14
 
 __device__ __forceinline__ bool qk_type_is_wide(qk_u32 t) { return t == QK_T_F64 || t == QK_T_I64 || t == QK_T_U64; }
 
@@ -2074,6 +2079,43 @@ __device__ __forceinline__ double qk_wave_add_f64(double d)
   v = (qk_u64)__double_as_longlong(d);
   return (__longlong_as_double((qk_s64)qk_readlane64(v, 0)) + __longlong_as_double((qk_s64)qk_readlane64(v, 16))) +
          (__longlong_as_double((qk_s64)qk_readlane64(v, 32)) + __longlong_as_double((qk_s64)qk_readlane64(v, 48)));
+}
+
+/* 32-bit forms: one DPP-fed operation per step */
+__device__ __forceinline__ qk_u32 qk_wave_add_u32(qk_u32 v)
+{
+  v += (qk_u32)__builtin_amdgcn_update_dpp(0, (int)v, QK_DPP_XOR1, 0xf, 0xf, false);
+  v += (qk_u32)__builtin_amdgcn_update_dpp(0, (int)v, QK_DPP_XOR2, 0xf, 0xf, false);
+  v += (qk_u32)__builtin_amdgcn_update_dpp(0, (int)v, QK_DPP_HALF_MIRROR, 0xf, 0xf, false);
+  v += (qk_u32)__builtin_amdgcn_update_dpp(0, (int)v, QK_DPP_MIRROR, 0xf, 0xf, false);
+  return (qk_u32)__builtin_amdgcn_readlane((int)v, 0) + (qk_u32)__builtin_amdgcn_readlane((int)v, 16) +
+         (qk_u32)__builtin_amdgcn_readlane((int)v, 32) + (qk_u32)__builtin_amdgcn_readlane((int)v, 48);
+}
+
+__device__ __forceinline__ void qk_wave_minmax_u32(qk_u32& mn, qk_u32& mx)
+{
+#define QK_MM32_STEP(C)                                                                   \
+  {                                                                                       \
+    const qk_u32 a2 = (qk_u32)__builtin_amdgcn_update_dpp(0, (int)mn, C, 0xf, 0xf, false); \
+    const qk_u32 b2 = (qk_u32)__builtin_amdgcn_update_dpp(0, (int)mx, C, 0xf, 0xf, false); \
+    mn = a2 < mn ? a2 : mn;                                                               \
+    mx = b2 > mx ? b2 : mx;                                                               \
+  }
+  QK_MM32_STEP(QK_DPP_XOR1)
+  QK_MM32_STEP(QK_DPP_XOR2)
+  QK_MM32_STEP(QK_DPP_HALF_MIRROR)
+  QK_MM32_STEP(QK_DPP_MIRROR)
+#undef QK_MM32_STEP
+  qk_u32 n0 = (qk_u32)__builtin_amdgcn_readlane((int)mn, 0), n1 = (qk_u32)__builtin_amdgcn_readlane((int)mn, 16);
+  qk_u32 n2 = (qk_u32)__builtin_amdgcn_readlane((int)mn, 32), n3 = (qk_u32)__builtin_amdgcn_readlane((int)mn, 48);
+  qk_u32 x0 = (qk_u32)__builtin_amdgcn_readlane((int)mx, 0), x1 = (qk_u32)__builtin_amdgcn_readlane((int)mx, 16);
+  qk_u32 x2 = (qk_u32)__builtin_amdgcn_readlane((int)mx, 32), x3 = (qk_u32)__builtin_amdgcn_readlane((int)mx, 48);
+  n0 = n1 < n0 ? n1 : n0;
+  n2 = n3 < n2 ? n3 : n2;
+  mn = n2 < n0 ? n2 : n0;
+  x0 = x1 > x0 ? x1 : x0;
+  x2 = x3 > x2 ? x3 : x2;
+  mx = x2 > x0 ? x2 : x0;
 }
 
 __device__ __forceinline__ void qk_wave_minmax_u64(qk_u64& mn, qk_u64& mx)
@@ -2206,12 +2248,23 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
   extern __shared__ __attribute__((aligned(16))) qk_u64 lds_u64[];
   const qk_cu32p prog = (qk_cu32p)a.ptr[0];
   const qk_cu64p cols = (qk_cu64p)a.ptr[2];
-  const qk_u32 Q = prog[0], W = prog[1], U = prog[3], stage_bytes = prog[4];
+  const qk_u32 Q = prog[0], W = prog[1], U = prog[3], stage_bytes = prog[4], WL = prog[5];
   const qk_cu32p ops = prog + prog[2];
   const qk_cu32p st = prog + QK_SS_HDR + Q;
   qk_u64* acc = lds_u64;
-  unsigned char* buf0 = (unsigned char*)lds_u64 + (((qk_u64)W * 8 + 15) & ~15ULL);
-  for (qk_u32 i = threadIdx.x; i < W; i += WG) acc[i] = ops[i] == 2u ? ~0ULL : 0ULL;
+  unsigned char* buf0 = (unsigned char*)lds_u64 + (((qk_u64)WL * 8 + 15) & ~15ULL);
+  {
+    /* every replica of a query's words starts at its op's identity */
+    const qk_u32 wave = (qk_u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63u;
+    for (qk_u32 q = wave; q < Q; q += WG / 64) {
+      const qk_cu32p rec = prog + prog[QK_SS_HDR + q];
+      const qk_u32 A = rec[6] < QK_SS_MAX_ARGS ? rec[6] : QK_SS_MAX_ARGS;
+      const qk_u32 Wq = rec[5] * (1u + 3u * A), reps = rec[13];
+      const qk_cu32p qops = ops + rec[11];
+      for (qk_u32 c = 0; c < reps; c++)
+        for (qk_u32 j = lane; j < Wq; j += 64) acc[rec[12] + c * Wq + j] = qops[j] == 2u ? ~0ULL : 0ULL;
+    }
+  }
   const qk_u64 rows = qk_push(a, QK_P_ROWS);
   const qk_u64 tile = (qk_u64)WG * ITEMS;
   const qk_u64 tiles = (rows + tile - 1) / tile;
@@ -2250,7 +2303,7 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
       qk_u32 m = valid;
 #pragma unroll 1
       for (qk_u32 l = 0; l < L; l++) {
-        const qk_cu32p w = rec + 12 + 6 * l;
+        const qk_cu32p w = rec + QK_SS_REC + 6 * l;
         const qk_u32 u = w[0], tw = w[1], op = w[2], type = tw & 0xffu;
         const unsigned char* cp = sb + st[3 * u + 2];
         if (tw & QK_LEAF_NEVER) {
@@ -2271,30 +2324,71 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
         }
       }
       const qk_u32 R = rec[5], A = rec[6] < QK_SS_MAX_ARGS ? rec[6] : QK_SS_MAX_ARGS;
-      qk_u64* qa = acc + rec[11];
+      qk_u64* qa = acc + rec[12];
       if (flags & 1u) {
         /* grouped: LDS atomics into the query's group table (only this
-         * wave writes it) */
-        if (!m) continue;
+         * wave writes it); lane l adds into replica l mod reps, so lanes
+         * whose rows share a group contend reps times less.  The rows are
+         * walked in a fixed unrolled order under a per-row predicate, with
+         * the group of every row and the argument values read first. */
+        if (!__any((int)(m != 0))) continue;
+        qa += (qk_u64)(lane & (rec[13] - 1u)) * (R * (1u + 3u * A));
         const qk_u32* kp = (const qk_u32*)(sb + st[3 * rec[2] + 2]);
-        const qk_u32 kt = rec[3], kmin = rec[4];
-        for (qk_u32 mm = m; mm; mm &= mm - 1) {
-          const qk_u32 i = (qk_u32)__builtin_ctz(mm);
-          const qk_u32 kv = kp[i * 64 + lane];
-          const qk_u32 g = (kt == QK_T_I32 ? (kv ^ 0x80000000u) : kv) - kmin;
-          if (g >= R) continue;
-          atomicAdd((unsigned long long*)&qa[g], 1ULL);
-          for (qk_u32 k = 0; k < A; k++) {
-            const qk_u32 at = rec[8 + 2 * k];
-            const double v = qk_ss_f64(sb + st[3 * rec[7 + 2 * k] + 2], at, i * 64 + lane);
-            qk_u64* qs = qa + R + (qk_u64)k * 3 * R;
+        const qk_u32 kx = rec[3] == QK_T_I32 ? 0x80000000u : 0u, kmin = rec[4];
+        qk_u32 g[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+          g[i] = (kp[i * 64 + lane] ^ kx) - kmin;
+          if (g[i] >= R) m &= ~(1u << i);
+        }
+#pragma unroll
+        for (int i = 0; i < RPL; i++)
+          if ((m >> i) & 1u) atomicAdd((unsigned long long*)&qa[g[i]], 1ULL);
+#pragma unroll 1
+        for (qk_u32 k = 0; k < A; k++) {
+          const qk_u32 at = rec[8 + 2 * k];
+          const unsigned char* ap = sb + st[3 * rec[7 + 2 * k] + 2];
+          qk_u64* qs = qa + R + (qk_u64)k * 3 * R;
+          if (qk_type_is_int32(at)) {
+            /* 32-bit integers: exact sums without a trip through f64 */
+            const qk_u32* av = (const qk_u32*)ap;
+            const bool sgn = at == QK_T_I32;
             if (flags & 2u) {
-              if (qk_type_is_int32(at)) atomicAdd((unsigned long long*)&qs[g], (unsigned long long)(qk_s64)v);
-              else atomicAdd((double*)&qs[g], v);
+#pragma unroll
+              for (int i = 0; i < RPL; i++) {
+                const qk_u32 x = av[i * 64 + lane];
+                if ((m >> i) & 1u)
+                  atomicAdd((unsigned long long*)&qs[g[i]], sgn ? (qk_u64)(qk_s64)(qk_s32)x : (qk_u64)x);
+              }
             }
             if (flags & 4u) {
-              atomicMin((unsigned long long*)&qs[R + g], (unsigned long long)qk_f64_key(v));
-              atomicMax((unsigned long long*)&qs[2 * R + g], (unsigned long long)qk_f64_key(v));
+#pragma unroll
+              for (int i = 0; i < RPL; i++) {
+                const qk_u32 x = av[i * 64 + lane];
+                const qk_u64 kk = qk_f64_key(sgn ? (double)(qk_s32)x : (double)x);
+                if ((m >> i) & 1u) {
+                  atomicMin((unsigned long long*)&qs[R + g[i]], (unsigned long long)kk);
+                  atomicMax((unsigned long long*)&qs[2 * R + g[i]], (unsigned long long)kk);
+                }
+              }
+            }
+          } else {
+            double v[RPL];
+            qk_ss_rows<RPL>(ap, at, lane, v);
+            if (flags & 2u) {
+#pragma unroll
+              for (int i = 0; i < RPL; i++)
+                if ((m >> i) & 1u) atomicAdd((double*)&qs[g[i]], v[i]);
+            }
+            if (flags & 4u) {
+#pragma unroll
+              for (int i = 0; i < RPL; i++) {
+                const qk_u64 kk = qk_f64_key(v[i]);
+                if ((m >> i) & 1u) {
+                  atomicMin((unsigned long long*)&qs[R + g[i]], (unsigned long long)kk);
+                  atomicMax((unsigned long long*)&qs[2 * R + g[i]], (unsigned long long)kk);
+                }
+              }
             }
           }
         }
@@ -2302,38 +2396,73 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
         /* ungrouped: the lane's rows in registers, then the wave (DPP),
          * then lane 0 into the query's words (only this wave writes them) */
         if (!__any((int)(m != 0))) continue;
-        const qk_u64 cnt = qk_wave_add_u64((qk_u64)__popc(m));
+        const qk_u32 cnt = qk_wave_add_u32((qk_u32)__popc(m)); /* at most 64 x 32 */
         if (lane == 0) qa[0] += cnt;
+#pragma unroll 1
         for (qk_u32 k = 0; k < A; k++) {
           const qk_u32 at = rec[8 + 2 * k];
           const unsigned char* ap = sb + st[3 * rec[7 + 2 * k] + 2];
           qk_u64* qs = qa + 1 + (qk_u64)k * 3;
-          double v[RPL];
-          qk_ss_rows<RPL>(ap, at, lane, v);
-          if (flags & 2u) {
-            if (qk_type_is_int32(at)) {
+          qk_u64 mn = ~0ULL, mx = 0ULL;
+          if (qk_type_is_int32(at)) {
+            /* 32-bit integers: exact integer sums, and the lane's min/max
+             * compared as integers (the f64 key is monotonic in the value)
+             * and keyed once */
+            const qk_u32* av = (const qk_u32*)ap;
+            const bool sgn = at == QK_T_I32;
+            const qk_u32 flip = sgn ? 0x80000000u : 0u;
+            if (flags & 2u) {
               qk_u64 su = 0;
 #pragma unroll
-              for (int i = 0; i < RPL; i++) su += (m & (1u << i)) ? (qk_u64)(qk_s64)v[i] : 0ULL;
+              for (int i = 0; i < RPL; i++) {
+                const qk_u32 x = av[i * 64 + lane];
+                su += ((m >> i) & 1u) ? (sgn ? (qk_u64)(qk_s64)(qk_s32)x : (qk_u64)x) : 0ULL;
+              }
               su = qk_wave_add_u64(su);
               if (lane == 0) qs[0] += su;
-            } else {
+            }
+            if (flags & 4u) {
+              /* order codes (value ^ flip) reduced as 32-bit integers; some
+               * lane of the wave has a row here, so the wave's codes are
+               * real values, keyed once */
+              qk_u32 lo = ~0u, hi = 0u;
+#pragma unroll
+              for (int i = 0; i < RPL; i++) {
+                const qk_u32 c = av[i * 64 + lane] ^ flip;
+                const bool in = (m >> i) & 1u;
+                lo = (in && c < lo) ? c : lo;
+                hi = (in && c > hi) ? c : hi;
+              }
+              qk_wave_minmax_u32(lo, hi);
+              if (lane == 0) {
+                const qk_u64 kn = qk_f64_key(sgn ? (double)(qk_s32)(lo ^ flip) : (double)lo);
+                const qk_u64 kx = qk_f64_key(sgn ? (double)(qk_s32)(hi ^ flip) : (double)hi);
+                qs[1] = kn < qs[1] ? kn : qs[1];
+                qs[2] = kx > qs[2] ? kx : qs[2];
+              }
+              continue;
+            }
+          } else {
+            double v[RPL];
+            qk_ss_rows<RPL>(ap, at, lane, v);
+            if (flags & 2u) {
               double sd = 0.0;
 #pragma unroll
               for (int i = 0; i < RPL; i++) sd += (m & (1u << i)) ? v[i] : 0.0;
               sd = qk_wave_add_f64(sd);
               if (lane == 0) qs[0] = (qk_u64)__double_as_longlong(__longlong_as_double((qk_s64)qs[0]) + sd);
             }
+            if (flags & 4u) {
+#pragma unroll
+              for (int i = 0; i < RPL; i++) {
+                const qk_u64 kk = qk_f64_key(v[i]);
+                const bool in = (m >> i) & 1u;
+                mn = (in && kk < mn) ? kk : mn;
+                mx = (in && kk > mx) ? kk : mx;
+              }
+            }
           }
           if (flags & 4u) {
-            qk_u64 mn = ~0ULL, mx = 0ULL;
-#pragma unroll
-            for (int i = 0; i < RPL; i++) {
-              const qk_u64 kk = qk_f64_key(v[i]);
-              const bool in = (m >> i) & 1u;
-              mn = (in && kk < mn) ? kk : mn;
-              mx = (in && kk > mx) ? kk : mx;
-            }
             qk_wave_minmax_u64(mn, mx);
             if (lane == 0) {
               qs[1] = mn < qs[1] ? mn : qs[1];
@@ -2345,8 +2474,27 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
     }
   }
   __syncthreads();
+  /* the block's words: each query's replicas folded with the word's op */
   qk_u64* out = qk_wptr<qk_u64>(a, 1) + (qk_u64)blockIdx.x * W;
-  for (qk_u32 i = threadIdx.x; i < W; i += WG) out[i] = acc[i];
+  const qk_u32 fwave = (qk_u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), flane = threadIdx.x & 63u;
+  for (qk_u32 q = fwave; q < Q; q += WG / 64) {
+    const qk_cu32p rec = prog + prog[QK_SS_HDR + q];
+    const qk_u32 A = rec[6] < QK_SS_MAX_ARGS ? rec[6] : QK_SS_MAX_ARGS;
+    const qk_u32 Wq = rec[5] * (1u + 3u * A), reps = rec[13], o = rec[11];
+    const qk_u64* qa = acc + rec[12];
+    for (qk_u32 j = flane; j < Wq; j += 64) {
+      const qk_u32 op = ops[o + j];
+      qk_u64 x = qa[j];
+      for (qk_u32 c = 1; c < reps; c++) {
+        const qk_u64 y = qa[c * Wq + j];
+        if (op == 0u) x += y;
+        else if (op == 1u) x = (qk_u64)__double_as_longlong(__longlong_as_double((qk_s64)x) + __longlong_as_double((qk_s64)y));
+        else if (op == 2u) x = y < x ? y : x;
+        else x = y > x ? y : x;
+      }
+      out[o + j] = x;
+    }
+  }
 }
 
 /* instances: shared_scan_<block size>_<rows per thread staged> */

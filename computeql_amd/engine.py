@@ -16,6 +16,15 @@ import numpy as np
 from . import ir
 from .native import QE_BatchConfig, QE_BatchQuery, QE_BatchTrace, QE_ResidentStats, QE_StreamStats, QE_Trace, load
 
+# QE_BatchQuery as a numpy record: a batch's records are filled column-wise
+_BQ_DTYPE = np.dtype({
+    "names": [f for f, _ in QE_BatchQuery._fields_],
+    "formats": ["<u8" if t in (C.c_void_p, C.c_uint64) else ("<i4" if t is C.c_int32 else "<u4")
+                for _, t in QE_BatchQuery._fields_],
+    "offsets": [getattr(QE_BatchQuery, f).offset for f, _ in QE_BatchQuery._fields_],
+    "itemsize": C.sizeof(QE_BatchQuery),
+})
+
 U32, U64, F32, F64, I32, I64 = 1, 2, 3, 4, 7, 8
 COUNT, SUM, AVG, MIN, MAX = 0, 1, 2, 3, 4
 FUNC_NAMES = {COUNT: "COUNT", SUM: "SUM", AVG: "AVG", MIN: "MIN", MAX: "MAX"}
@@ -267,6 +276,60 @@ class AggSpec:
     group: int | None = None
     cap: int | None = None
 
+    def prepared(self):
+        """(funcs, u32 function codes, i32 argument columns, IR nodes, and
+        their addresses: (nodes, node count, codes, columns)) as the C ABI
+        takes them, built once per spec (a spec is not changed after
+        it is built), so a query submitted again converts nothing."""
+        p = self.__dict__.get("_prepared")
+        if p is None:
+            funcs = list(self.funcs)
+            fa = np.asarray(funcs, dtype=np.uint32)
+            aa = np.asarray([-1 if a is None else int(a) for a in self.args], dtype=np.int32)
+            nodes = ir.to_nodes(self.where)
+            ptrs = (nodes.ctypes.data if nodes is not None else 0, len(nodes) if nodes is not None else 0,
+                    fa.ctypes.data, aa.ctypes.data)
+            p = (funcs, fa, aa, nodes, ptrs)
+            self._prepared = p
+        return p
+
+
+def _batch_records(specs):
+    """The QE_BatchQuery array of a batch as one numpy record array (the C
+    struct's layout) and the outputs: every query's keys and cells are
+    views of one arena.  A grouped query gets room for 4096 groups unless
+    its spec says otherwise (the shared scan fuses at most 1024; a larger
+    result is run again with room for all of it)."""
+    n = len(specs)
+    recs = np.zeros(max(n, 1), dtype=_BQ_DTYPE)
+    counts = np.zeros(max(n, 1), dtype=np.uint64)
+    preps = [sp.prepared() for sp in specs]
+    caps = [sp.cap if sp.cap is not None else (1 if sp.group is None else 1 << 12) for sp in specs]
+    caps = [max(c, 1) for c in caps]
+    widths = [len(p[0]) for p in preps]
+    sizes = [c * (1 + e) for c, e in zip(caps, widths)]
+    arena = np.empty(max(sum(sizes), 1), dtype=np.uint64)
+    base = arena.ctypes.data
+    offs = np.zeros(n, dtype=np.uint64)
+    if n > 1:
+        offs[1:] = np.cumsum(sizes[:-1], dtype=np.uint64)
+    outs = []
+    for (funcs, _, _, _, _), cap, e, o in zip(preps, caps, widths, offs.tolist()):
+        outs.append((funcs, arena[o:o + cap].view(np.float64), arena[o + cap:o + cap * (1 + e)].reshape(cap, e), cap))
+    if n:
+        recs["where"][:n] = [p[4][0] for p in preps]
+        recs["where_count"][:n] = [p[4][1] for p in preps]
+        recs["group_col"][:n] = [-1 if sp.group is None else int(sp.group) for sp in specs]
+        recs["funcs"][:n] = [p[4][2] for p in preps]
+        recs["arg_cols"][:n] = [p[4][3] for p in preps]
+        recs["expr_count"][:n] = widths
+        recs["cap"][:n] = caps
+        recs["out_group_count"][:n] = counts.ctypes.data + 8 * np.arange(n, dtype=np.uint64)
+        o = base + 8 * offs
+        recs["out_keys"][:n] = o
+        recs["out_cells"][:n] = o + 8 * np.asarray(caps, dtype=np.uint64)
+    return recs, counts, outs
+
 
 def aggregate_batch(table: Table, specs, with_trace: bool = False):
     """qe_aggregate_batch: many aggregate queries over one table in shared
@@ -275,46 +338,22 @@ def aggregate_batch(table: Table, specs, with_trace: bool = False):
     lib = load()
     specs = [s if isinstance(s, AggSpec) else AggSpec(**s) for s in specs]
     n = len(specs)
-    qs = (QE_BatchQuery * max(n, 1))()
-    keep = []
-    outs = []
-    for i, sp in enumerate(specs):
-        funcs = list(sp.funcs)
-        e = len(funcs)
-        fa = (C.c_uint32 * e)(*funcs)
-        aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in sp.args])
-        nodes = ir.to_nodes(sp.where)
-        cap = sp.cap if sp.cap is not None else (1 if sp.group is None else 1 << 17)
-        keys = np.empty(max(cap, 1), dtype=np.float64)
-        cells = np.empty((max(cap, 1), e), dtype=np.uint64)
-        g = C.c_uint64(0)
-        keep += [fa, aa, nodes, keys, cells, g]
-        q = qs[i]
-        q.where = nodes.ctypes.data if nodes is not None else None
-        q.where_count = len(nodes) if nodes is not None else 0
-        q.group_col = -1 if sp.group is None else int(sp.group)
-        q.funcs = C.addressof(fa)
-        q.arg_cols = C.addressof(aa)
-        q.expr_count = e
-        q.cap = cap
-        q.out_group_count = C.addressof(g)
-        q.out_keys = keys.ctypes.data
-        q.out_cells = cells.ctypes.data
-        outs.append((funcs, keys, cells, g, cap))
+    recs, counts, outs = _batch_records(specs)
     tr = QE_BatchTrace()
-    ok = lib.qe_aggregate_batch(table.handle, qs, n, C.byref(tr))
+    ok = lib.qe_aggregate_batch(table.handle, recs.ctypes.data, n, C.byref(tr))
     results = []
-    for i, (funcs, keys, cells, g, cap) in enumerate(outs):
-        if not qs[i].ok:
+    oks, fused = recs["ok"], recs["fused"]
+    for i, (funcs, keys, cells, cap) in enumerate(outs):
+        if not oks[i]:
             raise _err(f"aggregate_batch query {i}")
-        ng = int(g.value)
+        ng = int(counts[i])
         if ng > cap:
             sp = specs[i]
             r = table.aggregate(sp.funcs, sp.args, sp.where, sp.group, cap=ng)
             r.trace["fused"] = 0
             results.append(r)
             continue
-        results.append(AggResult(funcs, keys[:ng].copy(), cells[:ng].copy(), {"fused": int(qs[i].fused)}))
+        results.append(AggResult(funcs, keys[:ng], cells[:ng], {"fused": int(fused[i])}))
     if not ok:
         raise _err("aggregate_batch")
     return (results, tr.as_dict()) if with_trace else results

@@ -63,9 +63,17 @@ class Expr:
         return (self >= lo) & (self <= hi)
 
     def nodes(self) -> np.ndarray:
-        out: list[tuple] = []
-        self._emit(out)
-        return np.array(out, dtype=IR_DTYPE)
+        """The flattened tree, built on first use and kept: an expression is
+        not changed after it is built (the operators return new nodes), so
+        a query run many times converts its WHERE once.  Read-only."""
+        cached = self.__dict__.get("_nodes")
+        if cached is None:
+            out: list[tuple] = []
+            self._emit(out)
+            cached = np.array(out, dtype=IR_DTYPE)
+            cached.flags.writeable = False
+            self._nodes = cached
+        return cached
 
 
 class Column(Expr):
