@@ -198,7 +198,7 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
       mn = qe_key_to_f64(type, res[2 + 3 * a]);
       mx = qe_key_to_f64(type, res[3 + 3 * a]);
       if (qe_type_int32(type)) {
-        out_cells[e] = qe_agg_cell_int(p->funcs[e], count, (S64)res[1 + 3 * a], mn, mx);
+        out_cells[e] = qe_agg_cell_int(p->funcs[e], count, (S64)res[1 + 3 * a], mn, mx, qe_t_raw_int_sums);
         continue;
       }
     }
@@ -561,6 +561,7 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
     return 0;
   }
   U64 take = groups < cap ? groups : cap;
+  const B32 raw = qe_t_raw_int_sums;
   for (U64 g = 0; g < take; g++) {
     const U64* r = recs + g * rec_words;
     if (out_keys) out_keys[g] = key_col >= 0 ? qe_hash_key_to_f64(key_type, r[0]) : 0.0;
@@ -576,7 +577,7 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
         mn = qe_f64_from_order_code(r[2 + E + a]);
         mx = qe_f64_from_order_code(r[2 + 2 * E + a]);
         if (qe_type_int32(ty)) {
-          out_cells[g * p->expr_count + e] = qe_agg_cell_int(p->funcs[e], cnt, (S64)sv, mn, mx);
+          out_cells[g * p->expr_count + e] = qe_agg_cell_int(p->funcs[e], cnt, (S64)sv, mn, mx, raw);
           continue;
         }
       }
@@ -749,6 +750,7 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
   if (!ok) return 0;
   /* groups in key order (query_exec.c:8006-8039) */
   U64 g = 0;
+  const B32 raw = qe_t_raw_int_sums;
   for (U32 i = 0; i < range; i++) {
     U64 cnt = tot.count[i];
     if (!cnt) continue;
@@ -774,7 +776,7 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
             mn = ty == CQL_ColumnType_I32 ? (F64)(S32)(lo ^ 0x80000000u) : (F64)lo;
             mx = ty == CQL_ColumnType_I32 ? (F64)(S32)(hi ^ 0x80000000u) : (F64)hi;
           }
-          out_cells[g * expr_count + e] = qe_agg_cell_int(plan.funcs[e], cnt, isum, mn, mx);
+          out_cells[g * expr_count + e] = qe_agg_cell_int(plan.funcs[e], cnt, isum, mn, mx, raw);
           continue;
         }
         out_cells[g * expr_count + e] = qe_agg_cell(plan.funcs[e], cnt, sum, mn, mx);

@@ -194,6 +194,7 @@ void qe_table_release(QE_Table* table)
     }
     free(col->zone_min);
     free(col->zone_max);
+    free(col->super_zone);
   }
   free(table);
 }
@@ -284,6 +285,9 @@ B32 qe_table_build_zonemap(QE_Table* table, U32 column)
   free(col->zone_min);
   free(col->zone_max);
   col->zone_min = col->zone_max = 0;
+  free(col->super_zone);
+  col->super_zone = 0;
+  col->super_count = 0;
   col->zone_count = zones;
   col->zonemap_valid = 0;
   col->min_value = 0.0;
@@ -329,6 +333,7 @@ B32 qe_table_build_zonemap(QE_Table* table, U32 column)
   col->min_value = mn;
   col->max_value = mx;
   col->zonemap_valid = 1;
+  qe_zonemap_build_super(col);
   return 1;
 }
 
@@ -584,27 +589,6 @@ B32 qe_gen_fact_fk(QE_Table* table, U32 fk_col, U64 seed, QE_Table* dim, U32 dim
 /* query_exec.c:7269-7293)                                              */
 /* ------------------------------------------------------------------ */
 __thread B32 qe_t_raw_int_sums;
-
-U64 qe_agg_cell_int(U32 func, U64 count, S64 isum, F64 mn, F64 mx)
-{
-  if (qe_t_raw_int_sums && func == QE_AGG_FUNC_SUM) return (U64)isum;
-  return qe_agg_cell(func, count, (F64)isum, mn, mx);
-}
-
-U64 qe_agg_cell(U32 func, U64 count, F64 sum, F64 mn, F64 mx)
-{
-  F64 v;
-  switch (func) {
-    case QE_AGG_FUNC_COUNT: return count;
-    case QE_AGG_FUNC_SUM: v = sum; break;
-    case QE_AGG_FUNC_AVG: v = count ? sum / (F64)count : 0.0; break;
-    case QE_AGG_FUNC_MIN: v = mn; break;
-    default: v = mx; break;
-  }
-  U64 bits;
-  memcpy(&bits, &v, 8);
-  return bits;
-}
 
 /* ------------------------------------------------------------------ */
 /* bandwidth probe                                                      */

@@ -32,6 +32,11 @@ typedef struct QE_Column {
   B32 zonemap_valid;
   F64 min_value;
   F64 max_value;
+  /* groups of 64 zones: the min and max of the zones' minima and of their
+   * maxima (4 values a group), which decide most groups without looking
+   * at their zones (qe_zonemap_prune); 0 when absent */
+  F64* super_zone;
+  U64 super_count;
   /* narrow encoding (qe_table_encode_column): enc 0 none, 1 u8 codes, 2 u16
    * codes (in enc_buffer), 3 sequence (value = enc_base + row); dropped
    * whenever the column's data is rewritten */
@@ -160,6 +165,9 @@ typedef struct QE_Prune {
   U64 skip_words;
 } QE_Prune;
 void qe_zonemap_prune(QE_Table* table, const QE_Leaf* leaves, U32 leaf_count, QE_Prune* out);
+#define QE_SUPER_SHIFT 6 /* zones per zone group: 64 */
+/* (re)build a column's zone groups from its zone maps */
+void qe_zonemap_build_super(QE_Column* col);
 B32 qe_scan_filter_device(QE_Table* table, const QE_IRNode* where, U32 node_count, GPU_Buffer** out_ids, U64* out_count);
 void qe_prune_free(QE_Prune* p);
 
@@ -180,11 +188,33 @@ typedef struct QE_GroupTotals {
   F64 mx[QE_AGG_MAX_EXPRS];
 } QE_GroupTotals;
 
-U64 qe_agg_cell(U32 func, U64 count, F64 sum, F64 mn, F64 mx);
+/* result typing: COUNT -> U64, SUM/AVG/MIN/MAX -> F64 bits
+ * (qe_agg_func_policy, query_exec.h:917-926; qe_aggregate_finish,
+ * query_exec.c:7269-7293) */
+static inline U64 qe_agg_cell(U32 func, U64 count, F64 sum, F64 mn, F64 mx)
+{
+  F64 v;
+  switch (func) {
+    case QE_AGG_FUNC_COUNT: return count;
+    case QE_AGG_FUNC_SUM: v = sum; break;
+    case QE_AGG_FUNC_AVG: v = count ? sum / (F64)count : 0.0; break;
+    case QE_AGG_FUNC_MIN: v = mn; break;
+    default: v = mx; break;
+  }
+  U64 bits;
+  memcpy(&bits, &v, 8);
+  return bits;
+}
+
 /* the cell of an aggregate over a 32-bit integer column, its sum exact;
- * while qe_t_raw_int_sums is set on this thread (streamed aggregation
- * merging chunk results) a SUM cell holds the S64 sum, not its F64 */
-U64 qe_agg_cell_int(U32 func, U64 count, S64 isum, F64 mn, F64 mx);
+ * with `raw` (qe_t_raw_int_sums of the calling thread, read once per
+ * operator: set while streamed aggregation merges chunk results) a SUM
+ * cell holds the S64 sum, not its F64 */
+static inline U64 qe_agg_cell_int(U32 func, U64 count, S64 isum, F64 mn, F64 mx, B32 raw)
+{
+  if (raw && func == QE_AGG_FUNC_SUM) return (U64)isum;
+  return qe_agg_cell(func, count, (F64)isum, mn, mx);
+}
 extern __thread B32 qe_t_raw_int_sums;
 
 /* an aggregate's expressions resolved to distinct argument columns */

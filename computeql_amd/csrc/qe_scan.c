@@ -346,6 +346,59 @@ static B32 qe_zone_prunable(const QE_Leaf* leaf, F64 zmin, F64 zmax)
   }
 }
 
+void qe_zonemap_build_super(QE_Column* col)
+{
+  free(col->super_zone);
+  col->super_zone = 0;
+  col->super_count = 0;
+  if (!col->zone_count || !col->zone_min || !col->zone_max) return;
+  U64 n = (col->zone_count + (1ull << QE_SUPER_SHIFT) - 1) >> QE_SUPER_SHIFT;
+  F64* sz = (F64*)malloc(sizeof(F64) * 4 * n);
+  if (!sz) return;
+  for (U64 g = 0; g < n; g++) {
+    U64 z0 = g << QE_SUPER_SHIFT, z1 = z0 + (1ull << QE_SUPER_SHIFT);
+    if (z1 > col->zone_count) z1 = col->zone_count;
+    F64 mnmn = col->zone_min[z0], mxmn = col->zone_min[z0], mnmx = col->zone_max[z0], mxmx = col->zone_max[z0];
+    for (U64 z = z0 + 1; z < z1; z++) {
+      F64 a = col->zone_min[z], b = col->zone_max[z];
+      if (a < mnmn) mnmn = a;
+      if (a > mxmn) mxmn = a;
+      if (b < mnmx) mnmx = b;
+      if (b > mxmx) mxmx = b;
+    }
+    sz[4 * g] = mnmn;
+    sz[4 * g + 1] = mxmn;
+    sz[4 * g + 2] = mnmx;
+    sz[4 * g + 3] = mxmx;
+  }
+  col->super_zone = sz;
+  col->super_count = n;
+}
+
+/* a group of zones against one leaf: 1 every zone prunable, 0 none
+ * prunable, -1 undecided (each outcome implies the zone-by-zone one) */
+static int qe_super_state(const QE_Leaf* leaf, const F64* g)
+{
+  const F64 t = leaf->target, mnmn = g[0], mxmn = g[1], mnmx = g[2], mxmx = g[3];
+  switch (leaf->op) {
+    case 0:
+      if (t < mnmn || t > mxmx) return 1;
+      return (mxmn <= t && t <= mnmx) ? 0 : -1;
+    case 1:
+      if (mnmn >= t) return 1;
+      return mxmn < t ? 0 : -1;
+    case 2:
+      if (mnmn > t) return 1;
+      return mxmn <= t ? 0 : -1;
+    case 3:
+      if (mxmx <= t) return 1;
+      return mnmx > t ? 0 : -1;
+    default:
+      if (mxmx < t) return 1;
+      return mnmx >= t ? 0 : -1;
+  }
+}
+
 void qe_zonemap_prune(QE_Table* table, const QE_Leaf* leaves, U32 leaf_count, QE_Prune* out)
 {
   memset(out, 0, sizeof(*out));
@@ -361,17 +414,47 @@ void qe_zonemap_prune(QE_Table* table, const QE_Leaf* leaves, U32 leaf_count, QE
   U32* bits = (U32*)calloc(words, sizeof(U32));
   if (!bits) return;
   U64 left = 0, first = zones, last = 0;
-  for (U64 z = 0; z < zones; z++) {
-    B32 skip = 0;
-    for (U32 l = 0; l < leaf_count && !skip; l++) {
-      QE_Column* c = &table->columns[leaves[l].column];
-      skip = qe_zone_prunable(&leaves[l], c->zone_min[z], c->zone_max[z]);
+  /* QE_ZONEMAP_SUPER (1): zones in groups of 64, a group decided by its
+   * group values when they settle every zone of it */
+  B32 super = qe_settings_u64("QE_ZONEMAP_SUPER", 1) != 0;
+  for (U32 l = 0; l < leaf_count && super; l++) {
+    const QE_Column* c = &table->columns[leaves[l].column];
+    super = c->super_zone && c->super_count == ((zones + (1ull << QE_SUPER_SHIFT) - 1) >> QE_SUPER_SHIFT);
+  }
+  const U64 gz = 1ull << QE_SUPER_SHIFT;
+  for (U64 z0 = 0; z0 < zones; z0 += gz) {
+    U64 z1 = z0 + gz < zones ? z0 + gz : zones;
+    int state = super ? 0 : -1;
+    for (U32 l = 0; l < leaf_count && super; l++) {
+      int s = qe_super_state(&leaves[l], table->columns[leaves[l].column].super_zone + 4 * (z0 >> QE_SUPER_SHIFT));
+      if (s == 1) {
+        state = 1;
+        break;
+      }
+      if (s < 0) state = -1;
     }
-    if (skip) bits[z >> 5] |= 1u << (z & 31);
-    else {
-      left++;
-      if (z < first) first = z;
-      last = z;
+    if (state == 1) {
+      for (U64 z = z0; z < z1; z++) bits[z >> 5] |= 1u << (z & 31);
+      continue;
+    }
+    if (state == 0) {
+      left += z1 - z0;
+      if (z0 < first) first = z0;
+      last = z1 - 1;
+      continue;
+    }
+    for (U64 z = z0; z < z1; z++) {
+      B32 skip = 0;
+      for (U32 l = 0; l < leaf_count && !skip; l++) {
+        QE_Column* c = &table->columns[leaves[l].column];
+        skip = qe_zone_prunable(&leaves[l], c->zone_min[z], c->zone_max[z]);
+      }
+      if (skip) bits[z >> 5] |= 1u << (z & 31);
+      else {
+        left++;
+        if (z < first) first = z;
+        last = z;
+      }
     }
   }
   out->zones_left = left;

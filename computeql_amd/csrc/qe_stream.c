@@ -311,6 +311,7 @@ static B32 qe_host_zonemap(QE_Table* t, QE_Column* col)
   }
   col->min_value = mn;
   col->max_value = mx;
+  qe_zonemap_build_super(col);
   return 1;
 }
 
@@ -357,6 +358,7 @@ S32 qe_table_add_host_column(QE_Table* table, const char* name, U32 type, const 
     table->column_count--;
     free(col->zone_min);
     free(col->zone_max);
+    free(col->super_zone);
     gpu_hip_host_free(host);
     memset(col, 0, sizeof(*col));
     if (table->column_count == 0) table->host_backed = 0;
@@ -478,6 +480,12 @@ static void qe_view_of(QE_Pipeline* p, QE_ChunkSlot* s)
     d->zone_count = nz;
     d->zone_min = t->columns[c].zone_min ? t->columns[c].zone_min + z0 : 0;
     d->zone_max = t->columns[c].zone_max ? t->columns[c].zone_max + z0 : 0;
+    /* the column's zone groups serve the chunk when it starts on a group
+     * and ends on one or at the column's end */
+    const U64 gz = 1ull << QE_SUPER_SHIFT;
+    const B32 aligned = (z0 % gz) == 0 && ((nz % gz) == 0 || s->c0 + s->n == t->row_count);
+    d->super_zone = aligned && t->columns[c].super_zone ? t->columns[c].super_zone + 4 * (z0 / gz) : 0;
+    d->super_count = d->super_zone ? (nz + gz - 1) / gz : 0;
   }
 }
 
@@ -916,7 +924,7 @@ B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count
         if (f == QE_AGG_FUNC_COUNT || grp->count == 0) {
           *cell = qe_agg_cell(f, grp->count, 0.0, 0.0, 0.0);
         } else if (x.part_int[i]) {
-          *cell = qe_agg_cell_int(f, grp->count, (S64)grp->part[i], 0.0, 0.0);
+          *cell = qe_agg_cell_int(f, grp->count, (S64)grp->part[i], 0.0, 0.0, 0);
         } else {
           F64 v;
           memcpy(&v, &grp->part[i], 8);

@@ -117,6 +117,38 @@ def test_point_lookup_prunes_zones(events, orc):
     assert n3 == 0 and cq.SCAN_PATHS[tr3["path"]] == "zone_empty"
 
 
+def test_zone_groups_prune_like_zones(orc):
+    """Zones are pruned 64 at a time where a group's min/max of zone minima
+    and maxima settle every zone of it (QE_ZONEMAP_SUPER); the rows found
+    and the dispatch range equal zone-by-zone pruning, for every operator,
+    on sorted, stepped, random and signed columns whose row count is not a
+    multiple of a group."""
+    n = 10_000_000 + 12_345
+    r = np.random.default_rng(5)
+    cols = [np.arange(n, dtype=np.uint32), (np.arange(n) // 300_000).astype(np.uint32),
+            r.integers(0, 1000, n).astype(np.uint32), ((np.arange(n) // 50_000) % 7 - 3).astype(np.int32)]
+    t = cq.Table(n)
+    for i, c in enumerate(cols):
+        t.add_column(f"c{i}", cq.I32 if c.dtype == np.int32 else cq.U32, c)
+    wheres = [col(0) == 5_000_000, col(0) < 600_000, col(0) <= 524_287, col(0) > 9_999_000, col(0) >= 9_437_184,
+              col(1) == 17, (col(1) >= 10) & (col(2) < 5), col(3) == -3, (col(0) > 2_000_000) & (col(1) < 9),
+              col(1) > 100, col(2) == 999, col(0) == n - 1]
+    try:
+        for w in wheres:
+            cq.set_setting("QE_ZONEMAP_SUPER", 1)
+            na, ia, ta = t.scan(w)
+            ga = t.aggregate([cq.COUNT, cq.SUM], [None, 2], where=w)
+            cq.set_setting("QE_ZONEMAP_SUPER", 0)
+            nb, ib, tb = t.scan(w)
+            gb = t.aggregate([cq.COUNT, cq.SUM], [None, 2], where=w)
+            assert na == nb and np.array_equal(ia, ib) and ta["rows_scanned"] == tb["rows_scanned"], w
+            assert np.array_equal(ga.cells, gb.cells), w
+            assert np.array_equal(ia, orc.scan(cols, w)), w
+    finally:
+        cq.reset_settings()
+        t.close()
+
+
 def test_scan_output_capacity(events, orc):
     t, host = events
     expect = orc.scan(host, col(2) < 200)
