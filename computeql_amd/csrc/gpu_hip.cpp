@@ -38,6 +38,7 @@ struct GPU_Buffer {
   U32 flags;
   bool pooled;
   bool host_import;
+  bool wrapped; /* pinned host memory seen by the device; not ours to free */
   std::string pool_name;
 };
 
@@ -536,6 +537,10 @@ void gpu_hip_buffer_release(GPU_Buffer* buffer)
       s.mem.bytes_in_use -= old->alloc_size;
       delete old;
     }
+    return;
+  }
+  if (buffer->wrapped) {
+    delete buffer;
     return;
   }
   hipFree(buffer->ptr);
@@ -1065,6 +1070,23 @@ void gpu_hip_host_free(void* p)
 }
 
 void gpu_hip_set_evict_hook(U64 (*hook)(U64 bytes_wanted)) { st().evict_hook = hook; }
+
+GPU_Buffer* gpu_hip_buffer_wrap_host(void* host_ptr, U64 size)
+{
+  if (!st().initialised) gpu_hip_init();
+  if (!host_ptr) return nullptr;
+  void* dptr = nullptr;
+  if (!check(hipHostGetDevicePointer(&dptr, host_ptr, 0), "hipHostGetDevicePointer") || !dptr) return nullptr;
+  GPU_Buffer* b = new GPU_Buffer();
+  b->ptr = dptr;
+  b->size = size;
+  b->alloc_size = size;
+  b->flags = GPU_BufferFlag_Read;
+  b->pooled = false;
+  b->host_import = true;
+  b->wrapped = true;
+  return b;
+}
 
 GPU_Batch* gpu_hip_batch_begin_graph(const char* key)
 {

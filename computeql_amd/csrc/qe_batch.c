@@ -422,7 +422,9 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
     queries[i].fused = 0;
     QE_BatchStage one;
     memset(&one, 0, sizeof(one));
-    if (qe_batch_plan_query(table, &queries[i], &planned[np]) &&
+    /* a host-backed table's queries each stream through the resident chunk
+     * cache (qe_aggregate) rather than read its rows over the host link */
+    if (!table->host_backed && qe_batch_plan_query(table, &queries[i], &planned[np]) &&
         (qe_batch_stage_query(&one, &planned[np]),
          qe_batch_lds(planned[np].lds_words, qe_batch_stage_bytes(table, &one, 1)) <= max_lds)) {
       planned[np].index = i;

@@ -207,7 +207,7 @@ S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void*
     qe_set_error("qe_table_add_column: table full");
     return -1;
   }
-  if (!qe_require_device(table, "qe_table_add_column")) return -1;
+  if (!qe_refuse_host(table, "qe_table_add_column")) return -1;
   U32 width = qe_type_width(type);
   if (!width) {
     qe_set_error("qe_table_add_column: unsupported type %u", type);
@@ -274,7 +274,7 @@ B32 qe_table_column_read(QE_Table* table, U32 column, U64 row0, U64 rows, void* 
 
 B32 qe_table_build_zonemap(QE_Table* table, U32 column)
 {
-  if (!table || column >= table->column_count || !qe_require_device(table, "qe_table_build_zonemap")) return 0;
+  if (!table || column >= table->column_count || !qe_refuse_host(table, "qe_table_build_zonemap")) return 0;
   QE_Column* col = &table->columns[column];
   /* the data changed: an encoded copy is stale */
   if (col->enc_buffer) gpu_buffer_release(col->enc_buffer);
@@ -392,7 +392,7 @@ S32 qe_table_encode_column(QE_Table* table, U32 column, B32 allow_sequence)
     qe_set_error("qe_table_encode_column: bad column");
     return -1;
   }
-  if (!qe_require_device(table, "qe_table_encode_column")) return -1;
+  if (!qe_refuse_host(table, "qe_table_encode_column")) return -1;
   QE_Column* col = &table->columns[column];
   const U64 rows = table->row_count;
   if (col->enc) return (S32)col->enc;
@@ -469,7 +469,7 @@ static B32 qe_run_gen(QE_Table* table, GPU_Kernel* k, U64 rows)
 
 B32 qe_gen_events(QE_Table* table, U64 seed, S32 id_col, S32 region_col, S32 amount_col, S32 ts_col)
 {
-  if (!table || !qe_require_device(table, "qe_gen")) return 0;
+  if (!table || !qe_refuse_host(table, "qe_gen")) return 0;
   GPU_Kernel* k = qe_kernel("gen_events");
   if (!k) return 0;
   S32 cols[4] = {id_col, region_col, amount_col, ts_col};
@@ -494,7 +494,7 @@ B32 qe_gen_events(QE_Table* table, U64 seed, S32 id_col, S32 region_col, S32 amo
 
 B32 qe_gen_scale_events(QE_Table* table, S32 id_col, S32 region_col, S32 amount_col, S32 key_col)
 {
-  if (!table || !qe_require_device(table, "qe_gen")) return 0;
+  if (!table || !qe_refuse_host(table, "qe_gen")) return 0;
   GPU_Kernel* k = qe_kernel("gen_scale_events");
   if (!k) return 0;
   S32 cols[4] = {id_col, region_col, amount_col, key_col};
@@ -537,7 +537,7 @@ B32 qe_gen_dim(QE_Table* table, U64 seed, U32 attr_range, B32 spread_keys, S32 k
     qe_set_error("qe_gen_dim: bad arguments");
     return 0;
   }
-  if (!qe_require_device(table, "qe_gen_dim")) return 0;
+  if (!qe_refuse_host(table, "qe_gen_dim")) return 0;
   U64 n = table->row_count;
   U32* keys = (U32*)malloc(sizeof(U32) * (n ? n : 1));
   if (!keys) return 0;
@@ -570,7 +570,7 @@ B32 qe_gen_fact_fk(QE_Table* table, U32 fk_col, U64 seed, QE_Table* dim, U32 dim
     qe_set_error("qe_gen_fact_fk: bad arguments");
     return 0;
   }
-  if (!qe_require_device(table, "qe_gen_fact_fk") || !qe_require_device(dim, "qe_gen_fact_fk")) return 0;
+  if (!qe_refuse_host(table, "qe_gen_fact_fk") || !qe_require_device(dim, "qe_gen_fact_fk")) return 0;
   GPU_Kernel* k = qe_kernel("gen_fact_fk");
   if (!k) return 0;
   gpu_kernel_set_arg_buffer(k, 6, table->columns[fk_col].buffer);

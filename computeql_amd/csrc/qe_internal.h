@@ -59,8 +59,11 @@ struct QE_Table {
 
 U32 qe_type_width(U32 type); /* bytes per row of a column type, 0 if unsupported */
 
-/* operators that need every column whole in HBM refuse host-backed tables */
+/* operators that read column buffers: a host-backed table serves them
+ * through its zero-copy buffers (0 when the device cannot map them) */
 B32 qe_require_device(const QE_Table* t, const char* op);
+/* operators that write columns or add device state refuse host-backed tables */
+B32 qe_refuse_host(const QE_Table* t, const char* op);
 
 /* ---- streamed execution over host-backed tables (qe_stream.c) ---- */
 B32 qe_stream_scan(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only, U64* out_ids, U64 cap,

@@ -253,7 +253,20 @@ void qe_resident_stats(QE_ResidentStats* out)
 B32 qe_require_device(const QE_Table* t, const char* op)
 {
   if (t && t->host_backed) {
-    qe_set_error("%s: the table is host-backed; this operator needs its columns in HBM", op);
+    for (U32 c = 0; c < t->column_count; c++) {
+      if (!t->columns[c].buffer) {
+        qe_set_error("%s: the host-backed table's rows cannot be mapped for the device", op);
+        return 0;
+      }
+    }
+  }
+  return 1;
+}
+
+B32 qe_refuse_host(const QE_Table* t, const char* op)
+{
+  if (t && t->host_backed) {
+    qe_set_error("%s: the table is host-backed (its rows are read-only host memory)", op);
     return 0;
   }
   return 1;
@@ -349,6 +362,9 @@ S32 qe_table_add_host_column(QE_Table* table, const char* name, U32 type, const 
   col->type = type;
   col->width = width;
   col->host_data = host;
+  /* the rows in place for the operators that do not stream (0 if the
+   * device cannot map them: those operators then refuse the table) */
+  col->buffer = gpu_hip_buffer_wrap_host(host, bytes + 64);
   res_lock();
   col->uid = g_next_uid++;
   res_unlock();
@@ -359,6 +375,7 @@ S32 qe_table_add_host_column(QE_Table* table, const char* name, U32 type, const 
     free(col->zone_min);
     free(col->zone_max);
     free(col->super_zone);
+    if (col->buffer) gpu_buffer_release(col->buffer);
     gpu_hip_host_free(host);
     memset(col, 0, sizeof(*col));
     if (table->column_count == 0) table->host_backed = 0;

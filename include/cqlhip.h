@@ -227,6 +227,10 @@ void gpu_hip_host_free(void* p);
  * freed: returns the bytes it released, and the allocation is tried once
  * more (g_gpu_evict_hook, query_exec.c:1817-1822) */
 void gpu_hip_set_evict_hook(U64 (*hook)(U64 bytes_wanted));
+/* pinned host memory from gpu_hip_host_alloc as a buffer the kernels read
+ * in place over the host link (the reference's host-pointer import,
+ * gpu_vulkan.c:2213-2358, without the copy); releasing it frees nothing */
+GPU_Buffer* gpu_hip_buffer_wrap_host(void* host_ptr, U64 size);
 /* fault injection: fail the k-th device allocation from now (gpu.h:68) */
 void gpu_hip_fault_alloc_at(U64 k);
 const char* gpu_hip_last_error(void);
@@ -357,7 +361,9 @@ B32 qe_table_build_zonemap(QE_Table* table, U32 column);
  * qe_scan_filter and qe_aggregate over such a table stream it in chunks of
  * QE_STREAM_CHUNK_ROWS rows (default 16,777,216, query_exec.c:2096-2099)
  * through the resident chunk cache, copies of one chunk overlapping the
- * kernels of the one before; the other operators refuse it. */
+ * kernels of the one before.  The other read-only operators (gather,
+ * SELECT, joins, batches) read its rows in place over the host link;
+ * generators, encoding and device columns are refused. */
 S32 qe_table_add_host_column(QE_Table* table, const char* name, U32 type, const void* data);
 
 /* the resident chunk cache (QE_ResidentStats, query_exec.c:1700-1754):

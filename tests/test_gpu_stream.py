@@ -279,16 +279,57 @@ def test_release_drops_cached_chunks():
     assert cq.resident_stats()["entries"] == before - 1
 
 
-def test_host_tables_refuse_device_only_operators(tables):
+def test_host_tables_refuse_writes(tables):
     dev, host, _ = tables
     with pytest.raises(cq.QueryError):
-        host.gather([0], np.arange(10, dtype=np.uint64))
-    with pytest.raises(cq.QueryError):
         host.encode(0)
-    with pytest.raises(cq.QueryError):
-        cq.hash_join(host, 0, dev, 0)
     with pytest.raises(cq.QueryError):
         host.add_column("x", cq.U32, np.zeros(N, dtype=np.uint32))
     with pytest.raises(cq.QueryError):
         dev.add_host_column("x", cq.U32, np.zeros(N, dtype=np.uint32))
+    with pytest.raises(cq.QueryError):
+        host.gen_events(1, id=0)
     assert np.array_equal(host.read(2, 10, 5), np.arange(10, 15, dtype=np.uint32))
+
+
+def test_host_gather_and_select_read_in_place(tables):
+    """Operators that do not stream read the pinned rows in place."""
+    dev, host, _ = tables
+    rows = np.random.default_rng(3).integers(0, N, 50_000).astype(np.uint64)
+    for a, b in zip(host.gather([0, 3, 5, 6], rows), dev.gather([0, 3, 5, 6], rows)):
+        assert np.array_equal(a, b)
+    w = (col(0) < 3) & (col(1) > 9_000)
+    for a, b in zip(host.select([2, 3, 6], where=w), dev.select([2, 3, 6], where=w)):
+        assert np.array_equal(a, b)
+
+
+def test_host_joins(tables):
+    dev, host, _ = tables
+    small = cq.Table(5000)
+    small.add_column("k", cq.U32, (np.arange(5000, dtype=np.uint32) * 7) % 10_000)
+    small.add_column("g", cq.U32, np.arange(5000, dtype=np.uint32) % 13)
+    try:
+        for jt in ("inner", "semi", "anti", "left"):
+            a = cq.hash_join(host, 1, small, 0, jt)
+            b = cq.hash_join(dev, 1, small, 0, jt)
+            assert np.array_equal(a, b), jt
+        a = cq.hash_join(small, 0, host, 1, "inner", build_rows=np.arange(0, N, 97, dtype=np.uint64))
+        b = cq.hash_join(small, 0, dev, 1, "inner", build_rows=np.arange(0, N, 97, dtype=np.uint64))
+        assert np.array_equal(a, b)
+        ra = cq.aggregate_join(host, 1, small, 0, [cq.COUNT, cq.SUM], [None, 3], dim_group=1)
+        rb = cq.aggregate_join(dev, 1, small, 0, [cq.COUNT, cq.SUM], [None, 3], dim_group=1)
+        oa, ob = np.argsort(ra.keys), np.argsort(rb.keys)
+        assert np.array_equal(ra.keys[oa], rb.keys[ob]) and np.array_equal(ra.cells[oa], rb.cells[ob])
+    finally:
+        small.close()
+
+
+def test_host_batch_streams_each_query(tables):
+    dev, host, cols = tables
+    specs = [cq.AggSpec(*AGGS[0], where=WHERES[1], group=0), cq.AggSpec(*AGGS[1], where=WHERES[4]),
+             cq.AggSpec(*AGGS[0], group=3)]
+    ra = cq.aggregate_batch(host, specs)
+    rb = [dev.aggregate(sp.funcs, sp.args, where=sp.where, group=sp.group) for sp in specs]
+    for a, b, sp in zip(ra, rb, specs):
+        assert a.trace["fused"] == 0
+        _agg_equal(a, b, sp.funcs, sp.args, cols)
