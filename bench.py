@@ -31,6 +31,9 @@ The same JSON line also carries, measured after the timed steps:
   served   the request loop under Poisson arrivals at rising rates, plain
            (one query per launch) and batched: the highest rate whose p99
            latency stays within 50 ms, and the ratio of the two
+  roofline.ceilings  what this device sustains in the same run: a plain
+           4 GB read and a 4 GB copy (read + write bytes); the dominant
+           kernel's fraction of the matching one is frac_of_measured_ceiling
   cpu_baseline  the oracle (C restatement of the CPU operator path) on a
            bounded sample of the q1-q7 tables
 
@@ -57,6 +60,7 @@ QUERY_BYTES_PER_ROW = {"q1": 4, "q2": 4, "q3": 8, "q4": 4, "q5": 12, "q6": 8, "q
 # the kernels each query launches over the fact table, with the bytes per
 # fact row each one must move (reads + writes; the partitioned join's
 # scatter reads the key and amount and writes both back in partition order)
+WRITING_KERNELS = {"join_part_scatter*"}  # kernels whose algorithmic bytes include writes
 QUERY_KERNELS = {
     "q1": [("aggregate_global", 4)],
     "q2": [("scan_simple", 4)],
@@ -384,6 +388,30 @@ def streamed_run(cq, lib, t, rows: int, repeats: int = 3):
     return out
 
 
+def device_ceilings(cq, lib, t, reps: int = 3) -> dict:
+    """What this device sustains for the two traffic shapes of the suite,
+    measured in the same run with hipEvents: a plain read of one 4 GB column
+    (bandwidth_read) and a copy of it into a scratch buffer (bandwidth_copy,
+    counted as read + write bytes), each the best of three grid sizes."""
+    cus = cq.compute_units()
+    out = {}
+    for name, fn, factor in (("read", cq.bandwidth_read, 4.0), ("copy", cq.bandwidth_copy, 8.0)):
+        best = None
+        for blocks in (2 * cus, 4 * cus, 8 * cus):
+            fn(t, 2, blocks)
+            lib.gpu_hip_profile_reset()
+            for _ in range(reps):
+                fn(t, 2, blocks)
+            ns, d = profile_stats(lib, f"bandwidth_{name}")
+            if d:
+                gbs = factor * t.rows / (ns / d)
+                best = gbs if best is None or gbs > best else best
+        out[f"{name}_gbs"] = round(best, 1) if best else None
+    out["note"] = ("read: one 4 GB column read; copy: that column copied to a scratch buffer, read + write bytes; "
+                   "best of 2, 4 and 8 blocks per CU")
+    return out
+
+
 def profile_stats(lib, name: str):
     d = ctypes.c_uint64(0)
     ns = lib.gpu_hip_profile_query(name.encode(), ctypes.byref(d))
@@ -465,6 +493,12 @@ def main(argv=None) -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_GBS, 4), "traffic": measured_traffic(dom_name, rows),
                 "kernel": dom_name, "bytes_per_launch": bytes_per_launch, "avg_us": dom["avg_us"]}
+    ceilings = device_ceilings(cq, lib, t)
+    roofline["ceilings"] = ceilings
+    # the scatter's bytes are half reads, half writes: its ceiling is the copy's
+    ceil = ceilings["copy_gbs"] if dom_name.split(":", 1)[1] in WRITING_KERNELS else ceilings["read_gbs"]
+    roofline["frac_of_measured_ceiling"] = round(achieved / ceil, 4) if ceil else None
+    log(f"ceilings: read {ceilings['read_gbs']} GB/s, copy {ceilings['copy_gbs']} GB/s")
     if a.calibrate:
         # a plain read of one known 4-byte column: its counter reading scales
         # the FETCH_SIZE of the kernels in a --pmc run of this command

@@ -593,6 +593,34 @@ __thread B32 qe_t_raw_int_sums;
 /* ------------------------------------------------------------------ */
 /* bandwidth probe                                                      */
 /* ------------------------------------------------------------------ */
+B32 qe_bandwidth_copy(GPU_Buffer* src, GPU_Buffer* dst, U64 bytes, U32 blocks, QE_Trace* trace)
+{
+  U64 t0, p0;
+  qe_trace_begin(trace, &t0, &p0);
+  if (!src || !dst || bytes > gpu_hip_buffer_size(src) || bytes > gpu_hip_buffer_size(dst)) {
+    qe_set_error("bandwidth_copy: %llu bytes do not fit both buffers", (unsigned long long)bytes);
+    return 0;
+  }
+  GPU_Kernel* k = qe_kernel("bandwidth_copy");
+  if (!k) return 0;
+  U64 units = bytes / 16;
+  if (!blocks) blocks = qe_stream_blocks(units);
+  gpu_kernel_set_arg_buffer(k, 0, src);
+  gpu_kernel_set_arg_buffer(k, 1, dst);
+  gpu_kernel_set_arg_u64(k, 0, units);
+  GPU_Batch* b = gpu_batch_begin(0, 0);
+  gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+  B32 ok = gpu_batch_end(b);
+  if (trace) {
+    trace->kernel_count = 1;
+    trace->bytes_read = units * 16;
+    snprintf(trace->kernel, sizeof(trace->kernel), "bandwidth_copy");
+  }
+  qe_trace_end(trace, t0, p0);
+  if (!ok) qe_set_error("bandwidth_copy: %s", gpu_hip_last_error());
+  return ok;
+}
+
 B32 qe_bandwidth_read(GPU_Buffer* buffer, U64 bytes, U32 blocks, U64* out_sink, QE_Trace* trace)
 {
   U64 t0, p0;

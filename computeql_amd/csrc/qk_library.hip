@@ -49,30 +49,59 @@ extern "C" __global__ void LB k_fill_u32(QK_Args a)
 /* bandwidth_read: binding 0 = data, binding 1 = sink (u64);            */
 /* push 0 = 16-byte units to read                                       */
 /* ------------------------------------------------------------------ */
-extern "C" __global__ void LB k_bandwidth_read(QK_Args a)
+/* bandwidth probes: contiguous block ranges (as the scans walk them) in
+ * steps of WG x QK_BW_UNROLL 16-byte units, every load of a step in flight
+ * together.  bandwidth_copy: binding 0 source, 1 destination, push 0 units;
+ * every unit read and written back once. */
+#define QK_BW_UNROLL 8
+
+__device__ __forceinline__ void qk_bw_range(qk_u64 n, qk_u64& b0, qk_u64& b1)
+{
+  const qk_u64 step = (qk_u64)QK_WG * QK_BW_UNROLL;
+  qk_u64 per = (n + gridDim.x - 1) / gridDim.x;
+  per = (per + step - 1) / step * step;
+  b0 = per * blockIdx.x;
+  if (b0 > n) b0 = n;
+  b1 = b0 + per < n ? b0 + per : n;
+}
+
+extern "C" __global__ void LB k_bandwidth_copy(QK_Args a)
 {
   const qk_u32x4* p = qk_ptr<qk_u32x4>(a, 0);
-  qk_u64 n = qk_push(a, 0);
+  qk_u32x4* o = qk_wptr<qk_u32x4>(a, 1);
   qk_u64 b0, b1;
-  /* same contiguous-range walk as the scans, one 16-byte unit per quad */
-  {
-    qk_u64 step = (qk_u64)QK_WG * QK_UNROLL;
-    qk_u64 per = (n + gridDim.x - 1) / gridDim.x;
-    per = (per + step - 1) / step * step;
-    b0 = per * blockIdx.x;
-    if (b0 > n) b0 = n;
-    b1 = b0 + per < n ? b0 + per : n;
-  }
-  qk_u32x4 acc = {0, 0, 0, 0};
-  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
-    qk_u32x4 x[QK_UNROLL];
+  qk_bw_range(qk_push(a, 0), b0, b1);
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_BW_UNROLL) {
+    qk_u32x4 x[QK_BW_UNROLL];
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
+    for (int j = 0; j < QK_BW_UNROLL; j++) {
       qk_u64 i = s + (qk_u64)j * QK_WG + threadIdx.x;
       x[j] = i < b1 ? p[i] : qk_u32x4{0, 0, 0, 0};
     }
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) acc ^= x[j];
+    for (int j = 0; j < QK_BW_UNROLL; j++) {
+      qk_u64 i = s + (qk_u64)j * QK_WG + threadIdx.x;
+      if (i < b1) o[i] = x[j];
+    }
+  }
+}
+
+/* bandwidth_read: binding 0 the buffer, 1 a sink word; push 0 units */
+extern "C" __global__ void LB k_bandwidth_read(QK_Args a)
+{
+  const qk_u32x4* p = qk_ptr<qk_u32x4>(a, 0);
+  qk_u64 b0, b1;
+  qk_bw_range(qk_push(a, 0), b0, b1);
+  qk_u32x4 acc = {0, 0, 0, 0};
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_BW_UNROLL) {
+    qk_u32x4 x[QK_BW_UNROLL];
+#pragma unroll
+    for (int j = 0; j < QK_BW_UNROLL; j++) {
+      qk_u64 i = s + (qk_u64)j * QK_WG + threadIdx.x;
+      x[j] = i < b1 ? p[i] : qk_u32x4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int j = 0; j < QK_BW_UNROLL; j++) acc ^= x[j];
   }
   qk_u32 r = acc.x ^ acc.y ^ acc.z ^ acc.w;
   if (r == 0x9e3779b9u) qk_wptr<qk_u64>(a, 1)[0] = r; /* keeps the loads live */
@@ -3008,6 +3037,7 @@ extern "C" __global__ void LB k_zonemap_build(QK_Args a)
 static const QK_LibraryKernel g_library[] = {
     {"fill_u32", (const void*)k_fill_u32, QK_WG},
     {"bandwidth_read", (const void*)k_bandwidth_read, QK_WG},
+    {"bandwidth_copy", (const void*)k_bandwidth_copy, QK_WG},
     {"aggregate_global", (const void*)k_aggregate_global, QK_WG},
     {"aggregate_global_fold", (const void*)k_aggregate_global_fold, QK_WG},
     {"scan_simple", (const void*)k_scan_simple, QK_WG},

@@ -474,6 +474,24 @@ def bandwidth_read(table: Table, column: int, blocks: int = 0) -> dict:
     return tr.as_dict()
 
 
+def bandwidth_copy(table: Table, column: int, blocks: int = 0) -> dict:
+    """qe_bandwidth_copy of one 4-byte column into a scratch buffer of its
+    size (the buffer is freed after the copy)."""
+    lib = load()
+    src = lib.qe_table_column_buffer(table.handle, column)
+    nbytes = table.rows * 4
+    dst = lib.gpu_hip_buffer_alloc(nbytes + 64, (1 << 2) | (1 << 4), None)
+    if not dst:
+        raise _err("bandwidth_copy")
+    try:
+        tr = QE_Trace()
+        if not lib.qe_bandwidth_copy(src, dst, nbytes, blocks, C.byref(tr)):
+            raise _err("bandwidth_copy")
+        return tr.as_dict()
+    finally:
+        lib.gpu_hip_buffer_release(dst)
+
+
 def resident_stats() -> dict:
     """Counters of the resident chunk cache (qe_resident_stats)."""
     st = QE_ResidentStats()

@@ -214,6 +214,7 @@ SIGNATURES = [
     ("qe_server_stop", None, [P]),
     ("qe_server_drive", B32, [P, P, P, U32, C.c_double, U64, U64, P]),
     ("qe_bandwidth_read", B32, [P, U64, U32, P, P]),
+    ("qe_bandwidth_copy", B32, [P, P, U64, U32, P]),
 ]
 
 _lib = None

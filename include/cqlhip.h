@@ -588,6 +588,9 @@ B32 qe_server_drive(QE_Server* server, QE_Table* table, const QE_BatchQuery* poo
 /* bandwidth probe (bandwidth_read.comp): reads `bytes` of a buffer with
  * 16-byte loads; returns a sink value.  For measuring the HBM ceiling. */
 B32 qe_bandwidth_read(GPU_Buffer* buffer, U64 bytes, U32 blocks, U64* out_sink, QE_Trace* trace);
+/* the copy counterpart: `bytes` of src written to dst (the read + write
+ * ceiling the partition scatter is measured against) */
+B32 qe_bandwidth_copy(GPU_Buffer* src, GPU_Buffer* dst, U64 bytes, U32 blocks, QE_Trace* trace);
 
 #ifdef __cplusplus
 }
