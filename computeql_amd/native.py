@@ -176,6 +176,7 @@ SIGNATURES = [
     ("gpu_hip_host_alloc", P, [U64]),
     ("gpu_hip_host_free", None, [P]),
     ("gpu_hip_set_evict_hook", None, [P]),
+    ("gpu_hip_buffer_wrap_host", P, [P, U64]),
     ("gpu_hip_fault_alloc_at", None, [U64]),
     ("gpu_hip_last_error", C.c_char_p, []),
     ("qe_settings_set_u64", None, [C.c_char_p, U64]),
