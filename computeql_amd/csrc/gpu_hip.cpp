@@ -1071,6 +1071,20 @@ void gpu_hip_host_free(void* p)
 
 void gpu_hip_set_evict_hook(U64 (*hook)(U64 bytes_wanted)) { st().evict_hook = hook; }
 
+GPU_Buffer* gpu_hip_buffer_view(GPU_Buffer* parent, U64 offset, U64 size)
+{
+  if (!parent || offset > parent->alloc_size || size > parent->alloc_size - offset) return nullptr;
+  GPU_Buffer* b = new GPU_Buffer();
+  b->ptr = (char*)parent->ptr + offset;
+  b->size = size;
+  b->alloc_size = size;
+  b->flags = parent->flags;
+  b->pooled = false;
+  b->host_import = parent->host_import;
+  b->wrapped = true;
+  return b;
+}
+
 GPU_Buffer* gpu_hip_buffer_wrap_host(void* host_ptr, U64 size)
 {
   if (!st().initialised) gpu_hip_init();

@@ -538,8 +538,15 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
       for (U32 i = 0; i < cblocks; i++) groups += bcounts[i];
       U64 take = groups < cap ? groups : cap;
       recs = (U64*)malloc((take ? take : 1) * 8 * rec_words);
-      if (!recs) ok = 0;
-      else if (take) gpu_buffer_read(out, recs, take * 8 * rec_words);
+      if (!recs) {
+        ok = 0;
+      } else if (take) {
+        /* on a batch, so a failed read fails the query instead of leaving
+         * zeros in the records */
+        GPU_Batch* rb = gpu_batch_begin(0, take * 8 * rec_words);
+        gpu_batch_buffer_read(rb, out, recs, take * 8 * rec_words);
+        ok = gpu_batch_end(rb);
+      }
     }
     free(bcounts);
     qe_scratch_release(table);

@@ -41,16 +41,21 @@
 
 #define QE_RESIDENT_MAX_ENTRIES 4096
 
+#define QE_WHOLE_COLUMN (~0ull)
+
 typedef struct QE_ResidentEntry {
   B32 used;
-  B32 ready; /* the copy into the buffer has completed */
+  B32 ready; /* a chunk's copy has completed; for a whole column, every chunk's */
   U64 uid;
-  U64 chunk;
+  U64 chunk; /* chunk index, or QE_WHOLE_COLUMN: one buffer for all the rows */
   GPU_Buffer* buffer;
   U64 bytes;
   U64 last_use;
   U32 uses;
   U32 pins;
+  /* whole-column entries: per chunk 0 empty, 1 being copied, 2 filled */
+  U64 nchunks, nfilled;
+  U8* state;
 } QE_ResidentEntry;
 
 static struct {
@@ -100,6 +105,7 @@ static U64 res_budget_locked(void)
 static void res_free_entry(QE_ResidentEntry* e)
 {
   gpu_buffer_release(e->buffer);
+  free(e->state);
   g_res.bytes_in_use -= e->bytes < g_res.bytes_in_use ? e->bytes : g_res.bytes_in_use;
   g_res.entry_count--;
   memset(e, 0, sizeof(*e));
@@ -145,53 +151,106 @@ static U64 res_evict_hook(U64 wanted)
   return freed;
 }
 
-/* the pinned buffer of a chunk, and whether its rows must still be copied
- * in; 0 when the cache cannot hold it (the caller then streams through a
- * buffer of its own).  Every entry returned goes back through res_unpin. */
-static QE_ResidentEntry* res_acquire(U64 uid, U64 chunk, U64 bytes, B32* needs_copy)
+/* room for `bytes` under the budget, evicting what is not pinned */
+static B32 res_room_locked(U64 bytes)
+{
+  U64 budget = res_budget_locked();
+  if (bytes > budget) return 0;
+  if (g_res.bytes_in_use + bytes <= budget) return 1;
+  U64 over = g_res.bytes_in_use + bytes - budget;
+  return res_evict_locked(over) >= over;
+}
+
+static QE_ResidentEntry* res_new_entry_locked(QE_ResidentEntry* slot, U64 uid, U64 chunk, U64 bytes)
+{
+  GPU_Buffer* buf = gpu_buffer_alloc(bytes, GPU_BufferFlag_ReadWrite | GPU_BufferFlag_DeviceLocal, 0);
+  if (!buf) return 0;
+  memset(slot, 0, sizeof(*slot));
+  slot->used = 1;
+  slot->uid = uid;
+  slot->chunk = chunk;
+  slot->buffer = buf;
+  slot->bytes = bytes;
+  slot->last_use = ++g_res.tick;
+  slot->uses = 1;
+  g_res.bytes_in_use += bytes;
+  g_res.entry_count++;
+  return slot;
+}
+
+/* The buffer chunk `chunk` of column `col` is read from, pinned in the
+ * cache, and whether its rows must still be copied in; 0 when the cache
+ * cannot hold it (the caller then streams through a buffer of its own).
+ * While the whole column fits the budget (QE_RESIDENT_WHOLE, default 1)
+ * its chunks share one buffer of the column's size, which a warm query
+ * reads in one pass; otherwise each chunk is an entry of its own.  A chunk
+ * of a whole-column entry comes back as a view of that buffer (*view, the
+ * caller's to release).  Every entry returned goes back through
+ * res_unpin. */
+static QE_ResidentEntry* res_acquire(const QE_Column* col, U64 rows, U64 chunk, U64 chunk_rows, B32* needs_copy,
+                                     GPU_Buffer** view)
 {
   *needs_copy = 0;
-  if (qe_settings_u64("QE_RESIDENT", 1) == 0 || bytes == 0) return 0;
+  *view = 0;
+  const U64 c0 = chunk * chunk_rows;
+  const U64 n = rows - c0 < chunk_rows ? rows - c0 : chunk_rows;
+  const U64 bytes = n * col->width + 64; /* slack for the kernels' 16-byte loads */
+  if (qe_settings_u64("QE_RESIDENT", 1) == 0 || n == 0) return 0;
   gpu_hip_set_evict_hook(res_evict_hook);
   res_lock();
-  QE_ResidentEntry* free_slot = 0;
+  QE_ResidentEntry *free_slot = 0, *whole = 0, *got = 0;
   for (U32 i = 0; i < QE_RESIDENT_MAX_ENTRIES; i++) {
     QE_ResidentEntry* e = &g_res.entries[i];
     if (!e->used) {
       if (!free_slot) free_slot = e;
       continue;
     }
-    if (e->uid == uid && e->chunk == chunk && e->ready) {
-      e->last_use = ++g_res.tick;
-      e->uses++;
-      e->pins++;
-      g_res.hit_bytes += bytes;
-      res_unlock();
-      return e;
+    if (e->uid != col->uid) continue;
+    if (e->chunk == QE_WHOLE_COLUMN) whole = e;
+    else if (e->chunk == chunk && e->ready && !got) got = e;
+  }
+  if (!whole && !got && free_slot && qe_settings_u64("QE_RESIDENT_WHOLE", 1) && res_room_locked(rows * col->width + 64)) {
+    const U64 nchunks = (rows + chunk_rows - 1) / chunk_rows;
+    U8* state = (U8*)calloc(nchunks, 1);
+    if (state && (whole = res_new_entry_locked(free_slot, col->uid, QE_WHOLE_COLUMN, rows * col->width + 64))) {
+      whole->nchunks = nchunks;
+      whole->state = state;
+      whole->uses = 0;
+      free_slot = 0;
+    } else {
+      free(state);
     }
   }
-  QE_ResidentEntry* got = 0;
-  U64 budget = res_budget_locked();
-  if (free_slot && bytes <= budget) {
-    B32 room = 1;
-    if (g_res.bytes_in_use + bytes > budget) {
-      U64 over = g_res.bytes_in_use + bytes - budget;
-      room = res_evict_locked(over) >= over;
+  if (whole && chunk < whole->nchunks && whole->state[chunk] != 1) {
+    GPU_Buffer* v = gpu_hip_buffer_view(whole->buffer, c0 * col->width, bytes);
+    if (v) {
+      whole->pins++;
+      whole->last_use = ++g_res.tick;
+      if (chunk == 0) whole->uses++;
+      if (whole->state[chunk] == 2) {
+        g_res.hit_bytes += bytes;
+      } else {
+        whole->state[chunk] = 1;
+        g_res.miss_bytes += bytes;
+        *needs_copy = 1;
+      }
+      *view = v;
+      res_unlock();
+      return whole;
     }
-    GPU_Buffer* buf = room ? gpu_buffer_alloc(bytes, GPU_BufferFlag_ReadWrite | GPU_BufferFlag_DeviceLocal, 0) : 0;
-    if (buf) {
-      got = free_slot;
-      memset(got, 0, sizeof(*got));
-      got->used = 1;
-      got->uid = uid;
-      got->chunk = chunk;
-      got->buffer = buf;
-      got->bytes = bytes;
-      got->last_use = ++g_res.tick;
-      got->uses = 1;
+  }
+  if (got) {
+    got->last_use = ++g_res.tick;
+    got->uses++;
+    got->pins++;
+    g_res.hit_bytes += bytes;
+    res_unlock();
+    return got;
+  }
+  if (!whole && free_slot && res_room_locked(bytes)) {
+    got = res_new_entry_locked(free_slot, col->uid, chunk, bytes);
+    if (got) {
       got->pins = 1;
-      g_res.bytes_in_use += bytes;
-      g_res.entry_count++;
       g_res.miss_bytes += bytes;
       *needs_copy = 1;
     }
@@ -200,16 +259,63 @@ static QE_ResidentEntry* res_acquire(U64 uid, U64 chunk, U64 bytes, B32* needs_c
   return got;
 }
 
-/* `filled`: the rows are in the buffer; an entry never filled is freed */
-static void res_unpin(QE_ResidentEntry* e, B32 filled)
+/* `filled`: the chunk's rows are in the buffer; a chunk entry never filled
+ * is freed, a whole-column entry's chunk goes back to empty */
+static void res_unpin(QE_ResidentEntry* e, U64 chunk, B32 filled)
 {
   if (!e) return;
   res_lock();
   if (e->used) {
     if (e->pins) e->pins--;
-    if (filled) e->ready = 1;
-    else if (!e->ready && e->pins == 0) res_free_entry(e);
+    if (e->chunk == QE_WHOLE_COLUMN) {
+      if (chunk < e->nchunks && e->state[chunk] == 1) {
+        e->state[chunk] = filled ? 2 : 0;
+        if (filled) e->nfilled++;
+        e->ready = e->nfilled == e->nchunks;
+      }
+    } else if (filled) {
+      e->ready = 1;
+    } else if (!e->ready && e->pins == 0) {
+      res_free_entry(e);
+    }
   }
+  res_unlock();
+}
+
+/* every column of `cols` whole and filled in the cache: pin them and put
+ * their buffers in bufs[column] */
+static B32 res_acquire_whole(const QE_Table* t, const U32* cols, U32 n, QE_ResidentEntry** held, GPU_Buffer** bufs)
+{
+  if (qe_settings_u64("QE_RESIDENT", 1) == 0 || n == 0) return 0;
+  res_lock();
+  for (U32 i = 0; i < n; i++) {
+    held[i] = 0;
+    for (U32 k = 0; k < QE_RESIDENT_MAX_ENTRIES && !held[i]; k++) {
+      QE_ResidentEntry* e = &g_res.entries[k];
+      if (e->used && e->uid == t->columns[cols[i]].uid && e->chunk == QE_WHOLE_COLUMN && e->ready) held[i] = e;
+    }
+    if (!held[i]) {
+      res_unlock();
+      return 0;
+    }
+  }
+  for (U32 i = 0; i < n; i++) {
+    QE_ResidentEntry* e = held[i];
+    e->pins++;
+    e->uses++;
+    e->last_use = ++g_res.tick;
+    g_res.hit_bytes += e->bytes;
+    bufs[cols[i]] = e->buffer;
+  }
+  res_unlock();
+  return 1;
+}
+
+static void res_release_whole(QE_ResidentEntry** held, U32 n)
+{
+  res_lock();
+  for (U32 i = 0; i < n; i++)
+    if (held[i] && held[i]->pins) held[i]->pins--;
   res_unlock();
 }
 
@@ -398,7 +504,7 @@ typedef struct QE_ChunkSlot {
   U64 chunk, c0, n;
   B32 copying; /* copies were issued on this slot */
   QE_ResidentEntry* held[QE_MAX_COLUMNS];
-  GPU_Buffer* own[QE_MAX_COLUMNS]; /* buffers outside the cache */
+  GPU_Buffer* own[QE_MAX_COLUMNS]; /* buffers outside the cache, or views of a whole-column entry */
   GPU_Buffer* buf[QE_MAX_COLUMNS];
 } QE_ChunkSlot;
 
@@ -418,7 +524,7 @@ static void qe_slot_release(QE_Pipeline* p, QE_ChunkSlot* s, B32 filled)
 {
   for (U32 i = 0; i < p->col_count; i++) {
     U32 c = p->cols[i];
-    if (s->held[c]) res_unpin(s->held[c], filled);
+    if (s->held[c]) res_unpin(s->held[c], s->chunk, filled);
     if (s->own[c]) gpu_buffer_release(s->own[c]);
     s->held[c] = 0;
     s->own[c] = 0;
@@ -443,8 +549,10 @@ static B32 qe_slot_prepare(QE_Pipeline* p, U32 si, U64 chunk)
     /* 64 bytes of slack keep the kernels' 16-byte loads inside the buffer */
     U64 bytes = rows_bytes + 64;
     B32 needs_copy = 0;
-    QE_ResidentEntry* e = res_acquire(col->uid, chunk, bytes, &needs_copy);
-    GPU_Buffer* b = e ? e->buffer : 0;
+    GPU_Buffer* v = 0;
+    QE_ResidentEntry* e = res_acquire(col, p->t->row_count, chunk, p->chunk_rows, &needs_copy, &v);
+    GPU_Buffer* b = v ? v : (e ? e->buffer : 0);
+    if (v) s->own[p->cols[i]] = v;
     if (!b) {
       needs_copy = 1;
       b = gpu_buffer_alloc(bytes, GPU_BufferFlag_ReadWrite | GPU_BufferFlag_DeviceLocal, 0);
@@ -533,6 +641,36 @@ static B32 qe_pipeline_run(QE_Pipeline* p, const QE_Leaf* leaves, U32 leaf_count
   QE_Table* t = p->t;
   p->chunk_rows = qe_stream_chunk_rows();
   U64 chunks = (t->row_count + p->chunk_rows - 1) / p->chunk_rows;
+  /* warm: every column read is whole in the cache; one pass over them as
+   * over a device-resident table */
+  {
+    QE_ResidentEntry* held[QE_MAX_COLUMNS];
+    GPU_Buffer* bufs[QE_MAX_COLUMNS];
+    memset(bufs, 0, sizeof(bufs));
+    if (res_acquire_whole(t, p->cols, p->col_count, held, bufs)) {
+      QE_Table* v = (QE_Table*)malloc(sizeof(QE_Table));
+      B32 ok = v != 0;
+      if (ok) {
+        *v = *t;
+        v->host_backed = 0;
+        for (U32 c = 0; c < t->column_count; c++) {
+          QE_Column* d = &v->columns[c];
+          d->buffer = bufs[c];
+          d->host_data = 0;
+          d->uid = 0;
+        }
+        ok = fn(ctx, v, 0);
+        free(v);
+      } else {
+        qe_set_error("streamed scan: out of host memory");
+      }
+      res_release_whole(held, p->col_count);
+      qe_t_stream_stats.chunks = chunks;
+      qe_t_stream_stats.chunks_resident = chunks;
+      qe_t_stream_stats.resident_whole = 1;
+      return ok;
+    }
+  }
   QE_Prune prune;
   qe_zonemap_prune(t, leaves, leaf_count, &prune);
   U64* run = (U64*)malloc(sizeof(U64) * (chunks ? chunks : 1));
@@ -560,7 +698,10 @@ static B32 qe_pipeline_run(QE_Pipeline* p, const QE_Leaf* leaves, U32 leaf_count
 
   B32 ok = 1;
   p->copy = nrun ? gpu_hip_stream_alloc(2) : 0;
-  if (nrun && !p->copy) ok = 0;
+  if (nrun && !p->copy) {
+    qe_set_error("streamed scan: no copy stream: %s", gpu_hip_last_error());
+    ok = 0;
+  }
   if (ok && nrun) ok = qe_slot_prepare(p, 0, run[0]);
   for (U64 i = 0; ok && i < nrun; i++) {
     U32 si = (U32)(i & 1);

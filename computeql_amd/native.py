@@ -97,7 +97,7 @@ class QE_ResidentStats(C.Structure):
 
 class QE_StreamStats(C.Structure):
     _fields_ = [(n, U64) for n in ("chunks", "chunks_skipped", "chunks_run", "chunks_resident", "transient_chunks",
-                                    "bytes_copied", "copy_wait_ns", "wall_ns")]
+                                    "bytes_copied", "copy_wait_ns", "wall_ns", "resident_whole")]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -177,6 +177,7 @@ SIGNATURES = [
     ("gpu_hip_host_free", None, [P]),
     ("gpu_hip_set_evict_hook", None, [P]),
     ("gpu_hip_buffer_wrap_host", P, [P, U64]),
+    ("gpu_hip_buffer_view", P, [P, U64, U64]),
     ("gpu_hip_fault_alloc_at", None, [U64]),
     ("gpu_hip_last_error", C.c_char_p, []),
     ("qe_settings_set_u64", None, [C.c_char_p, U64]),

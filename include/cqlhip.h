@@ -231,6 +231,9 @@ void gpu_hip_set_evict_hook(U64 (*hook)(U64 bytes_wanted));
  * in place over the host link (the reference's host-pointer import,
  * gpu_vulkan.c:2213-2358, without the copy); releasing it frees nothing */
 GPU_Buffer* gpu_hip_buffer_wrap_host(void* host_ptr, U64 size);
+/* bytes [offset, offset + size) of a buffer as a buffer of its own, valid
+ * while the parent lives; releasing it frees nothing */
+GPU_Buffer* gpu_hip_buffer_view(GPU_Buffer* parent, U64 offset, U64 size);
 /* fault injection: fail the k-th device allocation from now (gpu.h:68) */
 void gpu_hip_fault_alloc_at(U64 k);
 const char* gpu_hip_last_error(void);
@@ -391,6 +394,7 @@ typedef struct QE_StreamStats {
   U64 bytes_copied;     /* host to device */
   U64 copy_wait_ns;     /* time the compute side waited for copies */
   U64 wall_ns;
+  U64 resident_whole;   /* 1: every column whole in the cache, read in one pass */
 } QE_StreamStats;
 void qe_stream_last_stats(QE_StreamStats* out);
 

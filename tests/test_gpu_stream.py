@@ -90,7 +90,8 @@ def test_stream_scan_equals_resident(orc, tables, wi):
     assert np.array_equal(idh, orc.scan(cols, w))
     assert host.count(w) == nd
     st = cq.stream_stats()
-    assert st["chunks"] == CHUNKS and st["chunks_run"] + st["chunks_skipped"] == CHUNKS
+    assert st["chunks"] == CHUNKS
+    assert st["resident_whole"] or st["chunks_run"] + st["chunks_skipped"] == CHUNKS
 
 
 def test_scan_cap_keeps_the_first_ids(tables):
@@ -158,21 +159,29 @@ def test_stream_group_by(orc, tables, key, wi, ai):
             assert np.array_equal(a.column(e).astype(np.float64), rv[o, e]), e
 
 
-def test_cold_then_warm(tables):
+@pytest.mark.parametrize("whole", [1, 0])
+def test_cold_then_warm(tables, whole):
+    """Cold: every chunk copied in.  Warm: nothing copied; with whole
+    columns cached (QE_RESIDENT_WHOLE) the query runs once over them."""
     dev, host, _ = tables
+    cq.set_setting("QE_RESIDENT_WHOLE", whole)
     funcs, args = AGGS[0]
     cq.resident_clear()
     s0 = cq.resident_stats()
     a = host.aggregate(funcs, args, group=0)
     cold = cq.stream_stats()
     assert cold["chunks_run"] == CHUNKS and cold["bytes_copied"] == N * 4 * 3     # region, amount, signed
+    assert not cold["resident_whole"]
     s1 = cq.resident_stats()
-    assert s1["miss_bytes"] > s0["miss_bytes"] and s1["entries"] == 3 * CHUNKS
+    assert s1["miss_bytes"] > s0["miss_bytes"] and s1["entries"] == (3 if whole else 3 * CHUNKS)
     b = host.aggregate(funcs, args, group=0)
     warm = cq.stream_stats()
     assert warm["bytes_copied"] == 0 and warm["chunks_resident"] == CHUNKS
+    assert warm["resident_whole"] == whole and warm["chunks_run"] == (0 if whole else CHUNKS)
     assert cq.resident_stats()["hit_bytes"] > s1["hit_bytes"]
     assert np.array_equal(a.cells, b.cells) and np.array_equal(a.keys, b.keys)
+    c = dev.aggregate(funcs, args, group=0)
+    assert np.array_equal(np.sort(c.keys), b.keys)
 
 
 def test_zone_pruned_chunks_are_not_copied(tables):
@@ -212,16 +221,26 @@ def test_cache_off_streams_through_own_buffers(tables):
     assert st["transient_chunks"] == 2 * CHUNKS and st["chunks_resident"] == 0      # amount, signed
 
 
-def test_failed_cache_allocation_falls_back(tables):
+@pytest.mark.parametrize("whole", [0, 1])
+def test_failed_cache_allocation_falls_back(tables, whole):
+    """The first buffer the cache asks for fails: a per-chunk entry falls
+    back to a buffer of the query's own, a whole column to a per-chunk
+    entry (the next chunk asks for the whole column again); the rows found
+    are the same."""
     dev, host, cols = tables
+    cq.set_setting("QE_RESIDENT_WHOLE", whole)
     cq.resident_clear()
     lib = cq.load()
-    lib.gpu_hip_fault_alloc_at(1)         # the first chunk buffer the cache asks for
+    lib.gpu_hip_fault_alloc_at(1)
     try:
         nh, idh, _ = host.scan(WHERES[2])
     finally:
         lib.gpu_hip_fault_alloc_at(0)
-    assert cq.stream_stats()["transient_chunks"] >= 1
+    st = cq.stream_stats()
+    if whole:
+        assert st["transient_chunks"] == 0 and cq.resident_stats()["entries"] >= 1
+    else:
+        assert st["transient_chunks"] >= 1
     nd, idd, _ = dev.scan(WHERES[2])
     assert nh == nd and np.array_equal(idh, idd)
 
