@@ -205,6 +205,29 @@ void give_stream(hipStream_t h)
   s.free_streams.push_back(h);
 }
 
+/* synchronous copies and fills run on a pooled non-blocking stream, never
+ * on the legacy null stream: batch streams are not ordered against the null
+ * stream, and other host threads may be capturing graphs meanwhile */
+hipError_t copy_sync(void* dst, const void* src, U64 size, hipMemcpyKind kind)
+{
+  hipStream_t h = take_stream();
+  if (!h) return hipErrorOutOfMemory;
+  hipError_t e = hipMemcpyAsync(dst, src, size, kind, h);
+  hipError_t e2 = hipStreamSynchronize(h);
+  give_stream(h);
+  return e != hipSuccess ? e : e2;
+}
+
+hipError_t zero_sync(void* p, U64 size)
+{
+  hipStream_t h = take_stream();
+  if (!h) return hipErrorOutOfMemory;
+  hipError_t e = hipMemsetAsync(p, 0, size, h);
+  hipError_t e2 = hipStreamSynchronize(h);
+  give_stream(h);
+  return e != hipSuccess ? e : e2;
+}
+
 void flush_profile(std::vector<PendingProfile>& prof)
 {
   HipState& s = st();
@@ -363,20 +386,20 @@ GPU_Buffer* alloc_impl(U64 size, U32 flags, void* data)
     if (s.mem.bytes_in_use > s.mem.bytes_peak) s.mem.bytes_peak = s.mem.bytes_in_use;
   }
   if (data && size) {
-    if (!check(hipMemcpy(p, data, size, hipMemcpyHostToDevice), "hipMemcpy H2D")) {
+    if (!check(copy_sync(p, data, size, hipMemcpyHostToDevice), "copy H2D")) {
       gpu_hip_buffer_release(b);
       return nullptr;
     }
   }
   /* new buffers start zeroed (kernels keep their completion tickets in
-   * scratch buffers and reset them themselves).  The zeroing runs on the
-   * null stream, which batch streams (hipStreamNonBlocking) are not ordered
-   * against: it must be complete before the buffer is handed out, or a
-   * kernel's writes into the fresh buffer can be overwritten by it. */
+   * scratch buffers and reset them themselves).  The zeroing completes
+   * before the buffer is handed out: the batch streams are not ordered
+   * against the stream it runs on, and a kernel's writes into the fresh
+   * buffer must not be overwritten by it. */
   hipError_t ze = hipSuccess;
-  if (!data || !size) ze = hipMemsetAsync(p, 0, padded, nullptr);
-  else if (padded > size) ze = hipMemsetAsync((char*)p + size, 0, padded - size, nullptr);
-  if (!check(ze, "hipMemset") || !check(hipStreamSynchronize(nullptr), "alloc zero")) {
+  if (!data || !size) ze = zero_sync(p, padded);
+  else if (padded > size) ze = zero_sync((char*)p + size, padded - size);
+  if (!check(ze, "alloc zero")) {
     gpu_hip_buffer_release(b);
     return nullptr;
   }
@@ -487,10 +510,7 @@ GPU_Buffer* gpu_hip_buffer_alloc_pooled(String8 name, U64 size, GPU_BufferFlags 
   }
   if (hit) {
     hit->size = size;
-    if (data && size) {
-      hipMemcpy(hit->ptr, data, size, hipMemcpyHostToDevice);
-      hipStreamSynchronize(nullptr);
-    }
+    if (data && size) check(copy_sync(hit->ptr, data, size, hipMemcpyHostToDevice), "pooled copy");
     hit->pooled = true;
     hit->pool_name = key;
     return hit;
@@ -554,16 +574,15 @@ void gpu_hip_buffer_write(GPU_Buffer* buffer, void* data, U64 size)
   if (!buffer || !data) return;
   if (size > buffer->alloc_size) size = buffer->alloc_size;
   /* complete before returning: batch streams are not ordered against the
-   * null stream this copy runs on */
-  if (check(hipMemcpy(buffer->ptr, data, size, hipMemcpyHostToDevice), "buffer_write"))
-    check(hipStreamSynchronize(nullptr), "buffer_write");
+   * stream this copy runs on */
+  check(copy_sync(buffer->ptr, data, size, hipMemcpyHostToDevice), "buffer_write");
 }
 
 void gpu_hip_buffer_read(GPU_Buffer* buffer, void* data, U64 size)
 {
   if (!buffer || !data) return;
   if (size > buffer->alloc_size) size = buffer->alloc_size;
-  if (!check(hipMemcpy(data, buffer->ptr, size, hipMemcpyDeviceToHost), "buffer_read")) memset(data, 0, size);
+  if (!check(copy_sync(data, buffer->ptr, size, hipMemcpyDeviceToHost), "buffer_read")) memset(data, 0, size);
 }
 
 GPU_Kernel* gpu_hip_kernel_alloc(String8 name)
