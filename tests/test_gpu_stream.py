@@ -94,6 +94,53 @@ def test_stream_scan_equals_resident(orc, tables, wi):
     assert st["resident_whole"] or st["chunks_run"] + st["chunks_skipped"] == CHUNKS
 
 
+MORE_WHERES = [
+    col(0).isin([1, 3, 49]),                       # IN list: the bytecode interpreter or a generated kernel
+    col(0).isin([1, 3], negate=True),
+    col(1) != 17,
+    (col(2) < 300_000) | (col(0) == 7),            # OR with a pruning-blind leaf
+]
+
+
+@pytest.mark.parametrize("wi", range(len(MORE_WHERES)))
+def test_stream_other_predicates(orc, tables, wi):
+    dev, host, cols = tables
+    w = MORE_WHERES[wi]
+    nd, idd, _ = dev.scan(w)
+    nh, idh, _ = host.scan(w)
+    assert nh == nd and np.array_equal(idh, idd)
+    assert np.array_equal(idh, orc.scan(cols, w))
+    a = host.aggregate([cq.COUNT, cq.SUM, cq.MAX], [None, 1, 3], where=w, group=0)
+    b = dev.aggregate([cq.COUNT, cq.SUM, cq.MAX], [None, 1, 3], where=w, group=0)
+    _agg_equal(a, b, [cq.COUNT, cq.SUM, cq.MAX], [None, 1, 3], cols)
+
+
+def test_stream_group_by_float_key():
+    """Keys of an f32 column (100 distinct values, -0.0 among them) merge by
+    value bits across chunks like the device hash GROUP BY groups them."""
+    n = 700_001
+    r = np.random.default_rng(9)
+    key = (r.integers(-50, 50, n) * 0.25).astype(np.float32)
+    key[r.integers(0, n, 100)] = np.float32(-0.0)
+    val = r.integers(0, 1000, n).astype(np.uint32)
+    dev, host = cq.Table(n), cq.Table(n)
+    try:
+        for t, add in ((dev, dev.add_column), (host, host.add_host_column)):
+            add("k", cq.F32, key)
+            add("v", cq.U32, val)
+        funcs, args = [cq.COUNT, cq.SUM, cq.MIN], [None, 1, 1]
+        a = host.aggregate(funcs, args, group=0)
+        b = dev.aggregate(funcs, args, group=0)
+        assert a.groups == b.groups
+        oa = np.lexsort((a.keys.view(np.uint64), a.keys))
+        ob = np.lexsort((b.keys.view(np.uint64), b.keys))
+        assert np.array_equal(a.keys[oa].view(np.uint64), b.keys[ob].view(np.uint64))
+        assert np.array_equal(a.cells[oa], b.cells[ob])
+    finally:
+        host.close()
+        dev.close()
+
+
 def test_scan_cap_keeps_the_first_ids(tables):
     dev, host, _ = tables
     w = col(0) < 25
