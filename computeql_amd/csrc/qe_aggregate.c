@@ -617,6 +617,11 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
     qe_set_error("qe_aggregate: null argument");
     return 0;
   }
+  if (!qe_where_non_null(table, where, where_count, "qe_aggregate") ||
+      !qe_require_non_null(table, group_col, "qe_aggregate"))
+    return 0;
+  for (U32 e = 0; e < expr_count; e++)
+    if (!qe_require_non_null(table, arg_cols[e], "qe_aggregate")) return 0;
   if (table->host_backed)
     return qe_stream_aggregate(table, where, where_count, group_col, funcs, arg_cols, expr_count, cap, out_group_count,
                                out_keys, out_cells, trace);

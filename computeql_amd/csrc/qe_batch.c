@@ -68,6 +68,13 @@ static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlan
   if (!q->funcs || !q->arg_cols || !q->out_group_count || !q->out_cells) return 0;
   if (!qe_agg_plan(t, q->funcs, q->arg_cols, q->expr_count, &out->plan)) return 0;
   if (out->plan.arg_count > 2) return 0;
+  /* a query touching a column with NULLs goes to qe_aggregate, which refuses it */
+  for (U32 a = 0; a < out->plan.arg_count; a++)
+    if (t->columns[out->plan.arg_col[a]].has_nulls) return 0;
+  if (q->group_col >= 0 && (U32)q->group_col < t->column_count && t->columns[q->group_col].has_nulls) return 0;
+  for (U32 i = 0; q->where && i < q->where_count; i++)
+    if (q->where[i].type == QE_IR_COLUMN && q->where[i].column < t->column_count && t->columns[q->where[i].column].has_nulls)
+      return 0;
   if (q->where && q->where_count) {
     static __thread QE_Bytecode prog;
     if (!qe_bytecode_program_build(t, q->where, q->where_count, &prog)) return 0;

@@ -213,16 +213,12 @@ S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void*
     qe_set_error("qe_table_add_column: unsupported type %u", type);
     return -1;
   }
-  if (null_flags) {
-    for (U64 i = 0; i < table->row_count; i++) {
-      if (null_flags[i]) {
-        qe_set_error("qe_table_add_column: NULL values are not supported on this path");
-        return -1;
-      }
-    }
-  }
+  B32 has_nulls = 0;
+  if (null_flags)
+    for (U64 i = 0; i < table->row_count && !has_nulls; i++) has_nulls = null_flags[i] != 0;
   QE_Column* col = &table->columns[table->column_count];
   memset(col, 0, sizeof(*col));
+  col->has_nulls = has_nulls;
   snprintf(col->name, sizeof(col->name), "%s", name ? name : "");
   col->type = type;
   col->width = width;
@@ -239,6 +235,28 @@ S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void*
   S32 index = (S32)table->column_count++;
   if (host_data && !qe_table_build_zonemap(table, (U32)index)) return -1;
   return index;
+}
+
+/* The operators here do not evaluate NULLs (the reference's GPU kernels do
+ * not either: its bytecode folds IS NULL to a constant, query_exec.c:
+ * 234-248, and its aggregates refuse nullable columns, :10312-10333).  A
+ * query that touches a column with NULLs is refused rather than answered
+ * on the stored values; other columns of the table stay usable. */
+B32 qe_require_non_null(const QE_Table* t, S32 column, const char* op)
+{
+  if (t && column >= 0 && (U32)column < t->column_count && t->columns[column].has_nulls) {
+    qe_set_error("%s: column '%s' has NULLs; NULL-aware evaluation is not supported on this path", op,
+                 t->columns[column].name);
+    return 0;
+  }
+  return 1;
+}
+
+B32 qe_where_non_null(const QE_Table* t, const QE_IRNode* where, U32 count, const char* op)
+{
+  for (U32 i = 0; where && i < count; i++)
+    if (where[i].type == QE_IR_COLUMN && !qe_require_non_null(t, (S32)where[i].column, op)) return 0;
+  return 1;
 }
 
 GPU_Buffer* qe_table_column_buffer(QE_Table* table, U32 column)
@@ -392,7 +410,8 @@ S32 qe_table_encode_column(QE_Table* table, U32 column, B32 allow_sequence)
     qe_set_error("qe_table_encode_column: bad column");
     return -1;
   }
-  if (!qe_refuse_host(table, "qe_table_encode_column")) return -1;
+  if (!qe_refuse_host(table, "qe_table_encode_column") || !qe_require_non_null(table, (S32)column, "qe_table_encode_column"))
+    return -1;
   QE_Column* col = &table->columns[column];
   const U64 rows = table->row_count;
   if (col->enc) return (S32)col->enc;

@@ -75,6 +75,8 @@ static B32 qe_gather_check(QE_Table* t, const U32* cols, U32 col_count, void* co
   }
   if (!qe_require_device(t, "gather")) return 0;
   for (U32 c = 0; c < col_count; c++)
+    if (!qe_require_non_null(t, (S32)cols[c], "gather")) return 0;
+  for (U32 c = 0; c < col_count; c++)
     if (cols[c] >= t->column_count || !out[c]) {
       qe_set_error("gather: bad column %u", c);
       return 0;

@@ -60,6 +60,9 @@ U32 qe_hash_join(QE_Table* probe, U32 probe_key_col, const U64* probe_rows, U64 
     return 0;
   }
   if (!qe_require_device(probe, "qe_hash_join") || !qe_require_device(build, "qe_hash_join")) return 0;
+  if (!qe_require_non_null(probe, (S32)probe_key_col, "qe_hash_join") ||
+      !qe_require_non_null(build, (S32)build_key_col, "qe_hash_join"))
+    return 0;
   if (probe_key_col >= probe->column_count || build_key_col >= build->column_count) {
     qe_set_error("qe_hash_join: unknown key column");
     return 0;

@@ -64,6 +64,9 @@ U32 qe_type_width(U32 type); /* bytes per row of a column type, 0 if unsupported
 B32 qe_require_device(const QE_Table* t, const char* op);
 /* operators that write columns or add device state refuse host-backed tables */
 B32 qe_refuse_host(const QE_Table* t, const char* op);
+/* a query touching a column with NULLs is refused (column < 0: none) */
+B32 qe_require_non_null(const QE_Table* t, S32 column, const char* op);
+B32 qe_where_non_null(const QE_Table* t, const QE_IRNode* where, U32 count, const char* op);
 
 /* ---- streamed execution over host-backed tables (qe_stream.c) ---- */
 B32 qe_stream_scan(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only, U64* out_ids, U64 cap,

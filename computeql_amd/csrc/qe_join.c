@@ -332,6 +332,13 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     return 0;
   }
   if (!qe_require_device(fact, "qe_aggregate_join") || !qe_require_device(dim, "qe_aggregate_join")) return 0;
+  if (!qe_require_non_null(fact, (S32)fact_key_col, "qe_aggregate_join") ||
+      !qe_require_non_null(dim, (S32)dim_key_col, "qe_aggregate_join") ||
+      !qe_require_non_null(dim, dim_group_col, "qe_aggregate_join") ||
+      !qe_where_non_null(dim, dim_where, dim_where_count, "qe_aggregate_join"))
+    return 0;
+  for (U32 e = 0; e < expr_count; e++)
+    if (!qe_require_non_null(fact, fact_arg_cols[e], "qe_aggregate_join")) return 0;
   if (qe_settings_u64("QE_JOIN_PUSHDOWN", 1) == 0) {
     qe_set_error("qe_aggregate_join: QE_JOIN_PUSHDOWN is off");
     return 0;

@@ -497,6 +497,7 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
 B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only, U64* out_ids, U64 cap,
                    U64* out_count, QE_Trace* trace)
 {
+  if (!qe_where_non_null(table, where, node_count, "qe_scan_filter")) return 0;
   if (table && out_count && table->host_backed)
     return qe_stream_scan(table, where, node_count, count_only, out_ids, cap, out_count, trace);
   return qe_scan_impl(table, where, node_count, count_only, out_ids, cap, out_count, trace, 0);
@@ -508,7 +509,8 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
 B32 qe_scan_filter_device(QE_Table* table, const QE_IRNode* where, U32 node_count, GPU_Buffer** out_ids, U64* out_count)
 {
   *out_ids = 0;
-  if (!qe_require_device(table, "qe_scan_filter_device")) return 0;
+  if (!qe_require_device(table, "qe_scan_filter_device") || !qe_where_non_null(table, where, node_count, "qe_scan_filter"))
+    return 0;
   return qe_scan_impl(table, where, node_count, 0, 0, ~0ull, out_count, 0, out_ids);
 }
 

@@ -114,14 +114,23 @@ class Table:
         except Exception:
             pass
 
-    def add_column(self, name: str, ctype: int, data: np.ndarray | None = None) -> int:
+    def add_column(self, name: str, ctype: int, data: np.ndarray | None = None, nulls: np.ndarray | None = None) -> int:
+        """nulls: a byte per row, nonzero = NULL (cql.h:283); queries that
+        touch such a column are refused (NULL-aware evaluation is not on
+        this path), the table's other columns stay usable."""
         ptr = None
         if data is not None:
             data = np.ascontiguousarray(data, dtype=_NP[ctype])
             if data.shape != (self.rows,):
                 raise ValueError(f"column {name}: expected {self.rows} rows, got {data.shape}")
             ptr = data.ctypes.data
-        idx = self._lib.qe_table_add_column(self.handle, name.encode(), ctype, ptr, None)
+        nptr = None
+        if nulls is not None:
+            nulls = np.ascontiguousarray(nulls, dtype=np.uint8)
+            if nulls.shape != (self.rows,):
+                raise ValueError(f"column {name}: expected {self.rows} NULL flags, got {nulls.shape}")
+            nptr = nulls.ctypes.data
+        idx = self._lib.qe_table_add_column(self.handle, name.encode(), ctype, ptr, nptr)
         if idx < 0:
             raise _err("add_column")
         self.names.append(name)

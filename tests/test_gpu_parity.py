@@ -374,3 +374,37 @@ def test_empty_and_tiny_tables(orc):
     res = t.aggregate([cq.COUNT, cq.SUM, cq.MIN], [None, 0, 0], where=col(0) > 5000)
     assert int(res.cells[0, 0]) == 0 and res.column(1)[0] == 0.0
     t.close()
+
+
+def test_columns_with_nulls_are_refused_per_query():
+    """A column with NULL flags loads; every query touching it is refused
+    (the reference's GPU kernels do not evaluate NULLs: its aggregates
+    refuse such columns, query_exec.c:10312-10333), while queries over the
+    table's other columns run as usual."""
+    n = 100_000
+    r = np.random.default_rng(4)
+    a = r.integers(0, 100, n).astype(np.uint32)
+    b = r.integers(0, 100, n).astype(np.uint32)
+    nulls = (r.random(n) < 0.02).astype(np.uint8)
+    t = cq.Table(n)
+    dim = cq.Table(100)
+    try:
+        t.add_column("a", cq.U32, a)
+        t.add_column("b", cq.U32, b, nulls=nulls)
+        dim.add_column("k", cq.U32, np.arange(100, dtype=np.uint32))
+        assert t.count(col(0) < 10) == int(np.count_nonzero(a < 10))
+        g = t.aggregate([cq.COUNT, cq.SUM], [None, 0], where=col(0) >= 50, group=0)
+        assert g.groups == 50
+        for bad in (lambda: t.scan(col(1) < 10), lambda: t.aggregate([cq.SUM], [1]),
+                    lambda: t.aggregate([cq.COUNT], [None], group=1), lambda: t.aggregate([cq.COUNT], [None], where=col(1) == 3),
+                    lambda: t.gather([1], np.arange(10, dtype=np.uint64)), lambda: cq.hash_join(t, 1, dim, 0),
+                    lambda: cq.aggregate_join(t, 1, dim, 0, [cq.COUNT], [None]), lambda: t.encode(1)):
+            with pytest.raises(cq.QueryError):
+                bad()
+        res = cq.aggregate_batch(t, [cq.AggSpec([cq.COUNT], [None], col(0) < 5), cq.AggSpec([cq.SUM], [0], group=0)])
+        assert int(res[0].cells[0, 0]) == int(np.count_nonzero(a < 5))
+        with pytest.raises(cq.QueryError):
+            cq.aggregate_batch(t, [cq.AggSpec([cq.COUNT], [None], col(0) < 5), cq.AggSpec([cq.SUM], [1])])
+    finally:
+        dim.close()
+        t.close()
