@@ -227,7 +227,7 @@ EVENT_ROWS = 10_000_000
 P99_LIMIT_MS = 50.0
 
 
-def batched_and_served(cq, lib, seconds_per_rate: float):
+def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True):
     """The batched and served forms on the 10M-row 30-column event table
     (synthetic, seeded): one 64-query mixed batch run plain (one query per
     launch) and as a shared scan, then the request loop under Poisson load
@@ -269,6 +269,9 @@ def batched_and_served(cq, lib, seconds_per_rate: float):
                "union_bytes_per_batch": tr["bytes_read"],
                "shared_scan_gbs": round(tr["bytes_read"] / (scan_ns / reps) if scan_ns else 0.0, 1),
                "shared_scan_timed": timed}
+    if not served_too:
+        ev.close()
+        return batched, None
     served = {"workload": batched["workload"] + ", Poisson arrivals cycling through the 64 queries",
               "p99_limit_ms": P99_LIMIT_MS, "seconds_per_rate": seconds_per_rate}
     for name, kw, rates in (("plain", dict(workers=2, max_batch=1), (2000, 4000, 6000, 8000, 10000, 12000, 16000)),
@@ -431,6 +434,9 @@ def main(argv=None) -> int:
     ap.add_argument("--served-seconds", type=float, default=1.0)
     ap.add_argument("--no-encoded", action="store_true", help="skip the narrow-encoded rerun of the steps")
     ap.add_argument("--no-streamed", action="store_true", help="skip the host-backed (cold/warm) streamed runs")
+    ap.add_argument("--only", choices=["batched"],
+                    help="run the 64-query batched comparison alone and print its JSON (for profiling it; not a "
+                         "bench line)")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, read one column once (the known-size read that scales "
                          "rocprofv3 FETCH_SIZE readings)")
@@ -443,6 +449,11 @@ def main(argv=None) -> int:
     import computeql_amd as cq
 
     lib = cq.load()
+    if a.only == "batched":
+        cq.profile(True)
+        batched, _ = batched_and_served(cq, lib, a.served_seconds, served_too=False)
+        print(json.dumps({"batched": batched}), flush=True)
+        return 0
     seed, rows = SEED, a.rows
     log(f"building tables: {rows} fact rows x 5 u32 columns in HBM")
     t0 = time.perf_counter()
