@@ -121,6 +121,7 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
       return 0;
     }
     qe_simple_leaf_words(&leaf_words[6 * l], slot, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+    qe_leaf_nulls(&leaf_words[6 * l], t, leaves[l].column);
     qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(t, leaves[l].column), t->columns[leaves[l].column].enc_base);
   }
   U64 r0 = prune->row_begin, r1 = prune->row_end;
@@ -276,6 +277,7 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
       return 0;
     }
     qe_simple_leaf_words(&leaf_words[6 * l], s, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+    qe_leaf_nulls(&leaf_words[6 * l], t, leaves[l].column);
     qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(t, leaves[l].column), t->columns[leaves[l].column].enc_base);
   }
   const U32 key_enc = qe_col_enc(t, key_col);
@@ -426,6 +428,7 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
       return 0;
     }
     qe_simple_leaf_words(&leaf_words[6 * l], sl, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+    qe_leaf_nulls(&leaf_words[6 * l], t, leaves[l].column);
   }
   if (slots.count > QE_MAX_COLUMN_BINDINGS || key_slot == ~0u) {
     qe_set_error("hash group by: too many columns");

@@ -223,8 +223,14 @@ S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void*
   col->type = type;
   col->width = width;
   /* 64 bytes of slack past the last row keep every 16-byte quad load of
-   * the kernels inside the allocation */
+   * the kernels inside the allocation; a column with NULLs carries its null
+   * bitmap after them (16-byte aligned, one bit a row, 16 bytes of slack) */
   U64 bytes = table->row_count * width + 64;
+  U64 bitmap_words = (table->row_count + 31) / 32;
+  if (has_nulls) {
+    col->null_offset = (bytes + 15) & ~15ull;
+    bytes = col->null_offset + bitmap_words * 4 + 16;
+  }
   col->buffer = gpu_buffer_alloc(bytes, GPU_BufferFlag_ReadWrite | GPU_BufferFlag_DeviceLocal, 0);
   if (!col->buffer) {
     qe_set_error("qe_table_add_column: allocation of %llu bytes failed: %s", (unsigned long long)bytes,
@@ -232,6 +238,22 @@ S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void*
     return -1;
   }
   if (host_data && table->row_count) gpu_buffer_write(col->buffer, (void*)host_data, table->row_count * width);
+  if (has_nulls) {
+    U32* bits = (U32*)calloc(bitmap_words + 4, 4);
+    GPU_Buffer* view = bits ? gpu_hip_buffer_view(col->buffer, col->null_offset, bitmap_words * 4 + 16) : 0;
+    if (!view) {
+      free(bits);
+      gpu_buffer_release(col->buffer);
+      memset(col, 0, sizeof(*col));
+      qe_set_error("qe_table_add_column: null bitmap: out of memory");
+      return -1;
+    }
+    for (U64 i = 0; i < table->row_count; i++)
+      if (null_flags[i]) bits[i >> 5] |= 1u << (i & 31);
+    gpu_buffer_write(view, bits, bitmap_words * 4 + 16);
+    gpu_buffer_release(view);
+    free(bits);
+  }
   S32 index = (S32)table->column_count++;
   if (host_data && !qe_table_build_zonemap(table, (U32)index)) return -1;
   return index;
@@ -254,9 +276,25 @@ B32 qe_require_non_null(const QE_Table* t, S32 column, const char* op)
 
 B32 qe_where_non_null(const QE_Table* t, const QE_IRNode* where, U32 count, const char* op)
 {
+  /* an AND of column-literal comparisons evaluates NULLs (a NULL row fails
+   * its comparison, query_exec.c:13971-14043); other WHERE shapes over a
+   * column with NULLs are refused */
+  B32 nullable = 0;
   for (U32 i = 0; where && i < count; i++)
-    if (where[i].type == QE_IR_COLUMN && !qe_require_non_null(t, (S32)where[i].column, op)) return 0;
-  return 1;
+    if (where[i].type == QE_IR_COLUMN && where[i].column < t->column_count && t->columns[where[i].column].has_nulls)
+      nullable = 1;
+  if (!nullable) return 1;
+  QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
+  U32 n = 0;
+  if (qe_collect_and_leaves(where, count, leaves, &n, QE_SCAN_MAX_LEAVES)) return 1;
+  qe_set_error("%s: a WHERE over a column with NULLs must be an AND of column-literal comparisons on this path", op);
+  return 0;
+}
+
+void qe_leaf_nulls(U32* words, const QE_Table* t, U32 column)
+{
+  const QE_Column* c = &t->columns[column];
+  words[5] = c->has_nulls ? (U32)(c->null_offset / 16) : 0u;
 }
 
 GPU_Buffer* qe_table_column_buffer(QE_Table* table, U32 column)

@@ -25,6 +25,9 @@ typedef struct QE_Column {
   U32 width; /* bytes per row */
   GPU_Buffer* buffer;
   B32 has_nulls;
+  /* with NULLs: byte offset in `buffer` of the null bitmap (bit r of word
+   * r / 32 set = row r is NULL), placed after the rows and their slack */
+  U64 null_offset;
   /* zone maps: min/max per 8192 rows, and of the whole column */
   U64 zone_count;
   F64* zone_min;
@@ -159,6 +162,10 @@ B32 qe_collect_and_leaves(const QE_IRNode* nodes, U32 count, QE_Leaf* leaves, U3
 /* conjuncts usable for zone-map pruning (any top-level AND leaf) */
 U32 qe_collect_prune_leaves(const QE_IRNode* nodes, U32 count, QE_Leaf* leaves, U32 max_leaves);
 void qe_simple_leaf_words(U32* words, U32 slot, U32 type, U32 op, F64 target);
+/* word 5 of a leaf over a column with NULLs: its null bitmap's offset in
+ * 16-byte units (0 = no NULLs); the leaf kernels then count a NULL row as
+ * failing the comparison (query_exec.c:13971-14043) */
+void qe_leaf_nulls(U32* words, const QE_Table* t, U32 column);
 B32 qe_scan_int_interval(U32 op, F64 target, F64 code_offset, F64 max_code, U32* out_lo, U32* out_span);
 
 /* zone-map pruning: dispatch range [row_begin,row_end) and a skip bitmap

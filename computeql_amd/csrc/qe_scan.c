@@ -578,7 +578,11 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
   B32 force_interp = qe_settings_u64("QE_SCAN_FORCE_INTERPRETER", 0) != 0;
   B32 perquery = qe_settings_u64("QE_PERQUERY_KERNELS", 1) != 0 && rows >= qe_settings_u64("QE_PERQUERY_MIN_ROWS", 0);
   B32 simple_ok = is_and && qe_settings_u64("QE_SCAN_SIMPLE_KERNEL", 1) != 0;
-  if (force_interp) path = QE_ScanPath_Interpreter;
+  /* NULLs are evaluated by the library scan_simple only */
+  B32 nullable = 0;
+  for (U32 l = 0; l < leaf_count && is_and; l++) nullable |= table->columns[leaves[l].column].has_nulls;
+  if (nullable) path = QE_ScanPath_Simple;
+  else if (force_interp) path = QE_ScanPath_Interpreter;
   else if (simple_ok) path = perquery && qe_settings_u64("QE_PERQUERY_SIMPLE", 0) ? QE_ScanPath_GeneratedSimple : QE_ScanPath_Simple;
   else if (perquery) path = QE_ScanPath_Generated;
   else path = QE_ScanPath_Interpreter;
@@ -627,6 +631,7 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
     for (U32 l = 0; l < leaf_count; l++) {
       U32 slot = qe_slot_of(&slots, leaves[l].column);
       qe_simple_leaf_words(&leaf_words[6 * l], slot, table->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
+      if (path == QE_ScanPath_Simple) qe_leaf_nulls(&leaf_words[6 * l], table, leaves[l].column);
       /* scan_simple reads encoded columns (a3) as codes */
       if (path == QE_ScanPath_Simple)
         qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(table, leaves[l].column), table->columns[leaves[l].column].enc_base);

@@ -235,7 +235,7 @@ __device__ __forceinline__ qk_u32 qk_leaves_quad(const QK_Args& a, const qk_u32*
  * loaded once (uniform, scalar loads) instead of once per step */
 struct QkLeafSet {
   qk_u32 n;
-  qk_u32 w[8][5];
+  qk_u32 w[8][6];
 };
 
 __device__ __forceinline__ QkLeafSet qk_load_leaves(const qk_u32* leaves, qk_u32 n)
@@ -245,7 +245,7 @@ __device__ __forceinline__ QkLeafSet qk_load_leaves(const qk_u32* leaves, qk_u32
 #pragma unroll
   for (int l = 0; l < 8; l++)
 #pragma unroll
-    for (int i = 0; i < 5; i++)
+    for (int i = 0; i < 6; i++)
       L.w[l][i] = __builtin_amdgcn_readfirstlane(((qk_u32)l < L.n) ? leaves[6 * l + i] : 0u);
   return L;
 }
@@ -291,6 +291,14 @@ __device__ __forceinline__ void qk_leaves_step(const QK_Args& a, const QkLeafSet
       qk_u32 ww[6] = {w[0], w[1], w[2], w[3], w[4], 0u};
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) m[j] &= qk_leaf_quad(a, ww, q[j]);
+    }
+    /* word 5: the column's null bitmap (16-byte units from its base); a
+     * NULL row fails the comparison.  Quad q's four bits sit at bit
+     * 4 (q mod 8) of word q / 8. */
+    if (w[5]) {
+      const qk_u32* nb = qk_gptr<const qk_u32>(a.ptr[QK_B_COL0 + slot] + (qk_u64)w[5] * 16);
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) m[j] &= ~((nb[q[j] >> 3] >> ((qk_u32)(q[j] & 7) * 4)) & 0xfu);
     }
   }
 }

@@ -109,7 +109,9 @@ def gen_scale_events(n: int, row0: int = 0):
 
 
 # ---------------- operator semantics ----------------
-def _columns(cols):
+def _columns(cols, nulls=None):
+    """orc_column[] over the arrays; nulls: {column index: byte-per-row
+    flags, nonzero = NULL} (cql.h:283)"""
     arr = (OrcColumn * len(cols))()
     keep = []
     for i, c in enumerate(cols):
@@ -118,6 +120,10 @@ def _columns(cols):
         arr[i].type = _TYPE_OF[c.dtype]
         arr[i].data = c.ctypes.data
         arr[i].null_flags = None
+        if nulls and nulls.get(i) is not None:
+            f = np.ascontiguousarray(nulls[i], dtype=np.uint8)
+            keep.append(f)
+            arr[i].null_flags = f.ctypes.data
     return arr, keep
 
 
@@ -134,9 +140,9 @@ def _nodes(where):
     return n, len(n)
 
 
-def scan(cols, where, nthreads: int | None = None, count_only: bool = False):
+def scan(cols, where, nthreads: int | None = None, count_only: bool = False, nulls=None):
     """qe_cpu_scan_filter: ascending row ids (or the count)."""
-    arr, keep = _columns(cols)
+    arr, keep = _columns(cols, nulls)
     nodes, nn = _nodes(where)
     rows = len(cols[0])
     lib = load()
@@ -160,10 +166,11 @@ def aggregate_global(cols, workers: int | None = None):
     return s, mn, mx
 
 
-def group_aggregate(cols, key_col, funcs, arg_cols, where=None, max_groups=None):
+def group_aggregate(cols, key_col, funcs, arg_cols, where=None, max_groups=None, nulls=None):
     """qe_aggregate_cpu_reduce over the (filtered) rows: (keys, results[g, e]),
-    groups in first-appearance order. key_col None = one group."""
-    arr, keep = _columns(cols)
+    groups in first-appearance order. key_col None = one group; nulls as
+    for scan (WHERE columns only: NULL arguments are not aggregated here)."""
+    arr, keep = _columns(cols, nulls)
     nodes, nn = _nodes(where)
     rows = len(cols[0])
     e = len(funcs)

@@ -376,35 +376,59 @@ def test_empty_and_tiny_tables(orc):
     t.close()
 
 
-def test_columns_with_nulls_are_refused_per_query():
-    """A column with NULL flags loads; every query touching it is refused
-    (the reference's GPU kernels do not evaluate NULLs: its aggregates
-    refuse such columns, query_exec.c:10312-10333), while queries over the
-    table's other columns run as usual."""
-    n = 100_000
+def test_columns_with_nulls(orc):
+    """A column with NULL flags: an AND of comparisons over it counts a NULL
+    row as failing (the CPU path's semantics, query_exec.c:13971-14043) in
+    scans and in the fused aggregates, equal to the oracle; other WHERE
+    shapes over it, and using it as a GROUP BY key, an aggregate argument,
+    a join key or a gathered column, are refused (the reference's GPU
+    kernels do not evaluate NULLs either: its aggregates refuse such
+    columns, query_exec.c:10312-10333).  The table's other columns work as
+    usual."""
+    n = 1_000_003
     r = np.random.default_rng(4)
     a = r.integers(0, 100, n).astype(np.uint32)
     b = r.integers(0, 100, n).astype(np.uint32)
+    c = r.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
     nulls = (r.random(n) < 0.02).astype(np.uint8)
+    nulls[:8192] = 1                      # one zone all NULL
+    cols, nm = [a, b, c], {1: nulls}
     t = cq.Table(n)
     dim = cq.Table(100)
     try:
         t.add_column("a", cq.U32, a)
         t.add_column("b", cq.U32, b, nulls=nulls)
+        t.add_column("c", cq.U32, c)
         dim.add_column("k", cq.U32, np.arange(100, dtype=np.uint32))
-        assert t.count(col(0) < 10) == int(np.count_nonzero(a < 10))
-        g = t.aggregate([cq.COUNT, cq.SUM], [None, 0], where=col(0) >= 50, group=0)
-        assert g.groups == 50
-        for bad in (lambda: t.scan(col(1) < 10), lambda: t.aggregate([cq.SUM], [1]),
-                    lambda: t.aggregate([cq.COUNT], [None], group=1), lambda: t.aggregate([cq.COUNT], [None], where=col(1) == 3),
+        for w in (col(1) < 10, (col(0) >= 20) & (col(1) == 5), col(1) >= 0, (col(1) > 90) & (col(1) < 95.5),
+                  col(0) < 3):
+            nn, ids, tr = t.scan(w)
+            expect = orc.scan(cols, w, nulls=nm)
+            assert np.array_equal(ids, expect), w
+            assert t.count(w) == len(expect), w
+        for w, group, funcs, args in ((col(1) < 50, 0, [cq.COUNT, cq.SUM, cq.MAX], [None, 0, 2]),
+                                      ((col(1) > 90) & (col(0) < 50), None, [cq.COUNT, cq.SUM, cq.MIN], [None, 0, 2]),
+                                      (col(1) < 5, 2, [cq.COUNT, cq.SUM], [None, 0])):
+            g = t.aggregate(funcs, args, where=w, group=group)
+            rk, rv = orc.group_aggregate(cols, group, funcs, args, where=w, nulls=nm)
+            o, og = np.argsort(rk, kind="stable"), np.argsort(g.keys, kind="stable")
+            assert g.groups == len(rk), w
+            if group is not None:
+                assert np.array_equal(g.keys[og], rk[o]), w
+            for e in range(len(funcs)):
+                assert np.array_equal(g.column(e).astype(np.float64)[og], rv[o, e]), (w, e)
+        res = cq.aggregate_batch(t, [cq.AggSpec([cq.COUNT], [None], col(0) < 5),
+                                     cq.AggSpec([cq.COUNT, cq.SUM], [None, 0], col(1) < 30, 0)])
+        assert int(res[0].cells[0, 0]) == int(np.count_nonzero(a < 5)) and res[1].trace["fused"] == 0
+        rk, rv = orc.group_aggregate(cols, 0, [cq.COUNT, cq.SUM], [None, 0], where=col(1) < 30, nulls=nm)
+        assert res[1].groups == len(rk) and int(res[1].cells[:, 0].sum()) == int(rv[:, 0].sum())
+        for bad in (lambda: t.scan((col(1) < 10) | (col(0) < 3)), lambda: t.scan(col(1).isin([1, 2])),
+                    lambda: t.aggregate([cq.SUM], [1]), lambda: t.aggregate([cq.COUNT], [None], group=1),
+                    lambda: t.aggregate([cq.COUNT], [None], where=(col(1) == 3) | (col(0) == 3)),
                     lambda: t.gather([1], np.arange(10, dtype=np.uint64)), lambda: cq.hash_join(t, 1, dim, 0),
                     lambda: cq.aggregate_join(t, 1, dim, 0, [cq.COUNT], [None]), lambda: t.encode(1)):
             with pytest.raises(cq.QueryError):
                 bad()
-        res = cq.aggregate_batch(t, [cq.AggSpec([cq.COUNT], [None], col(0) < 5), cq.AggSpec([cq.SUM], [0], group=0)])
-        assert int(res[0].cells[0, 0]) == int(np.count_nonzero(a < 5))
-        with pytest.raises(cq.QueryError):
-            cq.aggregate_batch(t, [cq.AggSpec([cq.COUNT], [None], col(0) < 5), cq.AggSpec([cq.SUM], [1])])
     finally:
         dim.close()
         t.close()

@@ -111,6 +111,32 @@ def test_scan_against_numpy(orc):
         assert np.array_equal(ids, np.flatnonzero(mask)), repr(where)
 
 
+def test_scan_with_nulls_against_numpy(orc):
+    """NULL semantics of the CPU path (query_exec.c:13871-14043): a NULL
+    operand fails every comparison, NULL [NOT] IN is never true, IS [NOT]
+    NULL tests the flag, AND/OR combine the results."""
+    cols = _brute_cols(orc)
+    _, region, amount, ts = cols
+    r = np.random.default_rng(8)
+    nr = (r.random(len(region)) < 0.1).astype(np.uint8)
+    na = (r.random(len(amount)) < 0.05).astype(np.uint8)
+    nulls = {1: nr, 2: na}
+    vr, va = nr == 0, na == 0
+    cases = [
+        (col(2) < 200, va & (amount < 200)),
+        ((col(1) == 7) & (col(2) >= 5000), vr & (region == 7) & va & (amount >= 5000)),
+        (col(2) != 17, va & (amount != 17)),
+        ((col(1) == 7) | (col(2) < 10), (vr & (region == 7)) | (va & (amount < 10))),
+        (col(1).isin([1, 3]), vr & np.isin(region, [1, 3])),
+        (col(1).isin([1, 3], negate=True), vr & ~np.isin(region, [1, 3])),
+        (col(1).is_null(), ~vr),
+        (col(2).is_not_null() & (col(3) < 2**31), va & (ts < 2**31)),
+    ]
+    for where, mask in cases:
+        ids = orc.scan(cols, where, nulls=nulls)
+        assert np.array_equal(ids, np.flatnonzero(mask)), repr(where)
+
+
 def test_group_aggregate_against_numpy(orc):
     cols = _brute_cols(orc)
     _, region, amount, ts = cols
