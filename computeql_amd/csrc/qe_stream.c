@@ -48,6 +48,7 @@ typedef struct QE_ResidentEntry {
   B32 ready; /* a chunk's copy has completed; for a whole column, every chunk's */
   U64 uid;
   U64 chunk; /* chunk index, or QE_WHOLE_COLUMN: one buffer for all the rows */
+  U64 chunk_rows; /* the chunk size the index and the chunk states refer to */
   GPU_Buffer* buffer;
   U64 bytes;
   U64 last_use;
@@ -161,7 +162,7 @@ static B32 res_room_locked(U64 bytes)
   return res_evict_locked(over) >= over;
 }
 
-static QE_ResidentEntry* res_new_entry_locked(QE_ResidentEntry* slot, U64 uid, U64 chunk, U64 bytes)
+static QE_ResidentEntry* res_new_entry_locked(QE_ResidentEntry* slot, U64 uid, U64 chunk, U64 chunk_rows, U64 bytes)
 {
   GPU_Buffer* buf = gpu_buffer_alloc(bytes, GPU_BufferFlag_ReadWrite | GPU_BufferFlag_DeviceLocal, 0);
   if (!buf) return 0;
@@ -169,6 +170,7 @@ static QE_ResidentEntry* res_new_entry_locked(QE_ResidentEntry* slot, U64 uid, U
   slot->used = 1;
   slot->uid = uid;
   slot->chunk = chunk;
+  slot->chunk_rows = chunk_rows;
   slot->buffer = buf;
   slot->bytes = bytes;
   slot->last_use = ++g_res.tick;
@@ -207,12 +209,12 @@ static QE_ResidentEntry* res_acquire(const QE_Column* col, U64 rows, U64 chunk, 
     }
     if (e->uid != col->uid) continue;
     if (e->chunk == QE_WHOLE_COLUMN) whole = e;
-    else if (e->chunk == chunk && e->ready && !got) got = e;
+    else if (e->chunk == chunk && e->chunk_rows == chunk_rows && e->ready && !got) got = e;
   }
   if (!whole && !got && free_slot && qe_settings_u64("QE_RESIDENT_WHOLE", 1) && res_room_locked(rows * col->width + 64)) {
     const U64 nchunks = (rows + chunk_rows - 1) / chunk_rows;
     U8* state = (U8*)calloc(nchunks, 1);
-    if (state && (whole = res_new_entry_locked(free_slot, col->uid, QE_WHOLE_COLUMN, rows * col->width + 64))) {
+    if (state && (whole = res_new_entry_locked(free_slot, col->uid, QE_WHOLE_COLUMN, chunk_rows, rows * col->width + 64))) {
       whole->nchunks = nchunks;
       whole->state = state;
       whole->uses = 0;
@@ -221,7 +223,20 @@ static QE_ResidentEntry* res_acquire(const QE_Column* col, U64 rows, U64 chunk, 
       free(state);
     }
   }
-  if (whole && chunk < whole->nchunks && whole->state[chunk] != 1) {
+  /* a filled column serves any chunk size; a partly filled one only the
+   * chunk size its states were kept in */
+  if (whole && whole->ready) {
+    GPU_Buffer* v = gpu_hip_buffer_view(whole->buffer, c0 * col->width, bytes);
+    if (v) {
+      whole->pins++;
+      whole->last_use = ++g_res.tick;
+      if (chunk == 0) whole->uses++;
+      g_res.hit_bytes += bytes;
+      *view = v;
+      res_unlock();
+      return whole;
+    }
+  } else if (whole && whole->chunk_rows == chunk_rows && chunk < whole->nchunks && whole->state[chunk] != 1) {
     GPU_Buffer* v = gpu_hip_buffer_view(whole->buffer, c0 * col->width, bytes);
     if (v) {
       whole->pins++;
@@ -248,7 +263,7 @@ static QE_ResidentEntry* res_acquire(const QE_Column* col, U64 rows, U64 chunk, 
     return got;
   }
   if (!whole && free_slot && res_room_locked(bytes)) {
-    got = res_new_entry_locked(free_slot, col->uid, chunk, bytes);
+    got = res_new_entry_locked(free_slot, col->uid, chunk, chunk_rows, bytes);
     if (got) {
       got->pins = 1;
       g_res.miss_bytes += bytes;
@@ -268,7 +283,7 @@ static void res_unpin(QE_ResidentEntry* e, U64 chunk, B32 filled)
   if (e->used) {
     if (e->pins) e->pins--;
     if (e->chunk == QE_WHOLE_COLUMN) {
-      if (chunk < e->nchunks && e->state[chunk] == 1) {
+      if (!e->ready && chunk < e->nchunks && e->state[chunk] == 1) {
         e->state[chunk] = filled ? 2 : 0;
         if (filled) e->nfilled++;
         e->ready = e->nfilled == e->nchunks;

@@ -239,6 +239,26 @@ def test_zone_pruned_chunks_are_not_copied(tables):
     assert st["chunks_skipped"] == CHUNKS - 2 and st["bytes_copied"] == 2 * CHUNK * 4
 
 
+def test_chunk_size_change_between_queries(tables):
+    """A column partly filled under one chunk size is not read under
+    another; once whole it serves any."""
+    dev, host, _ = tables
+    cq.resident_clear()
+    host.scan(col(2) < 200_000)                    # chunks 0-1 of 131072 rows filled
+    cq.set_setting("QE_STREAM_CHUNK_ROWS", 3 * CHUNK)
+    for w in (col(2) < 600_000, col(2) >= 0):
+        nh, idh, _ = host.scan(w)
+        nd, idd, _ = dev.scan(w)
+        assert nh == nd and np.array_equal(idh, idd), w
+    cq.set_setting("QE_STREAM_CHUNK_ROWS", CHUNK)
+    host.scan(col(2) >= 0)                         # fills the rest under the first size
+    cq.set_setting("QE_STREAM_CHUNK_ROWS", 5 * CHUNK)
+    nh, idh, _ = host.scan(col(2) > 123_456)
+    nd, idd, _ = dev.scan(col(2) > 123_456)
+    assert nh == nd and np.array_equal(idh, idd)
+    assert cq.stream_stats()["resident_whole"] == 1
+
+
 def test_small_budget_evicts_and_streams(tables):
     dev, host, cols = tables
     funcs, args = AGGS[1]
