@@ -79,6 +79,9 @@ static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlan
     static __thread QE_Bytecode prog;
     if (!qe_bytecode_program_build(t, q->where, q->where_count, &prog)) return 0;
     if (!qe_collect_and_leaves(q->where, q->where_count, out->leaves, &out->leaf_count, QE_SCAN_MAX_LEAVES)) return 0;
+    /* IS [NOT] NULL leaves run on the plain operators */
+    for (U32 l = 0; l < out->leaf_count; l++)
+      if (out->leaves[l].op >= QE_LEAF_IS_NULL) return 0;
   }
   out->group_col = q->group_col;
   out->range = 1;

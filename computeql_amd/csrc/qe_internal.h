@@ -128,6 +128,11 @@ U32 qe_slot_of(QE_ColumnSlots* slots, U32 column); /* ~0u when out of slots */
 #define QE_SCAN_INT_FLAG 0x10000u
 #define QE_SCAN_INT_NEVER 0x20000u
 #define QE_SCAN_INT_BIAS 0x40000u
+/* leaf ops past the five comparisons: IS NULL (5) and IS NOT NULL (6),
+ * flagged in word 1 so no value is compared */
+#define QE_LEAF_IS_NULL 5u
+#define QE_LEAF_IS_NOT_NULL 6u
+#define QE_SCAN_NULL_TEST 0x80000u
 
 /* the encoding kernels are given for a column (0 unless it is encoded and
  * QE_ENCODE is on), and the buffer they read for it */

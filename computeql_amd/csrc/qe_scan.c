@@ -229,6 +229,14 @@ static U32 qe_collect_and_nodes(const QE_IRNode* nodes, U32 i, U32* out, U32 n, 
 static B32 qe_resolve_leaf(const QE_IRNode* nodes, U32 i, QE_Leaf* leaf)
 {
   const QE_IRNode* n = &nodes[i];
+  /* `column IS [NOT] NULL` (query_exec.c:13902-13917) */
+  if (n->type == QE_IR_OPERATOR && n->child_count == 1 && (n->op == QE_IROP_IS_NULL || n->op == QE_IROP_IS_NOT_NULL) &&
+      nodes[i + 1].type == QE_IR_COLUMN) {
+    leaf->column = nodes[i + 1].column;
+    leaf->op = n->op == QE_IROP_IS_NULL ? QE_LEAF_IS_NULL : QE_LEAF_IS_NOT_NULL;
+    leaf->target = 0.0;
+    return 1;
+  }
   if (n->type != QE_IR_OPERATOR || n->child_count != 2) return 0;
   U32 left = i + 1, right = qe_ir_subtree_end(nodes, left);
   const QE_IRNode *l = &nodes[left], *r = &nodes[right];
@@ -318,6 +326,11 @@ void qe_simple_leaf_words(U32* words, U32 slot, U32 type, U32 op, F64 target)
   words[3] = (U32)(bits & 0xffffffffu);
   words[4] = (U32)(bits >> 32);
   words[5] = 0;
+  if (op >= QE_LEAF_IS_NULL) {
+    words[1] = type | QE_SCAN_NULL_TEST;
+    words[3] = words[4] = 0;
+    return;
+  }
   if ((type == CQL_ColumnType_U32 || type == CQL_ColumnType_I32) && qe_settings_u64("QE_SCAN_INT_COMPARE", 1) != 0) {
     F64 code_offset = 0.0, max_code = 4294967295.0;
     U32 flags = QE_SCAN_INT_FLAG;
@@ -337,6 +350,7 @@ void qe_simple_leaf_words(U32* words, U32 slot, U32 type, U32 op, F64 target)
 /* ---------------- zone maps ---------------- */
 static B32 qe_zone_prunable(const QE_Leaf* leaf, F64 zmin, F64 zmax)
 {
+  if (leaf->op >= QE_LEAF_IS_NULL) return 0; /* zone maps hold no NULL counts */
   switch (leaf->op) {
     case 0: return leaf->target < zmin || leaf->target > zmax;
     case 1: return zmin >= leaf->target;
@@ -380,6 +394,7 @@ void qe_zonemap_build_super(QE_Column* col)
 static int qe_super_state(const QE_Leaf* leaf, const F64* g)
 {
   const F64 t = leaf->target, mnmn = g[0], mxmn = g[1], mnmx = g[2], mxmx = g[3];
+  if (leaf->op >= QE_LEAF_IS_NULL) return 0;
   switch (leaf->op) {
     case 0:
       if (t < mnmn || t > mxmx) return 1;
@@ -530,8 +545,13 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
     qe_set_error("qe_scan_filter: %s", prog.error);
     return 0;
   }
-  /* constant predicates (no column read) */
-  if (prog.word_count == 2 && (prog.words[0] == QE_Opcode_PushTrue || prog.words[0] == QE_Opcode_PushFalse)) {
+  /* constant predicates (no column read); IS [NOT] NULL over a column
+   * with NULLs is not constant */
+  B32 reads_nulls = 0;
+  for (U32 i = 0; where && i < node_count; i++)
+    if (where[i].type == QE_IR_COLUMN && where[i].column < table->column_count && table->columns[where[i].column].has_nulls)
+      reads_nulls = 1;
+  if (!reads_nulls && prog.word_count == 2 && (prog.words[0] == QE_Opcode_PushTrue || prog.words[0] == QE_Opcode_PushFalse)) {
     U64 n = prog.words[0] == QE_Opcode_PushTrue ? rows : 0;
     *out_count = n;
     if (!count_only && out_ids)
@@ -580,7 +600,8 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
   B32 simple_ok = is_and && qe_settings_u64("QE_SCAN_SIMPLE_KERNEL", 1) != 0;
   /* NULLs are evaluated by the library scan_simple only */
   B32 nullable = 0;
-  for (U32 l = 0; l < leaf_count && is_and; l++) nullable |= table->columns[leaves[l].column].has_nulls;
+  for (U32 l = 0; l < leaf_count && is_and; l++)
+    nullable |= table->columns[leaves[l].column].has_nulls || leaves[l].op >= QE_LEAF_IS_NULL;
   if (nullable) path = QE_ScanPath_Simple;
   else if (force_interp) path = QE_ScanPath_Interpreter;
   else if (simple_ok) path = perquery && qe_settings_u64("QE_PERQUERY_SIMPLE", 0) ? QE_ScanPath_GeneratedSimple : QE_ScanPath_Simple;

@@ -51,6 +51,7 @@ typedef struct QK_Args {
 #define QK_LEAF_INT 0x10000u
 #define QK_LEAF_NEVER 0x20000u
 #define QK_LEAF_BIAS 0x40000u
+#define QK_LEAF_NULL_TEST 0x80000u /* op 5 IS NULL, 6 IS NOT NULL: the null bitmap alone */
 
 /* scan bindings (query_exec.h:4-40) */
 #define QK_B_LEAVES 0
@@ -260,6 +261,23 @@ __device__ __forceinline__ void qk_leaves_step(const QK_Args& a, const QkLeafSet
     const qk_u32* w = L.w[l];
     qk_u32 slot = w[0], tw = w[1];
     qk_u32 type = tw & 0xffu;
+    if (tw & QK_LEAF_NULL_TEST) {
+      /* IS NULL keeps the rows whose bit is set, IS NOT NULL the others; a
+       * column without NULLs (word 5 zero) has no bit set */
+      const bool is_null = w[2] == 5u;
+      if (w[5]) {
+        const qk_u32* nb = qk_gptr<const qk_u32>(a.ptr[QK_B_COL0 + slot] + (qk_u64)w[5] * 16);
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) {
+          const qk_u32 bits = (nb[q[j] >> 3] >> ((qk_u32)(q[j] & 7) * 4)) & 0xfu;
+          m[j] &= is_null ? bits : ~bits;
+        }
+      } else if (is_null) {
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) m[j] = 0u;
+      }
+      continue;
+    }
     if (tw & QK_LEAF_NEVER) {
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) m[j] = 0u;

@@ -401,14 +401,17 @@ def test_columns_with_nulls(orc):
         t.add_column("c", cq.U32, c)
         dim.add_column("k", cq.U32, np.arange(100, dtype=np.uint32))
         for w in (col(1) < 10, (col(0) >= 20) & (col(1) == 5), col(1) >= 0, (col(1) > 90) & (col(1) < 95.5),
-                  col(0) < 3):
+                  col(0) < 3, col(1).is_null(), col(1).is_not_null(), (col(0) < 7) & col(1).is_null(),
+                  col(1).is_not_null() & (col(0) == 3), col(0).is_null(), col(0).is_not_null() & (col(1) < 4)):
             nn, ids, tr = t.scan(w)
             expect = orc.scan(cols, w, nulls=nm)
             assert np.array_equal(ids, expect), w
             assert t.count(w) == len(expect), w
         for w, group, funcs, args in ((col(1) < 50, 0, [cq.COUNT, cq.SUM, cq.MAX], [None, 0, 2]),
                                       ((col(1) > 90) & (col(0) < 50), None, [cq.COUNT, cq.SUM, cq.MIN], [None, 0, 2]),
-                                      (col(1) < 5, 2, [cq.COUNT, cq.SUM], [None, 0])):
+                                      (col(1) < 5, 2, [cq.COUNT, cq.SUM], [None, 0]),
+                                      (col(1).is_null(), 0, [cq.COUNT, cq.SUM], [None, 2]),
+                                      (col(1).is_not_null() & (col(0) > 90), None, [cq.COUNT, cq.MAX], [None, 2])):
             g = t.aggregate(funcs, args, where=w, group=group)
             rk, rv = orc.group_aggregate(cols, group, funcs, args, where=w, nulls=nm)
             o, og = np.argsort(rk, kind="stable"), np.argsort(g.keys, kind="stable")
