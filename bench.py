@@ -30,7 +30,9 @@ The same JSON line also carries, measured after the timed steps:
            one query per launch vs one shared-scan batch (QPS, speedup)
   served   the request loop under Poisson arrivals at rising rates, plain
            (one query per launch) and batched: the highest rate whose p99
-           latency stays within 50 ms, and the ratio of the two
+           latency stays within 50 ms, and the ratio of the two; on the
+           event table's 64 queries, and (served.mix) on those plus five
+           query shapes over each of 4 x u32 tables of 1M, 100M and 1B rows
   roofline.ceilings  what this device sustains in the same run: a plain
            4 GB read and a 4 GB copy (read + write bytes); the dominant
            kernel's fraction of the matching one is frac_of_measured_ceiling
@@ -227,7 +229,37 @@ EVENT_ROWS = 10_000_000
 P99_LIMIT_MS = 50.0
 
 
-def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True):
+def served_sweep(Server, name, kw, rates, drive, seconds_per_rate):
+    """rates in rising order until p99 passes the limit or the achieved rate
+    falls 10% below the offered one; the highest rate that held"""
+    points, best = [], None
+    with Server(**kw) as srv:
+        drive(srv, rates[0], 0.3, 1)
+        for r in rates:
+            st = drive(srv, r, seconds_per_rate, r)
+            points.append({"offered": r, "qps": round(st["qps"], 1), "p50_ms": round(st["p50_ms"], 3),
+                           "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 1)})
+            log(f"{name} offered {r}: {st['qps']:.0f} qps p99 {st['p99_ms']:.2f} ms")
+            if st["p99_ms"] > P99_LIMIT_MS or st["qps"] < 0.9 * r:
+                break
+            best = points[-1]
+    return {"config": kw, "rates": points, "max_qps_at_p99": best["qps"] if best else 0.0,
+            "p99_ms_there": best["p99_ms"] if best else None}
+
+
+def mix_queries(cq, r: int):
+    """configs 1-2's query shapes over a 4 x u32 table (id, region, amount,
+    ts); joins are not served by the request loop"""
+    from computeql_amd.ir import col
+
+    return [cq.AggSpec([cq.SUM, cq.MIN, cq.MAX], [2, 2, 2]),
+            cq.AggSpec([cq.COUNT], [None], col(2) < 200),
+            cq.AggSpec([cq.COUNT, cq.SUM], [None, 2], None, 1),
+            cq.AggSpec([cq.COUNT, cq.SUM, cq.MIN, cq.MAX], [None, 2, 2, 2], None, 2),
+            cq.AggSpec([cq.COUNT], [None], ((col(1) == 7) & (col(2) < 5000)) | (col(3) > 4_000_000_000))]
+
+
+def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True, fact=None):
     """The batched and served forms on the 10M-row 30-column event table
     (synthetic, seeded): one 64-query mixed batch run plain (one query per
     launch) and as a shared scan, then the request loop under Poisson load
@@ -274,24 +306,45 @@ def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True
         return batched, None
     served = {"workload": batched["workload"] + ", Poisson arrivals cycling through the 64 queries",
               "p99_limit_ms": P99_LIMIT_MS, "seconds_per_rate": seconds_per_rate}
-    for name, kw, rates in (("plain", dict(workers=2, max_batch=1), (2000, 4000, 6000, 8000, 10000, 12000, 16000)),
-                            ("batched", dict(workers=3, max_batch=256),
-                             (8000, 16000, 32000, 48000, 64000, 80000, 96000))):
-        points, best = [], None
-        with Server(**kw) as srv:
-            srv.drive(ev, specs, rate_qps=rates[0], seconds=0.3, seed=1)
-            for r in rates:
-                st = srv.drive(ev, specs, rate_qps=r, seconds=seconds_per_rate, seed=r)
-                points.append({"offered": r, "qps": round(st["qps"], 1), "p50_ms": round(st["p50_ms"], 3),
-                               "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 1)})
-                log(f"served {name} offered {r}: {st['qps']:.0f} qps p99 {st['p99_ms']:.2f} ms")
-                if st["p99_ms"] > P99_LIMIT_MS or st["qps"] < 0.9 * r:
-                    break
-                best = points[-1]
-        served[name] = {"config": kw, "rates": points,
-                        "max_qps_at_p99": best["qps"] if best else 0.0, "p99_ms_there": best["p99_ms"] if best else None}
+    plain_kw, batched_kw = dict(workers=2, max_batch=1), dict(workers=3, max_batch=256)
+
+    def drive_ev(srv, r, secs, seed):
+        return srv.drive(ev, specs, rate_qps=r, seconds=secs, seed=seed)
+
+    served["plain"] = served_sweep(Server, "served plain", plain_kw, (2000, 4000, 6000, 8000, 10000, 12000, 16000),
+                                   drive_ev, seconds_per_rate)
+    served["batched"] = served_sweep(Server, "served batched", batched_kw,
+                                     (8000, 16000, 32000, 48000, 64000, 80000, 96000), drive_ev, seconds_per_rate)
     p = served["plain"]["max_qps_at_p99"]
     served["speedup_at_p99"] = round(served["batched"]["max_qps_at_p99"] / p, 2) if p else None
+    # the served mix: the event table's 64 queries and configs 1-2's five
+    # shapes over 4 x u32 tables of 1M, 100M and (when given) 1B rows
+    mix = None
+    if fact is not None:
+        small, mid = cq.Table(1_000_000), cq.Table(100_000_000)
+        for t4 in (small, mid):
+            for name in ("id", "region", "amount", "ts"):
+                t4.add_column(name, cq.U32)
+            t4.gen_events(SEED, id=0, region=1, amount=2, ts=3)
+        pairs = [(ev, sp) for sp in specs]
+        for t4 in (small, mid, fact):
+            pairs += [(t4, sp) for sp in mix_queries(cq, t4.rows)]
+
+        def drive_mix(srv, r, secs, seed):
+            return srv.drive_mix(pairs, rate_qps=r, seconds=secs, seed=seed)
+
+        mix = {"workload": f"{len(pairs)} queries cycled: the event table's 64 and five query shapes (configs 1-2) "
+                           "on each of 4 x u32 tables of 1M, 100M and 1B rows; Poisson arrivals",
+               "p99_limit_ms": P99_LIMIT_MS, "seconds_per_rate": seconds_per_rate}
+        mix["plain"] = served_sweep(Server, "served mix plain", plain_kw, (1000, 2000, 4000, 6000, 8000, 10000, 12000),
+                                    drive_mix, seconds_per_rate)
+        mix["batched"] = served_sweep(Server, "served mix batched", batched_kw,
+                                      (4000, 8000, 16000, 32000, 48000, 64000, 80000), drive_mix, seconds_per_rate)
+        p = mix["plain"]["max_qps_at_p99"]
+        mix["speedup_at_p99"] = round(mix["batched"]["max_qps_at_p99"] / p, 2) if p else None
+        small.close()
+        mid.close()
+    served["mix"] = mix
     ev.close()
     return batched, served
 
@@ -529,10 +582,10 @@ def main(argv=None) -> int:
 
     batched = served = None
     if not a.no_served:
-        t.close()
         dima.close()
         dimb.close()
-        batched, served = batched_and_served(cq, lib, a.served_seconds)
+        batched, served = batched_and_served(cq, lib, a.served_seconds, fact=t)
+        t.close()
 
     cpu = None
     if not a.no_cpu_baseline:

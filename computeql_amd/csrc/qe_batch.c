@@ -401,6 +401,62 @@ static void qe_batch_config_from_settings(QE_BatchConfig* c)
   c->lds_budget = qe_settings_u64("QE_BATCH_LDS_BYTES", 0);
 }
 
+/* the same query: WHERE nodes, key, functions, arguments and capacity */
+static B32 qe_batch_same(const QE_BatchQuery* a, const QE_BatchQuery* b)
+{
+  if (a->where_count != b->where_count || a->group_col != b->group_col || a->expr_count != b->expr_count ||
+      a->cap != b->cap)
+    return 0;
+  if (a->where_count && (!a->where || !b->where || memcmp(a->where, b->where, a->where_count * sizeof(QE_IRNode))))
+    return 0;
+  return memcmp(a->funcs, b->funcs, a->expr_count * 4) == 0 && memcmp(a->arg_cols, b->arg_cols, a->expr_count * 4) == 0;
+}
+
+static void qe_batch_copy_result(const QE_BatchQuery* from, QE_BatchQuery* to)
+{
+  to->ok = from->ok;
+  to->fused = from->fused;
+  if (!from->ok) return;
+  U64 g = *from->out_group_count;
+  *to->out_group_count = g;
+  U64 n = g < to->cap ? g : to->cap;
+  if (to->out_keys && from->out_keys) memcpy(to->out_keys, from->out_keys, n * sizeof(F64));
+  memcpy(to->out_cells, from->out_cells, n * to->expr_count * sizeof(U64));
+}
+
+/* A launch's queries as one shared scan, or one plain operator each: the
+ * shared scan reads the union of their columns once but costs more per
+ * query and row than a single-query kernel, which streams at the HBM rate.
+ * Estimates: shared = max(union bytes / BW, queries x rows / R) and plain
+ * = the queries' bytes / BW + queries x launch, with BW QE_BATCH_COST_BW_GBS
+ * (5500), R QE_BATCH_COST_SHARED_MROWS (600000 M query-rows/s, measured on
+ * the 64-query mix) and launch QE_BATCH_COST_LAUNCH_US (60). */
+static B32 qe_batch_plain_cheaper(QE_Table* t, const QE_BatchPlanned* qs, U32 n)
+{
+  if (qe_settings_u64("QE_BATCH_COST", 1) == 0) return 0;
+  const F64 bw = (F64)qe_settings_u64("QE_BATCH_COST_BW_GBS", 5500) * 1e9;
+  const F64 rate = (F64)qe_settings_u64("QE_BATCH_COST_SHARED_MROWS", 600000) * 1e6;
+  const F64 launch = (F64)qe_settings_u64("QE_BATCH_COST_LAUNCH_US", 60) * 1e-6;
+  const F64 rows = (F64)t->row_count;
+  QE_BatchStage all;
+  memset(&all, 0, sizeof(all));
+  F64 plain = 0.0;
+  for (U32 i = 0; i < n; i++) {
+    QE_BatchStage one;
+    memset(&one, 0, sizeof(one));
+    qe_batch_stage_query(&one, &qs[i]);
+    qe_batch_stage_query(&all, &qs[i]);
+    F64 bytes = 0.0;
+    for (U32 c = 0; c < one.count; c++) bytes += rows * t->columns[one.column[c]].width;
+    plain += bytes / bw + launch;
+  }
+  F64 ubytes = 0.0;
+  for (U32 c = 0; c < all.count; c++) ubytes += rows * t->columns[all.column[c]].width;
+  F64 shared = ubytes / bw;
+  if ((F64)n * rows / rate > shared) shared = (F64)n * rows / rate;
+  return plain < shared;
+}
+
 static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 count, const QE_BatchConfig* cfg_in,
                                   QE_BatchTrace* trace)
 {
@@ -427,9 +483,24 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
   }
   U32 np = 0, fallback = 0;
   B32 all_ok = 1;
+  /* identical queries of the batch run once; the others copy the result
+   * (QE_BATCH_DEDUPE, 1) */
+  U32* rep_of = (U32*)malloc(sizeof(U32) * (count ? count : 1));
+  if (!rep_of) {
+    free(planned);
+    qe_set_error("qe_aggregate_batch: out of memory");
+    return 0;
+  }
+  const B32 dedupe = qe_settings_u64("QE_BATCH_DEDUPE", 1) != 0;
+  for (U32 i = 0; i < count; i++) {
+    rep_of[i] = ~0u;
+    for (U32 j = 0; dedupe && j < i && rep_of[i] == ~0u; j++)
+      if (rep_of[j] == ~0u && qe_batch_same(&queries[i], &queries[j])) rep_of[i] = j;
+  }
   for (U32 i = 0; i < count; i++) {
     queries[i].ok = 0;
     queries[i].fused = 0;
+    if (rep_of[i] != ~0u) continue;
     QE_BatchStage one;
     memset(&one, 0, sizeof(one));
     /* a host-backed table's queries each stream through the resident chunk
@@ -461,6 +532,7 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
       free(ordered);
       free(taken);
       free(planned);
+      free(rep_of);
       qe_set_error("qe_aggregate_batch: out of memory");
       return 0;
     }
@@ -519,17 +591,32 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
         bq->ok = 1;
         bq->fused = 1;
       }
+    } else if (qe_batch_plain_cheaper(table, planned + start, end - start)) {
+      for (U32 i = start; i < end; i++) {
+        QE_BatchQuery* q = &queries[planned[i].index];
+        q->ok = qe_aggregate(table, q->where, q->where_count, q->group_col, q->funcs, q->arg_cols, q->expr_count, q->cap,
+                             q->out_group_count, q->out_keys, q->out_cells, 0);
+        q->fused = 0;
+        if (!q->ok) all_ok = 0;
+        fallback++;
+      }
     } else if (!qe_batch_launch(table, planned + start, end - start, queries, &cfg, trace)) {
       all_ok = 0;
       break;
     }
     start = end;
   }
+  for (U32 i = 0; i < count; i++)
+    if (rep_of[i] != ~0u) qe_batch_copy_result(&queries[rep_of[i]], &queries[i]);
+  free(rep_of);
   free(planned);
   if (trace) {
+    U32 fused = 0;
+    for (U32 i = 0; i < count; i++) fused += queries[i].fused ? 1u : 0u;
+    (void)fallback;
     trace->queries = count;
-    trace->fallback = fallback;
-    trace->fused = count - fallback;
+    trace->fallback = count - fused;
+    trace->fused = fused;
     QE_Trace tmp;
     qe_trace_end(&tmp, t0, p0);
     trace->gpu_kernel_ns = tmp.gpu_kernel_ns;

@@ -373,10 +373,41 @@ static U64 qe_drive_rand(U64* state)
   return z ^ (z >> 31);
 }
 
+static B32 qe_server_drive_impl(QE_Server* s, QE_Table* table, QE_Table* const* tables, const QE_BatchQuery* pool,
+                                 U32 pool_count, F64 rate_qps, U64 duration_ns, U64 seed, QE_ServerStats* out);
+
 B32 qe_server_drive(QE_Server* s, QE_Table* table, const QE_BatchQuery* pool, U32 pool_count, F64 rate_qps,
                     U64 duration_ns, U64 seed, QE_ServerStats* out)
 {
-  if (!s || !table || !pool || !pool_count || rate_qps <= 0.0) {
+  if (!table) {
+    qe_set_error("qe_server_drive: bad argument");
+    return 0;
+  }
+  return qe_server_drive_impl(s, table, 0, pool, pool_count, rate_qps, duration_ns, seed, out);
+}
+
+/* the same load over a mix of tables: request k runs pool[k % pool_count]
+ * on tables[k % pool_count] */
+B32 qe_server_drive_mix(QE_Server* s, QE_Table* const* tables, const QE_BatchQuery* pool, U32 pool_count,
+                        F64 rate_qps, U64 duration_ns, U64 seed, QE_ServerStats* out)
+{
+  if (!tables) {
+    qe_set_error("qe_server_drive_mix: bad argument");
+    return 0;
+  }
+  for (U32 i = 0; i < pool_count; i++) {
+    if (!tables[i]) {
+      qe_set_error("qe_server_drive_mix: no table for query %u", i);
+      return 0;
+    }
+  }
+  return qe_server_drive_impl(s, 0, tables, pool, pool_count, rate_qps, duration_ns, seed, out);
+}
+
+static B32 qe_server_drive_impl(QE_Server* s, QE_Table* table, QE_Table* const* tables, const QE_BatchQuery* pool,
+                                 U32 pool_count, F64 rate_qps, U64 duration_ns, U64 seed, QE_ServerStats* out)
+{
+  if (!s || !pool || !pool_count || rate_qps <= 0.0) {
     qe_set_error("qe_server_drive: bad argument");
     return 0;
   }
@@ -427,7 +458,7 @@ B32 qe_server_drive(QE_Server* s, QE_Table* table, const QE_BatchQuery* pool, U3
       ok = 0;
       break;
     }
-    slot->ticket = qe_server_submit(s, table, &slot->q);
+    slot->ticket = qe_server_submit(s, tables ? tables[k % pool_count] : table, &slot->q);
     if (!slot->ticket) {
       free(slot->q.out_keys);
       free(slot->q.out_cells);

@@ -215,6 +215,7 @@ SIGNATURES = [
     ("qe_server_stats_reset", None, [P]),
     ("qe_server_stop", None, [P]),
     ("qe_server_drive", B32, [P, P, P, U32, C.c_double, U64, U64, P]),
+    ("qe_server_drive_mix", B32, [P, P, P, U32, C.c_double, U64, U64, P]),
     ("qe_bandwidth_read", B32, [P, U64, U32, P, P]),
     ("qe_bandwidth_copy", B32, [P, P, U64, U32, P]),
 ]

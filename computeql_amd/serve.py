@@ -105,6 +105,25 @@ class Server:
         d["offered_qps"] = rate_qps
         return d
 
+    def drive_mix(self, pairs, rate_qps: float, seconds: float, seed: int = 1, cap: int = 1024) -> dict:
+        """drive() over a mix: pairs is a list of (table, spec); request k
+        runs pairs[k % len(pairs)]."""
+        keep: list = []
+        n = len(pairs)
+        pool = (QE_BatchQuery * n)()
+        tables = (C.c_void_p * n)(*[t.handle for t, _ in pairs])
+        for i, (_, sp) in enumerate(pairs):
+            q, _, _ = _query(sp, keep, cap=cap)
+            pool[i] = q
+        st = QE_ServerStats()
+        ok = self._lib.qe_server_drive_mix(self.handle, tables, pool, n, float(rate_qps), int(seconds * 1e9), int(seed),
+                                           C.byref(st))
+        if not ok:
+            raise _err("qe_server_drive_mix")
+        d = _stats_dict(st)
+        d["offered_qps"] = rate_qps
+        return d
+
     def close(self) -> None:
         if self.handle:
             self._lib.qe_server_stop(self.handle)

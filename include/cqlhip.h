@@ -588,6 +588,10 @@ void qe_server_stop(QE_Server* server);
  * the run (they are reset at its start). */
 B32 qe_server_drive(QE_Server* server, QE_Table* table, const QE_BatchQuery* pool, U32 pool_count, F64 rate_qps,
                     U64 duration_ns, U64 seed, QE_ServerStats* out);
+/* the same load over several tables: request k runs pool[k % pool_count]
+ * on tables[k % pool_count] (a mix of tables and query shapes) */
+B32 qe_server_drive_mix(QE_Server* server, QE_Table* const* tables, const QE_BatchQuery* pool, U32 pool_count,
+                        F64 rate_qps, U64 duration_ns, U64 seed, QE_ServerStats* out);
 
 /* bandwidth probe (bandwidth_read.comp): reads `bytes` of a buffer with
  * 16-byte loads; returns a sink value.  For measuring the HBM ceiling. */

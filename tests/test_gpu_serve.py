@@ -102,3 +102,13 @@ def test_drive_poisson_load(small):
         st = srv.drive(small, specs, rate_qps=3000, seconds=1.0, seed=5)
     assert st["failed"] == 0 and st["completed"] == st["submitted"] > 1000
     assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+
+
+def test_drive_mix_over_two_tables(events, small):
+    t, _ = events
+    pairs = [(t, sp) for sp in wl.mixed_queries(8, seed=5)]
+    pairs += [(small, cq.AggSpec([cq.COUNT, cq.SUM], [None, 2], col(2) < 100 * (i + 1), 1)) for i in range(4)]
+    with Server(workers=2, max_batch=64) as srv:
+        st = srv.drive_mix(pairs, rate_qps=2000, seconds=0.5, seed=3)
+    assert st["failed"] == 0 and st["completed"] == st["submitted"] > 500
+    assert 0 < st["p50_ms"] <= st["p99_ms"]
