@@ -493,6 +493,9 @@ def main(argv=None) -> int:
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, read one column once (the known-size read that scales "
                          "rocprofv3 FETCH_SIZE readings)")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a library setting (qe_settings_set_u64) before anything runs, e.g. "
+                         "QE_BATCH_DEDUPE=0; repeatable; the JSON line's config lists them")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}); this build has no multi-GPU path, "
@@ -502,6 +505,11 @@ def main(argv=None) -> int:
     import computeql_amd as cq
 
     lib = cq.load()
+    settings = {}
+    for kv in a.set:
+        k, _, v = kv.partition("=")
+        settings[k.strip()] = int(v, 0)
+        cq.set_setting(k.strip(), settings[k.strip()])
     if a.only == "batched":
         cq.profile(True)
         batched, _ = batched_and_served(cq, lib, a.served_seconds, served_too=False)
@@ -618,7 +626,8 @@ def main(argv=None) -> int:
         "config": {"workload": f"{rows}-row 4xu32 fact table + u32 fk, dim A 1M rows, dim B 50 rows; q1-q7 per step",
                    "rows": rows, "pct_hbm_peak": round(100 * value / PEAK_GBS, 2),
                    "per_query_wall_ms": {q: round(1e3 * v / a.steps, 3) for q, v in wall.items()},
-                   "kernels": kernels, "invariants": "ok" if not errs else sorted(set(errs))},
+                   "kernels": kernels, "invariants": "ok" if not errs else sorted(set(errs)),
+                   "settings": settings},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "encoded": encoded,

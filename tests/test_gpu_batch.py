@@ -177,3 +177,41 @@ def test_graph_replays_match_eager(events):
     lib.gpu_hip_graph_stats(C.byref(r1), C.byref(c1))
     assert c1.value - c0.value >= 1
     assert r1.value - r0.value >= 3
+
+
+def test_repeated_queries_run_once(events):
+    """A batch of 8 distinct queries each sent 8 times, interleaved: every
+    copy gets the result the distinct batch gives, with identical queries
+    run once and also with that switched off."""
+    t, _ = events
+    distinct = wl.mixed_queries(8, seed=21)
+    base = cq.aggregate_batch(t, distinct)
+    specs = [distinct[i % 8] for i in range(64)]
+    for dedupe in (1, 0):
+        cq.set_setting("QE_BATCH_DEDUPE", dedupe)
+        got = cq.aggregate_batch(t, specs)
+        for i, (sp, r) in enumerate(zip(specs, got)):
+            b = base[i % 8]
+            _same(sp, r.keys, _vals(r), b.keys, _vals(b), f"dedupe={dedupe} q{i}")
+
+
+def test_cost_estimate_picks_plain_or_shared(events):
+    """Launches the estimate prices as cheaper plain run one plain operator
+    per query, equal to calling it alone (f64 sums within 1e-12: the dense
+    GROUP BY adds them with atomics in no fixed order); with the estimate
+    off every query is fused into the shared scan."""
+    t, _ = events
+    specs = wl.mixed_queries(12, seed=23)
+    cq.set_setting("QE_BATCH_COST_SHARED_MROWS", 1)  # shared scan priced far slower than plain
+    plain_b, tr = cq.aggregate_batch(t, specs, with_trace=True)
+    assert tr["fused"] == 0 and tr["fallback"] == 12
+    for i, (sp, r) in enumerate(zip(specs, plain_b)):
+        alone = t.aggregate(sp.funcs, sp.args, where=sp.where, group=sp.group)
+        assert r.trace["fused"] == 0
+        _same(sp, r.keys, _vals(r), alone.keys, _vals(alone), f"plain q{i}")
+    cq.reset_settings()
+    cq.set_setting("QE_BATCH_COST", 0)
+    shared, tr = cq.aggregate_batch(t, specs, with_trace=True)
+    assert tr["fused"] == 12 and tr["fallback"] == 0
+    for i, (sp, a, b) in enumerate(zip(specs, shared, plain_b)):
+        _same(sp, a.keys, _vals(a), b.keys, _vals(b), f"q{i}")
