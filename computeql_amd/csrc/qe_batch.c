@@ -412,6 +412,64 @@ static B32 qe_batch_same(const QE_BatchQuery* a, const QE_BatchQuery* b)
   return memcmp(a->funcs, b->funcs, a->expr_count * 4) == 0 && memcmp(a->arg_cols, b->arg_cols, a->expr_count * 4) == 0;
 }
 
+/* FNV-1a over what qe_batch_same compares */
+static U64 qe_batch_query_hash(const QE_BatchQuery* q)
+{
+  U64 h = 0xcbf29ce484222325ull;
+#define QE_HASH_BYTES(p, n)                                                                                            \
+  do {                                                                                                                 \
+    const U8* b_ = (const U8*)(p);                                                                                     \
+    for (U64 k_ = 0; k_ < (U64)(n); k_++) h = (h ^ b_[k_]) * 0x100000001b3ull;                                         \
+  } while (0)
+  QE_HASH_BYTES(&q->where_count, sizeof(q->where_count));
+  QE_HASH_BYTES(&q->group_col, sizeof(q->group_col));
+  QE_HASH_BYTES(&q->expr_count, sizeof(q->expr_count));
+  QE_HASH_BYTES(&q->cap, sizeof(q->cap));
+  if (q->where_count && q->where) QE_HASH_BYTES(q->where, q->where_count * sizeof(QE_IRNode));
+  if (q->expr_count) {
+    QE_HASH_BYTES(q->funcs, q->expr_count * 4);
+    QE_HASH_BYTES(q->arg_cols, q->expr_count * 4);
+  }
+#undef QE_HASH_BYTES
+  return h;
+}
+
+/* rep_of[i] = the first earlier query identical to query i, or ~0u; an
+ * open-addressing table of first occurrences keyed by the query hash keeps
+ * this linear in the batch size */
+static B32 qe_batch_find_repeats(const QE_BatchQuery* queries, U32 count, U32* rep_of)
+{
+  for (U32 i = 0; i < count; i++) rep_of[i] = ~0u;
+  if (count < 2) return 1;
+  U32 cap = 4;
+  while (cap < 2 * count) cap <<= 1;
+  U32* slot = (U32*)malloc(sizeof(U32) * cap);
+  U64* hash = (U64*)malloc(sizeof(U64) * count);
+  if (!slot || !hash) {
+    free(slot);
+    free(hash);
+    return 0;
+  }
+  for (U32 i = 0; i < cap; i++) slot[i] = ~0u;
+  for (U32 i = 0; i < count; i++) {
+    hash[i] = qe_batch_query_hash(&queries[i]);
+    for (U32 s = (U32)(hash[i] ^ (hash[i] >> 32)) & (cap - 1);; s = (s + 1) & (cap - 1)) {
+      const U32 j = slot[s];
+      if (j == ~0u) {
+        slot[s] = i;
+        break;
+      }
+      if (hash[j] == hash[i] && qe_batch_same(&queries[i], &queries[j])) {
+        rep_of[i] = j;
+        break;
+      }
+    }
+  }
+  free(slot);
+  free(hash);
+  return 1;
+}
+
 static void qe_batch_copy_result(const QE_BatchQuery* from, QE_BatchQuery* to)
 {
   to->ok = from->ok;
@@ -491,11 +549,12 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
     qe_set_error("qe_aggregate_batch: out of memory");
     return 0;
   }
-  const B32 dedupe = qe_settings_u64("QE_BATCH_DEDUPE", 1) != 0;
-  for (U32 i = 0; i < count; i++) {
-    rep_of[i] = ~0u;
-    for (U32 j = 0; dedupe && j < i && rep_of[i] == ~0u; j++)
-      if (rep_of[j] == ~0u && qe_batch_same(&queries[i], &queries[j])) rep_of[i] = j;
+  for (U32 i = 0; i < count; i++) rep_of[i] = ~0u;
+  if (qe_settings_u64("QE_BATCH_DEDUPE", 1) != 0 && !qe_batch_find_repeats(queries, count, rep_of)) {
+    free(rep_of);
+    free(planned);
+    qe_set_error("qe_aggregate_batch: out of memory");
+    return 0;
   }
   for (U32 i = 0; i < count; i++) {
     queries[i].ok = 0;
