@@ -153,6 +153,8 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
   U32 words = hdr + n + 3 * st.count + rec_total + W;
   U32* p = (U32*)calloc(words, 4);
   if (!p) return 0;
+  /* grouped queries walk only their passing rows (QE_BATCH_COMPACT, 1) */
+  const B32 compact = qe_settings_u64("QE_BATCH_COMPACT", 1) != 0;
   U64 stage_bytes = qe_batch_stage_bytes(t, &st, items);
   p[0] = n;
   p[1] = W;
@@ -174,7 +176,8 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
     p[hdr + i] = at;
     U32* r = p + at;
     r[0] = q->leaf_count;
-    r[1] = (q->group_col >= 0 ? 1u : 0u) | (q->plan.need_sum ? 2u : 0u) | (q->plan.need_minmax ? 4u : 0u);
+    r[1] = (q->group_col >= 0 ? 1u : 0u) | (q->plan.need_sum ? 2u : 0u) | (q->plan.need_minmax ? 4u : 0u) |
+           (q->group_col >= 0 && compact ? 8u : 0u);
     r[2] = q->group_col >= 0 ? qe_batch_stage_of(&st, (U32)q->group_col) : 0;
     r[3] = q->group_col >= 0 ? t->columns[q->group_col].type : 0;
     r[4] = q->key_min;

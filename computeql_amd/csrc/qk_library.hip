@@ -2029,7 +2029,10 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /*   [6 + Q + 3u] staged column u: table column, width in bytes (4 or   */
 /*   8), byte offset of its rows in a stage buffer (16-byte aligned)    */
 /* query record: [0] leaves L (<= 8), [1] flags (1 grouped, 2 sums,     */
-/*   4 min/max), [2] key (staged index), [3] key type, [4] key_min code,*/
+/*   4 min/max, 8 grouped rows walked compactly where a wave's rows are */
+/*   sparse: a lane visits only the rows that passed), [2] key (staged  */
+/*   index), [3] key type,                                              */
+/*   [4] key_min code,                                                  */
 /*   [5] groups R (1 when ungrouped), [6] args A (<= 2), [7] arg 0      */
 /*   (staged index), [8] arg type 0, [9] arg 1, [10] arg type 1,        */
 /*   [11] offset of its words among the W, [12] offset of its LDS      */
@@ -2364,6 +2367,45 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
         qa += (qk_u64)(lane & (rec[13] - 1u)) * (R * (1u + 3u * A));
         const qk_u32* kp = (const qk_u32*)(sb + st[3 * rec[2] + 2]);
         const qk_u32 kx = rec[3] == QK_T_I32 ? 0x80000000u : 0u, kmin = rec[4];
+        if ((flags & 8u) && !__any((int)(__popc(m) > (qk_u32)RPL / 4u))) {
+          /* compact, for a wave whose lanes each hold at most RPL / 4
+           * passing rows: each lane walks only those, lowest first, so the
+           * wave loops as often as its busiest lane has rows rather than
+           * RPL times per key, count and argument (with more rows the
+           * unrolled walk below, which overlaps its LDS reads, is faster) */
+          qk_u32 mm = m;
+          while (mm) {
+            const qk_u32 r = (qk_u32)__builtin_ctz(mm) * 64u + lane;
+            mm &= mm - 1u;
+            const qk_u32 gi = (kp[r] ^ kx) - kmin;
+            if (gi >= R) continue;
+            atomicAdd((unsigned long long*)&qa[gi], 1ULL);
+#pragma unroll 1
+            for (qk_u32 k = 0; k < A; k++) {
+              const qk_u32 at = rec[8 + 2 * k];
+              const unsigned char* ap = sb + st[3 * rec[7 + 2 * k] + 2];
+              qk_u64* qs = qa + R + (qk_u64)k * 3 * R;
+              qk_u64 add;
+              double v;
+              if (qk_type_is_int32(at)) {
+                const qk_u32 x = ((const qk_u32*)ap)[r];
+                const bool sgn = at == QK_T_I32;
+                add = sgn ? (qk_u64)(qk_s64)(qk_s32)x : (qk_u64)x;
+                v = sgn ? (double)(qk_s32)x : (double)x;
+                if (flags & 2u) atomicAdd((unsigned long long*)&qs[gi], add);
+              } else {
+                v = qk_ss_f64(ap, at, r);
+                if (flags & 2u) atomicAdd((double*)&qs[gi], v);
+              }
+              if (flags & 4u) {
+                const qk_u64 kk = qk_f64_key(v);
+                atomicMin((unsigned long long*)&qs[R + gi], (unsigned long long)kk);
+                atomicMax((unsigned long long*)&qs[2 * R + gi], (unsigned long long)kk);
+              }
+            }
+          }
+          continue;
+        }
         qk_u32 g[RPL];
 #pragma unroll
         for (int i = 0; i < RPL; i++) {
