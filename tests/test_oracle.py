@@ -219,3 +219,51 @@ def test_int_interval_matches_f64_compare(orc, op, target):
         lo, span = iv
         got = ((vals - lo) & 0xFFFFFFFF) <= span
     assert np.array_equal(got, expect)
+
+
+@pytest.mark.parametrize("workers", [1, 3, 8])
+def test_scan_chunked_count_against_numpy(orc, workers):
+    """The CPU baseline's numeric-AND count (ops 0 ==, 1 <, 2 <=, 3 >, 4 >=)
+    against a numpy mask: small value ranges put many rows on each literal,
+    so a swapped strict/non-strict comparison changes the count; 300k rows
+    split over several tasks."""
+    rng = np.random.default_rng(5)
+    n = 300_007
+    a = rng.integers(0, 20, n).astype(np.uint32)
+    b = rng.integers(0, 7, n).astype(np.float64) * 0.5
+    c = rng.integers(-5, 5, n).astype(np.int64)
+    cols = [a, b, c]
+    npops = [np.equal, np.less, np.less_equal, np.greater, np.greater_equal]
+    cases = [[(0, op, 7.0)] for op in range(5)] + [[(1, op, 1.5)] for op in range(5)]
+    cases += [[(0, 4, 3.0), (1, 1, 2.0), (2, 2, 0.0)], [(2, 0, -3.0), (0, 3, 15.5)], [(0, 1, 0.0)]]
+    for leaves in cases:
+        mask = np.ones(n, dtype=bool)
+        for ci, op, x in leaves:
+            mask &= npops[op](cols[ci].astype(np.float64), x)
+        assert orc.scan_chunked_count(cols, leaves, workers=workers) == int(mask.sum()), leaves
+    assert orc.scan_chunked_count([a[:0]], [(0, 0, 1.0)], workers=workers) == 0
+
+
+@pytest.mark.parametrize("workers", [1, 4])
+def test_dense_group_u32_against_numpy(orc, workers):
+    """The CPU baseline's dense GROUP BY over u32 keys: per-group count, exact
+    u64 sum, min and max against numpy, with empty groups (count 0, min at
+    its identity 0xffffffff, max 0) and without an argument."""
+    rng = np.random.default_rng(6)
+    n, key_min, rng_ = 200_003, 1000, 64
+    key = (key_min + rng.integers(0, rng_ - 4, n)).astype(np.uint32)  # the last 4 groups stay empty
+    arg = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    cnt, s, mn, mx = orc.dense_group_u32(key, arg, key_min, rng_, workers=workers)
+    g = (key - key_min).astype(np.int64)
+    ecnt = np.bincount(g, minlength=rng_).astype(np.uint64)
+    esum = np.zeros(rng_, dtype=np.uint64)
+    np.add.at(esum, g, arg.astype(np.uint64))
+    emin = np.full(rng_, 0xFFFFFFFF, dtype=np.uint32)
+    np.minimum.at(emin, g, arg)
+    emax = np.zeros(rng_, dtype=np.uint32)
+    np.maximum.at(emax, g, arg)
+    assert np.array_equal(cnt, ecnt) and np.array_equal(s, esum)
+    assert np.array_equal(mn, emin) and np.array_equal(mx, emax)
+    cnt0, s0, mn0, mx0 = orc.dense_group_u32(key, None, key_min, rng_, workers=workers)
+    assert np.array_equal(cnt0, ecnt) and not s0.any() and not mx0.any()
+    assert np.array_equal(mn0, np.where(ecnt > 0, 0, 0xFFFFFFFF).astype(np.uint32))
