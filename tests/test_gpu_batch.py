@@ -215,3 +215,43 @@ def test_cost_estimate_picks_plain_or_shared(events):
     assert tr["fused"] == 12 and tr["fallback"] == 0
     for i, (sp, a, b) in enumerate(zip(specs, shared, plain_b)):
         _same(sp, a.keys, _vals(a), b.keys, _vals(b), f"q{i}")
+
+
+def test_i64_leaves_compare_like_f64():
+    """i64 WHERE leaves in the shared scan compare as f64, like the
+    reference's CPU path: every count equals numpy's f64 comparison and the
+    plain operator, at values around +-2^53 (where the conversion rounds)
+    and +-2^62, for every comparison, with integral, fractional and
+    unrepresentable-as-i64-exact literals."""
+    rng = np.random.default_rng(31)
+    big = 1 << 53
+    edge = np.array([-(1 << 62), -big - 1, -big, -big + 1, -1, 0, 1, 12345, big - 2, big - 1, big, big + 1,
+                     big + 2, 1 << 62], dtype=np.int64)
+    n = 50_000
+    x = np.concatenate([np.repeat(edge, 100), rng.integers(-(1 << 54), 1 << 54, n - 100 * len(edge))])
+    t = cq.Table(n)
+    t.add_column("x", cq.I64, x)
+    t.add_column("g", cq.U32, (np.arange(n) % 7).astype(np.uint32))
+    lits = [0, -1, 12345, big - 1, -(big - 1), big - 2, float(big), -float(big), 2.5, -7.5, float(big + 2)]
+    ops = [lambda c: col(0) == c, lambda c: col(0) < c, lambda c: col(0) <= c, lambda c: col(0) > c,
+           lambda c: col(0) >= c]
+    npops = [np.equal, np.less, np.less_equal, np.greater, np.greater_equal]
+    specs, expect = [], []
+    for c in lits:
+        for k in range(5):
+            specs.append(cq.AggSpec([cq.COUNT], [None], ops[k](c), 1 if k % 2 else None))
+            expect.append(npops[k](x.astype(np.float64), float(c)))
+    cq.set_setting("QE_BATCH_COST", 0)
+    res, tr = cq.aggregate_batch(t, specs, with_trace=True)
+    assert tr["fused"] == len(specs)
+    g = np.arange(n) % 7
+    for i, (sp, r, mask) in enumerate(zip(specs, res, expect)):
+        plain = t.aggregate(sp.funcs, sp.args, where=sp.where, group=sp.group)
+        if sp.group is None:
+            assert int(r.cells[0, 0]) == int(mask.sum()) == int(plain.cells[0, 0]), i
+        else:
+            want = np.bincount(g[mask], minlength=7)
+            got = dict(zip(r.keys.astype(np.int64), r.cells[:, 0].astype(np.int64)))
+            assert {k: v for k, v in got.items() if v} == {k: int(v) for k, v in enumerate(want) if v}, i
+            _same(sp, r.keys, _vals(r), plain.keys, _vals(plain), f"plain q{i}")
+    t.close()
