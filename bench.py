@@ -270,7 +270,7 @@ def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True
     from computeql_amd.serve import Server
 
     log(f"batched/served: building the {EVENT_ROWS}-row 30-column event table")
-    ev = wl.upload(wl.event_table(EVENT_ROWS, seed=7))
+    ev = wl.upload(wl.event_table(EVENT_ROWS, seed=7), wl.event_nulls(EVENT_ROWS, seed=7))  # 2% NULL timestamps
     specs = wl.mixed_queries(64, seed=11)
     for _ in range(2):
         for sp in specs:
@@ -294,7 +294,7 @@ def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True
     if not scan_ns:
         scan_ns, _ = profile_stats(lib, "graph:*")
         timed = "graph launches (shared_scan + shared_reduce)"
-    batched = {"workload": f"{EVENT_ROWS}-row 30-column event table, 64 mixed filter/GROUP BY aggregates",
+    batched = {"workload": f"{EVENT_ROWS}-row 30-column event table (2% NULLs in t0-t4), 64 mixed filter/GROUP BY aggregates",
                "plain_ms_per_batch": round(plain_s * 1e3, 3), "batched_ms_per_batch": round(batch_s * 1e3, 3),
                "plain_qps": round(64 / plain_s, 1), "batched_qps": round(64 / batch_s, 1),
                "speedup": round(plain_s / batch_s, 2), "launches": tr["launches"],

@@ -68,13 +68,12 @@ static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlan
   if (!q->funcs || !q->arg_cols || !q->out_group_count || !q->out_cells) return 0;
   if (!qe_agg_plan(t, q->funcs, q->arg_cols, q->expr_count, &out->plan)) return 0;
   if (out->plan.arg_count > 2) return 0;
-  /* a query touching a column with NULLs goes to qe_aggregate, which refuses it */
+  /* a nullable argument or key goes to qe_aggregate, which refuses it as
+   * the reference does (query_exec.c:10312-10333); nullable columns in
+   * the WHERE leaves run here, a NULL row failing its comparison */
   for (U32 a = 0; a < out->plan.arg_count; a++)
     if (t->columns[out->plan.arg_col[a]].has_nulls) return 0;
   if (q->group_col >= 0 && (U32)q->group_col < t->column_count && t->columns[q->group_col].has_nulls) return 0;
-  for (U32 i = 0; q->where && i < q->where_count; i++)
-    if (q->where[i].type == QE_IR_COLUMN && q->where[i].column < t->column_count && t->columns[q->where[i].column].has_nulls)
-      return 0;
   if (q->where && q->where_count) {
     static __thread QE_Bytecode prog;
     if (!qe_bytecode_program_build(t, q->where, q->where_count, &prog)) return 0;
@@ -191,8 +190,11 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
     r[12] = q->lds_off;
     r[13] = q->reps;
     for (U32 l = 0; l < q->leaf_count; l++)
+    {
       qe_simple_leaf_words(&r[QE_BATCH_REC_WORDS + 6 * l], qe_batch_stage_of(&st, q->leaves[l].column),
                            t->columns[q->leaves[l].column].type, q->leaves[l].op, q->leaves[l].target);
+      qe_leaf_nulls(&r[QE_BATCH_REC_WORDS + 6 * l], t, q->leaves[l].column);
+    }
     at += QE_BATCH_REC_WORDS + 6 * q->leaf_count;
     /* op of every accumulator word: count add; per arg sum (integer or
      * f64 add), min, max */

@@ -2039,7 +2039,7 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /*   words, [13] replicas (a power of two; lane l of the owning wave    */
 /*   adds into replica l mod replicas, the block folds them at its end),*/
 /*   [14..] leaves, 6 words each (qe_simple_leaf_words layout, slot =   */
-/*   staged index).                                                     */
+/*   staged index; word 5 a nullable column's bitmap offset / 16).      */
 /* Columns: binding 2 holds the base address of every table column.     */
 /* A block walks tiles of 256 x ITEMS rows.  The rows of every staged   */
 /* column of a tile are copied straight from HBM into an LDS stage      */
@@ -2353,6 +2353,22 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
           double x[RPL];
           qk_ss_rows<RPL>(cp, type, lane, x);
           m = qk_ss_cmp<RPL>(x, op, tv, m);
+        }
+        if (w[5]) {
+          /* a nullable column: its NULL rows fail the comparison.  The
+           * bitmap (bit r of word r / 32, 16 * w[5] bytes into the column)
+           * gives row i * 64 + lane of the tile at bit lane mod 32 of word
+           * base / 32 + 2 i + lane / 32; words past the last row are not
+           * read (those rows are already out of m). */
+          const qk_u32* nb = (const qk_u32*)(cols[st[3 * u]] + (qk_u64)w[5] * 16);
+          const qk_u64 w0 = (base >> 5) + (lane >> 5), wlast = (rows - 1) >> 5;
+          qk_u32 nm = 0;
+#pragma unroll
+          for (int i = 0; i < RPL; i++) {
+            const qk_u64 wi = w0 + 2 * (qk_u64)i;
+            nm |= ((wi <= wlast ? nb[wi] : 0u) >> (lane & 31u) & 1u) << i;
+          }
+          m &= ~nm;
         }
       }
       const qk_u32 R = rec[5], A = rec[6] < QK_SS_MAX_ARGS ? rec[6] : QK_SS_MAX_ARGS;

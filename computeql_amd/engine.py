@@ -115,9 +115,10 @@ class Table:
             pass
 
     def add_column(self, name: str, ctype: int, data: np.ndarray | None = None, nulls: np.ndarray | None = None) -> int:
-        """nulls: a byte per row, nonzero = NULL (cql.h:283); queries that
-        touch such a column are refused (NULL-aware evaluation is not on
-        this path), the table's other columns stay usable."""
+        """nulls: a byte per row, nonzero = NULL (cql.h:283).  WHEREs that are
+        an AND of comparisons and IS [NOT] NULL tests evaluate such a column
+        (a NULL row fails its comparison); other uses of it (a key, an
+        argument, OR/IN shapes, joins, gathers) are refused."""
         ptr = None
         if data is not None:
             data = np.ascontiguousarray(data, dtype=_NP[ctype])

@@ -8,8 +8,9 @@ event_table(rows, seed): the 30-column event table (host arrays, seeded):
     4-11  m0..m7 f64  lognormal(mu=3, sigma=1)
     12-21 t0..t9 i64  timestamps (t0-t4) and counters (t5-t9)
     22-29 u0..u7 u32  uniform over [0, 1e6)
-NULLs are not generated: the operators here take columns without NULL
-flags.
+event_nulls(rows, seed, frac) flags NULL rows of the nullable columns, the
+timestamps t0-t4 (frac of each, 2% by default); the mixed queries read them
+only in their WHERE, where a NULL row fails its comparison.
 
 mixed_queries(n, seed, rows): n filter / GROUP BY aggregates over that table
 with literals drawn from the seed; shapes cycle through the list below.
@@ -43,10 +44,19 @@ def event_table(rows: int, seed: int = 7) -> list:
     return cols
 
 
-def upload(cols: list) -> Table:
+NULLABLE = tuple(range(T0, T0 + 5))
+
+
+def event_nulls(rows: int, seed: int = 7, frac: float = 0.02) -> dict:
+    """{column: byte-per-row flags, nonzero = NULL} for the nullable columns."""
+    rng = np.random.default_rng(seed ^ 0x5EED)
+    return {c: (rng.random(rows) < frac).astype(np.uint8) for c in NULLABLE}
+
+
+def upload(cols: list, nulls: dict | None = None) -> Table:
     t = Table(len(cols[0]))
-    for (name, ty), c in zip(EVENT_COLUMNS, cols):
-        t.add_column(name, ty, c)
+    for i, ((name, ty), c) in enumerate(zip(EVENT_COLUMNS, cols)):
+        t.add_column(name, ty, c, nulls=None if nulls is None else nulls.get(i))
     return t
 
 

@@ -255,3 +255,27 @@ def test_i64_leaves_compare_like_f64():
             assert {k: v for k, v in got.items() if v} == {k: int(v) for k, v in enumerate(want) if v}, i
             _same(sp, r.keys, _vals(r), plain.keys, _vals(plain), f"plain q{i}")
     t.close()
+
+
+def test_mixed_batch_with_nullable_timestamps(orc):
+    """The 64-query mix over the event table with 2% NULLs in t0-t4: their
+    WHERE leaves run in the shared scan with NULL rows failing, and every
+    query equals the plain operator and the oracle's NULL-aware path."""
+    rows = 300_007
+    cols = wl.event_table(rows, seed=17)
+    nulls = wl.event_nulls(rows, seed=17)
+    t = wl.upload(cols, nulls)
+    specs = wl.mixed_queries(64, seed=11)
+    cq.set_setting("QE_BATCH_COST", 0)
+    res, tr = cq.aggregate_batch(t, specs, with_trace=True)
+    assert tr["fused"] == 64
+    hit = 0
+    for i, (sp, r) in enumerate(zip(specs, res)):
+        plain = t.aggregate(sp.funcs, sp.args, where=sp.where, group=sp.group)
+        _same(sp, r.keys, _vals(r), plain.keys, _vals(plain), f"plain q{i}")
+        if i % 8 == 2:  # the shape whose leaves read t0-t4
+            rk, rv = orc.group_aggregate(cols, sp.group, sp.funcs, sp.args, where=sp.where, nulls=nulls)
+            _same(sp, r.keys, _vals(r), rk, rv, f"oracle q{i}")
+            hit += 1
+    assert hit == 8
+    t.close()
