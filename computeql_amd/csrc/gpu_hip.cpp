@@ -140,6 +140,8 @@ struct HipState {
   std::map<std::string, std::pair<U64, U64>> profile_table; /* ns, dispatches */
   U64 fault_alloc_at = 0;
   U64 fault_alloc_counter = 0;
+  U64 fault_read_at = 0;
+  U64 fault_read_counter = 0;
   U64 (*evict_hook)(U64) = nullptr;
 };
 
@@ -510,7 +512,12 @@ GPU_Buffer* gpu_hip_buffer_alloc_pooled(String8 name, U64 size, GPU_BufferFlags 
   }
   if (hit) {
     hit->size = size;
-    if (data && size) check(copy_sync(hit->ptr, data, size, hipMemcpyHostToDevice), "pooled copy");
+    if (data && size && !check(copy_sync(hit->ptr, data, size, hipMemcpyHostToDevice), "pooled copy")) {
+      /* back to the pool, and no buffer for the caller */
+      std::lock_guard<std::mutex> g(s.lock);
+      s.pool.emplace(key, hit);
+      return nullptr;
+    }
     hit->pooled = true;
     hit->pool_name = key;
     return hit;
@@ -569,20 +576,66 @@ void gpu_hip_buffer_release(GPU_Buffer* buffer)
   delete buffer;
 }
 
+/* fault injection (gpu_hip_fault_read_at): true when this device-to-host
+ * read is the one set to fail */
+static bool fault_read_now()
+{
+  HipState& s = st();
+  std::lock_guard<std::mutex> g(s.lock);
+  if (!s.fault_read_at) return false;
+  if (++s.fault_read_counter != s.fault_read_at) return false;
+  s.fault_read_at = 0;
+  return true;
+}
+
+B32 gpu_hip_buffer_write_checked(GPU_Buffer* buffer, const void* data, U64 size)
+{
+  if (!buffer || (!data && size)) {
+    set_error("buffer_write: null buffer or data");
+    return 0;
+  }
+  if (size > buffer->alloc_size) {
+    set_error("buffer_write: %lld bytes exceed the buffer", "", (long long)size);
+    return 0;
+  }
+  if (!size) return 1;
+  /* complete before returning: batch streams are not ordered against the
+   * stream this copy runs on */
+  return check(copy_sync(buffer->ptr, data, size, hipMemcpyHostToDevice), "buffer_write") ? 1 : 0;
+}
+
+B32 gpu_hip_buffer_read_checked(GPU_Buffer* buffer, void* data, U64 size)
+{
+  if (!buffer || (!data && size)) {
+    set_error("buffer_read: null buffer or data");
+    return 0;
+  }
+  if (size > buffer->alloc_size) {
+    set_error("buffer_read: %lld bytes exceed the buffer", "", (long long)size);
+    return 0;
+  }
+  if (!size) return 1;
+  if (fault_read_now()) {
+    set_error("buffer_read: injected read failure");
+    return 0;
+  }
+  return check(copy_sync(data, buffer->ptr, size, hipMemcpyDeviceToHost), "buffer_read") ? 1 : 0;
+}
+
 void gpu_hip_buffer_write(GPU_Buffer* buffer, void* data, U64 size)
 {
   if (!buffer || !data) return;
   if (size > buffer->alloc_size) size = buffer->alloc_size;
-  /* complete before returning: batch streams are not ordered against the
-   * stream this copy runs on */
-  check(copy_sync(buffer->ptr, data, size, hipMemcpyHostToDevice), "buffer_write");
+  gpu_hip_buffer_write_checked(buffer, data, size);
 }
 
+/* the reference's contract: void, and zeros on failure (gpu_vulkan.c:
+ * 3107-3111); the operators use gpu_hip_buffer_read_checked */
 void gpu_hip_buffer_read(GPU_Buffer* buffer, void* data, U64 size)
 {
   if (!buffer || !data) return;
   if (size > buffer->alloc_size) size = buffer->alloc_size;
-  if (!check(copy_sync(data, buffer->ptr, size, hipMemcpyDeviceToHost), "buffer_read")) memset(data, 0, size);
+  if (!gpu_hip_buffer_read_checked(buffer, data, size)) memset(data, 0, size);
 }
 
 GPU_Kernel* gpu_hip_kernel_alloc(String8 name)
@@ -960,8 +1013,19 @@ void gpu_hip_batch_kernel_execute(GPU_Batch* batch, GPU_Kernel* kernel, U32 glob
 
 void gpu_hip_batch_buffer_read(GPU_Batch* batch, GPU_Buffer* buffer, void* out_data, U64 size)
 {
-  if (!batch || !buffer || !out_data) return;
-  if (size > buffer->alloc_size) size = buffer->alloc_size;
+  if (!batch) return;
+  if (!buffer || !out_data || size > buffer->alloc_size) {
+    /* a read that cannot be made fails the batch (its caller must not
+     * take the unwritten bytes for a result) */
+    set_error("batch_buffer_read: missing buffer or %lld bytes past it", "", (long long)size);
+    batch->failed = true;
+    return;
+  }
+  if (fault_read_now()) {
+    set_error("batch_buffer_read: injected read failure");
+    batch->failed = true;
+    return;
+  }
   if (batch->deferred) {
     char* p = staging_take(batch, size);
     if (!p) {
@@ -1324,6 +1388,13 @@ void gpu_hip_fault_alloc_at(U64 k)
   std::lock_guard<std::mutex> g(st().lock);
   st().fault_alloc_at = k;
   st().fault_alloc_counter = 0;
+}
+
+void gpu_hip_fault_read_at(U64 k)
+{
+  std::lock_guard<std::mutex> g(st().lock);
+  st().fault_read_at = k;
+  st().fault_read_counter = 0;
 }
 
 const char* gpu_hip_last_error(void) { return t_last_error; }

@@ -218,6 +218,59 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
   return 1;
 }
 
+/* ---------------- GROUP BY keys ---------------- */
+B32 qe_group_keys(const QE_Table* t, const S32* cols, U32 count, QE_GroupKeys* out)
+{
+  memset(out, 0, sizeof(*out));
+  if (count > QE_AGG_MAX_GROUP_COLS || (count && !cols)) {
+    qe_set_error("aggregate: at most %d GROUP BY columns", QE_AGG_MAX_GROUP_COLS);
+    return 0;
+  }
+  out->count = count;
+  for (U32 k = 0; k < count; k++) {
+    if (cols[k] < 0 || (U32)cols[k] >= t->column_count) {
+      qe_set_error("aggregate: unknown group column %d", cols[k]);
+      return 0;
+    }
+    const QE_Column* c = &t->columns[cols[k]];
+    out->col[k] = (U32)cols[k];
+    out->type[k] = c->type;
+    if (qe_type_int32(c->type) && c->zonemap_valid && c->max_value >= c->min_value) {
+      out->min_code[k] = c->type == CQL_ColumnType_I32 ? ((U32)(S32)c->min_value ^ 0x80000000u) : (U32)c->min_value;
+      out->range[k] = (U64)(c->max_value - c->min_value) + 1;
+    }
+  }
+  return 1;
+}
+
+U64 qe_group_keys_span(const QE_GroupKeys* k)
+{
+  U64 span = 1;
+  for (U32 i = 0; i < k->count; i++) {
+    if (!k->range[i]) return 0;
+    if (span > ~0ull / k->range[i]) return ~0ull;
+    span *= k->range[i];
+  }
+  return span;
+}
+
+F64 qe_group_key_value(const QE_GroupKeys* k, U32 i, U64 d)
+{
+  U32 code = k->min_code[i] + (U32)d;
+  return k->type[i] == CQL_ColumnType_I32 ? (F64)(S32)(code ^ 0x80000000u) : (F64)code;
+}
+
+/* the keys' values of combined group number g (digits, key 0 most
+ * significant) into out[0..K) */
+static void qe_group_digits(const QE_GroupKeys* k, U64 g, F64* out)
+{
+  for (U32 i = k->count; i-- > 0;) {
+    U64 d = g % k->range[i];
+    g /= k->range[i];
+    if (out) out[i] = qe_group_key_value(k, i, d);
+  }
+}
+
 /* ---------------- dense GROUP BY ---------------- */
 /* per-group device totals of aggregate_dense / join_probe: count u64[R];
  * sum u64[E][R]; min u32[E][R]; max u32[E][R] */
@@ -238,13 +291,20 @@ static B32 qe_dense_totals_alloc(QE_DenseTotals* d, U32 range, U32 args)
 
 U64 qe_dense_totals_bytes(U32 range, U32 args) { return (U64)range * 8 * (1 + args) + (U64)range * 4 * 2 * args; }
 
-/* runs aggregate_dense (key_col of `t`, args) or join_probe; fills totals */
-B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B32 need_sum, B32 need_minmax,
-                 const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune, U32 key_min_code, U32 range,
-                 QE_DenseTotals* out, QE_Trace* trace)
+/* runs aggregate_dense over the keys of `keys` (columns of `t`) and args;
+ * fills totals indexed by the keys' combined digits */
+B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32 arg_count, B32 need_sum,
+                 B32 need_minmax, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune, QE_DenseTotals* out,
+                 QE_Trace* trace)
 {
   GPU_Kernel* k = qe_kernel("aggregate_dense");
   if (!k) return 0;
+  const U64 span = qe_group_keys_span(keys);
+  if (keys->count < 1 || keys->count > QE_DENSE_MAX_KEYS || span == 0 || span > 0xffffffffull) {
+    qe_set_error("dense group by: 1..%d 32-bit integer keys with known ranges", QE_DENSE_MAX_KEYS);
+    return 0;
+  }
+  const U32 range = (U32)span;
   U32 flags = (need_sum ? 1u : 0u) | (need_minmax ? 2u : 0u);
   U64 lds = (U64)range * 4 + ((flags & 1u) ? (U64)arg_count * range * 8 : 0) + ((flags & 2u) ? (U64)arg_count * range * 8 : 0);
   U32 max_lds = gpu_hip_max_lds_bytes();
@@ -254,10 +314,21 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
   }
   QE_ColumnSlots slots;
   memset(&slots, 0, sizeof(slots));
-  U32 key_slot = qe_slot_of(&slots, key_col);
-  U32 slot_bits = 0, type_bits = 0;
   /* encoded columns (a3) are read as codes: type byte U32 | encoding << 4,
-   * the key's minimum rewritten to codes, the base added back below */
+   * the keys' minima rewritten to codes, the base added back below */
+  U32 kslot[QE_DENSE_MAX_KEYS], kbyte[QE_DENSE_MAX_KEYS], kmin[QE_DENSE_MAX_KEYS];
+  for (U32 i = 0; i < keys->count; i++) {
+    const U32 col = keys->col[i];
+    kslot[i] = qe_slot_of(&slots, col);
+    const U32 enc = qe_col_enc(t, col);
+    kbyte[i] = t->columns[col].type;
+    kmin[i] = keys->min_code[i];
+    if (enc) {
+      kbyte[i] = CQL_ColumnType_U32 | (enc << 4);
+      kmin[i] = keys->min_code[i] - t->columns[col].enc_base - (t->columns[col].type == CQL_ColumnType_I32 ? 0x80000000u : 0u);
+    }
+  }
+  U32 slot_bits = 0, type_bits = 0;
   for (U32 a = 0; a < arg_count; a++) {
     U32 s = qe_slot_of(&slots, arg_cols[a]);
     U32 ty = t->columns[arg_cols[a]].type;
@@ -280,33 +351,42 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
     qe_leaf_nulls(&leaf_words[6 * l], t, leaves[l].column);
     qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(t, leaves[l].column), t->columns[leaves[l].column].enc_base);
   }
-  const U32 key_enc = qe_col_enc(t, key_col);
-  U32 key_byte = t->columns[key_col].type, kmin = key_min_code;
-  if (key_enc) {
-    key_byte = CQL_ColumnType_U32 | (key_enc << 4);
-    kmin = key_min_code - t->columns[key_col].enc_base - (t->columns[key_col].type == CQL_ColumnType_I32 ? 0x80000000u : 0u);
+  for (U32 i = 0; i < keys->count; i++)
+    if (kslot[i] == ~0u) {
+      qe_set_error("dense group by: too many columns");
+      return 0;
+    }
+  /* several keys: their descriptors (u32 count, then slot | type << 8,
+   * code minimum, range per key) in binding 2 */
+  U32 kdesc[1 + 3 * QE_DENSE_MAX_KEYS];
+  kdesc[0] = keys->count;
+  for (U32 i = 0; i < keys->count; i++) {
+    kdesc[1 + 3 * i] = kslot[i] | (kbyte[i] << 8);
+    kdesc[2 + 3 * i] = kmin[i];
+    kdesc[3 + 3 * i] = (U32)keys->range[i];
   }
   if (!qe_dense_totals_alloc(out, range, arg_count)) return 0;
   U64 r0 = prune->row_begin, r1 = prune->row_end;
   U64 quads = (r1 + 3) / 4 - r0 / 4;
   /* one LDS table per block: fewer, fuller blocks when the table is big */
   U32 blocks = qe_stream_blocks(quads);
-  U32 per_cu_lds = blocks;
   U64 cus = gpu_hip_compute_units();
   U64 lds_blocks = lds ? (160ull * 1024 / (lds + 1024)) : 8;
   if (lds_blocks < 1) lds_blocks = 1;
   if (lds_blocks > qe_setting_blocks_per_cu()) lds_blocks = qe_setting_blocks_per_cu();
   if ((U64)blocks > cus * lds_blocks) blocks = (U32)(cus * lds_blocks);
-  (void)per_cu_lds;
   U64 gbytes = qe_dense_totals_bytes(range, arg_count);
   GPU_Buffer* acc = qe_scratch("dense_totals", gbytes);
   GPU_Buffer* leaf_buf = leaf_count ? qe_scratch("dense_leaves", sizeof(leaf_words)) : 0;
   GPU_Buffer* skip_buf = prune->skip_bits ? qe_scratch("dense_skip", prune->skip_words * 4) : 0;
-  B32 ok = acc && (!leaf_count || leaf_buf) && (!prune->skip_bits || skip_buf);
+  GPU_Buffer* key_buf = keys->count > 1 ? qe_scratch("dense_keys", sizeof(kdesc)) : 0;
+  B32 ok = acc && (!leaf_count || leaf_buf) && (!prune->skip_bits || skip_buf) && (keys->count == 1 || key_buf);
   if (ok) {
     if (leaf_buf) gpu_kernel_set_arg_buffer(k, 0, leaf_buf);
     else gpu_hip_kernel_clear_arg(k, 0);
     gpu_kernel_set_arg_buffer(k, 1, acc);
+    if (key_buf) gpu_kernel_set_arg_buffer(k, 2, key_buf);
+    else gpu_hip_kernel_clear_arg(k, 2);
     if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
     else gpu_hip_kernel_clear_arg(k, 5);
     for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
@@ -318,15 +398,16 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
     gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
     gpu_kernel_set_arg_u64(k, 5, r0);
     gpu_kernel_set_arg_u64(k, 7, leaf_count);
-    qe_kernel_set_word(k, QK_W_KEY_MIN, kmin);
+    qe_kernel_set_word(k, QK_W_KEY_MIN, kmin[0]);
     qe_kernel_set_word(k, QK_W_RANGE, range);
     qe_kernel_set_word(k, QK_W_ARGS, arg_count | (slot_bits << 8));
     qe_kernel_set_word(k, QK_W_TYPES, type_bits);
-    qe_kernel_set_word(k, QK_W_KEY, key_slot | (key_byte << 8) | (flags << 16));
+    qe_kernel_set_word(k, QK_W_KEY, kslot[0] | (kbyte[0] << 8) | (flags << 16));
     gpu_hip_kernel_set_lds_bytes(k, (U32)lds);
     GPU_Batch* b = gpu_batch_begin(0, gbytes);
     if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
     if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
+    if (key_buf) gpu_batch_buffer_write(b, key_buf, kdesc, 4 * (1 + 3 * (U64)keys->count));
     /* count/sum/max start at 0, min at all ones: fills run from offset 0,
      * so zero all, fill through the min array, zero count+sum again */
     U64 zero_bytes = (U64)range * 8 * (1 + arg_count);
@@ -339,10 +420,12 @@ B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B
     gpu_batch_buffer_read(b, acc, out->block, gbytes);
     ok = gpu_batch_end(b);
     gpu_hip_kernel_set_lds_bytes(k, 0);
+    gpu_hip_kernel_clear_arg(k, 2);
   }
   qe_scratch_release(acc);
   qe_scratch_release(leaf_buf);
   qe_scratch_release(skip_buf);
+  qe_scratch_release(key_buf);
   if (!ok) {
     free(out->block);
     out->block = 0;
@@ -400,21 +483,35 @@ static F64 qe_f64_from_order_code(U64 key)
   return d;
 }
 
-/* GROUP BY over the global hash table of aggregate_hash (key_col < 0: one
+/* how aggregate_hash forms a row's group code from the keys: 0 one key (its
+ * own code), 1 digits of 32-bit integer keys, 2 a hash of the key tuple */
+static U32 qe_hash_key_mode(const QE_GroupKeys* keys)
+{
+  if (keys->count <= 1) return 0;
+  const U64 span = qe_group_keys_span(keys);
+  return (span != 0 && span < (1ull << 63)) ? 1u : 2u;
+}
+
+/* GROUP BY over the global hash table of aggregate_hash (no keys: one
  * group).  Rows: the fused leaves over the pruned range, or `list` (device
  * row ids) when given.  Groups come out in table-slot order. */
-static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune,
-                             GPU_Buffer* list, U64 list_count, const QE_AggPlan* p, U64 cap, U64* out_group_count,
-                             F64* out_keys, U64* out_cells, QE_Trace* trace)
+static B32 qe_aggregate_hash(QE_Table* t, const QE_GroupKeys* keys, const QE_Leaf* leaves, U32 leaf_count,
+                             const QE_Prune* prune, GPU_Buffer* list, U64 list_count, const QE_AggPlan* p, U64 cap,
+                             U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace)
 {
   GPU_Kernel* k = qe_kernel("aggregate_hash");
   GPU_Kernel* kc = qe_kernel("hash_compact");
   if (!k || !kc) return 0;
   const U32 E = p->arg_count;
+  const U32 K = keys->count;
+  const U32 mode = qe_hash_key_mode(keys);
+  const U32 T = mode == 2 ? K : 0u;
   QE_ColumnSlots slots;
   memset(&slots, 0, sizeof(slots));
-  U32 key_slot = key_col >= 0 ? qe_slot_of(&slots, (U32)key_col) : 0;
-  U32 key_type = key_col >= 0 ? t->columns[key_col].type : CQL_ColumnType_U32;
+  U32 kslot[QE_AGG_MAX_GROUP_COLS];
+  for (U32 i = 0; i < K; i++) kslot[i] = qe_slot_of(&slots, keys->col[i]);
+  U32 key_slot = K ? kslot[0] : 0;
+  U32 key_type = K ? keys->type[0] : CQL_ColumnType_U32;
   U32 slot_bits = 0, type_bits = 0;
   for (U32 a = 0; a < E; a++) {
     slot_bits |= qe_slot_of(&slots, p->arg_col[a]) << (4 * a);
@@ -430,19 +527,31 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
     qe_simple_leaf_words(&leaf_words[6 * l], sl, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
     qe_leaf_nulls(&leaf_words[6 * l], t, leaves[l].column);
   }
-  if (slots.count > QE_MAX_COLUMN_BINDINGS || key_slot == ~0u) {
+  B32 slots_ok = slots.count <= QE_MAX_COLUMN_BINDINGS;
+  for (U32 i = 0; i < K; i++) slots_ok = slots_ok && kslot[i] != ~0u;
+  if (!slots_ok) {
     qe_set_error("hash group by: too many columns");
     return 0;
   }
+  /* several keys: mode, K, per key slot | type << 8, code minimum, range,
+   * then the tuple hash's seed */
+  U32 kdesc[3 + 3 * QE_AGG_MAX_GROUP_COLS];
+  kdesc[0] = mode;
+  kdesc[1] = K;
+  for (U32 i = 0; i < K; i++) {
+    kdesc[2 + 3 * i] = kslot[i] | (keys->type[i] << 8);
+    kdesc[3 + 3 * i] = keys->min_code[i];
+    kdesc[4 + 3 * i] = (U32)keys->range[i];
+  }
   U64 r0 = prune->row_begin, r1 = prune->row_end;
   U64 rows = list ? list_count : r1 - r0;
-  /* table size: twice the groups there can be (rows, or the key's value
-   * span from the zone map), grown and rerun if it fills */
+  /* table size: twice the groups there can be (rows, or the keys' value
+   * span from the zone maps), grown and rerun if it fills */
   U64 est = rows ? rows : 1;
-  if (key_col < 0) est = 1;
-  else if (qe_type_int32(key_type) && t->columns[key_col].zonemap_valid) {
-    F64 span = t->columns[key_col].max_value - t->columns[key_col].min_value + 1.0;
-    if (span >= 1.0 && span < (F64)est) est = (U64)span;
+  if (K == 0) est = 1;
+  else {
+    U64 span = qe_group_keys_span(keys);
+    if (span && span < est) est = span;
   }
   U32 log2 = 10;
   while (log2 < 28 && (1ull << log2) < 2 * est) log2++;
@@ -451,8 +560,9 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
   U32 L = (U32)qe_settings_u64("QE_AGG_HASH_LDS_SLOTS", 512);
   if (L & (L - 1)) L = 512;
   if (L > 4096) L = 4096;
+  if (mode == 2) L = 0; /* tuple keys skip the LDS stage */
   U64 lds = (U64)L * (8 + 24 * (U64)E + 4);
-  U32 flags = (p->need_sum ? 1u : 0u) | (p->need_minmax ? 2u : 0u) | (key_col < 0 ? 4u : 0u);
+  U32 flags = (p->need_sum ? 1u : 0u) | (p->need_minmax ? 2u : 0u) | (K == 0 ? 4u : 0u);
   U64 cus = gpu_hip_compute_units();
   U32 blocks;
   if (list) {
@@ -461,17 +571,19 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
   } else {
     blocks = qe_stream_blocks((r1 + 3) / 4 - r0 / 4);
   }
-  U32 rec_words = 2 + 3 * E;
+  U32 rec_words = 2 + 3 * E + 2 * T;
   GPU_Buffer* leaf_buf = leaf_count ? qe_scratch("hash_leaves", sizeof(leaf_words)) : 0;
   GPU_Buffer* skip_buf = (!list && prune->skip_bits) ? qe_scratch("hash_skip", prune->skip_words * 4) : 0;
   GPU_Buffer* flag_buf = qe_scratch("hash_flags", 64);
-  B32 ok = (!leaf_count || leaf_buf) && (list || !prune->skip_bits || skip_buf) && flag_buf;
+  GPU_Buffer* key_buf = mode ? qe_scratch("hash_keys", sizeof(kdesc)) : 0;
+  B32 ok = (!leaf_count || leaf_buf) && (list || !prune->skip_bits || skip_buf) && flag_buf && (!mode || key_buf);
   U64 groups = 0;
   U64* recs = 0;
   B32 err_set = 0;
-  for (U32 attempt = 0; ok && attempt < 4; attempt++, log2++) {
+  U32 seed = 0;
+  for (U32 attempt = 0; ok && attempt < 6; attempt++) {
     U64 C = (1ull << log2) + 1;
-    U64 table_bytes = C * 8 * (2 + 3 * (U64)E);
+    U64 table_bytes = C * 8 * (2 + 3 * (U64)E + 2 * (U64)T);
     U32 cblocks = (U32)((C + 4095) / 4096);
     if (cblocks > 1024) cblocks = 1024;
     GPU_Buffer* table = qe_scratch("hash_table", table_bytes);
@@ -484,12 +596,15 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
       ok = 0;
       break;
     }
+    kdesc[2 + 3 * K] = seed;
     if (leaf_buf) gpu_kernel_set_arg_buffer(k, 0, leaf_buf);
     else gpu_hip_kernel_clear_arg(k, 0);
     gpu_kernel_set_arg_buffer(k, 1, table);
     gpu_kernel_set_arg_buffer(k, 2, flag_buf);
     if (list) gpu_kernel_set_arg_buffer(k, 3, list);
     else gpu_hip_kernel_clear_arg(k, 3);
+    if (key_buf) gpu_kernel_set_arg_buffer(k, 4, key_buf);
+    else gpu_hip_kernel_clear_arg(k, 4);
     if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
     else gpu_hip_kernel_clear_arg(k, 5);
     for (U32 sl = 0; sl < QE_MAX_COLUMN_BINDINGS; sl++) {
@@ -510,7 +625,7 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
     gpu_kernel_set_arg_buffer(kc, 1, table);
     gpu_kernel_set_arg_buffer(kc, 2, bc);
     gpu_kernel_set_arg_buffer(kc, 3, out);
-    qe_kernel_set_word(kc, QK_W_ARGS, E);
+    qe_kernel_set_word(kc, QK_W_ARGS, E | (T << 8));
     qe_kernel_set_word(kc, QK_W_LOG2, log2);
     U32 fl[2] = {0, 0};
     U64* bcounts = (U64*)malloc((U64)cblocks * 8);
@@ -520,11 +635,12 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
       GPU_Batch* b = gpu_batch_begin(0, (U64)cblocks * 8 + 8);
       if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
       if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
+      if (key_buf) gpu_batch_buffer_write(b, key_buf, kdesc, 4 * (3 + 3 * (U64)K));
       gpu_batch_buffer_zero(b, flag_buf, 8);
-      /* keys and mins start at all ones: zero everything, then fill the
-       * leading key + min arrays */
+      /* keys, mins and key minima start at all ones: zero everything, then
+       * fill the leading key + min + key-min arrays */
       gpu_batch_buffer_zero(b, table, table_bytes);
-      gpu_batch_buffer_fill(b, table, C * 8 * (1 + (U64)E), 0xffffffffu);
+      gpu_batch_buffer_fill(b, table, C * 8 * (1 + (U64)E + (U64)T), 0xffffffffu);
       gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
       gpu_kernel_set_arg_u64(kc, 6, 1); /* push slot 6: mode, block counts */
       gpu_batch_kernel_execute(b, kc, cblocks * QE_WG, QE_WG);
@@ -534,12 +650,22 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
       gpu_batch_buffer_read(b, bc, bcounts, (U64)cblocks * 8);
       ok = gpu_batch_end(b);
       gpu_hip_kernel_set_lds_bytes(k, 0);
+      gpu_hip_kernel_clear_arg(k, 4);
     }
     B32 full = ok && fl[0] != 0;
+    if (ok && fl[1]) {
+      qe_set_error("hash group by: a key lies outside its column's zone-map range");
+      err_set = 1;
+      ok = 0;
+    }
+    B32 collided = 0;
     if (ok && !full) {
       groups = 0;
       for (U32 i = 0; i < cblocks; i++) groups += bcounts[i];
-      U64 take = groups < cap ? groups : cap;
+      /* tuple keys read every group back to see that no two keys shared a
+       * code; otherwise at most cap groups */
+      U64 take = (T || groups < cap) ? groups : cap;
+      free(recs);
       recs = (U64*)malloc((take ? take : 1) * 8 * rec_words);
       if (!recs) {
         ok = 0;
@@ -550,14 +676,23 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
         gpu_batch_buffer_read(rb, out, recs, take * 8 * rec_words);
         ok = gpu_batch_end(rb);
       }
+      for (U64 g = 0; ok && T && g < take && !collided; g++) {
+        const U64* r = recs + g * rec_words;
+        for (U32 i = 0; i < T; i++)
+          if (r[2 + 3 * E + i] != r[2 + 3 * E + T + i]) collided = 1;
+      }
     }
     free(bcounts);
     qe_scratch_release(table);
     qe_scratch_release(bc);
     qe_scratch_release(out);
-    if (!full) break;
-    if (attempt == 3) {
-      qe_set_error("hash group by: table of 2^%u slots full", log2);
+    if (!ok) break;
+    if (!full && !collided) break;
+    if (full) log2++;
+    if (collided) seed++;
+    if (attempt == 5 || log2 > 30) {
+      qe_set_error(full ? "hash group by: table of 2^%u slots full" : "hash group by: key hash collisions at 2^%u slots",
+                   log2);
       err_set = 1;
       ok = 0;
     }
@@ -565,6 +700,7 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
   qe_scratch_release(leaf_buf);
   qe_scratch_release(skip_buf);
   qe_scratch_release(flag_buf);
+  qe_scratch_release(key_buf);
   if (!ok) {
     free(recs);
     if (!err_set) qe_set_error("aggregate_hash: %s", gpu_hip_last_error());
@@ -574,7 +710,16 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
   const B32 raw = qe_t_raw_int_sums;
   for (U64 g = 0; g < take; g++) {
     const U64* r = recs + g * rec_words;
-    if (out_keys) out_keys[g] = key_col >= 0 ? qe_hash_key_to_f64(key_type, r[0]) : 0.0;
+    if (out_keys) {
+      F64* kv = out_keys + g * K;
+      if (mode == 0) {
+        if (K) kv[0] = qe_hash_key_to_f64(key_type, r[0]);
+      } else if (mode == 1) {
+        qe_group_digits(keys, r[0], kv);
+      } else {
+        for (U32 i = 0; i < K; i++) kv[i] = qe_hash_key_to_f64(keys->type[i], r[2 + 3 * E + i]);
+      }
+    }
     U64 cnt = r[1];
     for (U32 e = 0; e < p->expr_count; e++) {
       S32 a = p->expr_arg[e];
@@ -610,24 +755,27 @@ static B32 qe_aggregate_hash(QE_Table* t, S32 key_col, const QE_Leaf* leaves, U3
 }
 
 /* ---------------- entry point ---------------- */
-B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 group_col, const U32* funcs,
-                 const S32* arg_cols, U32 expr_count, U64 cap, U64* out_group_count, F64* out_keys, U64* out_cells,
-                 QE_Trace* trace)
+static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_count, const S32* group_cols,
+                            U32 group_count, const U32* funcs, const S32* arg_cols, U32 expr_count, U64 cap,
+                            U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace)
 {
   U64 t0, p0;
   qe_trace_begin(trace, &t0, &p0);
-  if (!table || !funcs || !arg_cols || !out_group_count || !out_cells) {
+  if (!table || !funcs || !arg_cols || !out_group_count || !out_cells || (group_count && !group_cols)) {
     qe_set_error("qe_aggregate: null argument");
     return 0;
   }
-  if (!qe_where_non_null(table, where, where_count, "qe_aggregate") ||
-      !qe_require_non_null(table, group_col, "qe_aggregate"))
-    return 0;
+  QE_GroupKeys keys;
+  if (!qe_group_keys(table, group_cols, group_count, &keys)) return 0;
+  const U32 K = keys.count;
+  if (!qe_where_non_null(table, where, where_count, "qe_aggregate")) return 0;
+  for (U32 i = 0; i < K; i++)
+    if (!qe_require_non_null(table, (S32)keys.col[i], "qe_aggregate")) return 0;
   for (U32 e = 0; e < expr_count; e++)
     if (!qe_require_non_null(table, arg_cols[e], "qe_aggregate")) return 0;
   if (table->host_backed)
-    return qe_stream_aggregate(table, where, where_count, group_col, funcs, arg_cols, expr_count, cap, out_group_count,
-                               out_keys, out_cells, trace);
+    return qe_stream_aggregate(table, where, where_count, group_cols, group_count, funcs, arg_cols, expr_count, cap,
+                               out_group_count, out_keys, out_cells, trace);
   QE_AggPlan plan;
   if (!qe_agg_plan(table, funcs, arg_cols, expr_count, &plan)) return 0;
   if (trace) trace->rows_before = table->row_count;
@@ -650,10 +798,6 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
       leaf_count = 0;
     }
   }
-  if (group_col >= 0 && (U32)group_col >= table->column_count) {
-    qe_set_error("qe_aggregate: unknown group column");
-    return 0;
-  }
   if (general_where) {
     GPU_Buffer* ids = 0;
     U64 n = 0;
@@ -663,16 +807,14 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
     all.row_end = table->row_count;
     U64 groups = 0;
     B32 ok = 1;
-    if (n) ok = qe_aggregate_hash(table, group_col, 0, 0, &all, ids, n, &plan, cap, &groups, out_keys, out_cells, trace);
+    if (n) ok = qe_aggregate_hash(table, &keys, 0, 0, &all, ids, n, &plan, cap, &groups, out_keys, out_cells, trace);
     qe_scratch_release(ids);
     if (!ok) return 0;
-    if (group_col < 0 && groups == 0) {
+    if (K == 0 && groups == 0) {
       /* an ungrouped aggregate over no rows is one row of zeros
        * (query_exec.c:10335-10346) */
-      if (cap >= 1) {
+      if (cap >= 1)
         for (U32 e = 0; e < expr_count; e++) out_cells[e] = qe_agg_cell(plan.funcs[e], 0, 0.0, 0.0, 0.0);
-        if (out_keys) out_keys[0] = 0.0;
-      }
       groups = 1;
     }
     *out_group_count = groups;
@@ -689,7 +831,7 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
   qe_zonemap_prune(table, prune_leaves, prune_count, &prune);
 
   B32 ok = 1;
-  if (group_col < 0) {
+  if (K == 0) {
     if (cap < 1) {
       *out_group_count = 1;
       qe_prune_free(&prune);
@@ -701,45 +843,47 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
     } else {
       ok = qe_aggregate_global(table, leaves, leaf_count, &prune, &plan, out_cells, trace);
     }
-    if (ok) {
-      *out_group_count = 1;
-      if (out_keys) out_keys[0] = 0.0;
-    }
+    if (ok) *out_group_count = 1;
     qe_prune_free(&prune);
     qe_trace_end(trace, t0, p0);
     return ok;
   }
 
-  QE_Column* key = &table->columns[group_col];
   if (prune.zones_left == 0 || table->row_count == 0) {
     qe_prune_free(&prune);
     *out_group_count = 0;
     qe_trace_end(trace, t0, p0);
     return 1;
   }
-  /* dense LDS tables when the key is a 32-bit integer whose range (from the
-   * zone map, query_exec.c:7636-7647) is small and the args are 32-bit
-   * integers; the hash table otherwise */
-  F64 kmin = key->min_value, kmax = key->max_value;
-  F64 span = kmax - kmin + 1.0;
+  /* dense LDS tables when every key is a 32-bit integer whose range (from
+   * the zone map, query_exec.c:7636-7647) is small, at most three keys
+   * whose ranges multiply to a small range (digits, query_exec.c:9462-9500),
+   * and the args are 32-bit integers; the hash table otherwise */
+  const U64 span = qe_group_keys_span(&keys);
   U64 max_range = qe_settings_u64("QE_AGG_DENSE_MAX_RANGE", 65536);
-  B32 dense = qe_settings_u64("QE_AGG_DENSE", 1) != 0 && qe_type_int32(key->type) && key->zonemap_valid &&
-              span <= (F64)max_range;
+  B32 dense = qe_settings_u64("QE_AGG_DENSE", 1) != 0 && K <= QE_DENSE_MAX_KEYS && span != 0 && span <= max_range;
   for (U32 a = 0; a < plan.arg_count && dense; a++)
     if (!qe_type_int32(table->columns[plan.arg_col[a]].type)) dense = 0;
+  /* args that are key columns fold into the counts: SUM = key * count,
+   * MIN = MAX = key (exact for whole-number keys) */
+  S32 arg_key[QE_AGG_MAX_ARGS];
+  B32 fold = qe_settings_u64("QE_AGG_KEY_FOLD", 1) != 0;
+  for (U32 a = 0; a < plan.arg_count; a++) {
+    arg_key[a] = -1;
+    for (U32 i = 0; i < K; i++)
+      if (plan.arg_col[a] == keys.col[i]) arg_key[a] = (S32)i;
+    if (arg_key[a] < 0) fold = 0;
+  }
   /* the block's LDS tables must fit: counts u32[R], sums u64[E][R],
-   * min/max u32[E][R] (args equal to the key fold into the counts) */
+   * min/max u32[E][R] */
   if (dense) {
-    B32 all_key = 1;
-    for (U32 a = 0; a < plan.arg_count; a++)
-      if (plan.arg_col[a] != (U32)group_col) all_key = 0;
-    const U64 e = (all_key && qe_settings_u64("QE_AGG_KEY_FOLD", 1)) ? 0 : plan.arg_count;
-    const U64 lds = (U64)span * 4 + (plan.need_sum ? (U64)span * 8 * e : 0) + (plan.need_minmax ? (U64)span * 8 * e : 0);
+    const U64 e = fold ? 0 : plan.arg_count;
+    const U64 lds = span * 4 + (plan.need_sum ? span * 8 * e : 0) + (plan.need_minmax ? span * 8 * e : 0);
     if (lds > gpu_hip_max_lds_bytes()) dense = 0;
   }
   if (!dense) {
     U64 groups = 0;
-    ok = qe_aggregate_hash(table, group_col, leaves, leaf_count, &prune, 0, 0, &plan, cap, &groups, out_keys, out_cells,
+    ok = qe_aggregate_hash(table, &keys, leaves, leaf_count, &prune, 0, 0, &plan, cap, &groups, out_keys, out_cells,
                            trace);
     qe_prune_free(&prune);
     if (!ok) return 0;
@@ -747,41 +891,36 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
     qe_trace_end(trace, t0, p0);
     return 1;
   }
-  U32 range = (U32)span;
-  U32 key_min_code = key->type == CQL_ColumnType_I32 ? ((U32)(S32)kmin ^ 0x80000000u) : (U32)kmin;
-
-  /* args equal to the key fold into the count: SUM = key*count,
-   * MIN = MAX = key (exact for whole-number keys) */
-  B32 fold = qe_settings_u64("QE_AGG_KEY_FOLD", 1) != 0;
-  for (U32 a = 0; a < plan.arg_count && fold; a++)
-    if (plan.arg_col[a] != (U32)group_col) fold = 0;
+  const U32 range = (U32)span;
   QE_DenseTotals tot;
   memset(&tot, 0, sizeof(tot));
-  if (fold) ok = qe_dense_run(table, (U32)group_col, 0, 0, 0, 0, leaves, leaf_count, &prune, key_min_code, range, &tot, trace);
+  if (fold) ok = qe_dense_run(table, &keys, 0, 0, 0, 0, leaves, leaf_count, &prune, &tot, trace);
   else
-    ok = qe_dense_run(table, (U32)group_col, plan.arg_col, plan.arg_count, plan.need_sum, plan.need_minmax, leaves,
-                      leaf_count, &prune, key_min_code, range, &tot, trace);
+    ok = qe_dense_run(table, &keys, plan.arg_col, plan.arg_count, plan.need_sum, plan.need_minmax, leaves, leaf_count,
+                      &prune, &tot, trace);
   qe_prune_free(&prune);
   if (!ok) return 0;
   /* groups in key order (query_exec.c:8006-8039) */
   U64 g = 0;
   const B32 raw = qe_t_raw_int_sums;
+  F64 kv[QE_AGG_MAX_GROUP_COLS];
   for (U32 i = 0; i < range; i++) {
     U64 cnt = tot.count[i];
     if (!cnt) continue;
     if (g < cap) {
-      U32 code = key_min_code + i;
-      F64 kv = key->type == CQL_ColumnType_I32 ? (F64)(S32)(code ^ 0x80000000u) : (F64)code;
-      if (out_keys) out_keys[g] = kv;
+      qe_group_digits(&keys, i, kv);
+      if (out_keys)
+        for (U32 k = 0; k < K; k++) out_keys[g * K + k] = kv[k];
       for (U32 e = 0; e < expr_count; e++) {
         S32 a = plan.expr_arg[e];
         F64 sum = 0.0, mn = 0.0, mx = 0.0;
         S64 isum = 0;
         if (a >= 0) {
           if (fold) {
-            isum = (S64)kv * (S64)cnt;
-            sum = kv * (F64)cnt;
-            mn = mx = kv;
+            const F64 v = kv[arg_key[a]];
+            isum = (S64)v * (S64)cnt;
+            sum = v * (F64)cnt;
+            mn = mx = v;
           } else {
             U32 ty = table->columns[plan.arg_col[a]].type;
             U64 s = tot.sum[(U64)a * range + i];
@@ -807,4 +946,19 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 g
   }
   qe_trace_end(trace, t0, p0);
   return 1;
+}
+
+B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, const S32* group_cols, U32 group_count,
+                 const U32* funcs, const S32* arg_cols, U32 expr_count, U64 cap, U64* out_group_count, F64* out_keys,
+                 U64* out_cells, QE_Trace* trace)
+{
+  B32 ok = qe_aggregate_run(table, where, where_count, group_cols, group_count, funcs, arg_cols, expr_count, cap,
+                            out_group_count, out_keys, out_cells, trace);
+  /* chunk partials of a streamed aggregate carry raw integer sums; the
+   * streamed operator checks its merged result instead */
+  if (ok && !qe_t_raw_int_sums) {
+    const U64 rows = *out_group_count < cap ? *out_group_count : cap;
+    ok = qe_check_sum_exact(table, funcs, arg_cols, expr_count, out_cells, rows);
+  }
+  return ok;
 }

@@ -73,7 +73,10 @@ static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlan
    * the WHERE leaves run here, a NULL row failing its comparison */
   for (U32 a = 0; a < out->plan.arg_count; a++)
     if (t->columns[out->plan.arg_col[a]].has_nulls) return 0;
-  if (q->group_col >= 0 && (U32)q->group_col < t->column_count && t->columns[q->group_col].has_nulls) return 0;
+  /* several GROUP BY keys run on qe_aggregate */
+  if (q->group_count > 1) return 0;
+  const S32 group_col = q->group_count ? q->group_cols[0] : -1;
+  if (group_col >= 0 && (U32)group_col < t->column_count && t->columns[group_col].has_nulls) return 0;
   if (q->where && q->where_count) {
     static __thread QE_Bytecode prog;
     if (!qe_bytecode_program_build(t, q->where, q->where_count, &prog)) return 0;
@@ -82,11 +85,11 @@ static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlan
     for (U32 l = 0; l < out->leaf_count; l++)
       if (out->leaves[l].op >= QE_LEAF_IS_NULL) return 0;
   }
-  out->group_col = q->group_col;
+  out->group_col = group_col;
   out->range = 1;
-  if (q->group_col >= 0) {
-    if ((U32)q->group_col >= t->column_count) return 0;
-    QE_Column* k = &t->columns[q->group_col];
+  if (group_col >= 0) {
+    if ((U32)group_col >= t->column_count) return 0;
+    QE_Column* k = &t->columns[group_col];
     if (!qe_batch_int32(k->type) || !k->zonemap_valid || t->row_count == 0) return 0;
     F64 span = k->max_value - k->min_value + 1.0;
     if (span < 1.0 || span > (F64)qe_settings_u64("QE_BATCH_MAX_GROUPS", 1024)) return 0;
@@ -94,7 +97,7 @@ static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlan
     out->key_min = k->type == CQL_ColumnType_I32 ? ((U32)(S32)k->min_value ^ 0x80000000u) : (U32)k->min_value;
   }
   out->acc_words = out->range * (1 + 3 * out->plan.arg_count);
-  out->reps = q->group_col >= 0 ? qe_batch_reps(out->range, out->acc_words) : 1;
+  out->reps = group_col >= 0 ? qe_batch_reps(out->range, out->acc_words) : 1;
   out->lds_words = out->acc_words * out->reps;
   return 1;
 }
@@ -228,13 +231,9 @@ static void qe_batch_decode(QE_Table* t, const QE_BatchPlanned* q, const QE_Batc
     U64 cnt = a[i];
     if (q->group_col >= 0 && !cnt) continue;
     if (g < bq->cap) {
-      if (bq->out_keys) {
-        F64 kv = 0.0;
-        if (q->group_col >= 0) {
-          U32 code = q->key_min + i;
-          kv = t->columns[q->group_col].type == CQL_ColumnType_I32 ? (F64)(S32)(code ^ 0x80000000u) : (F64)code;
-        }
-        bq->out_keys[g] = kv;
+      if (bq->out_keys && q->group_col >= 0) {
+        U32 code = q->key_min + i;
+        bq->out_keys[g] = t->columns[q->group_col].type == CQL_ColumnType_I32 ? (F64)(S32)(code ^ 0x80000000u) : (F64)code;
       }
       for (U32 e = 0; e < p->expr_count; e++) {
         S32 ai = p->expr_arg[e];
@@ -367,7 +366,9 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
     for (U32 i = 0; i < n; i++) {
       QE_BatchQuery* bq = &user[qs[i].index];
       qe_batch_decode(t, &qs[i], bq, acc);
-      bq->ok = 1;
+      /* a SUM reaching 2^53 fails this query (query_exec.c:7028-7049) */
+      const U64 g = *bq->out_group_count;
+      bq->ok = qe_check_sum_exact(t, bq->funcs, bq->arg_cols, bq->expr_count, bq->out_cells, g < bq->cap ? g : bq->cap);
       bq->fused = 1;
     }
     if (trace) {
@@ -406,12 +407,14 @@ static void qe_batch_config_from_settings(QE_BatchConfig* c)
   c->lds_budget = qe_settings_u64("QE_BATCH_LDS_BYTES", 0);
 }
 
-/* the same query: WHERE nodes, key, functions, arguments and capacity */
+/* the same query: WHERE nodes, keys, functions, arguments, capacity and
+ * whether it asks for its keys */
 static B32 qe_batch_same(const QE_BatchQuery* a, const QE_BatchQuery* b)
 {
-  if (a->where_count != b->where_count || a->group_col != b->group_col || a->expr_count != b->expr_count ||
-      a->cap != b->cap)
+  if (a->where_count != b->where_count || a->group_count != b->group_count || a->expr_count != b->expr_count ||
+      a->cap != b->cap || (a->out_keys != 0) != (b->out_keys != 0) || a->group_count > QE_AGG_MAX_GROUP_COLS)
     return 0;
+  if (memcmp(a->group_cols, b->group_cols, a->group_count * sizeof(S32))) return 0;
   if (a->where_count && (!a->where || !b->where || memcmp(a->where, b->where, a->where_count * sizeof(QE_IRNode))))
     return 0;
   return memcmp(a->funcs, b->funcs, a->expr_count * 4) == 0 && memcmp(a->arg_cols, b->arg_cols, a->expr_count * 4) == 0;
@@ -427,7 +430,10 @@ static U64 qe_batch_query_hash(const QE_BatchQuery* q)
     for (U64 k_ = 0; k_ < (U64)(n); k_++) h = (h ^ b_[k_]) * 0x100000001b3ull;                                         \
   } while (0)
   QE_HASH_BYTES(&q->where_count, sizeof(q->where_count));
-  QE_HASH_BYTES(&q->group_col, sizeof(q->group_col));
+  const U32 want_keys = q->out_keys != 0;
+  QE_HASH_BYTES(&want_keys, sizeof(want_keys));
+  QE_HASH_BYTES(&q->group_count, sizeof(q->group_count));
+  if (q->group_count <= QE_AGG_MAX_GROUP_COLS) QE_HASH_BYTES(q->group_cols, q->group_count * sizeof(S32));
   QE_HASH_BYTES(&q->expr_count, sizeof(q->expr_count));
   QE_HASH_BYTES(&q->cap, sizeof(q->cap));
   if (q->where_count && q->where) QE_HASH_BYTES(q->where, q->where_count * sizeof(QE_IRNode));
@@ -483,7 +489,7 @@ static void qe_batch_copy_result(const QE_BatchQuery* from, QE_BatchQuery* to)
   U64 g = *from->out_group_count;
   *to->out_group_count = g;
   U64 n = g < to->cap ? g : to->cap;
-  if (to->out_keys && from->out_keys) memcpy(to->out_keys, from->out_keys, n * sizeof(F64));
+  if (to->out_keys && from->out_keys) memcpy(to->out_keys, from->out_keys, n * to->group_count * sizeof(F64));
   memcpy(to->out_cells, from->out_cells, n * to->expr_count * sizeof(U64));
 }
 
@@ -577,7 +583,8 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
     } else {
       /* not shaped for the shared scan: the plain operator, same call */
       QE_BatchQuery* q = &queries[i];
-      q->ok = qe_aggregate(table, q->where, q->where_count, q->group_col, q->funcs, q->arg_cols, q->expr_count, q->cap,
+      q->ok = qe_aggregate(table, q->where, q->where_count, q->group_cols, q->group_count, q->funcs, q->arg_cols,
+                           q->expr_count, q->cap,
                            q->out_group_count, q->out_keys, q->out_cells, 0);
       if (!q->ok) all_ok = 0;
       fallback++;
@@ -658,7 +665,8 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
     } else if (qe_batch_plain_cheaper(table, planned + start, end - start)) {
       for (U32 i = start; i < end; i++) {
         QE_BatchQuery* q = &queries[planned[i].index];
-        q->ok = qe_aggregate(table, q->where, q->where_count, q->group_col, q->funcs, q->arg_cols, q->expr_count, q->cap,
+        q->ok = qe_aggregate(table, q->where, q->where_count, q->group_cols, q->group_count, q->funcs, q->arg_cols,
+                           q->expr_count, q->cap,
                              q->out_group_count, q->out_keys, q->out_cells, 0);
         q->fused = 0;
         if (!q->ok) all_ok = 0;
@@ -672,6 +680,8 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
   }
   for (U32 i = 0; i < count; i++)
     if (rep_of[i] != ~0u) qe_batch_copy_result(&queries[rep_of[i]], &queries[i]);
+  for (U32 i = 0; i < count; i++)
+    if (!queries[i].ok) all_ok = 0;
   free(rep_of);
   free(planned);
   if (trace) {

@@ -145,6 +145,41 @@ void qe_scratch_release(GPU_Buffer* b)
   if (b) gpu_buffer_release(b);
 }
 
+B32 qe_buffer_read_checked(GPU_Buffer* b, void* out, U64 size)
+{
+  if (gpu_hip_buffer_read_checked(b, out, size)) return 1;
+  qe_set_error("device read failed: %s", gpu_hip_last_error());
+  return 0;
+}
+
+B32 qe_buffer_write_checked(GPU_Buffer* b, const void* data, U64 size)
+{
+  if (gpu_hip_buffer_write_checked(b, data, size)) return 1;
+  qe_set_error("device write failed: %s", gpu_hip_last_error());
+  return 0;
+}
+
+B32 qe_check_sum_exact(const QE_Table* t, const U32* funcs, const S32* arg_cols, U32 expr_count, const U64* cells,
+                       U64 rows)
+{
+  if (qe_settings_u64("QE_WIDE_INT_CHECK", 1) == 0) return 1;
+  for (U32 e = 0; e < expr_count; e++) {
+    if (funcs[e] != QE_AGG_FUNC_SUM || arg_cols[e] < 0 || (U32)arg_cols[e] >= t->column_count) continue;
+    const U32 ty = t->columns[arg_cols[e]].type;
+    if (ty != CQL_ColumnType_U32 && ty != CQL_ColumnType_I32 && ty != CQL_ColumnType_U64 && ty != CQL_ColumnType_I64)
+      continue;
+    for (U64 g = 0; g < rows; g++) {
+      F64 v;
+      memcpy(&v, &cells[g * expr_count + e], 8);
+      if (v >= 9007199254740992.0 || v <= -9007199254740992.0) {
+        qe_set_error("SUM(%s) reached 2^53, which the engine adds as an inexact double", t->columns[arg_cols[e]].name);
+        return 0;
+      }
+    }
+  }
+  return 1;
+}
+
 U32 qe_setting_blocks_per_cu(void) { return (U32)qe_settings_u64("QE_HIP_BLOCKS_PER_CU", 4); }
 
 U32 qe_stream_blocks(U64 quads)
@@ -237,7 +272,11 @@ S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void*
                  gpu_hip_last_error());
     return -1;
   }
-  if (host_data && table->row_count) gpu_buffer_write(col->buffer, (void*)host_data, table->row_count * width);
+  if (host_data && table->row_count && !qe_buffer_write_checked(col->buffer, host_data, table->row_count * width)) {
+    gpu_buffer_release(col->buffer);
+    memset(col, 0, sizeof(*col));
+    return -1;
+  }
   if (has_nulls) {
     U32* bits = (U32*)calloc(bitmap_words + 4, 4);
     GPU_Buffer* view = bits ? gpu_hip_buffer_view(col->buffer, col->null_offset, bitmap_words * 4 + 16) : 0;
@@ -250,9 +289,14 @@ S32 qe_table_add_column(QE_Table* table, const char* name, U32 type, const void*
     }
     for (U64 i = 0; i < table->row_count; i++)
       if (null_flags[i]) bits[i >> 5] |= 1u << (i & 31);
-    gpu_buffer_write(view, bits, bitmap_words * 4 + 16);
+    const B32 wrote = qe_buffer_write_checked(view, bits, bitmap_words * 4 + 16);
     gpu_buffer_release(view);
     free(bits);
+    if (!wrote) {
+      gpu_buffer_release(col->buffer);
+      memset(col, 0, sizeof(*col));
+      return -1;
+    }
   }
   S32 index = (S32)table->column_count++;
   if (host_data && !qe_table_build_zonemap(table, (U32)index)) return -1;
@@ -314,18 +358,16 @@ B32 qe_table_column_read(QE_Table* table, U32 column, U64 row0, U64 rows, void* 
     memcpy(out, (const U8*)col->host_data + row0 * col->width, rows * col->width);
     return 1;
   }
-  if (row0 == 0) {
-    gpu_buffer_read(col->buffer, out, rows * col->width);
-    return 1;
+  if (row0 == 0) return qe_buffer_read_checked(col->buffer, out, rows * col->width);
+  /* the rows [row0, row0 + rows) as a view of the column */
+  GPU_Buffer* view = gpu_hip_buffer_view(col->buffer, row0 * col->width, rows * col->width);
+  if (!view) {
+    qe_set_error("qe_table_column_read: bad range");
+    return 0;
   }
-  /* read the prefix then keep the tail (buffers are read from offset 0) */
-  U64 bytes = (row0 + rows) * col->width;
-  U8* tmp = (U8*)malloc(bytes ? bytes : 1);
-  if (!tmp) return 0;
-  gpu_buffer_read(col->buffer, tmp, bytes);
-  memcpy(out, tmp + row0 * col->width, rows * col->width);
-  free(tmp);
-  return 1;
+  const B32 ok = qe_buffer_read_checked(view, out, rows * col->width);
+  gpu_buffer_release(view);
+  return ok;
 }
 
 B32 qe_table_build_zonemap(QE_Table* table, U32 column)
@@ -457,7 +499,7 @@ S32 qe_table_encode_column(QE_Table* table, U32 column, B32 allow_sequence)
   /* a sequence: value = first value + row, the last one in the type's range */
   if (allow_sequence && qe_settings_u64("QE_ENCODE_SEQUENCE", 1) && rows < (1ull << 32)) {
     U32 first = 0;
-    gpu_buffer_read(col->buffer, &first, 4);
+    if (!qe_buffer_read_checked(col->buffer, &first, 4)) return -1;
     B32 fits = col->type == CQL_ColumnType_I32 ? (S64)(S32)first + (S64)rows - 1 <= 2147483647ll
                                                : (U64)first + rows - 1 <= 0xffffffffull;
     GPU_Kernel* k = qe_kernel("seq_check");
@@ -608,13 +650,14 @@ B32 qe_gen_dim(QE_Table* table, U64 seed, U32 attr_range, B32 spread_keys, S32 k
   }
   if (spread_keys)
     for (U64 i = 0; i < n; i++) keys[i] = keys[i] * 0x85EBCA77u + 0x165667B1u;
-  gpu_buffer_write(table->columns[key_col].buffer, keys, n * 4);
-  if (attr_col >= 0) {
+  B32 wrote = qe_buffer_write_checked(table->columns[key_col].buffer, keys, n * 4);
+  if (wrote && attr_col >= 0) {
     U64 as = seed ^ 0x14057B7EF767814FULL;
     for (U64 i = 0; i < n; i++) keys[i] = (U32)(qe_splitmix64_at(as, i) % attr_range);
-    gpu_buffer_write(table->columns[attr_col].buffer, keys, n * 4);
+    wrote = qe_buffer_write_checked(table->columns[attr_col].buffer, keys, n * 4);
   }
   free(keys);
+  if (!wrote) return 0;
   if (!qe_table_build_zonemap(table, (U32)key_col)) return 0;
   return attr_col < 0 || qe_table_build_zonemap(table, (U32)attr_col);
 }

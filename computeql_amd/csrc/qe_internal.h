@@ -74,9 +74,9 @@ B32 qe_where_non_null(const QE_Table* t, const QE_IRNode* where, U32 count, cons
 /* ---- streamed execution over host-backed tables (qe_stream.c) ---- */
 B32 qe_stream_scan(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 count_only, U64* out_ids, U64 cap,
                    U64* out_count, QE_Trace* trace);
-B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 group_col, const U32* funcs,
-                        const S32* arg_cols, U32 expr_count, U64 cap, U64* out_group_count, F64* out_keys,
-                        U64* out_cells, QE_Trace* trace);
+B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, const S32* group_cols,
+                        U32 group_count, const U32* funcs, const S32* arg_cols, U32 expr_count, U64 cap,
+                        U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace);
 /* drop the cached chunks of a column (its table is being released) */
 void qe_resident_drop_column(U64 uid);
 
@@ -84,8 +84,6 @@ void qe_resident_drop_column(U64 uid);
 GPU_Backend* qe_gpu(void);
 #define gpu_buffer_alloc(s, f, d) (qe_gpu()->buffer_alloc((s), (f), (d)))
 #define gpu_buffer_release(b) (qe_gpu()->buffer_release((b)))
-#define gpu_buffer_read(b, d, s) (qe_gpu()->buffer_read((b), (d), (s)))
-#define gpu_buffer_write(b, d, s) (qe_gpu()->buffer_write((b), (d), (s)))
 #define gpu_kernel_alloc(n) (qe_gpu()->kernel_alloc((n)))
 #define gpu_kernel_alloc_from_spirv(n, c) (qe_gpu()->kernel_alloc_from_spirv((n), (c)))
 #define gpu_kernel_set_arg_buffer(k, i, b) (qe_gpu()->kernel_set_arg_buffer((k), (i), (b)))
@@ -97,6 +95,11 @@ GPU_Backend* qe_gpu(void);
 #define gpu_batch_kernel_execute(b, k, g, l) (qe_gpu()->batch_kernel_execute((b), (k), (g), (l)))
 #define gpu_batch_buffer_read(b, buf, d, s) (qe_gpu()->batch_buffer_read((b), (buf), (d), (s)))
 #define gpu_batch_end(b) (qe_gpu()->batch_end((b)))
+
+/* synchronous copies that fail the operator (0, qe_last_error()) when the
+ * copy fails; the backend's buffer_read zero-fills instead */
+B32 qe_buffer_read_checked(GPU_Buffer* b, void* out, U64 size);
+B32 qe_buffer_write_checked(GPU_Buffer* b, const void* data, U64 size);
 
 String8 qe_str8(const char* s);
 GPU_Kernel* qe_kernel(const char* name);
@@ -235,6 +238,14 @@ static inline U64 qe_agg_cell_int(U32 func, U64 count, S64 isum, F64 mn, F64 mx,
 }
 extern __thread B32 qe_t_raw_int_sums;
 
+/* qe_aggregate_check_sum_exact (query_exec.c:7028-7049, from
+ * qe_aggregate_build_output :7083): a SUM over a whole-number column
+ * (U32/I32/U64/I64) whose result reaches 2^53 in magnitude may have lost
+ * low digits as a double, so the query fails (setting QE_WIDE_INT_CHECK,
+ * default 1).  cells: rows x expr_count, arg_cols index `t`'s columns. */
+B32 qe_check_sum_exact(const QE_Table* t, const U32* funcs, const S32* arg_cols, U32 expr_count, const U64* cells,
+                       U64 rows);
+
 /* an aggregate's expressions resolved to distinct argument columns */
 #define QE_AGG_MAX_ARGS 4
 typedef struct QE_AggPlan {
@@ -247,6 +258,22 @@ typedef struct QE_AggPlan {
 } QE_AggPlan;
 B32 qe_agg_plan(QE_Table* t, const U32* funcs, const S32* arg_cols, U32 expr_count, QE_AggPlan* p);
 
+/* GROUP BY columns with what their zone maps say: for a 32-bit integer
+ * key, the code of its minimum (I32 biased by 2^31) and its span; range 0
+ * when the key is another type or has no zone map */
+typedef struct QE_GroupKeys {
+  U32 count;
+  U32 col[QE_AGG_MAX_GROUP_COLS];
+  U32 type[QE_AGG_MAX_GROUP_COLS];
+  U32 min_code[QE_AGG_MAX_GROUP_COLS];
+  U64 range[QE_AGG_MAX_GROUP_COLS];
+} QE_GroupKeys;
+B32 qe_group_keys(const QE_Table* t, const S32* cols, U32 count, QE_GroupKeys* out);
+/* the product of the key ranges (0 when a key has none; saturates at 2^64-1) */
+U64 qe_group_keys_span(const QE_GroupKeys* k);
+/* the value of key k whose digit (code - minimum) is d, as F64 */
+F64 qe_group_key_value(const QE_GroupKeys* k, U32 i, U64 d);
+
 /* per-group device totals of aggregate_dense / join_probe: count u64[R];
  * sum u64[E][R]; min u32[E][R]; max u32[E][R] */
 typedef struct QE_DenseTotals {
@@ -258,9 +285,13 @@ typedef struct QE_DenseTotals {
   void* block;
 } QE_DenseTotals;
 
-B32 qe_dense_run(QE_Table* t, U32 key_col, const U32* arg_cols, U32 arg_count, B32 need_sum, B32 need_minmax,
-                 const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune, U32 key_min_code, U32 range,
-                 QE_DenseTotals* out, QE_Trace* trace);
+/* dense GROUP BY over 1..3 32-bit integer keys whose ranges multiply to
+ * at most the LDS table's range: group = the keys' digits (key 0 most
+ * significant) */
+#define QE_DENSE_MAX_KEYS 3
+B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32 arg_count, B32 need_sum,
+                 B32 need_minmax, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune, QE_DenseTotals* out,
+                 QE_Trace* trace);
 void qe_dense_totals_free(QE_DenseTotals* d);
 U64 qe_dense_totals_bytes(U32 range, U32 args);
 

@@ -29,7 +29,7 @@
 
 
 typedef struct QE_GroupAcc {
-  U32 value;
+  U32 value[QE_AGG_MAX_GROUP_COLS]; /* the dim group columns' codes (I32 biased), 0 past K */
   U64 count;
   F64 sum[QE_AGG_MAX_EXPRS];
   F64 mn[QE_AGG_MAX_EXPRS];
@@ -37,10 +37,15 @@ typedef struct QE_GroupAcc {
   B32 seen;
 } QE_GroupAcc;
 
+/* ascending group values, column 0 first (codes of I32 columns are
+ * biased, so this is signed order for them) */
 static int qe_cmp_group(const void* a, const void* b)
 {
-  U32 x = ((const QE_GroupAcc*)a)->value, y = ((const QE_GroupAcc*)b)->value;
-  return x < y ? -1 : (x > y ? 1 : 0);
+  for (U32 k = 0; k < QE_AGG_MAX_GROUP_COLS; k++) {
+    U32 x = ((const QE_GroupAcc*)a)->value[k], y = ((const QE_GroupAcc*)b)->value[k];
+    if (x != y) return x < y ? -1 : 1;
+  }
+  return 0;
 }
 
 /* find-or-add group by value (open addressing over a pow2 table) */
@@ -62,18 +67,20 @@ static B32 qe_group_map_init(QE_GroupMap* m, U64 max_groups)
   return m->slots && m->groups;
 }
 
-static QE_GroupAcc* qe_group_get(QE_GroupMap* m, U32 value)
+static QE_GroupAcc* qe_group_get(QE_GroupMap* m, const U32* value)
 {
-  U64 h = ((U64)value * 0x9E3779B97F4A7C15ull) >> 29;
+  U64 h = 0;
+  for (U32 k = 0; k < QE_AGG_MAX_GROUP_COLS; k++) h = (h ^ value[k]) * 0x9E3779B97F4A7C15ull;
+  h >>= 29;
   for (U64 s = h & m->mask;; s = (s + 1) & m->mask) {
     U32 g = m->slots[s];
     if (!g) {
       QE_GroupAcc* a = &m->groups[m->count++];
-      a->value = value;
+      memcpy(a->value, value, sizeof(a->value));
       m->slots[s] = (U32)m->count;
       return a;
     }
-    if (m->groups[g - 1].value == value) return &m->groups[g - 1];
+    if (!memcmp(m->groups[g - 1].value, value, QE_AGG_MAX_GROUP_COLS * sizeof(U32))) return &m->groups[g - 1];
   }
 }
 
@@ -119,7 +126,7 @@ static B32 qe_dim_u32(QE_Table* dim, U32 col, U32* out)
  * partition from LDS.  Fills host_acc (layout of qe_dense_totals_bytes) and
  * fl[2] (duplicate key, table full). */
 static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_col, const U32* arg_type, U32 arg_count,
-                               B32 need_sum, B32 need_minmax, QE_Table* dim, U32 dim_key_col, S32 dim_group_col,
+                               B32 need_sum, B32 need_minmax, QE_Table* dim, U32 dim_key_col, GPU_Buffer* group_buf,
                                B32 group_mode, U32 gmin, B32 group_i32, GPU_Buffer* list, U64 pass_count, U64 R,
                                GPU_Buffer* acc, U64 acc_bytes, void* host_acc, U32* fl, QE_Trace* trace)
 {
@@ -220,14 +227,14 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   if (ok) {
     /* build */
     gpu_kernel_set_arg_buffer(kb, 0, dim->columns[dim_key_col].buffer);
-    if (dim_group_col >= 0) gpu_kernel_set_arg_buffer(kb, 1, dim->columns[dim_group_col].buffer);
+    if (group_buf) gpu_kernel_set_arg_buffer(kb, 1, group_buf);
     else gpu_hip_kernel_clear_arg(kb, 1);
     gpu_kernel_set_arg_buffer(kb, 2, table);
     gpu_kernel_set_arg_buffer(kb, 3, flags);
     if (list) gpu_kernel_set_arg_buffer(kb, 4, list);
     else gpu_hip_kernel_clear_arg(kb, 4);
     gpu_kernel_set_arg_u64(kb, 0, pass_count);
-    qe_kernel_set_word(kb, QK_W_PMODE, group_mode ? (dim_group_col >= 0 ? 1u : 2u) : 0u);
+    qe_kernel_set_word(kb, QK_W_PMODE, group_mode ? (group_buf ? 1u : 2u) : 0u);
     qe_kernel_set_word(kb, QK_W_GMIN, gmin);
     qe_kernel_set_word(kb, QK_W_GTYPE, group_i32 ? CQL_ColumnType_I32 : CQL_ColumnType_U32);
     qe_kernel_set_word(kb, QK_W_PBITS, pbits);
@@ -320,23 +327,31 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
 }
 
 B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_key_col, const QE_IRNode* dim_where,
-                      U32 dim_where_count, S32 dim_group_col, const U32* funcs, const S32* fact_arg_cols, U32 expr_count,
-                      U64 cap, U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace)
+                      U32 dim_where_count, const S32* dim_group_cols, U32 dim_group_count, const U32* funcs,
+                      const S32* fact_arg_cols, U32 expr_count, U64 cap, U64* out_group_count, F64* out_keys,
+                      U64* out_cells, QE_Trace* trace)
 {
   U64 t0, p0;
   qe_trace_begin(trace, &t0, &p0);
   if (!fact || !dim || !funcs || !fact_arg_cols || !out_group_count || !out_cells || fact_key_col >= fact->column_count ||
-      dim_key_col >= dim->column_count || (dim_group_col >= 0 && (U32)dim_group_col >= dim->column_count) ||
-      expr_count == 0 || expr_count > QE_AGG_MAX_EXPRS) {
+      dim_key_col >= dim->column_count || dim_group_count > QE_AGG_MAX_GROUP_COLS ||
+      (dim_group_count && !dim_group_cols) || expr_count == 0 || expr_count > QE_AGG_MAX_EXPRS) {
     qe_set_error("qe_aggregate_join: bad arguments");
     return 0;
   }
+  const U32 K = dim_group_count;
+  for (U32 k = 0; k < K; k++)
+    if (dim_group_cols[k] < 0 || (U32)dim_group_cols[k] >= dim->column_count) {
+      qe_set_error("qe_aggregate_join: bad dim group column");
+      return 0;
+    }
   if (!qe_require_device(fact, "qe_aggregate_join") || !qe_require_device(dim, "qe_aggregate_join")) return 0;
   if (!qe_require_non_null(fact, (S32)fact_key_col, "qe_aggregate_join") ||
       !qe_require_non_null(dim, (S32)dim_key_col, "qe_aggregate_join") ||
-      !qe_require_non_null(dim, dim_group_col, "qe_aggregate_join") ||
       !qe_where_non_null(dim, dim_where, dim_where_count, "qe_aggregate_join"))
     return 0;
+  for (U32 k = 0; k < K; k++)
+    if (!qe_require_non_null(dim, dim_group_cols[k], "qe_aggregate_join")) return 0;
   for (U32 e = 0; e < expr_count; e++)
     if (!qe_require_non_null(fact, fact_arg_cols[e], "qe_aggregate_join")) return 0;
   if (qe_settings_u64("QE_JOIN_PUSHDOWN", 1) == 0) {
@@ -389,30 +404,32 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
   if (trace) trace->rows_before = fact->row_count;
 
   B32 key_i32 = fkey->type == CQL_ColumnType_I32;
-  B32 group_i32 = dim_group_col >= 0 && dim->columns[dim_group_col].type == CQL_ColumnType_I32;
-  if (dim_group_col >= 0 && dim->columns[dim_group_col].type != CQL_ColumnType_U32 && !group_i32) {
-    qe_set_error("qe_aggregate_join: the dim group column must be U32/I32");
-    return 0;
-  }
   if (dim->columns[dim_key_col].type != CQL_ColumnType_U32 && dim->columns[dim_key_col].type != CQL_ColumnType_I32) {
     qe_set_error("qe_aggregate_join: the dim key must be U32/I32");
     return 0;
+  }
+  /* the dim group columns and the ranges their zone maps give */
+  QE_GroupKeys gk;
+  if (!qe_group_keys(dim, dim_group_cols, K, &gk)) return 0;
+  B32 gi32[QE_AGG_MAX_GROUP_COLS];
+  for (U32 k = 0; k < K; k++) {
+    gi32[k] = gk.type[k] == CQL_ColumnType_I32;
+    if (gk.type[k] != CQL_ColumnType_U32 && !gi32[k]) {
+      qe_set_error("qe_aggregate_join: dim group columns must be U32/I32");
+      return 0;
+    }
   }
   U64 dn = dim->row_count;
   F64 kspan = fkey->max_value - fkey->min_value + 1.0;
   B32 dense = fact->row_count > 0 && qe_settings_u64("QE_JOIN_DENSE", 1) != 0 &&
               kspan <= (F64)qe_settings_u64("QE_AGG_DENSE_MAX_RANGE", 65536);
-  /* group codes from the dim group column's zone map (a superset of the
-   * passing rows' range) */
-  U32 gmin = 0;
-  U64 grange = 1;
-  if (dim_group_col >= 0 && dn) {
-    QE_Column* gc = &dim->columns[dim_group_col];
-    gmin = group_i32 ? ((U32)(S32)gc->min_value ^ 0x80000000u) : (U32)gc->min_value;
-    grange = (U64)(gc->max_value - gc->min_value) + 1;
-  }
+  /* group codes from the dim group columns' zone maps (a superset of the
+   * passing rows' range); several columns combine as digits */
+  U32 gmin = K == 1 ? gk.min_code[0] : 0u;
+  U64 grange = K == 0 ? 1 : qe_group_keys_span(&gk);
+  if (!dn) grange = 1;
   U64 lds_need = grange * 4 + (need_sum ? grange * 8 * arg_count : 0) + (need_minmax ? grange * 8 * arg_count : 0);
-  B32 group_mode = !dense && qe_settings_u64("QE_JOIN_GROUP_PAYLOAD", 1) != 0 &&
+  B32 group_mode = !dense && qe_settings_u64("QE_JOIN_GROUP_PAYLOAD", 1) != 0 && grange != 0 &&
                    grange <= qe_settings_u64("QE_JOIN_GROUP_MAX", 8192) && lds_need <= gpu_hip_max_lds_bytes();
   if (!dense && qe_settings_u64("QE_JOIN_PROBE", 1) == 0) {
     qe_set_error("qe_aggregate_join: key range too wide for the dense form and QE_JOIN_PROBE is off");
@@ -425,7 +442,8 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
   U64 pass_count = dn;
   U64* pass = 0;
   U32* dkeys = 0;
-  U32* dgroups = 0;
+  U32* dgroups[QE_AGG_MAX_GROUP_COLS] = {0, 0, 0, 0};
+  GPU_Buffer* code_buf = 0; /* several group columns in group mode: each dim row's digits */
   B32 ok = 1;
   B32 host_dim = dense || !group_mode;
   if (dim_where && dim_where_count) {
@@ -436,10 +454,10 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
       ok = qe_scan_filter_device(dim, dim_where, dim_where_count, &pass_dev, &pass_count);
       if (ok && pass_dev && qe_settings_u64("QE_JOIN_VERIFY_LIST", 0)) {
         /* debug: the device selection must be strictly ascending row ids */
-        U64* chk = (U64*)malloc(sizeof(U64) * pass_count);
+        U64* chk = (U64*)malloc(sizeof(U64) * (pass_count ? pass_count : 1));
         if (chk) {
-          gpu_buffer_read(pass_dev, chk, pass_count * 8);
-          for (U64 i = 0; i < pass_count; i++) {
+          ok = qe_buffer_read_checked(pass_dev, chk, pass_count * 8);
+          for (U64 i = 0; ok && i < pass_count; i++) {
             if (chk[i] >= dn || (i && chk[i] <= chk[i - 1])) {
               qe_set_error("qe_aggregate_join: device selection broken at %llu: %llu after %llu", (unsigned long long)i,
                            (unsigned long long)chk[i], (unsigned long long)(i ? chk[i - 1] : 0));
@@ -458,10 +476,32 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
   }
   if (ok && host_dim) {
     dkeys = (U32*)malloc(sizeof(U32) * (dn ? dn : 1));
-    dgroups = dim_group_col >= 0 ? (U32*)malloc(sizeof(U32) * (dn ? dn : 1)) : 0;
-    ok = dkeys && (dim_group_col < 0 || dgroups);
-    if (ok) ok = qe_dim_u32(dim, dim_key_col, dkeys);
-    if (ok && dgroups) ok = qe_dim_u32(dim, (U32)dim_group_col, dgroups);
+    ok = dkeys && qe_dim_u32(dim, dim_key_col, dkeys);
+  }
+  if (ok && (host_dim || K > 1)) {
+    for (U32 k = 0; ok && k < K; k++) {
+      dgroups[k] = (U32*)malloc(sizeof(U32) * (dn ? dn : 1));
+      ok = dgroups[k] && qe_dim_u32(dim, (U32)dim_group_cols[k], dgroups[k]);
+    }
+  }
+  if (ok && group_mode && K > 1) {
+    /* the digits of each dim row's group values, key 0 most significant:
+     * the build kernel takes them as a U32 group column with minimum 0 */
+    U32* codes = (U32*)malloc(sizeof(U32) * (dn ? dn : 1));
+    ok = codes != 0;
+    for (U64 r = 0; ok && r < dn; r++) {
+      U64 c = 0;
+      for (U32 k = 0; k < K; k++) {
+        const U32 v = gi32[k] ? dgroups[k][r] ^ 0x80000000u : dgroups[k][r];
+        c = c * gk.range[k] + (U64)(v - gk.min_code[k]);
+      }
+      codes[r] = (U32)c;
+    }
+    if (ok) {
+      code_buf = qe_scratch("join_group_codes", (dn ? dn : 1) * 4);
+      ok = code_buf && qe_buffer_write_checked(code_buf, codes, dn * 4);
+    }
+    free(codes);
   }
 
   QE_GroupMap groups;
@@ -473,24 +513,38 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
   all.row_begin = 0;
   all.row_end = fact->row_count;
   all.zones_total = all.zones_left = 1;
+  /* a dim row's group tuple (codes, I32 biased) */
+  U32 gv[QE_AGG_MAX_GROUP_COLS];
+#define QE_DIM_GROUP(d)                                                                                                \
+  do {                                                                                                                 \
+    memset(gv, 0, sizeof(gv));                                                                                         \
+    for (U32 k_ = 0; k_ < K; k_++) gv[k_] = gi32[k_] ? dgroups[k_][(d)] ^ 0x80000000u : dgroups[k_][(d)];              \
+  } while (0)
 
   if (!ok || fact->row_count == 0 || pass_count == 0) {
     /* nothing to join */
   } else if (dense) {
     path = QE_AggPath_JoinDense;
     U32 range = (U32)kspan;
-    U32 kmin_code = key_i32 ? ((U32)(S32)fkey->min_value ^ 0x80000000u) : (U32)fkey->min_value;
+    QE_GroupKeys fk;
+    memset(&fk, 0, sizeof(fk));
+    fk.count = 1;
+    fk.col[0] = fact_key_col;
+    fk.type[0] = fkey->type;
+    fk.min_code[0] = key_i32 ? ((U32)(S32)fkey->min_value ^ 0x80000000u) : (U32)fkey->min_value;
+    fk.range[0] = range;
+    const U32 kmin_code = fk.min_code[0];
     QE_DenseTotals tot;
     memset(&tot, 0, sizeof(tot));
-    ok = qe_dense_run(fact, fact_key_col, arg_col, arg_count, need_sum, need_minmax, 0, 0, &all, kmin_code, range, &tot,
-                      trace);
+    ok = qe_dense_run(fact, &fk, arg_col, arg_count, need_sum, need_minmax, 0, 0, &all, &tot, trace);
     if (ok) {
       for (U64 v = 0; v < pass_count; v++) {
         U64 d = pass[v];
         U32 code = key_i32 ? (dkeys[d] ^ 0x80000000u) : dkeys[d];
         U32 kid = code - kmin_code;
         if (code < kmin_code || kid >= range || tot.count[kid] == 0) continue;
-        QE_GroupAcc* g = qe_group_get(&groups, dgroups ? dgroups[d] : 0u);
+        QE_DIM_GROUP(d);
+        QE_GroupAcc* g = qe_group_get(&groups, gv);
         qe_group_add(g, funcs, expr_arg, expr_count, arg_type, tot.count[kid], tot.sum, tot.mn, tot.mx, range, kid);
       }
       qe_dense_totals_free(&tot);
@@ -511,8 +565,15 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     GPU_Buffer* list_buf = 0;
     if (!group_mode && pass_count != dn) {
       list_buf = qe_scratch("join_list", pass_count * 8);
-      if (list_buf) gpu_buffer_write(list_buf, pass, pass_count * 8);
+      if (list_buf && !qe_buffer_write_checked(list_buf, pass, pass_count * 8)) {
+        qe_scratch_release(list_buf);
+        list_buf = 0;
+      }
     }
+    /* group mode's payload: the one group column, the digits of several,
+     * or none (one group) */
+    GPU_Buffer* group_buf = !group_mode ? 0 : (K == 1 ? dim->columns[dim_group_cols[0]].buffer : code_buf);
+    const B32 g_i32 = K == 1 && gi32[0];
     void* host_acc = malloc(acc_bytes ? acc_bytes : 8);
     ok = kb && kp && table && flags && acc && host_acc && (group_mode || pass_count == dn || list_buf);
     U32 fl[2] = {0, 0};
@@ -521,8 +582,8 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     if (ok && partitioned) {
       path = group_mode ? QE_AggPath_JoinPartGroup : QE_AggPath_JoinPartRow;
       ok = qe_join_partitioned(fact, fact_key_col, arg_col, arg_type, arg_count, need_sum, need_minmax, dim, dim_key_col,
-                               dim_group_col, group_mode, gmin, group_i32, pass_dev ? pass_dev : list_buf, pass_count, R,
-                               acc, acc_bytes, host_acc, fl, trace);
+                               group_buf, group_mode, gmin, g_i32, pass_dev ? pass_dev : list_buf, pass_count, R, acc,
+                               acc_bytes, host_acc, fl, trace);
       if (ok && fl[0]) {
         ok = 0;
         qe_set_error("qe_aggregate_join: duplicate dim keys (the probe form needs unique keys)");
@@ -533,7 +594,7 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
       }
     } else if (ok) {
       gpu_kernel_set_arg_buffer(kb, 0, dim->columns[dim_key_col].buffer);
-      if (dim_group_col >= 0) gpu_kernel_set_arg_buffer(kb, 1, dim->columns[dim_group_col].buffer);
+      if (group_buf) gpu_kernel_set_arg_buffer(kb, 1, group_buf);
       else gpu_hip_kernel_clear_arg(kb, 1);
       gpu_kernel_set_arg_buffer(kb, 2, table);
       gpu_kernel_set_arg_buffer(kb, 3, flags);
@@ -542,9 +603,9 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
       else gpu_hip_kernel_clear_arg(kb, 4);
       gpu_kernel_set_arg_u64(kb, 0, pass_count);
       qe_kernel_set_word(kb, QK_W_LOG2, lg);
-      qe_kernel_set_word(kb, QK_W_PMODE, group_mode ? (dim_group_col >= 0 ? 1u : 2u) : 0u);
+      qe_kernel_set_word(kb, QK_W_PMODE, group_mode ? (group_buf ? 1u : 2u) : 0u);
       qe_kernel_set_word(kb, QK_W_GMIN, gmin);
-      qe_kernel_set_word(kb, QK_W_GTYPE, group_i32 ? CQL_ColumnType_I32 : CQL_ColumnType_U32);
+      qe_kernel_set_word(kb, QK_W_GTYPE, g_i32 ? CQL_ColumnType_I32 : CQL_ColumnType_U32);
       QE_ColumnSlots slots;
       memset(&slots, 0, sizeof(slots));
       qe_slot_of(&slots, fact_key_col); /* slot 0: the fact key */
@@ -613,16 +674,25 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
       if (group_mode) {
         for (U64 gi = 0; gi < R; gi++) {
           if (!cnt[gi]) continue;
-          U32 code = gmin + (U32)gi;
-          U32 value = group_i32 ? code ^ 0x80000000u : code;
-          QE_GroupAcc* g = qe_group_get(&groups, dim_group_col >= 0 ? value : 0u);
+          memset(gv, 0, sizeof(gv));
+          if (K == 1) {
+            gv[0] = gmin + (U32)gi;
+          } else {
+            U64 c = gi;
+            for (U32 k = K; k-- > 0;) {
+              gv[k] = gk.min_code[k] + (U32)(c % gk.range[k]);
+              c /= gk.range[k];
+            }
+          }
+          QE_GroupAcc* g = qe_group_get(&groups, gv);
           qe_group_add(g, funcs, expr_arg, expr_count, arg_type, cnt[gi], sums, mins, maxs, R, gi);
         }
       } else {
         for (U64 v = 0; v < pass_count; v++) {
           U64 d = pass[v];
           if (!cnt[d]) continue;
-          QE_GroupAcc* g = qe_group_get(&groups, dgroups ? dgroups[d] : 0u);
+          QE_DIM_GROUP(d);
+          QE_GroupAcc* g = qe_group_get(&groups, gv);
           qe_group_add(g, funcs, expr_arg, expr_count, arg_type, cnt[d], sums, mins, maxs, R, d);
         }
       }
@@ -633,16 +703,15 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     qe_scratch_release(acc);
     qe_scratch_release(list_buf);
   }
+#undef QE_DIM_GROUP
   if (ok) {
-    /* ascending group value (signed order for I32 groups) */
-    if (group_i32)
-      for (U64 g = 0; g < groups.count; g++) groups.groups[g].value ^= 0x80000000u;
+    /* ascending group values (codes of I32 columns are biased: signed order) */
     qsort(groups.groups, groups.count, sizeof(QE_GroupAcc), qe_cmp_group);
-    if (group_i32)
-      for (U64 g = 0; g < groups.count; g++) groups.groups[g].value ^= 0x80000000u;
     for (U64 g = 0; g < groups.count && g < cap; g++) {
       QE_GroupAcc* a = &groups.groups[g];
-      if (out_keys) out_keys[g] = group_i32 ? (F64)(S32)a->value : (F64)a->value;
+      if (out_keys)
+        for (U32 k = 0; k < K; k++)
+          out_keys[g * K + k] = gi32[k] ? (F64)(S32)(a->value[k] ^ 0x80000000u) : (F64)a->value[k];
       for (U32 e = 0; e < expr_count; e++) out_cells[g * expr_count + e] = qe_agg_cell(funcs[e], a->count, a->sum[e], a->mn[e], a->mx[e]);
     }
     *out_group_count = groups.count;
@@ -650,11 +719,14 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
       trace->path = path;
       trace->rows_after = groups.count;
     }
+    /* the fact-side SUMs reach 2^53: query_exec.c:7028-7049 via :8771 */
+    ok = qe_check_sum_exact(fact, funcs, fact_arg_cols, expr_count, out_cells, groups.count < cap ? groups.count : cap);
   }
   qe_group_map_free(&groups);
   qe_scratch_release(pass_dev);
+  qe_scratch_release(code_buf);
   free(dkeys);
-  free(dgroups);
+  for (U32 k = 0; k < QE_AGG_MAX_GROUP_COLS; k++) free(dgroups[k]);
   free(pass);
   qe_trace_end(trace, t0, p0);
   return ok;

@@ -565,8 +565,12 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
         return 0;
       }
       for (U64 i = 0; i < n; i++) tmp[i] = i;
-      gpu_buffer_write(ids, tmp, n * 8);
+      const B32 wrote = qe_buffer_write_checked(ids, tmp, n * 8);
       free(tmp);
+      if (!wrote) {
+        qe_scratch_release(ids);
+        return 0;
+      }
       *device_ids = ids;
     }
     if (trace) {

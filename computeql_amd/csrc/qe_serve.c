@@ -59,15 +59,13 @@ struct QE_Server {
 static U64 qe_serve_footprint(QE_Table* t, const QE_BatchQuery* q)
 {
   U64 groups = 1;
-  if (q->group_col >= 0 && (U32)q->group_col < t->column_count) {
-    const QE_Column* c = &t->columns[q->group_col];
+  QE_GroupKeys keys;
+  if (q->group_count && qe_group_keys(t, q->group_cols, q->group_count, &keys)) {
     groups = t->row_count ? t->row_count : 1;
-    if (c->zonemap_valid && (c->type == CQL_ColumnType_U32 || c->type == CQL_ColumnType_I32)) {
-      F64 span = c->max_value - c->min_value + 1.0;
-      if (span >= 1.0 && span < (F64)groups) groups = (U64)span;
-    }
+    const U64 span = qe_group_keys_span(&keys);
+    if (span && span < groups) groups = span;
   }
-  return 65536 + groups * (1 + 3 * 4) * 8 * 4;
+  return 65536 + groups * (1 + 3 * 4 + 2 * QE_AGG_MAX_GROUP_COLS) * 8 * 4;
 }
 
 static int qe_u64_cmp(const void* a, const void* b)
@@ -127,7 +125,8 @@ static void* qe_serve_worker(void* arg)
     if (n == 1 && s->max_batch == 1) {
       /* one query per launch: the plain operator */
       QE_BatchQuery* q = &qs[0];
-      q->ok = qe_aggregate(table, q->where, q->where_count, q->group_col, q->funcs, q->arg_cols, q->expr_count, q->cap,
+      q->ok = qe_aggregate(table, q->where, q->where_count, q->group_cols, q->group_count, q->funcs, q->arg_cols,
+                           q->expr_count, q->cap,
                            q->out_group_count, q->out_keys, q->out_cells, 0);
       q->fused = 0;
       ok = q->ok;
@@ -449,7 +448,7 @@ static B32 qe_server_drive_impl(QE_Server* s, QE_Table* table, QE_Table* const* 
     slot->q = *p;
     U64 cap = p->cap ? p->cap : 1;
     slot->q.out_group_count = &slot->group_count;
-    slot->q.out_keys = (F64*)malloc(cap * sizeof(F64));
+    slot->q.out_keys = (F64*)malloc(cap * (p->group_count ? p->group_count : 1) * sizeof(F64));
     slot->q.out_cells = (U64*)malloc(cap * (p->expr_count ? p->expr_count : 1) * sizeof(U64));
     if (!slot->q.out_keys || !slot->q.out_cells) {
       free(slot->q.out_keys);

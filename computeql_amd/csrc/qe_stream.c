@@ -837,7 +837,7 @@ B32 qe_stream_scan(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
 #define QE_STREAM_MAX_PARTS (1 + 3 * QE_AGG_MAX_ARGS)
 
 typedef struct QE_Group {
-  U64 key_bits;
+  U64 key_bits[QE_AGG_MAX_GROUP_COLS]; /* F64 bits of the key values, 0 past K */
   U64 count;
   U64 part[QE_STREAM_MAX_PARTS]; /* S64 sum, or F64 bits of sum/min/max */
 } QE_Group;
@@ -845,7 +845,8 @@ typedef struct QE_Group {
 typedef struct QE_AggCtx {
   const QE_IRNode* where;
   U32 where_count;
-  S32 group_col;
+  const S32* group_cols;
+  U32 key_count;
   /* the partial expressions: 0 is COUNT(*), then per argument column its
    * SUM, MIN and MAX as the query needs them */
   U32 part_count;
@@ -873,14 +874,21 @@ static U64 qe_mix64(U64 x)
   return x ^ (x >> 33);
 }
 
-static QE_Group* qe_group_find(QE_AggCtx* x, U64 key_bits)
+static U64 qe_key_hash(const U64* key_bits)
+{
+  U64 h = 0;
+  for (U32 k = 0; k < QE_AGG_MAX_GROUP_COLS; k++) h = qe_mix64(h ^ (key_bits[k] + 0x9E3779B97F4A7C15ull * k));
+  return h;
+}
+
+static QE_Group* qe_group_find(QE_AggCtx* x, const U64* key_bits)
 {
   if ((x->group_count + 1) * 2 > x->slot_mask + 1) {
     U64 nslots = (x->slot_mask + 1) * 2;
     U64* ns = (U64*)calloc(nslots, sizeof(U64));
     if (!ns) return 0;
     for (U64 i = 0; i < x->group_count; i++) {
-      U64 h = qe_mix64(x->groups[i].key_bits) & (nslots - 1);
+      U64 h = qe_key_hash(x->groups[i].key_bits) & (nslots - 1);
       while (ns[h]) h = (h + 1) & (nslots - 1);
       ns[h] = i + 1;
     }
@@ -888,10 +896,10 @@ static QE_Group* qe_group_find(QE_AggCtx* x, U64 key_bits)
     x->slots = ns;
     x->slot_mask = nslots - 1;
   }
-  U64 h = qe_mix64(key_bits) & x->slot_mask;
+  U64 h = qe_key_hash(key_bits) & x->slot_mask;
   while (x->slots[h]) {
     QE_Group* g = &x->groups[x->slots[h] - 1];
-    if (g->key_bits == key_bits) return g;
+    if (!memcmp(g->key_bits, key_bits, sizeof(g->key_bits))) return g;
     h = (h + 1) & x->slot_mask;
   }
   if (x->group_count == x->group_cap) {
@@ -903,7 +911,7 @@ static QE_Group* qe_group_find(QE_AggCtx* x, U64 key_bits)
   }
   QE_Group* g = &x->groups[x->group_count];
   memset(g, 0, sizeof(*g));
-  g->key_bits = key_bits;
+  memcpy(g->key_bits, key_bits, sizeof(g->key_bits));
   x->slots[h] = ++x->group_count;
   return g;
 }
@@ -959,14 +967,14 @@ static B32 qe_agg_chunk(void* vctx, QE_Table* view, U64 c0)
       QE_Trace part;
       memset(&part, 0, sizeof(part));
       qe_t_raw_int_sums = 1;
-      B32 ok = qe_aggregate(view, x->where, x->where_count, x->group_col, funcs, cols, n, x->out_cap, &groups, x->keys,
-                            x->cells, &part);
+      B32 ok = qe_aggregate(view, x->where, x->where_count, x->group_cols, x->key_count, funcs, cols, n, x->out_cap,
+                            &groups, x->keys, x->cells, &part);
       qe_t_raw_int_sums = 0;
       if (!ok) return 0;
       qe_trace_add(x->trace, &part);
       if (groups <= x->out_cap) break;
       U64 nc = groups + groups / 4;
-      F64* nk = (F64*)realloc(x->keys, sizeof(F64) * nc);
+      F64* nk = (F64*)realloc(x->keys, sizeof(F64) * nc * (x->key_count ? x->key_count : 1));
       if (nk) x->keys = nk;
       U64* ncl = (U64*)realloc(x->cells, sizeof(U64) * nc * QE_AGG_MAX_EXPRS);
       if (ncl) x->cells = ncl;
@@ -979,8 +987,12 @@ static B32 qe_agg_chunk(void* vctx, QE_Table* view, U64 c0)
     for (U64 g = 0; g < groups; g++) {
       U64 cnt = x->cells[g * n];
       if (cnt == 0) continue; /* an ungrouped chunk with no row matching */
-      U64 key_bits = 0;
-      if (x->group_col >= 0) memcpy(&key_bits, &x->keys[g], 8);
+      U64 key_bits[QE_AGG_MAX_GROUP_COLS] = {0, 0, 0, 0};
+      for (U32 k = 0; k < x->key_count; k++) {
+        F64 v = x->keys[g * x->key_count + k];
+        if (v == 0.0) v = 0.0; /* -0.0 and 0.0 are one key */
+        memcpy(&key_bits[k], &v, 8);
+      }
       QE_Group* grp = qe_group_find(x, key_bits);
       if (!grp) {
         qe_set_error("streamed aggregate: out of host memory");
@@ -997,22 +1009,26 @@ static B32 qe_agg_chunk(void* vctx, QE_Table* view, U64 c0)
   return 1;
 }
 
+/* groups in ascending key order, key 0 first (unused key words are 0) */
 static int qe_key_order(const void* a, const void* b)
 {
-  F64 ka, kb;
-  memcpy(&ka, &((const QE_Group*)a)->key_bits, 8);
-  memcpy(&kb, &((const QE_Group*)b)->key_bits, 8);
-  if (ka < kb) return -1;
-  if (ka > kb) return 1;
-  U64 ba = ((const QE_Group*)a)->key_bits, bb = ((const QE_Group*)b)->key_bits;
-  return ba < bb ? -1 : ba > bb;
+  for (U32 k = 0; k < QE_AGG_MAX_GROUP_COLS; k++) {
+    F64 ka, kb;
+    const U64 ba = ((const QE_Group*)a)->key_bits[k], bb = ((const QE_Group*)b)->key_bits[k];
+    memcpy(&ka, &ba, 8);
+    memcpy(&kb, &bb, 8);
+    if (ka < kb) return -1;
+    if (ka > kb) return 1;
+    if (ba != bb) return ba < bb ? -1 : 1;
+  }
+  return 0;
 }
 
 static B32 qe_type_int32_col(U32 type) { return type == CQL_ColumnType_U32 || type == CQL_ColumnType_I32; }
 
-B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 group_col, const U32* funcs,
-                        const S32* arg_cols, U32 expr_count, U64 cap, U64* out_group_count, F64* out_keys,
-                        U64* out_cells, QE_Trace* trace)
+B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, const S32* group_cols,
+                        U32 group_count, const U32* funcs, const S32* arg_cols, U32 expr_count, U64 cap,
+                        U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace)
 {
   U64 t0 = qe_now_ns();
   memset(&qe_t_stream_stats, 0, sizeof(qe_t_stream_stats));
@@ -1022,15 +1038,15 @@ B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count
   }
   QE_AggPlan plan;
   if (!qe_agg_plan(table, funcs, arg_cols, expr_count, &plan)) return 0;
-  if (group_col >= 0 && (U32)group_col >= table->column_count) {
-    qe_set_error("qe_aggregate: unknown group column");
-    return 0;
-  }
+  QE_GroupKeys gk;
+  if (!qe_group_keys(table, group_cols, group_count, &gk)) return 0;
+  const B32 grouped = group_count > 0;
   QE_AggCtx x;
   memset(&x, 0, sizeof(x));
   x.where = where;
   x.where_count = where_count;
-  x.group_col = group_col;
+  x.group_cols = group_cols;
+  x.key_count = group_count;
   x.trace = trace;
   x.part_count = 1;
   x.part_func[0] = QE_AGG_FUNC_COUNT;
@@ -1057,12 +1073,12 @@ B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count
   memset(&p, 0, sizeof(p));
   p.t = table;
   B32 ok = qe_mark_where(&p, where, where ? where_count : 0);
-  if (ok && group_col >= 0) qe_mark_column(&p, (U32)group_col);
+  for (U32 k = 0; ok && k < group_count; k++) qe_mark_column(&p, (U32)group_cols[k]);
   for (U32 a = 0; ok && a < plan.arg_count; a++) qe_mark_column(&p, plan.arg_col[a]);
   x.slots = (U64*)calloc(1024, sizeof(U64));
   x.slot_mask = 1023;
-  x.out_cap = group_col >= 0 ? 65536 : 1;
-  x.keys = (F64*)malloc(sizeof(F64) * x.out_cap);
+  x.out_cap = grouped ? 65536 : 1;
+  x.keys = (F64*)malloc(sizeof(F64) * x.out_cap * (group_count ? group_count : 1));
   x.cells = (U64*)malloc(sizeof(U64) * x.out_cap * QE_AGG_MAX_EXPRS);
   if (ok && (!x.slots || !x.keys || !x.cells)) {
     qe_set_error("streamed aggregate: out of host memory");
@@ -1074,13 +1090,14 @@ B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count
     ok = qe_pipeline_run(&p, leaves, leaf_count, qe_agg_chunk, &x);
   }
   if (ok) {
-    if (group_col >= 0) {
+    if (grouped) {
       qsort(x.groups, x.group_count, sizeof(QE_Group), qe_key_order);
       *out_group_count = x.group_count;
     } else {
       /* one row, zeros over no rows (query_exec.c:10335-10346) */
       if (x.group_count == 0) {
-        QE_Group* g = qe_group_find(&x, 0);
+        const U64 none[QE_AGG_MAX_GROUP_COLS] = {0, 0, 0, 0};
+        QE_Group* g = qe_group_find(&x, none);
         if (!g) ok = 0;
       }
       *out_group_count = 1;
@@ -1090,7 +1107,8 @@ B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count
     U64 rows = x.group_count < cap ? x.group_count : cap;
     for (U64 g = 0; g < rows; g++) {
       const QE_Group* grp = &x.groups[g];
-      if (out_keys) memcpy(&out_keys[g], &grp->key_bits, 8);
+      if (out_keys)
+        for (U32 k = 0; k < group_count; k++) memcpy(&out_keys[g * group_count + k], &grp->key_bits[k], 8);
       for (U32 e = 0; e < expr_count; e++) {
         U32 f = plan.funcs[e], i = src[e];
         U64* cell = &out_cells[g * expr_count + e];
@@ -1112,7 +1130,7 @@ B32 qe_stream_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count
   free(x.cells);
   qe_t_stream_stats.wall_ns = qe_now_ns() - t0;
   if (ok && trace) {
-    if (!trace->path) trace->path = group_col >= 0 ? QE_AggPath_Dense : QE_AggPath_FusedGlobal;
+    if (!trace->path) trace->path = grouped ? QE_AggPath_Dense : QE_AggPath_FusedGlobal;
     trace->rows_after = *out_group_count;
     trace->wall_ns = qe_t_stream_stats.wall_ns;
   }

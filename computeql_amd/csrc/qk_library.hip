@@ -510,6 +510,11 @@ extern "C" __global__ void LB k_scan_filter(QK_Args a)
 /*  words[14] = key slot | key type << 8 | flags << 16                  */
 /*            flags: 1 sums, 2 min/max                                  */
 /*  words[10] = key code minimum, words[11] = range                     */
+/*  binding 2 (optional, several GROUP BY keys): u32 key count K <= 3,  */
+/*  then per key three words: slot | type << 8 | encoding << 12, code   */
+/*  minimum, range; a row's group is then the keys' digits (key 0 most  */
+/*  significant), the product of the ranges being the table's range    */
+/*  (query_exec.c:9462-9500 combines small-range keys the same way).    */
 /* Global layout (range R, args E): count u64[R]; sum u64[E][R];        */
 /* min u32[E][R]; max u32[E][R].  u32 keys of args are value for U32,   */
 /* value ^ 0x80000000 for I32 (order preserving).  Block-private LDS    */
@@ -536,6 +541,8 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
   const qk_u32 key_min = a.words[QK_W_KEY_MIN];
   const qk_u32 range = a.words[QK_W_RANGE];
   const QkLeafSet L = qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), leaf_count);
+  const qk_u32* kd = a.ptr[2] ? qk_ptr<qk_u32>(a, 2) : nullptr;
+  const qk_u32 nkeys = kd ? kd[0] : 1u;
 
   /* LDS: sums u64[E][R] | counts u32[R] | min u32[E][R] | max u32[E][R] */
   qk_u64* l_sum = lds_u64;
@@ -561,29 +568,58 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
     qk_u32x4 kv[QK_UNROLL];
     qk_step_quads(a, s, b1, r0, r1, q, m);
     if (L.n) qk_leaves_step(a, L, q, m);
-    if (key_enc == 0) {
-#pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
-    } else {
-#pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) kv[j] = qk_quad_codes(a.ptr[QK_B_COL0 + key_slot], key_enc, q[j]);
-    }
     qk_u32 g[QK_UNROLL][4];
+    if (nkeys == 1) {
+      if (key_enc == 0) {
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) {
-      g[j][0] = qk_code32(kv[j].x, key_type) - key_min;
-      g[j][1] = qk_code32(kv[j].y, key_type) - key_min;
-      g[j][2] = qk_code32(kv[j].z, key_type) - key_min;
-      g[j][3] = qk_code32(kv[j].w, key_type) - key_min;
-      /* a key outside [key_min, key_min + range) (stale zone map) is
-       * dropped rather than written out of the table */
+        for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
+      } else {
 #pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (g[j][k] >= range) m[j] &= ~(1u << k);
+        for (int j = 0; j < QK_UNROLL; j++) kv[j] = qk_quad_codes(a.ptr[QK_B_COL0 + key_slot], key_enc, q[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) {
+        g[j][0] = qk_code32(kv[j].x, key_type) - key_min;
+        g[j][1] = qk_code32(kv[j].y, key_type) - key_min;
+        g[j][2] = qk_code32(kv[j].z, key_type) - key_min;
+        g[j][3] = qk_code32(kv[j].w, key_type) - key_min;
+        /* a key outside [key_min, key_min + range) (stale zone map) is
+         * dropped rather than written out of the table */
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (g[j][k] >= range) m[j] &= ~(1u << k);
+      }
+    } else {
+      /* several keys: digits, each checked against its own range */
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) g[j][0] = g[j][1] = g[j][2] = g[j][3] = 0u;
+      for (qk_u32 kk = 0; kk < nkeys; kk++) {
+        const qk_u32 d = kd[1 + 3 * kk], kmin = kd[2 + 3 * kk], kr = kd[3 + 3 * kk];
+        const qk_u32 kt = (d >> 8) & 0xfu, ke = (d >> 12) & 3u;
+        const qk_u64 kp = a.ptr[QK_B_COL0 + (d & 0xffu)];
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) {
+          const qk_u32x4 x = ke == 0 ? qk_gptr<const qk_u32x4>(kp)[q[j]] : qk_quad_codes(kp, ke, q[j]);
+          const qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const qk_u32 dg = qk_code32(w[k], kt) - kmin;
+            if (dg >= kr) m[j] &= ~(1u << k);
+            g[j][k] = g[j][k] * kr + dg;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (g[j][k] >= range) m[j] &= ~(1u << k);
+    }
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++)
 #pragma unroll
       for (int k = 0; k < 4; k++)
         if (m[j] & (1u << k)) atomicAdd(&l_cnt[g[j][k]], 1u);
-    }
     for (qk_u32 c = 0; c < nargs; c++) {
       qk_u32 at = (types >> (8 * c)) & 0xfu, aenc = (types >> (8 * c + 4)) & 3u;
       const qk_u64 cp = a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)];
@@ -1713,10 +1749,23 @@ QK_PROBE_INSTANCE(_1sm_row, 1, true, true, true)
 /* probes go to a global open-addressing table of 2^QK_W_LOG2 slots in  */
 /* HBM.  The code ~0 (a 64-bit key only) has its own extra slot.        */
 /* Global layout, C = 2^LOG2 + 1 slots: key u64[C] | min u64[E][C] |    */
-/* count u64[C] | sum u64[E][C] | max u64[E][C]; sums are exact 64-bit  */
-/* integers for U32/I32 args and f64 bits otherwise; min/max are        */
-/* order-preserving codes of the f64 value.                             */
+/* kmin u64[T][C] | count u64[C] | sum u64[E][C] | max u64[E][C] |      */
+/* kmax u64[T][C]; sums are exact 64-bit integers for U32/I32 args and  */
+/* f64 bits otherwise; min/max are order-preserving codes of the f64    */
+/* value.  T is 0 except for tuple keys (below).                        */
+/* Several GROUP BY columns (binding 4: u32 words mode, K, then per key */
+/* slot | type << 8, code minimum, range):                              */
+/*  mode 1, digits: every key a U32/I32 column with values in [minimum, */
+/*   minimum + range); the code is the keys' digits, key 0 most         */
+/*   significant (the host keeps the product of the ranges below 2^64). */
+/*  mode 2, tuple: any key types; the code is a 64-bit hash of the key  */
+/*   codes, and each slot keeps the minimum and maximum of every key    */
+/*   code (T = K), so the host sees whether two different keys ever     */
+/*   shared a code (then it rehashes with another seed, word LOG2 >> 8).*/
+/*   Tuple keys skip the LDS stage.                                     */
+/* A digit outside its range (a stale zone map) sets flags[1].          */
 /* ------------------------------------------------------------------ */
+#define QK_MAX_KEYS 4
 #define QK_HASH_EMPTY 0xffffffffffffffffULL
 #define QK_HASH_MAX_ARGS 4
 
@@ -1786,13 +1835,15 @@ __device__ __forceinline__ qk_u64 qk_hash_slot(qk_u64* gkey, qk_u32 log2, qk_u64
 }
 
 struct QkHashCtx {
-  qk_u32 E, types, flags, L, log2;
+  qk_u32 E, types, flags, L, log2, T;
   qk_u64 C;
   qk_u64* gkey;
   qk_u64* gmin;
+  qk_u64* gkmin;
   qk_u64* gcnt;
   qk_u64* gsum;
   qk_u64* gmax;
+  qk_u64* gkmax;
   qk_u32* gflags;
   qk_u64* lkey;
   qk_u32* lcnt;
@@ -1802,11 +1853,17 @@ struct QkHashCtx {
 };
 
 __device__ __forceinline__ void qk_hash_global_add(const QkHashCtx& h, qk_u64 key, qk_u64 cnt, const qk_u64* sum,
-                                                   const qk_u64* mn, const qk_u64* mx)
+                                                   const qk_u64* mn, const qk_u64* mx, const qk_u64* kc = nullptr)
 {
   const qk_u64 s = qk_hash_slot(h.gkey, h.log2, key, h.gflags);
   if (s == ~0ULL) return;
   atomicAdd((unsigned long long*)&h.gcnt[s], (unsigned long long)cnt);
+  if (kc) {
+    for (qk_u32 i = 0; i < h.T; i++) {
+      atomicMin((unsigned long long*)&h.gkmin[(qk_u64)i * h.C + s], (unsigned long long)kc[i]);
+      atomicMax((unsigned long long*)&h.gkmax[(qk_u64)i * h.C + s], (unsigned long long)kc[i]);
+    }
+  }
   for (qk_u32 c = 0; c < h.E; c++) {
     const qk_u32 at = (h.types >> (8 * c)) & 0xffu;
     if (h.flags & 1u) {
@@ -1822,7 +1879,7 @@ __device__ __forceinline__ void qk_hash_global_add(const QkHashCtx& h, qk_u64 ke
 
 /* one row: into the block's LDS table when its key has (or gets) a slot
  * there within four probes, else straight into the global table */
-__device__ __forceinline__ void qk_hash_row(const QkHashCtx& h, qk_u64 key, const double* v)
+__device__ __forceinline__ void qk_hash_row(const QkHashCtx& h, qk_u64 key, const double* v, const qk_u64* kc = nullptr)
 {
   qk_u64 sum[QK_HASH_MAX_ARGS], code[QK_HASH_MAX_ARGS];
 #pragma unroll
@@ -1854,7 +1911,63 @@ __device__ __forceinline__ void qk_hash_row(const QkHashCtx& h, qk_u64 key, cons
       return;
     }
   }
-  qk_hash_global_add(h, key, 1, sum, code, code);
+  qk_hash_global_add(h, key, 1, sum, code, code, kc);
+}
+
+/* the group code of several keys (binding 4's descriptor) for one row:
+ * digits or the tuple hash (with the key codes in kc); *ok cleared when a
+ * digit is outside its range */
+__device__ __forceinline__ qk_u64 qk_multi_code1(const QK_Args& a, const qk_u32* kd, qk_u64 row, qk_u64* kc, bool* ok)
+{
+  const qk_u32 mode = kd[0], nk = kd[1];
+  qk_u64 code = mode == 2 ? (qk_u64)kd[2 + 3 * nk] * 0x9E3779B97F4A7C15ULL : 0ULL;
+  for (qk_u32 i = 0; i < nk; i++) {
+    const qk_u32 d = kd[2 + 3 * i], t = (d >> 8) & 0xfu;
+    const qk_u64 c = qk_load1_code(a, d & 0xffu, t, row);
+    if (mode == 1) {
+      const qk_u32 dg = qk_code32((qk_u32)c, t) - kd[3 + 3 * i], r = kd[4 + 3 * i];
+      if (dg >= r) *ok = false;
+      code = code * r + dg;
+    } else {
+      kc[i] = c;
+      code = qk_fmix64(code ^ (c + 0x9E3779B97F4A7C15ULL * (i + 1)));
+    }
+  }
+  return code;
+}
+
+/* the same for the four rows of quad q: codes c[4], key codes kc[key][4];
+ * a row with a digit outside its range leaves mask m */
+__device__ __forceinline__ void qk_multi_code4(const QK_Args& a, const qk_u32* kd, qk_u64 q, qk_u64 c[4],
+                                               qk_u64 kc[QK_MAX_KEYS][4], qk_u32& m, qk_u32& bad)
+{
+  const qk_u32 mode = kd[0], nk = kd[1];
+  const qk_u64 seed = mode == 2 ? (qk_u64)kd[2 + 3 * nk] * 0x9E3779B97F4A7C15ULL : 0ULL;
+#pragma unroll
+  for (int k = 0; k < 4; k++) c[k] = seed;
+  for (qk_u32 i = 0; i < nk && i < QK_MAX_KEYS; i++) {
+    const qk_u32 d = kd[2 + 3 * i], t = (d >> 8) & 0xfu;
+    qk_u64 x[4];
+    qk_load4_code(a, d & 0xffu, t, q, x);
+    if (mode == 1) {
+      const qk_u32 mn = kd[3 + 3 * i], r = kd[4 + 3 * i];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const qk_u32 dg = qk_code32((qk_u32)x[k], t) - mn;
+        if (dg >= r && (m & (1u << k))) {
+          m &= ~(1u << k);
+          bad = 1u;
+        }
+        c[k] = c[k] * r + dg;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        kc[i][k] = x[k];
+        c[k] = qk_fmix64(c[k] ^ (x[k] + 0x9E3779B97F4A7C15ULL * (i + 1)));
+      }
+    }
+  }
 }
 
 /* aggregate_hash: binding 0 leaves, 1 table (layout above), 2 flags
@@ -1878,14 +1991,20 @@ extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
   const qk_u32 key_type = (a.words[QK_W_KEY] >> 8) & 0xfu;
   const qk_u32 key_enc = (a.words[QK_W_KEY] >> 12) & 3u;
   h.flags = a.words[QK_W_KEY] >> 16;
-  h.log2 = a.words[QK_W_LOG2];
+  h.log2 = a.words[QK_W_LOG2] & 0xffu;
   h.L = a.words[QK_W_RANGE];
+  const qk_u32* kd = a.ptr[4] ? qk_ptr<qk_u32>(a, 4) : nullptr;
+  const bool tuple = kd && kd[0] == 2u;
+  h.T = tuple ? kd[1] : 0u;
+  if (tuple) h.L = 0;
   h.C = (1ULL << h.log2) + 1;
   h.gkey = qk_wptr<qk_u64>(a, 1);
   h.gmin = h.gkey + h.C;
-  h.gcnt = h.gmin + (qk_u64)h.E * h.C;
+  h.gkmin = h.gmin + (qk_u64)h.E * h.C;
+  h.gcnt = h.gkmin + (qk_u64)h.T * h.C;
   h.gsum = h.gcnt + h.C;
   h.gmax = h.gsum + (qk_u64)h.E * h.C;
+  h.gkmax = h.gmax + (qk_u64)h.E * h.C;
   h.gflags = qk_wptr<qk_u32>(a, 2);
   h.lkey = lds_u64;
   h.lmin = h.lkey + h.L;
@@ -1911,7 +2030,15 @@ extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
       double v[QK_HASH_MAX_ARGS] = {0.0, 0.0, 0.0, 0.0};
       for (qk_u32 c = 0; c < h.E; c++)
         v[c] = qk_load1_f64(a, (slots >> (4 * c)) & 0xfu, (h.types >> (8 * c)) & 0xffu, row);
-      qk_hash_row(h, no_key ? 0ULL : qk_load1_code(a, key_slot, key_type, row), v);
+      if (kd) {
+        qk_u64 kc[QK_MAX_KEYS];
+        bool ok = true;
+        const qk_u64 code = qk_multi_code1(a, kd, row, kc, &ok);
+        if (ok) qk_hash_row(h, code, v, tuple ? kc : nullptr);
+        else atomicOr(&h.gflags[1], 1u);
+      } else {
+        qk_hash_row(h, no_key ? 0ULL : qk_load1_code(a, key_slot, key_type, row), v);
+      }
     }
   } else {
     const qk_u64 rows = qk_push(a, QK_P_ROWS);
@@ -1929,7 +2056,14 @@ extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
       for (int j = 0; j < QK_UNROLL; j++) {
         if (!m[j]) continue;
         qk_u64 kc[4] = {0, 0, 0, 0};
-        if (!no_key) qk_load4_code(a, key_slot, key_type, q[j], kc);
+        qk_u64 kcs[QK_MAX_KEYS][4];
+        if (kd) {
+          qk_u32 bad = 0u;
+          qk_multi_code4(a, kd, q[j], kc, kcs, m[j], bad);
+          if (bad) atomicOr(&h.gflags[1], 1u);
+        } else if (!no_key) {
+          qk_load4_code(a, key_slot, key_type, q[j], kc);
+        }
         double v[QK_HASH_MAX_ARGS][4];
 #pragma unroll
         for (int c = 0; c < QK_HASH_MAX_ARGS; c++) {
@@ -1940,7 +2074,14 @@ extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
         for (int k = 0; k < 4; k++) {
           if (!(m[j] & (1u << k))) continue;
           const double vk[QK_HASH_MAX_ARGS] = {v[0][k], v[1][k], v[2][k], v[3][k]};
-          qk_hash_row(h, kc[k], vk);
+          if (tuple) {
+            qk_u64 kk[QK_MAX_KEYS];
+#pragma unroll
+            for (int i = 0; i < QK_MAX_KEYS; i++) kk[i] = kcs[i][k];
+            qk_hash_row(h, kc[k], vk, kk);
+          } else {
+            qk_hash_row(h, kc[k], vk);
+          }
         }
       }
     }
@@ -1963,21 +2104,25 @@ extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
 
 /* hash_compact: the occupied slots of aggregate_hash's table, in slot
  * order.  binding 1 table, 2 block counts (u64 per block), 3 out records
- * (u64[2 + 3E] each: key, count, sums, mins, maxs); push 6 mode (1 block
- * counts, 2 write), words ARGS (count), LOG2.  A slot is occupied when its
- * count is nonzero (rows only ever add). */
+ * (u64[2 + 3E + 2T] each: key, count, sums, mins, maxs, key minima, key
+ * maxima); push 6 mode (1 block counts, 2 write), words ARGS (count | T
+ * << 8), LOG2.  A slot is occupied when its count is nonzero (rows only
+ * ever add). */
 extern "C" __global__ void LB k_hash_compact(QK_Args a)
 {
   __shared__ qk_u32 s_scan[QK_WG / 64];
   __shared__ qk_u64 s_red[QK_WG / 64];
   const qk_u32 E = a.words[QK_W_ARGS] & 0xffu;
-  const qk_u32 log2 = a.words[QK_W_LOG2];
+  const qk_u32 T = (a.words[QK_W_ARGS] >> 8) & 0xffu;
+  const qk_u32 log2 = a.words[QK_W_LOG2] & 0xffu;
   const qk_u64 C = (1ULL << log2) + 1;
   const qk_u64* gkey = qk_ptr<qk_u64>(a, 1);
   const qk_u64* gmin = gkey + C;
-  const qk_u64* gcnt = gmin + (qk_u64)E * C;
+  const qk_u64* gkmin = gmin + (qk_u64)E * C;
+  const qk_u64* gcnt = gkmin + (qk_u64)T * C;
   const qk_u64* gsum = gcnt + C;
   const qk_u64* gmax = gsum + (qk_u64)E * C;
+  const qk_u64* gkmax = gmax + (qk_u64)E * C;
   qk_u64* bc = qk_wptr<qk_u64>(a, 2);
   const qk_u64 mode = qk_push(a, QK_P_MODE);
   const qk_u64 per = (C + gridDim.x - 1) / gridDim.x;
@@ -1994,7 +2139,8 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
   for (qk_u32 b = threadIdx.x; b < blockIdx.x; b += QK_WG) base += bc[b];
   base = qk_block_sum_u64(base, s_red);
   qk_u64* out = qk_wptr<qk_u64>(a, 3);
-  const qk_u64 cap = a.bytes[3] / (8ULL * (2 + 3 * E));
+  const qk_u64 rw = 2 + 3 * (qk_u64)E + 2 * (qk_u64)T;
+  const qk_u64 cap = a.bytes[3] / (8ULL * rw);
   for (qk_u64 c0 = s0; c0 < s1; c0 += QK_WG) {
     const qk_u64 s = c0 + threadIdx.x;
     const qk_u32 occ = (s < s1 && gcnt[s]) ? 1u : 0u;
@@ -2003,13 +2149,17 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
     if (occ) {
       const qk_u64 g = base + pre;
       if (g < cap) {
-        qk_u64* r = out + g * (2 + 3 * (qk_u64)E);
+        qk_u64* r = out + g * rw;
         r[0] = gkey[s];
         r[1] = gcnt[s];
         for (qk_u32 c = 0; c < E; c++) {
           r[2 + c] = gsum[(qk_u64)c * C + s];
           r[2 + E + c] = gmin[(qk_u64)c * C + s];
           r[2 + 2 * E + c] = gmax[(qk_u64)c * C + s];
+        }
+        for (qk_u32 i = 0; i < T; i++) {
+          r[2 + 3 * E + i] = gkmin[(qk_u64)i * C + s];
+          r[2 + 3 * E + T + i] = gkmax[(qk_u64)i * C + s];
         }
       }
     }

@@ -14,16 +14,45 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import ir
-from .native import QE_BatchConfig, QE_BatchQuery, QE_BatchTrace, QE_ResidentStats, QE_StreamStats, QE_Trace, load
+from .native import (MAX_GROUP_COLS, QE_BatchConfig, QE_BatchQuery, QE_BatchTrace, QE_ResidentStats, QE_StreamStats,
+                     QE_Trace, load)
+
+def _np_format(t):
+    if issubclass(t, C.Array):
+        return (_np_format(t._type_), (t._length_,))
+    return "<u8" if t in (C.c_void_p, C.c_uint64) else ("<i4" if t is C.c_int32 else "<u4")
+
 
 # QE_BatchQuery as a numpy record: a batch's records are filled column-wise
 _BQ_DTYPE = np.dtype({
     "names": [f for f, _ in QE_BatchQuery._fields_],
-    "formats": ["<u8" if t in (C.c_void_p, C.c_uint64) else ("<i4" if t is C.c_int32 else "<u4")
-                for _, t in QE_BatchQuery._fields_],
+    "formats": [_np_format(t) for _, t in QE_BatchQuery._fields_],
     "offsets": [getattr(QE_BatchQuery, f).offset for f, _ in QE_BatchQuery._fields_],
     "itemsize": C.sizeof(QE_BatchQuery),
 })
+
+
+def group_list(group) -> list:
+    """GROUP BY columns as a list: None -> [], a column -> [it], a list or
+    tuple of up to 4 columns as given."""
+    if group is None:
+        return []
+    if isinstance(group, (list, tuple)):
+        if len(group) > MAX_GROUP_COLS:
+            raise ValueError(f"at most {MAX_GROUP_COLS} GROUP BY columns")
+        return [int(g) for g in group]
+    return [int(group)]
+
+
+def shape_keys(group, keys: np.ndarray, n: int) -> np.ndarray:
+    """A result's keys as the caller asked for them: (n, K) for a list of
+    columns, (n,) for one column, n zeros without a GROUP BY.  `keys` holds
+    at least n * K values, K key values per group."""
+    k = len(group_list(group))
+    if k == 0:
+        return np.zeros(n, dtype=np.float64)
+    flat = np.asarray(keys, dtype=np.float64).reshape(-1)[: n * k]
+    return flat.reshape(n, k).copy() if isinstance(group, (list, tuple)) else flat.copy()
 
 U32, U64, F32, F64, I32, I64 = 1, 2, 3, 4, 7, 8
 COUNT, SUM, AVG, MIN, MAX = 0, 1, 2, 3, 4
@@ -68,13 +97,13 @@ def compute_units() -> int:
 @dataclass
 class AggResult:
     funcs: list
-    keys: np.ndarray
+    keys: np.ndarray   # (groups,) for one key (zeros without one), (groups, K) for a list of K keys
     cells: np.ndarray  # uint64 [groups, exprs]: COUNT as U64, others F64 bits
     trace: dict = field(default_factory=dict)
 
     @property
     def groups(self) -> int:
-        return len(self.keys)
+        return len(self.cells)
 
     def column(self, e: int) -> np.ndarray:
         c = self.cells[:, e]
@@ -83,7 +112,7 @@ class AggResult:
     def rows(self) -> list:
         out = []
         for g in range(self.groups):
-            row = [float(self.keys[g])]
+            row = [float(k) for k in np.atleast_1d(self.keys[g])]
             for e, f in enumerate(self.funcs):
                 v = self.cells[g, e]
                 row.append(int(v) if f == COUNT else float(np.array([v], dtype=np.uint64).view(np.float64)[0]))
@@ -252,29 +281,32 @@ class Table:
         n = min(int(cnt.value), cap)
         return [o[:n] for o in outs]
 
-    def aggregate(self, funcs, args, where=None, group: int | None = None, cap: int | None = None) -> AggResult:
-        """qe_aggregate_impl for the hot-path shapes (args[e] < 0 or None = COUNT(*))."""
+    def aggregate(self, funcs, args, where=None, group=None, cap: int | None = None) -> AggResult:
+        """qe_aggregate_impl for the hot-path shapes (args[e] < 0 or None =
+        COUNT(*)); group: None, a column, or a list of up to 4 columns."""
         funcs = list(funcs)
         e = len(funcs)
         fa = (C.c_uint32 * e)(*funcs)
         aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in args])
+        gl = group_list(group)
+        ga = (C.c_int32 * max(len(gl), 1))(*gl)
         nodes = ir.to_nodes(where)
         nptr = nodes.ctypes.data if nodes is not None else None
         ncount = len(nodes) if nodes is not None else 0
         if cap is None:
-            cap = 1 if group is None else 1 << 17
-        keys = np.empty(max(cap, 1), dtype=np.float64)
+            cap = 1 if not gl else 1 << 17
+        keys = np.empty(max(cap, 1) * max(len(gl), 1), dtype=np.float64)
         cells = np.empty((max(cap, 1), e), dtype=np.uint64)
         g = C.c_uint64(0)
         tr = QE_Trace()
-        ok = self._lib.qe_aggregate(self.handle, nptr, ncount, -1 if group is None else int(group), fa, aa, e, cap,
-                                    C.byref(g), keys.ctypes.data, cells.ctypes.data, C.byref(tr))
+        ok = self._lib.qe_aggregate(self.handle, nptr, ncount, ga, len(gl), fa, aa, e, cap, C.byref(g),
+                                    keys.ctypes.data, cells.ctypes.data, C.byref(tr))
         if not ok:
             raise _err("aggregate")
         n = int(g.value)
         if n > cap:
             return self.aggregate(funcs, args, where, group, cap=n)
-        return AggResult(funcs, keys[:n].copy(), cells[:n].copy(), tr.as_dict())
+        return AggResult(funcs, shape_keys(group, keys, n), cells[:n].copy(), tr.as_dict())
 
 
 @dataclass
@@ -283,7 +315,7 @@ class AggSpec:
     funcs: list
     args: list
     where: object = None
-    group: int | None = None
+    group: object = None  # None, a column, or a list of up to 4 columns
     cap: int | None = None
 
     def prepared(self):
@@ -314,22 +346,26 @@ def _batch_records(specs):
     recs = np.zeros(max(n, 1), dtype=_BQ_DTYPE)
     counts = np.zeros(max(n, 1), dtype=np.uint64)
     preps = [sp.prepared() for sp in specs]
-    caps = [sp.cap if sp.cap is not None else (1 if sp.group is None else 1 << 12) for sp in specs]
+    groups = [group_list(sp.group) for sp in specs]
+    caps = [sp.cap if sp.cap is not None else (1 if not gl else 1 << 12) for sp, gl in zip(specs, groups)]
     caps = [max(c, 1) for c in caps]
     widths = [len(p[0]) for p in preps]
-    sizes = [c * (1 + e) for c, e in zip(caps, widths)]
+    kw = [max(len(gl), 1) for gl in groups]
+    sizes = [c * (k + e) for c, k, e in zip(caps, kw, widths)]
     arena = np.empty(max(sum(sizes), 1), dtype=np.uint64)
     base = arena.ctypes.data
     offs = np.zeros(n, dtype=np.uint64)
     if n > 1:
         offs[1:] = np.cumsum(sizes[:-1], dtype=np.uint64)
     outs = []
-    for (funcs, _, _, _, _), cap, e, o in zip(preps, caps, widths, offs.tolist()):
-        outs.append((funcs, arena[o:o + cap].view(np.float64), arena[o + cap:o + cap * (1 + e)].reshape(cap, e), cap))
+    for (funcs, _, _, _, _), cap, k, e, o in zip(preps, caps, kw, widths, offs.tolist()):
+        outs.append((funcs, arena[o:o + cap * k].view(np.float64), arena[o + cap * k:o + cap * (k + e)].reshape(cap, e),
+                     cap))
     if n:
         recs["where"][:n] = [p[4][0] for p in preps]
         recs["where_count"][:n] = [p[4][1] for p in preps]
-        recs["group_col"][:n] = [-1 if sp.group is None else int(sp.group) for sp in specs]
+        recs["group_count"][:n] = [len(gl) for gl in groups]
+        recs["group_cols"][:n] = [gl + [-1] * (MAX_GROUP_COLS - len(gl)) for gl in groups]
         recs["funcs"][:n] = [p[4][2] for p in preps]
         recs["arg_cols"][:n] = [p[4][3] for p in preps]
         recs["expr_count"][:n] = widths
@@ -337,7 +373,7 @@ def _batch_records(specs):
         recs["out_group_count"][:n] = counts.ctypes.data + 8 * np.arange(n, dtype=np.uint64)
         o = base + 8 * offs
         recs["out_keys"][:n] = o
-        recs["out_cells"][:n] = o + 8 * np.asarray(caps, dtype=np.uint64)
+        recs["out_cells"][:n] = o + 8 * np.asarray(caps, dtype=np.uint64) * np.asarray(kw, dtype=np.uint64)
     return recs, counts, outs
 
 
@@ -363,7 +399,7 @@ def aggregate_batch(table: Table, specs, with_trace: bool = False):
             r.trace["fused"] = 0
             results.append(r)
             continue
-        results.append(AggResult(funcs, keys[:ng], cells[:ng], {"fused": int(fused[i])}))
+        results.append(AggResult(funcs, shape_keys(specs[i].group, keys, ng), cells[:ng], {"fused": int(fused[i])}))
     if not ok:
         raise _err("aggregate_batch")
     return (results, tr.as_dict()) if with_trace else results
@@ -383,15 +419,18 @@ def autotune_batch(table: Table, specs) -> dict:
         fa = (C.c_uint32 * e)(*funcs)
         aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in sp.args])
         nodes = ir.to_nodes(sp.where)
-        cap = sp.cap if sp.cap is not None else (1 if sp.group is None else 1 << 17)
-        keys = np.empty(max(cap, 1), dtype=np.float64)
+        gl = group_list(sp.group)
+        cap = sp.cap if sp.cap is not None else (1 if not gl else 1 << 17)
+        keys = np.empty(max(cap, 1) * max(len(gl), 1), dtype=np.float64)
         cells = np.empty((max(cap, 1), e), dtype=np.uint64)
         g = C.c_uint64(0)
         keep += [fa, aa, nodes, keys, cells, g]
         q = qs[i]
         q.where = nodes.ctypes.data if nodes is not None else None
         q.where_count = len(nodes) if nodes is not None else 0
-        q.group_col = -1 if sp.group is None else int(sp.group)
+        q.group_count = len(gl)
+        for j, c in enumerate(gl):
+            q.group_cols[j] = c
         q.funcs, q.arg_cols, q.expr_count, q.cap = C.addressof(fa), C.addressof(aa), e, cap
         q.out_group_count, q.out_keys, q.out_cells = C.addressof(g), keys.ctypes.data, cells.ctypes.data
     best = QE_BatchConfig()
@@ -404,29 +443,31 @@ def autotune_batch(table: Table, specs) -> dict:
 
 
 def aggregate_join(fact: Table, fact_key: int, dim: Table, dim_key: int, funcs, fact_args, dim_where=None,
-                   dim_group: int | None = None, cap: int = 1 << 16) -> AggResult:
-    """qe_aggregate_join_pushdown: aggregates of fact rows grouped by a dim column."""
+                   dim_group=None, cap: int = 1 << 16) -> AggResult:
+    """qe_aggregate_join_pushdown: aggregates of fact rows grouped by dim
+    columns (dim_group: None, a column, or a list of up to 4 columns)."""
     lib = load()
     funcs = list(funcs)
     e = len(funcs)
     fa = (C.c_uint32 * e)(*funcs)
     aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in fact_args])
+    gl = group_list(dim_group)
+    ga = (C.c_int32 * max(len(gl), 1))(*gl)
     nodes = ir.to_nodes(dim_where)
     nptr = nodes.ctypes.data if nodes is not None else None
     ncount = len(nodes) if nodes is not None else 0
-    keys = np.empty(max(cap, 1), dtype=np.float64)
+    keys = np.empty(max(cap, 1) * max(len(gl), 1), dtype=np.float64)
     cells = np.empty((max(cap, 1), e), dtype=np.uint64)
     g = C.c_uint64(0)
     tr = QE_Trace()
-    ok = lib.qe_aggregate_join(fact.handle, fact_key, dim.handle, dim_key, nptr, ncount,
-                               -1 if dim_group is None else int(dim_group), fa, aa, e, cap, C.byref(g),
-                               keys.ctypes.data, cells.ctypes.data, C.byref(tr))
+    ok = lib.qe_aggregate_join(fact.handle, fact_key, dim.handle, dim_key, nptr, ncount, ga, len(gl), fa, aa, e, cap,
+                               C.byref(g), keys.ctypes.data, cells.ctypes.data, C.byref(tr))
     if not ok:
         raise _err("aggregate_join")
     n = int(g.value)
     if n > cap:
         return aggregate_join(fact, fact_key, dim, dim_key, funcs, fact_args, dim_where, dim_group, cap=n)
-    return AggResult(funcs, keys[:n].copy(), cells[:n].copy(), tr.as_dict())
+    return AggResult(funcs, shape_keys(dim_group, keys, n), cells[:n].copy(), tr.as_dict())
 
 
 JOIN_TYPES = {"inner": 0, "left": 1, "semi": 2, "anti": 3}

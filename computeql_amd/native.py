@@ -33,11 +33,15 @@ class QE_Trace(C.Structure):
         return {f: (getattr(self, f).decode() if f == "kernel" else getattr(self, f)) for f, _ in self._fields_}
 
 
+MAX_GROUP_COLS = 4  # QE_AGG_MAX_GROUP_COLS
+
+
 class QE_BatchQuery(C.Structure):
     _fields_ = [
         ("where", C.c_void_p),
         ("where_count", U32),
-        ("group_col", C.c_int32),
+        ("group_count", U32),
+        ("group_cols", C.c_int32 * MAX_GROUP_COLS),
         ("funcs", C.c_void_p),
         ("arg_cols", C.c_void_p),
         ("expr_count", U32),
@@ -179,6 +183,9 @@ SIGNATURES = [
     ("gpu_hip_buffer_wrap_host", P, [P, U64]),
     ("gpu_hip_buffer_view", P, [P, U64, U64]),
     ("gpu_hip_fault_alloc_at", None, [U64]),
+    ("gpu_hip_fault_read_at", None, [U64]),
+    ("gpu_hip_buffer_write_checked", B32, [P, P, U64]),
+    ("gpu_hip_buffer_read_checked", B32, [P, P, U64]),
     ("gpu_hip_last_error", C.c_char_p, []),
     ("qe_settings_set_u64", None, [C.c_char_p, U64]),
     ("qe_settings_u64", U64, [C.c_char_p, U64]),
@@ -200,8 +207,8 @@ SIGNATURES = [
     ("qe_gen_fact_fk", B32, [P, U32, U64, P, U32]),
     ("qe_gen_dim", B32, [P, U64, U32, B32, S32, S32]),
     ("qe_scan_filter", B32, [P, P, U32, B32, P, U64, P, P]),
-    ("qe_aggregate", B32, [P, P, U32, S32, P, P, U32, U64, P, P, P, P]),
-    ("qe_aggregate_join", B32, [P, U32, P, U32, P, U32, S32, P, P, U32, U64, P, P, P, P]),
+    ("qe_aggregate", B32, [P, P, U32, P, U32, P, P, U32, U64, P, P, P, P]),
+    ("qe_aggregate_join", B32, [P, U32, P, U32, P, U32, P, U32, P, P, U32, U64, P, P, P, P]),
     ("qe_hash_join", U32, [P, U32, P, U64, P, U32, P, U64, U32, P, U64, P, P]),
     ("qe_gather", U32, [P, P, U32, P, U64, U64, U64, P]),
     ("qe_table_encode_column", S32, [P, U32, B32]),

@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import ir
-from .engine import AggResult, AggSpec, QueryError, Table, _err
+from .engine import AggResult, AggSpec, QueryError, Table, _err, group_list, shape_keys
 from .native import QE_BatchQuery, QE_ServerConfig, QE_ServerStats, load
 
 
@@ -27,12 +27,15 @@ def _query(spec: AggSpec, keep: list, cap: int | None = None):
     fa = (C.c_uint32 * e)(*funcs)
     aa = (C.c_int32 * e)(*[(-1 if a is None else int(a)) for a in spec.args])
     nodes = ir.to_nodes(spec.where)
+    gl = group_list(spec.group)
     if cap is None:
-        cap = spec.cap if spec.cap is not None else (1 if spec.group is None else 1 << 17)
+        cap = spec.cap if spec.cap is not None else (1 if not gl else 1 << 17)
     q = QE_BatchQuery()
     q.where = nodes.ctypes.data if nodes is not None else None
     q.where_count = len(nodes) if nodes is not None else 0
-    q.group_col = -1 if spec.group is None else int(spec.group)
+    q.group_count = len(gl)
+    for j, c in enumerate(gl):
+        q.group_cols[j] = c
     q.funcs = C.addressof(fa)
     q.arg_cols = C.addressof(aa)
     q.expr_count = e
@@ -42,9 +45,10 @@ def _query(spec: AggSpec, keep: list, cap: int | None = None):
 
 
 class Ticket:
-    def __init__(self, server: "Server", ticket: int, q, funcs, keys, cells, count, keep):
+    def __init__(self, server: "Server", ticket: int, q, funcs, keys, cells, count, keep, group=None):
         self._server, self._ticket, self._q = server, ticket, q
         self._funcs, self._keys, self._cells, self._count, self._keep = funcs, keys, cells, count, keep
+        self._group = group
         self._done = False
 
     def result(self, timeout: float = 60.0) -> AggResult:
@@ -55,9 +59,10 @@ class Ticket:
         if not self._q.ok:
             raise QueryError("served request failed")
         n = int(self._count.value)
-        if n > len(self._keys):
-            raise QueryError(f"{n} groups exceed the request's capacity {len(self._keys)}")
-        return AggResult(self._funcs, self._keys[:n].copy(), self._cells[:n].copy(), {"fused": int(self._q.fused)})
+        if n > len(self._cells):
+            raise QueryError(f"{n} groups exceed the request's capacity {len(self._cells)}")
+        return AggResult(self._funcs, shape_keys(self._group, self._keys, n), self._cells[:n].copy(),
+                         {"fused": int(self._q.fused)})
 
 
 class Server:
@@ -71,7 +76,7 @@ class Server:
     def submit(self, table: Table, spec: AggSpec) -> Ticket:
         keep: list = []
         q, funcs, cap = _query(spec, keep)
-        keys = np.empty(max(cap, 1), dtype=np.float64)
+        keys = np.empty(max(cap, 1) * max(len(group_list(spec.group)), 1), dtype=np.float64)
         cells = np.empty((max(cap, 1), len(funcs)), dtype=np.uint64)
         count = C.c_uint64(0)
         q.out_group_count = C.addressof(count)
@@ -80,7 +85,7 @@ class Server:
         t = self._lib.qe_server_submit(self.handle, table.handle, C.byref(q))
         if not t:
             raise _err("qe_server_submit")
-        return Ticket(self, t, q, funcs, keys, cells, count, keep)
+        return Ticket(self, t, q, funcs, keys, cells, count, keep, spec.group)
 
     def stats(self) -> dict:
         st = QE_ServerStats()

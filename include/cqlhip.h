@@ -236,6 +236,15 @@ GPU_Buffer* gpu_hip_buffer_wrap_host(void* host_ptr, U64 size);
 GPU_Buffer* gpu_hip_buffer_view(GPU_Buffer* parent, U64 offset, U64 size);
 /* fault injection: fail the k-th device allocation from now (gpu.h:68) */
 void gpu_hip_fault_alloc_at(U64 k);
+/* fault injection: the k-th device-to-host read from now (a synchronous
+ * read, or one recorded in a batch, which then fails) reports failure; no
+ * counterpart in the reference, whose reads cannot be made to fail */
+void gpu_hip_fault_read_at(U64 k);
+/* buffer_write / buffer_read that report failure (0, gpu_hip_last_error())
+ * instead of dropping it or zero-filling the destination; every operator
+ * of this library copies through these */
+B32 gpu_hip_buffer_write_checked(GPU_Buffer* buffer, const void* data, U64 size);
+B32 gpu_hip_buffer_read_checked(GPU_Buffer* buffer, void* data, U64 size);
 const char* gpu_hip_last_error(void);
 
 /* ------------------------------------------------------------------ */
@@ -272,6 +281,7 @@ enum {
  * qe_agg_func_policy (query_exec.h:917-926): COUNT -> U64, others F64 */
 enum { QE_AGG_FUNC_COUNT = 0, QE_AGG_FUNC_SUM = 1, QE_AGG_FUNC_AVG = 2, QE_AGG_FUNC_MIN = 3, QE_AGG_FUNC_MAX = 4 };
 #define QE_AGG_MAX_EXPRS 8
+#define QE_AGG_MAX_GROUP_COLS 4 /* query_exec.h:898 */
 
 /* WHERE tree as the planner hands it over (IR_Node, flattened in prefix
  * order with child counts).  OPERATOR nodes carry an op; COLUMN nodes a
@@ -411,24 +421,33 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
                    U64* out_ids, U64 cap, U64* out_count, QE_Trace* trace);
 
 /* qe_aggregate_impl (query_exec.c:10258) for the hot-path shapes: optional
- * WHERE, no key or one GROUP BY column, up to 8 aggregates (arg_cols[e] < 0
- * = COUNT(*)).  Writes out_group_count groups: out_keys[g] (key value as
- * F64), out_cells[g*E+e] (U64 bits for COUNT, F64 bits otherwise).  Groups
- * are in ascending key order on the dense path and in hash-table slot order
- * on the hash path (trace->path QE_AggPath_Hash: keys outside the dense
- * range, non-32-bit keys, or a WHERE that is not an AND of comparisons). */
-B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, S32 group_col,
-                 const U32* funcs, const S32* arg_cols, U32 expr_count, U64 cap,
+ * WHERE, 0..QE_AGG_MAX_GROUP_COLS GROUP BY columns (group_cols, K =
+ * group_count; query_exec.c:10280-10299), up to 8 aggregates (arg_cols[e]
+ * < 0 = COUNT(*)).  Writes out_group_count groups: out_keys[g*K+k] (key
+ * values as F64), out_cells[g*E+e] (U64 bits for COUNT, F64 bits
+ * otherwise).  Up to 3 32-bit integer keys whose zone-map ranges multiply
+ * to a small range run dense (their digits make one group number, as
+ * query_exec.c:9462-9500 does; groups in ascending key order, key 0 most
+ * significant); other keys run through the hash table (groups in table-
+ * slot order, trace->path QE_AggPath_Hash), as does a WHERE that is not an
+ * AND of comparisons.  A SUM over a whole-number column that reaches 2^53
+ * fails the query (qe_aggregate_check_sum_exact, query_exec.c:7028-7049;
+ * setting QE_WIDE_INT_CHECK, default 1). */
+B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, const S32* group_cols,
+                 U32 group_count, const U32* funcs, const S32* arg_cols, U32 expr_count, U64 cap,
                  U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace);
 
 /* qe_aggregate_join_pushdown (query_exec.c:8442-8775):
- *   SELECT [dim.g,] aggs(fact.*) FROM fact JOIN dim ON fact.k = dim.k
- *   [WHERE dim-predicate] [GROUP BY dim.g]
- * dim_group_col < 0 means one group.  Output as qe_aggregate, groups in
- * ascending group value order. */
+ *   SELECT [dim.g1, ...,] aggs(fact.*) FROM fact JOIN dim ON fact.k = dim.k
+ *   [WHERE dim-predicate] [GROUP BY dim.g1, ...]
+ * with 0..QE_AGG_MAX_GROUP_COLS U32/I32 dim group columns (K =
+ * dim_group_count, query_exec.c:8588-8607; 0 means one group).  Output as
+ * qe_aggregate (out_keys[g*K+k]), groups in ascending order of their
+ * values, column 0 first.  A fact-side SUM reaching 2^53 fails the query
+ * as in qe_aggregate. */
 B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_key_col,
-                      const QE_IRNode* dim_where, U32 dim_where_count, S32 dim_group_col,
-                      const U32* funcs, const S32* fact_arg_cols, U32 expr_count, U64 cap,
+                      const QE_IRNode* dim_where, U32 dim_where_count, const S32* dim_group_cols,
+                      U32 dim_group_count, const U32* funcs, const S32* fact_arg_cols, U32 expr_count, U64 cap,
                       U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace);
 
 /* Join types of qe_hash_join (the reference's join_type strings "inner",
@@ -490,21 +509,23 @@ U32 qe_scan_materialize(QE_Table* table, const QE_IRNode* where, U32 where_count
  * columns.  Each query is what qe_aggregate takes, and each result is what
  * qe_aggregate returns for it alone (f64 sums within 1e-12 relative: the
  * summation order differs).  Queries fuse when the WHERE is an AND of at
- * most 8 comparisons, the GROUP BY key (if any) is a 32-bit integer column
- * whose zone-map range is at most QE_BATCH_MAX_GROUPS (1024), and they
- * take at most 2 distinct argument columns; other queries run through
- * qe_aggregate inside the same call.  `ok` is set per query. */
+ * most 8 comparisons, there is at most one GROUP BY key and it is a 32-bit
+ * integer column whose zone-map range is at most QE_BATCH_MAX_GROUPS
+ * (1024), and they take at most 2 distinct argument columns; other
+ * queries (several keys among them) run through qe_aggregate inside the
+ * same call.  `ok` is set per query. */
 typedef struct QE_BatchQuery {
   const QE_IRNode* where;
   U32 where_count;
-  S32 group_col;
+  U32 group_count;                       /* GROUP BY columns, 0..QE_AGG_MAX_GROUP_COLS */
+  S32 group_cols[QE_AGG_MAX_GROUP_COLS];
   const U32* funcs;
   const S32* arg_cols;
   U32 expr_count;
   U64 cap;
   U64* out_group_count;
-  F64* out_keys;
-  U64* out_cells;
+  F64* out_keys;  /* cap x group_count key values (may be 0) */
+  U64* out_cells; /* cap x expr_count cells */
   B32 ok;
   U32 fused; /* out: 1 when the query ran in a shared scan */
 } QE_BatchQuery;

@@ -612,8 +612,10 @@ orc_u64 orc_aggregate_cpu_reduce(orc_u64 n, const orc_u64* rows, const orc_f64* 
   return groups;
 }
 
+/* qe_aggregate_impl's GROUP BY over 0..4 key columns (query_exec.c:
+ * 10280-10299 gathers each key to f64 and hashes the tuple, :7201-7234) */
 orc_u64 orc_group_aggregate(const orc_column* cols, orc_u64 rows, const orc_ir_node* where,
-                            orc_u32 where_count, int key_col, const orc_u32* funcs,
+                            orc_u32 where_count, const int* key_cols, orc_u32 key_count, const orc_u32* funcs,
                             const int* arg_cols, orc_u32 expr_count, orc_f64* out_keys,
                             orc_f64* out_res, orc_u64 max_groups)
 {
@@ -623,10 +625,11 @@ orc_u64 orc_group_aggregate(const orc_column* cols, orc_u64 rows, const orc_ir_n
     sel = (orc_u64*)malloc(sizeof(orc_u64) * (rows ? rows : 1));
     n = orc_cpu_scan_filter(where, where_count, cols, rows, sel, rows, 1);
   }
-  orc_f64* keyv = 0;
-  if (key_col >= 0) {
-    keyv = (orc_f64*)malloc(sizeof(orc_f64) * (n ? n : 1));
-    for (orc_u64 i = 0; i < n; i++) keyv[i] = orc_load(&cols[key_col], sel ? sel[i] : i);
+  orc_f64* keyv[4] = {0, 0, 0, 0};
+  if (key_count > 4) key_count = 4;
+  for (orc_u32 k = 0; k < key_count; k++) {
+    keyv[k] = (orc_f64*)malloc(sizeof(orc_f64) * (n ? n : 1));
+    for (orc_u64 i = 0; i < n; i++) keyv[k][i] = orc_load(&cols[key_cols[k]], sel ? sel[i] : i);
   }
   orc_f64** argv = (orc_f64**)calloc(expr_count ? expr_count : 1, sizeof(orc_f64*));
   for (orc_u32 e = 0; e < expr_count; e++) {
@@ -636,23 +639,22 @@ orc_u64 orc_group_aggregate(const orc_column* cols, orc_u64 rows, const orc_ir_n
   }
   orc_u64* reps = (orc_u64*)malloc(sizeof(orc_u64) * (n ? n : 1));
   orc_f64* res = (orc_f64*)malloc(sizeof(orc_f64) * (n ? n : 1) * (expr_count ? expr_count : 1));
-  const orc_f64* kp[1] = {keyv};
-  orc_u64 g = orc_aggregate_cpu_reduce(n, 0, kp, key_col >= 0 ? 1 : 0, funcs, (const orc_f64* const*)argv,
-                                       expr_count, reps, res);
+  const orc_f64* kp[4] = {keyv[0], keyv[1], keyv[2], keyv[3]};
+  orc_u64 g = orc_aggregate_cpu_reduce(n, 0, kp, key_count, funcs, (const orc_f64* const*)argv, expr_count, reps, res);
   /* an ungrouped aggregate over no rows still yields one row of zeros
    * (query_exec.c:10335-10346) */
-  if (key_col < 0 && g == 0) {
+  if (key_count == 0 && g == 0) {
     g = 1;
     for (orc_u32 e = 0; e < expr_count; e++) res[e] = 0.0;
     reps[0] = 0;
   }
   orc_u64 out_g = g < max_groups ? g : max_groups;
   for (orc_u64 i = 0; i < out_g; i++) {
-    if (out_keys) out_keys[i] = keyv && n ? keyv[reps[i]] : 0.0;
+    for (orc_u32 k = 0; out_keys && k < key_count; k++) out_keys[i * key_count + k] = n ? keyv[k][reps[i]] : 0.0;
     for (orc_u32 e = 0; e < expr_count; e++) out_res[i * expr_count + e] = res[i * expr_count + e];
   }
   free(sel);
-  free(keyv);
+  for (orc_u32 k = 0; k < 4; k++) free(keyv[k]);
   for (orc_u32 e = 0; e < expr_count; e++) free(argv[e]);
   free(argv);
   free(reps);
@@ -671,7 +673,6 @@ typedef struct {
   orc_u32 expr_count;
   const orc_u32* dim_key;
   const orc_u8* dim_pass;
-  const orc_u32* dim_group;
   orc_u32* heads; /* dim hash: slot -> first dim row + 1 */
   orc_u32* next;  /* dim row -> next dim row + 1 with the same slot */
   orc_u64 hash_mask;
@@ -717,15 +718,30 @@ static void orc_join_task(void* raw, orc_u64 t)
   }
 }
 
+/* the dim group columns of dim row d as one tuple (unused columns 0) */
+static void orc_dim_tuple(const orc_u32* const* dim_groups, orc_u32 group_count, orc_u64 d, orc_u32* out)
+{
+  for (orc_u32 k = 0; k < 4; k++) out[k] = k < group_count ? dim_groups[k][d] : 0u;
+}
+
+static orc_u64 orc_tuple_slot(const orc_u32* v, orc_u64 mask)
+{
+  orc_u64 h = 0;
+  for (orc_u32 k = 0; k < 4; k++) h = (h ^ v[k]) * 0x9E3779B97F4A7C15ULL;
+  return (h >> 29) & mask;
+}
+
 orc_u64 orc_join_aggregate(orc_u64 fact_rows, const orc_u32* fact_key, const orc_u32* const* fact_args,
                            const orc_u32* funcs, orc_u32 expr_count, orc_u64 dim_rows,
-                           const orc_u32* dim_key, const orc_u8* dim_pass, const orc_u32* dim_group,
-                           orc_f64* out_group_keys, orc_f64* out_res, orc_u64 max_groups, int threads)
+                           const orc_u32* dim_key, const orc_u8* dim_pass, const orc_u32* const* dim_groups,
+                           orc_u32 group_count, orc_f64* out_group_keys, orc_f64* out_res, orc_u64 max_groups,
+                           int threads)
 {
   orc_join_ctx c;
   memset(&c, 0, sizeof(c));
+  if (group_count > 4) group_count = 4;
   c.fact_rows = fact_rows; c.fact_key = fact_key; c.fact_args = fact_args; c.funcs = funcs;
-  c.expr_count = expr_count; c.dim_key = dim_key; c.dim_pass = dim_pass; c.dim_group = dim_group;
+  c.expr_count = expr_count; c.dim_key = dim_key; c.dim_pass = dim_pass;
   orc_u64 size = 64;
   while (size < dim_rows * 2 + 2) size <<= 1;
   c.hash_mask = size - 1;
@@ -737,26 +753,29 @@ orc_u64 orc_join_aggregate(orc_u64 fact_rows, const orc_u32* fact_key, const orc
     c.next[d - 1] = c.heads[s];
     c.heads[s] = (orc_u32)d;
   }
-  /* distinct group values of the dim rows (any order; output order is
-   * fixed below by first fact appearance) */
+  /* distinct group tuples of the dim rows (any order; output order is
+   * fixed below by first fact appearance): a group is a distinct
+   * combination of the group column values (query_exec.c:8621-8690) */
   c.group_of_dim = (orc_u32*)calloc(dim_rows ? dim_rows : 1, sizeof(orc_u32));
-  orc_u32* group_value = (orc_u32*)malloc(sizeof(orc_u32) * (dim_rows ? dim_rows : 1));
+  orc_u32* group_value = (orc_u32*)calloc(4 * (dim_rows ? dim_rows : 1), sizeof(orc_u32));
   c.group_count = 1;
-  if (dim_group) {
+  if (group_count) {
     orc_u64 gsize = 64;
     while (gsize < dim_rows * 2 + 2) gsize <<= 1;
     orc_u32* gslots = (orc_u32*)calloc(gsize, sizeof(orc_u32));
     c.group_count = 0;
     for (orc_u64 d = 0; d < dim_rows; d++) {
-      orc_u64 s = orc_dim_slot(dim_group[d], gsize - 1);
+      orc_u32 v[4];
+      orc_dim_tuple(dim_groups, group_count, d, v);
+      orc_u64 s = orc_tuple_slot(v, gsize - 1);
       for (;;) {
         if (gslots[s] == 0) {
-          group_value[c.group_count] = dim_group[d];
+          memcpy(&group_value[4 * c.group_count], v, sizeof(v));
           gslots[s] = (orc_u32)(++c.group_count);
           c.group_of_dim[d] = (orc_u32)(c.group_count - 1);
           break;
         }
-        if (group_value[gslots[s] - 1] == dim_group[d]) {
+        if (!memcmp(&group_value[4 * (gslots[s] - 1)], v, sizeof(v))) {
           c.group_of_dim[d] = gslots[s] - 1;
           break;
         }
@@ -764,8 +783,6 @@ orc_u64 orc_join_aggregate(orc_u64 fact_rows, const orc_u32* fact_key, const orc
       }
     }
     free(gslots);
-  } else {
-    group_value[0] = 0;
   }
   if (threads < 1) threads = 1;
   c.tasks = fact_rows < 65536 ? 1 : (orc_u64)threads * 4;
@@ -812,7 +829,8 @@ orc_u64 orc_join_aggregate(orc_u64 fact_rows, const orc_u32* fact_key, const orc
     if (best == ~(orc_u64)0) break;
     first[bg] = ~(orc_u64)0;
     if (out < max_groups) {
-      if (out_group_keys) out_group_keys[out] = (orc_f64)group_value[bg];
+      for (orc_u32 k = 0; out_group_keys && k < group_count; k++)
+        out_group_keys[out * group_count + k] = (orc_f64)group_value[4 * bg + k];
       for (orc_u64 e = 0; e < E; e++) out_res[out * E + e] = orc_finish(funcs[e], &tot[bg * E + e]);
     }
     out++;
@@ -826,6 +844,24 @@ orc_u64 orc_join_aggregate(orc_u64 fact_rows, const orc_u32* fact_key, const orc
   free(c.group_of_dim);
   free(group_value);
   return out;
+}
+
+/* qe_aggregate_check_sum_exact, query_exec.c:7028-7049 (called for every
+ * result column, :7083): a SUM over a whole-number column (U32/I32/U64/I64)
+ * whose value reached 2^53 fails the query.  Returns 1 when every such
+ * SUM stays below 2^53 in magnitude. */
+int orc_check_sum_exact(const orc_u32* funcs, const orc_u32* arg_types, orc_u32 expr_count, const orc_f64* res,
+                        orc_u64 groups)
+{
+  for (orc_u32 e = 0; e < expr_count; e++) {
+    const orc_u32 t = arg_types[e];
+    if (funcs[e] != ORC_AGG_SUM || (t != ORC_TYPE_U32 && t != ORC_TYPE_I32 && t != ORC_TYPE_U64 && t != ORC_TYPE_I64)) continue;
+    for (orc_u64 g = 0; g < groups; g++) {
+      const orc_f64 v = res[g * expr_count + e];
+      if (v >= 9007199254740992.0 || v <= -9007199254740992.0) return 0;
+    }
+  }
+  return 1;
 }
 
 /* ------------------------------------------------------------------ */

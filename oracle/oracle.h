@@ -129,21 +129,28 @@ orc_u64 orc_aggregate_cpu_reduce(orc_u64 n, const orc_u64* rows, const orc_f64* 
  * (IR) -- loads values as f64 the way qe_row_load_value does and calls
  * orc_aggregate_cpu_reduce.  arg_cols[e] < 0 means COUNT(*). */
 orc_u64 orc_group_aggregate(const orc_column* cols, orc_u64 rows, const orc_ir_node* where,
-                            orc_u32 where_count, int key_col, const orc_u32* funcs,
+                            orc_u32 where_count, const int* key_cols, orc_u32 key_count, const orc_u32* funcs,
                             const int* arg_cols, orc_u32 expr_count, orc_f64* out_keys,
                             orc_f64* out_res, orc_u64 max_groups);
+/* qe_aggregate_check_sum_exact (query_exec.c:7028-7049): 0 when a SUM over
+ * a whole-number argument type reached 2^53 */
+int orc_check_sum_exact(const orc_u32* funcs, const orc_u32* arg_types, orc_u32 expr_count, const orc_f64* res,
+                        orc_u64 groups);
 
 /* Join + aggregate, the semantics of qe_aggregate_join_pushdown
  * (query_exec.c:8442-8775) and of the general path it stands in for
  * (qe_hash_join_impl :14541 then qe_aggregate): every fact row whose key
  * equals the key of a dim row that passes the dim filter contributes to the
- * group of that dim row's group column (dim_group may be 0: one group).
+ * group of that dim row's group columns (group_count of them, 0..4: a
+ * group is one combination of their values; 0 columns: one group); group
+ * keys come out group_count values per group.
  * Fact-side aggregates over fact_arg columns (0 = COUNT).  Groups in first-
  * appearance order over fact rows.  Returns group count. */
 orc_u64 orc_join_aggregate(orc_u64 fact_rows, const orc_u32* fact_key, const orc_u32* const* fact_args,
                            const orc_u32* funcs, orc_u32 expr_count, orc_u64 dim_rows,
-                           const orc_u32* dim_key, const orc_u8* dim_pass, const orc_u32* dim_group,
-                           orc_f64* out_group_keys, orc_f64* out_res, orc_u64 max_groups, int threads);
+                           const orc_u32* dim_key, const orc_u8* dim_pass, const orc_u32* const* dim_groups,
+                           orc_u32 group_count, orc_f64* out_group_keys, orc_f64* out_res, orc_u64 max_groups,
+                           int threads);
 
 /* multithreaded numeric-AND bitmask count (qe_scan_filter_chunked_cpu,
  * query_exec.c:1240-1367, count_only) for leaves col op target (op 0..4).

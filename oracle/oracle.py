@@ -22,6 +22,11 @@ _TYPE_OF = {np.dtype(v): k for k, v in _NP.items()}
 _lib = None
 
 
+class ExactnessError(ArithmeticError):
+    """A SUM over a whole-number column reached 2^53: the CPU path fails
+    the query (qe_aggregate_check_sum_exact, query_exec.c:7028-7049)."""
+
+
 class OrcColumn(C.Structure):
     _fields_ = [("type", C.c_uint32), ("pad", C.c_uint32), ("data", C.c_void_p), ("null_flags", C.c_void_p)]
 
@@ -57,8 +62,9 @@ def load():
         "orc_scan_int_interval": (i32, [u32, f64, f64, f64, P, P]),
         "orc_aggregate_global_chunked": (None, [P, u32, u64, i32, P, P, P]),
         "orc_aggregate_cpu_reduce": (u64, [u64, P, P, u32, P, P, u32, P, P]),
-        "orc_group_aggregate": (u64, [P, u64, P, u32, i32, P, P, u32, P, P, u64]),
-        "orc_join_aggregate": (u64, [u64, P, P, P, u32, u64, P, P, P, P, P, u64, i32]),
+        "orc_group_aggregate": (u64, [P, u64, P, u32, P, u32, P, P, u32, P, P, u64]),
+        "orc_join_aggregate": (u64, [u64, P, P, P, u32, u64, P, P, P, u32, P, P, u64, i32]),
+        "orc_check_sum_exact": (i32, [P, P, u32, P, u64]),
         "orc_scan_chunked_count": (u64, [P, P, P, P, u32, u64, i32]),
         "orc_hash_join": (u64, [P, P, u64, P, P, u64, u32, P, u64]),
         "orc_dense_group_u32": (None, [P, P, u64, u32, u32, i32, P, P, P, P]),
@@ -166,30 +172,50 @@ def aggregate_global(cols, workers: int | None = None):
     return s, mn, mx
 
 
-def group_aggregate(cols, key_col, funcs, arg_cols, where=None, max_groups=None, nulls=None):
+def _check_exact(funcs, arg_types, res, groups, check):
+    if not check or not groups:
+        return
+    fa = np.ascontiguousarray(funcs, dtype=np.uint32)
+    ta = np.ascontiguousarray(arg_types, dtype=np.uint32)
+    r = np.ascontiguousarray(res[:groups], dtype=np.float64)
+    if not load().orc_check_sum_exact(fa.ctypes.data, ta.ctypes.data, len(fa), r.ctypes.data, groups):
+        raise ExactnessError("a SUM over a whole-number column reached 2^53")
+
+
+def group_aggregate(cols, key_col, funcs, arg_cols, where=None, max_groups=None, nulls=None, check_exact=True):
     """qe_aggregate_cpu_reduce over the (filtered) rows: (keys, results[g, e]),
-    groups in first-appearance order. key_col None = one group; nulls as
-    for scan (WHERE columns only: NULL arguments are not aggregated here)."""
+    groups in first-appearance order.  key_col: None = one group, a column
+    (keys shape (g,)), or a list of up to 4 columns (keys shape (g, K));
+    nulls as for scan (WHERE columns only: NULL arguments are not
+    aggregated here).  Raises ExactnessError when a SUM over a whole-number
+    column reaches 2^53 (check_exact False returns the rounded value)."""
     arr, keep = _columns(cols, nulls)
     nodes, nn = _nodes(where)
     rows = len(cols[0])
     e = len(funcs)
+    kl = [] if key_col is None else (list(key_col) if isinstance(key_col, (list, tuple)) else [key_col])
+    ka = np.array(kl or [0], dtype=np.int32)
     fa = np.array(funcs, dtype=np.uint32)
     aa = np.array([-1 if a is None else a for a in arg_cols], dtype=np.int32)
     cap = max_groups or max(rows, 1)
-    keys = np.empty(cap)
+    keys = np.empty(cap * max(len(kl), 1))
     res = np.empty((cap, e))
-    g = load().orc_group_aggregate(arr, rows, nodes.ctypes.data if nodes is not None else None, nn,
-                                   -1 if key_col is None else key_col, fa.ctypes.data, aa.ctypes.data, e,
-                                   keys.ctypes.data, res.ctypes.data, cap)
+    g = load().orc_group_aggregate(arr, rows, nodes.ctypes.data if nodes is not None else None, nn, ka.ctypes.data,
+                                   len(kl), fa.ctypes.data, aa.ctypes.data, e, keys.ctypes.data, res.ctypes.data, cap)
+    types = [0 if a is None else _TYPE_OF[np.asarray(cols[a]).dtype] for a in arg_cols]
     del keep
-    return keys[:g].copy(), res[:g].copy()
+    _check_exact(funcs, types, res, min(g, cap), check_exact)
+    if isinstance(key_col, (list, tuple)):
+        return keys[: g * len(kl)].reshape(g, len(kl)).copy(), res[:g].copy()
+    return (keys[:g].copy() if kl else np.zeros(g)), res[:g].copy()
 
 
 def join_aggregate(fact_key, fact_args, funcs, dim_key, dim_pass=None, dim_group=None, nthreads=None,
-                   max_groups=1 << 20):
+                   max_groups=1 << 20, check_exact=True):
     """Join + aggregate semantics: (group values, results[g, e]) in first
-    fact-appearance order."""
+    fact-appearance order.  dim_group: None (one group), one u32 array
+    (keys shape (g,)) or a list of up to 4 (keys shape (g, K)).  Raises
+    ExactnessError when a SUM reaches 2^53, as group_aggregate."""
     fk = np.ascontiguousarray(fact_key, dtype=np.uint32)
     e = len(funcs)
     args = [None if a is None else np.ascontiguousarray(a, dtype=np.uint32) for a in fact_args]
@@ -197,13 +223,20 @@ def join_aggregate(fact_key, fact_args, funcs, dim_key, dim_pass=None, dim_group
     fa = np.array(funcs, dtype=np.uint32)
     dk = np.ascontiguousarray(dim_key, dtype=np.uint32)
     dp = None if dim_pass is None else np.ascontiguousarray(dim_pass, dtype=np.uint8)
-    dg = None if dim_group is None else np.ascontiguousarray(dim_group, dtype=np.uint32)
-    keys = np.empty(max_groups)
+    many = isinstance(dim_group, (list, tuple))
+    gl = [] if dim_group is None else (list(dim_group) if many else [dim_group])
+    gl = [np.ascontiguousarray(x, dtype=np.uint32) for x in gl]
+    gp = (C.c_void_p * max(len(gl), 1))(*[x.ctypes.data for x in gl])
+    k = max(len(gl), 1)
+    keys = np.empty(max_groups * k)
     res = np.empty((max_groups, e))
     g = load().orc_join_aggregate(len(fk), fk.ctypes.data, argp, fa.ctypes.data, e, len(dk), dk.ctypes.data,
-                                  None if dp is None else dp.ctypes.data, None if dg is None else dg.ctypes.data,
-                                  keys.ctypes.data, res.ctypes.data, max_groups, nthreads or threads())
-    return keys[:g].copy(), res[:g].copy()
+                                  None if dp is None else dp.ctypes.data, gp, len(gl), keys.ctypes.data,
+                                  res.ctypes.data, max_groups, nthreads or threads())
+    _check_exact(funcs, [0 if a is None else U32 for a in fact_args], res, min(g, max_groups), check_exact)
+    if many:
+        return keys[: g * len(gl)].reshape(g, len(gl)).copy(), res[:g].copy()
+    return (keys[:g].copy() if gl else np.zeros(g)), res[:g].copy()
 
 
 JOIN_TYPES = {"inner": 0, "left": 1, "semi": 2, "anti": 3}

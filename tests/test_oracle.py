@@ -267,3 +267,102 @@ def test_dense_group_u32_against_numpy(orc, workers):
     cnt0, s0, mn0, mx0 = orc.dense_group_u32(key, None, key_min, rng_, workers=workers)
     assert np.array_equal(cnt0, ecnt) and not s0.any() and not mx0.any()
     assert np.array_equal(mn0, np.where(ecnt > 0, 0, 0xFFFFFFFF).astype(np.uint32))
+
+
+def _numpy_groups(keys, vals, where_mask=None):
+    """Brute force: per distinct key tuple, count / sum / min / max of vals
+    (python ints, so sums are exact), in first-appearance order."""
+    rows = np.flatnonzero(where_mask) if where_mask is not None else np.arange(len(keys[0]))
+    out = {}
+    for r in rows.tolist():
+        k = tuple(float(c[r]) for c in keys)
+        v = int(vals[r])
+        if k not in out:
+            out[k] = [0, 0, v, v]
+        a = out[k]
+        a[0] += 1
+        a[1] += v
+        a[2] = min(a[2], v)
+        a[3] = max(a[3], v)
+    return out
+
+
+@pytest.mark.parametrize("nkeys", [2, 3, 4])
+def test_group_aggregate_multi_key_against_numpy(orc, nkeys):
+    r = np.random.default_rng(7 + nkeys)
+    n = 20_011
+    a = r.integers(0, 7, n).astype(np.uint32)
+    b = r.integers(-3, 4, n).astype(np.int32)
+    c = r.integers(0, 5, n).astype(np.uint32)
+    d = (r.integers(0, 3, n) * 0.5).astype(np.float64)
+    v = r.integers(0, 1_000_000, n).astype(np.uint32)
+    cols = [a, b, c, d, v]
+    keys = list(range(nkeys))
+    where = col(4) < 900_000
+    funcs = [COUNT, SUM, MIN, MAX]
+    rk, rv = orc.group_aggregate(cols, keys, funcs, [None, 4, 4, 4], where=where)
+    expect = _numpy_groups([cols[k] for k in keys], v, v < 900_000)
+    assert rk.shape == (len(expect), nkeys)
+    # first-appearance order, as the CPU path emits groups
+    assert [tuple(x) for x in rk.tolist()] == list(expect)
+    for i, kt in enumerate(expect):
+        assert rv[i].tolist() == [float(x) for x in expect[kt]]
+    # a single key given as a list keeps its (g, 1) shape; as a column (g,)
+    k1, _ = orc.group_aggregate(cols, [0], [COUNT], [None])
+    k0, _ = orc.group_aggregate(cols, 0, [COUNT], [None])
+    assert k1.shape == (7, 1) and k0.shape == (7,) and np.array_equal(k1[:, 0], k0)
+
+
+def test_join_aggregate_multi_group_against_numpy(orc):
+    r = np.random.default_rng(3)
+    dn, n = 500, 50_003
+    dk = r.permutation(10_000)[:dn].astype(np.uint32)
+    g1 = r.integers(0, 4, dn).astype(np.uint32)
+    g2 = r.integers(0, 3, dn).astype(np.uint32)
+    keep = r.random(dn) < 0.8
+    fk = np.where(r.random(n) < 0.9, dk[r.integers(0, dn, n)], 20_000).astype(np.uint32)
+    amt = r.integers(0, 10_000, n).astype(np.uint32)
+    rk, rv = orc.join_aggregate(fk, [None, amt, amt], [COUNT, SUM, MAX], dk, dim_pass=keep, dim_group=[g1, g2])
+    row_of = {int(k): i for i, k in enumerate(dk.tolist())}
+    groups = {}
+    for f, x in zip(fk.tolist(), amt.tolist()):
+        d = row_of.get(f)
+        if d is None or not keep[d]:
+            continue
+        t = (float(g1[d]), float(g2[d]))
+        s = groups.setdefault(t, [0, 0, -1])
+        s[0] += 1
+        s[1] += x
+        s[2] = max(s[2], x)
+    assert rk.shape == (len(groups), 2)
+    got = {tuple(k): v.tolist() for k, v in zip(rk.tolist(), rv)}
+    assert got == {k: [float(c) for c in v] for k, v in groups.items()}
+
+
+def test_sum_exactness_fails_at_2_pow_53(orc):
+    """qe_aggregate_check_sum_exact (query_exec.c:7028-7049): a SUM of a
+    whole-number column reaching 2^53 fails the query; below it, and for
+    COUNT/AVG/MIN/MAX or float columns, results stand."""
+    top = np.full(2_097_153, 0xFFFFFFFF, dtype=np.uint32)      # sum = 2^53 + 2^21 - 2097153 > 2^53
+    below = top[: 2_097_151]                                   # sum < 2^53
+    assert (2_097_153 * 0xFFFFFFFF) >= 2**53 > 2_097_151 * 0xFFFFFFFF
+    with pytest.raises(orc.ExactnessError):
+        orc.group_aggregate([top], None, [SUM], [0])
+    k, v = orc.group_aggregate([below], None, [SUM, COUNT], [0, None])
+    assert v[0, 0] == float(2_097_151 * 0xFFFFFFFF) and v[0, 1] == 2_097_151
+    for f in (COUNT, AVG, MIN, MAX):
+        orc.group_aggregate([top], None, [f], [0])                 # no error
+    orc.group_aggregate([top.astype(np.float64)], None, [SUM], [0])  # floats are not checked
+    with pytest.raises(orc.ExactnessError):
+        orc.group_aggregate([np.array([2**53], dtype=np.int64)], None, [SUM], [0])
+    with pytest.raises(orc.ExactnessError):
+        orc.group_aggregate([np.array([-(2**53)], dtype=np.int64)], None, [SUM], [0])
+    grp = np.zeros(len(top), dtype=np.uint32)
+    with pytest.raises(orc.ExactnessError):
+        orc.group_aggregate([grp, top], 0, [COUNT, SUM], [None, 1])
+    # the planted mistake a check must catch: the guard switched off gives a
+    # rounded value that differs from the exact sum
+    _, v = orc.group_aggregate([top], None, [SUM], [0], check_exact=False)
+    assert v[0, 0] >= 2**53 and int(v[0, 0]) != 2_097_153 * 0xFFFFFFFF
+    with pytest.raises(orc.ExactnessError):
+        orc.join_aggregate(np.zeros(len(top), dtype=np.uint32), [top], [SUM], np.zeros(1, dtype=np.uint32))
