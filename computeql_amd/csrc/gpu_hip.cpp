@@ -287,6 +287,89 @@ bool prepare(GPU_Kernel* k, U32 global_work_size, U32 local_work_size, PreparedL
   return true;
 }
 
+/* ---- the no-allocation scratch of qe_aggregate_noalloc ------------------
+ * A file-scope device array of QE_NOALLOC_SCRATCH_BYTES: each GPU the
+ * library runs on holds its own copy until the process ends.  Host code
+ * never uses the array's name as an address (that is a host-side stand-in):
+ * gpu_hip_init takes each device's address once with hipGetSymbolAddress
+ * and keeps it, with a stream of default priority, per device. */
+static __device__ __align__(256) unsigned char scratch[QE_NOALLOC_SCRATCH_BYTES];
+
+#define QK_NOALLOC_MAX_DEVICES 64
+struct NoallocDevice {
+  void* ptr;
+  hipStream_t stream;
+};
+static NoallocDevice g_noalloc[QK_NOALLOC_MAX_DEVICES];
+
+/* once per device, with that device current */
+static bool noalloc_setup(int device)
+{
+  if (device < 0 || device >= QK_NOALLOC_MAX_DEVICES) return false;
+  if (g_noalloc[device].ptr) return true;
+  void* p = nullptr;
+  if (!check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)") || !p) return false;
+  hipStream_t h = nullptr;
+  if (!check(hipStreamCreateWithFlags(&h, hipStreamNonBlocking), "hipStreamCreate")) return false;
+  g_noalloc[device].stream = h;
+  g_noalloc[device].ptr = p;
+  return true;
+}
+
+extern "C" U64 gpu_hip_noalloc_scratch(void)
+{
+  HipState& s = st();
+  if (!s.initialised || s.device < 0 || s.device >= QK_NOALLOC_MAX_DEVICES) return 0;
+  return (U64)(uintptr_t)g_noalloc[s.device].ptr;
+}
+
+/* upload `upload_bytes` to the scratch at upload_offset, launch the library
+ * kernels in order with their args (QK_Args by value, `blocks[i]` blocks of
+ * the kernel's fixed size) and read `download_bytes` at download_offset
+ * back, all on the device's no-allocation stream; returns when done.
+ * Nothing is allocated: no buffers, no staging, no events. */
+extern "C" B32 gpu_hip_noalloc_run(const void* upload, U64 upload_bytes, U64 upload_offset, const char* const* kernels,
+                                   const QK_Args* args, const U32* blocks, U32 steps, void* download,
+                                   U64 download_bytes, U64 download_offset)
+{
+  HipState& s = st();
+  if (!s.initialised || s.device < 0 || s.device >= QK_NOALLOC_MAX_DEVICES || !g_noalloc[s.device].ptr) {
+    set_error("no-allocation scratch is not set up on this device");
+    return 0;
+  }
+  const NoallocDevice& d = g_noalloc[s.device];
+  if (upload_offset + upload_bytes > QE_NOALLOC_SCRATCH_BYTES ||
+      download_offset + download_bytes > QE_NOALLOC_SCRATCH_BYTES) {
+    set_error("no-allocation scratch: %lld bytes past its end", "",
+              (long long)(upload_offset + upload_bytes > download_offset + download_bytes
+                              ? upload_offset + upload_bytes : download_offset + download_bytes));
+    return 0;
+  }
+  char* base = (char*)d.ptr;
+  bool ok = true;
+  if (upload_bytes)
+    ok = check(hipMemcpyAsync(base + upload_offset, upload, upload_bytes, hipMemcpyHostToDevice, d.stream),
+               "noalloc upload");
+  for (U32 i = 0; ok && i < steps; i++) {
+    const QK_LibraryKernel* lib = qk_library_find(kernels[i]);
+    if (!lib) {
+      set_error("no kernel named %s", kernels[i]);
+      ok = false;
+      break;
+    }
+    QK_Args a = args[i];
+    void* params[1] = {&a};
+    ok = check(hipLaunchKernel(lib->host_fn, dim3(blocks[i]), dim3(lib->block), params, 0, d.stream), kernels[i]);
+  }
+  if (ok && download_bytes)
+    ok = check(hipMemcpyAsync(download, base + download_offset, download_bytes, hipMemcpyDeviceToHost, d.stream),
+               "noalloc download");
+  /* drain the stream even after a failed step, so nothing of this call is
+   * still in flight when the next one reuses the scratch */
+  const bool synced = check(hipStreamSynchronize(d.stream), "noalloc sync");
+  return ok && synced ? 1 : 0;
+}
+
 hipError_t enqueue(const PreparedLaunch& p, hipStream_t stream)
 {
   QK_Args args = p.args;
@@ -427,6 +510,9 @@ void gpu_hip_init(void)
   if (!check(hipSetDevice(s.device), "hipSetDevice")) return;
   if (!check(hipGetDeviceProperties(&s.props, s.device), "hipGetDeviceProperties")) return;
   if (!check(hipStreamCreateWithFlags(&s.default_stream, hipStreamNonBlocking), "hipStreamCreate")) return;
+  /* the no-allocation entry point's scratch address on this device; on
+   * failure that entry point reports it and every other one works */
+  noalloc_setup(s.device);
   s.initialised = true;
 }
 

@@ -95,85 +95,44 @@ static F64 qe_sum_to_f64(U32 type, U64 sum)
 }
 
 /* ---------------- global aggregate (one pass, optional fused WHERE) ---------------- */
-static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune,
-                               const QE_AggPlan* p, U64* out_cells, QE_Trace* trace)
-{
-  GPU_Kernel* k = qe_kernel("aggregate_global");
-  GPU_Kernel* kf = qe_kernel("aggregate_global_fold");
-  if (!k || !kf) return 0;
+/* the columns, arg slots / type bytes and leaf words aggregate_global takes */
+typedef struct QE_GlobalInputs {
   QE_ColumnSlots slots;
-  memset(&slots, 0, sizeof(slots));
-  U32 arg_slot[QE_AGG_MAX_ARGS];
-  U32 slot_bits = 0, type_bits = 0;
-  /* encoded columns (a3) are read as codes: an encoded arg is a U32 column
-   * of codes (type byte U32 | encoding << 4) whose base is added back below */
-  for (U32 a = 0; a < p->arg_count; a++) {
-    arg_slot[a] = qe_slot_of(&slots, p->arg_col[a]);
-    slot_bits |= arg_slot[a] << (4 * a);
-    U32 enc = qe_col_enc(t, p->arg_col[a]);
-    type_bits |= (enc ? (CQL_ColumnType_U32 | (enc << 4)) : t->columns[p->arg_col[a]].type) << (8 * a);
-  }
+  U32 slot_bits, type_bits;
   U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
+} QE_GlobalInputs;
+
+static B32 qe_global_prepare(QE_Table* t, const QE_Leaf* leaves, U32 leaf_count, const QE_AggPlan* p,
+                             QE_GlobalInputs* in)
+{
+  memset(in, 0, sizeof(*in));
+  /* encoded columns (a3) are read as codes: an encoded arg is a U32 column
+   * of codes (type byte U32 | encoding << 4) whose base is added back by
+   * qe_global_decode */
+  for (U32 a = 0; a < p->arg_count; a++) {
+    const U32 slot = qe_slot_of(&in->slots, p->arg_col[a]);
+    in->slot_bits |= slot << (4 * a);
+    U32 enc = qe_col_enc(t, p->arg_col[a]);
+    in->type_bits |= (enc ? (CQL_ColumnType_U32 | (enc << 4)) : t->columns[p->arg_col[a]].type) << (8 * a);
+  }
   for (U32 l = 0; l < leaf_count; l++) {
-    U32 slot = qe_slot_of(&slots, leaves[l].column);
+    U32 slot = qe_slot_of(&in->slots, leaves[l].column);
     if (slot == ~0u) {
       qe_set_error("aggregate: too many columns");
       return 0;
     }
-    qe_simple_leaf_words(&leaf_words[6 * l], slot, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
-    qe_leaf_nulls(&leaf_words[6 * l], t, leaves[l].column);
-    qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(t, leaves[l].column), t->columns[leaves[l].column].enc_base);
+    qe_simple_leaf_words(&in->leaf_words[6 * l], slot, t->columns[leaves[l].column].type, leaves[l].op,
+                         leaves[l].target);
+    qe_leaf_nulls(&in->leaf_words[6 * l], t, leaves[l].column);
+    qe_leaf_encode(&in->leaf_words[6 * l], qe_col_enc(t, leaves[l].column), t->columns[leaves[l].column].enc_base);
   }
-  U64 r0 = prune->row_begin, r1 = prune->row_end;
-  U64 quads = (r1 + 3) / 4 - r0 / 4;
-  U32 blocks = qe_stream_blocks(quads);
-  GPU_Buffer* result = qe_scratch("agg_global_result", 16 * 8);
-  GPU_Buffer* partials = qe_scratch("agg_global_partials", (U64)blocks * 13 * 8);
-  GPU_Buffer* leaf_buf = leaf_count ? qe_scratch("agg_global_leaves", sizeof(leaf_words)) : 0;
-  GPU_Buffer* skip_buf = prune->skip_bits ? qe_scratch("agg_global_skip", prune->skip_words * 4) : 0;
-  B32 ok = result && partials && (!leaf_count || leaf_buf) && (!prune->skip_bits || skip_buf);
-  U64 res[16];
-  memset(res, 0, sizeof(res));
-  if (ok) {
-    if (leaf_buf) gpu_kernel_set_arg_buffer(k, 0, leaf_buf);
-    else gpu_hip_kernel_clear_arg(k, 0);
-    gpu_kernel_set_arg_buffer(k, 1, result);
-    gpu_kernel_set_arg_buffer(k, 2, partials);
-    if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
-    else gpu_hip_kernel_clear_arg(k, 5);
-    for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
-      if (s < slots.count) gpu_kernel_set_arg_buffer(k, 6 + s, qe_col_kbuf(t, slots.column[s]));
-      else gpu_hip_kernel_clear_arg(k, 6 + s);
-    }
-    gpu_kernel_set_arg_u64(k, 0, r1 - r0);
-    gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
-    gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
-    gpu_kernel_set_arg_u64(k, 5, r0);
-    gpu_kernel_set_arg_u64(k, 7, leaf_count);
-    qe_kernel_set_word(k, QK_W_ARGS, p->arg_count | (slot_bits << 8));
-    qe_kernel_set_word(k, QK_W_TYPES, type_bits);
-    /* per-block partials, then one block folds them in block order */
-    gpu_kernel_set_arg_buffer(kf, 1, result);
-    gpu_kernel_set_arg_buffer(kf, 2, partials);
-    gpu_kernel_set_arg_u64(kf, 0, blocks);
-    qe_kernel_set_word(kf, QK_W_ARGS, p->arg_count);
-    qe_kernel_set_word(kf, QK_W_TYPES, type_bits);
-    GPU_Batch* b = gpu_batch_begin(0, sizeof(res));
-    if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
-    if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
-    gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
-    gpu_batch_kernel_execute(b, kf, QE_WG, QE_WG);
-    gpu_batch_buffer_read(b, result, res, sizeof(res));
-    ok = gpu_batch_end(b);
-  }
-  qe_scratch_release(result);
-  qe_scratch_release(partials);
-  qe_scratch_release(leaf_buf);
-  qe_scratch_release(skip_buf);
-  if (!ok) {
-    qe_set_error("aggregate_global: %s", gpu_hip_last_error());
-    return 0;
-  }
+  return 1;
+}
+
+/* aggregate_global_fold's 13 result words (count, then sum/min/max per
+ * argument) into the query's cells */
+static void qe_global_decode(QE_Table* t, const QE_AggPlan* p, U64* res, U64* out_cells)
+{
   U64 count = res[0];
   /* encoded args: the kernel saw codes = value - base */
   for (U32 a = 0; a < p->arg_count && count; a++) {
@@ -205,16 +164,191 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
     }
     out_cells[e] = qe_agg_cell(p->funcs[e], count, sum, mn, mx);
   }
+}
+
+static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune,
+                               const QE_AggPlan* p, U64* out_cells, QE_Trace* trace)
+{
+  GPU_Kernel* k = qe_kernel("aggregate_global");
+  GPU_Kernel* kf = qe_kernel("aggregate_global_fold");
+  if (!k || !kf) return 0;
+  QE_GlobalInputs in;
+  if (!qe_global_prepare(t, leaves, leaf_count, p, &in)) return 0;
+  const QE_ColumnSlots slots = in.slots;
+  U64 r0 = prune->row_begin, r1 = prune->row_end;
+  U64 quads = (r1 + 3) / 4 - r0 / 4;
+  U32 blocks = qe_stream_blocks(quads);
+  GPU_Buffer* result = qe_scratch("agg_global_result", 16 * 8);
+  GPU_Buffer* partials = qe_scratch("agg_global_partials", (U64)blocks * 13 * 8);
+  GPU_Buffer* leaf_buf = leaf_count ? qe_scratch("agg_global_leaves", sizeof(in.leaf_words)) : 0;
+  GPU_Buffer* skip_buf = prune->skip_bits ? qe_scratch("agg_global_skip", prune->skip_words * 4) : 0;
+  B32 ok = result && partials && (!leaf_count || leaf_buf) && (!prune->skip_bits || skip_buf);
+  U64 res[16];
+  memset(res, 0, sizeof(res));
+  if (ok) {
+    if (leaf_buf) gpu_kernel_set_arg_buffer(k, 0, leaf_buf);
+    else gpu_hip_kernel_clear_arg(k, 0);
+    gpu_kernel_set_arg_buffer(k, 1, result);
+    gpu_kernel_set_arg_buffer(k, 2, partials);
+    if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
+    else gpu_hip_kernel_clear_arg(k, 5);
+    for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
+      if (s < slots.count) gpu_kernel_set_arg_buffer(k, 6 + s, qe_col_kbuf(t, slots.column[s]));
+      else gpu_hip_kernel_clear_arg(k, 6 + s);
+    }
+    gpu_kernel_set_arg_u64(k, 0, r1 - r0);
+    gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
+    gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
+    gpu_kernel_set_arg_u64(k, 5, r0);
+    gpu_kernel_set_arg_u64(k, 7, leaf_count);
+    qe_kernel_set_word(k, QK_W_ARGS, p->arg_count | (in.slot_bits << 8));
+    qe_kernel_set_word(k, QK_W_TYPES, in.type_bits);
+    /* per-block partials, then one block folds them in block order */
+    gpu_kernel_set_arg_buffer(kf, 1, result);
+    gpu_kernel_set_arg_buffer(kf, 2, partials);
+    gpu_kernel_set_arg_u64(kf, 0, blocks);
+    qe_kernel_set_word(kf, QK_W_ARGS, p->arg_count);
+    qe_kernel_set_word(kf, QK_W_TYPES, in.type_bits);
+    GPU_Batch* b = gpu_batch_begin(0, sizeof(res));
+    if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, in.leaf_words, 24 * (U64)leaf_count);
+    if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
+    gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+    gpu_batch_kernel_execute(b, kf, QE_WG, QE_WG);
+    gpu_batch_buffer_read(b, result, res, sizeof(res));
+    ok = gpu_batch_end(b);
+  }
+  qe_scratch_release(result);
+  qe_scratch_release(partials);
+  qe_scratch_release(leaf_buf);
+  qe_scratch_release(skip_buf);
+  if (!ok) {
+    qe_set_error("aggregate_global: %s", gpu_hip_last_error());
+    return 0;
+  }
+  qe_global_decode(t, p, res, out_cells);
   if (trace) {
     trace->path = leaf_count ? QE_AggPath_FusedGlobal : QE_AggPath_Global;
     trace->kernel_count = 2;
     trace->rows_scanned = r1 - r0;
-    trace->rows_after = count;
+    trace->rows_after = res[0];
     U64 bytes = 0;
     for (U32 s = 0; s < slots.count; s++) bytes += (r1 - r0) * t->columns[slots.column[s]].width;
     trace->bytes_read = bytes;
     snprintf(trace->kernel, sizeof(trace->kernel), "aggregate_global");
   }
+  return 1;
+}
+
+/* ---------------- the same without allocating ---------------- */
+/* Scratch layout (QE_NOALLOC_SCRATCH_BYTES): leaf words [0, 256) | result
+ * words [256, 384) | per-block partials from 512, 104 bytes a block. */
+#define QE_NOALLOC_LEAVES 0u
+#define QE_NOALLOC_RESULT 256u
+#define QE_NOALLOC_PARTIALS 512u
+#define QE_NOALLOC_MAX_BLOCKS ((QE_NOALLOC_SCRATCH_BYTES - QE_NOALLOC_PARTIALS) / (13u * 8u))
+
+static void qe_args_bind(QK_Args* a, U32 index, GPU_Buffer* b)
+{
+  a->ptr[index] = b ? gpu_hip_buffer_device_address(b) : 0;
+  a->bytes[index] = b ? gpu_hip_buffer_size(b) : 0;
+}
+
+B32 qe_aggregate_noalloc(QE_Table* table, const QE_IRNode* where, U32 where_count, const U32* funcs,
+                         const S32* arg_cols, U32 expr_count, U64* out_cells, QE_Trace* trace)
+{
+  U64 t0, p0;
+  qe_trace_begin(trace, &t0, &p0);
+  if (!table || !funcs || !arg_cols || !out_cells) {
+    qe_set_error("qe_aggregate_noalloc: null argument");
+    return 0;
+  }
+  if (table->host_backed) {
+    qe_set_error("qe_aggregate_noalloc: the table's columns are not in device memory");
+    return 0;
+  }
+  if (!qe_where_non_null(table, where, where_count, "qe_aggregate_noalloc")) return 0;
+  for (U32 e = 0; e < expr_count; e++)
+    if (!qe_require_non_null(table, arg_cols[e], "qe_aggregate_noalloc")) return 0;
+  QE_AggPlan plan;
+  if (!qe_agg_plan(table, funcs, arg_cols, expr_count, &plan)) return 0;
+  QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
+  U32 leaf_count = 0;
+  if (where && where_count) {
+    static __thread QE_Bytecode prog;
+    if (!qe_bytecode_program_build(table, where, where_count, &prog)) {
+      qe_set_error("qe_aggregate_noalloc: %s", prog.error);
+      return 0;
+    }
+    if (!qe_collect_and_leaves(where, where_count, leaves, &leaf_count, QE_SCAN_MAX_LEAVES)) {
+      qe_set_error("qe_aggregate_noalloc: the WHERE must be an AND of at most %d column-literal comparisons",
+                   QE_SCAN_MAX_LEAVES);
+      return 0;
+    }
+  }
+  const U64 scratch = gpu_hip_noalloc_scratch();
+  if (!scratch) {
+    qe_set_error("qe_aggregate_noalloc: %s", *gpu_hip_last_error() ? gpu_hip_last_error() : "no scratch on this device");
+    return 0;
+  }
+  QE_GlobalInputs in;
+  if (!qe_global_prepare(table, leaves, leaf_count, &plan, &in)) return 0;
+  const U64 rows = table->row_count;
+  U64 res[16];
+  memset(res, 0, sizeof(res));
+  U32 blocks = qe_stream_blocks((rows + 3) / 4);
+  if (blocks > QE_NOALLOC_MAX_BLOCKS) blocks = QE_NOALLOC_MAX_BLOCKS; /* rows are grid-strided */
+  if (rows) {
+    QK_Args a[2];
+    memset(a, 0, sizeof(a));
+    QK_Args* k = &a[0];
+    if (leaf_count) {
+      k->ptr[0] = scratch + QE_NOALLOC_LEAVES;
+      k->bytes[0] = 24 * (U64)leaf_count;
+    }
+    k->ptr[1] = scratch + QE_NOALLOC_RESULT;
+    k->bytes[1] = 128;
+    k->ptr[2] = scratch + QE_NOALLOC_PARTIALS;
+    k->bytes[2] = (U64)blocks * 13 * 8;
+    for (U32 s = 0; s < in.slots.count; s++) qe_args_bind(k, 6 + s, qe_col_kbuf(table, in.slots.column[s]));
+    k->words[0] = (U32)rows;
+    k->words[1] = (U32)(rows >> 32);
+    k->words[6] = QE_ZONE_SHIFT;
+    k->words[14] = leaf_count;
+    k->words[QK_W_ARGS] = plan.arg_count | (in.slot_bits << 8);
+    k->words[QK_W_TYPES] = in.type_bits;
+    k->global_work_size = blocks * QE_WG;
+    k->grid_blocks = blocks;
+    QK_Args* f = &a[1];
+    f->ptr[1] = scratch + QE_NOALLOC_RESULT;
+    f->bytes[1] = 128;
+    f->ptr[2] = scratch + QE_NOALLOC_PARTIALS;
+    f->bytes[2] = (U64)blocks * 13 * 8;
+    f->words[0] = blocks;
+    f->words[QK_W_ARGS] = plan.arg_count;
+    f->words[QK_W_TYPES] = in.type_bits;
+    f->global_work_size = QE_WG;
+    f->grid_blocks = 1;
+    const char* names[2] = {"aggregate_global", "aggregate_global_fold"};
+    const U32 grids[2] = {blocks, 1};
+    if (!gpu_hip_noalloc_run(in.leaf_words, 24 * (U64)leaf_count, QE_NOALLOC_LEAVES, names, a, grids, 2, res,
+                             13 * 8, QE_NOALLOC_RESULT)) {
+      qe_set_error("qe_aggregate_noalloc: %s", gpu_hip_last_error());
+      return 0;
+    }
+  }
+  qe_global_decode(table, &plan, res, out_cells);
+  if (!qe_check_sum_exact(table, funcs, arg_cols, expr_count, out_cells, 1)) return 0;
+  if (trace) {
+    trace->path = leaf_count ? QE_AggPath_FusedGlobal : QE_AggPath_Global;
+    trace->kernel_count = rows ? 2 : 0;
+    trace->rows_before = trace->rows_scanned = rows;
+    trace->rows_after = res[0];
+    U64 bytes = 0;
+    for (U32 s = 0; s < in.slots.count; s++) bytes += rows * table->columns[in.slots.column[s]].width;
+    trace->bytes_read = bytes;
+    snprintf(trace->kernel, sizeof(trace->kernel), "aggregate_global");
+  }
+  qe_trace_end(trace, t0, p0);
   return 1;
 }
 

@@ -101,6 +101,14 @@ GPU_Backend* qe_gpu(void);
 B32 qe_buffer_read_checked(GPU_Buffer* b, void* out, U64 size);
 B32 qe_buffer_write_checked(GPU_Buffer* b, const void* data, U64 size);
 
+/* the no-allocation scratch of qe_aggregate_noalloc (gpu_hip.cpp): its
+ * device address on the current device (0 when not set up), and a run of
+ * library kernels over it that allocates nothing */
+U64 gpu_hip_noalloc_scratch(void);
+B32 gpu_hip_noalloc_run(const void* upload, U64 upload_bytes, U64 upload_offset, const char* const* kernels,
+                        const QK_Args* args, const U32* blocks, U32 steps, void* download, U64 download_bytes,
+                        U64 download_offset);
+
 String8 qe_str8(const char* s);
 GPU_Kernel* qe_kernel(const char* name);
 /* set push words directly (words[i]) */

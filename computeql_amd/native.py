@@ -208,6 +208,7 @@ SIGNATURES = [
     ("qe_gen_dim", B32, [P, U64, U32, B32, S32, S32]),
     ("qe_scan_filter", B32, [P, P, U32, B32, P, U64, P, P]),
     ("qe_aggregate", B32, [P, P, U32, P, U32, P, P, U32, U64, P, P, P, P]),
+    ("qe_aggregate_noalloc", B32, [P, P, U32, P, P, U32, P, P]),
     ("qe_aggregate_join", B32, [P, U32, P, U32, P, U32, P, U32, P, P, U32, U64, P, P, P, P]),
     ("qe_hash_join", U32, [P, U32, P, U64, P, U32, P, U64, U32, P, U64, P, P]),
     ("qe_gather", U32, [P, P, U32, P, U64, U64, U64, P]),

@@ -437,6 +437,23 @@ B32 qe_aggregate(QE_Table* table, const QE_IRNode* where, U32 where_count, const
                  U32 group_count, const U32* funcs, const S32* arg_cols, U32 expr_count, U64 cap,
                  U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace);
 
+/* qe_aggregate without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): an
+ * ungrouped COUNT/SUM/AVG/MIN/MAX over up to 4 argument columns of a
+ * device-resident table, with a WHERE that is an AND of at most 8
+ * column-literal comparisons (or none), computed by the same kernels as
+ * qe_aggregate, whose result it equals bit for bit.  Its scratch (leaf
+ * words, per-workgroup partials, the folded result) is a static device
+ * array of QE_NOALLOC_SCRATCH_BYTES that each GPU holds until the process
+ * ends; it calls no hipMalloc/hipFree and no allocating library routine.
+ * Larger shapes (more leaves or argument columns, other WHERE forms,
+ * host-backed tables) are refused with an error.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+#define QE_NOALLOC_SCRATCH_BYTES (256u * 1024u)
+B32 qe_aggregate_noalloc(QE_Table* table, const QE_IRNode* where, U32 where_count, const U32* funcs,
+                         const S32* arg_cols, U32 expr_count, U64* out_cells, QE_Trace* trace);
+
 /* qe_aggregate_join_pushdown (query_exec.c:8442-8775):
  *   SELECT [dim.g1, ...,] aggs(fact.*) FROM fact JOIN dim ON fact.k = dim.k
  *   [WHERE dim-predicate] [GROUP BY dim.g1, ...]
