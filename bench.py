@@ -410,7 +410,7 @@ def streamed_run(cq, lib, t, rows: int, repeats: int = 3):
         oa, ob = np.argsort(a.keys, kind="stable"), np.argsort(b.keys, kind="stable")
         return np.array_equal(a.keys[oa], b.keys[ob]) and np.array_equal(a.cells[oa], b.cells[ob])
 
-    out = {"rows": rows, "chunk_rows": int(cq.get_setting("QE_STREAM_CHUNK_ROWS", 16777216)),
+    out = {"rows": rows, "chunk_rows": int(cq.get_setting("QE_SCAN_STREAM_CHUNK_ROWS", 16777216)),
            "build_s": round(build_s, 2)}
     errs = []
     for q, (fn, bpr) in queries.items():

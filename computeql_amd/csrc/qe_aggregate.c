@@ -907,9 +907,14 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
     if (!qe_require_non_null(table, (S32)keys.col[i], "qe_aggregate")) return 0;
   for (U32 e = 0; e < expr_count; e++)
     if (!qe_require_non_null(table, arg_cols[e], "qe_aggregate")) return 0;
-  if (table->host_backed)
-    return qe_stream_aggregate(table, where, where_count, group_cols, group_count, funcs, arg_cols, expr_count, cap,
-                               out_group_count, out_keys, out_cells, trace);
+  /* host-backed rows: streamed per chunk (QE_AGG_STREAM, query_exec.c:
+   * 8842-9110, :10379), or with it 0 read in place over the host link */
+  if (table->host_backed) {
+    if (qe_settings_u64("QE_AGG_STREAM", 1) != 0)
+      return qe_stream_aggregate(table, where, where_count, group_cols, group_count, funcs, arg_cols, expr_count, cap,
+                                 out_group_count, out_keys, out_cells, trace);
+    if (!qe_require_device(table, "qe_aggregate")) return 0;
+  }
   QE_AggPlan plan;
   if (!qe_agg_plan(table, funcs, arg_cols, expr_count, &plan)) return 0;
   if (trace) trace->rows_before = table->row_count;
@@ -928,6 +933,13 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
      * kernels; any other WHERE runs as a scan whose row ids stay on the
      * device and feed the hash aggregate */
     if (!qe_collect_and_leaves(where, where_count, leaves, &leaf_count, QE_SCAN_MAX_LEAVES)) {
+      general_where = 1;
+      leaf_count = 0;
+    }
+    /* QE_FUSE_AGGREGATE (query_exec.c:9892) / QE_FUSE_GROUP_BY (:9447) 0:
+     * the WHERE runs as its own scan whose row ids feed the aggregate */
+    if (!general_where && leaf_count &&
+        qe_settings_u64(K ? "QE_FUSE_GROUP_BY" : "QE_FUSE_AGGREGATE", 1) == 0) {
       general_where = 1;
       leaf_count = 0;
     }

@@ -205,6 +205,9 @@ void qe_prune_free(QE_Prune* p);
  * when hiprtc is unavailable or compilation fails */
 GPU_Kernel* qk_scan_kernel(QE_Table* table, const QE_IRNode* nodes, U32 count, QE_ColumnSlots* slots,
                            F64* consts, U32* const_count, U64* compile_ns);
+/* QE_ROUTE_CONJUNCTIONS: 1 when this WHERE shape's first run goes to the
+ * library kernels (its kernel is generated when the shape comes back) */
+B32 qk_route_defer(const QE_Bytecode* p);
 GPU_Kernel* qk_simple_kernel(QE_Table* table, const QE_Leaf* leaves, U32 leaf_count, QE_ColumnSlots* slots,
                              U64* compile_ns);
 

@@ -164,9 +164,13 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   /* the histogram runs with the scatter's block count and block size */
   GPU_Kernel* kh = qe_kernel(wc ? "join_part_hist_512" : "join_part_hist");
   GPU_Kernel* ko = qe_kernel("join_part_offsets");
+  /* the instances specialised to the query's shape stand in for the
+   * reference's generated join kernel (QE_GENERATE_JOIN, query_exec.c:8235);
+   * 0 takes the generic forms */
+  const B32 specialized = qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1) && qe_settings_u64("QE_GENERATE_JOIN", 1);
   char sname[40] = "join_part_scatter";
   if (wc) snprintf(sname, sizeof(sname), "join_part_scatter_wc_%u", arg_count);
-  else if (arg_count <= 2 && qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1))
+  else if (arg_count <= 2 && specialized)
     snprintf(sname, sizeof(sname), "join_part_scatter_%u", arg_count);
   GPU_Kernel* ks = qe_kernel(sname);
   /* a probe instance specialised for this shape when one exists */
@@ -179,7 +183,7 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     if (arg_count == 0) snprintf(pname, sizeof(pname), "join_part_probe_0_row");
     else if (arg_count == 1 && need_sum) snprintf(pname, sizeof(pname), "join_part_probe_1%s_row", need_minmax ? "sm" : "s");
   }
-  if (!qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1)) snprintf(pname, sizeof(pname), "join_part_probe");
+  if (!specialized) snprintf(pname, sizeof(pname), "join_part_probe");
   /* QE_JOIN_PROBE_WG: 512 (default) or 256 threads a probe block */
   const U32 pwg = qe_settings_u64("QE_JOIN_PROBE_WG", 512) == 256 ? 256 : 512;
   if (pwg == 512) snprintf(pname + strlen(pname), sizeof(pname) - strlen(pname), "_512");
@@ -354,8 +358,9 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     if (!qe_require_non_null(dim, dim_group_cols[k], "qe_aggregate_join")) return 0;
   for (U32 e = 0; e < expr_count; e++)
     if (!qe_require_non_null(fact, fact_arg_cols[e], "qe_aggregate_join")) return 0;
-  if (qe_settings_u64("QE_JOIN_PUSHDOWN", 1) == 0) {
-    qe_set_error("qe_aggregate_join: QE_JOIN_PUSHDOWN is off");
+  /* the pushdown rests on the streamed reductions (query_exec.c:8446) */
+  if (qe_settings_u64("QE_JOIN_PUSHDOWN", 1) == 0 || qe_settings_u64("QE_AGG_STREAM", 1) == 0) {
+    qe_set_error("qe_aggregate_join: QE_JOIN_PUSHDOWN or QE_AGG_STREAM is off");
     return 0;
   }
   QE_Column* fkey = &fact->columns[fact_key_col];

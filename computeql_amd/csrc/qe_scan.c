@@ -513,8 +513,13 @@ B32 qe_scan_filter(QE_Table* table, const QE_IRNode* where, U32 node_count, B32 
                    U64* out_count, QE_Trace* trace)
 {
   if (!qe_where_non_null(table, where, node_count, "qe_scan_filter")) return 0;
-  if (table && out_count && table->host_backed)
-    return qe_stream_scan(table, where, node_count, count_only, out_ids, cap, out_count, trace);
+  /* host-backed rows stream through the chunk pipeline; with QE_SCAN_STREAM
+   * 0 (query_exec.c:2798) the kernels read them in place over the host link */
+  if (table && out_count && table->host_backed) {
+    if (qe_settings_u64("QE_SCAN_STREAM", 1) != 0)
+      return qe_stream_scan(table, where, node_count, count_only, out_ids, cap, out_count, trace);
+    if (!qe_require_device(table, "qe_scan_filter")) return 0;
+  }
   return qe_scan_impl(table, where, node_count, count_only, out_ids, cap, out_count, trace, 0);
 }
 
@@ -621,6 +626,10 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
       return 0;
     }
   }
+  /* QE_ROUTE_CONJUNCTIONS: a new shape's first run on the interpreter */
+  if (path == QE_ScanPath_Generated && qe_bytecode_max_stack_depth(&prog) <= 8 &&
+      qe_settings_u64("QE_SCAN_INTERPRETER", 1) != 0 && qk_route_defer(&prog))
+    path = QE_ScanPath_Interpreter;
   if (path == QE_ScanPath_Interpreter && qe_settings_u64("QE_SCAN_INTERPRETER", 1) == 0) {
     qe_prune_free(&prune);
     qe_set_error("qe_scan_filter: no GPU kernel for this WHERE (interpreter disabled)");

@@ -3,7 +3,7 @@
  * execution (the reference's cold path, query_exec.c:1674-2491).
  *
  * A host-backed table keeps its rows in pinned host memory.  Operators over
- * it run chunk by chunk: a chunk is QE_STREAM_CHUNK_ROWS rows (a multiple
+ * it run chunk by chunk: a chunk is QE_SCAN_STREAM_CHUNK_ROWS rows (a multiple
  * of the 8192-row zone), and each column a query reads is brought into HBM
  * per chunk through the resident chunk cache:
  *
@@ -80,12 +80,30 @@ static void res_unlock(void)
   pthread_mutex_unlock(&g_res_lock);
 }
 
-static U64 qe_stream_chunk_rows(void)
+/* rows of a streamed chunk: QE_SCAN_STREAM_CHUNK_ROWS (16,777,216,
+ * query_exec.c:2096-2099; QE_STREAM_CHUNK_ROWS is accepted as an older
+ * name), at most GPU_MAX_BUFFER_SIZE bytes of the widest column read when
+ * that is set (query_exec.c:2740-2756 sizes chunks by it; here columns are
+ * otherwise whole in HBM, so it has no default), a multiple of the
+ * 8192-row zone so a chunk's zone maps are a slice of the column's */
+static U64 qe_stream_chunk_rows(U32 widest)
 {
   U64 zone = 1ull << QE_ZONE_SHIFT;
-  U64 r = qe_settings_u64("QE_STREAM_CHUNK_ROWS", 16777216ull);
-  if (r < zone) r = zone;
-  return (r + zone - 1) & ~(zone - 1);
+  U64 r = qe_settings_u64("QE_SCAN_STREAM_CHUNK_ROWS", qe_settings_u64("QE_STREAM_CHUNK_ROWS", 16777216ull));
+  U64 max_bytes = qe_settings_u64("GPU_MAX_BUFFER_SIZE", 0);
+  if (max_bytes && widest && max_bytes / widest < r) r = max_bytes / widest;
+  if (r < zone) return zone;
+  return r & ~(zone - 1);
+}
+
+/* chunk slots in flight: QE_SCAN_STREAM_SLOTS (3, clamped to 2..8,
+ * query_exec.c:2253): the copies of the next slots - 1 chunks run while a
+ * chunk's kernels do */
+#define QE_STREAM_MAX_SLOTS 8
+static U32 qe_stream_slots(void)
+{
+  U64 n = qe_settings_u64("QE_SCAN_STREAM_SLOTS", 3);
+  return (U32)(n < 2 ? 2 : (n > QE_STREAM_MAX_SLOTS ? QE_STREAM_MAX_SLOTS : n));
 }
 
 /* callers hold the lock */
@@ -405,7 +423,7 @@ static B32 qe_host_zonemap(QE_Table* t, QE_Column* col)
   if (!zones) return 1;
   col->zone_min = (F64*)malloc(sizeof(F64) * zones);
   col->zone_max = (F64*)malloc(sizeof(F64) * zones);
-  U64 chunk = qe_stream_chunk_rows();
+  U64 chunk = qe_stream_chunk_rows(col->width);
   if (chunk > rows) chunk = (rows + (1ull << QE_ZONE_SHIFT) - 1) & ~((1ull << QE_ZONE_SHIFT) - 1);
   U64 czones = chunk >> QE_ZONE_SHIFT;
   GPU_Buffer* stage = gpu_buffer_alloc(chunk * col->width + 64, GPU_BufferFlag_ReadWrite | GPU_BufferFlag_DeviceLocal, 0);
@@ -531,7 +549,8 @@ typedef struct QE_Pipeline {
   U32 col_count;
   U64 chunk_rows;
   GPU_Stream* copy;
-  QE_ChunkSlot slot[2];
+  U32 nslots;
+  QE_ChunkSlot slot[QE_STREAM_MAX_SLOTS];
   QE_Table* view;
 } QE_Pipeline;
 
@@ -654,7 +673,11 @@ static B32 qe_mark_where(QE_Pipeline* p, const QE_IRNode* where, U32 count)
 static B32 qe_pipeline_run(QE_Pipeline* p, const QE_Leaf* leaves, U32 leaf_count, QE_ChunkFn fn, void* ctx)
 {
   QE_Table* t = p->t;
-  p->chunk_rows = qe_stream_chunk_rows();
+  U32 widest = 0;
+  for (U32 i = 0; i < p->col_count; i++)
+    if (t->columns[p->cols[i]].width > widest) widest = t->columns[p->cols[i]].width;
+  p->chunk_rows = qe_stream_chunk_rows(widest);
+  p->nslots = qe_stream_slots();
   U64 chunks = (t->row_count + p->chunk_rows - 1) / p->chunk_rows;
   /* warm: every column read is whole in the cache; one pass over them as
    * over a device-resident table */
@@ -712,15 +735,18 @@ static B32 qe_pipeline_run(QE_Pipeline* p, const QE_Leaf* leaves, U32 leaf_count
   qe_t_stream_stats.chunks_skipped = chunks - nrun;
 
   B32 ok = 1;
-  p->copy = nrun ? gpu_hip_stream_alloc(2) : 0;
+  const U32 S = p->nslots;
+  p->copy = nrun ? gpu_hip_stream_alloc(S) : 0;
   if (nrun && !p->copy) {
     qe_set_error("streamed scan: no copy stream: %s", gpu_hip_last_error());
     ok = 0;
   }
-  if (ok && nrun) ok = qe_slot_prepare(p, 0, run[0]);
+  /* chunk i runs in slot i % S; the copies of chunks i + 1 .. i + S - 1
+   * are in flight meanwhile */
+  for (U64 i = 0; ok && i + 1 < S && i < nrun; i++) ok = qe_slot_prepare(p, (U32)i, run[i]);
   for (U64 i = 0; ok && i < nrun; i++) {
-    U32 si = (U32)(i & 1);
-    if (i + 1 < nrun) ok = qe_slot_prepare(p, si ^ 1, run[i + 1]);
+    U32 si = (U32)(i % S);
+    if (i + S - 1 < nrun) ok = qe_slot_prepare(p, (U32)((i + S - 1) % S), run[i + S - 1]);
     QE_ChunkSlot* s = &p->slot[si];
     if (ok && s->copying) {
       U64 w0 = qe_now_ns();
@@ -738,7 +764,7 @@ static B32 qe_pipeline_run(QE_Pipeline* p, const QE_Leaf* leaves, U32 leaf_count
   }
   /* after a failure, copies still in flight finish before their buffers go */
   if (p->copy) {
-    for (U32 si = 0; si < 2; si++) {
+    for (U32 si = 0; si < S; si++) {
       if (p->slot[si].copying) gpu_hip_stream_wait(p->copy, si);
       qe_slot_release(p, &p->slot[si], 0);
     }

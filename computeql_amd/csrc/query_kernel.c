@@ -19,6 +19,7 @@
  */
 #define _GNU_SOURCE
 #include <hip/hiprtc.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdlib.h>
 #include <sys/stat.h>
@@ -131,11 +132,12 @@ static void cache_store(const char* name, const char* code, size_t size)
 }
 
 /* compile (or find) the kernel for `src`; names carry two hashes */
-static GPU_Kernel* qk_kernel_from_source(const char* prefix, QK_Src* src, U64* compile_ns)
+static GPU_Kernel* qk_kernel_from_source(const char* prefix, QK_Src* src, U64* compile_ns, char* out_name)
 {
-  char name[96];
+  char name[64];
   snprintf(name, sizeof(name), "%s_%016llx_%08x", prefix, (unsigned long long)hash_fnv1a(src->buf, src->len),
            hash_djb2(src->buf, src->len));
+  if (out_name) snprintf(out_name, 64, "%s", name);
   String8 empty = {0, 0};
   GPU_Kernel* k = gpu_kernel_alloc_from_spirv(qe_str8(name), empty);
   if (k) return k;
@@ -183,6 +185,62 @@ static GPU_Kernel* qk_kernel_from_source(const char* prefix, QK_Src* src, U64* c
   return k;
 }
 
+/* ---------------- WHERE shapes ---------------- */
+/* A generated kernel depends only on the bytecode words (operators, column
+ * slots with their types, constant indices): literals travel in a buffer.
+ * Shapes seen before map to their kernel's name, so a repeated shape skips
+ * generating and hashing its source; the words themselves are kept and
+ * compared, so two shapes never share an entry. */
+#define QK_SHAPES 512
+typedef struct QK_Shape {
+  U64 hash;
+  U32 word_count;
+  U32* words;
+  char name[64];  /* "" until its kernel is built */
+  B32 seen;       /* run once (QE_ROUTE_CONJUNCTIONS) */
+} QK_Shape;
+static QK_Shape g_shapes[QK_SHAPES];
+static pthread_mutex_t g_shape_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static U64 qk_shape_hash(const QE_Bytecode* p)
+{
+  return hash_fnv1a((const char*)p->words, p->word_count * sizeof(U32)) ^ ((U64)p->word_count << 52);
+}
+
+/* the entry of a shape (inserted when absent, evicting the slot's old
+ * shape); callers hold g_shape_lock */
+static QK_Shape* qk_shape_locked(const QE_Bytecode* p)
+{
+  const U64 h = qk_shape_hash(p);
+  QK_Shape* e = &g_shapes[h % QK_SHAPES];
+  if (e->words && e->hash == h && e->word_count == p->word_count &&
+      !memcmp(e->words, p->words, p->word_count * sizeof(U32)))
+    return e;
+  U32* w = (U32*)malloc(p->word_count * sizeof(U32) + 4);
+  if (!w) return 0;
+  memcpy(w, p->words, p->word_count * sizeof(U32));
+  free(e->words);
+  memset(e, 0, sizeof(*e));
+  e->hash = h;
+  e->word_count = p->word_count;
+  e->words = w;
+  return e;
+}
+
+/* QE_ROUTE_CONJUNCTIONS (query_kernel.c:1735-1760): with it set, the first
+ * run of a WHERE shape is left to the library kernels and its kernel is
+ * generated when the shape comes back; 1 = run this one on the library */
+B32 qk_route_defer(const QE_Bytecode* p)
+{
+  if (qe_settings_u64("QE_ROUTE_CONJUNCTIONS", 0) == 0) return 0;
+  pthread_mutex_lock(&g_shape_lock);
+  QK_Shape* e = qk_shape_locked(p);
+  B32 defer = e && !e->seen && !e->name[0];
+  if (e) e->seen = 1;
+  pthread_mutex_unlock(&g_shape_lock);
+  return defer;
+}
+
 /* ---------------- qk_scan_kernel: any AND/OR tree ---------------- */
 enum {
   OP_PUSH_TRUE = 0,
@@ -226,6 +284,24 @@ GPU_Kernel* qk_scan_kernel(QE_Table* table, const QE_IRNode* nodes, U32 count, Q
   if (!qe_bytecode_program_build(table, nodes, count, &prog)) {
     qe_set_error("qk_scan_kernel: %s", prog.error);
     return 0;
+  }
+  /* a shape built before: its kernel by name, no source generated */
+  char known[64] = "";
+  pthread_mutex_lock(&g_shape_lock);
+  {
+    QK_Shape* e = qk_shape_locked(&prog);
+    if (e) snprintf(known, sizeof(known), "%s", e->name);
+  }
+  pthread_mutex_unlock(&g_shape_lock);
+  if (known[0]) {
+    String8 empty = {0, 0};
+    GPU_Kernel* k = gpu_kernel_alloc_from_spirv(qe_str8(known), empty);
+    if (k) {
+      *slots = prog.slots;
+      memcpy(consts, prog.consts, sizeof(F64) * prog.const_count);
+      *const_count = prog.const_count;
+      return k;
+    }
   }
   QK_Expr stack[64];
   int sp = 0;
@@ -302,9 +378,16 @@ GPU_Kernel* qk_scan_kernel(QE_Table* table, const QE_IRNode* nodes, U32 count, Q
   src_printf(&s, "extern \"C\" __global__ void __launch_bounds__(QK_WG) qk_main(QK_Args a) {\n");
   src_printf(&s, "  QkPred p{&a};\n  qk_scan_driver(a, p);\n}\n");
   free(root);
-  GPU_Kernel* k = s.buf ? qk_kernel_from_source("qk_scan", &s, compile_ns) : 0;
+  char name[64] = "";
+  GPU_Kernel* k = s.buf ? qk_kernel_from_source("qk_scan", &s, compile_ns, name) : 0;
   free(s.buf);
   if (!k) return 0;
+  pthread_mutex_lock(&g_shape_lock);
+  {
+    QK_Shape* e = qk_shape_locked(&prog);
+    if (e) snprintf(e->name, sizeof(e->name), "%s", name);
+  }
+  pthread_mutex_unlock(&g_shape_lock);
   *slots = prog.slots;
   memcpy(consts, prog.consts, sizeof(F64) * prog.const_count);
   *const_count = prog.const_count;
@@ -354,7 +437,7 @@ GPU_Kernel* qk_simple_kernel(QE_Table* table, const QE_Leaf* leaves, U32 leaf_co
   src_printf(&s, "    return m;\n  }\n};\n");
   src_printf(&s, "extern \"C\" __global__ void __launch_bounds__(QK_WG) qk_main(QK_Args a) {\n");
   src_printf(&s, "  QkPred p{&a};\n  qk_scan_driver(a, p);\n}\n");
-  GPU_Kernel* k = s.buf ? qk_kernel_from_source("qk_simple", &s, compile_ns) : 0;
+  GPU_Kernel* k = s.buf ? qk_kernel_from_source("qk_simple", &s, compile_ns, 0) : 0;
   free(s.buf);
   if (k) *slots = sl;
   return k;

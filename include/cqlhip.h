@@ -372,7 +372,7 @@ B32 qe_table_build_zonemap(QE_Table* table, U32 column);
  * into pinned host memory; zone maps are built on the device as the rows
  * pass through once.  A table holds either host-backed or device columns.
  * qe_scan_filter and qe_aggregate over such a table stream it in chunks of
- * QE_STREAM_CHUNK_ROWS rows (default 16,777,216, query_exec.c:2096-2099)
+ * QE_SCAN_STREAM_CHUNK_ROWS rows (default 16,777,216, query_exec.c:2096-2099)
  * through the resident chunk cache, copies of one chunk overlapping the
  * kernels of the one before.  The other read-only operators (gather,
  * SELECT, joins, batches) read its rows in place over the host link;

@@ -172,7 +172,7 @@ def test_streamed_multi_key(orc):
     try:
         for name, arr in (("a", a), ("b", b), ("v", v)):
             t.add_host_column(name, cq.U32, arr)
-        cq.set_setting("QE_STREAM_CHUNK_ROWS", 1 << 20)
+        cq.set_setting("QE_SCAN_STREAM_CHUNK_ROWS", 1 << 20)
         funcs = [cq.COUNT, cq.SUM, cq.MIN]
         for keys in ([0, 1], [1, 0], [0]):
             res = t.aggregate(funcs, [None, 2, 2], where=col(2) > 1000, group=keys)

@@ -25,7 +25,7 @@ CHUNKS = (N + CHUNK - 1) // CHUNK
 @pytest.fixture(autouse=True)
 def _settings():
     cq.reset_settings()
-    cq.set_setting("QE_STREAM_CHUNK_ROWS", CHUNK)
+    cq.set_setting("QE_SCAN_STREAM_CHUNK_ROWS", CHUNK)
     yield
     cq.reset_settings()
 
@@ -50,7 +50,7 @@ TYPES = [cq.U32, cq.U32, cq.U32, cq.I32, cq.U32, cq.F32, cq.F64, cq.U32]
 @pytest.fixture(scope="module")
 def tables():
     cq.reset_settings()
-    cq.set_setting("QE_STREAM_CHUNK_ROWS", CHUNK)
+    cq.set_setting("QE_SCAN_STREAM_CHUNK_ROWS", CHUNK)
     cols = _columns()
     dev = cq.Table(N)
     host = cq.Table(N)
@@ -245,14 +245,14 @@ def test_chunk_size_change_between_queries(tables):
     dev, host, _ = tables
     cq.resident_clear()
     host.scan(col(2) < 200_000)                    # chunks 0-1 of 131072 rows filled
-    cq.set_setting("QE_STREAM_CHUNK_ROWS", 3 * CHUNK)
+    cq.set_setting("QE_SCAN_STREAM_CHUNK_ROWS", 3 * CHUNK)
     for w in (col(2) < 600_000, col(2) >= 0):
         nh, idh, _ = host.scan(w)
         nd, idd, _ = dev.scan(w)
         assert nh == nd and np.array_equal(idh, idd), w
-    cq.set_setting("QE_STREAM_CHUNK_ROWS", CHUNK)
+    cq.set_setting("QE_SCAN_STREAM_CHUNK_ROWS", CHUNK)
     host.scan(col(2) >= 0)                         # fills the rest under the first size
-    cq.set_setting("QE_STREAM_CHUNK_ROWS", 5 * CHUNK)
+    cq.set_setting("QE_SCAN_STREAM_CHUNK_ROWS", 5 * CHUNK)
     nh, idh, _ = host.scan(col(2) > 123_456)
     nd, idd, _ = dev.scan(col(2) > 123_456)
     assert nh == nd and np.array_equal(idh, idd)
