@@ -4,15 +4,19 @@
         markdown table of per-kernel dispatch counts and durations from a
         `rocprofv3 --kernel-trace --stats --output-format csv` run.
 
-    python -m computeql_amd.profiling traffic --fetch <counter_collection.csv>
+    python -m computeql_amd.profiling traffic --read <counter_collection.csv>
         --write <counter_collection.csv> --rows N [--command CMD] [--json OUT]
-        HBM bytes per dispatch from two `rocprofv3 --pmc FETCH_SIZE` /
-        `--pmc WRITE_SIZE` runs of `bench.py --calibrate`.  Each counter is
-        scaled by what it reads for a kernel of known traffic in the same
-        run: FETCH_SIZE by the calibration read of one 4-byte column
-        (k_bandwidth_read, rows x 4 bytes), WRITE_SIZE by the table
-        generator (k_gen_events, four 4-byte columns written, rows x 16
-        bytes).
+        HBM bytes per dispatch from two rocprofv3 --pmc runs: the TCC's read
+        requests to memory by size (TCC_EA0_RDREQ_32B/_64B/_128B_sum, with
+        TCC_EA0_RDREQ_sum, whose count they must add up to) and its write
+        requests (TCC_EA0_WRREQ_sum, 32 bytes each unless counted in
+        TCC_EA0_WRREQ_64B_sum).  bytes = 32 n32 + 64 n64 + 128 n128 read +
+        32 (w - w64) + 64 w64 written.  No scaling: a plain read of a 4 GB
+        column (k_bandwidth_read) and the table generator (k_gen_events,
+        16 GB written) come out at their known sizes, which the summary
+        shows as the check.  (The derived FETCH_SIZE of this ROCm takes
+        gfx950's 128-byte reads from TCC_BUBBLE, which does not count
+        them, so it undercounts streaming kernels by 2-4x.)
 """
 from __future__ import annotations
 
@@ -24,9 +28,10 @@ from collections import defaultdict
 
 # kernel-name prefixes bench.py reports (rocprofv3 names carry the k_ prefix
 # and the template-instance suffix)
-KERNELS = ["aggregate_global", "scan_simple", "scan_filter", "aggregate_dense", "qk_scan", "join_part_hist",
-           "join_part_scatter", "join_part_probe", "join_part_build", "join_probe", "join_build", "bandwidth_read",
-           "gen_events", "gen_fact_fk", "fill_u32"]
+KERNELS = ["aggregate_global", "aggregate_global_fold", "scan_simple", "scan_filter", "aggregate_dense",
+           "aggregate_hash", "hash_compact", "qk_scan", "join_part_hist", "join_part_offsets", "join_part_scatter",
+           "join_part_probe", "join_part_build", "join_probe", "join_build", "bandwidth_read", "bandwidth_copy",
+           "gen_events", "gen_fact_fk", "fill_u32", "zonemap_build"]
 
 
 def _key(name: str) -> str | None:
@@ -56,47 +61,50 @@ def stats_table(path: str, command: str = "") -> str:
     return "\n".join(out) + "\n"
 
 
-def counter_means(path: str, counter: str) -> dict:
-    """mean counter value per dispatch, by kernel key"""
-    acc = defaultdict(list)
+def counter_values(path: str) -> dict:
+    """every counter's value per dispatch, in dispatch order, by kernel key"""
+    acc = defaultdict(lambda: defaultdict(dict))
     for r in csv.DictReader(open(path)):
-        if r["Counter_Name"] != counter:
-            continue
         k = _key(r["Kernel_Name"])
         if k:
-            acc[k].append(float(r["Counter_Value"]))
-    return {k: sum(v) / len(v) for k, v in acc.items()}
+            acc[k][r["Counter_Name"]][int(r["Dispatch_Id"])] = float(r["Counter_Value"])
+    return {k: {c: [v[i] for i in sorted(v)] for c, v in d.items()} for k, d in acc.items()}
 
 
-def traffic(fetch_csv: str, write_csv: str, rows: int) -> dict:
-    fetch = counter_means(fetch_csv, "FETCH_SIZE")
-    write = counter_means(write_csv, "WRITE_SIZE")
-    if "bandwidth_read" not in fetch or "gen_events" not in write:
-        raise SystemExit("calibration kernels missing: run bench.py --calibrate under the profiler")
-    # KiB read per known byte, KiB written per known byte
-    fetch_scale = (rows * 4) / (fetch["bandwidth_read"] * 1024.0)
-    write_scale = (rows * 16) / (write["gen_events"] * 1024.0)
+READ_COUNTERS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum")
+WRITE_COUNTERS = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
+
+
+def traffic(read_csv: str, write_csv: str, rows: int) -> dict:
+    rd = counter_values(read_csv)
+    wr = counter_values(write_csv)
     kernels = {}
-    for k in sorted(set(fetch) | set(write)):
-        f = fetch.get(k, 0.0) * 1024.0 * fetch_scale
-        w = write.get(k, 0.0) * 1024.0 * write_scale
-        kernels[k] = {"fetch_kib": round(fetch.get(k, 0.0), 1), "write_kib": round(write.get(k, 0.0), 1),
-                      "bytes": int(f + w)}
-    return {"rows": rows, "fetch_scale": round(fetch_scale, 5), "write_scale": round(write_scale, 5),
-            "kernels": kernels}
+    mean = lambda v: sum(v) / len(v) if v else 0.0
+    for k in sorted(set(rd) | set(wr)):
+        r, w = rd.get(k, {}), wr.get(k, {})
+        n32, n64, n128, n = (r.get(c, []) for c in READ_COUNTERS)
+        wn, w64 = (w.get(c, []) for c in WRITE_COUNTERS)
+        reads = [32 * a + 64 * b + 128 * c for a, b, c in zip(n32, n64, n128)]
+        writes = [32 * (a - b) + 64 * b for a, b in zip(wn, w64)]
+        per = [int(x + y) for x, y in zip(reads, writes)] if len(reads) == len(writes) else []
+        kernels[k] = {"read_bytes": int(mean(reads)), "write_bytes": int(mean(writes)),
+                      "bytes": int(mean(reads) + mean(writes)), "read_requests": int(mean(n)),
+                      "sized_requests": int(mean(n32) + mean(n64) + mean(n128)),
+                      "dispatch_bytes": per}
+    return {"rows": rows, "method": "TCC read requests by size + write requests, no scaling", "kernels": kernels}
 
 
 def traffic_table(rec: dict, command: str = "") -> str:
     out = []
     if command:
         out.append(f"Command: `{command}`\n")
-    out.append(f"rows = {rec['rows']}; FETCH_SIZE scale {rec['fetch_scale']}, WRITE_SIZE scale {rec['write_scale']} "
-               "(known bytes / counter bytes of the calibration kernels)\n")
-    out.append("| kernel | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | HBM bytes/dispatch (scaled) | bytes/row |")
-    out.append("|---|---:|---:|---:|---:|")
+    out.append(f"rows = {rec['rows']}; {rec['method']}. Read bytes = 32/64/128 x the TCC's read requests of that "
+               "size; write bytes = 32 x 32-byte + 64 x 64-byte write requests.\n")
+    out.append("| kernel | read requests (sized / all) | read GB | write GB | HBM GB/dispatch | bytes/row |")
+    out.append("|---|---:|---:|---:|---:|---:|")
     for k, v in rec["kernels"].items():
-        out.append(f"| {k} | {v['fetch_kib']:.0f} | {v['write_kib']:.0f} | {v['bytes']} | "
-                   f"{v['bytes'] / rec['rows']:.2f} |")
+        out.append(f"| {k} | {v['sized_requests']} / {v['read_requests']} | {v['read_bytes'] / 1e9:.3f} | "
+                   f"{v['write_bytes'] / 1e9:.3f} | {v['bytes'] / 1e9:.3f} | {v['bytes'] / rec['rows']:.2f} |")
     return "\n".join(out) + "\n"
 
 
@@ -107,7 +115,7 @@ def main(argv=None) -> int:
     s.add_argument("csv")
     s.add_argument("--command", default="")
     t = sub.add_parser("traffic")
-    t.add_argument("--fetch", required=True)
+    t.add_argument("--read", required=True)
     t.add_argument("--write", required=True)
     t.add_argument("--rows", type=int, required=True)
     t.add_argument("--command", default="")
@@ -116,7 +124,7 @@ def main(argv=None) -> int:
     if a.cmd == "stats":
         sys.stdout.write(stats_table(a.csv, a.command))
     else:
-        rec = traffic(a.fetch, a.write, a.rows)
+        rec = traffic(a.read, a.write, a.rows)
         if a.command:
             rec["command"] = a.command
         if a.json:
