@@ -33,9 +33,20 @@ The same JSON line also carries, measured after the timed steps:
            latency stays within 50 ms, and the ratio of the two; on the
            event table's 64 queries, and (served.mix) on those plus five
            query shapes over each of 4 x u32 tables of 1M, 100M and 1B rows
+  roofline the query with the longest kernel time per step (the partitioned
+           join q6): its algorithmic bytes / the summed hipEvent time of its
+           kernels, and the HBM bytes they move (rocprofv3 TCC counters,
+           profiles/traffic.json); config.per_query carries the same for
+           every query
   roofline.ceilings  what this device sustains in the same run: a plain
            4 GB read and a 4 GB copy (read + write bytes); the dominant
-           kernel's fraction of the matching one is frac_of_measured_ceiling
+           query's fraction of the matching one is frac_of_measured_ceiling
+  extra    shapes timed beside the suite: COUNT/SUM of region and
+           COUNT/SUM/MIN/MAX of ts grouped by amount (10k groups), the
+           point lookup pruned vs in scan form, and the first compile of a
+           generated scan kernel
+  c1_10m   q1-q3 at 10M rows, with the last-level cache flushed between
+           runs and warm (those tables fit the 256 MB cache)
   cpu_baseline  the oracle (C restatement of the CPU operator path) on a
            bounded sample of the q1-q7 tables
 
@@ -62,14 +73,14 @@ QUERY_BYTES_PER_ROW = {"q1": 4, "q2": 4, "q3": 8, "q4": 4, "q5": 12, "q6": 8, "q
 # the kernels each query launches over the fact table, with the bytes per
 # fact row each one must move (reads + writes; the partitioned join's
 # scatter reads the key and amount and writes both back in partition order)
-WRITING_KERNELS = {"join_part_scatter*"}  # kernels whose algorithmic bytes include writes
 QUERY_KERNELS = {
-    "q1": [("aggregate_global", 4)],
+    "q1": [("aggregate_global", 4), ("aggregate_global_fold", 0)],
     "q2": [("scan_simple", 4)],
     "q3": [("aggregate_dense", 8)],
     "q4": [("aggregate_dense", 4)],
     "q5": [("qk_scan*", 12)],
-    "q6": [("join_part_hist", 4), ("join_part_scatter*", 16), ("join_part_probe*", 8), ("join_probe", 8)],
+    "q6": [("join_part_hist*", 4), ("join_part_offsets", 0), ("join_part_scatter*", 16), ("join_part_probe*", 8),
+           ("join_probe", 8)],
     "q7": [("aggregate_dense", 4)],
     "q8": [("scan_simple", 0)],
 }
@@ -208,21 +219,44 @@ def timed_cpu_baseline(sample_rows: int, seed: int, workers: int, min_seconds: f
             bytes_per_suite / sum(one.values()) / 1e9)
 
 
-def measured_traffic(kernel: str, rows: int):
-    """HBM bytes per launch of `kernel` from the rocprofv3 --pmc summary kept
-    in profiles/traffic.json (FETCH_SIZE + WRITE_SIZE, scaled by a plain
-    read of known size), when that summary covers this kernel at this row
-    count; else None."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+TRAFFIC_SOURCE = ("profiles/traffic.json: the TCC's HBM read requests by size and write requests, per dispatch, of "
+                  "`bench.py --steps 1 --warmup 0 --calibrate` under two rocprofv3 --pmc runs on this 1B-row "
+                  "workload (not this run; counters cannot be read from inside the process)")
+# dispatch order of one suite run: which dispatch of a shared kernel belongs to which query
+TRAFFIC_DISPATCH = {("q3", "aggregate_dense"): 0, ("q4", "aggregate_dense"): 1, ("q7", "aggregate_dense"): 2,
+                    ("q2", "scan_simple"): 0}
+
+
+def _traffic_record(rows: int):
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    k = rec.get("kernels", {}).get(kernel.split(":", 1)[-1].rstrip("*"))
-    if not k or rec.get("rows") != rows:
+    return rec if rec.get("rows") == rows else None
+
+
+def measured_traffic(query: str, kernels, rows: int):
+    """HBM bytes one run of `query` moves through `kernels` (profile names,
+    a trailing * a prefix), from profiles/traffic.json when it covers these
+    kernels at this row count; else None."""
+    rec = _traffic_record(rows)
+    if not rec:
         return None
-    return int(k["bytes"])
+    total = 0
+    for k in kernels:
+        base = k.split(":", 1)[-1].rstrip("*")
+        e = rec["kernels"].get(base)
+        if not e:
+            return None
+        i = TRAFFIC_DISPATCH.get((query, base))
+        if i is not None:
+            if i >= len(e.get("dispatch_bytes", [])):
+                return None
+            total += e["dispatch_bytes"][i]
+        else:
+            total += e["bytes"]
+    return total
 
 
 EVENT_ROWS = 10_000_000
@@ -271,6 +305,12 @@ def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True
 
     log(f"batched/served: building the {EVENT_ROWS}-row 30-column event table")
     ev = wl.upload(wl.event_table(EVENT_ROWS, seed=7), wl.event_nulls(EVENT_ROWS, seed=7))  # 2% NULL timestamps
+    # GROUP BY user_id (Zipf over 1M users) on the hash path: the
+    # reference's "group by user (1M groups)" (README.md:46, 56 ms)
+    wall, kms, res = _timed(lib, lambda: ev.aggregate([cq.COUNT, cq.SUM], [None, wl.M0], group=wl.USER))
+    by_user = {"groups": res.groups, "path": cq.AGG_PATHS.get(res.trace["path"]), "wall_ms": round(wall, 3),
+               "kernel_ms": round(kms, 3)}
+    log(f"group by user: {res.groups} groups, {kms:.2f} ms kernel, {wall:.2f} ms wall")
     specs = wl.mixed_queries(64, seed=11)
     for _ in range(2):
         for sp in specs:
@@ -300,7 +340,7 @@ def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True
                "speedup": round(plain_s / batch_s, 2), "launches": tr["launches"],
                "union_bytes_per_batch": tr["bytes_read"],
                "shared_scan_gbs": round(tr["bytes_read"] / (scan_ns / reps) if scan_ns else 0.0, 1),
-               "shared_scan_timed": timed}
+               "shared_scan_timed": timed, "group_by_user_hash": by_user}
     if not served_too:
         ev.close()
         return batched, None
@@ -444,6 +484,114 @@ def streamed_run(cq, lib, t, rows: int, repeats: int = 3):
     return out
 
 
+def _timed(lib, fn, reps: int = 3, flush=None):
+    """median wall ms and summed hipEvent kernel ms of fn over reps runs
+    (after one warm run); flush() runs before each timed run"""
+    fn()
+    walls, kns, out = [], [], None
+    for _ in range(reps):
+        if flush:
+            flush()
+        lib.gpu_hip_wait()
+        lib.gpu_hip_profile_reset()
+        t0 = time.perf_counter()
+        out = fn()
+        lib.gpu_hip_wait()
+        walls.append(1e3 * (time.perf_counter() - t0))
+        ns, _ = profile_stats(lib, "*")
+        kns.append(ns / 1e6)
+    walls.sort()
+    kns.sort()
+    return walls[len(walls) // 2], kns[len(kns) // 2], out
+
+
+def extra_measurements(cq, lib, t, rows: int) -> dict:
+    """Rows of the 1B-row configuration outside the timed suite:
+    - GROUP BY amount (10k groups) over non-key arguments: the reference's
+      published COUNT(*), SUM(region) (bench_select_scale.c:1153-1156) and
+      COUNT/SUM/MIN/MAX(ts), 8 B/row each;
+    - the point lookup's scan form (zone maps off: all 4 GB of id read);
+    - the first hiprtc compile of a new WHERE shape (disk cache off) and
+      the same shape again."""
+    from computeql_amd.ir import col
+
+    out = {}
+    for name, funcs, args in (("count_sum_region_by_amount", [cq.COUNT, cq.SUM], [None, 1]),
+                              ("count_sum_min_max_ts_by_amount", [cq.COUNT, cq.SUM, cq.MIN, cq.MAX], [None, 3, 3, 3])):
+        wall, kms, res = _timed(lib, lambda: t.aggregate(funcs, args, group=2))
+        nbytes = 8 * rows
+        out[name] = {"groups": res.groups, "path": cq.AGG_PATHS.get(res.trace["path"]), "wall_ms": round(wall, 3),
+                     "kernel_ms": round(kms, 3), "kernel_gbs": round(nbytes / kms / 1e6, 1),
+                     "frac": round(nbytes / kms / 1e6 / PEAK_GBS, 4)}
+        log(f"{name}: {out[name]['path']} {kms:.2f} ms kernel, {wall:.2f} ms wall")
+    k = point_key(7, rows)
+    cq.set_setting("QE_ZONEMAP", 0)
+    try:
+        wall, kms, res = _timed(lib, lambda: t.scan(col(0) == k))
+    finally:
+        cq.set_setting("QE_ZONEMAP", 1)
+    pruned_wall, _, _ = _timed(lib, lambda: t.scan(col(0) == k))
+    out["point_lookup"] = {"pruned_wall_ms": round(pruned_wall, 3), "scan_form_wall_ms": round(wall, 3),
+                           "scan_form_kernel_ms": round(kms, 3), "scan_form_gbs": round(4 * rows / kms / 1e6, 1),
+                           "found": int(res[0])}
+    cq.set_setting("QE_KERNEL_DISK_CACHE", 0)
+    try:
+        # a shape no other query of this run has: the first call compiles
+        w = ((col(1) == 11) | (col(2) == 4242) | (col(1) == 13)) & ((col(3) < 12345) | (col(2) > 9990))
+        _, _, tr1 = t.scan(w, count_only=True)
+        t0 = time.perf_counter()
+        _, _, tr2 = t.scan(w, count_only=True)
+        second = 1e3 * (time.perf_counter() - t0)
+    finally:
+        cq.set_setting("QE_KERNEL_DISK_CACHE", 1)
+    out["hiprtc_first_compile"] = {"compile_ms": round(tr1["compile_ns"] / 1e6, 1),
+                                   "first_wall_ms": round(tr1["wall_ns"] / 1e6, 1),
+                                   "second_wall_ms": round(second, 3), "second_compile_ms": tr2["compile_ns"] / 1e6}
+    log(f"hiprtc first compile {tr1['compile_ns'] / 1e6:.1f} ms")
+    return out
+
+
+def c1_measurements(cq, lib, big, seed: int) -> dict:
+    """Config 1: the 10M-row table (40 MB a column), q1-q3.  40 MB sits in
+    the 256 MB last-level cache, so each query is timed both with that cache
+    flushed first (a 4 GB column of the big table read in between) and
+    warm; only the flushed figure is an HBM rate."""
+    from computeql_amd.ir import col
+
+    n = 10_000_000
+    t = cq.Table(n)
+    for name in ("id", "region", "amount", "ts"):
+        t.add_column(name, cq.U32)
+    t.gen_events(seed, id=0, region=1, amount=2, ts=3)
+    flush = lambda: cq.bandwidth_read(big, 3)
+    qs = {"q1": (lambda: t.aggregate([cq.SUM, cq.MIN, cq.MAX], [2, 2, 2]), 4),
+          "q2": (lambda: t.scan(col(2) < 200, count_only=True), 4),
+          "q3": (lambda: t.aggregate([cq.COUNT, cq.SUM], [None, 2], group=1), 8)}
+    out = {"rows": n}
+    for q, (fn, bpr) in qs.items():
+        cw, ck, _ = _timed(lib, fn, reps=5, flush=flush)
+        ww, wk, _ = _timed(lib, fn, reps=5)
+        nbytes = bpr * n
+        out[q] = {"llc_flushed": {"wall_us": round(1e3 * cw, 1), "kernel_us": round(1e3 * ck, 1),
+                                  "kernel_gbs": round(nbytes / ck / 1e6, 1)},
+                  "llc_warm_cache_bound": {"wall_us": round(1e3 * ww, 1), "kernel_us": round(1e3 * wk, 1),
+                                           "kernel_gbs": round(nbytes / wk / 1e6, 1)}}
+    t.close()
+    log(f"c1: q1 {out['q1']['llc_flushed']['kernel_us']} us flushed, {out['q1']['llc_warm_cache_bound']['kernel_us']} us warm")
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def device_ceilings(cq, lib, t, reps: int = 3) -> dict:
     """What this device sustains for the two traffic shapes of the suite,
     measured in the same run with hipEvents: a plain read of one 4 GB column
@@ -546,37 +694,59 @@ def main(argv=None) -> int:
     value = total_bytes / elapsed / 1e9
     # per-query kernel times from HIP events recorded on the launch stream
     kernels = {}
+    kernel_ns = {q: 0 for q in QUERY_KERNELS}
+    kernel_names = {q: [] for q in QUERY_KERNELS}
     for q, ks in QUERY_KERNELS.items():
         for k, bpr in ks:
             ns, d = profile_stats(lib, f"{q}:{k}")
             if not d:
                 continue
-            avg_us = ns / d / 1e3
-            entry = {"dispatches": d, "avg_us": round(avg_us, 2), "bytes_per_row": bpr}
-            if bpr:
-                entry["gbs"] = round(bpr * rows / (avg_us * 1e-6) / 1e9, 1)
-            kernels[f"{q}:{k}"] = entry
-    # dominant kernel: the most device time among kernels that stream the table
-    dom_name = max((n for n, e in kernels.items() if e["bytes_per_row"]),
-                   key=lambda n: kernels[n]["avg_us"] * kernels[n]["dispatches"])
-    dom = kernels[dom_name]
-    bytes_per_launch = dom["bytes_per_row"] * rows
-    achieved = bytes_per_launch / (dom["avg_us"] * 1e-6) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_GBS, 4), "traffic": measured_traffic(dom_name, rows),
-                "kernel": dom_name, "bytes_per_launch": bytes_per_launch, "avg_us": dom["avg_us"]}
+            kernel_ns[q] += ns
+            kernel_names[q].append(k)
+            kernels[f"{q}:{k}"] = {"dispatches": d, "avg_us": round(ns / d / 1e3, 2), "bytes_per_row": bpr}
     ceilings = device_ceilings(cq, lib, t)
-    roofline["ceilings"] = ceilings
-    # the scatter's bytes are half reads, half writes: its ceiling is the copy's
-    ceil = ceilings["copy_gbs"] if dom_name.split(":", 1)[1] in WRITING_KERNELS else ceilings["read_gbs"]
-    roofline["frac_of_measured_ceiling"] = round(achieved / ceil, 4) if ceil else None
     log(f"ceilings: read {ceilings['read_gbs']} GB/s, copy {ceilings['copy_gbs']} GB/s")
+    # each query's roofline on the algorithmic bytes of SURVEY 8(d): rows x
+    # 4 B per u32 column it reads in full, over its kernels' time per run
+    per_query = {}
+    for q, bpr in QUERY_BYTES_PER_ROW.items():
+        kms = kernel_ns[q] / a.steps / 1e6
+        nbytes = bpr * rows
+        gbs = nbytes / kms / 1e6 if kms else None
+        traffic = measured_traffic(q, kernel_names[q], rows)
+        per_query[q] = {"algorithmic_bytes": nbytes, "kernel_ms": round(kms, 3),
+                        "wall_ms": round(1e3 * wall[q] / a.steps, 3), "kernel_gbs": round(gbs, 1) if gbs else None,
+                        "frac": round(gbs / PEAK_GBS, 4) if gbs else None,
+                        "frac_of_read_ceiling": round(gbs / ceilings["read_gbs"], 4) if gbs and ceilings["read_gbs"] else None,
+                        "wall_gbs": round(nbytes / (wall[q] / a.steps) / 1e9, 1) if wall[q] else None,
+                        "traffic_bytes": traffic,
+                        "traffic_over_algorithmic": round(traffic / nbytes, 2) if traffic else None,
+                        "kernels": kernel_names[q]}
+    # the dominant unit: the query with the most device time, its kernels
+    # together (their bytes are the query's algorithmic bytes)
+    dom_q = max(per_query, key=lambda q: per_query[q]["kernel_ms"])
+    dom = per_query[dom_q]
+    chain = " + ".join(k.rstrip("*") for k in kernel_names[dom_q])
+    if dom_q == "q6":
+        chain = chain.replace("join_part_probe", res["q6"].trace["kernel"])
+    roofline = {"bound": "hbm", "achieved": dom["kernel_gbs"], "peak": PEAK_GBS, "unit": "GB/s", "frac": dom["frac"],
+                "traffic": dom["traffic_bytes"], "traffic_source": TRAFFIC_SOURCE,
+                "kernel": f"{dom_q}: {chain}", "bytes_per_launch": dom["algorithmic_bytes"],
+                "avg_us": round(1e3 * dom["kernel_ms"], 1),
+                "note": "achieved = the query's algorithmic bytes (rows x 4 B x u32 columns read in full) / the summed "
+                        "hipEvent time of its kernels per run; traffic = HBM bytes those kernels move per run",
+                "ceilings": ceilings,
+                "frac_of_measured_ceiling": round(dom["kernel_gbs"] / ceilings["read_gbs"], 4)
+                if ceilings["read_gbs"] else None}
     if a.calibrate:
         # a plain read of one known 4-byte column: its counter reading scales
         # the FETCH_SIZE of the kernels in a --pmc run of this command
         log("calibration read")
         cq.bandwidth_read(t, 2)
         lib.gpu_hip_wait()
+
+    extra = extra_measurements(cq, lib, t, rows)
+    c1 = c1_measurements(cq, lib, t, seed)
 
     encoded = None
     if not a.no_encoded:
@@ -603,7 +773,7 @@ def main(argv=None) -> int:
             workers = os.cpu_count() or 1
         log(f"cpu baseline: {a.cpu_sample_rows} rows, {workers} threads, at least {a.cpu_seconds} s")
         gbs, reps, secs, gbs1 = timed_cpu_baseline(a.cpu_sample_rows, seed, workers, a.cpu_seconds)
-        cpu = {"value": round(gbs, 3), "unit": "GB/s", "cores": workers, "kind": "port",
+        cpu = {"value": round(gbs, 3), "unit": "GB/s", "cores": workers, "kind": "port", "cpu_model": cpu_model(),
                "sample": f"q1-q7 run {reps} times over a {a.cpu_sample_rows}-row sample of the same seeded tables "
                          f"(the oracle's C restatement of the CPU operator path, {workers} threads, "
                          f"{sum(secs.values()):.1f} s)",
@@ -626,10 +796,13 @@ def main(argv=None) -> int:
         "config": {"workload": f"{rows}-row 4xu32 fact table + u32 fk, dim A 1M rows, dim B 50 rows; q1-q7 per step",
                    "rows": rows, "pct_hbm_peak": round(100 * value / PEAK_GBS, 2),
                    "per_query_wall_ms": {q: round(1e3 * v / a.steps, 3) for q, v in wall.items()},
-                   "kernels": kernels, "invariants": "ok" if not errs else sorted(set(errs)),
+                   "kernels": kernels, "per_query": per_query,
+                   "invariants": "ok" if not errs else sorted(set(errs)),
                    "settings": settings},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extra": extra,
+        "c1_10m": c1,
         "encoded": encoded,
         "streamed": streamed,
         "batched": batched,

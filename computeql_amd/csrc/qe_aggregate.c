@@ -425,6 +425,24 @@ static B32 qe_dense_totals_alloc(QE_DenseTotals* d, U32 range, U32 args)
 
 U64 qe_dense_totals_bytes(U32 range, U32 args) { return (U64)range * 8 * (1 + args) + (U64)range * 4 * 2 * args; }
 
+/* LDS bytes of a dense GROUP BY block: wide tables count u32 | sums u64 |
+ * min/max u32 per argument; narrow ones count+carries u32 | sum low word
+ * u32 | min/max u32 for one argument */
+U64 qe_dense_lds(U32 range, U32 args, B32 need_sum, B32 need_minmax, B32 narrow)
+{
+  if (narrow) return (U64)range * (4 + (need_sum ? 4 : 0) + (need_minmax ? 8 : 0));
+  return (U64)range * 4 + (need_sum ? (U64)args * range * 8 : 0) + (need_minmax ? (U64)args * range * 8 : 0);
+}
+
+/* the narrow kernels take one U32 argument (or its narrow codes) and at
+ * least one of SUM and MIN/MAX */
+B32 qe_dense_narrow_ok(const QE_Table* t, const U32* arg_cols, U32 args, B32 need_sum, B32 need_minmax, U32 range)
+{
+  return qe_settings_u64("QE_AGG_DENSE_NARROW", 1) != 0 && args == 1 && (need_sum || need_minmax) &&
+         t->columns[arg_cols[0]].type == CQL_ColumnType_U32 &&
+         qe_dense_lds(range, 1, need_sum, need_minmax, 1) <= gpu_hip_max_lds_bytes();
+}
+
 /* runs aggregate_dense over the keys of `keys` (columns of `t`) and args;
  * fills totals indexed by the keys' combined digits */
 B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32 arg_count, B32 need_sum,
@@ -440,12 +458,23 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
   }
   const U32 range = (U32)span;
   U32 flags = (need_sum ? 1u : 0u) | (need_minmax ? 2u : 0u);
-  U64 lds = (U64)range * 4 + ((flags & 1u) ? (U64)arg_count * range * 8 : 0) + ((flags & 2u) ? (U64)arg_count * range * 8 : 0);
+  U64 lds = qe_dense_lds(range, arg_count, need_sum, need_minmax, 0);
   U32 max_lds = gpu_hip_max_lds_bytes();
+  /* the narrow tables (aggregate_dense_narrow_*) when the wide ones do not
+   * fit or hold a CU to one 256-thread block */
+  const B32 narrow = qe_dense_narrow_ok(t, arg_cols, arg_count, need_sum, need_minmax, range) &&
+                     (lds > max_lds || lds > qe_settings_u64("QE_AGG_DENSE_NARROW_ABOVE", 65536));
+  if (narrow) lds = qe_dense_lds(range, arg_count, need_sum, need_minmax, 1);
   if (lds > max_lds) {
     qe_set_error("dense group by: %llu bytes of LDS needed, %u available", (unsigned long long)lds, max_lds);
     return 0;
   }
+  if (narrow) {
+    k = qe_kernel(need_sum && need_minmax ? "aggregate_dense_narrow_sm"
+                                          : (need_sum ? "aggregate_dense_narrow_s" : "aggregate_dense_narrow_m"));
+    if (!k) return 0;
+  }
+  const U32 wg = narrow ? 1024u : QE_WG;
   QE_ColumnSlots slots;
   memset(&slots, 0, sizeof(slots));
   /* encoded columns (a3) are read as codes: type byte U32 | encoding << 4,
@@ -508,6 +537,11 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
   U64 lds_blocks = lds ? (160ull * 1024 / (lds + 1024)) : 8;
   if (lds_blocks < 1) lds_blocks = 1;
   if (lds_blocks > qe_setting_blocks_per_cu()) lds_blocks = qe_setting_blocks_per_cu();
+  if (narrow) {
+    /* 1024-thread blocks of 4096-quad steps: as many as fit, one step each at least */
+    U64 need = (quads + 4096 - 1) / 4096;
+    blocks = (U32)(need < cus * lds_blocks ? (need ? need : 1) : cus * lds_blocks);
+  }
   if ((U64)blocks > cus * lds_blocks) blocks = (U32)(cus * lds_blocks);
   U64 gbytes = qe_dense_totals_bytes(range, arg_count);
   GPU_Buffer* acc = qe_scratch("dense_totals", gbytes);
@@ -550,7 +584,7 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
       gpu_batch_buffer_fill(b, acc, zero_bytes + (U64)range * 4 * arg_count, 0xffffffffu);
       gpu_batch_buffer_zero(b, acc, zero_bytes);
     }
-    gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
+    gpu_batch_kernel_execute(b, k, blocks * wg, wg);
     gpu_batch_buffer_read(b, acc, out->block, gbytes);
     ok = gpu_batch_end(b);
     gpu_hip_kernel_set_lds_bytes(k, 0);
@@ -587,7 +621,7 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     U64 bytes = 0;
     for (U32 s = 0; s < slots.count; s++) bytes += (r1 - r0) * t->columns[slots.column[s]].width;
     trace->bytes_read = bytes;
-    snprintf(trace->kernel, sizeof(trace->kernel), "aggregate_dense");
+    snprintf(trace->kernel, sizeof(trace->kernel), "%s", narrow ? "aggregate_dense_narrow" : "aggregate_dense");
   }
   return 1;
 }
@@ -1021,11 +1055,13 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
     if (arg_key[a] < 0) fold = 0;
   }
   /* the block's LDS tables must fit: counts u32[R], sums u64[E][R],
-   * min/max u32[E][R] */
+   * min/max u32[E][R]; or the narrow tables of one argument */
   if (dense) {
-    const U64 e = fold ? 0 : plan.arg_count;
-    const U64 lds = span * 4 + (plan.need_sum ? span * 8 * e : 0) + (plan.need_minmax ? span * 8 * e : 0);
-    if (lds > gpu_hip_max_lds_bytes()) dense = 0;
+    const U32 e = fold ? 0 : plan.arg_count;
+    const U64 lds = qe_dense_lds((U32)span, e, plan.need_sum, plan.need_minmax, 0);
+    if (lds > gpu_hip_max_lds_bytes() &&
+        (fold || !qe_dense_narrow_ok(table, plan.arg_col, plan.arg_count, plan.need_sum, plan.need_minmax, (U32)span)))
+      dense = 0;
   }
   if (!dense) {
     U64 groups = 0;

@@ -304,6 +304,8 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
                  B32 need_minmax, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune, QE_DenseTotals* out,
                  QE_Trace* trace);
 void qe_dense_totals_free(QE_DenseTotals* d);
+U64 qe_dense_lds(U32 range, U32 args, B32 need_sum, B32 need_minmax, B32 narrow);
+B32 qe_dense_narrow_ok(const QE_Table* t, const U32* arg_cols, U32 args, B32 need_sum, B32 need_minmax, U32 range);
 U64 qe_dense_totals_bytes(U32 range, U32 args);
 
 #endif

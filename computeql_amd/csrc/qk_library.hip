@@ -670,6 +670,180 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
 }
 
 /* ------------------------------------------------------------------ */
+/* aggregate_dense_narrow: aggregate_dense for group ranges too large    */
+/* for its LDS tables (10k groups x 20 B), with one U32 argument, in     */
+/* 1024-thread blocks and at most 16 bytes a group:                      */
+/*   cc u32[R]     count in bits 0-15, carries of the sum's low word in  */
+/*                 bits 16-31                                             */
+/*   lo u32[R]     the sum's low 32 bits (SUM)                            */
+/*   mn, mx u32[R] (MM)                                                   */
+/* The lane whose add takes a group's count half from 0x7fff to 0x8000,  */
+/* or its carry half likewise, moves 0x8000 of it to the global totals   */
+/* at once (an LDS subtract and a global add): each half only grows      */
+/* between that add and the subtract, so neither half wraps however the  */
+/* rows fall, and the block adds what is left at its end.  Bindings,     */
+/* push and words as aggregate_dense (binding 2 several keys), E = 1;    */
+/* global layout count u64[R] | sum u64[R] | min u32[R] | max u32[R].    */
+/* ------------------------------------------------------------------ */
+#define QK_NARROW_WG 1024
+template <bool SUM, bool MM>
+__device__ __forceinline__ void qk_dense_narrow(const QK_Args& a)
+{
+  constexpr int WG = QK_NARROW_WG, U = QK_UNROLL;
+  extern __shared__ qk_u32 lds_n[];
+  const qk_u64 rows = qk_push(a, QK_P_ROWS);
+  const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
+  const qk_u64 r1 = r0 + rows;
+  const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
+  const qk_u32 types = a.words[QK_W_TYPES];
+  const qk_u32 key_slot = a.words[QK_W_KEY] & 0xffu;
+  const qk_u32 key_type = (a.words[QK_W_KEY] >> 8) & 0xfu;
+  const qk_u32 key_enc = (a.words[QK_W_KEY] >> 12) & 3u;
+  const qk_u32 key_min = a.words[QK_W_KEY_MIN];
+  const qk_u32 range = a.words[QK_W_RANGE];
+  const QkLeafSet L = qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES));
+  const qk_u32* kd = a.ptr[2] ? qk_ptr<qk_u32>(a, 2) : nullptr;
+  const qk_u32 nkeys = kd ? kd[0] : 1u;
+  qk_u32* l_cc = lds_n;
+  qk_u32* l_lo = l_cc + range;
+  qk_u32* l_mn = l_lo + (SUM ? range : 0u);
+  qk_u32* l_mx = l_mn + (MM ? range : 0u);
+  for (qk_u32 i = threadIdx.x; i < range; i += WG) {
+    l_cc[i] = 0u;
+    if (SUM) l_lo[i] = 0u;
+    if (MM) {
+      l_mn[i] = 0xffffffffu;
+      l_mx[i] = 0u;
+    }
+  }
+  __syncthreads();
+  qk_u64* g_cnt = qk_wptr<qk_u64>(a, 1);
+  qk_u64* g_sum = g_cnt + range;
+  qk_u32* g_min = (qk_u32*)(g_sum + range);
+  qk_u32* g_max = g_min + range;
+  const qk_u32 aenc = (types >> 4) & 3u;
+  const qk_u64 cp = a.ptr[QK_B_COL0 + (slots & 0xfu)];
+  /* the block's contiguous quad range, whole steps of WG x U quads */
+  qk_u64 b0, b1;
+  {
+    const qk_u64 q0 = r0 / 4, q1 = (r1 + 3) / 4, step = (qk_u64)WG * U;
+    qk_u64 per = (q1 - q0 + gridDim.x - 1) / gridDim.x;
+    per = (per + step - 1) / step * step;
+    b0 = q0 + per * blockIdx.x;
+    if (b0 > q1) b0 = q1;
+    b1 = b0 + per < q1 ? b0 + per : q1;
+  }
+  for (qk_u64 s = b0; s < b1; s += (qk_u64)WG * U) {
+    qk_u64 q[U];
+    qk_u32 m[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const qk_u64 qq = s + (qk_u64)j * WG + threadIdx.x;
+      const bool in = qq < b1;
+      qk_u32 v = in ? qk_quad_valid(qq, r0, r1) : 0u;
+      if (v && qk_quad_skipped(a, qq)) v = 0u;
+      m[j] = v;
+      q[j] = in ? qq : b1 - 1;
+    }
+    if (L.n) qk_leaves_step(a, L, q, m);
+    qk_u32 g[U][4];
+    if (nkeys == 1) {
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const qk_u32x4 kv = key_enc == 0 ? qk_ptr<qk_u32x4>(a, QK_B_COL0 + key_slot)[q[j]]
+                                         : qk_quad_codes(a.ptr[QK_B_COL0 + key_slot], key_enc, q[j]);
+        const qk_u32 w[4] = {kv.x, kv.y, kv.z, kv.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          g[j][k] = qk_code32(w[k], key_type) - key_min;
+          if (g[j][k] >= range) m[j] &= ~(1u << k);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < U; j++) g[j][0] = g[j][1] = g[j][2] = g[j][3] = 0u;
+      for (qk_u32 kk = 0; kk < nkeys; kk++) {
+        const qk_u32 d = kd[1 + 3 * kk], kmin = kd[2 + 3 * kk], kr = kd[3 + 3 * kk];
+        const qk_u32 kt = (d >> 8) & 0xfu, ke = (d >> 12) & 3u;
+        const qk_u64 kp = a.ptr[QK_B_COL0 + (d & 0xffu)];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+          const qk_u32x4 x = ke == 0 ? qk_gptr<const qk_u32x4>(kp)[q[j]] : qk_quad_codes(kp, ke, q[j]);
+          const qk_u32 w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const qk_u32 dg = qk_code32(w[k], kt) - kmin;
+            if (dg >= kr) m[j] &= ~(1u << k);
+            g[j][k] = g[j][k] * kr + dg;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (g[j][k] >= range) m[j] &= ~(1u << k);
+    }
+    qk_u32x4 x[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) x[j] = aenc == 0 ? qk_gptr<const qk_u32x4>(cp)[q[j]] : qk_quad_codes(cp, aenc, q[j]);
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!(m[j] & (1u << k))) continue;
+        const qk_u32 gi = g[j][k], v = w[k];
+        qk_u32 inc = 1u;
+        if (SUM) {
+          const qk_u32 old = atomicAdd(&l_lo[gi], v);
+          if (old + v < old) inc += 0x10000u;
+        }
+        const qk_u32 o = atomicAdd(&l_cc[gi], inc);
+        if ((o & 0xffffu) == 0x7fffu) {
+          atomicSub(&l_cc[gi], 0x8000u);
+          atomicAdd(&g_cnt[gi], 0x8000ULL);
+        }
+        if (SUM && (inc >> 16) && (o >> 16) == 0x7fffu) {
+          atomicSub(&l_cc[gi], 0x80000000u);
+          atomicAdd(&g_sum[gi], 0x8000ULL << 32);
+        }
+        if (MM) {
+          atomicMin(&l_mn[gi], v);
+          atomicMax(&l_mx[gi], v);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (qk_u32 i = threadIdx.x; i < range; i += WG) {
+    const qk_u32 c = l_cc[i];
+    if (c & 0xffffu) atomicAdd(&g_cnt[i], (qk_u64)(c & 0xffffu));
+    if (SUM) {
+      const qk_u64 add = ((qk_u64)(c >> 16) << 32) + l_lo[i];
+      if (add) atomicAdd(&g_sum[i], add);
+    }
+    if (MM && l_mn[i] <= l_mx[i]) {
+      atomicMin(&g_min[i], l_mn[i]);
+      atomicMax(&g_max[i], l_mx[i]);
+    }
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(QK_NARROW_WG) k_aggregate_dense_narrow_s(QK_Args a)
+{
+  qk_dense_narrow<true, false>(a);
+}
+extern "C" __global__ void __launch_bounds__(QK_NARROW_WG) k_aggregate_dense_narrow_m(QK_Args a)
+{
+  qk_dense_narrow<false, true>(a);
+}
+extern "C" __global__ void __launch_bounds__(QK_NARROW_WG) k_aggregate_dense_narrow_sm(QK_Args a)
+{
+  qk_dense_narrow<true, true>(a);
+}
+
+/* ------------------------------------------------------------------ */
 /* join_build: open-addressing table of the dim keys on the device.     */
 /*  binding 0 keys (u32), 1 dim group column (u32/i32, payload mode 1), */
 /*  2 table (u64 slots, 0 = empty, entry = (payload+1) << 32 | key),    */
@@ -3251,6 +3425,9 @@ static const QK_LibraryKernel g_library[] = {
     {"scan_simple", (const void*)k_scan_simple, QK_WG},
     {"scan_filter", (const void*)k_scan_filter, QK_WG},
     {"aggregate_dense", (const void*)k_aggregate_dense, QK_WG},
+    {"aggregate_dense_narrow_s", (const void*)k_aggregate_dense_narrow_s, QK_NARROW_WG},
+    {"aggregate_dense_narrow_m", (const void*)k_aggregate_dense_narrow_m, QK_NARROW_WG},
+    {"aggregate_dense_narrow_sm", (const void*)k_aggregate_dense_narrow_sm, QK_NARROW_WG},
     {"join_build", (const void*)k_join_build, QK_WG},
     {"join_probe", (const void*)k_join_probe, QK_WG},
     {"aggregate_hash", (const void*)k_aggregate_hash, QK_WG},

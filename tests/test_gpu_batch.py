@@ -279,3 +279,28 @@ def test_mixed_batch_with_nullable_timestamps(orc):
             hit += 1
     assert hit == 8
     t.close()
+
+
+@pytest.mark.parametrize("compact", [0, 1])
+def test_sparse_and_dense_walks_match_oracle(events, orc, compact):
+    """Grouped queries whose passing rows are sparse in a wave walk only
+    those rows (QE_BATCH_COMPACT 1), denser waves the unrolled walk; with 0
+    every wave takes the unrolled walk.  A very selective WHERE (every wave
+    sparse) and a dense one, over f64 and i32-range u32 arguments with SUM,
+    MIN and MAX, and keys outside the query's WHERE range, against the
+    oracle either way."""
+    t, cols = events
+    m, u = wl.M0 + 2, wl.U0 + 1
+    specs = [
+        cq.AggSpec([cq.COUNT, cq.SUM, cq.MIN, cq.MAX], [None, m, m, m], col(u) < 2000, wl.REGION),          # ~0.2%
+        cq.AggSpec([cq.COUNT, cq.SUM, cq.MIN, cq.MAX], [None, u, u, u], col(u) < 1500, wl.STATUS),          # sparse
+        cq.AggSpec([cq.COUNT, cq.SUM, cq.MIN, cq.MAX], [None, m, m, m], col(u) >= 1000, wl.CATEGORY),       # dense
+        cq.AggSpec([cq.SUM, cq.MAX, cq.MIN], [u, u, m], (col(wl.REGION) < 45) & (col(u) > 10), wl.STATUS),  # dense
+    ]
+    cq.set_setting("QE_BATCH_COST", 0)
+    cq.set_setting("QE_BATCH_COMPACT", compact)
+    res = cq.aggregate_batch(t, specs)
+    for sp, r in zip(specs, res):
+        assert r.trace["fused"] == 1
+        rk, rv = orc.group_aggregate(cols, sp.group, sp.funcs, sp.args, where=sp.where)
+        _same(sp, r.keys, _vals(r), rk, rv, (compact, sp.funcs))
