@@ -27,7 +27,10 @@
 
 typedef struct QE_Request {
   QE_Table* table;
+  U32 kind; /* QE_REQUEST_* */
   QE_BatchQuery* q;
+  QE_JoinQuery* join;
+  QE_ScanQuery* scan;
   U64 footprint;
   U64 t_submit, t_done;
   U32 state; /* 0 queued, 1 running, 2 done */
@@ -46,7 +49,7 @@ struct QE_Server {
   U64 inflight_bytes;
   B32 stop;
   /* stats since the last reset */
-  U64 submitted, completed, failed, batches, batched_queries;
+  U64 submitted, completed, failed, batches, batched_queries, joins, scans;
   U64 t_first_submit, t_last_done;
   U64 inflight_peak_bytes, queue_peak;
   U64* lat;
@@ -66,6 +69,20 @@ static U64 qe_serve_footprint(QE_Table* t, const QE_BatchQuery* q)
     if (span && span < groups) groups = span;
   }
   return 65536 + groups * (1 + 3 * 4 + 2 * QE_AGG_MAX_GROUP_COLS) * 8 * 4;
+}
+
+/* a join's partition buffers (key and arguments of every fact row, in
+ * partition order) and its dimension's table; a scan's id buffer and block
+ * counts */
+static U64 qe_serve_footprint_join(QE_Table* fact, const QE_JoinQuery* j)
+{
+  const U64 dim_rows = j->dim ? j->dim->row_count : 0;
+  return 65536 + fact->row_count * 4 * (1 + (U64)j->expr_count) + dim_rows * 16 + j->cap * 8 * 8;
+}
+
+static U64 qe_serve_footprint_scan(QE_Table* t, const QE_ScanQuery* q)
+{
+  return 65536 + (q->count_only ? 0 : q->cap * 8) + t->row_count / 256;
 }
 
 static int qe_u64_cmp(const void* a, const void* b)
@@ -92,12 +109,21 @@ static void* qe_serve_worker(void* arg)
      * the batch and the memory budget (the head always, when nothing else
      * is in flight, so an oversized request still runs) */
     QE_Table* table = s->head->table;
+    const U32 kind = s->head->kind;
     U32 n = 0;
     QE_Request** link = &s->head;
     QE_Request* prev = 0;
-    while (*link && n < s->max_batch) {
+    while (*link && n < (kind == QE_REQUEST_AGGREGATE ? s->max_batch : 1)) {
       QE_Request* r = *link;
       B32 fits = s->inflight_bytes + r->footprint <= s->hbm_budget || (n == 0 && s->inflight_bytes == 0);
+      if (r->kind != kind) {
+        /* joins and scans run one per launch; aggregates batch with the
+         * aggregates queued behind them */
+        if (n == 0) break;
+        prev = r;
+        link = &r->next;
+        continue;
+      }
       if (r->table == table && fits) {
         *link = r->next;
         if (s->tail == r) s->tail = prev;
@@ -120,9 +146,24 @@ static void* qe_serve_worker(void* arg)
     }
     pthread_mutex_unlock(&s->lock);
 
-    for (U32 i = 0; i < n; i++) qs[i] = *batch[i]->q;
     B32 ok;
-    if (n == 1 && s->max_batch == 1) {
+    if (kind == QE_REQUEST_JOIN) {
+      QE_JoinQuery* j = batch[0]->join;
+      memset(&qs[0], 0, sizeof(qs[0]));
+      qs[0].ok = qe_aggregate_join(table, j->fact_key_col, j->dim, j->dim_key_col, j->dim_where, j->dim_where_count,
+                                   j->group_cols, j->group_count, j->funcs, j->arg_cols, j->expr_count, j->cap,
+                                   j->out_group_count, j->out_keys, j->out_cells, 0);
+    } else if (kind == QE_REQUEST_SCAN) {
+      QE_ScanQuery* q = batch[0]->scan;
+      memset(&qs[0], 0, sizeof(qs[0]));
+      qs[0].ok = qe_scan_filter(table, q->where, q->where_count, q->count_only, q->count_only ? 0 : q->out_ids,
+                                q->count_only ? 0 : q->cap, q->out_count, 0);
+    } else {
+      for (U32 i = 0; i < n; i++) qs[i] = *batch[i]->q;
+    }
+    if (kind != QE_REQUEST_AGGREGATE) {
+      /* ran above */
+    } else if (n == 1 && s->max_batch == 1) {
       /* one query per launch: the plain operator */
       QE_BatchQuery* q = &qs[0];
       q->ok = qe_aggregate(table, q->where, q->where_count, q->group_cols, q->group_count, q->funcs, q->arg_cols,
@@ -139,8 +180,16 @@ static void* qe_serve_worker(void* arg)
     pthread_mutex_lock(&s->lock);
     for (U32 i = 0; i < n; i++) {
       QE_Request* r = batch[i];
-      r->q->ok = qs[i].ok;
-      r->q->fused = qs[i].fused;
+      if (kind == QE_REQUEST_JOIN) {
+        r->join->ok = qs[i].ok;
+        s->joins += qs[i].ok ? 1 : 0;
+      } else if (kind == QE_REQUEST_SCAN) {
+        r->scan->ok = qs[i].ok;
+        s->scans += qs[i].ok ? 1 : 0;
+      } else {
+        r->q->ok = qs[i].ok;
+        r->q->fused = qs[i].fused;
+      }
       r->t_done = now;
       r->state = 2;
       s->inflight_bytes -= r->footprint;
@@ -159,8 +208,10 @@ static void* qe_serve_worker(void* arg)
         s->failed++;
       }
     }
-    s->batches++;
-    s->batched_queries += n;
+    if (kind == QE_REQUEST_AGGREGATE) {
+      s->batches++;
+      s->batched_queries += n;
+    }
     s->t_last_done = now;
     pthread_cond_broadcast(&s->done_cv);
     pthread_cond_broadcast(&s->work_cv);
@@ -196,6 +247,8 @@ QE_Server* qe_server_start(const QE_ServerConfig* config)
   return s;
 }
 
+static U64 qe_server_enqueue(QE_Server* s, QE_Request* r);
+
 U64 qe_server_submit(QE_Server* s, QE_Table* table, QE_BatchQuery* query)
 {
   if (!s || !table || !query || !query->funcs || !query->arg_cols || !query->out_group_count || !query->out_cells) {
@@ -209,10 +262,58 @@ U64 qe_server_submit(QE_Server* s, QE_Table* table, QE_BatchQuery* query)
     return 0;
   }
   r->table = table;
+  r->kind = QE_REQUEST_AGGREGATE;
   r->q = query;
   r->footprint = qe_serve_footprint(table, query);
-  r->t_submit = qe_now_ns();
   query->ok = 0;
+  return qe_server_enqueue(s, r);
+}
+
+U64 qe_server_submit_join(QE_Server* s, QE_Table* fact, QE_JoinQuery* query)
+{
+  if (!s || !fact || !query || !query->dim || !query->funcs || !query->arg_cols || !query->out_group_count ||
+      !query->out_cells || query->group_count > QE_AGG_MAX_GROUP_COLS) {
+    qe_set_error("qe_server_submit_join: bad argument");
+    return 0;
+  }
+  if (!qe_require_device(fact, "qe_server_submit_join") || !qe_require_device(query->dim, "qe_server_submit_join"))
+    return 0;
+  QE_Request* r = (QE_Request*)calloc(1, sizeof(QE_Request));
+  if (!r) {
+    qe_set_error("qe_server_submit_join: out of memory");
+    return 0;
+  }
+  r->table = fact;
+  r->kind = QE_REQUEST_JOIN;
+  r->join = query;
+  r->footprint = qe_serve_footprint_join(fact, query);
+  query->ok = 0;
+  return qe_server_enqueue(s, r);
+}
+
+U64 qe_server_submit_scan(QE_Server* s, QE_Table* table, QE_ScanQuery* query)
+{
+  if (!s || !table || !query || !query->out_count || (!query->count_only && !query->out_ids)) {
+    qe_set_error("qe_server_submit_scan: null argument");
+    return 0;
+  }
+  if (!qe_require_device(table, "qe_server_submit_scan")) return 0;
+  QE_Request* r = (QE_Request*)calloc(1, sizeof(QE_Request));
+  if (!r) {
+    qe_set_error("qe_server_submit_scan: out of memory");
+    return 0;
+  }
+  r->table = table;
+  r->kind = QE_REQUEST_SCAN;
+  r->scan = query;
+  r->footprint = qe_serve_footprint_scan(table, query);
+  query->ok = 0;
+  return qe_server_enqueue(s, r);
+}
+
+static U64 qe_server_enqueue(QE_Server* s, QE_Request* r)
+{
+  r->t_submit = qe_now_ns();
   pthread_mutex_lock(&s->lock);
   if (s->stop) {
     pthread_mutex_unlock(&s->lock);
@@ -266,6 +367,8 @@ void qe_server_stats(QE_Server* s, QE_ServerStats* out)
   out->elapsed_ns = s->t_last_done > s->t_first_submit ? s->t_last_done - s->t_first_submit : 0;
   out->inflight_peak_bytes = s->inflight_peak_bytes;
   out->queue_peak = s->queue_peak;
+  out->joins = s->joins;
+  out->scans = s->scans;
   U64 n = s->lat_count;
   U64* c = n ? (U64*)malloc(n * sizeof(U64)) : 0;
   if (c) memcpy(c, s->lat, n * sizeof(U64));
@@ -286,7 +389,7 @@ void qe_server_stats_reset(QE_Server* s)
 {
   if (!s) return;
   pthread_mutex_lock(&s->lock);
-  s->submitted = s->completed = s->failed = s->batches = s->batched_queries = 0;
+  s->submitted = s->completed = s->failed = s->batches = s->batched_queries = s->joins = s->scans = 0;
   s->t_first_submit = s->t_last_done = 0;
   s->inflight_peak_bytes = s->inflight_bytes;
   s->queue_peak = s->queued;
@@ -324,10 +427,68 @@ void qe_server_stop(QE_Server* s)
  * reset before the first arrival. */
 typedef struct QE_DriveSlot {
   U64 ticket;
+  U32 kind;
   QE_BatchQuery q;
+  QE_JoinQuery j;
+  QE_ScanQuery s;
   U64 group_count;
+  void* out_a; /* keys or ids */
+  void* out_b; /* cells */
   struct QE_DriveSlot* next;
 } QE_DriveSlot;
+
+static void qe_drive_slot_free(QE_DriveSlot* slot)
+{
+  free(slot->out_a);
+  free(slot->out_b);
+  free(slot);
+}
+
+/* a copy of the pool entry with output buffers of its own, submitted */
+static QE_DriveSlot* qe_drive_submit(QE_Server* s, const QE_ServeRequest* p)
+{
+  QE_DriveSlot* slot = (QE_DriveSlot*)calloc(1, sizeof(QE_DriveSlot));
+  if (!slot) return 0;
+  slot->kind = p->kind;
+  if (p->kind == QE_REQUEST_SCAN) {
+    slot->s = *(const QE_ScanQuery*)p->query;
+    const U64 cap = slot->s.count_only ? 0 : slot->s.cap;
+    slot->out_a = malloc((cap ? cap : 1) * sizeof(U64));
+    slot->s.out_ids = (U64*)slot->out_a;
+    slot->s.out_count = &slot->group_count;
+    if (slot->out_a) slot->ticket = qe_server_submit_scan(s, p->table, &slot->s);
+  } else {
+    const U32 groups = p->kind == QE_REQUEST_JOIN ? ((const QE_JoinQuery*)p->query)->group_count
+                                                  : ((const QE_BatchQuery*)p->query)->group_count;
+    const U32 exprs = p->kind == QE_REQUEST_JOIN ? ((const QE_JoinQuery*)p->query)->expr_count
+                                                 : ((const QE_BatchQuery*)p->query)->expr_count;
+    U64 cap = p->kind == QE_REQUEST_JOIN ? ((const QE_JoinQuery*)p->query)->cap
+                                         : ((const QE_BatchQuery*)p->query)->cap;
+    if (!cap) cap = 1;
+    slot->out_a = malloc(cap * (groups ? groups : 1) * sizeof(F64));
+    slot->out_b = malloc(cap * (exprs ? exprs : 1) * sizeof(U64));
+    if (slot->out_a && slot->out_b) {
+      if (p->kind == QE_REQUEST_JOIN) {
+        slot->j = *(const QE_JoinQuery*)p->query;
+        slot->j.out_group_count = &slot->group_count;
+        slot->j.out_keys = (F64*)slot->out_a;
+        slot->j.out_cells = (U64*)slot->out_b;
+        slot->ticket = qe_server_submit_join(s, p->table, &slot->j);
+      } else {
+        slot->q = *(const QE_BatchQuery*)p->query;
+        slot->q.out_group_count = &slot->group_count;
+        slot->q.out_keys = (F64*)slot->out_a;
+        slot->q.out_cells = (U64*)slot->out_b;
+        slot->ticket = qe_server_submit(s, p->table, &slot->q);
+      }
+    }
+  }
+  if (!slot->ticket) {
+    qe_drive_slot_free(slot);
+    return 0;
+  }
+  return slot;
+}
 
 typedef struct QE_Drive {
   QE_Server* server;
@@ -357,9 +518,7 @@ static void* qe_drive_collect(void* arg)
       d->lost++; /* not released: its buffers stay allocated */
       continue;
     }
-    free(slot->q.out_keys);
-    free(slot->q.out_cells);
-    free(slot);
+    qe_drive_slot_free(slot);
   }
   return 0;
 }
@@ -372,8 +531,32 @@ static U64 qe_drive_rand(U64* state)
   return z ^ (z >> 31);
 }
 
-static B32 qe_server_drive_impl(QE_Server* s, QE_Table* table, QE_Table* const* tables, const QE_BatchQuery* pool,
-                                 U32 pool_count, F64 rate_qps, U64 duration_ns, U64 seed, QE_ServerStats* out);
+static B32 qe_server_drive_impl(QE_Server* s, const QE_ServeRequest* pool, U32 pool_count, F64 rate_qps,
+                                 U64 duration_ns, U64 seed, QE_ServerStats* out);
+
+/* the aggregate pools of qe_server_drive / _mix as request pools */
+static B32 qe_server_drive_aggregates(QE_Server* s, QE_Table* table, QE_Table* const* tables,
+                                      const QE_BatchQuery* pool, U32 pool_count, F64 rate_qps, U64 duration_ns,
+                                      U64 seed, QE_ServerStats* out)
+{
+  if (!pool || !pool_count) {
+    qe_set_error("qe_server_drive: bad argument");
+    return 0;
+  }
+  QE_ServeRequest* req = (QE_ServeRequest*)calloc(pool_count, sizeof(QE_ServeRequest));
+  if (!req) {
+    qe_set_error("qe_server_drive: out of memory");
+    return 0;
+  }
+  for (U32 i = 0; i < pool_count; i++) {
+    req[i].kind = QE_REQUEST_AGGREGATE;
+    req[i].table = tables ? tables[i] : table;
+    req[i].query = &pool[i];
+  }
+  const B32 ok = qe_server_drive_impl(s, req, pool_count, rate_qps, duration_ns, seed, out);
+  free(req);
+  return ok;
+}
 
 B32 qe_server_drive(QE_Server* s, QE_Table* table, const QE_BatchQuery* pool, U32 pool_count, F64 rate_qps,
                     U64 duration_ns, U64 seed, QE_ServerStats* out)
@@ -382,7 +565,7 @@ B32 qe_server_drive(QE_Server* s, QE_Table* table, const QE_BatchQuery* pool, U3
     qe_set_error("qe_server_drive: bad argument");
     return 0;
   }
-  return qe_server_drive_impl(s, table, 0, pool, pool_count, rate_qps, duration_ns, seed, out);
+  return qe_server_drive_aggregates(s, table, 0, pool, pool_count, rate_qps, duration_ns, seed, out);
 }
 
 /* the same load over a mix of tables: request k runs pool[k % pool_count]
@@ -400,11 +583,27 @@ B32 qe_server_drive_mix(QE_Server* s, QE_Table* const* tables, const QE_BatchQue
       return 0;
     }
   }
-  return qe_server_drive_impl(s, 0, tables, pool, pool_count, rate_qps, duration_ns, seed, out);
+  return qe_server_drive_aggregates(s, 0, tables, pool, pool_count, rate_qps, duration_ns, seed, out);
 }
 
-static B32 qe_server_drive_impl(QE_Server* s, QE_Table* table, QE_Table* const* tables, const QE_BatchQuery* pool,
-                                 U32 pool_count, F64 rate_qps, U64 duration_ns, U64 seed, QE_ServerStats* out)
+B32 qe_server_drive_requests(QE_Server* s, const QE_ServeRequest* pool, U32 pool_count, F64 rate_qps,
+                             U64 duration_ns, U64 seed, QE_ServerStats* out)
+{
+  if (!pool) {
+    qe_set_error("qe_server_drive_requests: bad argument");
+    return 0;
+  }
+  for (U32 i = 0; i < pool_count; i++) {
+    if (!pool[i].table || !pool[i].query || pool[i].kind > QE_REQUEST_SCAN) {
+      qe_set_error("qe_server_drive_requests: bad request %u", i);
+      return 0;
+    }
+  }
+  return qe_server_drive_impl(s, pool, pool_count, rate_qps, duration_ns, seed, out);
+}
+
+static B32 qe_server_drive_impl(QE_Server* s, const QE_ServeRequest* pool, U32 pool_count, F64 rate_qps,
+                                 U64 duration_ns, U64 seed, QE_ServerStats* out)
 {
   if (!s || !pool || !pool_count || rate_qps <= 0.0) {
     qe_set_error("qe_server_drive: bad argument");
@@ -439,29 +638,8 @@ static B32 qe_server_drive_impl(QE_Server* s, QE_Table* table, QE_Table* const* 
         nanosleep(&ts, 0);
       }
     }
-    const QE_BatchQuery* p = &pool[k % pool_count];
-    QE_DriveSlot* slot = (QE_DriveSlot*)calloc(1, sizeof(QE_DriveSlot));
+    QE_DriveSlot* slot = qe_drive_submit(s, &pool[k % pool_count]);
     if (!slot) {
-      ok = 0;
-      break;
-    }
-    slot->q = *p;
-    U64 cap = p->cap ? p->cap : 1;
-    slot->q.out_group_count = &slot->group_count;
-    slot->q.out_keys = (F64*)malloc(cap * (p->group_count ? p->group_count : 1) * sizeof(F64));
-    slot->q.out_cells = (U64*)malloc(cap * (p->expr_count ? p->expr_count : 1) * sizeof(U64));
-    if (!slot->q.out_keys || !slot->q.out_cells) {
-      free(slot->q.out_keys);
-      free(slot->q.out_cells);
-      free(slot);
-      ok = 0;
-      break;
-    }
-    slot->ticket = qe_server_submit(s, tables ? tables[k % pool_count] : table, &slot->q);
-    if (!slot->ticket) {
-      free(slot->q.out_keys);
-      free(slot->q.out_cells);
-      free(slot);
       ok = 0;
       break;
     }

@@ -73,6 +73,45 @@ class QE_BatchConfig(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class QE_JoinQuery(C.Structure):
+    _fields_ = [
+        ("dim", C.c_void_p),
+        ("fact_key_col", U32),
+        ("dim_key_col", U32),
+        ("dim_where", C.c_void_p),
+        ("dim_where_count", U32),
+        ("group_count", U32),
+        ("group_cols", C.c_int32 * MAX_GROUP_COLS),
+        ("funcs", C.c_void_p),
+        ("arg_cols", C.c_void_p),
+        ("expr_count", U32),
+        ("cap", U64),
+        ("out_group_count", C.c_void_p),
+        ("out_keys", C.c_void_p),
+        ("out_cells", C.c_void_p),
+        ("ok", U32),
+    ]
+
+
+class QE_ScanQuery(C.Structure):
+    _fields_ = [
+        ("where", C.c_void_p),
+        ("where_count", U32),
+        ("count_only", U32),
+        ("cap", U64),
+        ("out_ids", C.c_void_p),
+        ("out_count", C.c_void_p),
+        ("ok", U32),
+    ]
+
+
+REQUEST_AGGREGATE, REQUEST_JOIN, REQUEST_SCAN = 0, 1, 2
+
+
+class QE_ServeRequest(C.Structure):
+    _fields_ = [("kind", U32), ("table", C.c_void_p), ("query", C.c_void_p)]
+
+
 class QE_ServerConfig(C.Structure):
     _fields_ = [("workers", U32), ("max_batch", U32), ("hbm_budget", U64)]
 
@@ -80,7 +119,7 @@ class QE_ServerConfig(C.Structure):
 class QE_ServerStats(C.Structure):
     _fields_ = [(n, U64) for n in ("submitted", "completed", "failed", "batches", "batched_queries", "p50_ns",
                                     "p99_ns", "max_ns", "mean_ns", "elapsed_ns", "inflight_peak_bytes",
-                                    "queue_peak")]
+                                    "queue_peak", "joins", "scans")]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -224,6 +263,9 @@ SIGNATURES = [
     ("qe_server_stop", None, [P]),
     ("qe_server_drive", B32, [P, P, P, U32, C.c_double, U64, U64, P]),
     ("qe_server_drive_mix", B32, [P, P, P, U32, C.c_double, U64, U64, P]),
+    ("qe_server_submit_join", U64, [P, P, P]),
+    ("qe_server_submit_scan", U64, [P, P, P]),
+    ("qe_server_drive_requests", B32, [P, P, U32, C.c_double, U64, U64, P]),
     ("qe_bandwidth_read", B32, [P, U64, U32, P, P]),
     ("qe_bandwidth_copy", B32, [P, P, U64, U32, P]),
 ]

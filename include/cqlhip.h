@@ -607,6 +607,7 @@ typedef struct QE_ServerStats {
   U64 p50_ns, p99_ns, max_ns, mean_ns; /* submit to completion */
   U64 elapsed_ns;                      /* first submit to last completion */
   U64 inflight_peak_bytes, queue_peak;
+  U64 joins, scans; /* completed requests of those kinds */
 } QE_ServerStats;
 
 QE_Server* qe_server_start(const QE_ServerConfig* config);
@@ -630,6 +631,59 @@ B32 qe_server_drive(QE_Server* server, QE_Table* table, const QE_BatchQuery* poo
  * on tables[k % pool_count] (a mix of tables and query shapes) */
 B32 qe_server_drive_mix(QE_Server* server, QE_Table* const* tables, const QE_BatchQuery* pool, U32 pool_count,
                         F64 rate_qps, U64 duration_ns, U64 seed, QE_ServerStats* out);
+
+/* Served joins and scans.  Besides aggregates, the reference's server runs
+ * every other query its clients send through the same one-at-a-time
+ * executor (pg_server.c:643-760): here a pushed-down join aggregate
+ * (qe_aggregate_join's arguments; the submit call's table is the fact
+ * table) and a row-id scan or count (qe_scan_filter's, the point lookup
+ * among them).  Each runs alone on a worker while the other workers run
+ * aggregate batches, and is admitted by its device footprint like them
+ * (a join's partition buffers: fact rows x 4 B x (1 + arguments) plus its
+ * dimension's table; a scan's id buffer).  The struct and the arrays it
+ * points to stay owned by the caller until qe_server_wait returns 1. */
+typedef struct QE_JoinQuery {
+  QE_Table* dim;
+  U32 fact_key_col, dim_key_col;
+  const QE_IRNode* dim_where;
+  U32 dim_where_count;
+  U32 group_count; /* dim GROUP BY columns, 0..QE_AGG_MAX_GROUP_COLS */
+  S32 group_cols[QE_AGG_MAX_GROUP_COLS];
+  const U32* funcs;
+  const S32* arg_cols; /* fact columns */
+  U32 expr_count;
+  U64 cap;
+  U64* out_group_count;
+  F64* out_keys; /* cap x group_count key values (may be 0) */
+  U64* out_cells; /* cap x expr_count cells */
+  B32 ok;
+} QE_JoinQuery;
+
+typedef struct QE_ScanQuery {
+  const QE_IRNode* where;
+  U32 where_count;
+  B32 count_only;
+  U64 cap;       /* out_ids capacity (ignored when count_only) */
+  U64* out_ids;  /* matching row ids, ascending */
+  U64* out_count;
+  B32 ok;
+} QE_ScanQuery;
+
+U64 qe_server_submit_join(QE_Server* server, QE_Table* fact, QE_JoinQuery* query);
+U64 qe_server_submit_scan(QE_Server* server, QE_Table* table, QE_ScanQuery* query);
+
+enum { QE_REQUEST_AGGREGATE = 0, QE_REQUEST_JOIN = 1, QE_REQUEST_SCAN = 2 };
+
+typedef struct QE_ServeRequest {
+  U32 kind;          /* QE_REQUEST_* */
+  QE_Table* table;   /* the table scanned (a join's fact table) */
+  const void* query; /* QE_BatchQuery, QE_JoinQuery or QE_ScanQuery */
+} QE_ServeRequest;
+
+/* the load driver over requests of every kind: request k is a copy of
+ * pool[k % pool_count] with output buffers of its own, sized by its cap */
+B32 qe_server_drive_requests(QE_Server* server, const QE_ServeRequest* pool, U32 pool_count, F64 rate_qps,
+                             U64 duration_ns, U64 seed, QE_ServerStats* out);
 
 /* bandwidth probe (bandwidth_read.comp): reads `bytes` of a buffer with
  * 16-byte loads; returns a sink value.  For measuring the HBM ceiling. */

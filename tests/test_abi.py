@@ -64,3 +64,48 @@ def test_ir_layout_matches_oracle():
     assert a.dtype.itemsize == 24 and a.dtype == oracle.IR_DTYPE
     assert list(a["child_count"]) == [2, 2, 0, 0, 2, 0, 2, 0, 0]
     assert np.array_equal(a["type"], [1, 1, 2, 3, 1, 2, 4, 3, 3])
+
+
+STRUCTS = {
+    "QE_BatchQuery": "QE_BatchQuery",
+    "QE_JoinQuery": "QE_JoinQuery",
+    "QE_ScanQuery": "QE_ScanQuery",
+    "QE_ServeRequest": "QE_ServeRequest",
+    "QE_ServerConfig": "QE_ServerConfig",
+    "QE_ServerStats": "QE_ServerStats",
+}
+
+
+def test_struct_layouts_match_ctypes(tmp_path):
+    """Every field offset and struct size the Python binding assumes is the
+    one a C compiler gives the header's structs."""
+    import shutil
+    import subprocess
+
+    from computeql_amd import native
+
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cqlhip.h"', "int main(void) {"]
+    for c_name, py_name in STRUCTS.items():
+        cls = getattr(native, py_name)
+        lines.append(f'  printf("{py_name} size %zu\\n", sizeof({c_name}));')
+        for f, _ in cls._fields_:
+            lines.append(f'  printf("{py_name} {f} %zu\\n", offsetof({c_name}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            s, f, v = line.split()
+            got[(s, f)] = int(v)
+    for py_name in STRUCTS.values():
+        cls = getattr(native, py_name)
+        assert got[(py_name, "size")] == ctypes.sizeof(cls), py_name
+        for f, _ in cls._fields_:
+            assert got[(py_name, f)] == getattr(cls, f).offset, (py_name, f)
