@@ -99,11 +99,30 @@ static F64 qe_sum_to_f64(U32 type, U64 sum)
 typedef struct QE_GlobalInputs {
   QE_ColumnSlots slots;
   U32 slot_bits, type_bits;
-  U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
+  U32 leaf_words[QE_WHERE_WORDS];
+  U32 leaf_bytes, leaf_packed;
 } QE_GlobalInputs;
 
-static B32 qe_global_prepare(QE_Table* t, const QE_Leaf* leaves, U32 leaf_count, const QE_AggPlan* p,
-                             QE_GlobalInputs* in)
+/* a WHERE's device words over a kernel's column slots (added to `slots`
+ * as needed): the leaves with NULL bitmaps and encodings, then the terms */
+static B32 qe_where_prepare(QE_Table* t, const QE_Where* w, QE_ColumnSlots* slots, B32 enc, U32* words, U32* bytes,
+                            U32* packed)
+{
+  *bytes = *packed = 0;
+  if (!w || !w->leaf_count) return 1;
+  U32 slot[QE_SCAN_MAX_LEAVES];
+  for (U32 l = 0; l < w->leaf_count; l++) {
+    slot[l] = qe_slot_of(slots, w->leaf[l].column);
+    if (slot[l] == ~0u) {
+      qe_set_error("aggregate: too many columns");
+      return 0;
+    }
+  }
+  *bytes = qe_where_words(t, w, slot, 1, enc, words, packed);
+  return 1;
+}
+
+static B32 qe_global_prepare(QE_Table* t, const QE_Where* w, const QE_AggPlan* p, QE_GlobalInputs* in)
 {
   memset(in, 0, sizeof(*in));
   /* encoded columns (a3) are read as codes: an encoded arg is a U32 column
@@ -115,18 +134,7 @@ static B32 qe_global_prepare(QE_Table* t, const QE_Leaf* leaves, U32 leaf_count,
     U32 enc = qe_col_enc(t, p->arg_col[a]);
     in->type_bits |= (enc ? (CQL_ColumnType_U32 | (enc << 4)) : t->columns[p->arg_col[a]].type) << (8 * a);
   }
-  for (U32 l = 0; l < leaf_count; l++) {
-    U32 slot = qe_slot_of(&in->slots, leaves[l].column);
-    if (slot == ~0u) {
-      qe_set_error("aggregate: too many columns");
-      return 0;
-    }
-    qe_simple_leaf_words(&in->leaf_words[6 * l], slot, t->columns[leaves[l].column].type, leaves[l].op,
-                         leaves[l].target);
-    qe_leaf_nulls(&in->leaf_words[6 * l], t, leaves[l].column);
-    qe_leaf_encode(&in->leaf_words[6 * l], qe_col_enc(t, leaves[l].column), t->columns[leaves[l].column].enc_base);
-  }
-  return 1;
+  return qe_where_prepare(t, w, &in->slots, 1, in->leaf_words, &in->leaf_bytes, &in->leaf_packed);
 }
 
 /* aggregate_global_fold's 13 result words (count, then sum/min/max per
@@ -166,14 +174,15 @@ static void qe_global_decode(QE_Table* t, const QE_AggPlan* p, U64* res, U64* ou
   }
 }
 
-static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune,
-                               const QE_AggPlan* p, U64* out_cells, QE_Trace* trace)
+static B32 qe_aggregate_global(QE_Table* t, const QE_Where* w, const QE_Prune* prune, const QE_AggPlan* p,
+                               U64* out_cells, QE_Trace* trace)
 {
   GPU_Kernel* k = qe_kernel("aggregate_global");
   GPU_Kernel* kf = qe_kernel("aggregate_global_fold");
   if (!k || !kf) return 0;
   QE_GlobalInputs in;
-  if (!qe_global_prepare(t, leaves, leaf_count, p, &in)) return 0;
+  if (!qe_global_prepare(t, w, p, &in)) return 0;
+  const U32 leaf_count = w ? w->leaf_count : 0;
   const QE_ColumnSlots slots = in.slots;
   U64 r0 = prune->row_begin, r1 = prune->row_end;
   U64 quads = (r1 + 3) / 4 - r0 / 4;
@@ -200,7 +209,7 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
     gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
     gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
     gpu_kernel_set_arg_u64(k, 5, r0);
-    gpu_kernel_set_arg_u64(k, 7, leaf_count);
+    gpu_kernel_set_arg_u64(k, 7, in.leaf_packed);
     qe_kernel_set_word(k, QK_W_ARGS, p->arg_count | (in.slot_bits << 8));
     qe_kernel_set_word(k, QK_W_TYPES, in.type_bits);
     /* per-block partials, then one block folds them in block order */
@@ -210,7 +219,7 @@ static B32 qe_aggregate_global(QE_Table* t, const QE_Leaf* leaves, U32 leaf_coun
     qe_kernel_set_word(kf, QK_W_ARGS, p->arg_count);
     qe_kernel_set_word(kf, QK_W_TYPES, in.type_bits);
     GPU_Batch* b = gpu_batch_begin(0, sizeof(res));
-    if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, in.leaf_words, 24 * (U64)leaf_count);
+    if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, in.leaf_words, in.leaf_bytes);
     if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
     gpu_batch_kernel_execute(b, k, blocks * QE_WG, QE_WG);
     gpu_batch_kernel_execute(b, kf, QE_WG, QE_WG);
@@ -271,27 +280,28 @@ B32 qe_aggregate_noalloc(QE_Table* table, const QE_IRNode* where, U32 where_coun
     if (!qe_require_non_null(table, arg_cols[e], "qe_aggregate_noalloc")) return 0;
   QE_AggPlan plan;
   if (!qe_agg_plan(table, funcs, arg_cols, expr_count, &plan)) return 0;
-  QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
-  U32 leaf_count = 0;
+  QE_Where w;
+  memset(&w, 0, sizeof(w));
   if (where && where_count) {
     static __thread QE_Bytecode prog;
     if (!qe_bytecode_program_build(table, where, where_count, &prog)) {
       qe_set_error("qe_aggregate_noalloc: %s", prog.error);
       return 0;
     }
-    if (!qe_collect_and_leaves(where, where_count, leaves, &leaf_count, QE_SCAN_MAX_LEAVES)) {
-      qe_set_error("qe_aggregate_noalloc: the WHERE must be an AND of at most %d column-literal comparisons",
-                   QE_SCAN_MAX_LEAVES);
+    if (!qe_where_build(table, where, where_count, &w)) {
+      qe_set_error("qe_aggregate_noalloc: the WHERE must be an AND / OR of at most %d column-literal comparisons "
+                   "(%d conjunctions)", QE_SCAN_MAX_LEAVES, QE_WHERE_MAX_TERMS);
       return 0;
     }
   }
+  const U32 leaf_count = w.leaf_count;
   const U64 scratch = gpu_hip_noalloc_scratch();
   if (!scratch) {
     qe_set_error("qe_aggregate_noalloc: %s", *gpu_hip_last_error() ? gpu_hip_last_error() : "no scratch on this device");
     return 0;
   }
   QE_GlobalInputs in;
-  if (!qe_global_prepare(table, leaves, leaf_count, &plan, &in)) return 0;
+  if (!qe_global_prepare(table, &w, &plan, &in)) return 0;
   const U64 rows = table->row_count;
   U64 res[16];
   memset(res, 0, sizeof(res));
@@ -303,7 +313,7 @@ B32 qe_aggregate_noalloc(QE_Table* table, const QE_IRNode* where, U32 where_coun
     QK_Args* k = &a[0];
     if (leaf_count) {
       k->ptr[0] = scratch + QE_NOALLOC_LEAVES;
-      k->bytes[0] = 24 * (U64)leaf_count;
+      k->bytes[0] = in.leaf_bytes;
     }
     k->ptr[1] = scratch + QE_NOALLOC_RESULT;
     k->bytes[1] = 128;
@@ -313,7 +323,7 @@ B32 qe_aggregate_noalloc(QE_Table* table, const QE_IRNode* where, U32 where_coun
     k->words[0] = (U32)rows;
     k->words[1] = (U32)(rows >> 32);
     k->words[6] = QE_ZONE_SHIFT;
-    k->words[14] = leaf_count;
+    k->words[14] = in.leaf_packed;
     k->words[QK_W_ARGS] = plan.arg_count | (in.slot_bits << 8);
     k->words[QK_W_TYPES] = in.type_bits;
     k->global_work_size = blocks * QE_WG;
@@ -330,7 +340,7 @@ B32 qe_aggregate_noalloc(QE_Table* table, const QE_IRNode* where, U32 where_coun
     f->grid_blocks = 1;
     const char* names[2] = {"aggregate_global", "aggregate_global_fold"};
     const U32 grids[2] = {blocks, 1};
-    if (!gpu_hip_noalloc_run(in.leaf_words, 24 * (U64)leaf_count, QE_NOALLOC_LEAVES, names, a, grids, 2, res,
+    if (!gpu_hip_noalloc_run(in.leaf_words, in.leaf_bytes, QE_NOALLOC_LEAVES, names, a, grids, 2, res,
                              13 * 8, QE_NOALLOC_RESULT)) {
       qe_set_error("qe_aggregate_noalloc: %s", gpu_hip_last_error());
       return 0;
@@ -446,7 +456,7 @@ B32 qe_dense_narrow_ok(const QE_Table* t, const U32* arg_cols, U32 args, B32 nee
 /* runs aggregate_dense over the keys of `keys` (columns of `t`) and args;
  * fills totals indexed by the keys' combined digits */
 B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32 arg_count, B32 need_sum,
-                 B32 need_minmax, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune, QE_DenseTotals* out,
+                 B32 need_minmax, const QE_Where* w, const QE_Prune* prune, QE_DenseTotals* out,
                  QE_Trace* trace)
 {
   GPU_Kernel* k = qe_kernel("aggregate_dense");
@@ -503,17 +513,9 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     slot_bits |= s << (4 * a);
     type_bits |= (enc ? (CQL_ColumnType_U32 | (enc << 4)) : ty) << (8 * a);
   }
-  U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
-  for (U32 l = 0; l < leaf_count; l++) {
-    U32 s = qe_slot_of(&slots, leaves[l].column);
-    if (s == ~0u) {
-      qe_set_error("dense group by: too many columns");
-      return 0;
-    }
-    qe_simple_leaf_words(&leaf_words[6 * l], s, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
-    qe_leaf_nulls(&leaf_words[6 * l], t, leaves[l].column);
-    qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(t, leaves[l].column), t->columns[leaves[l].column].enc_base);
-  }
+  U32 leaf_words[QE_WHERE_WORDS], leaf_bytes = 0, leaf_packed = 0;
+  if (!qe_where_prepare(t, w, &slots, 1, leaf_words, &leaf_bytes, &leaf_packed)) return 0;
+  const U32 leaf_count = w ? w->leaf_count : 0;
   for (U32 i = 0; i < keys->count; i++)
     if (kslot[i] == ~0u) {
       qe_set_error("dense group by: too many columns");
@@ -565,7 +567,7 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
     gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
     gpu_kernel_set_arg_u64(k, 5, r0);
-    gpu_kernel_set_arg_u64(k, 7, leaf_count);
+    gpu_kernel_set_arg_u64(k, 7, leaf_packed);
     qe_kernel_set_word(k, QK_W_KEY_MIN, kmin[0]);
     qe_kernel_set_word(k, QK_W_RANGE, range);
     qe_kernel_set_word(k, QK_W_ARGS, arg_count | (slot_bits << 8));
@@ -573,7 +575,7 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     qe_kernel_set_word(k, QK_W_KEY, kslot[0] | (kbyte[0] << 8) | (flags << 16));
     gpu_hip_kernel_set_lds_bytes(k, (U32)lds);
     GPU_Batch* b = gpu_batch_begin(0, gbytes);
-    if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
+    if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, leaf_bytes);
     if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
     if (key_buf) gpu_batch_buffer_write(b, key_buf, kdesc, 4 * (1 + 3 * (U64)keys->count));
     /* count/sum/max start at 0, min at all ones: fills run from offset 0,
@@ -663,8 +665,7 @@ static U32 qe_hash_key_mode(const QE_GroupKeys* keys)
 /* GROUP BY over the global hash table of aggregate_hash (no keys: one
  * group).  Rows: the fused leaves over the pruned range, or `list` (device
  * row ids) when given.  Groups come out in table-slot order. */
-static B32 qe_aggregate_hash(QE_Table* t, const QE_GroupKeys* keys, const QE_Leaf* leaves, U32 leaf_count,
-                             const QE_Prune* prune, GPU_Buffer* list, U64 list_count, const QE_AggPlan* p, U64 cap,
+static B32 qe_aggregate_hash(QE_Table* t, const QE_GroupKeys* keys, const QE_Where* w, const QE_Prune* prune, GPU_Buffer* list, U64 list_count, const QE_AggPlan* p, U64 cap,
                              U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace)
 {
   GPU_Kernel* k = qe_kernel("aggregate_hash");
@@ -685,16 +686,10 @@ static B32 qe_aggregate_hash(QE_Table* t, const QE_GroupKeys* keys, const QE_Lea
     slot_bits |= qe_slot_of(&slots, p->arg_col[a]) << (4 * a);
     type_bits |= t->columns[p->arg_col[a]].type << (8 * a);
   }
-  U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
-  for (U32 l = 0; l < leaf_count; l++) {
-    U32 sl = qe_slot_of(&slots, leaves[l].column);
-    if (sl == ~0u) {
-      qe_set_error("hash group by: too many columns");
-      return 0;
-    }
-    qe_simple_leaf_words(&leaf_words[6 * l], sl, t->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
-    qe_leaf_nulls(&leaf_words[6 * l], t, leaves[l].column);
-  }
+  /* the hash kernel reads full-width columns: no encodings */
+  U32 leaf_words[QE_WHERE_WORDS], leaf_bytes = 0, leaf_packed = 0;
+  if (!qe_where_prepare(t, w, &slots, 0, leaf_words, &leaf_bytes, &leaf_packed)) return 0;
+  const U32 leaf_count = w ? w->leaf_count : 0;
   B32 slots_ok = slots.count <= QE_MAX_COLUMN_BINDINGS;
   for (U32 i = 0; i < K; i++) slots_ok = slots_ok && kslot[i] != ~0u;
   if (!slots_ok) {
@@ -783,7 +778,7 @@ static B32 qe_aggregate_hash(QE_Table* t, const QE_GroupKeys* keys, const QE_Lea
     gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
     gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
     gpu_kernel_set_arg_u64(k, 5, list ? 0 : r0);
-    gpu_kernel_set_arg_u64(k, 7, list ? 0 : leaf_count);
+    gpu_kernel_set_arg_u64(k, 7, list ? 0 : leaf_packed);
     qe_kernel_set_word(k, QK_W_ARGS, E | (slot_bits << 8));
     qe_kernel_set_word(k, QK_W_TYPES, type_bits);
     qe_kernel_set_word(k, QK_W_KEY, key_slot | (key_type << 8) | (flags << 16));
@@ -801,7 +796,7 @@ static B32 qe_aggregate_hash(QE_Table* t, const QE_GroupKeys* keys, const QE_Lea
       ok = 0;
     } else {
       GPU_Batch* b = gpu_batch_begin(0, (U64)cblocks * 8 + 8);
-      if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, 24 * (U64)leaf_count);
+      if (leaf_buf) gpu_batch_buffer_write(b, leaf_buf, leaf_words, leaf_bytes);
       if (skip_buf) gpu_batch_buffer_write(b, skip_buf, prune->skip_bits, prune->skip_words * 4);
       if (key_buf) gpu_batch_buffer_write(b, key_buf, kdesc, 4 * (3 + 3 * (U64)K));
       gpu_batch_buffer_zero(b, flag_buf, 8);
@@ -953,31 +948,29 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
   if (!qe_agg_plan(table, funcs, arg_cols, expr_count, &plan)) return 0;
   if (trace) trace->rows_before = table->row_count;
 
-  /* WHERE: numeric AND leaves fuse into the aggregate kernels */
-  QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
-  U32 leaf_count = 0;
+  /* WHERE: an AND / OR of column-literal comparisons, IN lists and
+   * IS [NOT] NULL over at most 8 leaves fuses into the aggregate kernels */
+  QE_Where wh;
+  memset(&wh, 0, sizeof(wh));
   B32 general_where = 0;
   if (where && where_count) {
     static __thread QE_Bytecode prog;
-    if (!qe_bytecode_program_build(table, where, where_count, &prog)) {
-      qe_set_error("qe_aggregate: %s", prog.error);
-      return 0;
-    }
-    /* an AND of column-literal comparisons fuses into the aggregate
-     * kernels; any other WHERE runs as a scan whose row ids stay on the
-     * device and feed the hash aggregate */
-    if (!qe_collect_and_leaves(where, where_count, leaves, &leaf_count, QE_SCAN_MAX_LEAVES)) {
+    if (!qe_where_build(table, where, where_count, &wh)) {
+      /* any other WHERE runs as a scan whose row ids stay on the device
+       * and feed the hash aggregate */
+      if (!qe_bytecode_program_build(table, where, where_count, &prog)) {
+        qe_set_error("qe_aggregate: %s", prog.error);
+        return 0;
+      }
       general_where = 1;
-      leaf_count = 0;
     }
     /* QE_FUSE_AGGREGATE (query_exec.c:9892) / QE_FUSE_GROUP_BY (:9447) 0:
      * the WHERE runs as its own scan whose row ids feed the aggregate */
-    if (!general_where && leaf_count &&
-        qe_settings_u64(K ? "QE_FUSE_GROUP_BY" : "QE_FUSE_AGGREGATE", 1) == 0) {
+    if (!general_where && wh.leaf_count &&
+        qe_settings_u64(K ? "QE_FUSE_GROUP_BY" : "QE_FUSE_AGGREGATE", 1) == 0)
       general_where = 1;
-      leaf_count = 0;
-    }
   }
+  const QE_Where* wp = general_where ? 0 : &wh;
   if (general_where) {
     GPU_Buffer* ids = 0;
     U64 n = 0;
@@ -987,7 +980,7 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
     all.row_end = table->row_count;
     U64 groups = 0;
     B32 ok = 1;
-    if (n) ok = qe_aggregate_hash(table, &keys, 0, 0, &all, ids, n, &plan, cap, &groups, out_keys, out_cells, trace);
+    if (n) ok = qe_aggregate_hash(table, &keys, 0, &all, ids, n, &plan, cap, &groups, out_keys, out_cells, trace);
     qe_scratch_release(ids);
     if (!ok) return 0;
     if (K == 0 && groups == 0) {
@@ -1021,7 +1014,7 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
       for (U32 e = 0; e < expr_count; e++) out_cells[e] = qe_agg_cell(plan.funcs[e], 0, 0.0, 0.0, 0.0);
       if (trace) trace->path = prune.zones_left == 0 ? QE_AggPath_FusedGlobal : QE_AggPath_Global;
     } else {
-      ok = qe_aggregate_global(table, leaves, leaf_count, &prune, &plan, out_cells, trace);
+      ok = qe_aggregate_global(table, wp, &prune, &plan, out_cells, trace);
     }
     if (ok) *out_group_count = 1;
     qe_prune_free(&prune);
@@ -1065,8 +1058,7 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
   }
   if (!dense) {
     U64 groups = 0;
-    ok = qe_aggregate_hash(table, &keys, leaves, leaf_count, &prune, 0, 0, &plan, cap, &groups, out_keys, out_cells,
-                           trace);
+    ok = qe_aggregate_hash(table, &keys, wp, &prune, 0, 0, &plan, cap, &groups, out_keys, out_cells, trace);
     qe_prune_free(&prune);
     if (!ok) return 0;
     *out_group_count = groups;
@@ -1076,10 +1068,10 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
   const U32 range = (U32)span;
   QE_DenseTotals tot;
   memset(&tot, 0, sizeof(tot));
-  if (fold) ok = qe_dense_run(table, &keys, 0, 0, 0, 0, leaves, leaf_count, &prune, &tot, trace);
+  if (fold) ok = qe_dense_run(table, &keys, 0, 0, 0, 0, wp, &prune, &tot, trace);
   else
-    ok = qe_dense_run(table, &keys, plan.arg_col, plan.arg_count, plan.need_sum, plan.need_minmax, leaves, leaf_count,
-                      &prune, &tot, trace);
+    ok = qe_dense_run(table, &keys, plan.arg_col, plan.arg_count, plan.need_sum, plan.need_minmax, wp, &prune, &tot,
+                      trace);
   qe_prune_free(&prune);
   if (!ok) return 0;
   /* groups in key order (query_exec.c:8006-8039) */
@@ -1123,7 +1115,7 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
   qe_dense_totals_free(&tot);
   *out_group_count = g;
   if (trace) {
-    trace->path = leaf_count ? QE_AggPath_FusedDense : (fold ? QE_AggPath_DenseKeyFold : QE_AggPath_Dense);
+    trace->path = (wp && wp->leaf_count) ? QE_AggPath_FusedDense : (fold ? QE_AggPath_DenseKeyFold : QE_AggPath_Dense);
     trace->rows_after = g;
   }
   qe_trace_end(trace, t0, p0);

@@ -29,8 +29,7 @@
 typedef struct QE_BatchPlanned {
   U32 index; /* into the caller's array */
   QE_AggPlan plan;
-  QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
-  U32 leaf_count;
+  QE_Where where;
   S32 group_col;
   U32 key_min, range;
   U32 acc_off, acc_words;
@@ -77,14 +76,8 @@ static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlan
   if (q->group_count > 1) return 0;
   const S32 group_col = q->group_count ? q->group_cols[0] : -1;
   if (group_col >= 0 && (U32)group_col < t->column_count && t->columns[group_col].has_nulls) return 0;
-  if (q->where && q->where_count) {
-    static __thread QE_Bytecode prog;
-    if (!qe_bytecode_program_build(t, q->where, q->where_count, &prog)) return 0;
-    if (!qe_collect_and_leaves(q->where, q->where_count, out->leaves, &out->leaf_count, QE_SCAN_MAX_LEAVES)) return 0;
-    /* IS [NOT] NULL leaves run on the plain operators */
-    for (U32 l = 0; l < out->leaf_count; l++)
-      if (out->leaves[l].op >= QE_LEAF_IS_NULL) return 0;
-  }
+  /* any AND / OR / IN WHERE of at most 8 leaves and 16 terms */
+  if (q->where && q->where_count && !qe_where_build(t, q->where, q->where_count, &out->where)) return 0;
   out->group_col = group_col;
   out->range = 1;
   if (group_col >= 0) {
@@ -100,6 +93,12 @@ static B32 qe_batch_plan_query(QE_Table* t, const QE_BatchQuery* q, QE_BatchPlan
   out->reps = group_col >= 0 ? qe_batch_reps(out->range, out->acc_words) : 1;
   out->lds_words = out->acc_words * out->reps;
   return 1;
+}
+
+/* words of a query's record: the fixed part, 6 per leaf, the terms */
+static U32 qe_batch_rec_words(const QE_BatchPlanned* q)
+{
+  return QE_BATCH_REC_WORDS + 6 * q->where.leaf_count + (q->where.term_count + 3) / 4;
 }
 
 /* staged columns of a launch: the union of the columns its queries read,
@@ -119,7 +118,7 @@ static U32 qe_batch_stage_of(QE_BatchStage* st, U32 column)
 
 static void qe_batch_stage_query(QE_BatchStage* st, const QE_BatchPlanned* q)
 {
-  for (U32 l = 0; l < q->leaf_count; l++) qe_batch_stage_of(st, q->leaves[l].column);
+  for (U32 l = 0; l < q->where.leaf_count; l++) qe_batch_stage_of(st, q->where.leaf[l].column);
   if (q->group_col >= 0) qe_batch_stage_of(st, (U32)q->group_col);
   for (U32 a = 0; a < q->plan.arg_count; a++) qe_batch_stage_of(st, q->plan.arg_col[a]);
 }
@@ -150,7 +149,7 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
     qe_batch_stage_query(&st, &qs[i]);
   }
   U32 rec_total = 0;
-  for (U32 i = 0; i < n; i++) rec_total += QE_BATCH_REC_WORDS + 6 * qs[i].leaf_count;
+  for (U32 i = 0; i < n; i++) rec_total += qe_batch_rec_words(&qs[i]);
   const U32 hdr = 6;
   U32 words = hdr + n + 3 * st.count + rec_total + W;
   U32* p = (U32*)calloc(words, 4);
@@ -177,7 +176,7 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
     QE_BatchPlanned* q = &qs[i];
     p[hdr + i] = at;
     U32* r = p + at;
-    r[0] = q->leaf_count;
+    r[0] = q->where.leaf_count | (q->where.term_count << 8);
     r[1] = (q->group_col >= 0 ? 1u : 0u) | (q->plan.need_sum ? 2u : 0u) | (q->plan.need_minmax ? 4u : 0u) |
            (q->group_col >= 0 && compact ? 8u : 0u);
     r[2] = q->group_col >= 0 ? qe_batch_stage_of(&st, (U32)q->group_col) : 0;
@@ -192,13 +191,16 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
     r[11] = q->acc_off;
     r[12] = q->lds_off;
     r[13] = q->reps;
-    for (U32 l = 0; l < q->leaf_count; l++)
-    {
-      qe_simple_leaf_words(&r[QE_BATCH_REC_WORDS + 6 * l], qe_batch_stage_of(&st, q->leaves[l].column),
-                           t->columns[q->leaves[l].column].type, q->leaves[l].op, q->leaves[l].target);
-      qe_leaf_nulls(&r[QE_BATCH_REC_WORDS + 6 * l], t, q->leaves[l].column);
+    if (q->where.leaf_count) {
+      /* the staged columns hold full-width values: no encodings */
+      U32 slot[QE_SCAN_MAX_LEAVES], words[QE_WHERE_WORDS], packed = 0;
+      for (U32 l = 0; l < q->where.leaf_count; l++) slot[l] = qe_batch_stage_of(&st, q->where.leaf[l].column);
+      qe_where_words(t, &q->where, slot, 1, 0, words, &packed);
+      const U32 L = q->where.leaf_count;
+      memcpy(&r[QE_BATCH_REC_WORDS], words, 24 * L);
+      for (U32 k = 0; k < (q->where.term_count + 3) / 4; k++) r[QE_BATCH_REC_WORDS + 6 * L + k] = words[6 * QE_SCAN_MAX_LEAVES + k];
     }
-    at += QE_BATCH_REC_WORDS + 6 * q->leaf_count;
+    at += qe_batch_rec_words(q);
     /* op of every accumulator word: count add; per arg sum (integer or
      * f64 add), min, max */
     U32* ops = p + p[2] + q->acc_off;
@@ -375,7 +377,7 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
       U8 used[QE_MAX_COLUMNS];
       memset(used, 0, sizeof(used));
       for (U32 i = 0; i < n; i++) {
-        for (U32 l = 0; l < qs[i].leaf_count; l++) used[qs[i].leaves[l].column] = 1;
+        for (U32 l = 0; l < qs[i].where.leaf_count; l++) used[qs[i].where.leaf[l].column] = 1;
         for (U32 a = 0; a < qs[i].plan.arg_count; a++) used[qs[i].plan.arg_col[a]] = 1;
         if (qs[i].group_col >= 0) used[qs[i].group_col] = 1;
       }

@@ -320,18 +320,19 @@ B32 qe_require_non_null(const QE_Table* t, S32 column, const char* op)
 
 B32 qe_where_non_null(const QE_Table* t, const QE_IRNode* where, U32 count, const char* op)
 {
-  /* an AND of column-literal comparisons evaluates NULLs (a NULL row fails
-   * its comparison, query_exec.c:13971-14043); other WHERE shapes over a
-   * column with NULLs are refused */
+  /* a WHERE the leaf kernels evaluate (AND / OR of column-literal
+   * comparisons, IN lists, IS [NOT] NULL, at most 8 leaves) evaluates NULLs
+   * as the reference does (a NULL row fails its leaf, query_exec.c:
+   * 13873-14043); other WHERE shapes over a column with NULLs are refused */
   B32 nullable = 0;
   for (U32 i = 0; where && i < count; i++)
     if (where[i].type == QE_IR_COLUMN && where[i].column < t->column_count && t->columns[where[i].column].has_nulls)
       nullable = 1;
   if (!nullable) return 1;
-  QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
-  U32 n = 0;
-  if (qe_collect_and_leaves(where, count, leaves, &n, QE_SCAN_MAX_LEAVES)) return 1;
-  qe_set_error("%s: a WHERE over a column with NULLs must be an AND of column-literal comparisons on this path", op);
+  static __thread QE_Where w;
+  if (qe_where_build(t, where, count, &w)) return 1;
+  qe_set_error("%s: a WHERE over a column with NULLs must be an AND / OR of at most %d column-literal comparisons "
+               "(%d conjunctions) on this path", op, QE_SCAN_MAX_LEAVES, QE_WHERE_MAX_TERMS);
   return 0;
 }
 

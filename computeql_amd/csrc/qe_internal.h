@@ -144,6 +144,14 @@ U32 qe_slot_of(QE_ColumnSlots* slots, U32 column); /* ~0u when out of slots */
 #define QE_LEAF_IS_NULL 5u
 #define QE_LEAF_IS_NOT_NULL 6u
 #define QE_SCAN_NULL_TEST 0x80000u
+/* op 7: the value is one of the integers lo + i for the bits i of a 64-bit
+ * set (an IN list over a 32-bit integer column): word 2 holds bits 0-31,
+ * word 4 bits 32-63, word 3 lo (as for an integer interval) */
+#define QE_LEAF_IN_SET 7u
+#define QE_SCAN_INT_SET 0x200000u
+/* the leaf passes where its comparison fails (!=, NOT IN); a NULL row
+ * still fails it */
+#define QE_SCAN_NEGATE 0x100000u
 
 /* the encoding kernels are given for a column (0 unless it is encoded and
  * QE_ENCODE is on), and the buffer they read for it */
@@ -169,9 +177,36 @@ U32 qe_ir_subtree_end(const QE_IRNode* nodes, U32 i);
 
 typedef struct QE_Leaf {
   U32 column;
-  U32 op; /* 0 EQ, 1 LT, 2 LE, 3 GT, 4 GE (query_exec.c QE_CPU_OP_*) */
+  U32 op; /* 0 EQ, 1 LT, 2 LE, 3 GT, 4 GE (query_exec.c QE_CPU_OP_*), 5 IS
+             NULL, 6 IS NOT NULL, 7 IN a set (target = its lowest value) */
   F64 target;
+  B32 negate; /* != and NOT IN: the comparison's complement, NULL failing */
+  U64 set;    /* op 7: bit i = target + i is in the set */
 } QE_Leaf;
+
+/* A WHERE of AND, OR, comparisons, IN / NOT IN and IS [NOT] NULL as a
+ * disjunction of conjunctions over at most QE_SCAN_MAX_LEAVES distinct
+ * leaves: the row passes when every leaf of some term passes.  A leaf over
+ * a NULL fails, and AND / OR are two-valued, as the reference's row
+ * evaluation has it (qe_row_condition_eval, query_exec.c:13873-14043), so
+ * NULLs are exact under OR as well.  term_count 0 = the AND of every
+ * leaf; a WHERE that is never true has one leaf that never passes. */
+#define QE_WHERE_MAX_TERMS 16
+/* device words: 6 per leaf, then the terms' leaf bits, a byte each */
+#define QE_WHERE_WORDS (6 * QE_SCAN_MAX_LEAVES + QE_WHERE_MAX_TERMS / 4)
+typedef struct QE_Where {
+  QE_Leaf leaf[QE_SCAN_MAX_LEAVES];
+  U32 leaf_count;
+  U32 term_count;
+  U8 term[QE_WHERE_MAX_TERMS];
+} QE_Where;
+/* 0 when the tree has another shape or needs more leaves or terms */
+B32 qe_where_build(const QE_Table* t, const QE_IRNode* nodes, U32 count, QE_Where* out);
+/* the words the leaf kernels take for `w`, leaf l over column slot
+ * slot[l], with NULL bitmaps and column encodings applied when asked;
+ * returns the bytes to upload and sets *packed to the leaf-count word the
+ * kernels take (leaves | terms << 8) */
+U32 qe_where_words(const QE_Table* t, const QE_Where* w, const U32* slot, B32 nulls, B32 enc, U32* words, U32* packed);
 
 /* top-level AND of `column op literal` comparisons; 0 if the tree is not */
 B32 qe_collect_and_leaves(const QE_IRNode* nodes, U32 count, QE_Leaf* leaves, U32* leaf_count, U32 max_leaves);
@@ -301,7 +336,7 @@ typedef struct QE_DenseTotals {
  * significant) */
 #define QE_DENSE_MAX_KEYS 3
 B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32 arg_count, B32 need_sum,
-                 B32 need_minmax, const QE_Leaf* leaves, U32 leaf_count, const QE_Prune* prune, QE_DenseTotals* out,
+                 B32 need_minmax, const QE_Where* where, const QE_Prune* prune, QE_DenseTotals* out,
                  QE_Trace* trace);
 void qe_dense_totals_free(QE_DenseTotals* d);
 U64 qe_dense_lds(U32 range, U32 args, B32 need_sum, B32 need_minmax, B32 narrow);

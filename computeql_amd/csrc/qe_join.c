@@ -541,7 +541,7 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     const U32 kmin_code = fk.min_code[0];
     QE_DenseTotals tot;
     memset(&tot, 0, sizeof(tot));
-    ok = qe_dense_run(fact, &fk, arg_col, arg_count, need_sum, need_minmax, 0, 0, &all, &tot, trace);
+    ok = qe_dense_run(fact, &fk, arg_col, arg_count, need_sum, need_minmax, 0, &all, &tot, trace);
     if (ok) {
       for (U64 v = 0; v < pass_count; v++) {
         U64 d = pass[v];

@@ -229,6 +229,8 @@ static U32 qe_collect_and_nodes(const QE_IRNode* nodes, U32 i, U32* out, U32 n, 
 static B32 qe_resolve_leaf(const QE_IRNode* nodes, U32 i, QE_Leaf* leaf)
 {
   const QE_IRNode* n = &nodes[i];
+  leaf->negate = 0;
+  leaf->set = 0;
   /* `column IS [NOT] NULL` (query_exec.c:13902-13917) */
   if (n->type == QE_IR_OPERATOR && n->child_count == 1 && (n->op == QE_IROP_IS_NULL || n->op == QE_IROP_IS_NOT_NULL) &&
       nodes[i + 1].type == QE_IR_COLUMN) {
@@ -275,6 +277,239 @@ B32 qe_collect_and_leaves(const QE_IRNode* nodes, U32 count, QE_Leaf* leaves, U3
     if (!qe_resolve_leaf(nodes, idx[k], &leaves[k])) return 0;
   *leaf_count = n;
   return 1;
+}
+
+/* ---------------- any AND / OR WHERE: disjunction of conjunctions ---------------- */
+typedef struct QE_TermSet {
+  U32 n;
+  U8 t[QE_WHERE_MAX_TERMS];
+} QE_TermSet;
+
+static B32 qe_leaf_same(const QE_Leaf* a, const QE_Leaf* b)
+{
+  return a->column == b->column && a->op == b->op && a->negate == b->negate && a->set == b->set &&
+         memcmp(&a->target, &b->target, sizeof(F64)) == 0;
+}
+
+/* index of the leaf in w (added when new), -1 when there is no room */
+static S32 qe_where_leaf(QE_Where* w, const QE_Leaf* leaf)
+{
+  for (U32 i = 0; i < w->leaf_count; i++)
+    if (qe_leaf_same(&w->leaf[i], leaf)) return (S32)i;
+  if (w->leaf_count >= QE_SCAN_MAX_LEAVES) return -1;
+  w->leaf[w->leaf_count] = *leaf;
+  return (S32)w->leaf_count++;
+}
+
+static B32 qe_terms_add(QE_TermSet* s, U8 term)
+{
+  for (U32 i = 0; i < s->n; i++)
+    if (s->t[i] == term) return 1;
+  if (s->n >= QE_WHERE_MAX_TERMS) return 0;
+  s->t[s->n++] = term;
+  return 1;
+}
+
+static B32 qe_terms_leaf(QE_Where* w, const QE_Leaf* leaf, QE_TermSet* out)
+{
+  const S32 i = qe_where_leaf(w, leaf);
+  out->n = 0;
+  return i >= 0 && qe_terms_add(out, (U8)(1u << i));
+}
+
+static B32 qe_item_integral(F64 v, F64 lo, F64 hi) { return v == floor(v) && v >= lo && v <= hi; }
+
+/* IN / NOT IN over a column: one set leaf when the column is a 32-bit
+ * integer and the items that can match lie within 64 consecutive values;
+ * otherwise an OR of = leaves (an AND of != leaves) */
+static B32 qe_terms_in(const QE_Table* t, const QE_IRNode* nodes, U32 i, QE_Where* w, QE_TermSet* out)
+{
+  const QE_IRNode* n = &nodes[i];
+  const U32 left = i + 1, right = qe_ir_subtree_end(nodes, left);
+  if (n->child_count != 2 || nodes[left].type != QE_IR_COLUMN || nodes[right].type != QE_IR_INLIST) return 0;
+  const U32 col = nodes[left].column;
+  if (col >= t->column_count) return 0;
+  const B32 is_not = n->op == QE_IROP_NOT_IN;
+  const U32 items = nodes[right].child_count;
+  for (U32 k = 0; k < items; k++)
+    if (nodes[right + 1 + k].type != QE_IR_NUMERIC) return 0;
+  QE_Leaf leaf;
+  memset(&leaf, 0, sizeof(leaf));
+  leaf.column = col;
+  const U32 type = t->columns[col].type;
+  if (type == CQL_ColumnType_U32 || type == CQL_ColumnType_I32) {
+    const F64 vlo = type == CQL_ColumnType_U32 ? 0.0 : -2147483648.0;
+    const F64 vhi = type == CQL_ColumnType_U32 ? 4294967295.0 : 2147483647.0;
+    F64 mn = 0.0, mx = 0.0;
+    U32 hits = 0;
+    for (U32 k = 0; k < items; k++) {
+      const F64 v = nodes[right + 1 + k].value;
+      if (!qe_item_integral(v, vlo, vhi)) continue; /* matches no value of the column */
+      mn = hits ? fmin(mn, v) : v;
+      mx = hits ? fmax(mx, v) : v;
+      hits++;
+    }
+    if (hits == 0 || mx - mn < 64.0) {
+      /* the items no value equals drop out: IN keeps none, NOT IN passes
+       * every non-NULL row */
+      leaf.op = QE_LEAF_IN_SET;
+      leaf.target = hits ? mn : 0.0;
+      for (U32 k = 0; k < items; k++) {
+        const F64 v = nodes[right + 1 + k].value;
+        if (qe_item_integral(v, vlo, vhi)) leaf.set |= 1ull << (U32)(v - mn);
+      }
+      leaf.negate = is_not;
+      return qe_terms_leaf(w, &leaf, out);
+    }
+  }
+  out->n = 0;
+  if (items == 0) {
+    if (!is_not) return 1; /* IN (): never true */
+    leaf.op = QE_LEAF_IS_NOT_NULL;
+    return qe_terms_leaf(w, &leaf, out);
+  }
+  U8 all = 0;
+  for (U32 k = 0; k < items; k++) {
+    leaf.op = 0;
+    leaf.target = nodes[right + 1 + k].value;
+    leaf.negate = is_not;
+    const S32 li = qe_where_leaf(w, &leaf);
+    if (li < 0) return 0;
+    if (is_not) all |= (U8)(1u << li);
+    else if (!qe_terms_add(out, (U8)(1u << li))) return 0;
+  }
+  if (is_not) return qe_terms_add(out, all);
+  return 1;
+}
+
+static B32 qe_terms_of(const QE_Table* t, const QE_IRNode* nodes, U32 i, QE_Where* w, QE_TermSet* out)
+{
+  const QE_IRNode* n = &nodes[i];
+  out->n = 0;
+  if (n->type == QE_IR_OPERATOR && (n->op == QE_IROP_AND || n->op == QE_IROP_OR) && n->child_count == 2) {
+    const U32 left = i + 1, right = qe_ir_subtree_end(nodes, left);
+    QE_TermSet a, b;
+    if (!qe_terms_of(t, nodes, left, w, &a) || !qe_terms_of(t, nodes, right, w, &b)) return 0;
+    if (n->op == QE_IROP_OR) {
+      for (U32 k = 0; k < a.n; k++)
+        if (!qe_terms_add(out, a.t[k])) return 0;
+      for (U32 k = 0; k < b.n; k++)
+        if (!qe_terms_add(out, b.t[k])) return 0;
+      return 1;
+    }
+    for (U32 x = 0; x < a.n; x++)
+      for (U32 y = 0; y < b.n; y++)
+        if (!qe_terms_add(out, (U8)(a.t[x] | b.t[y]))) return 0;
+    return 1;
+  }
+  if (n->type == QE_IR_OPERATOR && (n->op == QE_IROP_IN || n->op == QE_IROP_NOT_IN)) return qe_terms_in(t, nodes, i, w, out);
+  QE_Leaf leaf;
+  memset(&leaf, 0, sizeof(leaf));
+  if (n->type == QE_IR_OPERATOR && n->op == QE_IROP_NE && n->child_count == 2) {
+    /* a != b: the complement of a = b, NULL failing both */
+    const U32 left = i + 1, right = qe_ir_subtree_end(nodes, left);
+    const QE_IRNode *l = &nodes[left], *r = &nodes[right];
+    if (l->type == QE_IR_COLUMN && r->type == QE_IR_NUMERIC) {
+      leaf.column = l->column;
+      leaf.target = r->value;
+    } else if (l->type == QE_IR_NUMERIC && r->type == QE_IR_COLUMN) {
+      leaf.column = r->column;
+      leaf.target = l->value;
+    } else {
+      return 0;
+    }
+    leaf.op = 0;
+    leaf.negate = 1;
+  } else if (!qe_resolve_leaf(nodes, i, &leaf)) {
+    return 0;
+  }
+  if (leaf.column >= t->column_count) return 0;
+  return qe_terms_leaf(w, &leaf, out);
+}
+
+B32 qe_where_build(const QE_Table* t, const QE_IRNode* nodes, U32 count, QE_Where* out)
+{
+  memset(out, 0, sizeof(*out));
+  if (!nodes || count == 0) return 1;
+  if (qe_ir_subtree_end(nodes, 0) != count) return 0;
+  QE_TermSet s;
+  if (!qe_terms_of(t, nodes, 0, out, &s)) return 0;
+  if (s.n == 0) {
+    /* never true: one leaf no row passes (= NaN) */
+    QE_Leaf never;
+    memset(&never, 0, sizeof(never));
+    never.column = ~0u;
+    for (U32 k = 0; k < count && never.column == ~0u; k++)
+      if (nodes[k].type == QE_IR_COLUMN) never.column = nodes[k].column;
+    never.target = NAN;
+    out->leaf_count = 1;
+    out->leaf[0] = never;
+    out->term_count = 0;
+    return never.column < t->column_count;
+  }
+  /* absorption: a term holding every leaf of another adds no row */
+  U8 kept[QE_WHERE_MAX_TERMS];
+  U32 nk = 0;
+  for (U32 a = 0; a < s.n; a++) {
+    B32 redundant = 0;
+    for (U32 b = 0; b < s.n && !redundant; b++)
+      if (b != a && (s.t[a] & s.t[b]) == s.t[b] && s.t[a] != s.t[b]) redundant = 1;
+    if (!redundant) kept[nk++] = s.t[a];
+  }
+  /* leaves no kept term reads are dropped, and the rest renumbered */
+  U8 used = 0;
+  for (U32 k = 0; k < nk; k++) used |= kept[k];
+  U32 map[QE_SCAN_MAX_LEAVES];
+  U32 m = 0;
+  for (U32 l = 0; l < out->leaf_count; l++) {
+    map[l] = m;
+    if (used & (1u << l)) out->leaf[m++] = out->leaf[l];
+  }
+  out->leaf_count = m;
+  for (U32 k = 0; k < nk; k++) {
+    U8 r = 0;
+    for (U32 l = 0; l < QE_SCAN_MAX_LEAVES; l++)
+      if (kept[k] & (1u << l)) r |= (U8)(1u << map[l]);
+    kept[k] = r;
+  }
+  const U8 all = (U8)((1u << m) - 1u);
+  if (nk == 1 && kept[0] == all) {
+    out->term_count = 0;
+  } else {
+    out->term_count = nk;
+    memcpy(out->term, kept, nk);
+  }
+  return 1;
+}
+
+U32 qe_where_words(const QE_Table* t, const QE_Where* w, const U32* slot, B32 nulls, B32 enc, U32* words, U32* packed)
+{
+  memset(words, 0, QE_WHERE_WORDS * 4);
+  for (U32 l = 0; l < w->leaf_count; l++) {
+    const QE_Leaf* f = &w->leaf[l];
+    U32* lw = &words[6 * l];
+    if (f->op == QE_LEAF_IN_SET) {
+      /* the set's lowest value as an integer interval's lo (the i32 bias
+       * and encodings apply to it as to an interval); the bits in words
+       * 2 and 4 */
+      const U32 type = t->columns[f->column].type;
+      const U32 bias = type == CQL_ColumnType_I32 ? QE_SCAN_INT_BIAS : 0u;
+      lw[0] = slot[l];
+      lw[1] = type | QE_SCAN_INT_FLAG | QE_SCAN_INT_SET | bias | (f->set ? 0u : QE_SCAN_INT_NEVER);
+      lw[2] = (U32)f->set;
+      lw[3] = type == CQL_ColumnType_I32 ? ((U32)(S32)f->target ^ 0x80000000u) : (U32)f->target;
+      lw[4] = (U32)(f->set >> 32);
+      lw[5] = 0;
+    } else {
+      qe_simple_leaf_words(lw, slot[l], t->columns[f->column].type, f->op, f->target);
+    }
+    if (f->negate) lw[1] |= QE_SCAN_NEGATE;
+    if (nulls) qe_leaf_nulls(lw, t, f->column);
+    if (enc) qe_leaf_encode(lw, qe_col_enc(t, f->column), t->columns[f->column].enc_base);
+  }
+  for (U32 k = 0; k < w->term_count; k++) words[6 * QE_SCAN_MAX_LEAVES + k / 4] |= (U32)w->term[k] << (8 * (k % 4));
+  *packed = w->leaf_count | (w->term_count << 8);
+  return w->term_count ? QE_WHERE_WORDS * 4 : 24 * w->leaf_count;
 }
 
 U32 qe_collect_prune_leaves(const QE_IRNode* nodes, U32 count, QE_Leaf* leaves, U32 max_leaves)
@@ -589,6 +824,9 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
   QE_Leaf leaves[QE_SCAN_MAX_LEAVES];
   U32 leaf_count = 0;
   B32 is_and = qe_collect_and_leaves(where, node_count, leaves, &leaf_count, QE_SCAN_MAX_LEAVES);
+  /* the library scan_simple evaluates any AND / OR of at most 8 leaves */
+  static __thread QE_Where wh;
+  const B32 leaf_form = qe_where_build(table, where, node_count, &wh);
   QE_Leaf prune_leaves[32];
   U32 prune_count = qe_collect_prune_leaves(where, node_count, prune_leaves, 32);
   QE_Prune prune;
@@ -609,8 +847,9 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
   B32 simple_ok = is_and && qe_settings_u64("QE_SCAN_SIMPLE_KERNEL", 1) != 0;
   /* NULLs are evaluated by the library scan_simple only */
   B32 nullable = 0;
-  for (U32 l = 0; l < leaf_count && is_and; l++)
-    nullable |= table->columns[leaves[l].column].has_nulls || leaves[l].op >= QE_LEAF_IS_NULL;
+  for (U32 l = 0; l < wh.leaf_count && leaf_form; l++)
+    nullable |= table->columns[wh.leaf[l].column].has_nulls || wh.leaf[l].op == QE_LEAF_IS_NULL ||
+                wh.leaf[l].op == QE_LEAF_IS_NOT_NULL;
   if (nullable) path = QE_ScanPath_Simple;
   else if (force_interp) path = QE_ScanPath_Interpreter;
   else if (simple_ok) path = perquery && qe_settings_u64("QE_PERQUERY_SIMPLE", 0) ? QE_ScanPath_GeneratedSimple : QE_ScanPath_Simple;
@@ -639,7 +878,8 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
   GPU_Kernel* k = 0;
   QE_ColumnSlots slots;
   memset(&slots, 0, sizeof(slots));
-  U32 leaf_words[6 * QE_SCAN_MAX_LEAVES];
+  U32 leaf_words[QE_WHERE_WORDS];
+  U32 leaf_packed = leaf_count, leaf_bytes = 24 * leaf_count;
   F64 consts[256];
   U32 const_count = 0;
   U64 compile_ns = 0;
@@ -662,13 +902,16 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
       memset(&slots, 0, sizeof(slots));
       k = qe_kernel("scan_simple");
     }
-    for (U32 l = 0; l < leaf_count; l++) {
+    if (path == QE_ScanPath_Simple) {
+      /* scan_simple reads encoded columns (a3) as codes */
+      U32 slot[QE_SCAN_MAX_LEAVES];
+      for (U32 l = 0; l < wh.leaf_count; l++) slot[l] = qe_slot_of(&slots, wh.leaf[l].column);
+      leaf_bytes = qe_where_words(table, &wh, slot, 1, 1, leaf_words, &leaf_packed);
+    }
+    /* the generated form reads full-width columns without NULLs */
+    for (U32 l = 0; l < leaf_count && path == QE_ScanPath_GeneratedSimple; l++) {
       U32 slot = qe_slot_of(&slots, leaves[l].column);
       qe_simple_leaf_words(&leaf_words[6 * l], slot, table->columns[leaves[l].column].type, leaves[l].op, leaves[l].target);
-      if (path == QE_ScanPath_Simple) qe_leaf_nulls(&leaf_words[6 * l], table, leaves[l].column);
-      /* scan_simple reads encoded columns (a3) as codes */
-      if (path == QE_ScanPath_Simple)
-        qe_leaf_encode(&leaf_words[6 * l], qe_col_enc(table, leaves[l].column), table->columns[leaves[l].column].enc_base);
     }
   }
   if (path == QE_ScanPath_Interpreter) {
@@ -691,7 +934,7 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
   if (path == QE_ScanPath_Simple || path == QE_ScanPath_GeneratedSimple) {
     prog_buf = qe_scratch("scan_leaves", sizeof(leaf_words));
     prog_src = leaf_words;
-    prog_bytes = 6 * 4 * (U64)leaf_count;
+    prog_bytes = leaf_bytes;
   } else if (path == QE_ScanPath_Interpreter) {
     prog_buf = qe_scratch("scan_bytecode", prog.word_count * 4);
     const_buf = qe_scratch("scan_consts", (prog.const_count ? prog.const_count : 1) * 8);
@@ -726,7 +969,7 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
     gpu_kernel_set_arg_u64(k, 3, QE_ZONE_SHIFT);
     gpu_kernel_set_arg_u64(k, 4, skip_buf ? 1 : 0);
     gpu_kernel_set_arg_u64(k, 5, prune.row_begin);
-    gpu_kernel_set_arg_u64(k, 7, leaf_count);
+    gpu_kernel_set_arg_u64(k, 7, leaf_packed);
     gpu_hip_kernel_clear_arg(k, 3);
   }
   U64 total = 0;

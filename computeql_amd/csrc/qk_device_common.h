@@ -52,6 +52,11 @@ typedef struct QK_Args {
 #define QK_LEAF_NEVER 0x20000u
 #define QK_LEAF_BIAS 0x40000u
 #define QK_LEAF_NULL_TEST 0x80000u /* op 5 IS NULL, 6 IS NOT NULL: the null bitmap alone */
+#define QK_LEAF_NEGATE 0x100000u   /* the comparison's complement (!=, NOT IN); NULL still fails */
+#define QK_LEAF_SET 0x200000u      /* IN a set: (code - lo) < 64 and bit (code - lo) of words 2 | 4 << 32 */
+/* a WHERE's leaves are followed by its terms (the leaves of each
+ * conjunction of a disjunction, a byte each) at this word */
+#define QK_TERM_WORD 48
 
 /* scan bindings (query_exec.h:4-40) */
 #define QK_B_LEAVES 0
@@ -232,92 +237,148 @@ __device__ __forceinline__ qk_u32 qk_leaves_quad(const QK_Args& a, const qk_u32*
   return mask;
 }
 
-/* the leaves of a numeric-AND WHERE held in registers for a whole kernel:
- * loaded once (uniform, scalar loads) instead of once per step */
+/* the leaves of a WHERE held in registers for a whole kernel: loaded once
+ * (uniform, scalar loads) instead of once per step.  `packed` is the leaf
+ * count, with the term count in bits 8-15: 0 terms = the AND of the
+ * leaves, else a row passes when every leaf of one of the terms passes */
 struct QkLeafSet {
   qk_u32 n;
+  qk_u32 terms;
   qk_u32 w[8][6];
+  qk_u32 term_words[4];
 };
 
-__device__ __forceinline__ QkLeafSet qk_load_leaves(const qk_u32* leaves, qk_u32 n)
+__device__ __forceinline__ QkLeafSet qk_load_leaves(const qk_u32* leaves, qk_u32 packed)
 {
   QkLeafSet L;
+  const qk_u32 n = packed & 0xffu, terms = (packed >> 8) & 0xffu;
   L.n = n < 8u ? n : 8u;
+  L.terms = terms < 16u ? terms : 16u;
 #pragma unroll
   for (int l = 0; l < 8; l++)
 #pragma unroll
     for (int i = 0; i < 6; i++)
       L.w[l][i] = __builtin_amdgcn_readfirstlane(((qk_u32)l < L.n) ? leaves[6 * l + i] : 0u);
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    L.term_words[i] = __builtin_amdgcn_readfirstlane(L.terms ? leaves[QK_TERM_WORD + i] : 0u);
   return L;
 }
 
-/* the AND over the QK_UNROLL quads of a step, leaf-major: the loads of one
- * leaf for every quad are issued together before any compare */
+/* the rows of a quad passing a disjunction of conjunctions: p holds leaf
+ * l's four row bits at bits 4l..4l+3; a term passes a row when no leaf of
+ * the term has the row's bit clear */
+__device__ __forceinline__ qk_u32 qk_terms_quad(const QkLeafSet& L, qk_u32 p)
+{
+  qk_u32 out = 0u;
+  for (qk_u32 t = 0; t < L.terms; t++) {
+    const qk_u32 tm = (L.term_words[t >> 2] >> (8u * (t & 3u))) & 0xffu;
+    qk_u32 miss = 0u;
+#pragma unroll
+    for (int l = 0; l < 8; l++)
+      if (tm & (1u << l)) miss |= ~(p >> (4 * l));
+    out |= ~miss & 0xfu;
+  }
+  return out;
+}
+
+/* the WHERE over the QK_UNROLL quads of a step, leaf-major: the loads of
+ * one leaf for every quad are issued together before any compare.  Each
+ * leaf's row bits are ANDed into m (no terms) or gathered four bits a leaf
+ * and matched against the terms */
 __device__ __forceinline__ void qk_leaves_step(const QK_Args& a, const QkLeafSet& L, const qk_u64* q, qk_u32* m)
 {
+  qk_u32 p[QK_UNROLL];
+#pragma unroll
+  for (int j = 0; j < QK_UNROLL; j++) p[j] = 0u;
 #pragma unroll
   for (int l = 0; l < 8; l++) {
     if ((qk_u32)l >= L.n) break;
     const qk_u32* w = L.w[l];
     qk_u32 slot = w[0], tw = w[1];
     qk_u32 type = tw & 0xffu;
+    qk_u32 mm[QK_UNROLL];
+    /* word 5: the column's null bitmap (16-byte units from its base).
+     * Quad q's four bits sit at bit 4 (q mod 8) of word q / 8. */
+    qk_u32 nul[QK_UNROLL];
+    if (w[5]) {
+      const qk_u32* nb = qk_gptr<const qk_u32>(a.ptr[QK_B_COL0 + slot] + (qk_u64)w[5] * 16);
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) nul[j] = (nb[q[j] >> 3] >> ((qk_u32)(q[j] & 7) * 4)) & 0xfu;
+    } else {
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) nul[j] = 0u;
+    }
     if (tw & QK_LEAF_NULL_TEST) {
       /* IS NULL keeps the rows whose bit is set, IS NOT NULL the others; a
        * column without NULLs (word 5 zero) has no bit set */
       const bool is_null = w[2] == 5u;
-      if (w[5]) {
-        const qk_u32* nb = qk_gptr<const qk_u32>(a.ptr[QK_B_COL0 + slot] + (qk_u64)w[5] * 16);
 #pragma unroll
-        for (int j = 0; j < QK_UNROLL; j++) {
-          const qk_u32 bits = (nb[q[j] >> 3] >> ((qk_u32)(q[j] & 7) * 4)) & 0xfu;
-          m[j] &= is_null ? bits : ~bits;
-        }
-      } else if (is_null) {
-#pragma unroll
-        for (int j = 0; j < QK_UNROLL; j++) m[j] = 0u;
-      }
-      continue;
-    }
-    if (tw & QK_LEAF_NEVER) {
-#pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) m[j] = 0u;
-      continue;
-    }
-    if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
-      qk_u32 lo = w[3], span = w[4];
-      qk_u32 bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
-      const qk_u32 enc = (tw >> 8) & 0xffu;
-      qk_u32x4 x[QK_UNROLL];
-      if (enc == 0) {
-        const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot);
-#pragma unroll
-        for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
-      } else {
-#pragma unroll
-        for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_quad_codes(a.ptr[QK_B_COL0 + slot], enc, q[j]);
-      }
-#pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) {
-        qk_u32 mm = 0;
-        mm |= (((x[j].x ^ bias) - lo) <= span) ? 1u : 0u;
-        mm |= (((x[j].y ^ bias) - lo) <= span) ? 2u : 0u;
-        mm |= (((x[j].z ^ bias) - lo) <= span) ? 4u : 0u;
-        mm |= (((x[j].w ^ bias) - lo) <= span) ? 8u : 0u;
-        m[j] &= mm;
-      }
+      for (int j = 0; j < QK_UNROLL; j++) mm[j] = is_null ? nul[j] : (~nul[j] & 0xfu);
     } else {
-      qk_u32 ww[6] = {w[0], w[1], w[2], w[3], w[4], 0u};
+      if (tw & QK_LEAF_NEVER) {
 #pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) m[j] &= qk_leaf_quad(a, ww, q[j]);
-    }
-    /* word 5: the column's null bitmap (16-byte units from its base); a
-     * NULL row fails the comparison.  Quad q's four bits sit at bit
-     * 4 (q mod 8) of word q / 8. */
-    if (w[5]) {
-      const qk_u32* nb = qk_gptr<const qk_u32>(a.ptr[QK_B_COL0 + slot] + (qk_u64)w[5] * 16);
+        for (int j = 0; j < QK_UNROLL; j++) mm[j] = 0u;
+      } else if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
+        const qk_u32 lo = w[3];
+        const qk_u32 bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
+        const qk_u32 enc = (tw >> 8) & 0xffu;
+        qk_u32x4 x[QK_UNROLL];
+        if (enc == 0) {
+          const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot);
 #pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) m[j] &= ~((nb[q[j] >> 3] >> ((qk_u32)(q[j] & 7) * 4)) & 0xfu);
+          for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+        } else {
+#pragma unroll
+          for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_quad_codes(a.ptr[QK_B_COL0 + slot], enc, q[j]);
+        }
+        if (tw & QK_LEAF_SET) {
+          /* IN a set of values lo + i, i < 64 */
+          const qk_u64 set = (qk_u64)w[2] | ((qk_u64)w[4] << 32);
+#pragma unroll
+          for (int j = 0; j < QK_UNROLL; j++) {
+            const qk_u32 d0 = (x[j].x ^ bias) - lo, d1 = (x[j].y ^ bias) - lo;
+            const qk_u32 d2 = (x[j].z ^ bias) - lo, d3 = (x[j].w ^ bias) - lo;
+            qk_u32 r = 0;
+            r |= (d0 < 64u && ((set >> d0) & 1u)) ? 1u : 0u;
+            r |= (d1 < 64u && ((set >> d1) & 1u)) ? 2u : 0u;
+            r |= (d2 < 64u && ((set >> d2) & 1u)) ? 4u : 0u;
+            r |= (d3 < 64u && ((set >> d3) & 1u)) ? 8u : 0u;
+            mm[j] = r;
+          }
+        } else {
+          const qk_u32 span = w[4];
+#pragma unroll
+          for (int j = 0; j < QK_UNROLL; j++) {
+            qk_u32 r = 0;
+            r |= (((x[j].x ^ bias) - lo) <= span) ? 1u : 0u;
+            r |= (((x[j].y ^ bias) - lo) <= span) ? 2u : 0u;
+            r |= (((x[j].z ^ bias) - lo) <= span) ? 4u : 0u;
+            r |= (((x[j].w ^ bias) - lo) <= span) ? 8u : 0u;
+            mm[j] = r;
+          }
+        }
+      } else {
+        qk_u32 ww[6] = {w[0], w[1] & ~QK_LEAF_NEGATE, w[2], w[3], w[4], 0u};
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) mm[j] = qk_leaf_quad(a, ww, q[j]);
+      }
+      /* != and NOT IN take the complement; a NULL row fails the leaf */
+      const qk_u32 flip = (tw & QK_LEAF_NEGATE) ? 0xfu : 0u;
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) mm[j] = (mm[j] ^ flip) & ~nul[j];
     }
+    if (L.terms == 0u) {
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) m[j] &= mm[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) p[j] |= (mm[j] & 0xfu) << (4 * l);
+    }
+  }
+  if (L.terms) {
+#pragma unroll
+    for (int j = 0; j < QK_UNROLL; j++) m[j] &= qk_terms_quad(L, p[j]);
   }
 }
 

@@ -2352,7 +2352,9 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /*   [6 + q] offset of query q's record                                 */
 /*   [6 + Q + 3u] staged column u: table column, width in bytes (4 or   */
 /*   8), byte offset of its rows in a stage buffer (16-byte aligned)    */
-/* query record: [0] leaves L (<= 8), [1] flags (1 grouped, 2 sums,     */
+/* query record: [0] leaves L (<= 8) | terms T << 8 (0 = the AND of   */
+/*   the leaves, else the rows of any term whose leaves all pass),      */
+/*   [1] flags (1 grouped, 2 sums,                                      */
 /*   4 min/max, 8 grouped rows walked compactly where a wave's rows are */
 /*   sparse: a lane visits only the rows that passed), [2] key (staged  */
 /*   index), [3] key type,                                              */
@@ -2363,7 +2365,8 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /*   words, [13] replicas (a power of two; lane l of the owning wave    */
 /*   adds into replica l mod replicas, the block folds them at its end),*/
 /*   [14..] leaves, 6 words each (qe_simple_leaf_words layout, slot =   */
-/*   staged index; word 5 a nullable column's bitmap offset / 16).      */
+/*   staged index; word 5 a nullable column's bitmap offset / 16),     */
+/*   then the T terms' leaf bits, a byte each.                          */
 /* Columns: binding 2 holds the base address of every table column.     */
 /* A block walks tiles of 256 x ITEMS rows.  The rows of every staged   */
 /* column of a tile are copied straight from HBM into an LDS stage      */
@@ -2654,36 +2657,55 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
 #pragma unroll 1
     for (qk_u32 q = wave; q < Q; q += WG / 64) {
       const qk_cu32p rec = prog + prog[QK_SS_HDR + q];
-      const qk_u32 L = rec[0] < 8u ? rec[0] : 8u;
+      const qk_u32 L = (rec[0] & 0xffu) < 8u ? (rec[0] & 0xffu) : 8u;
+      const qk_u32 T = (rec[0] >> 8) & 0xffu; /* terms: 0 = the AND of the leaves */
       const qk_u32 flags = rec[1];
       qk_u32 m = valid;
+      qk_u32 lmv[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) lmv[k] = 0u;
 #pragma unroll 1
       for (qk_u32 l = 0; l < L; l++) {
         const qk_cu32p w = rec + QK_SS_REC + 6 * l;
         const qk_u32 u = w[0], tw = w[1], op = w[2], type = tw & 0xffu;
         const unsigned char* cp = sb + st[3 * u + 2];
-        if (tw & QK_LEAF_NEVER) {
-          m = 0;
-          break;
-        }
-        if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
-          const qk_u32 lo = w[3], span = w[4], bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
+        qk_u32 lm = ~0u;
+        const bool null_test = (tw & QK_LEAF_NULL_TEST) != 0u; /* IS [NOT] NULL: the bitmap alone */
+        if (null_test) {
+          lm = op == 5u ? 0u : ~0u; /* a column without NULLs */
+        } else if (tw & QK_LEAF_NEVER) {
+          lm = 0u;
+        } else if ((tw & QK_LEAF_INT) && (type == QK_T_U32 || type == QK_T_I32)) {
+          const qk_u32 lo = w[3], bias = (tw & QK_LEAF_BIAS) ? 0x80000000u : 0u;
           const qk_u32* sv = (const qk_u32*)cp;
+          if (tw & QK_LEAF_SET) {
+            const qk_u64 set = (qk_u64)w[2] | ((qk_u64)w[4] << 32);
 #pragma unroll
-          for (int i = 0; i < RPL; i++)
-            if (((sv[i * 64 + lane] ^ bias) - lo) > span) m &= ~(1u << i);
+            for (int i = 0; i < RPL; i++) {
+              const qk_u32 d = (sv[i * 64 + lane] ^ bias) - lo;
+              if (!(d < 64u && ((set >> d) & 1u))) lm &= ~(1u << i);
+            }
+          } else {
+            const qk_u32 span = w[4];
+#pragma unroll
+            for (int i = 0; i < RPL; i++)
+              if (((sv[i * 64 + lane] ^ bias) - lo) > span) lm &= ~(1u << i);
+          }
         } else {
           const double tv = __longlong_as_double((qk_s64)((qk_u64)w[3] | ((qk_u64)w[4] << 32)));
           double x[RPL];
           qk_ss_rows<RPL>(cp, type, lane, x);
-          m = qk_ss_cmp<RPL>(x, op, tv, m);
+          lm = qk_ss_cmp<RPL>(x, op, tv, lm);
         }
+        /* != and NOT IN: the complement (a NULL row still fails below) */
+        if (!null_test && (tw & QK_LEAF_NEGATE)) lm = ~lm;
         if (w[5]) {
-          /* a nullable column: its NULL rows fail the comparison.  The
-           * bitmap (bit r of word r / 32, 16 * w[5] bytes into the column)
-           * gives row i * 64 + lane of the tile at bit lane mod 32 of word
-           * base / 32 + 2 i + lane / 32; words past the last row are not
-           * read (those rows are already out of m). */
+          /* a nullable column: its NULL rows fail the comparison (IS NULL
+           * keeps just them, IS NOT NULL the others).  The bitmap (bit r
+           * of word r / 32, 16 * w[5] bytes into the column) gives row
+           * i * 64 + lane of the tile at bit lane mod 32 of word base / 32
+           * + 2 i + lane / 32; words past the last row are not read (those
+           * rows are outside `valid`). */
           const qk_u32* nb = (const qk_u32*)(cols[st[3 * u]] + (qk_u64)w[5] * 16);
           const qk_u64 w0 = (base >> 5) + (lane >> 5), wlast = (rows - 1) >> 5;
           qk_u32 nm = 0;
@@ -2692,8 +2714,31 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
             const qk_u64 wi = w0 + 2 * (qk_u64)i;
             nm |= ((wi <= wlast ? nb[wi] : 0u) >> (lane & 31u) & 1u) << i;
           }
-          m &= ~nm;
+          lm = null_test ? (op == 5u ? nm : ~nm) : (lm & ~nm);
         }
+        if (T == 0u) {
+          m &= lm;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            if ((qk_u32)k == l) lmv[k] = lm;
+        }
+      }
+      if (T) {
+        /* a disjunction of conjunctions: the terms' leaf bits follow the
+         * leaves, a byte each */
+        const qk_cu32p tw4 = rec + QK_SS_REC + 6 * L;
+        qk_u32 r = 0u;
+#pragma unroll 1
+        for (qk_u32 t = 0; t < T && t < 16u; t++) {
+          const qk_u32 tm = (tw4[t >> 2] >> (8u * (t & 3u))) & 0xffu;
+          qk_u32 c = valid;
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            if (tm & (1u << k)) c &= lmv[k];
+          r |= c;
+        }
+        m = r;
       }
       const qk_u32 R = rec[5], A = rec[6] < QK_SS_MAX_ARGS ? rec[6] : QK_SS_MAX_ARGS;
       qk_u64* qa = acc + rec[12];

@@ -42,6 +42,7 @@ QUERIES = [
     ([cq.COUNT, cq.SUM, cq.AVG, cq.MAX], [None, 2, 3, 3], (col(1) <= 25) & (col(2) >= 100)),
     ([cq.SUM, cq.MIN, cq.MAX, cq.SUM, cq.AVG], [4, 4, 5, 5, 6], col(3) > 2**31),
     ([cq.COUNT, cq.SUM], [None, 3], col(1) == 7.5),  # a literal no integer equals
+    ([cq.COUNT, cq.SUM, cq.MIN], [None, 2, 4], ((col(1) == 3) & (col(2) < 5000)) | col(1).isin([7, 8, 40])),
 ]
 
 
@@ -89,8 +90,11 @@ def test_noalloc_refuses_what_its_scratch_does_not_hold(orc):
             nine = nine & (col(2) != i)
         with pytest.raises(cq.QueryError, match="at most 8"):
             _noalloc(t, [cq.COUNT], [None], nine)
-        with pytest.raises(cq.QueryError, match="AND"):
-            _noalloc(t, [cq.COUNT], [None], (col(1) == 3) | (col(2) < 5))
+        # eight leaves in 18 conjunctions
+        wide = (((col(1) == 1) | (col(1) == 2) | (col(1) == 30)) & ((col(2) < 5) | (col(2) > 9000) | (col(3) < 7))
+                & ((col(0) < 100) | (col(0) > 200)))
+        with pytest.raises(cq.QueryError, match="conjunctions"):
+            _noalloc(t, [cq.COUNT], [None], wide)
         with pytest.raises(cq.QueryError, match="4 distinct"):
             _noalloc(t, [cq.SUM] * 5, [0, 1, 2, 3, 4])
         # the refusals left the scratch usable
