@@ -28,11 +28,14 @@ The same JSON line also carries, measured after the timed steps:
            and warm (chunks held by the resident chunk cache)
   batched  a 64-query mixed batch over the 10M-row 30-column event table,
            one query per launch vs one shared-scan batch (QPS, speedup)
-  served   the request loop under Poisson arrivals at rising rates, plain
-           (one query per launch) and batched: the highest rate whose p99
-           latency stays within 50 ms, and the ratio of the two; on the
-           event table's 64 queries, and (served.mix) on those plus five
-           query shapes over each of 4 x u32 tables of 1M, 100M and 1B rows
+  served   the request loop under Poisson arrivals at rates rising by half
+           until p99 passes 50 ms, plain (one query per launch) and batched
+           (with identical queries of a batch folded and without): the
+           highest rate whose p99 latency stays within 50 ms, and the ratio
+           of the two; on 2048 event-table queries with literals drawn per
+           query, and (served.mix) on those plus configs 1-3's eight shapes
+           (aggregates, filters, point lookups, both joins) over each of
+           4 x u32 fact tables of 1M, 100M and 1B rows
   roofline the query with the longest kernel time per step (the partitioned
            join q6): its algorithmic bytes / the summed hipEvent time of its
            kernels, and the HBM bytes they move (rocprofv3 TCC counters,
@@ -263,37 +266,70 @@ EVENT_ROWS = 10_000_000
 P99_LIMIT_MS = 50.0
 
 
-def served_sweep(Server, name, kw, rates, drive, seconds_per_rate):
-    """rates in rising order until p99 passes the limit or the achieved rate
-    falls 10% below the offered one; the highest rate that held"""
-    points, best = [], None
-    with Server(**kw) as srv:
-        drive(srv, rates[0], 0.3, 1)
-        for r in rates:
-            st = drive(srv, r, seconds_per_rate, r)
-            points.append({"offered": r, "qps": round(st["qps"], 1), "p50_ms": round(st["p50_ms"], 3),
-                           "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 1)})
-            log(f"{name} offered {r}: {st['qps']:.0f} qps p99 {st['p99_ms']:.2f} ms")
-            if st["p99_ms"] > P99_LIMIT_MS or st["qps"] < 0.9 * r:
-                break
-            best = points[-1]
-    return {"config": kw, "rates": points, "max_qps_at_p99": best["qps"] if best else 0.0,
-            "p99_ms_there": best["p99_ms"] if best else None}
+def served_sweep(Server, name, kw, start, drive, seconds_per_rate, dedupe=None):
+    """offered rates rising by half from `start` until p99 passes the limit
+    or the achieved rate falls 10% below the offered one (at most 16
+    rates); the highest rate that held.  dedupe: QE_BATCH_DEDUPE for the
+    sweep (identical queries of a batch run once), None = as set."""
+    import computeql_amd as cq
+
+    old = cq.get_setting("QE_BATCH_DEDUPE", 1)
+    if dedupe is not None:
+        cq.set_setting("QE_BATCH_DEDUPE", int(dedupe))
+    points, best, crossed = [], None, None
+    try:
+        with Server(**kw) as srv:
+            drive(srv, start, 0.3, 1)
+            r = float(start)
+            for _ in range(16):
+                st = drive(srv, r, seconds_per_rate, int(r))
+                points.append({"offered": round(r), "qps": round(st["qps"], 1), "p50_ms": round(st["p50_ms"], 3),
+                               "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 1),
+                               "joins": st["joins"], "scans": st["scans"]})
+                log(f"{name} offered {r:.0f}: {st['qps']:.0f} qps p99 {st['p99_ms']:.2f} ms")
+                if st["failed"]:
+                    raise RuntimeError(f"{name}: {st['failed']} requests failed")
+                if st["p99_ms"] > P99_LIMIT_MS or st["qps"] < 0.9 * r:
+                    crossed = "p99" if st["p99_ms"] > P99_LIMIT_MS else "rate"
+                    break
+                best = points[-1]
+                r *= 1.5
+    finally:
+        cq.set_setting("QE_BATCH_DEDUPE", old)
+    return {"config": kw, "dedupe": old if dedupe is None else int(dedupe), "rates": points,
+            "max_qps_at_p99": best["qps"] if best else 0.0, "p99_ms_there": best["p99_ms"] if best else None,
+            "stopped_by": crossed or "last rate"}
 
 
-def mix_queries(cq, r: int):
-    """configs 1-2's query shapes over a 4 x u32 table (id, region, amount,
-    ts); joins are not served by the request loop"""
+def mix_requests(cq, t4, dima, dimb, variants: int, seed: int):
+    """configs 1-3's query shapes over a 4 x u32 fact table (id, region,
+    amount, ts, fk) with literals drawn per request: SUM/MIN/MAX(amount);
+    COUNT(*) WHERE amount < a; COUNT, SUM(amount) GROUP BY region;
+    COUNT/SUM/MIN/MAX(amount) GROUP BY amount; COUNT(*) under the
+    three-predicate AND/OR filter; the point lookup id = k (row ids); join
+    A GROUP BY segment, SUM(amount); join B WHERE attr < b, COUNT(*)"""
+    import numpy as np
+
     from computeql_amd.ir import col
+    from computeql_amd.serve import JoinSpec, ScanSpec
 
-    return [cq.AggSpec([cq.SUM, cq.MIN, cq.MAX], [2, 2, 2]),
-            cq.AggSpec([cq.COUNT], [None], col(2) < 200),
-            cq.AggSpec([cq.COUNT, cq.SUM], [None, 2], None, 1),
-            cq.AggSpec([cq.COUNT, cq.SUM, cq.MIN, cq.MAX], [None, 2, 2, 2], None, 2),
-            cq.AggSpec([cq.COUNT], [None], ((col(1) == 7) & (col(2) < 5000)) | (col(3) > 4_000_000_000))]
+    r = np.random.default_rng(seed)
+    out = []
+    for _ in range(variants):
+        reg, amt, ts = int(r.integers(0, 50)), int(r.integers(100, 10000)), int(r.integers(2**31, 2**32 - 2**28))
+        out += [cq.AggSpec([cq.SUM, cq.MIN, cq.MAX], [2, 2, 2]),
+                cq.AggSpec([cq.COUNT], [None], col(2) < int(r.integers(1, 10000))),
+                cq.AggSpec([cq.COUNT, cq.SUM], [None, 2], None, 1),
+                cq.AggSpec([cq.COUNT, cq.SUM, cq.MIN, cq.MAX], [None, 2, 2, 2], None, 2, cap=10_000),
+                cq.AggSpec([cq.COUNT], [None], ((col(1) == reg) & (col(2) < amt)) | (col(3) > ts)),
+                ScanSpec(col(0) == int(r.integers(1, t4.rows + 1)), cap=16),
+                JoinSpec(dima, 4, 0, [cq.SUM], [2], dim_group=1, cap=128),
+                JoinSpec(dimb, 1, 0, [cq.COUNT], [None], dim_where=col(1) < int(r.integers(1, 10)))]
+    return [(t4, sp) for sp in out]
 
 
-def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True, fact=None):
+def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True, fact=None, dima=None,
+                       dimb=None):
     """The batched and served forms on the 10M-row 30-column event table
     (synthetic, seeded): one 64-query mixed batch run plain (one query per
     launch) and as a shared scan, then the request loop under Poisson load
@@ -344,44 +380,66 @@ def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True
     if not served_too:
         ev.close()
         return batched, None
-    served = {"workload": batched["workload"] + ", Poisson arrivals cycling through the 64 queries",
+    # served: request k runs pool[k % len(pool)]; the pool holds 2048
+    # queries of the mixed shapes with literals drawn per query, so a batch
+    # (at most a few hundred consecutive arrivals) holds distinct requests
+    # except where two draws coincide.  Batched sweeps run with identical
+    # queries folded (QE_BATCH_DEDUPE 1, the default) and without.
+    pool = wl.mixed_queries(2048, seed=12)
+    served = {"workload": f"{EVENT_ROWS}-row 30-column event table, Poisson arrivals over 2048 mixed queries "
+                          "with literals drawn per query",
               "p99_limit_ms": P99_LIMIT_MS, "seconds_per_rate": seconds_per_rate}
     plain_kw, batched_kw = dict(workers=2, max_batch=1), dict(workers=3, max_batch=256)
 
     def drive_ev(srv, r, secs, seed):
-        return srv.drive(ev, specs, rate_qps=r, seconds=secs, seed=seed)
+        return srv.drive(ev, pool, rate_qps=r, seconds=secs, seed=seed)
 
-    served["plain"] = served_sweep(Server, "served plain", plain_kw, (2000, 4000, 6000, 8000, 10000, 12000, 16000),
-                                   drive_ev, seconds_per_rate)
-    served["batched"] = served_sweep(Server, "served batched", batched_kw,
-                                     (8000, 16000, 32000, 48000, 64000, 80000, 96000), drive_ev, seconds_per_rate)
+    served["plain"] = served_sweep(Server, "served plain", plain_kw, 2000, drive_ev, seconds_per_rate)
+    served["batched"] = served_sweep(Server, "served batched", batched_kw, 8000, drive_ev, seconds_per_rate, dedupe=1)
+    served["batched_no_folding"] = served_sweep(Server, "served batched, no folding", batched_kw, 8000, drive_ev,
+                                                seconds_per_rate, dedupe=0)
     p = served["plain"]["max_qps_at_p99"]
     served["speedup_at_p99"] = round(served["batched"]["max_qps_at_p99"] / p, 2) if p else None
-    # the served mix: the event table's 64 queries and configs 1-2's five
-    # shapes over 4 x u32 tables of 1M, 100M and (when given) 1B rows
+    served["speedup_at_p99_no_folding"] = round(served["batched_no_folding"]["max_qps_at_p99"] / p, 2) if p else None
+    # the served mix (config 5): the event table's queries and configs 1-3's
+    # eight shapes (aggregates, filters, point lookups, both joins) over 4 x
+    # u32 fact tables of 1M, 100M and 1B rows, literals drawn per request
     mix = None
     if fact is not None:
+        import numpy as np
+
         small, mid = cq.Table(1_000_000), cq.Table(100_000_000)
         for t4 in (small, mid):
-            for name in ("id", "region", "amount", "ts"):
+            for name in ("id", "region", "amount", "ts", "fk"):
                 t4.add_column(name, cq.U32)
             t4.gen_events(SEED, id=0, region=1, amount=2, ts=3)
-        pairs = [(ev, sp) for sp in specs]
-        for t4 in (small, mid, fact):
-            pairs += [(t4, sp) for sp in mix_queries(cq, t4.rows)]
+            t4.gen_fact_fk(4, SEED, dima, 0)
+        pairs = [(ev, sp) for sp in wl.mixed_queries(1024, seed=13)]
+        for k, t4 in enumerate((small, mid, fact)):
+            pairs += mix_requests(cq, t4, dima, dimb, 64, seed=100 + k)
+        order = np.random.default_rng(5).permutation(len(pairs))
+        pairs = [pairs[i] for i in order]
+        shapes = ["sum_min_max", "count_amount_lt", "by_region", "by_amount", "and_or_count", "point_lookup",
+                  "join_a_by_segment", "join_b_count"]
 
         def drive_mix(srv, r, secs, seed):
             return srv.drive_mix(pairs, rate_qps=r, seconds=secs, seed=seed)
 
-        mix = {"workload": f"{len(pairs)} queries cycled: the event table's 64 and five query shapes (configs 1-2) "
-                           "on each of 4 x u32 tables of 1M, 100M and 1B rows; Poisson arrivals",
+        mix = {"workload": f"{len(pairs)} requests cycled in a seeded order: 1024 of the event table's mixed queries "
+                           f"and 64 draws of each of eight shapes ({', '.join(shapes)}) on each of 4 x u32 fact tables "
+                           "of 1M, 100M and 1B rows (join A: 1M-key dim, join B: 50-row dim); literals drawn per "
+                           "request; Poisson arrivals",
                "p99_limit_ms": P99_LIMIT_MS, "seconds_per_rate": seconds_per_rate}
-        mix["plain"] = served_sweep(Server, "served mix plain", plain_kw, (1000, 2000, 4000, 6000, 8000, 10000, 12000),
-                                    drive_mix, seconds_per_rate)
-        mix["batched"] = served_sweep(Server, "served mix batched", batched_kw,
-                                      (4000, 8000, 16000, 32000, 48000, 64000, 80000), drive_mix, seconds_per_rate)
+        mix_batched_kw = dict(workers=3, max_batch=256, hbm_budget=96 << 30)
+        mix["plain"] = served_sweep(Server, "served mix plain", dict(workers=2, max_batch=1, hbm_budget=96 << 30),
+                                    500, drive_mix, seconds_per_rate)
+        mix["batched"] = served_sweep(Server, "served mix batched", mix_batched_kw, 1000, drive_mix,
+                                      seconds_per_rate, dedupe=1)
+        mix["batched_no_folding"] = served_sweep(Server, "served mix batched, no folding", mix_batched_kw, 1000,
+                                                 drive_mix, seconds_per_rate, dedupe=0)
         p = mix["plain"]["max_qps_at_p99"]
         mix["speedup_at_p99"] = round(mix["batched"]["max_qps_at_p99"] / p, 2) if p else None
+        mix["speedup_at_p99_no_folding"] = round(mix["batched_no_folding"]["max_qps_at_p99"] / p, 2) if p else None
         small.close()
         mid.close()
     served["mix"] = mix
@@ -635,9 +693,9 @@ def main(argv=None) -> int:
     ap.add_argument("--served-seconds", type=float, default=1.0)
     ap.add_argument("--no-encoded", action="store_true", help="skip the narrow-encoded rerun of the steps")
     ap.add_argument("--no-streamed", action="store_true", help="skip the host-backed (cold/warm) streamed runs")
-    ap.add_argument("--only", choices=["batched"],
-                    help="run the 64-query batched comparison alone and print its JSON (for profiling it; not a "
-                         "bench line)")
+    ap.add_argument("--only", choices=["batched", "served"],
+                    help="run the 64-query batched comparison (batched), or it and the served sweeps over "
+                         "tables of --rows (served), alone and print their JSON (not a bench line)")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, read one column once (the known-size read that scales "
                          "rocprofv3 FETCH_SIZE readings)")
@@ -669,6 +727,12 @@ def main(argv=None) -> int:
     t, dima, dimb = build_tables(cq, rows, seed)
     lib.gpu_hip_wait()
     log(f"tables ready in {time.perf_counter() - t0:.1f}s")
+    if a.only == "served":
+        batched, served = batched_and_served(cq, lib, a.served_seconds, fact=t, dima=dima, dimb=dimb)
+        print(json.dumps({"batched": batched, "served": served}), flush=True)
+        for x in (t, dima, dimb):
+            x.close()
+        return 0
 
     cq.profile(True)
     errs = []
@@ -760,10 +824,7 @@ def main(argv=None) -> int:
 
     batched = served = None
     if not a.no_served:
-        dima.close()
-        dimb.close()
-        batched, served = batched_and_served(cq, lib, a.served_seconds, fact=t)
-        t.close()
+        batched, served = batched_and_served(cq, lib, a.served_seconds, fact=t, dima=dima, dimb=dimb)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -809,10 +870,9 @@ def main(argv=None) -> int:
         "served": served,
     }
     print(json.dumps(out), flush=True)
-    if a.no_served:
-        t.close()
-        dima.close()
-        dimb.close()
+    t.close()
+    dima.close()
+    dimb.close()
     return 0 if not errs else 1
 
 

@@ -177,7 +177,8 @@ static void qe_global_decode(QE_Table* t, const QE_AggPlan* p, U64* res, U64* ou
 static B32 qe_aggregate_global(QE_Table* t, const QE_Where* w, const QE_Prune* prune, const QE_AggPlan* p,
                                U64* out_cells, QE_Trace* trace)
 {
-  GPU_Kernel* k = qe_kernel("aggregate_global");
+  const B32 terms = w && w->term_count;
+  GPU_Kernel* k = qe_kernel(terms ? "aggregate_global_or" : "aggregate_global");
   GPU_Kernel* kf = qe_kernel("aggregate_global_fold");
   if (!k || !kf) return 0;
   QE_GlobalInputs in;
@@ -338,7 +339,7 @@ B32 qe_aggregate_noalloc(QE_Table* table, const QE_IRNode* where, U32 where_coun
     f->words[QK_W_TYPES] = in.type_bits;
     f->global_work_size = QE_WG;
     f->grid_blocks = 1;
-    const char* names[2] = {"aggregate_global", "aggregate_global_fold"};
+    const char* names[2] = {w.term_count ? "aggregate_global_or" : "aggregate_global", "aggregate_global_fold"};
     const U32 grids[2] = {blocks, 1};
     if (!gpu_hip_noalloc_run(in.leaf_words, in.leaf_bytes, QE_NOALLOC_LEAVES, names, a, grids, 2, res,
                              13 * 8, QE_NOALLOC_RESULT)) {
@@ -459,7 +460,8 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
                  B32 need_minmax, const QE_Where* w, const QE_Prune* prune, QE_DenseTotals* out,
                  QE_Trace* trace)
 {
-  GPU_Kernel* k = qe_kernel("aggregate_dense");
+  const B32 terms = w && w->term_count;
+  GPU_Kernel* k = qe_kernel(terms ? "aggregate_dense_or" : "aggregate_dense");
   if (!k) return 0;
   const U64 span = qe_group_keys_span(keys);
   if (keys->count < 1 || keys->count > QE_DENSE_MAX_KEYS || span == 0 || span > 0xffffffffull) {
@@ -480,8 +482,11 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     return 0;
   }
   if (narrow) {
-    k = qe_kernel(need_sum && need_minmax ? "aggregate_dense_narrow_sm"
-                                          : (need_sum ? "aggregate_dense_narrow_s" : "aggregate_dense_narrow_m"));
+    const char* nk = need_sum && need_minmax ? "aggregate_dense_narrow_sm"
+                                             : (need_sum ? "aggregate_dense_narrow_s" : "aggregate_dense_narrow_m");
+    char name[64];
+    snprintf(name, sizeof(name), "%s%s", nk, terms ? "_or" : "");
+    k = qe_kernel(name);
     if (!k) return 0;
   }
   const U32 wg = narrow ? 1024u : QE_WG;
@@ -668,7 +673,7 @@ static U32 qe_hash_key_mode(const QE_GroupKeys* keys)
 static B32 qe_aggregate_hash(QE_Table* t, const QE_GroupKeys* keys, const QE_Where* w, const QE_Prune* prune, GPU_Buffer* list, U64 list_count, const QE_AggPlan* p, U64 cap,
                              U64* out_group_count, F64* out_keys, U64* out_cells, QE_Trace* trace)
 {
-  GPU_Kernel* k = qe_kernel("aggregate_hash");
+  GPU_Kernel* k = qe_kernel(w && w->term_count ? "aggregate_hash_or" : "aggregate_hash");
   GPU_Kernel* kc = qe_kernel("hash_compact");
   if (!k || !kc) return 0;
   const U32 E = p->arg_count;

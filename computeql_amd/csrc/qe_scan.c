@@ -900,7 +900,7 @@ static B32 qe_scan_impl(QE_Table* table, const QE_IRNode* where, U32 node_count,
   if (path == QE_ScanPath_Simple || path == QE_ScanPath_GeneratedSimple) {
     if (path == QE_ScanPath_Simple) {
       memset(&slots, 0, sizeof(slots));
-      k = qe_kernel("scan_simple");
+      k = qe_kernel(leaf_form && wh.term_count ? "scan_simple_or" : "scan_simple");
     }
     if (path == QE_ScanPath_Simple) {
       /* scan_simple reads encoded columns (a3) as codes */

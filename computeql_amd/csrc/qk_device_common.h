@@ -248,12 +248,13 @@ struct QkLeafSet {
   qk_u32 term_words[4];
 };
 
+template <bool TERMS = false>
 __device__ __forceinline__ QkLeafSet qk_load_leaves(const qk_u32* leaves, qk_u32 packed)
 {
   QkLeafSet L;
   const qk_u32 n = packed & 0xffu, terms = (packed >> 8) & 0xffu;
   L.n = n < 8u ? n : 8u;
-  L.terms = terms < 16u ? terms : 16u;
+  L.terms = TERMS ? (terms < 16u ? terms : 16u) : 0u;
 #pragma unroll
   for (int l = 0; l < 8; l++)
 #pragma unroll
@@ -261,22 +262,30 @@ __device__ __forceinline__ QkLeafSet qk_load_leaves(const qk_u32* leaves, qk_u32
       L.w[l][i] = __builtin_amdgcn_readfirstlane(((qk_u32)l < L.n) ? leaves[6 * l + i] : 0u);
 #pragma unroll
   for (int i = 0; i < 4; i++)
-    L.term_words[i] = __builtin_amdgcn_readfirstlane(L.terms ? leaves[QK_TERM_WORD + i] : 0u);
+    L.term_words[i] = TERMS ? __builtin_amdgcn_readfirstlane(L.terms ? leaves[QK_TERM_WORD + i] : 0u) : 0u;
   return L;
 }
 
 /* the rows of a quad passing a disjunction of conjunctions: p holds leaf
- * l's four row bits at bits 4l..4l+3; a term passes a row when no leaf of
- * the term has the row's bit clear */
+ * l's four row bits at bits 4l..4l+3.  A term's leaf byte is spread to a
+ * nibble mask (uniform, scalar work); a row misses the term when one of
+ * the term's leaves has the row's bit clear.  The term words are read
+ * with static indexes, so the leaf set stays in registers. */
 __device__ __forceinline__ qk_u32 qk_terms_quad(const QkLeafSet& L, qk_u32 p)
 {
   qk_u32 out = 0u;
+  const qk_u32 w0 = L.term_words[0], w1 = L.term_words[1], w2 = L.term_words[2], w3 = L.term_words[3];
   for (qk_u32 t = 0; t < L.terms; t++) {
-    const qk_u32 tm = (L.term_words[t >> 2] >> (8u * (t & 3u))) & 0xffu;
-    qk_u32 miss = 0u;
-#pragma unroll
-    for (int l = 0; l < 8; l++)
-      if (tm & (1u << l)) miss |= ~(p >> (4 * l));
+    const qk_u32 word = t < 4u ? w0 : (t < 8u ? w1 : (t < 12u ? w2 : w3));
+    qk_u32 n = (word >> (8u * (t & 3u))) & 0xffu;
+    n = (n | (n << 12)) & 0x000f000fu;
+    n = (n | (n << 6)) & 0x03030303u;
+    n = (n | (n << 3)) & 0x11111111u;
+    const qk_u32 mask = n * 0xfu;
+    qk_u32 miss = ~p & mask;
+    miss |= miss >> 16;
+    miss |= miss >> 8;
+    miss |= miss >> 4;
     out |= ~miss & 0xfu;
   }
   return out;
@@ -284,13 +293,17 @@ __device__ __forceinline__ qk_u32 qk_terms_quad(const QkLeafSet& L, qk_u32 p)
 
 /* the WHERE over the QK_UNROLL quads of a step, leaf-major: the loads of
  * one leaf for every quad are issued together before any compare.  Each
- * leaf's row bits are ANDed into m (no terms) or gathered four bits a leaf
- * and matched against the terms */
+ * leaf's row bits are ANDed into m, or with TERMS (a kernel form of its
+ * own, so the AND form keeps its registers) gathered four bits a leaf and
+ * matched against the terms */
+template <bool TERMS = false>
 __device__ __forceinline__ void qk_leaves_step(const QK_Args& a, const QkLeafSet& L, const qk_u64* q, qk_u32* m)
 {
   qk_u32 p[QK_UNROLL];
+  if (TERMS) {
 #pragma unroll
-  for (int j = 0; j < QK_UNROLL; j++) p[j] = 0u;
+    for (int j = 0; j < QK_UNROLL; j++) p[j] = 0u;
+  }
 #pragma unroll
   for (int l = 0; l < 8; l++) {
     if ((qk_u32)l >= L.n) break;
@@ -298,23 +311,21 @@ __device__ __forceinline__ void qk_leaves_step(const QK_Args& a, const QkLeafSet
     qk_u32 slot = w[0], tw = w[1];
     qk_u32 type = tw & 0xffu;
     qk_u32 mm[QK_UNROLL];
-    /* word 5: the column's null bitmap (16-byte units from its base).
-     * Quad q's four bits sit at bit 4 (q mod 8) of word q / 8. */
-    qk_u32 nul[QK_UNROLL];
-    if (w[5]) {
-      const qk_u32* nb = qk_gptr<const qk_u32>(a.ptr[QK_B_COL0 + slot] + (qk_u64)w[5] * 16);
-#pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) nul[j] = (nb[q[j] >> 3] >> ((qk_u32)(q[j] & 7) * 4)) & 0xfu;
-    } else {
-#pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) nul[j] = 0u;
-    }
     if (tw & QK_LEAF_NULL_TEST) {
       /* IS NULL keeps the rows whose bit is set, IS NOT NULL the others; a
        * column without NULLs (word 5 zero) has no bit set */
       const bool is_null = w[2] == 5u;
+      if (w[5]) {
+        const qk_u32* nb = qk_gptr<const qk_u32>(a.ptr[QK_B_COL0 + slot] + (qk_u64)w[5] * 16);
 #pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) mm[j] = is_null ? nul[j] : (~nul[j] & 0xfu);
+        for (int j = 0; j < QK_UNROLL; j++) {
+          const qk_u32 bits = (nb[q[j] >> 3] >> ((qk_u32)(q[j] & 7) * 4)) & 0xfu;
+          mm[j] = is_null ? bits : (~bits & 0xfu);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) mm[j] = is_null ? 0u : 0xfu;
+      }
     } else {
       if (tw & QK_LEAF_NEVER) {
 #pragma unroll
@@ -363,20 +374,29 @@ __device__ __forceinline__ void qk_leaves_step(const QK_Args& a, const QkLeafSet
 #pragma unroll
         for (int j = 0; j < QK_UNROLL; j++) mm[j] = qk_leaf_quad(a, ww, q[j]);
       }
-      /* != and NOT IN take the complement; a NULL row fails the leaf */
-      const qk_u32 flip = (tw & QK_LEAF_NEGATE) ? 0xfu : 0u;
+      /* != and NOT IN take the complement */
+      if (tw & QK_LEAF_NEGATE) {
 #pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) mm[j] = (mm[j] ^ flip) & ~nul[j];
+        for (int j = 0; j < QK_UNROLL; j++) mm[j] ^= 0xfu;
+      }
+      /* word 5: the column's null bitmap (16-byte units from its base); a
+       * NULL row fails the leaf.  Quad q's four bits sit at bit
+       * 4 (q mod 8) of word q / 8. */
+      if (w[5]) {
+        const qk_u32* nb = qk_gptr<const qk_u32>(a.ptr[QK_B_COL0 + slot] + (qk_u64)w[5] * 16);
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) mm[j] &= ~((nb[q[j] >> 3] >> ((qk_u32)(q[j] & 7) * 4)) & 0xfu);
+      }
     }
-    if (L.terms == 0u) {
-#pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) m[j] &= mm[j];
-    } else {
+    if (TERMS) {
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) p[j] |= (mm[j] & 0xfu) << (4 * l);
+    } else {
+#pragma unroll
+      for (int j = 0; j < QK_UNROLL; j++) m[j] &= mm[j];
     }
   }
-  if (L.terms) {
+  if (TERMS) {
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) m[j] &= qk_terms_quad(L, p[j]);
   }

@@ -201,7 +201,8 @@ __device__ __forceinline__ qk_u64 agg_combine_sum(qk_u32 type, qk_u64 x, qk_u64 
   return (qk_u64)__double_as_longlong(__longlong_as_double((qk_s64)x) + __longlong_as_double((qk_s64)y));
 }
 
-extern "C" __global__ void LB k_aggregate_global(QK_Args a)
+template <bool TERMS>
+__device__ __forceinline__ void qk_aggregate_global(const QK_Args& a)
 {
   __shared__ qk_u64 s_part[QK_WG / 64][13];
   const qk_u64 rows = qk_push(a, QK_P_ROWS);
@@ -211,7 +212,7 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
   const qk_u32 nargs = a.words[QK_W_ARGS] & 0xffu;
   const qk_u32 slots = a.words[QK_W_ARGS] >> 8;
   const qk_u32 types = a.words[QK_W_TYPES];
-  const QkLeafSet L = qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), leaf_count);
+  const QkLeafSet L = qk_load_leaves<TERMS>(qk_ptr<qk_u32>(a, QK_B_LEAVES), leaf_count);
 
   qk_u64 count = 0;
   AggAcc acc[4];
@@ -227,7 +228,7 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
     qk_u64 q[QK_UNROLL];
     qk_u32 m[QK_UNROLL];
     qk_step_quads(a, s, b1, r0, r1, q, m);
-    if (L.n) qk_leaves_step(a, L, q, m);
+    if (L.n) qk_leaves_step<TERMS>(a, L, q, m);
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) count += (qk_u64)__popc(m[j]);
 #pragma unroll
@@ -308,6 +309,10 @@ extern "C" __global__ void LB k_aggregate_global(QK_Args a)
   }
 }
 
+extern "C" __global__ void LB k_aggregate_global(QK_Args a) { qk_aggregate_global<false>(a); }
+/* the same over a WHERE of several conjunctions */
+extern "C" __global__ void LB k_aggregate_global_or(QK_Args a) { qk_aggregate_global<true>(a); }
+
 /* aggregate_global_fold: one block folds aggregate_global's per-block
  * partials (binding 2, 13 words a block; push 0 = blocks) into binding 1's
  * words 0..12, in a fixed order: thread t walks blocks t, t+256, ... in
@@ -380,6 +385,7 @@ extern "C" __global__ void LB k_aggregate_global_fold(QK_Args a)
 /* scan_simple: numeric-AND leaves, count / block counts / ordered ids  */
 /* (scan bindings and push slots of query_exec.h:4-40)                  */
 /* ------------------------------------------------------------------ */
+template <bool TERMS>
 struct SimpleStepPred {
   const QK_Args* a;
   QkLeafSet L;
@@ -387,13 +393,20 @@ struct SimpleStepPred {
   {
 #pragma unroll
     for (int j = 0; j < QK_UNROLL; j++) m[j] = valid[j];
-    qk_leaves_step(*a, L, q, m);
+    qk_leaves_step<TERMS>(*a, L, q, m);
   }
 };
 
 extern "C" __global__ void LB k_scan_simple(QK_Args a)
 {
-  SimpleStepPred p{&a, qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES))};
+  SimpleStepPred<false> p{&a, qk_load_leaves<false>(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES))};
+  qk_scan_driver_step(a, p);
+}
+
+/* the same over a WHERE of several conjunctions */
+extern "C" __global__ void LB k_scan_simple_or(QK_Args a)
+{
+  SimpleStepPred<true> p{&a, qk_load_leaves<true>(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES))};
   qk_scan_driver_step(a, p);
 }
 
@@ -523,7 +536,8 @@ extern "C" __global__ void LB k_scan_filter(QK_Args a)
 /* ------------------------------------------------------------------ */
 __device__ __forceinline__ qk_u32 qk_code32(qk_u32 v, qk_u32 type) { return type == QK_T_I32 ? (v ^ 0x80000000u) : v; }
 
-extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
+template <bool TERMS>
+__device__ __forceinline__ void qk_aggregate_dense(const QK_Args& a)
 {
   extern __shared__ qk_u64 lds_u64[];
   const qk_u64 rows = qk_push(a, QK_P_ROWS);
@@ -540,7 +554,7 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
   const qk_u32 flags = a.words[QK_W_KEY] >> 16;
   const qk_u32 key_min = a.words[QK_W_KEY_MIN];
   const qk_u32 range = a.words[QK_W_RANGE];
-  const QkLeafSet L = qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), leaf_count);
+  const QkLeafSet L = qk_load_leaves<TERMS>(qk_ptr<qk_u32>(a, QK_B_LEAVES), leaf_count);
   const qk_u32* kd = a.ptr[2] ? qk_ptr<qk_u32>(a, 2) : nullptr;
   const qk_u32 nkeys = kd ? kd[0] : 1u;
 
@@ -567,7 +581,7 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
     qk_u32 m[QK_UNROLL];
     qk_u32x4 kv[QK_UNROLL];
     qk_step_quads(a, s, b1, r0, r1, q, m);
-    if (L.n) qk_leaves_step(a, L, q, m);
+    if (L.n) qk_leaves_step<TERMS>(a, L, q, m);
     qk_u32 g[QK_UNROLL][4];
     if (nkeys == 1) {
       if (key_enc == 0) {
@@ -669,6 +683,10 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
   }
 }
 
+extern "C" __global__ void LB k_aggregate_dense(QK_Args a) { qk_aggregate_dense<false>(a); }
+/* the same over a WHERE of several conjunctions */
+extern "C" __global__ void LB k_aggregate_dense_or(QK_Args a) { qk_aggregate_dense<true>(a); }
+
 /* ------------------------------------------------------------------ */
 /* aggregate_dense_narrow: aggregate_dense for group ranges too large    */
 /* for its LDS tables (10k groups x 20 B), with one U32 argument, in     */
@@ -686,7 +704,7 @@ extern "C" __global__ void LB k_aggregate_dense(QK_Args a)
 /* global layout count u64[R] | sum u64[R] | min u32[R] | max u32[R].    */
 /* ------------------------------------------------------------------ */
 #define QK_NARROW_WG 1024
-template <bool SUM, bool MM>
+template <bool SUM, bool MM, bool TERMS>
 __device__ __forceinline__ void qk_dense_narrow(const QK_Args& a)
 {
   constexpr int WG = QK_NARROW_WG, U = QK_UNROLL;
@@ -701,7 +719,7 @@ __device__ __forceinline__ void qk_dense_narrow(const QK_Args& a)
   const qk_u32 key_enc = (a.words[QK_W_KEY] >> 12) & 3u;
   const qk_u32 key_min = a.words[QK_W_KEY_MIN];
   const qk_u32 range = a.words[QK_W_RANGE];
-  const QkLeafSet L = qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES));
+  const QkLeafSet L = qk_load_leaves<TERMS>(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES));
   const qk_u32* kd = a.ptr[2] ? qk_ptr<qk_u32>(a, 2) : nullptr;
   const qk_u32 nkeys = kd ? kd[0] : 1u;
   qk_u32* l_cc = lds_n;
@@ -745,7 +763,7 @@ __device__ __forceinline__ void qk_dense_narrow(const QK_Args& a)
       m[j] = v;
       q[j] = in ? qq : b1 - 1;
     }
-    if (L.n) qk_leaves_step(a, L, q, m);
+    if (L.n) qk_leaves_step<TERMS>(a, L, q, m);
     qk_u32 g[U][4];
     if (nkeys == 1) {
 #pragma unroll
@@ -830,18 +848,19 @@ __device__ __forceinline__ void qk_dense_narrow(const QK_Args& a)
   }
 }
 
-extern "C" __global__ void __launch_bounds__(QK_NARROW_WG) k_aggregate_dense_narrow_s(QK_Args a)
-{
-  qk_dense_narrow<true, false>(a);
-}
-extern "C" __global__ void __launch_bounds__(QK_NARROW_WG) k_aggregate_dense_narrow_m(QK_Args a)
-{
-  qk_dense_narrow<false, true>(a);
-}
-extern "C" __global__ void __launch_bounds__(QK_NARROW_WG) k_aggregate_dense_narrow_sm(QK_Args a)
-{
-  qk_dense_narrow<true, true>(a);
-}
+#define QK_NARROW_KERNEL(name, SUM, MM)                                                                   \
+  extern "C" __global__ void __launch_bounds__(QK_NARROW_WG) k_aggregate_dense_narrow_##name(QK_Args a)  \
+  {                                                                                                       \
+    qk_dense_narrow<SUM, MM, false>(a);                                                                   \
+  }                                                                                                       \
+  extern "C" __global__ void __launch_bounds__(QK_NARROW_WG) k_aggregate_dense_narrow_##name##_or(QK_Args a) \
+  {                                                                                                       \
+    qk_dense_narrow<SUM, MM, true>(a);                                                                    \
+  }
+QK_NARROW_KERNEL(s, true, false)
+QK_NARROW_KERNEL(m, false, true)
+QK_NARROW_KERNEL(sm, true, true)
+#undef QK_NARROW_KERNEL
 
 /* ------------------------------------------------------------------ */
 /* join_build: open-addressing table of the dim keys on the device.     */
@@ -2152,7 +2171,8 @@ __device__ __forceinline__ void qk_multi_code4(const QK_Args& a, const qk_u32* k
  * every row in group 0), LOG2, RANGE (LDS slots, a power of two or 0).
  * LDS: key u64[L] | min u64[E][L] | max u64[E][L] | sum u64[E][L] |
  * count u32[L] */
-extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
+template <bool TERMS>
+__device__ __forceinline__ void qk_aggregate_hash(const QK_Args& a)
 {
   extern __shared__ qk_u64 lds_u64[];
   QkHashCtx h;
@@ -2218,14 +2238,14 @@ extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
     const qk_u64 rows = qk_push(a, QK_P_ROWS);
     const qk_u64 r0 = qk_push(a, QK_P_ROW_BEGIN);
     const qk_u64 r1 = r0 + rows;
-    const QkLeafSet L = qk_load_leaves(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES));
+    const QkLeafSet L = qk_load_leaves<TERMS>(qk_ptr<qk_u32>(a, QK_B_LEAVES), (qk_u32)qk_push(a, QK_P_LEAVES));
     qk_u64 b0, b1;
     qk_block_range(r0 / 4, (r1 + 3) / 4, b0, b1);
     for (qk_u64 s = b0; s < b1; s += (qk_u64)QK_WG * QK_UNROLL) {
       qk_u64 q[QK_UNROLL];
       qk_u32 m[QK_UNROLL];
       qk_step_quads(a, s, b1, r0, r1, q, m);
-      if (L.n) qk_leaves_step(a, L, q, m);
+      if (L.n) qk_leaves_step<TERMS>(a, L, q, m);
 #pragma unroll 1
       for (int j = 0; j < QK_UNROLL; j++) {
         if (!m[j]) continue;
@@ -2275,6 +2295,10 @@ extern "C" __global__ void LB k_aggregate_hash(QK_Args a)
     qk_hash_global_add(h, h.lkey[i], cnt, sum, mn, mx);
   }
 }
+
+extern "C" __global__ void LB k_aggregate_hash(QK_Args a) { qk_aggregate_hash<false>(a); }
+/* the same over a WHERE of several conjunctions */
+extern "C" __global__ void LB k_aggregate_hash_or(QK_Args a) { qk_aggregate_hash<true>(a); }
 
 /* hash_compact: the occupied slots of aggregate_hash's table, in slot
  * order.  binding 1 table, 2 block counts (u64 per block), 3 out records
@@ -3466,16 +3490,23 @@ static const QK_LibraryKernel g_library[] = {
     {"bandwidth_read", (const void*)k_bandwidth_read, QK_WG},
     {"bandwidth_copy", (const void*)k_bandwidth_copy, QK_WG},
     {"aggregate_global", (const void*)k_aggregate_global, QK_WG},
+    {"aggregate_global_or", (const void*)k_aggregate_global_or, QK_WG},
     {"aggregate_global_fold", (const void*)k_aggregate_global_fold, QK_WG},
     {"scan_simple", (const void*)k_scan_simple, QK_WG},
+    {"scan_simple_or", (const void*)k_scan_simple_or, QK_WG},
     {"scan_filter", (const void*)k_scan_filter, QK_WG},
     {"aggregate_dense", (const void*)k_aggregate_dense, QK_WG},
+    {"aggregate_dense_or", (const void*)k_aggregate_dense_or, QK_WG},
     {"aggregate_dense_narrow_s", (const void*)k_aggregate_dense_narrow_s, QK_NARROW_WG},
     {"aggregate_dense_narrow_m", (const void*)k_aggregate_dense_narrow_m, QK_NARROW_WG},
     {"aggregate_dense_narrow_sm", (const void*)k_aggregate_dense_narrow_sm, QK_NARROW_WG},
+    {"aggregate_dense_narrow_s_or", (const void*)k_aggregate_dense_narrow_s_or, QK_NARROW_WG},
+    {"aggregate_dense_narrow_m_or", (const void*)k_aggregate_dense_narrow_m_or, QK_NARROW_WG},
+    {"aggregate_dense_narrow_sm_or", (const void*)k_aggregate_dense_narrow_sm_or, QK_NARROW_WG},
     {"join_build", (const void*)k_join_build, QK_WG},
     {"join_probe", (const void*)k_join_probe, QK_WG},
     {"aggregate_hash", (const void*)k_aggregate_hash, QK_WG},
+    {"aggregate_hash_or", (const void*)k_aggregate_hash_or, QK_WG},
     {"shared_scan_256_1", (const void*)k_shared_scan_256_1, 256},
     {"shared_scan_256_2", (const void*)k_shared_scan_256_2, 256},
     {"shared_scan_256_4", (const void*)k_shared_scan_256_4, 256},
