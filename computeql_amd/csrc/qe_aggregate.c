@@ -420,29 +420,46 @@ static void qe_group_digits(const QE_GroupKeys* k, U64 g, F64* out)
 /* per-group device totals of aggregate_dense / join_probe: count u64[R];
  * sum u64[E][R]; min u32[E][R]; max u32[E][R] */
 
-static B32 qe_dense_totals_alloc(QE_DenseTotals* d, U32 range, U32 args)
+static B32 qe_dense_totals_alloc(QE_DenseTotals* d, U32 range, U32 args, B32 wide)
 {
   d->range = range;
   d->args = args;
-  U64 bytes = (U64)range * 8 * (1 + args) + (U64)range * 4 * 2 * args;
+  d->wide = wide;
+  U64 bytes = (U64)range * 8 * (1 + args) + (U64)range * (wide ? 8 : 4) * 2 * args;
   d->block = malloc(bytes ? bytes : 8);
   if (!d->block) return 0;
   d->count = (U64*)d->block;
   d->sum = d->count + range;
   d->mn = (U32*)(d->sum + (U64)range * args);
   d->mx = d->mn + (U64)range * args;
+  d->mn64 = d->sum + (U64)range * args;
+  d->mx64 = d->mn64 + (U64)range * args;
   return 1;
 }
 
 U64 qe_dense_totals_bytes(U32 range, U32 args) { return (U64)range * 8 * (1 + args) + (U64)range * 4 * 2 * args; }
 
-/* LDS bytes of a dense GROUP BY block: wide tables count u32 | sums u64 |
- * min/max u32 per argument; narrow ones count+carries u32 | sum low word
- * u32 | min/max u32 for one argument */
+/* LDS bytes of a dense GROUP BY block: count u32 | sums u64 | min/max u32
+ * per argument (u64 when an argument is wider than a 32-bit integer);
+ * narrow tables count+carries u32 | sum low word u32 | min/max u32 for
+ * one argument */
 U64 qe_dense_lds(U32 range, U32 args, B32 need_sum, B32 need_minmax, B32 narrow)
 {
   if (narrow) return (U64)range * (4 + (need_sum ? 4 : 0) + (need_minmax ? 8 : 0));
   return (U64)range * 4 + (need_sum ? (U64)args * range * 8 : 0) + (need_minmax ? (U64)args * range * 8 : 0);
+}
+
+static U64 qe_dense_lds_wide(U32 range, U32 args, B32 need_sum, B32 need_minmax)
+{
+  return (U64)range * 4 + (need_sum ? (U64)args * range * 8 : 0) + (need_minmax ? (U64)args * range * 16 : 0);
+}
+
+/* arguments other than 32-bit integers take the wide tables */
+static B32 qe_dense_wide_args(const QE_Table* t, const U32* arg_cols, U32 args)
+{
+  for (U32 a = 0; a < args; a++)
+    if (!qe_type_int32(t->columns[arg_cols[a]].type)) return 1;
+  return 0;
 }
 
 /* the narrow kernels take one U32 argument (or its narrow codes) and at
@@ -461,7 +478,11 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
                  QE_Trace* trace)
 {
   const B32 terms = w && w->term_count;
-  GPU_Kernel* k = qe_kernel(terms ? "aggregate_dense_or" : "aggregate_dense");
+  /* arguments wider than 32-bit integers: the wide tables, over full-width
+   * columns (no encodings) */
+  const B32 wide = qe_dense_wide_args(t, arg_cols, arg_count);
+  GPU_Kernel* k = qe_kernel(wide ? (terms ? "aggregate_dense_wide_or" : "aggregate_dense_wide")
+                                 : (terms ? "aggregate_dense_or" : "aggregate_dense"));
   if (!k) return 0;
   const U64 span = qe_group_keys_span(keys);
   if (keys->count < 1 || keys->count > QE_DENSE_MAX_KEYS || span == 0 || span > 0xffffffffull) {
@@ -470,11 +491,12 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
   }
   const U32 range = (U32)span;
   U32 flags = (need_sum ? 1u : 0u) | (need_minmax ? 2u : 0u);
-  U64 lds = qe_dense_lds(range, arg_count, need_sum, need_minmax, 0);
+  U64 lds = wide ? qe_dense_lds_wide(range, arg_count, need_sum, need_minmax)
+                 : qe_dense_lds(range, arg_count, need_sum, need_minmax, 0);
   U32 max_lds = gpu_hip_max_lds_bytes();
-  /* the narrow tables (aggregate_dense_narrow_*) when the wide ones do not
+  /* the narrow tables (aggregate_dense_narrow_*) when the full ones do not
    * fit or hold a CU to one 256-thread block */
-  const B32 narrow = qe_dense_narrow_ok(t, arg_cols, arg_count, need_sum, need_minmax, range) &&
+  const B32 narrow = !wide && qe_dense_narrow_ok(t, arg_cols, arg_count, need_sum, need_minmax, range) &&
                      (lds > max_lds || lds > qe_settings_u64("QE_AGG_DENSE_NARROW_ABOVE", 65536));
   if (narrow) lds = qe_dense_lds(range, arg_count, need_sum, need_minmax, 1);
   if (lds > max_lds) {
@@ -498,7 +520,7 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
   for (U32 i = 0; i < keys->count; i++) {
     const U32 col = keys->col[i];
     kslot[i] = qe_slot_of(&slots, col);
-    const U32 enc = qe_col_enc(t, col);
+    const U32 enc = wide ? 0u : qe_col_enc(t, col);
     kbyte[i] = t->columns[col].type;
     kmin[i] = keys->min_code[i];
     if (enc) {
@@ -510,16 +532,12 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
   for (U32 a = 0; a < arg_count; a++) {
     U32 s = qe_slot_of(&slots, arg_cols[a]);
     U32 ty = t->columns[arg_cols[a]].type;
-    if (ty != CQL_ColumnType_U32 && ty != CQL_ColumnType_I32) {
-      qe_set_error("dense group by: arguments must be U32/I32");
-      return 0;
-    }
-    U32 enc = qe_col_enc(t, arg_cols[a]);
+    U32 enc = wide ? 0u : qe_col_enc(t, arg_cols[a]);
     slot_bits |= s << (4 * a);
     type_bits |= (enc ? (CQL_ColumnType_U32 | (enc << 4)) : ty) << (8 * a);
   }
   U32 leaf_words[QE_WHERE_WORDS], leaf_bytes = 0, leaf_packed = 0;
-  if (!qe_where_prepare(t, w, &slots, 1, leaf_words, &leaf_bytes, &leaf_packed)) return 0;
+  if (!qe_where_prepare(t, w, &slots, !wide, leaf_words, &leaf_bytes, &leaf_packed)) return 0;
   const U32 leaf_count = w ? w->leaf_count : 0;
   for (U32 i = 0; i < keys->count; i++)
     if (kslot[i] == ~0u) {
@@ -535,7 +553,7 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     kdesc[2 + 3 * i] = kmin[i];
     kdesc[3 + 3 * i] = (U32)keys->range[i];
   }
-  if (!qe_dense_totals_alloc(out, range, arg_count)) return 0;
+  if (!qe_dense_totals_alloc(out, range, arg_count, wide)) return 0;
   U64 r0 = prune->row_begin, r1 = prune->row_end;
   U64 quads = (r1 + 3) / 4 - r0 / 4;
   /* one LDS table per block: fewer, fuller blocks when the table is big */
@@ -550,7 +568,8 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     blocks = (U32)(need < cus * lds_blocks ? (need ? need : 1) : cus * lds_blocks);
   }
   if ((U64)blocks > cus * lds_blocks) blocks = (U32)(cus * lds_blocks);
-  U64 gbytes = qe_dense_totals_bytes(range, arg_count);
+  const U64 mm_width = wide ? 8 : 4;
+  U64 gbytes = (U64)range * 8 * (1 + arg_count) + (U64)range * mm_width * 2 * arg_count;
   GPU_Buffer* acc = qe_scratch("dense_totals", gbytes);
   GPU_Buffer* leaf_buf = leaf_count ? qe_scratch("dense_leaves", sizeof(leaf_words)) : 0;
   GPU_Buffer* skip_buf = prune->skip_bits ? qe_scratch("dense_skip", prune->skip_words * 4) : 0;
@@ -565,7 +584,8 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     if (skip_buf) gpu_kernel_set_arg_buffer(k, 5, skip_buf);
     else gpu_hip_kernel_clear_arg(k, 5);
     for (U32 s = 0; s < QE_MAX_COLUMN_BINDINGS; s++) {
-      if (s < slots.count) gpu_kernel_set_arg_buffer(k, 6 + s, qe_col_kbuf(t, slots.column[s]));
+      if (s < slots.count)
+        gpu_kernel_set_arg_buffer(k, 6 + s, wide ? t->columns[slots.column[s]].buffer : qe_col_kbuf(t, slots.column[s]));
       else gpu_hip_kernel_clear_arg(k, 6 + s);
     }
     gpu_kernel_set_arg_u64(k, 0, r1 - r0);
@@ -588,7 +608,7 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     U64 zero_bytes = (U64)range * 8 * (1 + arg_count);
     gpu_batch_buffer_zero(b, acc, gbytes);
     if (arg_count && need_minmax) {
-      gpu_batch_buffer_fill(b, acc, zero_bytes + (U64)range * 4 * arg_count, 0xffffffffu);
+      gpu_batch_buffer_fill(b, acc, zero_bytes + (U64)range * mm_width * arg_count, 0xffffffffu);
       gpu_batch_buffer_zero(b, acc, zero_bytes);
     }
     gpu_batch_kernel_execute(b, k, blocks * wg, wg);
@@ -608,7 +628,7 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     return 0;
   }
   /* encoded args: per group, the kernel saw codes = value - base */
-  for (U32 a = 0; a < arg_count; a++) {
+  for (U32 a = 0; a < arg_count && !wide; a++) {
     if (!qe_col_enc(t, arg_cols[a])) continue;
     const QE_Column* col = &t->columns[arg_cols[a]];
     const U32 base = col->enc_base, flip = col->type == CQL_ColumnType_I32 ? 0x80000000u : 0u;
@@ -628,7 +648,8 @@ B32 qe_dense_run(QE_Table* t, const QE_GroupKeys* keys, const U32* arg_cols, U32
     U64 bytes = 0;
     for (U32 s = 0; s < slots.count; s++) bytes += (r1 - r0) * t->columns[slots.column[s]].width;
     trace->bytes_read = bytes;
-    snprintf(trace->kernel, sizeof(trace->kernel), "%s", narrow ? "aggregate_dense_narrow" : "aggregate_dense");
+    snprintf(trace->kernel, sizeof(trace->kernel), "%s",
+             narrow ? "aggregate_dense_narrow" : (wide ? "aggregate_dense_wide" : "aggregate_dense"));
   }
   return 1;
 }
@@ -1040,8 +1061,10 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
   const U64 span = qe_group_keys_span(&keys);
   U64 max_range = qe_settings_u64("QE_AGG_DENSE_MAX_RANGE", 65536);
   B32 dense = qe_settings_u64("QE_AGG_DENSE", 1) != 0 && K <= QE_DENSE_MAX_KEYS && span != 0 && span <= max_range;
-  for (U32 a = 0; a < plan.arg_count && dense; a++)
-    if (!qe_type_int32(table->columns[plan.arg_col[a]].type)) dense = 0;
+  /* arguments wider than 32-bit integers take the wide tables
+   * (QE_AGG_DENSE_WIDE, default 1), else the hash table */
+  const B32 wide_args = qe_dense_wide_args(table, plan.arg_col, plan.arg_count);
+  if (wide_args && qe_settings_u64("QE_AGG_DENSE_WIDE", 1) == 0) dense = 0;
   /* args that are key columns fold into the counts: SUM = key * count,
    * MIN = MAX = key (exact for whole-number keys) */
   S32 arg_key[QE_AGG_MAX_ARGS];
@@ -1056,10 +1079,14 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
    * min/max u32[E][R]; or the narrow tables of one argument */
   if (dense) {
     const U32 e = fold ? 0 : plan.arg_count;
-    const U64 lds = qe_dense_lds((U32)span, e, plan.need_sum, plan.need_minmax, 0);
-    if (lds > gpu_hip_max_lds_bytes() &&
-        (fold || !qe_dense_narrow_ok(table, plan.arg_col, plan.arg_count, plan.need_sum, plan.need_minmax, (U32)span)))
-      dense = 0;
+    if (wide_args && !fold) {
+      if (qe_dense_lds_wide((U32)span, e, plan.need_sum, plan.need_minmax) > gpu_hip_max_lds_bytes()) dense = 0;
+    } else {
+      const U64 lds = qe_dense_lds((U32)span, e, plan.need_sum, plan.need_minmax, 0);
+      if (lds > gpu_hip_max_lds_bytes() &&
+          (fold || !qe_dense_narrow_ok(table, plan.arg_col, plan.arg_count, plan.need_sum, plan.need_minmax, (U32)span)))
+        dense = 0;
+    }
   }
   if (!dense) {
     U64 groups = 0;
@@ -1103,11 +1130,23 @@ static B32 qe_aggregate_run(QE_Table* table, const QE_IRNode* where, U32 where_c
           } else {
             U32 ty = table->columns[plan.arg_col[a]].type;
             U64 s = tot.sum[(U64)a * range + i];
-            isum = (S64)s;
-            sum = ty == CQL_ColumnType_I32 ? (F64)(S64)s : (F64)s;
-            U32 lo = tot.mn[(U64)a * range + i], hi = tot.mx[(U64)a * range + i];
-            mn = ty == CQL_ColumnType_I32 ? (F64)(S32)(lo ^ 0x80000000u) : (F64)lo;
-            mx = ty == CQL_ColumnType_I32 ? (F64)(S32)(hi ^ 0x80000000u) : (F64)hi;
+            if (tot.wide) {
+              /* f64 sums for the wider types, min/max as f64 order keys */
+              mn = qe_f64_from_order_code(tot.mn64[(U64)a * range + i]);
+              mx = qe_f64_from_order_code(tot.mx64[(U64)a * range + i]);
+              if (!qe_type_int32(ty)) {
+                memcpy(&sum, &s, 8);
+                out_cells[g * expr_count + e] = qe_agg_cell(plan.funcs[e], cnt, sum, mn, mx);
+                continue;
+              }
+              isum = (S64)s;
+            } else {
+              isum = (S64)s;
+              sum = ty == CQL_ColumnType_I32 ? (F64)(S64)s : (F64)s;
+              U32 lo = tot.mn[(U64)a * range + i], hi = tot.mx[(U64)a * range + i];
+              mn = ty == CQL_ColumnType_I32 ? (F64)(S32)(lo ^ 0x80000000u) : (F64)lo;
+              mx = ty == CQL_ColumnType_I32 ? (F64)(S32)(hi ^ 0x80000000u) : (F64)hi;
+            }
           }
           out_cells[g * expr_count + e] = qe_agg_cell_int(plan.funcs[e], cnt, isum, mn, mx, raw);
           continue;

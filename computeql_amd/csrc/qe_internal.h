@@ -324,10 +324,13 @@ F64 qe_group_key_value(const QE_GroupKeys* k, U32 i, U64 d);
  * sum u64[E][R]; min u32[E][R]; max u32[E][R] */
 typedef struct QE_DenseTotals {
   U32 range, args;
+  B32 wide; /* arguments wider than 32-bit integers: sums f64 bits for those, min/max as f64 order keys */
   U64* count;
   U64* sum;
   U32* mn;
   U32* mx;
+  U64* mn64;
+  U64* mx64;
   void* block;
 } QE_DenseTotals;
 

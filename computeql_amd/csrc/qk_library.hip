@@ -536,7 +536,7 @@ extern "C" __global__ void LB k_scan_filter(QK_Args a)
 /* ------------------------------------------------------------------ */
 __device__ __forceinline__ qk_u32 qk_code32(qk_u32 v, qk_u32 type) { return type == QK_T_I32 ? (v ^ 0x80000000u) : v; }
 
-template <bool TERMS>
+template <bool TERMS, bool WIDE>
 __device__ __forceinline__ void qk_aggregate_dense(const QK_Args& a)
 {
   extern __shared__ qk_u64 lds_u64[];
@@ -558,18 +558,28 @@ __device__ __forceinline__ void qk_aggregate_dense(const QK_Args& a)
   const qk_u32* kd = a.ptr[2] ? qk_ptr<qk_u32>(a, 2) : nullptr;
   const qk_u32 nkeys = kd ? kd[0] : 1u;
 
-  /* LDS: sums u64[E][R] | counts u32[R] | min u32[E][R] | max u32[E][R] */
-  qk_u64* l_sum = lds_u64;
-  qk_u32* l_cnt = (qk_u32*)(lds_u64 + ((flags & 1u) ? (qk_u64)nargs * range : 0));
-  qk_u32* l_min = l_cnt + range;
-  qk_u32* l_max = l_min + ((flags & 2u) ? (qk_u64)nargs * range : 0);
+  /* LDS: sums u64[E][R] | counts u32[R] | min u32[E][R] | max u32[E][R];
+   * WIDE (any argument wider than a 32-bit integer): sums u64[E][R] (exact
+   * integers for U32/I32 arguments, f64 otherwise) | min u64[E][R] | max
+   * u64[E][R] (order-preserving keys of the f64 value) | counts u32[R] */
   const qk_u32 sum_words = (flags & 1u) ? nargs * range : 0u;
   const qk_u32 mm_words = (flags & 2u) ? nargs * range : 0u;
+  qk_u64* l_sum = lds_u64;
+  qk_u64* l_min64 = l_sum + sum_words;
+  qk_u64* l_max64 = l_min64 + mm_words;
+  qk_u32* l_cnt = WIDE ? (qk_u32*)(l_max64 + mm_words) : (qk_u32*)(lds_u64 + sum_words);
+  qk_u32* l_min = l_cnt + range;
+  qk_u32* l_max = l_min + mm_words;
   for (qk_u32 i = threadIdx.x; i < sum_words; i += QK_WG) l_sum[i] = 0;
   for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) l_cnt[i] = 0;
   for (qk_u32 i = threadIdx.x; i < mm_words; i += QK_WG) {
-    l_min[i] = 0xffffffffu;
-    l_max[i] = 0u;
+    if (WIDE) {
+      l_min64[i] = ~0ULL;
+      l_max64[i] = 0ULL;
+    } else {
+      l_min[i] = 0xffffffffu;
+      l_max[i] = 0u;
+    }
   }
   __syncthreads();
 
@@ -637,6 +647,27 @@ __device__ __forceinline__ void qk_aggregate_dense(const QK_Args& a)
     for (qk_u32 c = 0; c < nargs; c++) {
       qk_u32 at = (types >> (8 * c)) & 0xfu, aenc = (types >> (8 * c + 4)) & 3u;
       const qk_u64 cp = a.ptr[QK_B_COL0 + ((slots >> (4 * c)) & 0xfu)];
+      if (WIDE && !qk_type_is_int32(at)) {
+        /* f32 / f64 / i64 / u64: the value as f64, summed as f64 */
+        const qk_u32 aslot = (slots >> (4 * c)) & 0xfu;
+#pragma unroll
+        for (int j = 0; j < QK_UNROLL; j++) {
+          double v[4];
+          qk_load4_f64(a, aslot, at, q[j], v);
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            if (!(m[j] & (1u << k))) continue;
+            const qk_u64 i = (qk_u64)c * range + g[j][k];
+            if (flags & 1u) atomicAdd((double*)&l_sum[i], v[k]);
+            if (flags & 2u) {
+              const qk_u64 key = qk_f64_key(v[k]);
+              atomicMin(&l_min64[i], key);
+              atomicMax(&l_max64[i], key);
+            }
+          }
+        }
+        continue;
+      }
       qk_u32x4 x[QK_UNROLL];
       if (aenc == 0) {
 #pragma unroll
@@ -656,36 +687,59 @@ __device__ __forceinline__ void qk_aggregate_dense(const QK_Args& a)
             atomicAdd(&l_sum[c * range + g[j][k]], add);
           }
           if (flags & 2u) {
-            qk_u32 code = qk_code32(w[k], at);
-            atomicMin(&l_min[c * range + g[j][k]], code);
-            atomicMax(&l_max[c * range + g[j][k]], code);
+            if (WIDE) {
+              const qk_u64 key = qk_f64_key(at == QK_T_I32 ? (double)(qk_s32)w[k] : (double)w[k]);
+              atomicMin(&l_min64[c * range + g[j][k]], key);
+              atomicMax(&l_max64[c * range + g[j][k]], key);
+            } else {
+              qk_u32 code = qk_code32(w[k], at);
+              atomicMin(&l_min[c * range + g[j][k]], code);
+              atomicMax(&l_max[c * range + g[j][k]], code);
+            }
           }
         }
       }
     }
   }
   __syncthreads();
+  /* global: count u64[R] | sum u64[E][R] | min, max u32[E][R] (WIDE: u64) */
   qk_u64* g_cnt = qk_wptr<qk_u64>(a, 1);
   qk_u64* g_sum = g_cnt + range;
   qk_u32* g_min = (qk_u32*)(g_sum + (qk_u64)nargs * range);
   qk_u32* g_max = g_min + (qk_u64)nargs * range;
+  qk_u64* g_min64 = g_sum + (qk_u64)nargs * range;
+  qk_u64* g_max64 = g_min64 + (qk_u64)nargs * range;
   for (qk_u32 i = threadIdx.x; i < range; i += QK_WG) {
     qk_u32 cnt = l_cnt[i];
     if (!cnt) continue;
     atomicAdd(&g_cnt[i], (qk_u64)cnt);
     for (qk_u32 c = 0; c < nargs; c++) {
-      if (flags & 1u) atomicAdd(&g_sum[(qk_u64)c * range + i], l_sum[c * range + i]);
+      const qk_u64 gi = (qk_u64)c * range + i;
+      if (flags & 1u) {
+        if (WIDE && !qk_type_is_int32((types >> (8 * c)) & 0xfu))
+          atomicAdd((double*)&g_sum[gi], __longlong_as_double((qk_s64)l_sum[c * range + i]));
+        else
+          atomicAdd(&g_sum[gi], l_sum[c * range + i]);
+      }
       if (flags & 2u) {
-        atomicMin(&g_min[(qk_u64)c * range + i], l_min[c * range + i]);
-        atomicMax(&g_max[(qk_u64)c * range + i], l_max[c * range + i]);
+        if (WIDE) {
+          atomicMin(&g_min64[gi], l_min64[c * range + i]);
+          atomicMax(&g_max64[gi], l_max64[c * range + i]);
+        } else {
+          atomicMin(&g_min[gi], l_min[c * range + i]);
+          atomicMax(&g_max[gi], l_max[c * range + i]);
+        }
       }
     }
   }
 }
 
-extern "C" __global__ void LB k_aggregate_dense(QK_Args a) { qk_aggregate_dense<false>(a); }
+extern "C" __global__ void LB k_aggregate_dense(QK_Args a) { qk_aggregate_dense<false, false>(a); }
 /* the same over a WHERE of several conjunctions */
-extern "C" __global__ void LB k_aggregate_dense_or(QK_Args a) { qk_aggregate_dense<true>(a); }
+extern "C" __global__ void LB k_aggregate_dense_or(QK_Args a) { qk_aggregate_dense<true, false>(a); }
+/* arguments wider than 32-bit integers (f32, f64, i64, u64) */
+extern "C" __global__ void LB k_aggregate_dense_wide(QK_Args a) { qk_aggregate_dense<false, true>(a); }
+extern "C" __global__ void LB k_aggregate_dense_wide_or(QK_Args a) { qk_aggregate_dense<true, true>(a); }
 
 /* ------------------------------------------------------------------ */
 /* aggregate_dense_narrow: aggregate_dense for group ranges too large    */
@@ -3497,6 +3551,8 @@ static const QK_LibraryKernel g_library[] = {
     {"scan_filter", (const void*)k_scan_filter, QK_WG},
     {"aggregate_dense", (const void*)k_aggregate_dense, QK_WG},
     {"aggregate_dense_or", (const void*)k_aggregate_dense_or, QK_WG},
+    {"aggregate_dense_wide", (const void*)k_aggregate_dense_wide, QK_WG},
+    {"aggregate_dense_wide_or", (const void*)k_aggregate_dense_wide_or, QK_WG},
     {"aggregate_dense_narrow_s", (const void*)k_aggregate_dense_narrow_s, QK_NARROW_WG},
     {"aggregate_dense_narrow_m", (const void*)k_aggregate_dense_narrow_m, QK_NARROW_WG},
     {"aggregate_dense_narrow_sm", (const void*)k_aggregate_dense_narrow_sm, QK_NARROW_WG},
