@@ -101,6 +101,18 @@ static U32 qe_batch_rec_words(const QE_BatchPlanned* q)
   return QE_BATCH_REC_WORDS + 6 * q->where.leaf_count + (q->where.term_count + 3) / 4;
 }
 
+/* a query's relative cost per tile in the shared scan: its leaves, and
+ * per argument the LDS atomics (grouped) or wave reductions (ungrouped)
+ * of its sum and min/max */
+static U32 qe_batch_cost(const QE_BatchPlanned* q)
+{
+  const U32 per_arg = (q->plan.need_sum ? 1u : 0u) + (q->plan.need_minmax ? 2u : 0u);
+  U32 c = 2 * q->where.leaf_count + q->where.term_count;
+  if (q->group_col >= 0) c += 4 + 4 * q->plan.arg_count * per_arg;
+  else c += 2 + 2 * q->plan.arg_count * per_arg;
+  return c;
+}
+
 /* staged columns of a launch: the union of the columns its queries read,
  * in first-use order; returns the stage index of `column` (adding it) */
 typedef struct QE_BatchStage {
@@ -138,6 +150,7 @@ static U64 qe_batch_lds(U64 W, U64 stage_bytes) { return qe_batch_lds_n(W, stage
 static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items, U32* out_words, U32* out_w,
                              U32* out_wl, U64* out_stage_bytes)
 {
+  if (n > QE_BATCH_MAX_LAUNCH_QUERIES) return 0; /* launches are split below that */
   QE_BatchStage st;
   memset(&st, 0, sizeof(st));
   U32 W = 0, WL = 0;
@@ -171,10 +184,29 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
     p[hdr + n + 3 * u + 2] = off;
     off += QE_WG * items * w;
   }
+  /* the records are listed most expensive first: the kernel's waves take
+   * them in that order, so the long ones start first and the short ones
+   * fill in (longest-first list scheduling) */
+  U32 order[QE_BATCH_MAX_LAUNCH_QUERIES], cost[QE_BATCH_MAX_LAUNCH_QUERIES];
+  for (U32 i = 0; i < n; i++) {
+    order[i] = i;
+    cost[i] = qe_batch_cost(&qs[i]);
+  }
+  for (U32 i = 1; i < n; i++) {
+    const U32 o = order[i];
+    U32 j = i;
+    while (j > 0 && cost[order[j - 1]] < cost[o]) {
+      order[j] = order[j - 1];
+      j--;
+    }
+    order[j] = o;
+  }
+  U32 rank_of[QE_BATCH_MAX_LAUNCH_QUERIES];
+  for (U32 r = 0; r < n; r++) rank_of[order[r]] = r;
   U32 at = hdr + n + 3 * st.count;
   for (U32 i = 0; i < n; i++) {
     QE_BatchPlanned* q = &qs[i];
-    p[hdr + i] = at;
+    p[hdr + rank_of[i]] = at;
     U32* r = p + at;
     r[0] = q->where.leaf_count | (q->where.term_count << 8);
     r[1] = (q->group_col >= 0 ? 1u : 0u) | (q->plan.need_sum ? 2u : 0u) | (q->plan.need_minmax ? 4u : 0u) |

@@ -2451,8 +2451,9 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /* buffer (global_load_lds, 16 bytes per lane, no registers in          */
 /* between), the copies of the next tile are issued into the other      */
 /* buffer before the current one is evaluated, and the queries are then */
-/* evaluated on the tile from LDS, wave w taking queries w, w + 4, ...  */
-/* over all the tile's rows, so each column is read from HBM once per   */
+/* evaluated on the tile from LDS, each wave taking the next query of  */
+/* the list (most expensive first) from an LDS counter until none is    */
+/* left, over all the tile's rows, so each column is read from HBM once per */
 /* launch however many queries read it and each query's record and     */
 /* reductions are paid once per tile and wave.                          */
 /* Accumulators of a query, u64 words from its offset: count[R], then   */
@@ -2690,6 +2691,16 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
   const qk_cu32p st = prog + QK_SS_HDR + Q;
   qk_u64* acc = lds_u64;
   unsigned char* buf0 = (unsigned char*)lds_u64 + (((qk_u64)WL * 8 + 15) & ~15ULL);
+  /* the waves take a tile's queries from a counter (in the order the
+   * host listed them, most expensive first), so a wave that drew cheap
+   * queries takes more instead of waiting at the next tile's barrier;
+   * tile t uses counter t & 1, reset for tile t + 1 once every wave has
+   * passed tile t's barrier */
+  __shared__ qk_u32 s_qnext[2];
+  if (threadIdx.x == 0) {
+    s_qnext[0] = 0u;
+    s_qnext[1] = 0u;
+  }
   {
     /* every replica of a query's words starts at its op's identity */
     const qk_u32 wave = (qk_u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63u;
@@ -2724,16 +2735,21 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
     const unsigned char* sb = buf0 + (dbl ? ((t - t_begin) & 1u) * stage_bytes : 0);
     if (dbl && t + 1 < t_end)
       qk_ss_stage_issue<WG>(st, U, cols, (t + 1) * tile, tile, rows, buf0 + ((t + 1 - t_begin) & 1u) * stage_bytes);
-    /* wave w evaluates queries w, w + 4, ... over all rows of the tile:
+    /* a wave evaluates each query it takes over all rows of the tile:
      * lane l takes rows l, l + 64, ... (RPL rows) */
     const qk_u64 base = t * tile;
     const qk_u32 lane = threadIdx.x & 63u;
     qk_u32 valid = 0;
 #pragma unroll
     for (int i = 0; i < RPL; i++) valid |= (base + (qk_u64)i * 64 + lane < rows) ? (1u << i) : 0u;
-    const qk_u32 wave = (qk_u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const qk_u32 cur = (qk_u32)((t - t_begin) & 1u);
+    if (threadIdx.x == 0) s_qnext[cur ^ 1u] = 0u;
 #pragma unroll 1
-    for (qk_u32 q = wave; q < Q; q += WG / 64) {
+    for (;;) {
+      qk_u32 q = 0u;
+      if (lane == 0) q = atomicAdd(&s_qnext[cur], 1u);
+      q = (qk_u32)__builtin_amdgcn_readfirstlane((int)q);
+      if (q >= Q) break;
       const qk_cu32p rec = prog + prog[QK_SS_HDR + q];
       const qk_u32 L = (rec[0] & 0xffu) < 8u ? (rec[0] & 0xffu) : 8u;
       const qk_u32 T = (rec[0] >> 8) & 0xffu; /* terms: 0 = the AND of the leaves */
