@@ -76,7 +76,7 @@ extern "C" __global__ void LB k_bandwidth_copy(QK_Args a)
 #pragma unroll
     for (int j = 0; j < QK_BW_UNROLL; j++) {
       qk_u64 i = s + (qk_u64)j * QK_WG + threadIdx.x;
-      x[j] = i < b1 ? p[i] : qk_u32x4{0, 0, 0, 0};
+      x[j] = i < b1 ? qk_ld(p + i) : qk_u32x4{0, 0, 0, 0};
     }
 #pragma unroll
     for (int j = 0; j < QK_BW_UNROLL; j++) {
@@ -98,7 +98,7 @@ extern "C" __global__ void LB k_bandwidth_read(QK_Args a)
 #pragma unroll
     for (int j = 0; j < QK_BW_UNROLL; j++) {
       qk_u64 i = s + (qk_u64)j * QK_WG + threadIdx.x;
-      x[j] = i < b1 ? p[i] : qk_u32x4{0, 0, 0, 0};
+      x[j] = i < b1 ? qk_ld(p + i) : qk_u32x4{0, 0, 0, 0};
     }
 #pragma unroll
     for (int j = 0; j < QK_BW_UNROLL; j++) acc ^= x[j];
@@ -139,7 +139,7 @@ __device__ __forceinline__ void agg_fold_quad(const QK_Args& a, qk_u32 slot, qk_
 {
   qk_u64 base = a.ptr[QK_B_COL0 + slot];
   if (type == QK_T_U32) {
-    qk_u32x4 x = qk_gptr<const qk_u32x4>(base)[q];
+    qk_u32x4 x = qk_ld(qk_gptr<const qk_u32x4>(base) + q);
     if (m == 0xfu) {
       acc.sum += (qk_u64)x.x + (qk_u64)x.y + (qk_u64)x.z + (qk_u64)x.w;
       qk_u32 lo = min(min(x.x, x.y), min(x.z, x.w));
@@ -158,7 +158,7 @@ __device__ __forceinline__ void agg_fold_quad(const QK_Args& a, qk_u32 slot, qk_
       }
     }
   } else if (type == QK_T_I32) {
-    qk_u32x4 x = qk_gptr<const qk_u32x4>(base)[q];
+    qk_u32x4 x = qk_ld(qk_gptr<const qk_u32x4>(base) + q);
     qk_u32 w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -242,7 +242,7 @@ __device__ __forceinline__ void qk_aggregate_global(const QK_Args& a)
         if (enc == 0) {
           const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + slot);
 #pragma unroll
-          for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+          for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_ld(col + q[j]);
         } else {
 #pragma unroll
           for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_quad_codes(a.ptr[QK_B_COL0 + slot], enc, q[j]);
@@ -596,7 +596,7 @@ __device__ __forceinline__ void qk_aggregate_dense(const QK_Args& a)
     if (nkeys == 1) {
       if (key_enc == 0) {
 #pragma unroll
-        for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[q[j]];
+        for (int j = 0; j < QK_UNROLL; j++) kv[j] = qk_ld(keys + q[j]);
       } else {
 #pragma unroll
         for (int j = 0; j < QK_UNROLL; j++) kv[j] = qk_quad_codes(a.ptr[QK_B_COL0 + key_slot], key_enc, q[j]);
@@ -623,7 +623,7 @@ __device__ __forceinline__ void qk_aggregate_dense(const QK_Args& a)
         const qk_u64 kp = a.ptr[QK_B_COL0 + (d & 0xffu)];
 #pragma unroll
         for (int j = 0; j < QK_UNROLL; j++) {
-          const qk_u32x4 x = ke == 0 ? qk_gptr<const qk_u32x4>(kp)[q[j]] : qk_quad_codes(kp, ke, q[j]);
+          const qk_u32x4 x = ke == 0 ? qk_ld(qk_gptr<const qk_u32x4>(kp) + q[j]) : qk_quad_codes(kp, ke, q[j]);
           const qk_u32 w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
           for (int k = 0; k < 4; k++) {
@@ -671,7 +671,7 @@ __device__ __forceinline__ void qk_aggregate_dense(const QK_Args& a)
       qk_u32x4 x[QK_UNROLL];
       if (aenc == 0) {
 #pragma unroll
-        for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_gptr<const qk_u32x4>(cp)[q[j]];
+        for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_ld(qk_gptr<const qk_u32x4>(cp) + q[j]);
       } else {
 #pragma unroll
         for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_quad_codes(cp, aenc, q[j]);
@@ -822,7 +822,7 @@ __device__ __forceinline__ void qk_dense_narrow(const QK_Args& a)
     if (nkeys == 1) {
 #pragma unroll
       for (int j = 0; j < U; j++) {
-        const qk_u32x4 kv = key_enc == 0 ? qk_ptr<qk_u32x4>(a, QK_B_COL0 + key_slot)[q[j]]
+        const qk_u32x4 kv = key_enc == 0 ? qk_ld(qk_ptr<qk_u32x4>(a, QK_B_COL0 + key_slot) + q[j])
                                          : qk_quad_codes(a.ptr[QK_B_COL0 + key_slot], key_enc, q[j]);
         const qk_u32 w[4] = {kv.x, kv.y, kv.z, kv.w};
 #pragma unroll
@@ -840,7 +840,7 @@ __device__ __forceinline__ void qk_dense_narrow(const QK_Args& a)
         const qk_u64 kp = a.ptr[QK_B_COL0 + (d & 0xffu)];
 #pragma unroll
         for (int j = 0; j < U; j++) {
-          const qk_u32x4 x = ke == 0 ? qk_gptr<const qk_u32x4>(kp)[q[j]] : qk_quad_codes(kp, ke, q[j]);
+          const qk_u32x4 x = ke == 0 ? qk_ld(qk_gptr<const qk_u32x4>(kp) + q[j]) : qk_quad_codes(kp, ke, q[j]);
           const qk_u32 w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
           for (int k = 0; k < 4; k++) {
@@ -858,7 +858,7 @@ __device__ __forceinline__ void qk_dense_narrow(const QK_Args& a)
     }
     qk_u32x4 x[U];
 #pragma unroll
-    for (int j = 0; j < U; j++) x[j] = aenc == 0 ? qk_gptr<const qk_u32x4>(cp)[q[j]] : qk_quad_codes(cp, aenc, q[j]);
+    for (int j = 0; j < U; j++) x[j] = aenc == 0 ? qk_ld(qk_gptr<const qk_u32x4>(cp) + q[j]) : qk_quad_codes(cp, aenc, q[j]);
 #pragma unroll
     for (int j = 0; j < U; j++) {
       const qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
@@ -1029,7 +1029,7 @@ extern "C" __global__ void LB k_join_probe(QK_Args a)
     qk_u32x4 kv[QK_UNROLL];
     qk_step_quads(a, s, b1, r0, r1, qs, m);
 #pragma unroll
-    for (int j = 0; j < QK_UNROLL; j++) kv[j] = keys[qs[j]];
+    for (int j = 0; j < QK_UNROLL; j++) kv[j] = qk_ld(keys + qs[j]);
     /* the home slots of all 16 keys are loaded together; only keys whose
      * home slot holds another key walk on (linear probing) */
     qk_u32 pay[QK_UNROLL][4];
@@ -1079,7 +1079,7 @@ extern "C" __global__ void LB k_join_probe(QK_Args a)
       const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + ((slots >> (4 * c)) & 0xfu));
       qk_u32x4 x[QK_UNROLL];
 #pragma unroll
-      for (int j = 0; j < QK_UNROLL; j++) x[j] = col[qs[j]];
+      for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_ld(col + qs[j]);
 #pragma unroll
       for (int j = 0; j < QK_UNROLL; j++) {
         qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
@@ -1231,7 +1231,7 @@ __device__ __forceinline__ void qk_part_hist(const QK_Args& a)
       qk_u32 v = in ? qk_quad_valid(qq, r0, r1) : 0u;
       if (v && qk_quad_skipped(a, qq)) v = 0u;
       m[j] = v;
-      kv[j] = keys[in ? qq : b1 - 1];
+      kv[j] = qk_ld(keys + (in ? qq : b1 - 1));
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {
@@ -1369,7 +1369,7 @@ __device__ __forceinline__ void qk_part_scatter(const QK_Args& a)
       q[j] = nq[j];
       m[j] = nm[j];
       kv[j] = nkv[j];
-      if (nargs) x0[j] = col0[q[j]];
+      if (nargs) x0[j] = qk_ld(col0 + q[j]);
     }
     load_step(s + qstep < b1 ? s + qstep : s);
     /* a row is stored with its key's hash h = fmix32(key) in place of the
@@ -1440,7 +1440,7 @@ __device__ __forceinline__ void qk_part_scatter(const QK_Args& a)
         const qk_u32x4* col = qk_ptr<qk_u32x4>(a, QK_B_COL0 + ((slots >> (4 * c)) & 0xfu));
         qk_u32x4 x[QK_UNROLL];
 #pragma unroll
-        for (int j = 0; j < QK_UNROLL; j++) x[j] = col[q[j]];
+        for (int j = 0; j < QK_UNROLL; j++) x[j] = qk_ld(col + q[j]);
 #pragma unroll
         for (int j = 0; j < QK_UNROLL; j++) {
           qk_u32 w[4] = {x[j].x, x[j].y, x[j].z, x[j].w};
@@ -1589,8 +1589,8 @@ __device__ __forceinline__ void qk_part_scatter_wc(const QK_Args& a)
       if (v && qk_quad_skipped(a, qq)) v = 0u;
       nm[j] = v;
       const qk_u64 q = in ? qq : b1 - 1;
-      nkv[j] = keys[q];
-      if (NA) nx[j] = col0[q];
+      nkv[j] = qk_ld(keys + q);
+      if (NA) nx[j] = qk_ld(col0 + q);
     }
   };
   if (b0 < b1) load_step(b0);
@@ -1837,9 +1837,9 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
       qk_u64 x = r + (qk_u64)j * WG + threadIdx.x;
       qk_u64 y = x < re ? x : rs;
       if (W == 2) {
-        uint2 v = ((const uint2*)rowsbuf)[y];
-        nkey[j] = v.x;
-        narg[j] = v.y;
+        const qk_u64 v = qk_ld((const qk_u64*)rowsbuf + y);
+        nkey[j] = (qk_u32)v;
+        narg[j] = (qk_u32)(v >> 32);
       } else {
         nkey[j] = rowsbuf[y * W];
         narg[j] = W > 1 ? rowsbuf[y * W + 1] : 0u;
@@ -2033,7 +2033,7 @@ __device__ __forceinline__ void qk_load4_code(const QK_Args& a, qk_u32 slot, qk_
 {
   const qk_u64 base = a.ptr[QK_B_COL0 + slot];
   if (type == QK_T_U32 || type == QK_T_I32 || type == QK_T_F32) {
-    const qk_u32x4 x = qk_gptr<const qk_u32x4>(base)[q];
+    const qk_u32x4 x = qk_ld(qk_gptr<const qk_u32x4>(base) + q);
     const qk_u32 w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
     for (int k = 0; k < 4; k++)
@@ -2041,7 +2041,7 @@ __device__ __forceinline__ void qk_load4_code(const QK_Args& a, qk_u32 slot, qk_
                               : (type == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)w[k] : qk_key_code_f64((double)__uint_as_float(w[k])));
   } else {
     const qk_u64x2* p = qk_gptr<const qk_u64x2>(base) + 2 * q;
-    const qk_u64x2 x0 = p[0], x1 = p[1];
+    const qk_u64x2 x0 = qk_ld(p), x1 = qk_ld(p + 1);
     const qk_u64 w[4] = {x0.x, x0.y, x1.x, x1.y};
 #pragma unroll
     for (int k = 0; k < 4; k++) c[k] = type == QK_T_F64 ? qk_key_code_f64(__longlong_as_double((qk_s64)w[k])) : w[k];
