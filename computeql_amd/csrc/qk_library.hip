@@ -2673,7 +2673,7 @@ __device__ __forceinline__ void qk_ss_stage_issue(qk_cu32p st, qk_u32 U, qk_cu64
       qk_u64 src = cb + lane * 16u;
       src = src <= last ? src : last;
       __builtin_amdgcn_global_load_lds((const QK_GLOBAL void*)(cols[c] + src),
-                                       (__attribute__((address_space(3))) void*)(buf + off + (k << 10)), 16, 0, 0);
+                                       (__attribute__((address_space(3))) void*)(buf + off + (k << 10)), 16, 0, 2 /* nt */);
     }
   }
 }
