@@ -49,6 +49,7 @@ struct GPU_Kernel {
   hipFunction_t fn;
   QK_Args args;
   U32 lds_bytes;
+  U32 static_lds; /* the kernel's own __shared__ bytes, beside lds_bytes */
   U32 missing_mask; /* bindings set to a null buffer (gpu_dispatch.c:260-306) */
   bool generated;
 };
@@ -276,6 +277,16 @@ bool prepare(GPU_Kernel* k, U32 global_work_size, U32 local_work_size, PreparedL
   if (blocks > 0x7fffffffULL) {
     set_error("%s: grid too large", k->name.c_str());
     return false;
+  }
+  {
+    /* a launch whose dynamic and static LDS exceed a workgroup's would
+     * abort the queue: refuse it here with the sizes */
+    const U64 limit = gpu_hip_max_lds_bytes();
+    if ((U64)k->lds_bytes + k->static_lds > limit) {
+      snprintf(t_last_error, sizeof(t_last_error), "%s: %u bytes of dynamic LDS and %u static exceed the %llu a workgroup has",
+               k->name.c_str(), k->lds_bytes, k->static_lds, (unsigned long long)limit);
+      return false;
+    }
   }
   out.k = k;
   out.args = k->args; /* snapshot */
@@ -750,6 +761,10 @@ GPU_Kernel* gpu_hip_kernel_alloc(String8 name)
     k->generated = true;
     k->missing_mask = 0;
     k->lds_bytes = 0;
+    {
+      int v = 0;
+      k->static_lds = hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, k->fn) == hipSuccess ? (U32)v : 0u;
+    }
     gi->second.tick = ++s.generated_tick;
     ks[key] = k;
     return k;
@@ -763,6 +778,11 @@ GPU_Kernel* gpu_hip_kernel_alloc(String8 name)
   k->generated = false;
   k->missing_mask = 0;
   k->lds_bytes = 0;
+  {
+    hipFuncAttributes fa;
+    memset(&fa, 0, sizeof(fa));
+    k->static_lds = hipFuncGetAttributes(&fa, lib->host_fn) == hipSuccess ? (U32)fa.sharedSizeBytes : 0u;
+  }
   ks[key] = k;
   return k;
 }
@@ -1435,6 +1455,13 @@ U32 gpu_hip_max_lds_bytes(void)
   size_t optin = st().props.sharedMemPerBlockOptin;
   size_t base = st().props.sharedMemPerBlock;
   return (U32)(optin > base ? optin : base);
+}
+
+U32 gpu_hip_kernel_max_lds_bytes(GPU_Kernel* kernel)
+{
+  const U32 max = gpu_hip_max_lds_bytes();
+  if (!kernel) return max;
+  return kernel->static_lds < max ? max - kernel->static_lds : 0u;
 }
 
 U64 gpu_hip_buffer_device_address(GPU_Buffer* b) { return b ? (U64)(uintptr_t)b->ptr : 0; }

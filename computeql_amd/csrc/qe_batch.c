@@ -143,6 +143,10 @@ static U64 qe_batch_stage_bytes(QE_Table* t, const QE_BatchStage* st, U32 items)
   return b;
 }
 
+/* the dynamic LDS a shared-scan block may take: every instance keeps the
+ * same static words (its query counters) beside the stage and tables */
+static U32 qe_batch_max_lds(void) { return gpu_hip_kernel_max_lds_bytes(qe_kernel("shared_scan_256_1")); }
+
 static U64 qe_batch_lds_n(U64 W, U64 stage_bytes, U32 buffers) { return ((W * 8 + 15) & ~15ull) + buffers * stage_bytes; }
 static U64 qe_batch_lds(U64 W, U64 stage_bytes) { return qe_batch_lds_n(W, stage_bytes, 1); }
 
@@ -312,7 +316,7 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
   }
   U64 want_wg = cfg->wg;
   U64 dbl_mode = cfg->buffers;
-  U32 max_lds = gpu_hip_max_lds_bytes();
+  U32 max_lds = qe_batch_max_lds();
   U32 wg = 256, items = 1, buffers = 1;
   B32 found = 0;
   for (U32 i = 0; i < QE_BATCH_SHAPES && !found; i++) {
@@ -577,7 +581,7 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
   /* launches are packed so one block's LDS holds a 1024-row stage of
    * their columns and their accumulators (lds_budget, at most the LDS of
    * a CU; 0 = all of it) */
-  U32 max_lds = gpu_hip_max_lds_bytes();
+  U32 max_lds = qe_batch_max_lds();
   U64 lds_budget = cfg.lds_budget && cfg.lds_budget < max_lds ? cfg.lds_budget : max_lds;
   QE_BatchPlanned* planned = (QE_BatchPlanned*)calloc(count ? count : 1, sizeof(QE_BatchPlanned));
   if (!planned) {

@@ -208,6 +208,7 @@ SIGNATURES = [
     ("gpu_hip_buffer_device_address", U64, [P]),
     ("gpu_hip_kernel_set_lds_bytes", None, [P, U32]),
     ("gpu_hip_max_lds_bytes", U32, []),
+    ("gpu_hip_kernel_max_lds_bytes", U32, [P]),
     ("gpu_hip_kernel_clear_arg", None, [P, U32]),
     ("gpu_hip_kernel_set_arg_u32", None, [P, U32, U32]),
     ("gpu_hip_profile_enable", None, [B32]),

@@ -197,6 +197,10 @@ void gpu_hip_kernel_clear_arg(GPU_Kernel* kernel, U32 index);
 /* set one 32-bit push word (the half of U64 slot word/2) */
 void gpu_hip_kernel_set_arg_u32(GPU_Kernel* kernel, U32 word, U32 value);
 U32 gpu_hip_max_lds_bytes(void);          /* LDS a workgroup may use */
+/* the dynamic LDS one kernel's workgroup may take: gpu_hip_max_lds_bytes
+ * less the kernel's own static __shared__ bytes; a dispatch asking for
+ * more fails with an error instead of reaching the queue */
+U32 gpu_hip_kernel_max_lds_bytes(GPU_Kernel* kernel);
 /* kernel-time profiling: hipEvents around every dispatch of `name` (a
  * trailing * matches a prefix)
  * (gpu.h:38-66 g_gpu_kernel_profile).  Returns dispatches and total ns. */
@@ -668,6 +672,23 @@ typedef struct QE_ScanQuery {
   U64* out_count;
   B32 ok;
 } QE_ScanQuery;
+
+/* Many joins over one fact table at once (no counterpart in the
+ * reference, which runs each join alone: query_exec.c:8442-8775).
+ * Identical joins run once (QE_BATCH_DEDUPE); joins that need the same
+ * fact side share it: on the partitioned path the fact rows in partition
+ * order (same dim size, fact key and argument columns; each join builds
+ * its own table and probes), on the dense path the per-key totals (same
+ * fact key and arguments; each join combines them with its own dim rows).
+ * Every query's `ok` says whether it ran; the call returns 0 only for bad
+ * arguments.  Results are what qe_aggregate_join returns for each. */
+typedef struct QE_JoinBatchTrace {
+  U32 queries;     /* joins in the batch */
+  U32 runs;        /* joins run (identical ones folded) */
+  U32 fact_passes; /* fact sides computed; the other runs reused one */
+  U64 wall_ns;
+} QE_JoinBatchTrace;
+B32 qe_aggregate_join_batch(QE_Table* fact, QE_JoinQuery* queries, U32 count, QE_JoinBatchTrace* trace);
 
 U64 qe_server_submit_join(QE_Server* server, QE_Table* fact, QE_JoinQuery* query);
 U64 qe_server_submit_scan(QE_Server* server, QE_Table* table, QE_ScanQuery* query);

@@ -76,3 +76,32 @@ def test_checked_reads_report_failure(events):
     assert b"injected" in lib.gpu_hip_last_error()
     # a read past the buffer is refused, not clipped
     assert lib.gpu_hip_buffer_read_checked(buf, None, 1 << 40) == 0
+
+
+def test_dispatch_beyond_the_lds_limit_is_refused():
+    """A kernel's own __shared__ bytes count against a workgroup's LDS beside
+    the dynamic bytes the host asks for.  The shared scan keeps 8 static
+    bytes (its query counters), so its dynamic stage may take that much less
+    than gpu_hip_max_lds_bytes; a dispatch asking for more fails with an
+    error on the host and never reaches the queue (there it would abort the
+    queue with an invalid-allocation error)."""
+    import ctypes as C
+
+    from computeql_amd.native import String8
+
+    lib = cq.load()
+    name = C.create_string_buffer(b"shared_scan_1024_1")
+    k = lib.gpu_hip_kernel_alloc(String8(C.cast(name, C.c_void_p), len(name.value)))
+    assert k
+    mx = lib.gpu_hip_max_lds_bytes()
+    lim = lib.gpu_hip_kernel_max_lds_bytes(k)
+    assert 0 < lim < mx
+    try:
+        lib.gpu_hip_kernel_set_lds_bytes(k, lim + 1)
+        b = lib.gpu_hip_batch_begin(0, 0)
+        lib.gpu_hip_batch_kernel_execute(b, k, 1024, 1024)
+        assert lib.gpu_hip_batch_end(b) == 0
+        assert b"exceed" in lib.gpu_hip_last_error()
+    finally:
+        lib.gpu_hip_kernel_set_lds_bytes(k, 0)
+        lib.gpu_hip_kernel_release(k)
