@@ -26,7 +26,49 @@
 
 #include "qe_internal.h"
 
+/* qe_aggregate_join_batch keeps the fact-side work one join computed for
+ * the next join of the batch that needs the same: the fact rows in
+ * partition order (fact, key column, argument columns, partition bits and
+ * block ranges alike) and the dense per-key totals (fact, key column and
+ * range, arguments, sums and min/max alike).  Outside a batch no work is
+ * kept. */
+typedef struct QE_JoinShare {
+  B32 part_valid;
+  const QE_Table* part_fact;
+  U32 part_key_col, part_pbits, part_arg_count, part_B;
+  U32 part_args[4];
+  GPU_Buffer* hist;
+  GPU_Buffer* off;
+  GPU_Buffer* totals;
+  GPU_Buffer* rows;
+  B32 dense_valid;
+  const QE_Table* dense_fact;
+  U32 dense_key_col, dense_arg_count, dense_range, dense_kmin;
+  U32 dense_args[4];
+  B32 dense_sum, dense_mm;
+  QE_DenseTotals dense_tot;
+  U32 fact_passes_shared; /* joins that reused a kept fact side */
+} QE_JoinShare;
 
+static _Thread_local QE_JoinShare* t_join_share;
+
+static void qe_join_share_drop_part(QE_JoinShare* sh)
+{
+  if (!sh->part_valid) return;
+  qe_scratch_release(sh->hist);
+  qe_scratch_release(sh->off);
+  qe_scratch_release(sh->totals);
+  qe_scratch_release(sh->rows);
+  sh->hist = sh->off = sh->totals = sh->rows = 0;
+  sh->part_valid = 0;
+}
+
+static void qe_join_share_drop_dense(QE_JoinShare* sh)
+{
+  if (!sh->dense_valid) return;
+  qe_dense_totals_free(&sh->dense_tot);
+  sh->dense_valid = 0;
+}
 
 typedef struct QE_GroupAcc {
   U32 value[QE_AGG_MAX_GROUP_COLS]; /* the dim group columns' codes (I32 biased), 0 past K */
@@ -135,8 +177,12 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
    * hash bits below the partition bits) has its top bit free for the
    * occupied mark */
   const U32 sbits = 13, S = 1u << sbits;
+  /* in a batch the tables are sized for the whole dim, so joins whose dim
+   * WHEREs keep different row counts still partition the fact alike */
+  QE_JoinShare* sh = t_join_share;
+  const U64 size_keys = sh && dim->row_count > pass_count ? dim->row_count : pass_count;
   U32 pbits = 1;
-  while (((U64)1 << pbits) * (S / 2) < pass_count && pbits < 12) pbits++;
+  while (((U64)1 << pbits) * (S / 2) < size_keys && pbits < 12) pbits++;
   const U64 P = 1ull << pbits;
   if (P * (S * 7 / 10) < pass_count) {
     qe_set_error("qe_aggregate_join: %llu dim keys exceed the partitioned tables", (unsigned long long)pass_count);
@@ -190,11 +236,17 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   GPU_Kernel* kp = qe_kernel(pname);
   GPU_Buffer* table = qe_scratch("jp_table", P * S * 8);
   GPU_Buffer* flags = qe_scratch("join_flags", 64);
-  GPU_Buffer* hist = qe_scratch("jp_hist", (U64)B * P * 4);
-  GPU_Buffer* off = qe_scratch("jp_off", (U64)B * P * 4);
-  GPU_Buffer* totals = qe_scratch("jp_totals", P * 4);
+  /* the fact rows in partition order that an earlier join of the batch
+   * left: the same fact, key, arguments, partitions and block ranges */
+  const B32 reuse = sh && sh->part_valid && sh->part_fact == fact && sh->part_key_col == fact_key_col &&
+                    sh->part_pbits == pbits && sh->part_B == B && sh->part_arg_count == arg_count &&
+                    memcmp(sh->part_args, arg_col, arg_count * sizeof(U32)) == 0;
+  if (sh && !reuse) qe_join_share_drop_part(sh);
+  GPU_Buffer* hist = reuse ? sh->hist : qe_scratch("jp_hist", (U64)B * P * 4);
+  GPU_Buffer* off = reuse ? sh->off : qe_scratch("jp_off", (U64)B * P * 4);
+  GPU_Buffer* totals = reuse ? sh->totals : qe_scratch("jp_totals", P * 4);
   /* one row past the partitions' rows takes the scatter's discarded stores */
-  GPU_Buffer* rowsbuf = qe_scratch("jp_rows", (rows + 1) * W * 4);
+  GPU_Buffer* rowsbuf = reuse ? sh->rows : qe_scratch("jp_rows", (rows + 1) * W * 4);
   B32 ok = kb && kh && ko && ks && kp && table && flags && hist && off && totals && rowsbuf;
   U64 cus = gpu_hip_compute_units();
   U32 split = (U32)((cus * qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU", 12) + P - 1) / P);
@@ -212,12 +264,12 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     const U64 copy_bytes = (R | 1) * 4 + (need_sum && ew ? (ew | 1) * 8 : 0) + (need_minmax && ew ? 2 * (ew | 1) * 4 : 0);
     const U64 max_copies = qe_settings_u64("QE_JOIN_PROBE_COPIES", 2);
     U64 fit = qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU_LDS", 3);
-    const U64 per_block = gpu_hip_max_lds_bytes() / (fit ? fit : 1);
+    const U64 per_block = gpu_hip_kernel_max_lds_bytes(kp) / (fit ? fit : 1);
     while (clog < 4 && (2ull << clog) <= max_copies && table_lds + (2ull << clog) * copy_bytes + 64 <= per_block) clog++;
     acc_lds = ((U64)1 << clog) * copy_bytes;
   }
   U64 probe_lds = table_lds + acc_lds;
-  if (ok && probe_lds > gpu_hip_max_lds_bytes()) {
+  if (ok && probe_lds > gpu_hip_kernel_max_lds_bytes(kp)) {
     qe_set_error("qe_aggregate_join: partition probe needs %llu bytes of LDS", (unsigned long long)probe_lds);
     ok = 0;
   }
@@ -303,9 +355,11 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
       gpu_batch_buffer_zero(b, acc, zero_bytes);
     }
     gpu_batch_kernel_execute(b, kb, (U32)build_threads, QE_WG);
-    gpu_batch_kernel_execute(b, kh, B * swg, swg);
-    gpu_batch_kernel_execute(b, ko, (U32)(P * QE_WG), QE_WG);
-    gpu_batch_kernel_execute(b, ks, B * swg, swg);
+    if (!reuse) {
+      gpu_batch_kernel_execute(b, kh, B * swg, swg);
+      gpu_batch_kernel_execute(b, ko, (U32)(P * QE_WG), QE_WG);
+      gpu_batch_kernel_execute(b, ks, B * swg, swg);
+    }
     gpu_batch_kernel_execute(b, kp, (U32)(P * split * pwg), pwg);
     gpu_batch_buffer_read(b, flags, fl, 8);
     gpu_batch_buffer_read(b, acc, host_acc, acc_bytes);
@@ -315,18 +369,41 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     gpu_hip_kernel_set_lds_bytes(kp, 0);
     if (!ok) qe_set_error("partitioned join: %s", gpu_hip_last_error());
     if (trace) {
-      trace->kernel_count = 5;
+      trace->kernel_count = reuse ? 2 : 5;
       trace->rows_scanned = rows;
-      trace->bytes_read = rows * 4 * (U64)slots.count;
+      trace->bytes_read = reuse ? rows * 4 * (U64)W : rows * 4 * (U64)slots.count;
       snprintf(trace->kernel, sizeof(trace->kernel), "%s", pname);
     }
   }
   qe_scratch_release(table);
   qe_scratch_release(flags);
-  qe_scratch_release(hist);
-  qe_scratch_release(off);
-  qe_scratch_release(totals);
-  qe_scratch_release(rowsbuf);
+  if (sh && ok) {
+    /* kept for the next join of the batch */
+    if (reuse) {
+      sh->fact_passes_shared++;
+    } else {
+      sh->part_valid = 1;
+      sh->part_fact = fact;
+      sh->part_key_col = fact_key_col;
+      sh->part_pbits = pbits;
+      sh->part_B = B;
+      sh->part_arg_count = arg_count;
+      memcpy(sh->part_args, arg_col, arg_count * sizeof(U32));
+      sh->hist = hist;
+      sh->off = off;
+      sh->totals = totals;
+      sh->rows = rowsbuf;
+    }
+  } else {
+    if (sh && reuse) {
+      sh->hist = sh->off = sh->totals = sh->rows = 0;
+      sh->part_valid = 0;
+    }
+    qe_scratch_release(hist);
+    qe_scratch_release(off);
+    qe_scratch_release(totals);
+    qe_scratch_release(rowsbuf);
+  }
   return ok;
 }
 
@@ -541,7 +618,32 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
     const U32 kmin_code = fk.min_code[0];
     QE_DenseTotals tot;
     memset(&tot, 0, sizeof(tot));
-    ok = qe_dense_run(fact, &fk, arg_col, arg_count, need_sum, need_minmax, 0, &all, &tot, trace);
+    /* per-key totals an earlier join of the batch left: same fact, key and
+     * range, arguments, sums and min/max */
+    QE_JoinShare* sh = t_join_share;
+    const B32 reuse = sh && sh->dense_valid && sh->dense_fact == fact && sh->dense_key_col == fact_key_col &&
+                      sh->dense_range == range && sh->dense_kmin == kmin_code && sh->dense_arg_count == arg_count &&
+                      sh->dense_sum == need_sum && sh->dense_mm == need_minmax &&
+                      memcmp(sh->dense_args, arg_col, arg_count * sizeof(U32)) == 0;
+    if (reuse) {
+      tot = sh->dense_tot;
+      sh->fact_passes_shared++;
+    } else {
+      if (sh) qe_join_share_drop_dense(sh);
+      ok = qe_dense_run(fact, &fk, arg_col, arg_count, need_sum, need_minmax, 0, &all, &tot, trace);
+      if (ok && sh) {
+        sh->dense_valid = 1;
+        sh->dense_fact = fact;
+        sh->dense_key_col = fact_key_col;
+        sh->dense_range = range;
+        sh->dense_kmin = kmin_code;
+        sh->dense_arg_count = arg_count;
+        memcpy(sh->dense_args, arg_col, arg_count * sizeof(U32));
+        sh->dense_sum = need_sum;
+        sh->dense_mm = need_minmax;
+        sh->dense_tot = tot;
+      }
+    }
     if (ok) {
       for (U64 v = 0; v < pass_count; v++) {
         U64 d = pass[v];
@@ -552,7 +654,7 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
         QE_GroupAcc* g = qe_group_get(&groups, gv);
         qe_group_add(g, funcs, expr_arg, expr_count, arg_type, tot.count[kid], tot.sum, tot.mn, tot.mx, range, kid);
       }
-      qe_dense_totals_free(&tot);
+      if (!sh) qe_dense_totals_free(&tot);
     }
   } else {
     path = group_mode ? QE_AggPath_JoinProbeGroup : QE_AggPath_JoinProbeRow;
@@ -735,4 +837,120 @@ B32 qe_aggregate_join(QE_Table* fact, U32 fact_key_col, QE_Table* dim, U32 dim_k
   free(pass);
   qe_trace_end(trace, t0, p0);
   return ok;
+}
+
+/* the same join: dim and keys, dim WHERE nodes, group columns, functions,
+ * arguments, capacity and whether it asks for its keys */
+static B32 qe_join_same(const QE_JoinQuery* a, const QE_JoinQuery* b)
+{
+  if (a->dim != b->dim || a->fact_key_col != b->fact_key_col || a->dim_key_col != b->dim_key_col ||
+      a->dim_where_count != b->dim_where_count || a->group_count != b->group_count ||
+      a->expr_count != b->expr_count || a->cap != b->cap || (a->out_keys != 0) != (b->out_keys != 0) ||
+      a->group_count > QE_AGG_MAX_GROUP_COLS)
+    return 0;
+  if (memcmp(a->group_cols, b->group_cols, a->group_count * sizeof(S32))) return 0;
+  if (a->dim_where_count &&
+      (!a->dim_where || !b->dim_where || memcmp(a->dim_where, b->dim_where, a->dim_where_count * sizeof(QE_IRNode))))
+    return 0;
+  return memcmp(a->funcs, b->funcs, a->expr_count * 4) == 0 && memcmp(a->arg_cols, b->arg_cols, a->expr_count * 4) == 0;
+}
+
+/* run order: joins that can share a fact side next to each other (same dim
+ * and keys, then the same argument columns), arrival order otherwise */
+static const QE_JoinQuery* t_join_order_q;
+
+static int qe_join_order_cmp(const void* pa, const void* pb)
+{
+  const U32 ia = *(const U32*)pa, ib = *(const U32*)pb;
+  const QE_JoinQuery* a = &t_join_order_q[ia];
+  const QE_JoinQuery* b = &t_join_order_q[ib];
+  if (a->dim != b->dim) return (uintptr_t)a->dim < (uintptr_t)b->dim ? -1 : 1;
+  if (a->fact_key_col != b->fact_key_col) return a->fact_key_col < b->fact_key_col ? -1 : 1;
+  if (a->dim_key_col != b->dim_key_col) return a->dim_key_col < b->dim_key_col ? -1 : 1;
+  U32 sa[QE_AGG_MAX_EXPRS], sb[QE_AGG_MAX_EXPRS], na = 0, nb = 0;
+  for (U32 e = 0; e < a->expr_count && e < QE_AGG_MAX_EXPRS; e++)
+    if (a->arg_cols[e] >= 0 && a->funcs[e] != QE_AGG_FUNC_COUNT) sa[na++] = (U32)a->arg_cols[e];
+  for (U32 e = 0; e < b->expr_count && e < QE_AGG_MAX_EXPRS; e++)
+    if (b->arg_cols[e] >= 0 && b->funcs[e] != QE_AGG_FUNC_COUNT) sb[nb++] = (U32)b->arg_cols[e];
+  if (na != nb) return na < nb ? -1 : 1;
+  for (U32 i = 0; i < na; i++)
+    if (sa[i] != sb[i]) return sa[i] < sb[i] ? -1 : 1;
+  return ia < ib ? -1 : (ia > ib ? 1 : 0);
+}
+
+static void qe_join_copy_result(const QE_JoinQuery* from, QE_JoinQuery* to)
+{
+  to->ok = from->ok;
+  if (!from->ok) return;
+  const U64 g = *from->out_group_count;
+  *to->out_group_count = g;
+  const U64 n = g < to->cap ? g : to->cap;
+  if (to->out_keys && from->out_keys) memcpy(to->out_keys, from->out_keys, n * to->group_count * sizeof(F64));
+  memcpy(to->out_cells, from->out_cells, n * to->expr_count * sizeof(U64));
+}
+
+B32 qe_aggregate_join_batch(QE_Table* fact, QE_JoinQuery* queries, U32 count, QE_JoinBatchTrace* trace)
+{
+  const U64 t0 = qe_now_ns();
+  if (trace) memset(trace, 0, sizeof(*trace));
+  if (!fact || (!queries && count)) {
+    qe_set_error("qe_aggregate_join_batch: null argument");
+    return 0;
+  }
+  for (U32 i = 0; i < count; i++)
+    if (!queries[i].dim || !queries[i].funcs || !queries[i].arg_cols || !queries[i].out_group_count ||
+        !queries[i].out_cells) {
+      qe_set_error("qe_aggregate_join_batch: query %u has a null argument", i);
+      return 0;
+    }
+  U32* rep_of = (U32*)malloc(sizeof(U32) * (count ? count : 1));
+  U32* order = (U32*)malloc(sizeof(U32) * (count ? count : 1));
+  if (!rep_of || !order) {
+    free(rep_of);
+    free(order);
+    qe_set_error("qe_aggregate_join_batch: out of memory");
+    return 0;
+  }
+  /* QE_BATCH_DEDUPE (default 1): identical joins run once */
+  const B32 dedupe = qe_settings_u64("QE_BATCH_DEDUPE", 1) != 0;
+  U32 runs = 0;
+  for (U32 i = 0; i < count; i++) {
+    rep_of[i] = ~0u;
+    for (U32 j = 0; dedupe && j < i; j++)
+      if (rep_of[j] == ~0u && qe_join_same(&queries[i], &queries[j])) {
+        rep_of[i] = j;
+        break;
+      }
+    if (rep_of[i] == ~0u) order[runs++] = i;
+  }
+  t_join_order_q = queries;
+  qsort(order, runs, sizeof(U32), qe_join_order_cmp);
+  t_join_order_q = 0;
+  QE_JoinShare share;
+  memset(&share, 0, sizeof(share));
+  t_join_share = &share;
+  char first_error[256] = "";
+  for (U32 r = 0; r < runs; r++) {
+    QE_JoinQuery* q = &queries[order[r]];
+    q->ok = qe_aggregate_join(fact, q->fact_key_col, q->dim, q->dim_key_col, q->dim_where, q->dim_where_count,
+                              q->group_cols, q->group_count, q->funcs, q->arg_cols, q->expr_count, q->cap,
+                              q->out_group_count, q->out_keys, q->out_cells, 0);
+    if (!q->ok && !first_error[0]) snprintf(first_error, sizeof(first_error), "%s", qe_last_error());
+  }
+  t_join_share = 0;
+  const U32 shared = share.fact_passes_shared;
+  qe_join_share_drop_part(&share);
+  qe_join_share_drop_dense(&share);
+  for (U32 i = 0; i < count; i++)
+    if (rep_of[i] != ~0u) qe_join_copy_result(&queries[rep_of[i]], &queries[i]);
+  free(rep_of);
+  free(order);
+  if (first_error[0]) qe_set_error("%s", first_error);
+  if (trace) {
+    trace->queries = count;
+    trace->runs = runs;
+    trace->fact_passes = runs - shared;
+    trace->wall_ns = qe_now_ns() - t0;
+  }
+  return 1;
 }

@@ -96,9 +96,11 @@ static void* qe_serve_worker(void* arg)
   QE_Server* s = (QE_Server*)arg;
   QE_Request** batch = (QE_Request**)calloc(s->max_batch, sizeof(QE_Request*));
   QE_BatchQuery* qs = (QE_BatchQuery*)calloc(s->max_batch, sizeof(QE_BatchQuery));
-  if (!batch || !qs) {
+  QE_JoinQuery* jq = (QE_JoinQuery*)calloc(s->max_batch, sizeof(QE_JoinQuery));
+  if (!batch || !qs || !jq) {
     free(batch);
     free(qs);
+    free(jq);
     return 0;
   }
   pthread_mutex_lock(&s->lock);
@@ -113,12 +115,13 @@ static void* qe_serve_worker(void* arg)
     U32 n = 0;
     QE_Request** link = &s->head;
     QE_Request* prev = 0;
-    while (*link && n < (kind == QE_REQUEST_AGGREGATE ? s->max_batch : 1)) {
+    while (*link && n < (kind == QE_REQUEST_SCAN ? 1 : s->max_batch)) {
       QE_Request* r = *link;
       B32 fits = s->inflight_bytes + r->footprint <= s->hbm_budget || (n == 0 && s->inflight_bytes == 0);
       if (r->kind != kind) {
-        /* joins and scans run one per launch; aggregates batch with the
-         * aggregates queued behind them */
+        /* scans run one per launch; aggregates batch with the aggregates
+         * queued behind them, joins with the joins on the same fact table
+         * (qe_aggregate_join_batch: identical ones once, a fact side shared) */
         if (n == 0) break;
         prev = r;
         link = &r->next;
@@ -147,12 +150,20 @@ static void* qe_serve_worker(void* arg)
     pthread_mutex_unlock(&s->lock);
 
     B32 ok;
-    if (kind == QE_REQUEST_JOIN) {
+    if (kind == QE_REQUEST_JOIN && n == 1) {
       QE_JoinQuery* j = batch[0]->join;
       memset(&qs[0], 0, sizeof(qs[0]));
       qs[0].ok = qe_aggregate_join(table, j->fact_key_col, j->dim, j->dim_key_col, j->dim_where, j->dim_where_count,
                                    j->group_cols, j->group_count, j->funcs, j->arg_cols, j->expr_count, j->cap,
                                    j->out_group_count, j->out_keys, j->out_cells, 0);
+    } else if (kind == QE_REQUEST_JOIN) {
+      for (U32 i = 0; i < n; i++) jq[i] = *batch[i]->join;
+      QE_JoinBatchTrace jt;
+      const B32 ran = qe_aggregate_join_batch(table, jq, n, &jt);
+      for (U32 i = 0; i < n; i++) {
+        memset(&qs[i], 0, sizeof(qs[i]));
+        qs[i].ok = ran && jq[i].ok;
+      }
     } else if (kind == QE_REQUEST_SCAN) {
       QE_ScanQuery* q = batch[0]->scan;
       memset(&qs[0], 0, sizeof(qs[0]));
@@ -219,6 +230,7 @@ static void* qe_serve_worker(void* arg)
   pthread_mutex_unlock(&s->lock);
   free(batch);
   free(qs);
+  free(jq);
   return 0;
 }
 

@@ -93,6 +93,13 @@ class QE_JoinQuery(C.Structure):
     ]
 
 
+class QE_JoinBatchTrace(C.Structure):
+    _fields_ = [("queries", U32), ("runs", U32), ("fact_passes", U32), ("wall_ns", U64)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class QE_ScanQuery(C.Structure):
     _fields_ = [
         ("where", C.c_void_p),
@@ -265,6 +272,7 @@ SIGNATURES = [
     ("qe_server_drive", B32, [P, P, P, U32, C.c_double, U64, U64, P]),
     ("qe_server_drive_mix", B32, [P, P, P, U32, C.c_double, U64, U64, P]),
     ("qe_server_submit_join", U64, [P, P, P]),
+    ("qe_aggregate_join_batch", B32, [P, P, U32, P]),
     ("qe_server_submit_scan", U64, [P, P, P]),
     ("qe_server_drive_requests", B32, [P, P, U32, C.c_double, U64, U64, P]),
     ("qe_bandwidth_read", B32, [P, U64, U32, P, P]),

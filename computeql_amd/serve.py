@@ -8,7 +8,8 @@
     stats = srv.drive(table, specs, rate_qps=20000, seconds=2.0)
     srv.close()
 
-Aggregates batch into shared scans; joins (qe_aggregate_join) and row-id
+Aggregates batch into shared scans; joins on one fact table batch through
+qe_aggregate_join_batch (identical joins once, a fact side shared); row-id
 scans (qe_scan_filter) run one per launch beside them.  drive() and
 drive_mix() run the native Poisson load driver (arrivals generated in C, so
 the offered rate is not bounded by Python) and return the server's stats
@@ -24,8 +25,8 @@ import numpy as np
 
 from . import ir
 from .engine import AggResult, AggSpec, QueryError, Table, _err, group_list, shape_keys
-from .native import (REQUEST_AGGREGATE, REQUEST_JOIN, REQUEST_SCAN, QE_BatchQuery, QE_JoinQuery, QE_ScanQuery,
-                     QE_ServeRequest, QE_ServerConfig, QE_ServerStats, load)
+from .native import (REQUEST_AGGREGATE, REQUEST_JOIN, REQUEST_SCAN, QE_BatchQuery, QE_JoinBatchTrace, QE_JoinQuery,
+                     QE_ScanQuery, QE_ServeRequest, QE_ServerConfig, QE_ServerStats, load)
 
 
 @dataclass
@@ -97,6 +98,39 @@ def _join_query(spec: JoinSpec, keep: list, cap: int | None = None):
     q.cap = cap
     keep += [fa, aa, nodes, spec.dim]
     return q, funcs, cap
+
+
+def aggregate_join_batch(fact: Table, specs, with_trace: bool = False):
+    """qe_aggregate_join_batch: many joins over one fact table, identical
+    ones run once and a fact side (partitioned rows, per-key totals)
+    shared by the joins that need the same.  A list of AggResult (and the
+    batch trace: queries, runs, fact_passes, wall_ns)."""
+    lib = load()
+    keep: list = []
+    arr = (QE_JoinQuery * max(len(specs), 1))()
+    outs = []
+    for i, sp in enumerate(specs):
+        q, funcs, cap = _join_query(sp, keep)
+        keys = np.empty(max(cap, 1) * max(len(group_list(sp.dim_group)), 1), dtype=np.float64)
+        cells = np.empty((max(cap, 1), len(funcs)), dtype=np.uint64)
+        count = C.c_uint64(0)
+        q.out_group_count = C.addressof(count)
+        q.out_keys = keys.ctypes.data
+        q.out_cells = cells.ctypes.data
+        arr[i] = q
+        outs.append((funcs, keys, cells, count, sp))
+    tr = QE_JoinBatchTrace()
+    if not lib.qe_aggregate_join_batch(fact.handle, arr, len(specs), C.byref(tr)):
+        raise _err("aggregate_join_batch")
+    res = []
+    for i, (funcs, keys, cells, count, sp) in enumerate(outs):
+        if not arr[i].ok:
+            raise QueryError(f"aggregate_join_batch: join {i} failed: {lib.qe_last_error().decode()}")
+        n = int(count.value)
+        if n > len(cells):
+            raise QueryError(f"aggregate_join_batch: join {i} has {n} groups, more than its cap {len(cells)}")
+        res.append(AggResult(funcs, shape_keys(sp.dim_group, keys, n), cells[:n].copy(), {}))
+    return (res, tr.as_dict()) if with_trace else res
 
 
 def _scan_query(spec: ScanSpec, keep: list):
