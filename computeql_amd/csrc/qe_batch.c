@@ -167,8 +167,11 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
   }
   U32 rec_total = 0;
   for (U32 i = 0; i < n; i++) rec_total += qe_batch_rec_words(&qs[i]);
-  const U32 hdr = 6;
-  U32 words = hdr + n + 3 * st.count + rec_total + W;
+  const U32 hdr = 8;
+  /* the chunk table: a tile's stage as 1 KB chunks */
+  U32 chunks = 0;
+  for (U32 u = 0; u < st.count; u++) chunks += (U32)((U64)QE_WG * items * t->columns[st.column[u]].width / 1024);
+  U32 words = hdr + n + 3 * st.count + rec_total + W + chunks;
   U32* p = (U32*)calloc(words, 4);
   if (!p) return 0;
   /* grouped queries walk only their passing rows (QE_BATCH_COMPACT, 1) */
@@ -180,6 +183,15 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
   p[3] = st.count;
   p[4] = (U32)stage_bytes;
   p[5] = WL;
+  p[6] = p[2] + W;
+  p[7] = chunks;
+  {
+    U32 f = 0;
+    for (U32 u = 0; u < st.count; u++) {
+      const U32 cu = (U32)((U64)QE_WG * items * t->columns[st.column[u]].width / 1024);
+      for (U32 k = 0; k < cu; k++) p[p[6] + f++] = u | (k << 16);
+    }
+  }
   U32 off = 0;
   for (U32 u = 0; u < st.count; u++) {
     U32 w = t->columns[st.column[u]].width;
@@ -337,6 +349,16 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
     qe_set_error("qe_aggregate_batch: out of memory");
     return 0;
   }
+  /* stage mode (push slot 6): 1 two buffers copied into LDS, 0 one, 2 one
+   * filled from registers that take the next tile's loads while a tile is
+   * evaluated (1024-thread blocks, a stage of at most 160 chunks of 1 KB;
+   * QE_BATCH_STAGE_REGS, default 1) */
+  U32 stage_mode = buffers == 2 ? 1u : 0u;
+  if (wg == 1024 && stage_bytes <= 160u * 1024u && (stage_bytes & 1023u) == 0 &&
+      qe_settings_u64("QE_BATCH_STAGE_REGS", 1) && dbl_mode != 1) {
+    stage_mode = 2;
+    buffers = 1;
+  }
   U64 lds = qe_batch_lds_n(WL, stage_bytes, buffers);
   U64 rows = t->row_count;
   U64 tile = (U64)wg * items;
@@ -367,7 +389,7 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
     gpu_kernel_set_arg_buffer(k, 2, cb);
     gpu_kernel_set_arg_u64(k, 0, rows);
     gpu_kernel_set_arg_u64(k, 5, 0);
-    gpu_kernel_set_arg_u64(k, 6, buffers == 2); /* push slot 6: double-buffered stage */
+    gpu_kernel_set_arg_u64(k, 6, stage_mode); /* push slot 6: how the stage is filled */
     gpu_hip_kernel_set_lds_bytes(k, (U32)lds);
     gpu_kernel_set_arg_buffer(kr, 0, pb);
     gpu_kernel_set_arg_buffer(kr, 1, part);
@@ -381,7 +403,7 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
     GPU_Batch* b;
     if (qe_settings_u64("QE_BATCH_GRAPH", 1)) {
       char key[160];
-      snprintf(key, sizeof(key), "batch:%s:%p:%u:%u:%u:%llu:%llu", kname, (void*)t, words, W, buffers,
+      snprintf(key, sizeof(key), "batch:%s:%p:%u:%u:%u:%llu:%llu", kname, (void*)t, words, W, stage_mode,
                (unsigned long long)blocks, (unsigned long long)lds);
       b = gpu_hip_batch_begin_graph(key);
     } else {

@@ -181,7 +181,15 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
    * WHEREs keep different row counts still partition the fact alike */
   QE_JoinShare* sh = t_join_share;
   const U64 size_keys = sh && dim->row_count > pass_count ? dim->row_count : pass_count;
-  U32 pbits = 1;
+  /* the packed table (group codes below 8192 beside the key bits in one
+   * word, QE_JOIN_PROBE_PACKED, default 1) for the group-mode shapes that
+   * have a packed probe; its key field needs at least 64 partitions */
+  const B32 specialized = qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1) && qe_settings_u64("QE_GENERATE_JOIN", 1);
+  const U32 pwg = qe_settings_u64("QE_JOIN_PROBE_WG", 512) == 256 ? 256 : 512;
+  const B32 pk_shape = arg_count == 0 || (arg_count == 1 && (need_sum || need_minmax)) || (arg_count == 2 && need_sum);
+  const B32 packed = group_mode && specialized && pwg == 512 && pk_shape && R <= 8192 &&
+                     qe_settings_u64("QE_JOIN_PROBE_PACKED", 1) != 0;
+  U32 pbits = packed ? 6 : 1;
   while (((U64)1 << pbits) * (S / 2) < size_keys && pbits < 12) pbits++;
   const U64 P = 1ull << pbits;
   if (P * (S * 7 / 10) < pass_count) {
@@ -210,10 +218,9 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   /* the histogram runs with the scatter's block count and block size */
   GPU_Kernel* kh = qe_kernel(wc ? "join_part_hist_512" : "join_part_hist");
   GPU_Kernel* ko = qe_kernel("join_part_offsets");
-  /* the instances specialised to the query's shape stand in for the
-   * reference's generated join kernel (QE_GENERATE_JOIN, query_exec.c:8235);
-   * 0 takes the generic forms */
-  const B32 specialized = qe_settings_u64("QE_JOIN_PROBE_SPECIALIZED", 1) && qe_settings_u64("QE_GENERATE_JOIN", 1);
+  /* the instances specialised to the query's shape (`specialized` above)
+   * stand in for the reference's generated join kernel (QE_GENERATE_JOIN,
+   * query_exec.c:8235); 0 takes the generic forms */
   char sname[40] = "join_part_scatter";
   if (wc) snprintf(sname, sizeof(sname), "join_part_scatter_wc_%u", arg_count);
   else if (arg_count <= 2 && specialized)
@@ -231,7 +238,11 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   }
   if (!specialized) snprintf(pname, sizeof(pname), "join_part_probe");
   /* QE_JOIN_PROBE_WG: 512 (default) or 256 threads a probe block */
-  const U32 pwg = qe_settings_u64("QE_JOIN_PROBE_WG", 512) == 256 ? 256 : 512;
+  if (packed) {
+    const char* shape = arg_count == 0 ? "0" : (arg_count == 1 ? (need_sum ? (need_minmax ? "1sm" : "1s") : "1m")
+                                                               : (need_minmax ? "2sm" : "2s"));
+    snprintf(pname, sizeof(pname), "join_part_probe_pk_%s", shape);
+  }
   if (pwg == 512) snprintf(pname + strlen(pname), sizeof(pname) - strlen(pname), "_512");
   GPU_Kernel* kp = qe_kernel(pname);
   GPU_Buffer* table = qe_scratch("jp_table", P * S * 8);
@@ -249,14 +260,14 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   GPU_Buffer* rowsbuf = reuse ? sh->rows : qe_scratch("jp_rows", (rows + 1) * W * 4);
   B32 ok = kb && kh && ko && ks && kp && table && flags && hist && off && totals && rowsbuf;
   U64 cus = gpu_hip_compute_units();
-  U32 split = (U32)((cus * qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU", 12) + P - 1) / P);
+  U32 split = (U32)((cus * qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU", 24) + P - 1) / P);
   if (split < 1) split = 1;
   /* key words u32[S], then payloads: u16 group codes or u32 dim rows; in
    * group mode 2^clog copies of the group tables (cnt u32[R], sums u64[E][R],
    * min/max u32[E][R], each with an odd stride).  The copies are as many as
    * QE_JOIN_PROBE_COPIES (default 2) allows while QE_JOIN_PROBE_BLOCKS_PER_CU_LDS
    * (default 3) probe blocks still fit a CU's LDS, at least one. */
-  const U64 table_lds = (U64)S * 4 + (U64)S * (group_mode ? 2 : 4);
+  const U64 table_lds = (U64)S * 4 + (packed ? 0 : (U64)S * (group_mode ? 2 : 4));
   U32 clog = 0;
   U64 acc_lds = 0;
   if (group_mode) {
@@ -290,7 +301,7 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     if (list) gpu_kernel_set_arg_buffer(kb, 4, list);
     else gpu_hip_kernel_clear_arg(kb, 4);
     gpu_kernel_set_arg_u64(kb, 0, pass_count);
-    qe_kernel_set_word(kb, QK_W_PMODE, group_mode ? (group_buf ? 1u : 2u) : 0u);
+    qe_kernel_set_word(kb, QK_W_PMODE, (group_mode ? (group_buf ? 1u : 2u) : 0u) | (packed ? 256u : 0u));
     qe_kernel_set_word(kb, QK_W_GMIN, gmin);
     qe_kernel_set_word(kb, QK_W_GTYPE, group_i32 ? CQL_ColumnType_I32 : CQL_ColumnType_U32);
     qe_kernel_set_word(kb, QK_W_PBITS, pbits);
@@ -342,6 +353,15 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
     qe_kernel_set_word(kp, QK_W_TYPES, type_bits);
     qe_kernel_set_word(kp, QK_W_RANGE, (U32)R);
     qe_kernel_set_word(kp, QK_W_KEY, kflags << 16);
+    {
+      /* the packed probe's count-and-sum atomic: the argument's largest
+       * value (zone maps), all of U32 when they are not built */
+      U32 amax = 0xffffffffu;
+      if (arg_count == 1 && arg_type[0] == CQL_ColumnType_U32 && fact->columns[arg_col[0]].zonemap_valid &&
+          fact->columns[arg_col[0]].max_value >= 0.0 && fact->columns[arg_col[0]].max_value < 4294967295.0)
+        amax = (U32)fact->columns[arg_col[0]].max_value;
+      qe_kernel_set_word(kp, QK_W_GMIN, amax);
+    }
     gpu_hip_kernel_set_lds_bytes(kp, (U32)probe_lds);
     U64 build_threads = pass_count < 256 ? 256 : pass_count;
     if (build_threads > cus * 4 * 256) build_threads = cus * 4 * 256;

@@ -1156,10 +1156,18 @@ __device__ __forceinline__ qk_u32 qk_fmix32(qk_u32 h)
  * right after.  The slots form aligned 4-slot buckets; a key goes to the
  * first bucket from its home slot's bucket on with an empty slot, so a
  * lookup that meets a bucket with an empty slot can stop there. */
+/* Packed form (PMODE bit 8, group payloads < 8192, PBITS >= 6): one u32
+ * per slot, key field << 13 | group code, the key field 1 << 18 (occupied)
+ * | d << RB | the low RB = 32 - PBITS - (SBITS - 2) bits of h, d the
+ * bucket distance from the home bucket (< 2^(18 - RB)); h's partition and
+ * home bucket bits are the slot's position less d.  A lookup subtracts the
+ * expected key field << 13 from each slot word: the match alone leaves a
+ * value below 8192, its group code. */
 extern "C" __global__ void LB k_join_part_build(QK_Args a)
 {
   const qk_u64 n = qk_push(a, 0);
-  const qk_u32 pmode = a.words[QK_W_PMODE], gmin = a.words[QK_W_GMIN], gtype = a.words[QK_W_GTYPE];
+  const qk_u32 pmode = a.words[QK_W_PMODE] & 0xffu, gmin = a.words[QK_W_GMIN], gtype = a.words[QK_W_GTYPE];
+  const bool packed = (a.words[QK_W_PMODE] >> 8) & 1u;
   const qk_u32 pbits = a.words[QK_W_PBITS], sbits = a.words[QK_W_SBITS];
   const qk_u32 smask = (1u << sbits) - 1u;
   const qk_u32 low = 0xffffffffu >> pbits;
@@ -1176,6 +1184,28 @@ extern "C" __global__ void LB k_join_part_build(QK_Args a)
     qk_u32 h = qk_fmix32(key);
     qk_u32 kw = (h & low) | 0x80000000u;
     qk_u64 tbase = (qk_u64)(h >> (32 - pbits)) << sbits;
+    if (packed) {
+      const qk_u32 rb = 32u - pbits - (sbits - 2u);
+      const qk_u32 rem = h & ((1u << rb) - 1u);
+      const qk_u32 nbk = 1u << (sbits - 2u), hb = (h >> rb) & (nbk - 1u);
+      const qk_u32 maxd = 1u << (18u - rb);
+      for (qk_u32 t = 0;; t++) {
+        const qk_u32 d = t >> 2;
+        if (d >= maxd || d >= nbk) {
+          atomicOr(&flags[1], 1u);
+          break;
+        }
+        const qk_u32 slot = (((hb + d) & (nbk - 1u)) << 2) + ((rem + t) & 3u);
+        const qk_u32 kf = 0x40000u | (d << rb) | rem;
+        qk_u32 prev = atomicCAS(&tkeys[tbase + slot], 0u, (kf << 13) | (pay & 0x1fffu));
+        if (prev == 0u) break;
+        if ((prev >> 13) == kf) {
+          atomicOr(&flags[0], 1u);
+          break;
+        }
+      }
+      continue;
+    }
     /* probe i takes bucket i / 4 after the home slot's bucket, and in it
      * the slot (home + i) mod 4: the home bucket's slots are all tried
      * before any later bucket's, starting at the home slot so that keys of
@@ -1744,7 +1774,12 @@ __device__ __forceinline__ qk_u32 qk_bucket_match(const QkBucket& k, qk_u32 v)
   return m;
 }
 
-template <int NA, bool SUMS, bool MM, bool ROW, int WG, int ROWS, int G>
+/* PK: the table in join_part_build's packed form (group mode): a bucket
+ * read gives keys and group codes at once, and a row's home bucket and the
+ * next are matched by subtraction and unsigned minimum (the matching slot
+ * alone leaves its code, below 8192); a block whose rows' count and sum of
+ * one U32 argument fit 24 + 40 bits adds both with one 64-bit LDS atomic. */
+template <int NA, bool SUMS, bool MM, bool ROW, int WG, int ROWS, int G, bool PK = false>
 __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
 {
   /* 16-byte aligned: the bucket reads are 16-byte LDS loads */
@@ -1768,14 +1803,20 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
   /* LDS: keys u32[S] | pays (u16 group codes or u32 dim rows)[S] | then,
    * in group mode, C copies each of sums u64[E][R] | cnt u32[R] |
    * min u32[E][R] | max u32[E][R], every copy's stride odd */
+  static_assert(!PK || !ROW, "the packed table holds group codes");
   qk_u32* tk = (qk_u32*)lds_u64;
   unsigned short* tp16 = (unsigned short*)(tk + S);
   qk_u32* tp32 = tk + S;
+  /* packed: low bits of h below the bucket index, and the key-field step
+   * of one bucket of distance */
+  const qk_u32 rb = 32u - pbits - (sbits - 2u);
+  const qk_u32 rmask = (1u << rb) - 1u, nbk = S >> 2;
+  const qk_u32 dstep = 1u << (rb + 13u), maxd = 1u << (18u - rb);
   const qk_u32 C = per_row ? 1u : (1u << ((a.words[QK_W_ARGS] >> 16) & 0xfu));
   const qk_u32 sw = (!per_row && do_sum) ? nargs * range : 0u;
   const qk_u32 mw = (!per_row && do_mm) ? nargs * range : 0u;
   const qk_u32 sstr = sw ? (sw | 1u) : 0u, cstr = per_row ? 0u : (range | 1u), mstr = mw ? (mw | 1u) : 0u;
-  qk_u64* l_sum = (qk_u64*)(per_row ? (void*)(tp32 + S) : (void*)(tp16 + S));
+  qk_u64* l_sum = (qk_u64*)(PK ? (void*)(tk + S) : (per_row ? (void*)(tp32 + S) : (void*)(tp16 + S)));
   qk_u32* l_cnt = (qk_u32*)(l_sum + (qk_u64)C * sstr);
   qk_u32* l_min = l_cnt + C * cstr;
   qk_u32* l_max = l_min + C * mstr;
@@ -1789,6 +1830,7 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
     const qk_u32x4* gp = qk_ptr<qk_u32x4>(a, 2) + ((((qk_u64)P + part) << sbits) >> 2);
     for (qk_u32 i = threadIdx.x; i < S / 4; i += WG) {
       ((qk_u32x4*)tk)[i] = gk[i];
+      if (PK) continue;
       qk_u32x4 v = gp[i];
       if (per_row) {
         ((qk_u32x4*)tp32)[i] = v;
@@ -1819,17 +1861,125 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
   const qk_u64 chunk = (plen + split - 1) / split;
   const qk_u64 rs = pstart + chunk * piece < pend ? pstart + chunk * piece : pend;
   const qk_u64 re = rs + chunk < pend ? rs + chunk : pend;
+  /* one atomic for count and sum: a U32 argument whose block total stays
+   * below 2^40 (its zone-map maximum, word GMIN, times the block's rows)
+   * and a count below 2^24 */
+  const bool pk_cs = PK && NA == 1 && SUMS && !MM && (types & 0xffu) == QK_T_U32 && re - rs < (1ull << 23) &&
+                     (re - rs) * (qk_u64)a.words[QK_W_GMIN] < (1ull << 40);
   const qk_u32* rowsbuf = qk_ptr<qk_u32>(a, 3);
   qk_u64* g_cnt = qk_wptr<qk_u64>(a, 1);
   qk_u64* g_sum = g_cnt + range;
   qk_u32* g_min = (qk_u32*)(g_sum + (qk_u64)nargs * range);
   qk_u32* g_max = g_min + (qk_u64)nargs * range;
+  const qk_u64 pstep = (qk_u64)WG * ROWS;
+  if constexpr (PK && NA <= 1) {
+    /* two register sets of rows alternate: one set's loads are in flight
+     * while the other is probed.  A step's rows are r + j * WG + tid; in
+     * the last step the rows at or past re read row r instead and are
+     * masked, `lim` being the step's rows (all ones for a full step). */
+    constexpr int NX = NA > 0 ? NA : 1;
+    qk_u32 k0[ROWS], k1[ROWS], x0[ROWS][NX], x1[ROWS][NX];
+    auto load_set = [&](qk_u32(&kk)[ROWS], qk_u32(&xx)[ROWS][NX], qk_u64 r) {
+      const qk_u64 left = re - r;
+      const qk_u32 lim = left < pstep ? (qk_u32)left : 0xffffffffu;
+      const qk_u32* base = rowsbuf + (r + threadIdx.x) * W;
+#pragma unroll
+      for (int j = 0; j < ROWS; j++) {
+        const qk_u32 o = (qk_u32)j * WG;
+        const qk_u32* p = (lim == 0xffffffffu || o + threadIdx.x < lim) ? base + (qk_u64)o * W
+                                                                         : rowsbuf + r * W;
+        if (W == 2) {
+          const qk_u64 v = qk_ld((const qk_u64*)p);
+          kk[j] = (qk_u32)v;
+          xx[j][0] = (qk_u32)(v >> 32);
+        } else {
+          kk[j] = qk_ld(p);
+#pragma unroll
+          for (int c = 0; c < NA; c++) xx[j][c] = qk_ld(p + 1 + c);
+        }
+      }
+    };
+    auto probe_set = [&](const qk_u32(&kk)[ROWS], const qk_u32(&xx)[ROWS][NX], qk_u64 r) {
+      const qk_u64 left = re - r;
+      const qk_u32 lim = left < pstep ? (qk_u32)left : 0xffffffffu;
+#pragma unroll
+      for (int sb = 0; sb < ROWS; sb += G) {
+        qk_u32 e0[G], home[G], d[G];
+        QkBucket bk0[G], bk1[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          const qk_u32 h = kk[sb + j];
+          e0[j] = (0x40000u | (h & rmask)) << 13;
+          home[j] = ((h >> rb) & (nbk - 1u)) << 2;
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          bk0[j] = qk_bucket_read(tk, home[j]);
+          bk1[j] = qk_bucket_read(tk, (home[j] + QK_BUCKET) & smask);
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          const qk_u32 a0 = e0[j], a1 = a0 + dstep;
+          qk_u32 m = min(min(bk0[j].w[0] - a0, bk0[j].w[1] - a0), min(bk0[j].w[2] - a0, bk0[j].w[3] - a0));
+          m = min(m, min(min(bk1[j].w[0] - a1, bk1[j].w[1] - a1), min(bk1[j].w[2] - a1, bk1[j].w[3] - a1)));
+          if (m >= 8192u) {
+            /* not in the first two buckets: on while none of the buckets
+             * read had an empty slot (rare) */
+            qk_u32 z = min(min(min(bk0[j].w[0], bk0[j].w[1]), min(bk0[j].w[2], bk0[j].w[3])),
+                           min(min(bk1[j].w[0], bk1[j].w[1]), min(bk1[j].w[2], bk1[j].w[3])));
+            qk_u32 b = (home[j] + QK_BUCKET) & smask, e = a1;
+            for (qk_u32 n = 2; z != 0u && m >= 8192u && n < maxd && n < nbk; n++) {
+              b = (b + QK_BUCKET) & smask;
+              e += dstep;
+              const QkBucket nb = qk_bucket_read(tk, b);
+              m = min(m, min(min(nb.w[0] - e, nb.w[1] - e), min(nb.w[2] - e, nb.w[3] - e)));
+              z = min(min(nb.w[0], nb.w[1]), min(nb.w[2], nb.w[3]));
+            }
+          }
+          /* a miss leaves m >= 8192 >= range; rows past the step's end
+           * are dropped */
+          d[j] = (m < range && (qk_u32)(sb + j) * WG + threadIdx.x < lim) ? m : 0xffffffffu;
+        }
+        if (pk_cs) {
+#pragma unroll
+          for (int j = 0; j < G; j++)
+            if (d[j] != 0xffffffffu) atomicAdd(&my_sum[d[j]], (1ULL << 40) | (qk_u64)xx[sb + j][0]);
+          continue;
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++)
+          if (d[j] != 0xffffffffu) atomicAdd(&my_cnt[d[j]], 1u);
+#pragma unroll
+        for (int c = 0; c < NA; c++) {
+          const qk_u32 at = (types >> (8 * c)) & 0xffu;
+#pragma unroll
+          for (int j = 0; j < G; j++) {
+            if (d[j] == 0xffffffffu) continue;
+            const qk_u32 v = xx[sb + j][c], g = d[j];
+            if (SUMS) atomicAdd(&my_sum[c * range + g], at == QK_T_I32 ? (qk_u64)(qk_s64)(qk_s32)v : (qk_u64)v);
+            if (MM) {
+              const qk_u32 code = qk_code32(v, at);
+              atomicMin(&my_min[c * range + g], code);
+              atomicMax(&my_max[c * range + g], code);
+            }
+          }
+        }
+      }
+    };
+    if (rs < re) load_set(k0, x0, rs);
+    for (qk_u64 r = rs; r < re; r += 2 * pstep) {
+      if (r + pstep < re) load_set(k1, x1, r + pstep);
+      probe_set(k0, x0, r);
+      if (r + pstep >= re) break;
+      if (r + 2 * pstep < re) load_set(k0, x0, r + 2 * pstep);
+      probe_set(k1, x1, r + pstep);
+    }
+  } else {
   /* ROWS rows per thread per step.  A row's key and first arg
    * load together, one whole step ahead of their use, so a step's loads are
    * in flight while the step before probes the LDS table; the rows are then
    * probed G at a time to bound the registers held per row.  Rows past
    * re clamp to rs (in bounds: the loop runs only when rs < re). */
-  const qk_u64 pstep = (qk_u64)WG * ROWS;
   qk_u32 nkey[ROWS], narg[ROWS];
   auto load_rows = [&](qk_u64 r) {
 #pragma unroll
@@ -1862,6 +2012,43 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
       qk_u32 kw[G], home[G], pay[G];
       QkBucket bk0[G], bk1[G];
       bool hit[G];
+      if constexpr (PK) {
+        /* kw: the expected slot word less the code for distance 0; home:
+         * the home bucket's first slot */
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          const qk_u32 h = ckey[sb + j];
+          kw[j] = (0x40000u | (h & rmask)) << 13;
+          home[j] = ((h >> rb) & (nbk - 1u)) << 2;
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          bk0[j] = qk_bucket_read(tk, home[j]);
+          bk1[j] = qk_bucket_read(tk, (home[j] + QK_BUCKET) & smask);
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          const qk_u32 e0 = kw[j], e1 = e0 + dstep;
+          qk_u32 d = min(min(bk0[j].w[0] - e0, bk0[j].w[1] - e0), min(bk0[j].w[2] - e0, bk0[j].w[3] - e0));
+          d = min(d, min(min(bk1[j].w[0] - e1, bk1[j].w[1] - e1), min(bk1[j].w[2] - e1, bk1[j].w[3] - e1)));
+          if (d >= 8192u) {
+            /* not in the first two buckets: on while none of the buckets
+             * read had an empty slot (rare) */
+            qk_u32 z = min(min(min(bk0[j].w[0], bk0[j].w[1]), min(bk0[j].w[2], bk0[j].w[3])),
+                           min(min(bk1[j].w[0], bk1[j].w[1]), min(bk1[j].w[2], bk1[j].w[3])));
+            qk_u32 b = (home[j] + QK_BUCKET) & smask, e = e1;
+            for (qk_u32 n = 2; z != 0u && d >= 8192u && n < maxd && n < nbk; n++) {
+              b = (b + QK_BUCKET) & smask;
+              e += dstep;
+              const QkBucket nb = qk_bucket_read(tk, b);
+              d = min(d, min(min(nb.w[0] - e, nb.w[1] - e), min(nb.w[2] - e, nb.w[3] - e)));
+              z = min(min(nb.w[0], nb.w[1]), min(nb.w[2], nb.w[3]));
+            }
+          }
+          hit[j] = d < range && (full || (r + (qk_u64)(sb + j) * WG + threadIdx.x) < re);
+          pay[j] = d;
+        }
+      } else {
 #pragma unroll
       for (int j = 0; j < G; j++) {
         const qk_u32 h = ckey[sb + j];
@@ -1893,6 +2080,14 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
         qk_u32 pj = m ? (per_row ? tp32[idx] : (qk_u32)tp16[idx]) : 0u;
         hit[j] = m && (full || (r + (qk_u64)(sb + j) * WG + threadIdx.x) < re) && pj < range;
         pay[j] = pj;
+      }
+      }
+      if (pk_cs) {
+        /* count << 40 | the argument, in one atomic */
+#pragma unroll
+        for (int j = 0; j < G; j++)
+          if (hit[j]) atomicAdd(&my_sum[pay[j]], (1ULL << 40) | (qk_u64)carg[sb + j]);
+        continue;
       }
 #pragma unroll
       for (int j = 0; j < G; j++) {
@@ -1931,8 +2126,23 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
       }
     }
   }
+  }
   if (per_row) return;
   __syncthreads();
+  if (pk_cs) {
+    for (qk_u32 i = threadIdx.x; i < range; i += WG) {
+      qk_u64 cnt = 0, sum = 0;
+      for (qk_u32 k = 0; k < C; k++) {
+        const qk_u64 x = l_sum[k * sstr + i];
+        cnt += x >> 40;
+        sum += x & ((1ULL << 40) - 1ULL);
+      }
+      if (!cnt) continue;
+      atomicAdd(&g_cnt[i], cnt);
+      atomicAdd(&g_sum[i], sum);
+    }
+    return;
+  }
   for (qk_u32 i = threadIdx.x; i < range; i += WG) {
     qk_u32 cnt = 0;
     for (qk_u32 k = 0; k < C; k++) cnt += l_cnt[k * cstr + i];
@@ -1975,6 +2185,19 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
   {                                                                                                                \
     qk_part_probe<na, sums, mm, row, 512, 8, 2>(a);                                                                   \
   }
+/* the packed-table forms of the group-mode shapes, 512-thread blocks */
+#define QK_PROBE_PK_INSTANCE(name, na, sums, mm)                                                                     \
+  extern "C" __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6)))                        \
+      k_join_part_probe_pk##name##_512(QK_Args a)                                                                  \
+  {                                                                                                                \
+    qk_part_probe<na, sums, mm, false, 512, 8, 2, true>(a);                                                           \
+  }
+QK_PROBE_PK_INSTANCE(_0, 0, false, false)
+QK_PROBE_PK_INSTANCE(_1s, 1, true, false)
+QK_PROBE_PK_INSTANCE(_1m, 1, false, true)
+QK_PROBE_PK_INSTANCE(_1sm, 1, true, true)
+QK_PROBE_PK_INSTANCE(_2s, 2, true, false)
+QK_PROBE_PK_INSTANCE(_2sm, 2, true, true)
 QK_PROBE_INSTANCE(, -1, false, false, false)
 QK_PROBE_INSTANCE(_0, 0, false, false, false)
 QK_PROBE_INSTANCE(_1s, 1, true, false, false)
@@ -2427,8 +2650,11 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /*   [0] queries Q  [1] accumulator words W  [2] offset of the op table */
 /*   [3] staged columns U  [4] bytes of one stage buffer                */
 /*   [5] LDS accumulator words WL (W with every replica)                */
-/*   [6 + q] offset of query q's record                                 */
-/*   [6 + Q + 3u] staged column u: table column, width in bytes (4 or   */
+/*   [6] offset of the chunk table, [7] its chunks C: a tile's stage as  */
+/*   1 KB chunks, entry f = staged column u | chunk k << 16 (bytes       */
+/*   k * 1024 on of the column's rows in the tile)                      */
+/*   [8 + q] offset of query q's record                                 */
+/*   [8 + Q + 3u] staged column u: table column, width in bytes (4 or   */
 /*   8), byte offset of its rows in a stage buffer (16-byte aligned)    */
 /* query record: [0] leaves L (<= 8) | terms T << 8 (0 = the AND of   */
 /*   the leaves, else the rows of any term whose leaves all pass),      */
@@ -2466,7 +2692,7 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /* stage buffer 1 (when double-buffered, push slot 6).                  */
 /* ------------------------------------------------------------------ */
 #define QK_SS_MAX_ARGS 2
-#define QK_SS_HDR 6
+#define QK_SS_HDR 8
 #define QK_SS_REC 14
 
 __device__ __forceinline__ bool qk_type_is_wide(qk_u32 t) { return t == QK_T_F64 || t == QK_T_I64 || t == QK_T_U64; }
@@ -2591,53 +2817,56 @@ __device__ __forceinline__ double qk_ss_f64(const unsigned char* p, qk_u32 t, qk
   return t == QK_T_U32 ? (double)x : (t == QK_T_I32 ? (double)(qk_s32)x : (double)__uint_as_float(x));
 }
 
-/* the lane's RPL rows (i * 64 + lane) of a staged column as f64, with the
- * column type dispatched once for all of them */
-template <int RPL, typename T>
-__device__ __forceinline__ void qk_ss_rows_t(const unsigned char* p, qk_u32 lane, double* v)
+/* N of the lane's rows, i0 + i for i < N (row (i0 + i) * 64 + lane of the
+ * tile), of a staged column as f64, with the column type dispatched once
+ * for all of them.  Callers walk a lane's rows in groups of N <= 8 with i0
+ * an unrolled constant, so at most 8 values are live at a time. */
+template <int N, typename T>
+__device__ __forceinline__ void qk_ss_rows_t(const unsigned char* p, qk_u32 lane, int i0, double* v)
 {
   const T* sv = (const T*)p;
 #pragma unroll
-  for (int i = 0; i < RPL; i++) v[i] = (double)sv[i * 64 + lane];
+  for (int i = 0; i < N; i++) v[i] = (double)sv[(i0 + i) * 64 + lane];
 }
 
-template <int RPL>
-__device__ __forceinline__ void qk_ss_rows(const unsigned char* p, qk_u32 t, qk_u32 lane, double* v)
+template <int N>
+__device__ __forceinline__ void qk_ss_rows(const unsigned char* p, qk_u32 t, qk_u32 lane, int i0, double* v)
 {
   switch (t) {
-    case QK_T_U32: qk_ss_rows_t<RPL, qk_u32>(p, lane, v); break;
-    case QK_T_I32: qk_ss_rows_t<RPL, qk_s32>(p, lane, v); break;
-    case QK_T_F32: qk_ss_rows_t<RPL, float>(p, lane, v); break;
-    case QK_T_F64: qk_ss_rows_t<RPL, double>(p, lane, v); break;
-    case QK_T_I64: qk_ss_rows_t<RPL, qk_s64>(p, lane, v); break;
-    default: qk_ss_rows_t<RPL, qk_u64>(p, lane, v); break;
+    case QK_T_U32: qk_ss_rows_t<N, qk_u32>(p, lane, i0, v); break;
+    case QK_T_I32: qk_ss_rows_t<N, qk_s32>(p, lane, i0, v); break;
+    case QK_T_F32: qk_ss_rows_t<N, float>(p, lane, i0, v); break;
+    case QK_T_F64: qk_ss_rows_t<N, double>(p, lane, i0, v); break;
+    case QK_T_I64: qk_ss_rows_t<N, qk_s64>(p, lane, i0, v); break;
+    default: qk_ss_rows_t<N, qk_u64>(p, lane, i0, v); break;
   }
 }
 
-/* clear the bits of rows whose value fails `x op tv` (op dispatched once) */
-template <int RPL>
-__device__ __forceinline__ qk_u32 qk_ss_cmp(const double* x, qk_u32 op, double tv, qk_u32 m)
+/* clear the bits (i0 + i) of rows whose value fails `x[i] op tv` (op
+ * dispatched once) */
+template <int N>
+__device__ __forceinline__ qk_u32 qk_ss_cmp(const double* x, qk_u32 op, double tv, qk_u32 m, int i0)
 {
   switch (op) {
     case 0:
 #pragma unroll
-      for (int i = 0; i < RPL; i++) m &= (x[i] == tv) ? ~0u : ~(1u << i);
+      for (int i = 0; i < N; i++) m &= (x[i] == tv) ? ~0u : ~(1u << (i0 + i));
       break;
     case 1:
 #pragma unroll
-      for (int i = 0; i < RPL; i++) m &= (x[i] < tv) ? ~0u : ~(1u << i);
+      for (int i = 0; i < N; i++) m &= (x[i] < tv) ? ~0u : ~(1u << (i0 + i));
       break;
     case 2:
 #pragma unroll
-      for (int i = 0; i < RPL; i++) m &= (x[i] <= tv) ? ~0u : ~(1u << i);
+      for (int i = 0; i < N; i++) m &= (x[i] <= tv) ? ~0u : ~(1u << (i0 + i));
       break;
     case 3:
 #pragma unroll
-      for (int i = 0; i < RPL; i++) m &= (x[i] > tv) ? ~0u : ~(1u << i);
+      for (int i = 0; i < N; i++) m &= (x[i] > tv) ? ~0u : ~(1u << (i0 + i));
       break;
     default:
 #pragma unroll
-      for (int i = 0; i < RPL; i++) m &= (x[i] >= tv) ? ~0u : ~(1u << i);
+      for (int i = 0; i < N; i++) m &= (x[i] >= tv) ? ~0u : ~(1u << (i0 + i));
       break;
   }
   return m;
@@ -2678,10 +2907,57 @@ __device__ __forceinline__ void qk_ss_stage_issue(qk_cu32p st, qk_u32 U, qk_cu64
   }
 }
 
+/* the register-staged form (1024-thread blocks): wave w holds chunks w,
+ * w + 16, ... of the chunk table in PF registers of 16 bytes a lane.  The
+ * loads of tile t + 1 are issued right after tile t is in LDS and land
+ * while the waves evaluate tile t; stored into the one stage buffer once
+ * every wave is done with tile t.  Lanes past a column's end read its last
+ * 16 bytes (those rows are masked). */
+template <int WG, int PF>
+__device__ __forceinline__ void qk_ss_pf_issue(qk_cu32p prog, qk_cu32p st, qk_cu64p cols, qk_u64 base, qk_u64 rows_end,
+                                               qk_u32x4 (&pf)[PF])
+{
+  const qk_u32 wave = (qk_u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63u;
+  const qk_cu32p ct = prog + prog[6];
+  const qk_u32 tot = prog[7];
+#pragma unroll
+  for (int s = 0; s < PF; s++) {
+    const qk_u32 f = wave + (qk_u32)s * (WG / 64);
+    if (f < tot) {
+      const qk_u32 e = ct[f], u = e & 0xffffu, k = e >> 16;
+      const qk_u32 width = st[3 * u + 1];
+      const qk_u64 last = ((rows_end * width - 1) & ~15ULL);
+      qk_u64 src = base * width + ((qk_u64)k << 10) + lane * 16u;
+      src = src <= last ? src : last;
+      pf[s] = qk_ld(qk_gptr<const qk_u32x4>(cols[st[3 * u]] + src));
+    }
+  }
+}
+
+template <int WG, int PF>
+__device__ __forceinline__ void qk_ss_pf_store(qk_cu32p prog, qk_cu32p st, const qk_u32x4 (&pf)[PF], unsigned char* buf)
+{
+  const qk_u32 wave = (qk_u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63u;
+  const qk_cu32p ct = prog + prog[6];
+  const qk_u32 tot = prog[7];
+#pragma unroll
+  for (int s = 0; s < PF; s++) {
+    const qk_u32 f = wave + (qk_u32)s * (WG / 64);
+    if (f < tot) {
+      const qk_u32 e = ct[f], u = e & 0xffffu, k = e >> 16;
+      *(qk_u32x4*)(buf + st[3 * u + 2] + (k << 10) + lane * 16u) = pf[s];
+    }
+  }
+}
+
 template <int WG, int ITEMS>
 __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
 {
+  /* 16-byte registers a lane holds for the register-staged form (1024-
+   * thread blocks: a 160 KB stage is 160 chunks, 10 per wave) */
+  constexpr int PF = WG == 1024 ? 10 : 1;
   constexpr int RPL = WG * ITEMS / 64; /* rows per lane of a tile */
+  constexpr int HF = RPL > 8 ? 8 : RPL; /* f64 values of a lane's rows live at once */
   static_assert(RPL <= 32, "row masks are 32 bits");
   extern __shared__ __attribute__((aligned(16))) qk_u64 lds_u64[];
   const qk_cu32p prog = (qk_cu32p)a.ptr[0];
@@ -2718,23 +2994,36 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
   const qk_u64 tiles = (rows + tile - 1) / tile;
   const qk_u64 per = (tiles + gridDim.x - 1) / gridDim.x;
   const qk_u64 t_begin = per * blockIdx.x, t_end = t_begin + per < tiles ? t_begin + per : tiles;
-  /* push slot 6: 1 = two stage buffers (tile t+1 loads while tile t is
-   * evaluated), 0 = one (a larger tile in the same LDS; other blocks on
-   * the CU cover the load) */
-  const bool dbl = qk_push(a, QK_P_MODE) != 0;
+  /* push slot 6: 1 = two stage buffers (tile t+1 copied into LDS while
+   * tile t is evaluated), 0 = one (a larger tile in the same LDS; other
+   * blocks on the CU cover the load), 2 = one buffer filled from registers
+   * (1024-thread blocks whose stage is at most 16 x PF chunks) */
+  const qk_u32 smode = (qk_u32)qk_push(a, QK_P_MODE);
+  const bool dbl = smode == 1u;
+  const bool regs = WG == 1024 && smode == 2u;
+  qk_u32x4 pf[PF];
+  if (regs && t_begin < t_end) qk_ss_pf_issue<WG, PF>(prog, st, cols, t_begin * tile, rows, pf);
   if (dbl && t_begin < t_end) qk_ss_stage_issue<WG>(st, U, cols, t_begin * tile, tile, rows, buf0);
   for (qk_u64 t = t_begin; t < t_end; t++) {
-    if (!dbl) {
-      if (t != t_begin) __syncthreads(); /* every wave is done with tile t-1 */
-      qk_ss_stage_issue<WG>(st, U, cols, t * tile, tile, rows, buf0);
+    if (regs) {
+      /* every wave is done with tile t-1; tile t's registers into LDS */
+      if (t != t_begin) __syncthreads();
+      qk_ss_pf_store<WG, PF>(prog, st, pf, buf0);
+      __syncthreads();
+      if (t + 1 < t_end) qk_ss_pf_issue<WG, PF>(prog, st, cols, (t + 1) * tile, rows, pf);
+    } else {
+      if (!dbl) {
+        if (t != t_begin) __syncthreads(); /* every wave is done with tile t-1 */
+        qk_ss_stage_issue<WG>(st, U, cols, t * tile, tile, rows, buf0);
+      }
+      /* tile t has landed in its buffer, and every wave is done with tile
+       * t-1, whose buffer the copies of tile t+1 now take */
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      if (dbl && t + 1 < t_end)
+        qk_ss_stage_issue<WG>(st, U, cols, (t + 1) * tile, tile, rows, buf0 + ((t + 1 - t_begin) & 1u) * stage_bytes);
     }
-    /* tile t has landed in its buffer, and every wave is done with tile
-     * t-1, whose buffer the copies of tile t+1 now take */
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
     const unsigned char* sb = buf0 + (dbl ? ((t - t_begin) & 1u) * stage_bytes : 0);
-    if (dbl && t + 1 < t_end)
-      qk_ss_stage_issue<WG>(st, U, cols, (t + 1) * tile, tile, rows, buf0 + ((t + 1 - t_begin) & 1u) * stage_bytes);
     /* a wave evaluates each query it takes over all rows of the tile:
      * lane l takes rows l, l + 64, ... (RPL rows) */
     const qk_u64 base = t * tile;
@@ -2787,9 +3076,12 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
           }
         } else {
           const double tv = __longlong_as_double((qk_s64)((qk_u64)w[3] | ((qk_u64)w[4] << 32)));
-          double x[RPL];
-          qk_ss_rows<RPL>(cp, type, lane, x);
-          lm = qk_ss_cmp<RPL>(x, op, tv, lm);
+#pragma unroll
+          for (int h = 0; h < RPL; h += HF) {
+            double x[HF];
+            qk_ss_rows<HF>(cp, type, lane, h, x);
+            lm = qk_ss_cmp<HF>(x, op, tv, lm, h);
+          }
         }
         /* != and NOT IN: the complement (a NULL row still fails below) */
         if (!null_test && (tw & QK_LEAF_NEGATE)) lm = ~lm;
@@ -2885,58 +3177,63 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
           }
           continue;
         }
-        qk_u32 g[RPL];
+        /* the lane's rows in groups of HF: each group's keys, then its
+         * count and argument atomics, so only HF group indices are live */
 #pragma unroll
-        for (int i = 0; i < RPL; i++) {
-          g[i] = (kp[i * 64 + lane] ^ kx) - kmin;
-          if (g[i] >= R) m &= ~(1u << i);
-        }
+        for (int h = 0; h < RPL; h += HF) {
+          qk_u32 g[HF];
 #pragma unroll
-        for (int i = 0; i < RPL; i++)
-          if ((m >> i) & 1u) atomicAdd((unsigned long long*)&qa[g[i]], 1ULL);
+          for (int i = 0; i < HF; i++) {
+            g[i] = (kp[(h + i) * 64 + lane] ^ kx) - kmin;
+            if (g[i] >= R) m &= ~(1u << (h + i));
+          }
+#pragma unroll
+          for (int i = 0; i < HF; i++)
+            if ((m >> (h + i)) & 1u) atomicAdd((unsigned long long*)&qa[g[i]], 1ULL);
 #pragma unroll 1
-        for (qk_u32 k = 0; k < A; k++) {
-          const qk_u32 at = rec[8 + 2 * k];
-          const unsigned char* ap = sb + st[3 * rec[7 + 2 * k] + 2];
-          qk_u64* qs = qa + R + (qk_u64)k * 3 * R;
-          if (qk_type_is_int32(at)) {
-            /* 32-bit integers: exact sums without a trip through f64 */
-            const qk_u32* av = (const qk_u32*)ap;
-            const bool sgn = at == QK_T_I32;
-            if (flags & 2u) {
+          for (qk_u32 k = 0; k < A; k++) {
+            const qk_u32 at = rec[8 + 2 * k];
+            const unsigned char* ap = sb + st[3 * rec[7 + 2 * k] + 2];
+            qk_u64* qs = qa + R + (qk_u64)k * 3 * R;
+            if (qk_type_is_int32(at)) {
+              /* 32-bit integers: exact sums without a trip through f64 */
+              const qk_u32* av = (const qk_u32*)ap;
+              const bool sgn = at == QK_T_I32;
+              if (flags & 2u) {
 #pragma unroll
-              for (int i = 0; i < RPL; i++) {
-                const qk_u32 x = av[i * 64 + lane];
-                if ((m >> i) & 1u)
-                  atomicAdd((unsigned long long*)&qs[g[i]], sgn ? (qk_u64)(qk_s64)(qk_s32)x : (qk_u64)x);
-              }
-            }
-            if (flags & 4u) {
-#pragma unroll
-              for (int i = 0; i < RPL; i++) {
-                const qk_u32 x = av[i * 64 + lane];
-                const qk_u64 kk = qk_f64_key(sgn ? (double)(qk_s32)x : (double)x);
-                if ((m >> i) & 1u) {
-                  atomicMin((unsigned long long*)&qs[R + g[i]], (unsigned long long)kk);
-                  atomicMax((unsigned long long*)&qs[2 * R + g[i]], (unsigned long long)kk);
+                for (int i = 0; i < HF; i++) {
+                  const qk_u32 x = av[(h + i) * 64 + lane];
+                  if ((m >> (h + i)) & 1u)
+                    atomicAdd((unsigned long long*)&qs[g[i]], sgn ? (qk_u64)(qk_s64)(qk_s32)x : (qk_u64)x);
                 }
               }
-            }
-          } else {
-            double v[RPL];
-            qk_ss_rows<RPL>(ap, at, lane, v);
-            if (flags & 2u) {
+              if (flags & 4u) {
 #pragma unroll
-              for (int i = 0; i < RPL; i++)
-                if ((m >> i) & 1u) atomicAdd((double*)&qs[g[i]], v[i]);
-            }
-            if (flags & 4u) {
+                for (int i = 0; i < HF; i++) {
+                  const qk_u32 x = av[(h + i) * 64 + lane];
+                  const qk_u64 kk = qk_f64_key(sgn ? (double)(qk_s32)x : (double)x);
+                  if ((m >> (h + i)) & 1u) {
+                    atomicMin((unsigned long long*)&qs[R + g[i]], (unsigned long long)kk);
+                    atomicMax((unsigned long long*)&qs[2 * R + g[i]], (unsigned long long)kk);
+                  }
+                }
+              }
+            } else {
+              double v[HF];
+              qk_ss_rows<HF>(ap, at, lane, h, v);
+              if (flags & 2u) {
 #pragma unroll
-              for (int i = 0; i < RPL; i++) {
-                const qk_u64 kk = qk_f64_key(v[i]);
-                if ((m >> i) & 1u) {
-                  atomicMin((unsigned long long*)&qs[R + g[i]], (unsigned long long)kk);
-                  atomicMax((unsigned long long*)&qs[2 * R + g[i]], (unsigned long long)kk);
+                for (int i = 0; i < HF; i++)
+                  if ((m >> (h + i)) & 1u) atomicAdd((double*)&qs[g[i]], v[i]);
+              }
+              if (flags & 4u) {
+#pragma unroll
+                for (int i = 0; i < HF; i++) {
+                  const qk_u64 kk = qk_f64_key(v[i]);
+                  if ((m >> (h + i)) & 1u) {
+                    atomicMin((unsigned long long*)&qs[R + g[i]], (unsigned long long)kk);
+                    atomicMax((unsigned long long*)&qs[2 * R + g[i]], (unsigned long long)kk);
+                  }
                 }
               }
             }
@@ -2993,23 +3290,29 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
               continue;
             }
           } else {
-            double v[RPL];
-            qk_ss_rows<RPL>(ap, at, lane, v);
-            if (flags & 2u) {
-              double sd = 0.0;
+            /* the lane's rows in order, at most HF values live at once */
+            double sd = 0.0;
 #pragma unroll
-              for (int i = 0; i < RPL; i++) sd += (m & (1u << i)) ? v[i] : 0.0;
+            for (int h = 0; h < RPL; h += HF) {
+              double v[HF];
+              qk_ss_rows<HF>(ap, at, lane, h, v);
+              if (flags & 2u) {
+#pragma unroll
+                for (int i = 0; i < HF; i++) sd += (m & (1u << (h + i))) ? v[i] : 0.0;
+              }
+              if (flags & 4u) {
+#pragma unroll
+                for (int i = 0; i < HF; i++) {
+                  const qk_u64 kk = qk_f64_key(v[i]);
+                  const bool in = (m >> (h + i)) & 1u;
+                  mn = (in && kk < mn) ? kk : mn;
+                  mx = (in && kk > mx) ? kk : mx;
+                }
+              }
+            }
+            if (flags & 2u) {
               sd = qk_wave_add_f64(sd);
               if (lane == 0) qs[0] = (qk_u64)__double_as_longlong(__longlong_as_double((qk_s64)qs[0]) + sd);
-            }
-            if (flags & 4u) {
-#pragma unroll
-              for (int i = 0; i < RPL; i++) {
-                const qk_u64 kk = qk_f64_key(v[i]);
-                const bool in = (m >> i) & 1u;
-                mn = (in && kk < mn) ? kk : mn;
-                mx = (in && kk > mx) ? kk : mx;
-              }
             }
           }
           if (flags & 4u) {
@@ -3617,6 +3920,12 @@ static const QK_LibraryKernel g_library[] = {
     {"join_part_probe_1s_row_512", (const void*)k_join_part_probe_1s_row_512, 512},
     {"join_part_probe_1sm_row", (const void*)k_join_part_probe_1sm_row, QK_WG},
     {"join_part_probe_1sm_row_512", (const void*)k_join_part_probe_1sm_row_512, 512},
+    {"join_part_probe_pk_0_512", (const void*)k_join_part_probe_pk_0_512, 512},
+    {"join_part_probe_pk_1s_512", (const void*)k_join_part_probe_pk_1s_512, 512},
+    {"join_part_probe_pk_1m_512", (const void*)k_join_part_probe_pk_1m_512, 512},
+    {"join_part_probe_pk_1sm_512", (const void*)k_join_part_probe_pk_1sm_512, 512},
+    {"join_part_probe_pk_2s_512", (const void*)k_join_part_probe_pk_2s_512, 512},
+    {"join_part_probe_pk_2sm_512", (const void*)k_join_part_probe_pk_2sm_512, 512},
     {"encode_column", (const void*)k_encode_column, QK_WG},
     {"seq_check", (const void*)k_seq_check, QK_WG},
     {"gather", (const void*)k_gather, QK_WG},

@@ -93,6 +93,23 @@ def test_batch_configs_agree(events, items, bpc, lds, wg, dbl):
         _same(sp, b.keys, _vals(b), a.keys, _vals(a), f"q{i}")
 
 
+@pytest.mark.parametrize("regs", [0, 1])
+def test_register_and_lds_staging_agree(events, orc, regs):
+    """the 1024-thread shared scan fills its stage from registers (the next
+    tile's loads in flight while a tile is evaluated, QE_BATCH_STAGE_REGS 1)
+    or by direct copies into LDS (0); both match the oracle"""
+    t, cols = events
+    specs = wl.mixed_queries(32, seed=21)
+    cq.set_setting("QE_BATCH_WG", 1024)
+    cq.set_setting("QE_BATCH_COST", 0)
+    cq.set_setting("QE_BATCH_STAGE_REGS", regs)
+    got, tr = cq.aggregate_batch(t, specs, with_trace=True)
+    assert tr["fused"] == 32
+    for i, (sp, res) in enumerate(zip(specs, got)):
+        rk, rv = orc.group_aggregate(cols, sp.group, sp.funcs, sp.args, where=sp.where)
+        _same(sp, res.keys, _vals(res), rk, rv, f"oracle q{i}")
+
+
 def test_alone_equals_in_the_mix(events):
     t, _ = events
     specs = wl.mixed_queries(16, seed=9)

@@ -71,7 +71,7 @@ def test_generate_join_off_takes_generic_instances(events):
     a = cq.aggregate_join(t, 4, dim, 0, [cq.COUNT, cq.SUM], [None, 2], dim_group=1)
     cq.set_setting("QE_GENERATE_JOIN", 0)
     b = cq.aggregate_join(t, 4, dim, 0, [cq.COUNT, cq.SUM], [None, 2], dim_group=1)
-    assert a.trace["kernel"] == "join_part_probe_1s_512" and b.trace["kernel"] == "join_part_probe_512"
+    assert a.trace["kernel"] == "join_part_probe_pk_1s_512" and b.trace["kernel"] == "join_part_probe_512"
     assert np.array_equal(a.keys, b.keys) and np.array_equal(a.cells, b.cells)
 
 

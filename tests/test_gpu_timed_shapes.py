@@ -57,9 +57,9 @@ def test_join_a_tables_match_oracle(join_a):
 
 
 @pytest.mark.parametrize("funcs,probe", [
-    ([cq.SUM], "join_part_probe_1s_512"),                      # the bench's q6
-    ([cq.COUNT, cq.SUM], "join_part_probe_1s_512"),
-    ([cq.COUNT, cq.SUM, cq.MIN, cq.MAX, cq.AVG], "join_part_probe_1sm_512"),
+    ([cq.SUM], "join_part_probe_pk_1s_512"),                      # the bench's q6
+    ([cq.COUNT, cq.SUM], "join_part_probe_pk_1s_512"),
+    ([cq.COUNT, cq.SUM, cq.MIN, cq.MAX, cq.AVG], "join_part_probe_pk_1sm_512"),
 ])
 def test_join_a_1m_keys_group_by_segment(join_a, orc, funcs, probe):
     t, dim, amount, fk, dk, seg = join_a
@@ -70,6 +70,34 @@ def test_join_a_1m_keys_group_by_segment(join_a, orc, funcs, probe):
     rk, rv = orc.join_aggregate(fk, [None if a is None else amount for a in args], funcs, dk, dim_group=seg)
     rk, rv = _norm(rk, rv)
     assert res.groups == 100 and np.array_equal(res.keys, rk)
+    for e in range(len(funcs)):
+        assert np.array_equal(res.column(e), rv[:, e]), (funcs, e)
+
+
+@pytest.mark.parametrize("funcs,args,shape", [
+    ([cq.COUNT], [None], "0"),
+    ([cq.COUNT, cq.SUM], [None, 3], "1s"),           # SUM(ts): too wide for one count-and-sum atomic
+    ([cq.MIN, cq.MAX], [2, 2], "1m"),
+    ([cq.COUNT, cq.SUM, cq.SUM], [None, 2, 1], "2s"),
+    ([cq.SUM, cq.MAX, cq.MIN], [2, 1, 2], "2sm"),
+])
+def test_join_a_packed_and_unpacked_tables(join_a, orc, funcs, args, shape):
+    """The packed partition table (keys and group codes in one word, matched
+    by subtraction) against the two-array table and the oracle."""
+    t, dim, amount, fk, dk, seg = join_a
+    res = cq.aggregate_join(t, 4, dim, 0, funcs, args, dim_group=1)
+    assert res.trace["kernel"] == f"join_part_probe_pk_{shape}_512"
+    cq.set_setting("QE_JOIN_PROBE_PACKED", 0)
+    try:
+        two = cq.aggregate_join(t, 4, dim, 0, funcs, args, dim_group=1)
+    finally:
+        cq.reset_settings()
+    assert two.trace["kernel"] == f"join_part_probe_{shape}_512"
+    assert np.array_equal(res.keys, two.keys) and np.array_equal(res.cells, two.cells)
+    host = orc.gen_events(FACT_ROWS, seed=orc.SEED)
+    rk, rv = orc.join_aggregate(fk, [None if a is None else host[a] for a in args], funcs, dk, dim_group=seg)
+    rk, rv = _norm(rk, rv)
+    assert np.array_equal(res.keys, rk)
     for e in range(len(funcs)):
         assert np.array_equal(res.column(e), rv[:, e]), (funcs, e)
 
