@@ -114,10 +114,14 @@ static U32 qe_batch_cost(const QE_BatchPlanned* q)
 }
 
 /* staged columns of a launch: the union of the columns its queries read,
- * in first-use order; returns the stage index of `column` (adding it) */
+ * in first-use order; returns the stage index of `column` (adding it).
+ * The null bitmaps of the nullable columns its WHERE leaves read are staged
+ * beside them (`nulls`, in first-use order). */
 typedef struct QE_BatchStage {
   U32 count;
   U32 column[QE_MAX_COLUMNS];
+  U32 nulls;
+  U32 null_column[QE_MAX_COLUMNS];
 } QE_BatchStage;
 
 static U32 qe_batch_stage_of(QE_BatchStage* st, U32 column)
@@ -128,19 +132,46 @@ static U32 qe_batch_stage_of(QE_BatchStage* st, U32 column)
   return st->count++;
 }
 
-static void qe_batch_stage_query(QE_BatchStage* st, const QE_BatchPlanned* q)
+static U32 qe_batch_null_of(QE_BatchStage* st, U32 column)
 {
-  for (U32 l = 0; l < q->where.leaf_count; l++) qe_batch_stage_of(st, q->where.leaf[l].column);
+  for (U32 i = 0; i < st->nulls; i++)
+    if (st->null_column[i] == column) return i;
+  st->null_column[st->nulls] = column;
+  return st->nulls++;
+}
+
+static void qe_batch_stage_query(const QE_Table* t, QE_BatchStage* st, const QE_BatchPlanned* q)
+{
+  for (U32 l = 0; l < q->where.leaf_count; l++) {
+    qe_batch_stage_of(st, q->where.leaf[l].column);
+    if (t->columns[q->where.leaf[l].column].has_nulls) qe_batch_null_of(st, q->where.leaf[l].column);
+  }
   if (q->group_col >= 0) qe_batch_stage_of(st, (U32)q->group_col);
   for (U32 a = 0; a < q->plan.arg_count; a++) qe_batch_stage_of(st, q->plan.arg_col[a]);
 }
 
-/* bytes of one stage buffer for tiles of 256 * items rows */
-static U64 qe_batch_stage_bytes(QE_Table* t, const QE_BatchStage* st, U32 items)
+/* the staged columns' bytes in a stage buffer for tiles of 256 * items rows
+ * (each a multiple of 1 KB) */
+static U64 qe_batch_stage_column_bytes(QE_Table* t, const QE_BatchStage* st, U32 items)
 {
   U64 b = 0;
   for (U32 i = 0; i < st->count; i++) b += (U64)QE_WG * items * t->columns[st->column[i]].width;
   return b;
+}
+
+/* 1 KB chunks of staged null bitmaps: a bitmap is tile / 8 bytes, a chunk
+ * holds 8192 / tile of them */
+static U32 qe_batch_null_chunks(const QE_BatchStage* st, U32 items)
+{
+  const U32 per = 8192u / (QE_WG * items);
+  return (st->nulls + per - 1) / per;
+}
+
+/* bytes of one stage buffer for tiles of 256 * items rows: the columns,
+ * then the null bitmaps */
+static U64 qe_batch_stage_bytes(QE_Table* t, const QE_BatchStage* st, U32 items)
+{
+  return qe_batch_stage_column_bytes(t, st, items) + 1024ull * qe_batch_null_chunks(st, items);
 }
 
 /* the dynamic LDS a shared-scan block may take: every instance keeps the
@@ -163,15 +194,18 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
     W += qs[i].acc_words;
     qs[i].lds_off = WL;
     WL += qs[i].lds_words;
-    qe_batch_stage_query(&st, &qs[i]);
+    qe_batch_stage_query(t, &st, &qs[i]);
   }
   U32 rec_total = 0;
   for (U32 i = 0; i < n; i++) rec_total += qe_batch_rec_words(&qs[i]);
-  const U32 hdr = 8;
-  /* the chunk table: a tile's stage as 1 KB chunks */
+  const U32 hdr = 12;
+  /* the chunk table: a tile's stage as 1 KB chunks, the columns' then the
+   * null bitmaps' */
   U32 chunks = 0;
   for (U32 u = 0; u < st.count; u++) chunks += (U32)((U64)QE_WG * items * t->columns[st.column[u]].width / 1024);
-  U32 words = hdr + n + 3 * st.count + rec_total + W + chunks;
+  const U32 null_chunks = qe_batch_null_chunks(&st, items);
+  chunks += null_chunks;
+  U32 words = hdr + n + 3 * st.count + rec_total + W + chunks + 2 * st.nulls;
   U32* p = (U32*)calloc(words, 4);
   if (!p) return 0;
   /* grouped queries walk only their passing rows (QE_BATCH_COMPACT, 1) */
@@ -185,12 +219,25 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
   p[5] = WL;
   p[6] = p[2] + W;
   p[7] = chunks;
+  /* the staged null bitmaps: count, their table (table column, byte
+   * offset of its bitmap / 16), the bitmap area's byte offset in a stage
+   * buffer and its chunks */
+  const U32 bitmap_area = (U32)qe_batch_stage_column_bytes(t, &st, items);
+  p[8] = st.nulls;
+  p[9] = p[6] + chunks;
+  p[10] = bitmap_area;
+  p[11] = null_chunks;
+  for (U32 i = 0; i < st.nulls; i++) {
+    p[p[9] + 2 * i] = st.null_column[i];
+    p[p[9] + 2 * i + 1] = (U32)(t->columns[st.null_column[i]].null_offset / 16);
+  }
   {
     U32 f = 0;
     for (U32 u = 0; u < st.count; u++) {
       const U32 cu = (U32)((U64)QE_WG * items * t->columns[st.column[u]].width / 1024);
       for (U32 k = 0; k < cu; k++) p[p[6] + f++] = u | (k << 16);
     }
+    for (U32 j = 0; j < null_chunks; j++) p[p[6] + f++] = 0x8000u | (j << 16);
   }
   U32 off = 0;
   for (U32 u = 0; u < st.count; u++) {
@@ -245,6 +292,11 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
       for (U32 l = 0; l < q->where.leaf_count; l++) slot[l] = qe_batch_stage_of(&st, q->where.leaf[l].column);
       qe_where_words(t, &q->where, slot, 1, 0, words, &packed);
       const U32 L = q->where.leaf_count;
+      /* word 5 of a nullable column's leaf: 1 + its staged bitmap's byte
+       * offset in a stage buffer / 16 */
+      for (U32 l = 0; l < L; l++)
+        if (words[6 * l + 5])
+          words[6 * l + 5] = 1u + (bitmap_area + qe_batch_null_of(&st, q->where.leaf[l].column) * (QE_WG * items / 8)) / 16;
       memcpy(&r[QE_BATCH_REC_WORDS], words, 24 * L);
       for (U32 k = 0; k < (q->where.term_count + 3) / 4; k++) r[QE_BATCH_REC_WORDS + 6 * L + k] = words[6 * QE_SCAN_MAX_LEAVES + k];
     }
@@ -323,7 +375,7 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
   memset(&st, 0, sizeof(st));
   U64 Wq = 0;
   for (U32 i = 0; i < n; i++) {
-    qe_batch_stage_query(&st, &qs[i]);
+    qe_batch_stage_query(t, &st, &qs[i]);
     Wq += qs[i].lds_words;
   }
   U64 want_wg = cfg->wg;
@@ -573,8 +625,8 @@ static B32 qe_batch_plain_cheaper(QE_Table* t, const QE_BatchPlanned* qs, U32 n)
   for (U32 i = 0; i < n; i++) {
     QE_BatchStage one;
     memset(&one, 0, sizeof(one));
-    qe_batch_stage_query(&one, &qs[i]);
-    qe_batch_stage_query(&all, &qs[i]);
+    qe_batch_stage_query(t, &one, &qs[i]);
+    qe_batch_stage_query(t, &all, &qs[i]);
     F64 bytes = 0.0;
     for (U32 c = 0; c < one.count; c++) bytes += rows * t->columns[one.column[c]].width;
     plain += bytes / bw + launch;
@@ -636,7 +688,7 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
     /* a host-backed table's queries each stream through the resident chunk
      * cache (qe_aggregate) rather than read its rows over the host link */
     if (!table->host_backed && qe_batch_plan_query(table, &queries[i], &planned[np]) &&
-        (qe_batch_stage_query(&one, &planned[np]),
+        (qe_batch_stage_query(table, &one, &planned[np]),
          qe_batch_lds(planned[np].lds_words, qe_batch_stage_bytes(table, &one, 1)) <= max_lds)) {
       planned[np].index = i;
       np++;
@@ -679,7 +731,7 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
         for (U32 i = 0; i < np; i++) {
           if (taken[i]) continue;
           QE_BatchStage grown = st;
-          qe_batch_stage_query(&grown, &planned[i]);
+          qe_batch_stage_query(table, &grown, &planned[i]);
           U64 add = qe_batch_stage_bytes(table, &grown, 4) - base;
           if (in_launch && qe_batch_lds(words + planned[i].lds_words, base + add) > lds_budget) continue;
           if (add < best_add) {
@@ -688,7 +740,7 @@ static B32 qe_aggregate_batch_run(QE_Table* table, QE_BatchQuery* queries, U32 c
           }
         }
         if (best == np || in_launch == QE_BATCH_MAX_LAUNCH_QUERIES) break;
-        qe_batch_stage_query(&st, &planned[best]);
+        qe_batch_stage_query(table, &st, &planned[best]);
         words += planned[best].lds_words;
         taken[best] = 1;
         ordered[done++] = planned[best];
@@ -804,7 +856,7 @@ static void qe_tune_key(QE_Table* t, QE_BatchQuery* q, U32 count, U32* rows_log2
   memset(&st, 0, sizeof(st));
   QE_BatchPlanned p;
   for (U32 i = 0; i < count; i++)
-    if (qe_batch_plan_query(t, &q[i], &p)) qe_batch_stage_query(&st, &p);
+    if (qe_batch_plan_query(t, &q[i], &p)) qe_batch_stage_query(t, &st, &p);
   U64 w = 0;
   for (U32 i = 0; i < st.count; i++) w += t->columns[st.column[i]].width;
   *rows_log2 = qe_log2_ceil(t->row_count ? t->row_count : 1);

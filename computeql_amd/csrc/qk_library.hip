@@ -2652,9 +2652,16 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /*   [5] LDS accumulator words WL (W with every replica)                */
 /*   [6] offset of the chunk table, [7] its chunks C: a tile's stage as  */
 /*   1 KB chunks, entry f = staged column u | chunk k << 16 (bytes       */
-/*   k * 1024 on of the column's rows in the tile)                      */
-/*   [8 + q] offset of query q's record                                 */
-/*   [8 + Q + 3u] staged column u: table column, width in bytes (4 or   */
+/*   k * 1024 on of the column's rows in the tile), or 0x8000 | j << 16  */
+/*   for chunk j of the null bitmaps                                    */
+/*   [8] null bitmaps NB staged (of the nullable columns WHERE leaves    */
+/*   read), [9] offset of their table (per bitmap: table column, byte   */
+/*   offset of its bitmap / 16), [10] byte offset of the bitmap area in */
+/*   a stage buffer, [11] its 1 KB chunks: a tile of T rows holds bitmap */
+/*   s as T / 8 bytes at area + s * T / 8 (bit r of word r / 32 set =   */
+/*   tile row r NULL)                                                   */
+/*   [12 + q] offset of query q's record                                */
+/*   [12 + Q + 3u] staged column u: table column, width in bytes (4 or  */
 /*   8), byte offset of its rows in a stage buffer (16-byte aligned)    */
 /* query record: [0] leaves L (<= 8) | terms T << 8 (0 = the AND of   */
 /*   the leaves, else the rows of any term whose leaves all pass),      */
@@ -2669,7 +2676,8 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /*   words, [13] replicas (a power of two; lane l of the owning wave    */
 /*   adds into replica l mod replicas, the block folds them at its end),*/
 /*   [14..] leaves, 6 words each (qe_simple_leaf_words layout, slot =   */
-/*   staged index; word 5 a nullable column's bitmap offset / 16),     */
+/*   staged index; word 5 of a nullable column's leaf 1 + its staged    */
+/*   bitmap's byte offset in a stage buffer / 16),                     */
 /*   then the T terms' leaf bits, a byte each.                          */
 /* Columns: binding 2 holds the base address of every table column.     */
 /* A block walks tiles of 256 x ITEMS rows.  The rows of every staged   */
@@ -2692,7 +2700,7 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /* stage buffer 1 (when double-buffered, push slot 6).                  */
 /* ------------------------------------------------------------------ */
 #define QK_SS_MAX_ARGS 2
-#define QK_SS_HDR 8
+#define QK_SS_HDR 12
 #define QK_SS_REC 14
 
 __device__ __forceinline__ bool qk_type_is_wide(qk_u32 t) { return t == QK_T_F64 || t == QK_T_I64 || t == QK_T_U64; }
@@ -2883,8 +2891,29 @@ typedef const QK_CONST qk_u64* qk_cu64p;
  * column into the stage buffer at `buf`: 1 KB per wave instruction, the
  * chunks dealt round-robin over the block's waves; lanes past the
  * column's end read its last 16 bytes instead (those rows are masked) */
+/* the address lane `lane` copies 16 bytes from for chunk j of a tile's
+ * staged null bitmaps (tile rows from `base`): bitmap j * (64 / lpb) +
+ * lane / lpb, bytes (lane mod lpb) * 16 on of the tile's part, lpb = tile
+ * / 128 lanes a bitmap.  Lanes past the last bitmap read the last one
+ * again and past the bitmap's end its last 16 bytes (rows past the table's
+ * end are masked); a bitmap has (rows + 31) / 32 words and 16 bytes of
+ * slack after them. */
+__device__ __forceinline__ qk_u64 qk_ss_bitmap_src(qk_cu32p prog, qk_cu64p cols, qk_u32 j, qk_u32 lane, qk_u64 base,
+                                                   qk_u64 tile, qk_u64 rows_end)
+{
+  const qk_u32 nb = prog[8];
+  const qk_cu32p tab = prog + prog[9];
+  const qk_u32 lpb = (qk_u32)(tile >> 7);
+  qk_u32 s = j * (64u / lpb) + lane / lpb;
+  s = s < nb ? s : nb - 1u;
+  const qk_u64 last = ((((rows_end + 31) >> 5) * 4 - 1) & ~15ULL);
+  qk_u64 off = (base >> 3) + (qk_u64)(lane % lpb) * 16u;
+  off = off <= last ? off : last;
+  return cols[tab[2 * s]] + (qk_u64)tab[2 * s + 1] * 16u + off;
+}
+
 template <int WG>
-__device__ __forceinline__ void qk_ss_stage_issue(qk_cu32p st, qk_u32 U, qk_cu64p cols, qk_u64 base,
+__device__ __forceinline__ void qk_ss_stage_issue(qk_cu32p prog, qk_cu32p st, qk_u32 U, qk_cu64p cols, qk_u64 base,
                                                   qk_u64 tile, qk_u64 rows_end, unsigned char* buf)
 {
   /* the wave index is uniform; readfirstlane tells the compiler so (the
@@ -2904,6 +2933,13 @@ __device__ __forceinline__ void qk_ss_stage_issue(qk_cu32p st, qk_u32 U, qk_cu64
       __builtin_amdgcn_global_load_lds((const QK_GLOBAL void*)(cols[c] + src),
                                        (__attribute__((address_space(3))) void*)(buf + off + (k << 10)), 16, 0, 2 /* nt */);
     }
+  }
+  /* the null bitmaps' chunks */
+  const qk_u32 nbc = prog[11], area = prog[10];
+  for (qk_u32 j = wave; j < nbc; j += WG / 64) {
+    const qk_u64 src = qk_ss_bitmap_src(prog, cols, j, lane, base, tile, rows_end);
+    __builtin_amdgcn_global_load_lds((const QK_GLOBAL void*)src,
+                                     (__attribute__((address_space(3))) void*)(buf + area + (j << 10)), 16, 0, 2 /* nt */);
   }
 }
 
@@ -2925,6 +2961,11 @@ __device__ __forceinline__ void qk_ss_pf_issue(qk_cu32p prog, qk_cu32p st, qk_cu
     const qk_u32 f = wave + (qk_u32)s * (WG / 64);
     if (f < tot) {
       const qk_u32 e = ct[f], u = e & 0xffffu, k = e >> 16;
+      if (u & 0x8000u) {
+        /* chunk k of the null bitmaps (tiles of WG rows) */
+        pf[s] = qk_ld(qk_gptr<const qk_u32x4>(qk_ss_bitmap_src(prog, cols, k, lane, base, WG, rows_end)));
+        continue;
+      }
       const qk_u32 width = st[3 * u + 1];
       const qk_u64 last = ((rows_end * width - 1) & ~15ULL);
       qk_u64 src = base * width + ((qk_u64)k << 10) + lane * 16u;
@@ -2945,7 +2986,8 @@ __device__ __forceinline__ void qk_ss_pf_store(qk_cu32p prog, qk_cu32p st, const
     const qk_u32 f = wave + (qk_u32)s * (WG / 64);
     if (f < tot) {
       const qk_u32 e = ct[f], u = e & 0xffffu, k = e >> 16;
-      *(qk_u32x4*)(buf + st[3 * u + 2] + (k << 10) + lane * 16u) = pf[s];
+      const qk_u32 dst = (u & 0x8000u) ? prog[10] : st[3 * u + 2];
+      *(qk_u32x4*)(buf + dst + (k << 10) + lane * 16u) = pf[s];
     }
   }
 }
@@ -3003,7 +3045,7 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
   const bool regs = WG == 1024 && smode == 2u;
   qk_u32x4 pf[PF];
   if (regs && t_begin < t_end) qk_ss_pf_issue<WG, PF>(prog, st, cols, t_begin * tile, rows, pf);
-  if (dbl && t_begin < t_end) qk_ss_stage_issue<WG>(st, U, cols, t_begin * tile, tile, rows, buf0);
+  if (dbl && t_begin < t_end) qk_ss_stage_issue<WG>(prog, st, U, cols, t_begin * tile, tile, rows, buf0);
   for (qk_u64 t = t_begin; t < t_end; t++) {
     if (regs) {
       /* every wave is done with tile t-1; tile t's registers into LDS */
@@ -3014,14 +3056,14 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
     } else {
       if (!dbl) {
         if (t != t_begin) __syncthreads(); /* every wave is done with tile t-1 */
-        qk_ss_stage_issue<WG>(st, U, cols, t * tile, tile, rows, buf0);
+        qk_ss_stage_issue<WG>(prog, st, U, cols, t * tile, tile, rows, buf0);
       }
       /* tile t has landed in its buffer, and every wave is done with tile
        * t-1, whose buffer the copies of tile t+1 now take */
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
       if (dbl && t + 1 < t_end)
-        qk_ss_stage_issue<WG>(st, U, cols, (t + 1) * tile, tile, rows, buf0 + ((t + 1 - t_begin) & 1u) * stage_bytes);
+        qk_ss_stage_issue<WG>(prog, st, U, cols, (t + 1) * tile, tile, rows, buf0 + ((t + 1 - t_begin) & 1u) * stage_bytes);
     }
     const unsigned char* sb = buf0 + (dbl ? ((t - t_begin) & 1u) * stage_bytes : 0);
     /* a wave evaluates each query it takes over all rows of the tile:
@@ -3087,19 +3129,15 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
         if (!null_test && (tw & QK_LEAF_NEGATE)) lm = ~lm;
         if (w[5]) {
           /* a nullable column: its NULL rows fail the comparison (IS NULL
-           * keeps just them, IS NOT NULL the others).  The bitmap (bit r
-           * of word r / 32, 16 * w[5] bytes into the column) gives row
-           * i * 64 + lane of the tile at bit lane mod 32 of word base / 32
-           * + 2 i + lane / 32; words past the last row are not read (those
-           * rows are outside `valid`). */
-          const qk_u32* nb = (const qk_u32*)(cols[st[3 * u]] + (qk_u64)w[5] * 16);
-          const qk_u64 w0 = (base >> 5) + (lane >> 5), wlast = (rows - 1) >> 5;
+           * keeps just them, IS NOT NULL the others).  The tile's part of
+           * the bitmap is staged (16 * (w[5] - 1) bytes into the stage):
+           * row i * 64 + lane of the tile at bit lane mod 32 of word
+           * 2 i + lane / 32; rows past the table's end are outside
+           * `valid`. */
+          const qk_u32* nb = (const qk_u32*)(sb + (qk_u64)(w[5] - 1u) * 16u);
           qk_u32 nm = 0;
 #pragma unroll
-          for (int i = 0; i < RPL; i++) {
-            const qk_u64 wi = w0 + 2 * (qk_u64)i;
-            nm |= ((wi <= wlast ? nb[wi] : 0u) >> (lane & 31u) & 1u) << i;
-          }
+          for (int i = 0; i < RPL; i++) nm |= ((nb[2 * i + (lane >> 5)] >> (lane & 31u)) & 1u) << i;
           lm = null_test ? (op == 5u ? nm : ~nm) : (lm & ~nm);
         }
         if (T == 0u) {
