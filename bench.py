@@ -269,8 +269,10 @@ P99_LIMIT_MS = 50.0
 def served_sweep(Server, name, kw, start, drive, seconds_per_rate, dedupe=None):
     """offered rates rising by half from `start` until p99 passes the limit
     or the achieved rate falls 10% below the offered one (at most 16
-    rates); the highest rate that held.  dedupe: QE_BATCH_DEDUPE for the
-    sweep (identical queries of a batch run once), None = as set."""
+    rates), then one rate at the geometric middle of the last one that held
+    and the first that did not; the highest rate that held.  dedupe:
+    QE_BATCH_DEDUPE for the sweep (identical queries of a batch run once),
+    None = as set."""
     import computeql_amd as cq
 
     old = cq.get_setting("QE_BATCH_DEDUPE", 1)
@@ -291,6 +293,18 @@ def served_sweep(Server, name, kw, start, drive, seconds_per_rate, dedupe=None):
                     raise RuntimeError(f"{name}: {st['failed']} requests failed")
                 if st["p99_ms"] > P99_LIMIT_MS or st["qps"] < 0.9 * r:
                     crossed = "p99" if st["p99_ms"] > P99_LIMIT_MS else "rate"
+                    if best is not None:
+                        mid = (best["offered"] * r) ** 0.5
+                        st = drive(srv, mid, seconds_per_rate, int(mid))
+                        points.append({"offered": round(mid), "qps": round(st["qps"], 1),
+                                       "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                                       "mean_batch": round(st["mean_batch"], 1), "joins": st["joins"],
+                                       "scans": st["scans"]})
+                        log(f"{name} offered {mid:.0f}: {st['qps']:.0f} qps p99 {st['p99_ms']:.2f} ms")
+                        if st["failed"]:
+                            raise RuntimeError(f"{name}: {st['failed']} requests failed")
+                        if st["p99_ms"] <= P99_LIMIT_MS and st["qps"] >= 0.9 * mid:
+                            best = points[-1]
                     break
                 best = points[-1]
                 r *= 1.5
@@ -343,11 +357,16 @@ def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True
     ev = wl.upload(wl.event_table(EVENT_ROWS, seed=7), wl.event_nulls(EVENT_ROWS, seed=7))  # 2% NULL timestamps
     # GROUP BY user_id (Zipf over 1M users) on the hash path: the
     # reference's "group by user (1M groups)" (README.md:46, 56 ms)
+    cq.profile(True)
     wall, kms, res = _timed(lib, lambda: ev.aggregate([cq.COUNT, cq.SUM], [None, wl.M0], group=wl.USER))
     by_user = {"groups": res.groups, "path": cq.AGG_PATHS.get(res.trace["path"]), "wall_ms": round(wall, 3),
                "kernel_ms": round(kms, 3)}
     log(f"group by user: {res.groups} groups, {kms:.2f} ms kernel, {wall:.2f} ms wall")
     specs = wl.mixed_queries(64, seed=11)
+    # the plain and batched sides, and the served sweeps below, are timed
+    # without per-dispatch hipEvents: those cost each launch of the
+    # one-query-per-launch side far more than the batched side's few
+    cq.profile(False)
     for _ in range(2):
         for sp in specs:
             ev.aggregate(sp.funcs, sp.args, where=sp.where, group=sp.group)
@@ -358,13 +377,19 @@ def batched_and_served(cq, lib, seconds_per_rate: float, served_too: bool = True
         for sp in specs:
             ev.aggregate(sp.funcs, sp.args, where=sp.where, group=sp.group)
     plain_s = (time.perf_counter() - t0) / reps
-    lib.gpu_hip_profile_reset()
     t0 = time.perf_counter()
     for _ in range(reps):
         _, tr = cq.aggregate_batch(ev, specs, with_trace=True)
     batch_s = (time.perf_counter() - t0) / reps
-    # batches run as HIP graphs (QE_BATCH_GRAPH): their device time is the
-    # graph launch (shared_scan + shared_reduce), timed as "graph:..."
+    # the shared scan's device time from a separate profiled pass.  Batches
+    # run as HIP graphs (QE_BATCH_GRAPH): their device time is the graph
+    # launch (shared_scan + shared_reduce), timed as "graph:..."
+    cq.profile(True)
+    lib.gpu_hip_profile_reset()
+    for _ in range(reps):
+        cq.aggregate_batch(ev, specs)
+    lib.gpu_hip_wait()
+    cq.profile(False)
     scan_ns, _ = profile_stats(lib, "shared_scan*")
     timed = "shared_scan kernels"
     if not scan_ns:

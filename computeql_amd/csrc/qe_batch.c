@@ -274,13 +274,14 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
     r[0] = q->where.leaf_count | (q->where.term_count << 8);
     r[1] = (q->group_col >= 0 ? 1u : 0u) | (q->plan.need_sum ? 2u : 0u) | (q->plan.need_minmax ? 4u : 0u) |
            (q->group_col >= 0 && compact ? 8u : 0u);
-    r[2] = q->group_col >= 0 ? qe_batch_stage_of(&st, (U32)q->group_col) : 0;
+    /* the key's and arguments' rows as byte offsets in a stage buffer */
+    r[2] = q->group_col >= 0 ? p[hdr + n + 3 * qe_batch_stage_of(&st, (U32)q->group_col) + 2] : 0;
     r[3] = q->group_col >= 0 ? t->columns[q->group_col].type : 0;
     r[4] = q->key_min;
     r[5] = q->range;
     r[6] = q->plan.arg_count;
     for (U32 a = 0; a < q->plan.arg_count; a++) {
-      r[7 + 2 * a] = qe_batch_stage_of(&st, q->plan.arg_col[a]);
+      r[7 + 2 * a] = p[hdr + n + 3 * qe_batch_stage_of(&st, q->plan.arg_col[a]) + 2];
       r[8 + 2 * a] = t->columns[q->plan.arg_col[a]].type;
     }
     r[11] = q->acc_off;
@@ -292,8 +293,10 @@ static U32* qe_batch_program(QE_Table* t, QE_BatchPlanned* qs, U32 n, U32 items,
       for (U32 l = 0; l < q->where.leaf_count; l++) slot[l] = qe_batch_stage_of(&st, q->where.leaf[l].column);
       qe_where_words(t, &q->where, slot, 1, 0, words, &packed);
       const U32 L = q->where.leaf_count;
-      /* word 5 of a nullable column's leaf: 1 + its staged bitmap's byte
-       * offset in a stage buffer / 16 */
+      /* word 0 of a leaf: its column's byte offset in a stage buffer (in
+       * place of the stage index); word 5 of a nullable column's leaf: 1 +
+       * its staged bitmap's byte offset in a stage buffer / 16 */
+      for (U32 l = 0; l < L; l++) words[6 * l] = p[hdr + n + 3 * slot[l] + 2];
       for (U32 l = 0; l < L; l++)
         if (words[6 * l + 5])
           words[6 * l + 5] = 1u + (bitmap_area + qe_batch_null_of(&st, q->where.leaf[l].column) * (QE_WG * items / 8)) / 16;

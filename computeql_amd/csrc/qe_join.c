@@ -877,13 +877,10 @@ static B32 qe_join_same(const QE_JoinQuery* a, const QE_JoinQuery* b)
 
 /* run order: joins that can share a fact side next to each other (same dim
  * and keys, then the same argument columns), arrival order otherwise */
-static const QE_JoinQuery* t_join_order_q;
-
-static int qe_join_order_cmp(const void* pa, const void* pb)
+static int qe_join_order_cmp(const QE_JoinQuery* queries, U32 ia, U32 ib)
 {
-  const U32 ia = *(const U32*)pa, ib = *(const U32*)pb;
-  const QE_JoinQuery* a = &t_join_order_q[ia];
-  const QE_JoinQuery* b = &t_join_order_q[ib];
+  const QE_JoinQuery* a = &queries[ia];
+  const QE_JoinQuery* b = &queries[ib];
   if (a->dim != b->dim) return (uintptr_t)a->dim < (uintptr_t)b->dim ? -1 : 1;
   if (a->fact_key_col != b->fact_key_col) return a->fact_key_col < b->fact_key_col ? -1 : 1;
   if (a->dim_key_col != b->dim_key_col) return a->dim_key_col < b->dim_key_col ? -1 : 1;
@@ -943,9 +940,18 @@ B32 qe_aggregate_join_batch(QE_Table* fact, QE_JoinQuery* queries, U32 count, QE
       }
     if (rep_of[i] == ~0u) order[runs++] = i;
   }
-  t_join_order_q = queries;
-  qsort(order, runs, sizeof(U32), qe_join_order_cmp);
-  t_join_order_q = 0;
+  /* insertion sort (a batch holds at most a server's max_batch joins); the
+   * comparator takes the queries as an argument, so concurrent batches on
+   * other threads share no state */
+  for (U32 i = 1; i < runs; i++) {
+    const U32 o = order[i];
+    U32 j = i;
+    while (j > 0 && qe_join_order_cmp(queries, order[j - 1], o) > 0) {
+      order[j] = order[j - 1];
+      j--;
+    }
+    order[j] = o;
+  }
   QE_JoinShare share;
   memset(&share, 0, sizeof(share));
   t_join_share = &share;

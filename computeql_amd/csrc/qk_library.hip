@@ -2667,17 +2667,17 @@ extern "C" __global__ void LB k_hash_compact(QK_Args a)
 /*   the leaves, else the rows of any term whose leaves all pass),      */
 /*   [1] flags (1 grouped, 2 sums,                                      */
 /*   4 min/max, 8 grouped rows walked compactly where a wave's rows are */
-/*   sparse: a lane visits only the rows that passed), [2] key (staged  */
-/*   index), [3] key type,                                              */
+/*   sparse: a lane visits only the rows that passed), [2] key (byte   */
+/*   offset of its rows in a stage buffer), [3] key type,               */
 /*   [4] key_min code,                                                  */
 /*   [5] groups R (1 when ungrouped), [6] args A (<= 2), [7] arg 0      */
-/*   (staged index), [8] arg type 0, [9] arg 1, [10] arg type 1,        */
+/*   (stage byte offset), [8] arg type 0, [9] arg 1, [10] arg type 1,   */
 /*   [11] offset of its words among the W, [12] offset of its LDS      */
 /*   words, [13] replicas (a power of two; lane l of the owning wave    */
 /*   adds into replica l mod replicas, the block folds them at its end),*/
-/*   [14..] leaves, 6 words each (qe_simple_leaf_words layout, slot =   */
-/*   staged index; word 5 of a nullable column's leaf 1 + its staged    */
-/*   bitmap's byte offset in a stage buffer / 16),                     */
+/*   [14..] leaves, 6 words each (qe_simple_leaf_words layout, word 0  */
+/*   the column's stage byte offset; word 5 of a nullable column's leaf */
+/*   1 + its staged bitmap's byte offset in a stage buffer / 16),       */
 /*   then the T terms' leaf bits, a byte each.                          */
 /* Columns: binding 2 holds the base address of every table column.     */
 /* A block walks tiles of 256 x ITEMS rows.  The rows of every staged   */
@@ -3075,12 +3075,15 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
     for (int i = 0; i < RPL; i++) valid |= (base + (qk_u64)i * 64 + lane < rows) ? (1u << i) : 0u;
     const qk_u32 cur = (qk_u32)((t - t_begin) & 1u);
     if (threadIdx.x == 0) s_qnext[cur ^ 1u] = 0u;
+    /* the wave takes its next query while it evaluates this one, so the
+     * counter's round trip overlaps the work */
+    qk_u32 qn = 0u;
+    if (lane == 0) qn = atomicAdd(&s_qnext[cur], 1u);
 #pragma unroll 1
     for (;;) {
-      qk_u32 q = 0u;
-      if (lane == 0) q = atomicAdd(&s_qnext[cur], 1u);
-      q = (qk_u32)__builtin_amdgcn_readfirstlane((int)q);
+      const qk_u32 q = (qk_u32)__builtin_amdgcn_readfirstlane((int)qn);
       if (q >= Q) break;
+      if (lane == 0) qn = atomicAdd(&s_qnext[cur], 1u);
       const qk_cu32p rec = prog + prog[QK_SS_HDR + q];
       const qk_u32 L = (rec[0] & 0xffu) < 8u ? (rec[0] & 0xffu) : 8u;
       const qk_u32 T = (rec[0] >> 8) & 0xffu; /* terms: 0 = the AND of the leaves */
@@ -3093,7 +3096,7 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
       for (qk_u32 l = 0; l < L; l++) {
         const qk_cu32p w = rec + QK_SS_REC + 6 * l;
         const qk_u32 u = w[0], tw = w[1], op = w[2], type = tw & 0xffu;
-        const unsigned char* cp = sb + st[3 * u + 2];
+        const unsigned char* cp = sb + u;
         qk_u32 lm = ~0u;
         const bool null_test = (tw & QK_LEAF_NULL_TEST) != 0u; /* IS [NOT] NULL: the bitmap alone */
         if (null_test) {
@@ -3174,7 +3177,7 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
          * the group of every row and the argument values read first. */
         if (!__any((int)(m != 0))) continue;
         qa += (qk_u64)(lane & (rec[13] - 1u)) * (R * (1u + 3u * A));
-        const qk_u32* kp = (const qk_u32*)(sb + st[3 * rec[2] + 2]);
+        const qk_u32* kp = (const qk_u32*)(sb + rec[2]);
         const qk_u32 kx = rec[3] == QK_T_I32 ? 0x80000000u : 0u, kmin = rec[4];
         if ((flags & 8u) && !__any((int)(__popc(m) > (qk_u32)RPL / 4u))) {
           /* compact, for a wave whose lanes each hold at most RPL / 4
@@ -3192,7 +3195,7 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
 #pragma unroll 1
             for (qk_u32 k = 0; k < A; k++) {
               const qk_u32 at = rec[8 + 2 * k];
-              const unsigned char* ap = sb + st[3 * rec[7 + 2 * k] + 2];
+              const unsigned char* ap = sb + rec[7 + 2 * k];
               qk_u64* qs = qa + R + (qk_u64)k * 3 * R;
               qk_u64 add;
               double v;
@@ -3231,7 +3234,7 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
 #pragma unroll 1
           for (qk_u32 k = 0; k < A; k++) {
             const qk_u32 at = rec[8 + 2 * k];
-            const unsigned char* ap = sb + st[3 * rec[7 + 2 * k] + 2];
+            const unsigned char* ap = sb + rec[7 + 2 * k];
             qk_u64* qs = qa + R + (qk_u64)k * 3 * R;
             if (qk_type_is_int32(at)) {
               /* 32-bit integers: exact sums without a trip through f64 */
@@ -3286,7 +3289,7 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
 #pragma unroll 1
         for (qk_u32 k = 0; k < A; k++) {
           const qk_u32 at = rec[8 + 2 * k];
-          const unsigned char* ap = sb + st[3 * rec[7 + 2 * k] + 2];
+          const unsigned char* ap = sb + rec[7 + 2 * k];
           qk_u64* qs = qa + 1 + (qk_u64)k * 3;
           qk_u64 mn = ~0ULL, mx = 0ULL;
           if (qk_type_is_int32(at)) {

@@ -157,3 +157,39 @@ def test_served_joins_batch(star, orc):
         st = srv.stats()
     assert st["joins"] == len(specs)
     _check(orc, star, specs, results)
+
+
+def test_join_batches_on_concurrent_threads(star, orc):
+    """Join batches sorted and run on several host threads at once (as the
+    served loop's workers run them): each thread's batch keeps its own run
+    order and shared fact side, and every result matches the oracle."""
+    import threading
+
+    t, dim, dimb = star[0], star[1], star[2]
+    mixes = [
+        [_spec(dimb, 1, [cq.COUNT], [None], literal=b) for b in (2, 7, 4)]
+        + [_spec(dim, 4, [cq.COUNT, cq.SUM], [None, 2], literal=lim, group=1) for lim in (30, 70)],
+        [_spec(dim, 4, [cq.SUM], [2], group=1), _spec(dimb, 1, [cq.COUNT], [None], literal=5)],
+        [_spec(dim, 4, [cq.COUNT, cq.SUM], [None, 2], literal=lim, group=1) for lim in (10, 90, 50, 20)],
+    ]
+    out, errors = {}, []
+
+    def run(k):
+        try:
+            for rep in range(6):
+                out[(k, rep)] = aggregate_join_batch(t, mixes[k])
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(len(mixes))]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for k, specs in enumerate(mixes):
+        first = out[(k, 0)]
+        _check(orc, star, specs, first)
+        for rep in range(1, 6):
+            for a, b in zip(first, out[(k, rep)]):
+                assert np.array_equal(a.keys, b.keys) and np.array_equal(a.cells, b.cells)
