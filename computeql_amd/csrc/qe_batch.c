@@ -611,16 +611,23 @@ static void qe_batch_copy_result(const QE_BatchQuery* from, QE_BatchQuery* to)
 /* A launch's queries as one shared scan, or one plain operator each: the
  * shared scan reads the union of their columns once but costs more per
  * query and row than a single-query kernel, which streams at the HBM rate.
- * Estimates: shared = max(union bytes / BW, queries x rows / R) and plain
- * = the queries' bytes / BW + queries x launch, with BW QE_BATCH_COST_BW_GBS
- * (5500), R QE_BATCH_COST_SHARED_MROWS (600000 M query-rows/s, measured on
- * the 64-query mix) and launch QE_BATCH_COST_LAUNCH_US (60). */
+ * Estimates: shared = max(union bytes / BW, max(queries, Q0) x rows / R)
+ * and plain = the queries' bytes / BW + queries x launch, with BW
+ * QE_BATCH_COST_BW_GBS (5500), R QE_BATCH_COST_SHARED_MROWS (600000 M
+ * query-rows/s, measured on the 64-query mix) and launch
+ * QE_BATCH_COST_LAUNCH_US (60).  Q0 (QE_BATCH_COST_MIN_QUERIES, 8): the
+ * shared scan's waves take a tile's queries one per wave, so a launch of
+ * few queries leaves most of a block's 16 waves idle; measured, one query
+ * alone costs about what eight cost (a 3-leaf AND/OR count over 1B rows
+ * 14.0 ms shared against 1.8 ms plain, GROUP BY region over 100M rows 1.5
+ * against 0.17 ms). */
 static B32 qe_batch_plain_cheaper(QE_Table* t, const QE_BatchPlanned* qs, U32 n)
 {
   if (qe_settings_u64("QE_BATCH_COST", 1) == 0) return 0;
   const F64 bw = (F64)qe_settings_u64("QE_BATCH_COST_BW_GBS", 5500) * 1e9;
   const F64 rate = (F64)qe_settings_u64("QE_BATCH_COST_SHARED_MROWS", 600000) * 1e6;
   const F64 launch = (F64)qe_settings_u64("QE_BATCH_COST_LAUNCH_US", 60) * 1e-6;
+  const F64 q0 = (F64)qe_settings_u64("QE_BATCH_COST_MIN_QUERIES", 8);
   const F64 rows = (F64)t->row_count;
   QE_BatchStage all;
   memset(&all, 0, sizeof(all));
@@ -637,7 +644,8 @@ static B32 qe_batch_plain_cheaper(QE_Table* t, const QE_BatchPlanned* qs, U32 n)
   F64 ubytes = 0.0;
   for (U32 c = 0; c < all.count; c++) ubytes += rows * t->columns[all.column[c]].width;
   F64 shared = ubytes / bw;
-  if ((F64)n * rows / rate > shared) shared = (F64)n * rows / rate;
+  const F64 qn = (F64)n > q0 ? (F64)n : q0;
+  if (qn * rows / rate > shared) shared = qn * rows / rate;
   return plain < shared;
 }
 

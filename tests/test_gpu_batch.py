@@ -237,6 +237,32 @@ def test_cost_estimate_picks_plain_or_shared(events):
         _same(sp, a.keys, _vals(a), b.keys, _vals(b), f"q{i}")
 
 
+def test_few_queries_on_a_large_table_run_plain():
+    """A launch of fewer queries than a block has waves leaves waves idle in
+    the shared scan, so the estimate charges it at least eight queries'
+    worth of query-rows: one 3-leaf AND/OR count over 16M rows runs as the
+    plain operator, and with that minimum set to 1 it is fused instead.
+    Both counts equal numpy's."""
+    n = 16_000_000
+    rng = np.random.default_rng(31)
+    cols = [rng.integers(0, 1000, n, dtype=np.uint32) for _ in range(3)]
+    t = cq.Table(n)
+    try:
+        for i, c in enumerate(cols):
+            t.add_column(f"c{i}", cq.U32, c)
+        sp = cq.AggSpec([cq.COUNT], [None], ((col(0) == 7) & (col(1) < 500)) | (col(2) > 990))
+        expect = int(np.count_nonzero(((cols[0] == 7) & (cols[1] < 500)) | (cols[2] > 990)))
+        res, tr = cq.aggregate_batch(t, [sp], with_trace=True)
+        assert tr["fused"] == 0 and tr["fallback"] == 1
+        assert int(res[0].cells[0, 0]) == expect
+        cq.set_setting("QE_BATCH_COST_MIN_QUERIES", 1)
+        res, tr = cq.aggregate_batch(t, [sp], with_trace=True)
+        assert tr["fused"] == 1 and tr["fallback"] == 0
+        assert int(res[0].cells[0, 0]) == expect
+    finally:
+        t.close()
+
+
 def test_i64_leaves_compare_like_f64():
     """i64 WHERE leaves in the shared scan compare as f64, like the
     reference's CPU path: every count equals numpy's f64 comparison and the
