@@ -425,8 +425,10 @@ static B32 qe_batch_launch(QE_Table* t, QE_BatchPlanned* qs, U32 n, QE_BatchQuer
   if (fit < 1) fit = 1;
   U64 blocks = cus * (bpc < fit ? bpc : fit);
   if (blocks > tiles) blocks = tiles ? tiles : 1;
+  /* the register-staged form has an instance of its own (no LDS-direct
+   * copies in it, so no waits for them at its LDS atomics) */
   char kname[32];
-  snprintf(kname, sizeof(kname), "shared_scan_%u_%u", wg, items);
+  snprintf(kname, sizeof(kname), "shared_scan_%u_%u%s", wg, items, stage_mode == 2 ? "_r" : "");
   GPU_Kernel* k = qe_kernel(kname);
   GPU_Kernel* kr = qe_kernel("shared_reduce");
   GPU_Buffer* pb = qe_scratch("batch_program", (U64)words * 4);

@@ -2992,7 +2992,13 @@ __device__ __forceinline__ void qk_ss_pf_store(qk_cu32p prog, qk_cu32p st, const
   }
 }
 
-template <int WG, int ITEMS>
+/* SM: the stage mode fixed at compile time (2, registers) or read from
+ * push slot 6 (-1).  An instance without the LDS-direct copies matters:
+ * while such copies may be in flight the compiler waits for every
+ * outstanding global load before each LDS store or atomic, which in the
+ * register-staged form would wait out the next tile's loads at every
+ * group-table atomic. */
+template <int WG, int ITEMS, int SM = -1>
 __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
 {
   /* 16-byte registers a lane holds for the register-staged form (1024-
@@ -3040,7 +3046,7 @@ __device__ __forceinline__ void qk_shared_scan(const QK_Args& a)
    * tile t is evaluated), 0 = one (a larger tile in the same LDS; other
    * blocks on the CU cover the load), 2 = one buffer filled from registers
    * (1024-thread blocks whose stage is at most 16 x PF chunks) */
-  const qk_u32 smode = (qk_u32)qk_push(a, QK_P_MODE);
+  const qk_u32 smode = SM >= 0 ? (qk_u32)SM : (qk_u32)qk_push(a, QK_P_MODE);
   const bool dbl = smode == 1u;
   const bool regs = WG == 1024 && smode == 2u;
   qk_u32x4 pf[PF];
@@ -3398,6 +3404,8 @@ extern "C" __global__ void LB k_shared_scan_256_4(QK_Args a) { qk_shared_scan<25
 extern "C" __global__ void __launch_bounds__(512) k_shared_scan_512_1(QK_Args a) { qk_shared_scan<512, 1>(a); }
 extern "C" __global__ void __launch_bounds__(512) k_shared_scan_512_2(QK_Args a) { qk_shared_scan<512, 2>(a); }
 extern "C" __global__ void __launch_bounds__(1024) k_shared_scan_1024_1(QK_Args a) { qk_shared_scan<1024, 1>(a); }
+/* the register-staged form alone (stage mode 2) */
+extern "C" __global__ void __launch_bounds__(1024) k_shared_scan_1024_1_r(QK_Args a) { qk_shared_scan<1024, 1, 2>(a); }
 
 /* shared_reduce: word w of the result folds the blocks' words w with the
  * word's op.  One wave per word: lane l folds blocks l, l + 64, ... in
@@ -3929,6 +3937,7 @@ static const QK_LibraryKernel g_library[] = {
     {"shared_scan_512_1", (const void*)k_shared_scan_512_1, 512},
     {"shared_scan_512_2", (const void*)k_shared_scan_512_2, 512},
     {"shared_scan_1024_1", (const void*)k_shared_scan_1024_1, 1024},
+    {"shared_scan_1024_1_r", (const void*)k_shared_scan_1024_1_r, 1024},
     {"shared_reduce", (const void*)k_shared_reduce, QK_WG},
     {"hash_compact", (const void*)k_hash_compact, QK_WG},
     {"join_part_build", (const void*)k_join_part_build, QK_WG},
