@@ -1875,19 +1875,24 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
   if constexpr (PK && NA <= 1) {
     /* two register sets of rows alternate: one set's loads are in flight
      * while the other is probed.  A step's rows are r + j * WG + tid; in
-     * the last step the rows at or past re read row r instead and are
-     * masked, `lim` being the step's rows (all ones for a full step). */
+     * the last step the rows at or past re read row rs instead (in range:
+     * the loop runs only when rs < re) and are masked, `lim` being the
+     * step's rows (all ones for a full step, 0 for a step at or past re).
+     * The loads are unconditional, a step past re reading row rs alone:
+     * loaded under a condition, a set's values would be copied into the
+     * registers that carry it round the loop, and the copies wait for the
+     * loads at once instead of when the set is probed. */
     constexpr int NX = NA > 0 ? NA : 1;
     qk_u32 k0[ROWS], k1[ROWS], x0[ROWS][NX], x1[ROWS][NX];
     auto load_set = [&](qk_u32(&kk)[ROWS], qk_u32(&xx)[ROWS][NX], qk_u64 r) {
-      const qk_u64 left = re - r;
+      const qk_u64 left = r < re ? re - r : 0u;
       const qk_u32 lim = left < pstep ? (qk_u32)left : 0xffffffffu;
       const qk_u32* base = rowsbuf + (r + threadIdx.x) * W;
 #pragma unroll
       for (int j = 0; j < ROWS; j++) {
         const qk_u32 o = (qk_u32)j * WG;
         const qk_u32* p = (lim == 0xffffffffu || o + threadIdx.x < lim) ? base + (qk_u64)o * W
-                                                                         : rowsbuf + r * W;
+                                                                         : rowsbuf + rs * W;
         if (W == 2) {
           const qk_u64 v = qk_ld((const qk_u64*)p);
           kk[j] = (qk_u32)v;
@@ -1968,10 +1973,10 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
     };
     if (rs < re) load_set(k0, x0, rs);
     for (qk_u64 r = rs; r < re; r += 2 * pstep) {
-      if (r + pstep < re) load_set(k1, x1, r + pstep);
+      load_set(k1, x1, r + pstep);
       probe_set(k0, x0, r);
       if (r + pstep >= re) break;
-      if (r + 2 * pstep < re) load_set(k0, x0, r + 2 * pstep);
+      load_set(k0, x0, r + 2 * pstep);
       probe_set(k1, x1, r + pstep);
     }
   } else {
