@@ -256,8 +256,10 @@ static B32 qe_join_partitioned(QE_Table* fact, U32 fact_key_col, const U32* arg_
   GPU_Buffer* hist = reuse ? sh->hist : qe_scratch("jp_hist", (U64)B * P * 4);
   GPU_Buffer* off = reuse ? sh->off : qe_scratch("jp_off", (U64)B * P * 4);
   GPU_Buffer* totals = reuse ? sh->totals : qe_scratch("jp_totals", P * 4);
-  /* one row past the partitions' rows takes the scatter's discarded stores */
-  GPU_Buffer* rowsbuf = reuse ? sh->rows : qe_scratch("jp_rows", (rows + 1) * W * 4);
+  /* rows past the partitions' rows: the last takes the scatter's discarded
+   * stores, and the packed probe's 16-byte loads of 2 or 4 rows may reach
+   * up to 3 rows past the partitions' last row */
+  GPU_Buffer* rowsbuf = reuse ? sh->rows : qe_scratch("jp_rows", (rows + 4) * W * 4);
   B32 ok = kb && kh && ko && ks && kp && table && flags && hist && off && totals && rowsbuf;
   U64 cus = gpu_hip_compute_units();
   U32 split = (U32)((cus * qe_settings_u64("QE_JOIN_PROBE_BLOCKS_PER_CU", 24) + P - 1) / P);

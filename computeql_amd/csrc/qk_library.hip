@@ -1873,40 +1873,49 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
   qk_u32* g_max = g_min + (qk_u64)nargs * range;
   const qk_u64 pstep = (qk_u64)WG * ROWS;
   if constexpr (PK && NA <= 1) {
-    /* two register sets of rows alternate: one set's loads are in flight
-     * while the other is probed.  A step's rows are r + j * WG + tid; in
-     * the last step the rows at or past re read row rs instead (in range:
-     * the loop runs only when rs < re) and are masked, `lim` being the
-     * step's rows (all ones for a full step, 0 for a step at or past re).
-     * The loads are unconditional, a step past re reading row rs alone:
-     * loaded under a condition, a set's values would be copied into the
-     * registers that carry it round the loop, and the copies wait for the
-     * loads at once instead of when the set is probed. */
+    /* Rows come in 16-byte loads, RL consecutive rows a lane per load:
+     * the load unit spends its time per load instruction, and with 8-byte
+     * loads it was busy for nearly all of the probe.  Lane t's rows in a
+     * step from r are RL * (j * WG + t) + e (e < RL); the block's rows
+     * start at rs rounded down to a multiple of RL (a0), so every load is
+     * 16-byte aligned, and rows before rs or at or past re are masked.  A
+     * group of rows that starts before re stays inside the buffer (the host
+     * leaves RL - 1 rows past the partitions' rows); a group at or past re
+     * reads the group at a0 instead.  Two register sets of rows alternate:
+     * one set's loads are in flight while the other is probed.  The loads
+     * are unconditional: loaded under a condition, a set's values would be
+     * copied into the registers that carry it round the loop, and the
+     * copies would wait for the loads at once instead of when the set is
+     * probed. */
     constexpr int NX = NA > 0 ? NA : 1;
+    constexpr qk_u32 RL = NA > 0 ? 2u : 4u;
+    static_assert(ROWS % RL == 0, "whole 16-byte groups of rows a step");
+    const qk_u64 a0 = rs & ~(qk_u64)(RL - 1u);
     qk_u32 k0[ROWS], k1[ROWS], x0[ROWS][NX], x1[ROWS][NX];
     auto load_set = [&](qk_u32(&kk)[ROWS], qk_u32(&xx)[ROWS][NX], qk_u64 r) {
       const qk_u64 left = r < re ? re - r : 0u;
-      const qk_u32 lim = left < pstep ? (qk_u32)left : 0xffffffffu;
-      const qk_u32* base = rowsbuf + (r + threadIdx.x) * W;
 #pragma unroll
-      for (int j = 0; j < ROWS; j++) {
-        const qk_u32 o = (qk_u32)j * WG;
-        const qk_u32* p = (lim == 0xffffffffu || o + threadIdx.x < lim) ? base + (qk_u64)o * W
-                                                                         : rowsbuf + rs * W;
-        if (W == 2) {
-          const qk_u64 v = qk_ld((const qk_u64*)p);
-          kk[j] = (qk_u32)v;
-          xx[j][0] = (qk_u32)(v >> 32);
+      for (int j = 0; j < ROWS / (int)RL; j++) {
+        const qk_u32 o = RL * ((qk_u32)j * WG + threadIdx.x);
+        const qk_u32x4 v = qk_ld((const qk_u32x4*)(rowsbuf + (o < left ? r + o : a0) * W));
+        if constexpr (NA > 0) {
+          kk[RL * j] = v.x;
+          xx[RL * j][0] = v.y;
+          kk[RL * j + 1] = v.z;
+          xx[RL * j + 1][0] = v.w;
         } else {
-          kk[j] = qk_ld(p);
-#pragma unroll
-          for (int c = 0; c < NA; c++) xx[j][c] = qk_ld(p + 1 + c);
+          kk[RL * j] = v.x;
+          kk[RL * j + 1] = v.y;
+          kk[RL * j + 2] = v.z;
+          kk[RL * j + 3] = v.w;
         }
       }
     };
     auto probe_set = [&](const qk_u32(&kk)[ROWS], const qk_u32(&xx)[ROWS][NX], qk_u64 r) {
       const qk_u64 left = re - r;
       const qk_u32 lim = left < pstep ? (qk_u32)left : 0xffffffffu;
+      const qk_u32 lo = rs > r ? (qk_u32)(rs - r) : 0u; /* below RL, in the first step only */
+      const qk_u32 tl = RL * threadIdx.x;
 #pragma unroll
       for (int sb = 0; sb < ROWS; sb += G) {
         qk_u32 e0[G], home[G], d[G];
@@ -1941,9 +1950,10 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
               z = min(min(nb.w[0], nb.w[1]), min(nb.w[2], nb.w[3]));
             }
           }
-          /* a miss leaves m >= 8192 >= range; rows past the step's end
-           * are dropped */
-          d[j] = (m < range && (qk_u32)(sb + j) * WG + threadIdx.x < lim) ? m : 0xffffffffu;
+          /* a miss leaves m >= 8192 >= range; rows outside [rs, re) are
+           * dropped */
+          const qk_u32 idx = RL * (qk_u32)((sb + j) / (int)RL) * WG + tl + (qk_u32)((sb + j) % (int)RL);
+          d[j] = (m < range && idx >= lo && idx < lim) ? m : 0xffffffffu;
         }
         if (pk_cs) {
 #pragma unroll
@@ -1971,8 +1981,8 @@ __device__ __forceinline__ void qk_part_probe(const QK_Args& a)
         }
       }
     };
-    if (rs < re) load_set(k0, x0, rs);
-    for (qk_u64 r = rs; r < re; r += 2 * pstep) {
+    if (rs < re) load_set(k0, x0, a0);
+    for (qk_u64 r = a0; r < re; r += 2 * pstep) {
       load_set(k1, x1, r + pstep);
       probe_set(k0, x0, r);
       if (r + pstep >= re) break;
