@@ -1561,6 +1561,7 @@ __device__ __forceinline__ void qk_part_scatter_wc(const QK_Args& a)
   constexpr int WG = QK_WC_WG, U = 2;
   constexpr qk_u32 W = 1 + NA;
   constexpr qk_u32 CH = 32 / W, CHB = W == 1 ? 5 : 4;
+  constexpr qk_u32 RT = 4 / W; /* rows in 16 bytes */
   static_assert(NA == 0 || NA == 1, "rows of 1 or 2 words");
   using Row = QkRowW<W>;
   extern __shared__ qk_u64 lds_u64[];
@@ -1702,14 +1703,38 @@ __device__ __forceinline__ void qk_part_scatter_wc(const QK_Args& a)
       }
     }
     qk_lds_barrier();
-    /* write the whole lines: CH consecutive threads take one line */
-    for (qk_u32 idx = threadIdx.x; idx < nlines * CH; idx += WG) {
-      const QkWcLine L = lines[idx >> CHB];
-      const qk_u32 jj = idx & (CH - 1);
+    /* write the whole lines: CH / RT consecutive threads take one line, a
+     * thread RT rows (16 bytes) in one store; the store unit spends its
+     * time per instruction.  A thread whose rows straddle the block's
+     * leading skip or the buffer's end stores them one by one. */
+    for (qk_u32 idx = threadIdx.x; idx < nlines * (CH / RT); idx += WG) {
+      const QkWcLine L = lines[idx / (CH / RT)];
+      const qk_u32 jj = RT * (idx % (CH / RT));
       const qk_u32 c = L.cs & 0xffffu, skip = L.cs >> 16;
-      const Row r = jj < c ? carry[(qk_u64)L.p * CH + jj] : buf[L.bo + (qk_s32)jj];
+      Row r[RT];
+#pragma unroll
+      for (qk_u32 e = 0; e < RT; e++)
+        r[e] = jj + e < c ? carry[(qk_u64)L.p * CH + jj + e] : buf[L.bo + (qk_s32)(jj + e)];
       const qk_u64 d = ((qk_u64)L.dst << CHB) + jj;
-      if (jj >= skip && d < cap) out[d] = r;
+      if (jj >= skip && d + RT <= cap) {
+        qk_u32x4 v;
+        if (W == 2) {
+          v.x = r[0].w[0];
+          v.y = r[0].w[W - 1];
+          v.z = r[RT - 1].w[0];
+          v.w = r[RT - 1].w[W - 1];
+        } else {
+          v.x = r[0].w[0];
+          v.y = r[1 % RT].w[0];
+          v.z = r[2 % RT].w[0];
+          v.w = r[3 % RT].w[0];
+        }
+        *(qk_u32x4*)(out + d) = v;
+      } else {
+#pragma unroll
+        for (qk_u32 e = 0; e < RT; e++)
+          if (jj + e >= skip && d + e < cap) out[d + e] = r[e];
+      }
     }
     qk_lds_barrier();
     /* carry the rows past each partition's last whole line */
