@@ -1141,6 +1141,18 @@ __device__ __forceinline__ qk_u32 qk_fmix32(qk_u32 h)
   return h;
 }
 
+/* qk_fmix32(h) with its top 16 bits exact: the last step (h ^= h >> 16)
+ * leaves them as they are, so a partition index (at most 12 bits, from
+ * the top) needs only the steps before it */
+__device__ __forceinline__ qk_u32 qk_fmix32_top16(qk_u32 h)
+{
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  return h;
+}
+
 /* partition tables: buckets of QK_BUCKET slots (one 16-byte LDS read) */
 #define QK_BUCKET 4
 
@@ -1268,7 +1280,7 @@ __device__ __forceinline__ void qk_part_hist(const QK_Args& a)
       qk_u32 w[4] = {kv[j].x, kv[j].y, kv[j].z, kv[j].w};
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        if (m[j] & (1u << k)) atomicAdd(&hist[pbits ? (qk_fmix32(w[k]) >> (32 - pbits)) : 0u], 1u);
+        if (m[j] & (1u << k)) atomicAdd(&hist[pbits ? (qk_fmix32_top16(w[k]) >> (32 - pbits)) : 0u], 1u);
     }
   }
   __syncthreads();
