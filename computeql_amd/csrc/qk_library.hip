@@ -1749,13 +1749,27 @@ __device__ __forceinline__ void qk_part_scatter_wc(const QK_Args& a)
       }
     }
     qk_lds_barrier();
-    /* carry the rows past each partition's last whole line */
-    for (qk_u32 t = threadIdx.x; t < P * CH; t += WG) {
-      const qk_u32 p = t >> CHB, jj = t & (CH - 1);
-      const qk_u32 info = nf[p];
-      const qk_u32 n = info & 0xffffu, c = info >> 16;
-      const qk_u32 j = ((n >> CHB) << CHB) + jj;
-      if (j < n && j >= c) carry[t] = buf[start[p] + j - c];
+    /* carry the rows past each partition's last whole line, stored from
+     * the registers that still hold them: a row at position c + rank of
+     * its partition's c carried and n rows is past the whole lines when
+     * that position is at or past n rounded down to a line (the lines
+     * above have read the carried rows of the step before) */
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const qk_u32 x[4] = {xv[j].x, xv[j].y, xv[j].z, xv[j].w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!(m[j] & (1u << k))) continue;
+        const qk_u32 p = pbits ? hv[j][k] >> pshift : 0u;
+        const qk_u32 info = nf[p];
+        const qk_u32 pos = (info >> 16) + rank[j][k], fl = (info & 0xffffu) & ~(CH - 1u);
+        if (pos >= fl) {
+          Row r;
+          r.w[0] = hv[j][k];
+          if (W == 2) r.w[W - 1] = x[k];
+          carry[(qk_u64)p * CH + (pos - fl)] = r;
+        }
+      }
     }
   }
   qk_lds_barrier();
